@@ -1,0 +1,377 @@
+"""CPU oracle for the AHMED H-matrix hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package; it is the checker, never the product.
+
+``Oracle`` wraps ``liboracle.so`` (clean-room C restatement, oracle.c).
+``Reference`` wraps ``_ref/libahmedref.so`` (the reference's own sources,
+built in place by ``make -C oracle ref``) where that library exists.
+Both expose the same calls so tests can run one against the other.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REFLIB = os.path.join(HERE, "_ref", "libahmedref.so")
+REFSRC = "/root/reference"
+
+TREE_BBX, TREE_PCA = 0, 1
+KERNEL_LAPLACE, KERNEL_YUKAWA, KERNEL_EXPR = 0, 1, 2
+
+_u = C.c_uint
+_d = C.c_double
+_p = C.c_void_p
+
+
+def build(ref: bool = True) -> None:
+    """Compile liboracle.so (and the reference library where its sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+    if ref and os.path.isdir(REFSRC):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_p)
+
+
+class _Tree:
+    def __init__(self, api, handle, N):
+        self.api, self.h, self.N = api, handle, N
+
+    def __del__(self):
+        try:
+            self.api._tree_free(self.h)
+        except Exception:
+            pass
+
+    def perm(self):
+        op = np.empty(self.N, np.uint32)
+        po = np.empty(self.N, np.uint32)
+        self.api._tree_perm(self.h, _ptr(op), _ptr(po))
+        return op, po
+
+    def leaves(self):
+        nl = self.api._tree_nleaves(self.h)
+        b1, n1, b2, n2, i0 = (np.empty(nl, np.uint32) for _ in range(5))
+        adm = np.empty(nl, np.uint8)
+        self.api._tree_leaves(self.h, _ptr(b1), _ptr(n1), _ptr(b2), _ptr(n2), _ptr(adm), _ptr(i0))
+        return dict(b1=b1, n1=n1, b2=b2, n2=n2, adm=adm, i0=i0)
+
+
+class _HMat:
+    """H-matrix handle: leaves in idx order, mblock data layout."""
+
+    def __init__(self, api, handle, N, tree=None):
+        self.api, self.h, self.N, self.tree = api, handle, N, tree
+
+    def __del__(self):
+        try:
+            self.api._hmat_free(self.h)
+        except Exception:
+            pass
+
+    def layout(self):
+        return self.api._layout(self)
+
+    def matvec(self, x, d=1.0, y=None):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.N) if y is None else np.ascontiguousarray(y, np.float64).copy()
+        self.api._matvec(self.h, _d(d), _ptr(x), _ptr(y))
+        return y
+
+    def matmat(self, X, d=1.0, Y=None):
+        X = np.asfortranarray(X, np.float64)
+        p = X.shape[1]
+        Y = np.zeros((self.N, p), order="F") if Y is None else np.asfortranarray(Y, np.float64).copy()
+        self.api._matmat(self.h, _d(d), _u(p), _ptr(X), _u(self.N), _ptr(Y), _u(self.N))
+        return Y
+
+    def gmres(self, b, x0=None, eps=1e-8, m=30, nsteps=10000):
+        b = np.ascontiguousarray(b, np.float64).copy()
+        x = np.zeros(self.N) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+        e = _d(eps)
+        ns = _u(nsteps)
+        rc = self.api._gmres(self.h, _ptr(b), _ptr(x), C.byref(e), _u(m), C.byref(ns))
+        return x, int(rc), e.value, ns.value
+
+    def cg(self, b, x0=None, eps=1e-8, nsteps=10000):
+        b = np.ascontiguousarray(b, np.float64).copy()
+        x = np.zeros(self.N) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+        e = _d(eps)
+        ns = _u(nsteps)
+        rc = self.api._cg(self.h, _ptr(b), _ptr(x), C.byref(e), C.byref(ns))
+        return x, int(rc), e.value, ns.value
+
+    def to_dense(self):
+        """Expand every leaf into the N x N matrix (permuted ordering)."""
+        L = self.layout()
+        A = np.zeros((self.N, self.N))
+        data = L["data"]
+        for l in range(len(L["b1"])):
+            b1, n1, b2, n2, k, o = (int(L[f][l]) for f in ("b1", "n1", "b2", "n2", "rank", "offset"))
+            if k < 0:
+                A[b1:b1 + n1, b2:b2 + n2] = data[o:o + n1 * n2].reshape(n2, n1).T
+            elif k > 0:
+                U = data[o:o + n1 * k].reshape(k, n1).T
+                V = data[o + n1 * k:o + (n1 + n2) * k].reshape(k, n2).T
+                A[b1:b1 + n1, b2:b2 + n2] = U @ V.T
+        return A
+
+
+class Oracle:
+    """ctypes binding of liboracle.so."""
+
+    def __init__(self, path: str = LIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        L = self.lib = C.CDLL(path, mode=C.RTLD_GLOBAL)  # MKL kernels resolve against libmkl_core
+        L.or_tree_build.restype = _p
+        L.or_tree_build.argtypes = [_p, _u, _u, _d, C.c_int]
+        L.or_tree_free.argtypes = [_p]
+        L.or_tree_nleaves.restype = _u
+        L.or_tree_nleaves.argtypes = [_p]
+        L.or_tree_nclusters.restype = _u
+        L.or_tree_nclusters.argtypes = [_p]
+        L.or_tree_perm.argtypes = [_p, _p, _p]
+        L.or_tree_leaves.argtypes = [_p] * 7
+        L.or_tree_clusters.argtypes = [_p] * 6
+        L.or_kernel_block.argtypes = [C.c_int, _d, _d, _p, _p, _u, _u, _u, _u, _p]
+        L.or_kernel_scale.restype = _d
+        L.or_kernel_scale.argtypes = [C.c_int, _d, _d, _p, _p, _u, _u, _u, _u]
+        L.or_aca_dense.restype = C.c_int
+        L.or_aca_dense.argtypes = [_p, _u, _u, _d, _u, _u, _d, C.POINTER(_u), _p, _p]
+        L.or_addtrll.argtypes = [_u, _u, _u, _p, _p, _d, _u, C.POINTER(_u), _p, _p]
+        L.or_hmat_assemble.restype = _p
+        L.or_hmat_assemble.argtypes = [_p, _p, C.c_int, _d, _d, _d, _u, C.c_int]
+        L.or_hmat_from_leaves.restype = _p
+        L.or_hmat_from_leaves.argtypes = [_u, _u, _p, _p, _p, _p, _p, _p, _p]
+        L.or_hmat_free.argtypes = [_p]
+        L.or_hmat_nleaves.restype = _u
+        L.or_hmat_nleaves.argtypes = [_p]
+        L.or_hmat_nvals.restype = C.c_size_t
+        L.or_hmat_nvals.argtypes = [_p]
+        L.or_hmat_leaves.argtypes = [_p] * 7
+        L.or_hmat_data.restype = C.POINTER(C.c_double)
+        L.or_hmat_data.argtypes = [_p]
+        L.or_hmat_matvec.restype = C.c_int
+        L.or_hmat_matvec.argtypes = [_p, _d, _p, _p]
+        L.or_hmat_matvec_omp.restype = C.c_int
+        L.or_hmat_matvec_omp.argtypes = [_p, _d, _p, _p, C.c_int]
+        L.or_hmat_matmat.restype = C.c_int
+        L.or_hmat_matmat.argtypes = [_p, _d, _u, _p, _u, _p, _u]
+        L.or_gmres.restype = _u
+        L.or_gmres.argtypes = [_p, _p, _p, C.POINTER(_d), _u, C.POINTER(_u), _p, _u]
+        L.or_cg.restype = _u
+        L.or_cg.argtypes = [_p, _p, _p, C.POINTER(_d), C.POINTER(_u)]
+        L.or_gmres_dense.restype = _u
+        L.or_gmres_dense.argtypes = [_u, _p, _p, _p, C.POINTER(_d), _u, C.POINTER(_u)]
+        L.or_cg_dense.restype = _u
+        L.or_cg_dense.argtypes = [_u, _p, _p, _p, C.POINTER(_d), C.POINTER(_u)]
+        L.or_blas_version.argtypes = [C.c_char_p, C.c_int]
+        self._tree_free = L.or_tree_free
+        self._tree_perm = L.or_tree_perm
+        self._tree_nleaves = L.or_tree_nleaves
+        self._tree_leaves = L.or_tree_leaves
+        self._hmat_free = L.or_hmat_free
+        self._matvec = L.or_hmat_matvec
+        self._matmat = L.or_hmat_matmat
+        self._cg = L.or_cg
+
+    def _gmres(self, h, b, x, e, m, ns):
+        return self.lib.or_gmres(h, b, x, e, m, ns, None, _u(0))
+
+    def blas_version(self) -> str:
+        buf = C.create_string_buffer(256)
+        self.lib.or_blas_version(buf, 255)
+        return buf.value.decode()
+
+    # -- trees ---------------------------------------------------------------
+    def tree(self, pts, bmin, eta2, kind=TREE_BBX):
+        pts = np.ascontiguousarray(pts, np.float64)
+        h = self.lib.or_tree_build(_ptr(pts), _u(len(pts)), _u(bmin), _d(eta2), kind)
+        return _Tree(self, h, len(pts))
+
+    def clusters(self, tree):
+        nc = self.lib.or_tree_nclusters(tree.h)
+        nb, ne, ic = (np.empty(nc, np.uint32) for _ in range(3))
+        s0, s1 = np.empty(nc, np.int32), np.empty(nc, np.int32)
+        self.lib.or_tree_clusters(tree.h, _ptr(nb), _ptr(ne), _ptr(s0), _ptr(s1), _ptr(ic))
+        return dict(nbeg=nb, nend=ne, son0=s0, son1=s1, icom=ic)
+
+    # -- kernels -------------------------------------------------------------
+    def kernel_block(self, kernel, kappa, pts, op, b1, n1, b2, n2, weight=1.0):
+        pts = np.ascontiguousarray(pts, np.float64)
+        op = np.ascontiguousarray(op, np.uint32)
+        out = np.empty(n1 * n2)
+        self.lib.or_kernel_block(kernel, _d(kappa), _d(weight), _ptr(pts), _ptr(op), b1, n1, b2, n2, _ptr(out))
+        return out.reshape(n2, n1).T
+
+    # -- ACA / recompression on explicit data ----------------------------------
+    def aca_dense(self, A, eps, kmax, i0=0, scale=None):
+        A = np.asfortranarray(A, np.float64)
+        n1, n2 = A.shape
+        if scale is None:
+            scale = abs(A[n1 // 2, n2 // 2])
+        U = np.zeros((kmax + 1) * n1)
+        V = np.zeros((kmax + 1) * n2)
+        k = _u(0)
+        ok = self.lib.or_aca_dense(_ptr(A), n1, n2, _d(eps), kmax, i0, _d(scale), C.byref(k), _ptr(U), _ptr(V))
+        k = k.value
+        return bool(ok), U[:k * n1].reshape(k, n1).T.copy(), V[:k * n2].reshape(k, n2).T.copy()
+
+    def addtrll(self, U, V, delta, kgoal=None):
+        U = np.asfortranarray(U, np.float64)
+        V = np.asfortranarray(V, np.float64)
+        n1, k = U.shape
+        n2 = V.shape[0]
+        kgoal = k if kgoal is None else kgoal
+        Uo = np.zeros(n1 * max(k, 1))
+        Vo = np.zeros(n2 * max(k, 1))
+        kt = _u(0)
+        self.lib.or_addtrll(n1, n2, k, _ptr(U), _ptr(V), _d(delta), kgoal, C.byref(kt), _ptr(Uo), _ptr(Vo))
+        kt = kt.value
+        return Uo[:n1 * kt].reshape(kt, n1).T.copy(), Vo[:n2 * kt].reshape(kt, n2).T.copy()
+
+    # -- H-matrices ----------------------------------------------------------
+    def assemble(self, tree, pts, kernel=KERNEL_LAPLACE, kappa=0.0, eps=1e-6, rankmax=1 << 30, nthreads=0,
+                 weight=1.0):
+        pts = np.ascontiguousarray(pts, np.float64)
+        h = self.lib.or_hmat_assemble(tree.h, _ptr(pts), kernel, _d(kappa), _d(weight), _d(eps), _u(rankmax),
+                                      nthreads)
+        return _HMat(self, h, tree.N, tree)
+
+    def from_leaves(self, N, b1, n1, b2, n2, rank, offset, data):
+        arrs = [np.ascontiguousarray(a, np.uint32) for a in (b1, n1, b2, n2)]
+        rank = np.ascontiguousarray(rank, np.int32)
+        offset = np.ascontiguousarray(offset, np.uint64)
+        data = np.ascontiguousarray(data, np.float64)
+        h = self.lib.or_hmat_from_leaves(N, len(rank), *[_ptr(a) for a in arrs], _ptr(rank), _ptr(offset), _ptr(data))
+        return _HMat(self, h, N)
+
+    def _layout(self, H):
+        nl = self.lib.or_hmat_nleaves(H.h)
+        b1, n1, b2, n2 = (np.empty(nl, np.uint32) for _ in range(4))
+        rank = np.empty(nl, np.int32)
+        off = np.empty(nl, np.uint64)
+        self.lib.or_hmat_leaves(H.h, _ptr(b1), _ptr(n1), _ptr(b2), _ptr(n2), _ptr(rank), _ptr(off))
+        nv = self.lib.or_hmat_nvals(H.h)
+        ptr = self.lib.or_hmat_data(H.h)
+        data = np.ctypeslib.as_array(ptr, shape=(nv,)).copy() if nv else np.zeros(0)
+        return dict(b1=b1, n1=n1, b2=b2, n2=n2, rank=rank, offset=off, data=data)
+
+    def matvec_omp(self, H, x, d=1.0, nthreads=0):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(H.N)
+        self.lib.or_hmat_matvec_omp(H.h, _d(d), _ptr(x), _ptr(y), nthreads)
+        return y
+
+    def gmres_dense(self, A, b, eps=1e-8, m=30, nsteps=10000):
+        A = np.asfortranarray(A, np.float64)
+        n = A.shape[0]
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(n)
+        e, ns = _d(eps), _u(nsteps)
+        rc = self.lib.or_gmres_dense(n, _ptr(A), _ptr(b), _ptr(x), C.byref(e), m, C.byref(ns))
+        return x, int(rc), e.value, ns.value
+
+    def cg_dense(self, A, b, eps=1e-8, nsteps=10000):
+        A = np.asfortranarray(A, np.float64)
+        n = A.shape[0]
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(n)
+        e, ns = _d(eps), _u(nsteps)
+        rc = self.lib.or_cg_dense(n, _ptr(A), _ptr(b), _ptr(x), C.byref(e), C.byref(ns))
+        return x, int(rc), e.value, ns.value
+
+
+class Reference:
+    """ctypes binding of the compiled reference (oracle/_ref/libahmedref.so)."""
+
+    def __init__(self, path: str = REFLIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle ref`")
+        L = self.lib = C.CDLL(path, mode=C.RTLD_GLOBAL)  # MKL kernels resolve against libmkl_core
+        L.ref_tree_build.restype = _p
+        L.ref_tree_build.argtypes = [_p, _u, _u, _d, C.c_int]
+        L.ref_tree_free.argtypes = [_p]
+        L.ref_tree_nleaves.restype = _u
+        L.ref_tree_nleaves.argtypes = [_p]
+        L.ref_tree_perm.argtypes = [_p, _p, _p]
+        L.ref_tree_leaves.argtypes = [_p] * 7
+        L.ref_hmat_assemble.restype = _p
+        L.ref_hmat_assemble.argtypes = [_p, C.c_int, _d, _d, _d, _u]
+        L.ref_hmat_free.argtypes = [_p]
+        L.ref_hmat_layout.restype = C.c_size_t
+        L.ref_hmat_layout.argtypes = [_p, _p, _p]
+        L.ref_hmat_data.argtypes = [_p, _p]
+        L.ref_hmat_matvec.restype = C.c_int
+        L.ref_hmat_matvec.argtypes = [_p, _d, _p, _p]
+        L.ref_hmat_matmat.restype = C.c_int
+        L.ref_hmat_matmat.argtypes = [_p, _d, _u, _p, _u, _p, _u]
+        L.ref_gmres.restype = _u
+        L.ref_gmres.argtypes = [_p, _p, _p, C.POINTER(_d), _u, C.POINTER(_u)]
+        L.ref_cg.restype = _u
+        L.ref_cg.argtypes = [_p, _p, _p, C.POINTER(_d), C.POINTER(_u)]
+        L.ref_aca_dense.restype = C.c_int
+        L.ref_aca_dense.argtypes = [_p, _u, _u, _d, _u, _u, _d, C.POINTER(_u), _p, _p]
+        L.ref_addtrll.argtypes = [_u, _u, _u, _p, _p, _d, _u, C.POINTER(_u), _p, _p]
+        self._tree_free = L.ref_tree_free
+        self._tree_perm = L.ref_tree_perm
+        self._tree_nleaves = L.ref_tree_nleaves
+        self._tree_leaves = L.ref_tree_leaves
+        self._hmat_free = L.ref_hmat_free
+        self._matvec = L.ref_hmat_matvec
+        self._matmat = L.ref_hmat_matmat
+        self._gmres = L.ref_gmres
+        self._cg = L.ref_cg
+
+    def tree(self, pts, bmin, eta2, kind=TREE_BBX):
+        pts = np.ascontiguousarray(pts, np.float64)
+        h = self.lib.ref_tree_build(_ptr(pts), _u(len(pts)), _u(bmin), _d(eta2), kind)
+        return _Tree(self, h, len(pts))
+
+    def assemble(self, tree, pts=None, kernel=KERNEL_LAPLACE, kappa=0.0, eps=1e-6, rankmax=1 << 30, nthreads=0,
+                 weight=1.0):
+        h = self.lib.ref_hmat_assemble(tree.h, kernel, _d(kappa), _d(weight), _d(eps), _u(rankmax))
+        return _HMat(self, h, tree.N, tree)
+
+    def _layout(self, H):
+        lv = H.tree.leaves()
+        nl = len(lv["b1"])
+        rank = np.empty(nl, np.int32)
+        off = np.empty(nl, np.uint64)
+        nv = self.lib.ref_hmat_layout(H.h, _ptr(rank), _ptr(off))
+        data = np.empty(nv)
+        self.lib.ref_hmat_data(H.h, _ptr(data))
+        return dict(b1=lv["b1"], n1=lv["n1"], b2=lv["b2"], n2=lv["n2"], rank=rank, offset=off, data=data)
+
+    def aca_dense(self, A, eps, kmax, i0=0, scale=None):
+        A = np.asfortranarray(A, np.float64)
+        n1, n2 = A.shape
+        if scale is None:
+            scale = abs(A[n1 // 2, n2 // 2])
+        U = np.zeros((kmax + 1) * n1)
+        V = np.zeros((kmax + 1) * n2)
+        k = _u(0)
+        ok = self.lib.ref_aca_dense(_ptr(A), n1, n2, _d(eps), kmax, i0, _d(scale), C.byref(k), _ptr(U), _ptr(V))
+        k = k.value
+        return bool(ok), U[:k * n1].reshape(k, n1).T.copy(), V[:k * n2].reshape(k, n2).T.copy()
+
+    def addtrll(self, U, V, delta, kgoal=None):
+        U = np.asfortranarray(U, np.float64)
+        V = np.asfortranarray(V, np.float64)
+        n1, k = U.shape
+        n2 = V.shape[0]
+        kgoal = k if kgoal is None else kgoal
+        Uo = np.zeros(n1 * max(k, 1))
+        Vo = np.zeros(n2 * max(k, 1))
+        kt = _u(0)
+        self.lib.ref_addtrll(n1, n2, k, _ptr(U), _ptr(V), _d(delta), kgoal, C.byref(kt), _ptr(Uo), _ptr(Vo))
+        kt = kt.value
+        return Uo[:n1 * kt].reshape(kt, n1).T.copy(), Vo[:n2 * kt].reshape(kt, n2).T.copy()

@@ -1,0 +1,329 @@
+// ref_harness.cpp -- drives the COMPILED REFERENCE (AHMED sources under
+// /root/reference, built by oracle/Makefile into oracle/_ref/) to write golden
+// vectors and to cross-check the oracle restatement in this container.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is our own driver code: it includes the
+// reference's public headers and calls the reference's own functions
+// (cluster3d_bbx, bemcluster, blcluster::subdivide, matgenGeH_omp/apprx_unsym,
+// ACA, mblock::cpyLrM_cmpr, mltaGeHVec, mltaGeHGeM, GMRes, CG).  The MATGEN_T
+// kernel class below is ours (the reference ships none).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <vector>
+
+#include "H.h"
+#include "apprx.h"
+#include "bemblcluster.h"
+#include "bemcluster.h"
+#include "blcluster.h"
+#include "cluster_bbx.h"
+#include "matgen_omp.h"
+#include "matrix.h"
+#include "solvers.h"
+
+extern "C" int MKL_CBWR_Set(int);
+extern void gen_BlSequence(blcluster*, blcluster**&);
+
+namespace {
+
+struct Point {
+  double x[3];
+  double getcenter(unsigned i) const { return x[i]; }
+  double getradius2() const { return 0.0; }
+};
+
+const double ONE_OVER_4PI = 0.079577471545947667884;
+
+// MATGEN_T plugin (contract Include/matgen_sqntl.h:21-29): same kernel
+// definitions as the oracle's, entries in permuted ordering.
+struct KernelGen {
+  int kernel;
+  double kappa;
+  double weight;
+  const Point* pts;
+  const unsigned* op;
+  double val(unsigned i, unsigned j) const {
+    const unsigned oi = op[i], oj = op[j];
+    if (oi == oj) return 1.0;
+    const double dx = pts[oi].x[0] - pts[oj].x[0], dy = pts[oi].x[1] - pts[oj].x[1],
+                 dz = pts[oi].x[2] - pts[oj].x[2];
+    const double r = std::sqrt(std::fma(dz, dz, std::fma(dy, dy, dx * dx)));
+    const double c = kernel == 2 ? weight : weight * ONE_OVER_4PI;
+    if (kernel == 0) return c / r;
+    return (std::exp(-kappa * r) * c) / r;
+  }
+  void cmpbl(unsigned b1, unsigned n1, unsigned b2, unsigned n2, double* d) const {
+    for (unsigned j = 0; j < n2; ++j)
+      for (unsigned i = 0; i < n1; ++i) d[i + (size_t)j * n1] = val(b1 + i, b2 + j);
+  }
+  double scale(unsigned b1, unsigned n1, unsigned b2, unsigned n2) const {
+    return std::fabs(val(b1 + n1 / 2, b2 + n2 / 2));
+  }
+};
+
+struct DenseGen {
+  const double* A;
+  unsigned lda;
+  double sc;
+  void cmpbl(unsigned b1, unsigned n1, unsigned b2, unsigned n2, double* d) const {
+    for (unsigned j = 0; j < n2; ++j)
+      for (unsigned i = 0; i < n1; ++i) d[i + (size_t)j * n1] = A[(b1 + i) + (size_t)(b2 + j) * lda];
+  }
+  double scale(unsigned, unsigned, unsigned, unsigned) const { return sc; }
+};
+
+struct Tree {
+  int kind;
+  unsigned N;
+  std::vector<Point> pts;
+  std::vector<unsigned> op, po;
+  cluster* root;
+  blcluster* bl;
+  unsigned nblcks;
+  std::vector<blcluster*> leaves;  // by idx
+  std::vector<unsigned> i0;
+};
+
+struct HMat {
+  Tree* t;
+  mblock<double>** A;
+};
+
+// ACA start row for bbx trees: position in the row cluster closest to its
+// centre of mass (first minimum), same rule as the oracle.
+unsigned bbx_icom(const Tree* t, unsigned b1, unsigned n1) {
+  double com[3] = {0, 0, 0};
+  for (unsigned i = b1; i < b1 + n1; ++i)
+    for (int j = 0; j < 3; ++j) com[j] += t->pts[t->op[i]].x[j];
+  const double d = 1.0 / n1;
+  for (int j = 0; j < 3; ++j) com[j] *= d;
+  double smin = INFINITY;
+  unsigned ic = b1;
+  for (unsigned i = b1; i < b1 + n1; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < 3; ++j) {
+      const double e = t->pts[t->op[i]].x[j] - com[j];
+      s += e * e;
+    }
+    if (s < smin) { smin = s; ic = i; }
+  }
+  return ic;
+}
+
+// Include/apprx.h:136-172 restated for a plain blcluster with an explicit
+// start row (apprx_unsym itself needs a bemblcluster).
+void apprx_bbx(KernelGen& G, mblock<double>*& mbl, blcluster* bl, unsigned i0, double eps,
+               unsigned rankmax) {
+  unsigned n1 = bl->getn1(), n2 = bl->getn2(), b1 = bl->getb1(), b2 = bl->getb2();
+  unsigned maxk = n1 * n2 / (n1 + n2) + 4;
+  if (maxk > rankmax) maxk = rankmax;
+  mbl = new mblock<double>(n1, n2);
+  bool succ = false;
+  if (bl->isadm() && maxk) {
+    unsigned k;
+    double *U, *V;
+    succ = ::ACA(G, b1, n1, b2, n2, eps, maxk, i0, k, U, V);
+    if (succ) mbl->cpyLrM_cmpr(k, U, n1, V, n2, eps, k);
+    delete[] V;
+    delete[] U;
+  }
+  if (!succ) {
+    mbl->setGeM();
+    G.cmpbl(b1, n1, b2, n2, mbl->getdata());
+  }
+}
+
+void init() {
+  static bool done = false;
+  if (!done) { MKL_CBWR_Set(2); done = true; }
+}
+
+struct Silence {
+  std::stringstream sink;
+  std::streambuf* old;
+  Silence() : old(std::cout.rdbuf(sink.rdbuf())) {}
+  ~Silence() { std::cout.rdbuf(old); }
+};
+
+}  // namespace
+
+extern "C" {
+
+void* ref_tree_build(const double* pts, unsigned N, unsigned bmin, double eta2, int kind) {
+  init();
+  Tree* t = new Tree;
+  t->kind = kind;
+  t->N = N;
+  t->pts.resize(N);
+  for (unsigned i = 0; i < N; ++i)
+    for (int j = 0; j < 3; ++j) t->pts[i].x[j] = pts[3 * (size_t)i + j];
+  t->op.resize(N);
+  t->po.resize(N);
+  for (unsigned i = 0; i < N; ++i) t->op[i] = t->po[i] = i;
+  if (kind == 0) {
+    cluster3d_bbx<Point>* r = new cluster3d_bbx<Point>(t->pts.data(), t->op.data(), 0, N);
+    r->createClusterTree(bmin, t->op.data(), t->po.data());
+    t->root = r;
+    t->bl = new blcluster(0, 0, N, N);
+  } else {
+    bemcluster<Point>* r = new bemcluster<Point>(t->pts.data(), t->op.data(), 0, N);
+    r->createClusterTree(bmin, t->op.data(), t->po.data());
+    t->root = r;
+    t->bl = new bemblcluster<Point, Point>(r, r);
+  }
+  t->bl->subdivide(t->root, t->root, eta2, t->nblcks);
+  blcluster** list;
+  gen_BlSequence(t->bl, list);
+  t->leaves.assign(t->nblcks, nullptr);
+  for (unsigned i = 0; i < t->nblcks; ++i) t->leaves[list[i]->getidx()] = list[i];
+  delete[] list;
+  t->i0.resize(t->nblcks);
+  for (unsigned i = 0; i < t->nblcks; ++i) {
+    blcluster* b = t->leaves[i];
+    if (kind == 0) t->i0[i] = bbx_icom(t, b->getb1(), b->getn1()) - b->getb1();
+    else t->i0[i] = ((bemblcluster<Point, Point>*)b)->getcl1()->geticom() - b->getb1();
+  }
+  return t;
+}
+
+void ref_tree_free(void* p) {
+  Tree* t = (Tree*)p;
+  delete t->bl;
+  delete t->root;
+  delete t;
+}
+
+unsigned ref_tree_nleaves(void* p) { return ((Tree*)p)->nblcks; }
+
+void ref_tree_perm(void* p, unsigned* op, unsigned* po) {
+  Tree* t = (Tree*)p;
+  std::memcpy(op, t->op.data(), sizeof(unsigned) * t->N);
+  std::memcpy(po, t->po.data(), sizeof(unsigned) * t->N);
+}
+
+void ref_tree_leaves(void* p, unsigned* b1, unsigned* n1, unsigned* b2, unsigned* n2,
+                     unsigned char* adm, unsigned* i0) {
+  Tree* t = (Tree*)p;
+  for (unsigned i = 0; i < t->nblcks; ++i) {
+    blcluster* b = t->leaves[i];
+    b1[i] = b->getb1(); n1[i] = b->getn1(); b2[i] = b->getb2(); n2[i] = b->getn2();
+    adm[i] = b->isadm() ? 1 : 0;
+    i0[i] = t->i0[i];
+  }
+}
+
+void* ref_hmat_assemble(void* tp, int kernel, double kappa, double weight, double eps,
+                        unsigned rankmax) {
+  init();
+  Tree* t = (Tree*)tp;
+  HMat* h = new HMat;
+  h->t = t;
+  allocmbls(t->bl, h->A);
+  KernelGen G = {kernel, kappa, weight, t->pts.data(), t->op.data()};
+  Silence quiet;
+  if (t->kind == 1) {
+    matgenGeH_omp(G, t->nblcks, (bemblcluster<Point, Point>*)t->bl, eps, rankmax, h->A);
+  } else {
+    for (unsigned i = 0; i < t->nblcks; ++i)
+      apprx_bbx(G, h->A[i], t->leaves[i], t->i0[i], eps, rankmax);
+  }
+  return h;
+}
+
+void ref_hmat_free(void* p) {
+  HMat* h = (HMat*)p;
+  freembls(h->t->bl, h->A);
+  delete h;
+}
+
+// rank[i] = -1 for dense leaves; returns total stored values
+size_t ref_hmat_layout(void* p, int* rank, size_t* offset) {
+  HMat* h = (HMat*)p;
+  size_t off = 0;
+  for (unsigned i = 0; i < h->t->nblcks; ++i) {
+    mblock<double>* m = h->A[i];
+    rank[i] = m->isLrM() ? (int)m->rank() : -1;
+    offset[i] = off;
+    off += m->nvals();
+  }
+  return off;
+}
+
+void ref_hmat_data(void* p, double* out) {
+  HMat* h = (HMat*)p;
+  size_t off = 0;
+  for (unsigned i = 0; i < h->t->nblcks; ++i) {
+    mblock<double>* m = h->A[i];
+    const size_t nv = m->nvals();
+    if (nv) std::memcpy(out + off, m->getdata(), sizeof(double) * nv);
+    off += nv;
+  }
+}
+
+int ref_hmat_matvec(void* p, double d, double* x, double* y) {
+  HMat* h = (HMat*)p;
+  return mltaGeHVec(d, h->t->bl, h->A, x, y) ? 1 : 0;
+}
+
+int ref_hmat_matmat(void* p, double d, unsigned pp, double* X, unsigned ldX, double* Y,
+                    unsigned ldY) {
+  HMat* h = (HMat*)p;
+  return mltaGeHGeM(d, h->t->bl, h->A, pp, X, ldX, Y, ldY) ? 1 : 0;
+}
+
+unsigned ref_gmres(void* p, double* b, double* x, double* eps, unsigned m, unsigned* nsteps) {
+  HMat* h = (HMat*)p;
+  GeHMatrix<double> M(h->t->N, h->t->N, h->t->bl);
+  M.blcks = h->A;
+  double e = *eps;
+  unsigned ns = *nsteps;
+  unsigned r = GMRes(M, b, x, e, m, ns);
+  M.blcks = NULL;  // the H-matrix stays owned by h
+  *eps = e;
+  *nsteps = ns;
+  return r;
+}
+
+unsigned ref_cg(void* p, double* b, double* x, double* eps, unsigned* nsteps) {
+  HMat* h = (HMat*)p;
+  GeHMatrix<double> M(h->t->N, h->t->N, h->t->bl);
+  M.blcks = h->A;
+  double e = *eps;
+  unsigned ns = *nsteps;
+  unsigned r = CG(M, b, x, e, ns);
+  M.blcks = NULL;
+  *eps = e;
+  *nsteps = ns;
+  return r;
+}
+
+// ACA on an explicit dense block; U,V hold (kmax+1)*n1, (kmax+1)*n2
+int ref_aca_dense(const double* A, unsigned n1, unsigned n2, double eps, unsigned kmax,
+                  unsigned i0, double scale, unsigned* k, double* U, double* V) {
+  init();
+  DenseGen G = {A, n1, scale};
+  double *Up, *Vp;
+  bool ok = ::ACA(G, 0, n1, 0, n2, eps, kmax, i0, *k, Up, Vp);
+  std::memcpy(U, Up, sizeof(double) * (size_t)(kmax + 1) * n1);
+  std::memcpy(V, Vp, sizeof(double) * (size_t)(kmax + 1) * n2);
+  delete[] Up;
+  delete[] Vp;
+  return ok ? 1 : 0;
+}
+
+void ref_addtrll(unsigned n1, unsigned n2, unsigned k, double* U, double* V, double delta,
+                 unsigned kgoal, unsigned* kt, double* Uo, double* Vo) {
+  init();
+  mblock<double> m(n1, n2);
+  m.cpyLrM_cmpr(k, U, n1, V, n2, delta, kgoal);
+  *kt = m.rank();
+  if (*kt) {
+    std::memcpy(Uo, m.getdata(), sizeof(double) * (size_t)n1 * *kt);
+    std::memcpy(Vo, m.getdata() + (size_t)n1 * *kt, sizeof(double) * (size_t)n2 * *kt);
+  }
+}
+
+}  // extern "C"

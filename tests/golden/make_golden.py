@@ -1,0 +1,144 @@
+"""Write golden vectors from the COMPILED REFERENCE (run in the build container).
+
+Usage:  make -C oracle ref && python tests/golden/make_golden.py
+
+The reference's own sources (read in place under /root/reference, compiled by
+oracle/Makefile into oracle/_ref/libahmedref.so) produce every expected value
+here: cluster permutation, block-tree leaves, ACA ranks, a SHA-256 of all
+stored leaf values plus each leaf's Frobenius norm, H*x, H*X, GMRes and CG results, plus ACA / addtrll on
+explicit blocks.  Only data is written (golden_*.npz); nothing from the
+reference travels in any other form.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+from ahmed_amd import geometry  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+# name, geometry, N, seed, bmin, eta, tree kind, kernel, kappa, weight, eps,
+# gmres (m, tol, nsteps), cg (tol, nsteps)
+CASES = [
+    ("cfg1_sphere8192", "sphere", 8192, 1, 32, 0.8, O.TREE_BBX, O.KERNEL_LAPLACE, 0.0, 1.0, 1e-6,
+     (30, 1e-8, 60), None),
+    ("cfg1w_sphere8192", "sphere", 8192, 1, 32, 0.8, O.TREE_BBX, O.KERNEL_LAPLACE, 0.0, "area", 1e-6,
+     (30, 1e-8, 300), (1e-8, 300)),
+    ("pca_yukawa1500", "sphere", 1500, 7, 16, 0.8, O.TREE_PCA, O.KERNEL_YUKAWA, 2.0, "area", 1e-8,
+     (50, 1e-10, 300), (1e-10, 300)),
+    ("cube_expr2048", "cube", 2048, 3, 16, 0.8, O.TREE_BBX, O.KERNEL_EXPR, 2.0, "area", 1e-5,
+     (50, 1e-8, 400), None),
+    ("ellip_yukawa1800", "ellipsoid", 1800, 11, 32, 0.8, O.TREE_BBX, O.KERNEL_YUKAWA, 3.1, "area", 1e-6,
+     (30, 1e-10, 300), None),
+]
+
+AREA = {"sphere": 4 * np.pi, "cube": 6.0}
+
+
+def points(geo, N, seed):
+    if geo == "sphere":
+        return geometry.sphere(N, seed)
+    if geo == "cube":
+        return geometry.cube_surface(N, seed)
+    return geometry.ellipsoid(N, (1.0, 2.5, 3.7), seed)
+
+
+def weight_of(w, geo, N):
+    if w != "area":
+        return float(w)
+    if geo == "ellipsoid":
+        # surface area of the (1, 2.5, 3.7) ellipsoid by Knud Thomsen's formula
+        a, b, c, p = 1.0, 2.5, 3.7, 1.6075
+        area = 4 * np.pi * (((a * b) ** p + (a * c) ** p + (b * c) ** p) / 3) ** (1 / p)
+        return area / N
+    return AREA[geo] / N
+
+
+def data_hash(L):
+    return hashlib.sha256(np.ascontiguousarray(L["data"]).tobytes()).hexdigest()
+
+
+def leaf_fro(L):
+    data = L["data"]
+    fro = np.zeros(len(L["rank"]))
+    for l in range(len(L["rank"])):
+        n1, n2, k, o = int(L["n1"][l]), int(L["n2"][l]), int(L["rank"][l]), int(L["offset"][l])
+        if k < 0:
+            fro[l] = np.linalg.norm(data[o:o + n1 * n2])
+        elif k > 0:
+            U = data[o:o + n1 * k].reshape(k, n1).T
+            V = data[o + n1 * k:o + (n1 + n2) * k].reshape(k, n2).T
+            fro[l] = np.linalg.norm(U @ V.T)
+    return fro
+
+
+def main():
+    R = O.Reference()
+    ver = O.Oracle().blas_version()
+    for (name, geo, N, seed, bmin, eta, kind, kern, kappa, w, eps, gm, cgp) in CASES:
+        pts = points(geo, N, seed)
+        weight = weight_of(w, geo, N)
+        T = R.tree(pts, bmin, eta * eta, kind)
+        op, po = T.perm()
+        lv = T.leaves()
+        H = R.assemble(T, kernel=kern, kappa=kappa, eps=eps, weight=weight)
+        L = H.layout()
+        rng = np.random.default_rng(100 + seed)
+        x = rng.uniform(-1, 1, N)
+        y = H.matvec(x, d=1.0)
+        X = rng.standard_normal((N, 2))
+        Y = H.matmat(X, d=-0.5)
+        rec = dict(
+            N=N, seed=seed, bmin=bmin, eta2=eta * eta, kind=kind, kernel=kern, kappa=kappa,
+            weight=weight, eps=eps, geometry=geo, blas=ver,
+            op_perm=op, po_perm=po,
+            b1=lv["b1"], n1=lv["n1"], b2=lv["b2"], n2=lv["n2"], adm=lv["adm"], i0=lv["i0"],
+            rank=L["rank"], data_sha256=data_hash(L), leaf_fro=leaf_fro(L),
+            nvals=len(L["data"]), x=x, y=y, X=X, Y=Y,
+        )
+        b = H.matvec(np.ones(N))
+        m, tol, ns = gm
+        xg, rc, e, n = H.gmres(b, eps=tol, m=m, nsteps=ns)
+        rec.update(gmres_m=m, gmres_tol=tol, gmres_nsteps_in=ns, gmres_x=xg, gmres_rc=rc,
+                   gmres_eps=e, gmres_nsteps=n, b=b)
+        if cgp is not None:
+            tol, ns = cgp
+            xc, rc, e, n = H.cg(b, eps=tol, nsteps=ns)
+            rec.update(cg_tol=tol, cg_nsteps_in=ns, cg_x=xc, cg_rc=rc, cg_eps=e, cg_nsteps=n)
+        np.savez_compressed(os.path.join(OUT, f"golden_{name}.npz"), **rec)
+        print(name, "leaves", len(lv["b1"]), "adm", int(lv["adm"].sum()),
+              "lowrank", int((L["rank"] >= 0).sum()), "nvals", len(L["data"]), "gmres", rc, e, n)
+
+    # ACA and addtrll on explicit blocks (kernel blocks of two separated clusters)
+    rng = np.random.default_rng(42)
+    aca = {}
+    for c, (n1, n2, dist, eps, kmax, i0) in enumerate(
+            [(40, 55, 2.0, 1e-6, 30, 3), (64, 64, 1.5, 1e-8, 40, 0), (31, 77, 3.0, 1e-4, 20, 30),
+             (90, 45, 1.2, 1e-10, 12, 10)]):
+        s = rng.random((n1, 3))
+        t = rng.random((n2, 3)) + np.array([dist, 0.3, -0.2])
+        A = 1.0 / np.linalg.norm(s[:, None, :] - t[None, :, :], axis=2)
+        ok, U, V = R.aca_dense(A, eps, kmax, i0)
+        aca[f"aca{c}_A"] = A
+        aca[f"aca{c}_par"] = np.array([eps, kmax, i0, abs(A[n1 // 2, n2 // 2])])
+        aca[f"aca{c}_ok"] = ok
+        aca[f"aca{c}_U"] = U
+        aca[f"aca{c}_V"] = V
+        Uo, Vo = R.addtrll(U, V, eps)
+        aca[f"trl{c}_U"] = Uo
+        aca[f"trl{c}_V"] = Vo
+        print("aca", c, ok, U.shape[1], "->", Uo.shape[1])
+    aca["blas"] = ver
+    np.savez_compressed(os.path.join(OUT, "golden_aca.npz"), **aca)
+
+
+if __name__ == "__main__":
+    main()

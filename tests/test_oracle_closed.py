@@ -1,0 +1,162 @@
+"""Closed-form, invariant and brute-force pins of the oracle (no reference needed).
+
+Each check is chosen so that a plausible mistake in the restatement (a dropped
+term, a transposed operand, a wrong index or sign) fails it.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from ahmed_amd import geometry
+
+
+def test_kernel_entries_closed_form(oracle_api):
+    pts = geometry.sphere(50, 3)
+    op = np.arange(50, dtype=np.uint32)[::-1].copy()
+    for kern, kap, w in ((O.KERNEL_LAPLACE, 0.0, 1.0), (O.KERNEL_YUKAWA, 1.7, 0.3), (O.KERNEL_EXPR, 2.0, 2.0)):
+        B = oracle_api.kernel_block(kern, kap, pts, op, 5, 20, 10, 30, weight=w)
+        P, Q = pts[op[5:25]], pts[op[10:40]]
+        r = np.linalg.norm(P[:, None] - Q[None], axis=2)
+        with np.errstate(divide="ignore"):
+            if kern == O.KERNEL_LAPLACE:
+                E = w / (4 * np.pi * r)
+            elif kern == O.KERNEL_YUKAWA:
+                E = w * np.exp(-kap * r) / (4 * np.pi * r)
+            else:
+                E = w * np.exp(-kap * r) / r
+        E[op[5:25][:, None] == op[10:40][None]] = 1.0
+        np.testing.assert_allclose(B, E, rtol=1e-14)
+
+
+@pytest.mark.parametrize("kind", [O.TREE_BBX, O.TREE_PCA])
+def test_tree_invariants(oracle_api, kind):
+    N, bmin = 3000, 24
+    pts = geometry.sphere(N, 9)
+    T = oracle_api.tree(pts, bmin, 0.64, kind)
+    op, po = T.perm()
+    assert np.array_equal(np.sort(op), np.arange(N))
+    assert np.array_equal(po[op], np.arange(N))
+    lv = T.leaves()
+    cover = np.zeros((N, N), np.int8)
+    for b1, n1, b2, n2 in zip(lv["b1"], lv["n1"], lv["b2"], lv["n2"]):
+        cover[b1:b1 + n1, b2:b2 + n2] += 1
+    assert cover.min() == 1 and cover.max() == 1   # leaves tile N x N exactly once
+    cl = oracle_api.clusters(T)
+    leafcl = cl["son0"] < 0
+    sizes = cl["nend"][leafcl] - cl["nbeg"][leafcl]
+    assert sizes.min() >= bmin                     # sons exist only if both halves > / >= bmin
+    # admissibility predicate holds for every admissible leaf (brute force)
+    adm = lv["adm"].astype(bool)
+    idx = np.flatnonzero(adm)[:300]
+    for l in idx:
+        P = pts[op[lv["b1"][l]:lv["b1"][l] + lv["n1"][l]]]
+        Q = pts[op[lv["b2"][l]:lv["b2"][l] + lv["n2"][l]]]
+        if kind == O.TREE_BBX:
+            lo1, hi1, lo2, hi2 = P.min(0), P.max(0), Q.min(0), Q.max(0)
+            d2 = min(((hi1 - lo1) ** 2).sum(), ((hi2 - lo2) ** 2).sum())
+            gap = np.maximum(0, np.maximum(lo2 - hi1, lo1 - hi2))
+            assert d2 < 0.64 * (gap ** 2).sum()
+        else:
+            c1, c2 = P.mean(0), Q.mean(0)
+            d2 = min((((P - c1) * 2) ** 2).sum(1).max(), (((Q - c2) * 2) ** 2).sum(1).max())
+            assert d2 < 0.64 * ((c1 - c2) ** 2).sum() * (1 + 1e-12)
+
+
+@pytest.fixture(scope="module")
+def small_h(oracle_api):
+    N = 1200
+    pts = geometry.sphere(N, 4)
+    T = oracle_api.tree(pts, 16, 0.64)
+    H = oracle_api.assemble(T, pts, O.KERNEL_LAPLACE, 0.0, 1e-7, weight=4 * np.pi / N)
+    op, _ = T.perm()
+    K = oracle_api.kernel_block(O.KERNEL_LAPLACE, 0.0, pts, op, 0, N, 0, N, weight=4 * np.pi / N)
+    return T, H, K
+
+
+def test_hmatrix_approximates_kernel(small_h):
+    T, H, K = small_h
+    A = H.to_dense()
+    assert np.linalg.norm(A - K) <= 20 * 1e-7 * np.linalg.norm(K)
+    L = H.layout()
+    lv = T.leaves()
+    # each low-rank leaf within the reference's own ACA tolerance (Include/apprx.h:47)
+    for l in np.flatnonzero(L["rank"] >= 0)[:400]:
+        b1, n1, b2, n2 = (int(lv[f][l]) for f in ("b1", "n1", "b2", "n2"))
+        blk = K[b1:b1 + n1, b2:b2 + n2]
+        assert np.linalg.norm(A[b1:b1 + n1, b2:b2 + n2] - blk) <= 20 * 1e-7 * np.linalg.norm(blk)
+        assert L["rank"][l] <= n1 * n2 // (n1 + n2) + 4
+    dense = L["rank"] < 0
+    assert np.all((lv["adm"] == 0) <= dense)        # inadmissible leaves are dense
+
+
+def test_matvec_equals_dense_product(small_h):
+    _, H, _ = small_h
+    A = H.to_dense()
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(H.N)
+    y0 = rng.standard_normal(H.N)
+    np.testing.assert_allclose(H.matvec(x, d=-0.7, y=y0), y0 - 0.7 * A @ x, rtol=0,
+                               atol=1e-13 * np.abs(A @ x).max())
+    X = rng.standard_normal((H.N, 3))
+    Y = H.matmat(X, d=1.3)
+    np.testing.assert_allclose(Y, 1.3 * A @ X, rtol=0, atol=1e-13 * np.abs(A @ X).max())
+
+
+def test_aca_error_bound_and_rank(oracle_api):
+    rng = np.random.default_rng(1)
+    s = rng.random((70, 3))
+    t = rng.random((50, 3)) + [2.5, 0, 0]
+    A = 1 / np.linalg.norm(s[:, None] - t[None], axis=2)
+    for eps in (1e-4, 1e-7, 1e-10):
+        ok, U, V = oracle_api.aca_dense(A, eps, 49)
+        assert ok
+        assert np.linalg.norm(A - U @ V.T) <= 20 * eps * np.linalg.norm(A)
+    # exact low-rank input: ACA terminates with the exact rank
+    B = rng.standard_normal((60, 4)) @ rng.standard_normal((4, 45))
+    ok, U, V = oracle_api.aca_dense(B, 1e-8, 30)
+    assert ok and U.shape[1] == 4
+    assert np.linalg.norm(B - U @ V.T) <= 1e-10 * np.linalg.norm(B)
+    # zero block: first row step finds no pivot, success with rank 0
+    ok, U, V = oracle_api.aca_dense(np.zeros((10, 12)), 1e-6, 5, scale=1.0)
+    assert ok and U.shape[1] == 0
+
+
+def test_addtrll_truncates_like_svd(oracle_api):
+    rng = np.random.default_rng(2)
+    n1, n2, k = 40, 33, 12
+    s = np.logspace(0, -11, k)
+    Q1, _ = np.linalg.qr(rng.standard_normal((n1, k)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n2, k)))
+    U, V = Q1 * s, Q2
+    for delta in (3e-4, 3e-7, 3e-10):
+        Uo, Vo = oracle_api.addtrll(U, V, delta)
+        P = U @ V.T
+        sv = np.linalg.svd(P, compute_uv=False)
+        assert Uo.shape[1] == int(np.sum(sv > delta * sv[0]))
+        assert np.linalg.norm(P - Uo @ Vo.T, 2) <= 2 * delta * sv[0]
+        # singular values are folded into U: columns of V orthonormal
+        np.testing.assert_allclose(Vo.T @ Vo, np.eye(Vo.shape[1]), atol=1e-12)
+
+
+def test_gmres_and_cg_dense(oracle_api):
+    rng = np.random.default_rng(3)
+    n = 120
+    A = np.eye(n) * 4 + rng.standard_normal((n, n)) / np.sqrt(n)
+    b = rng.standard_normal(n)
+    x, rc, e, ns = oracle_api.gmres_dense(A, b, 1e-12, 20, 500)
+    assert rc == 0 and e < 1e-12
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=0, atol=1e-10)
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-11
+    # CG on a diagonal SPD matrix with 7 distinct eigenvalues: exact in 7 steps
+    d = np.repeat(np.arange(1.0, 8.0), 10)
+    D = np.diag(d)
+    bb = rng.standard_normal(len(d))
+    x, rc, e, ns = oracle_api.cg_dense(D, bb, 1e-12, 100)
+    assert rc == 0 and ns <= 8
+    np.testing.assert_allclose(x, bb / d, atol=1e-10)
+    # b = 0: early exit, x = 0
+    x, rc, e, ns = oracle_api.gmres_dense(A, np.zeros(n), 1e-8, 10, 10)
+    assert rc == 0 and ns == 0 and not x.any()
+    # too few steps: returns 1 with the achieved residual
+    x, rc, e, ns = oracle_api.gmres_dense(A, b, 1e-14, 5, 3)
+    assert rc == 1 and e > 1e-14
