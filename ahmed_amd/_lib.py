@@ -1,0 +1,249 @@
+"""ctypes binding of libahmed_amd.so (the C ABI declared in include/ahm.h).
+
+The HIP library is the product; there is no CPU fallback.  ``load()`` raises
+if the shared library is missing.  Device vectors are torch tensors on the
+library context's device; every call runs on torch's current HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libahmed_amd.so")
+
+F64, F32, BF16 = 0, 1, 2
+OK = 0
+STATUS = {0: "AHM_OK", 1: "AHM_ERR_INVALID", 2: "AHM_ERR_HIP", 3: "AHM_ERR_NOMEM",
+          4: "AHM_ERR_UNSUPPORTED", 5: "AHM_ERR_NOT_CONVERGED"}
+
+
+class LeafDesc(C.Structure):
+    _fields_ = [("b1", C.c_uint32), ("n1", C.c_uint32), ("b2", C.c_uint32), ("n2", C.c_uint32),
+                ("rank", C.c_int32), ("flags", C.c_uint32), ("offset", C.c_uint64)]
+
+
+LEAF_DTYPE = np.dtype([("b1", "<u4"), ("n1", "<u4"), ("b2", "<u4"), ("n2", "<u4"),
+                       ("rank", "<i4"), ("flags", "<u4"), ("offset", "<u8")])
+assert LEAF_DTYPE.itemsize == C.sizeof(LeafDesc)
+
+
+class HMatStats(C.Structure):
+    _fields_ = [("n_leaves", C.c_uint64), ("n_dense", C.c_uint64), ("n_lowrank", C.c_uint64),
+                ("nvals", C.c_uint64), ("t_len", C.c_uint64), ("bytes_per_value", C.c_uint64),
+                ("n_tasks_a", C.c_uint64), ("n_tasks_b", C.c_uint64), ("max_rank", C.c_uint32),
+                ("n_rows", C.c_uint32), ("n_cols", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class AhmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# every symbol declared in include/ahm.h, with (restype, argtypes)
+_p, _u32, _i, _d = C.c_void_p, C.c_uint32, C.c_int, C.c_double
+SIGNATURES = {
+    "ahm_last_error": (C.c_char_p, []),
+    "ahm_ctx_create": (_i, [_i, C.POINTER(_p)]),
+    "ahm_ctx_destroy": (_i, [_p]),
+    "ahm_ctx_set_stream": (_i, [_p, _p]),
+    "ahm_ctx_sync": (_i, [_p]),
+    "ahm_tree_build": (_i, [_p, _u32, _u32, _d, _i, C.POINTER(_p)]),
+    "ahm_tree_destroy": (_i, [_p]),
+    "ahm_tree_info": (_i, [_p, C.POINTER(_u32), C.POINTER(_u32)]),
+    "ahm_tree_perm": (_i, [_p, _p, _p]),
+    "ahm_tree_leaves": (_i, [_p, _p, _p, _p, _p, _p, _p]),
+    "ahm_hmat_upload": (_i, [_p, _u32, _u32, _u32, _p, _p, _i, _i, _i, C.POINTER(_p)]),
+    "ahm_hmat_convert": (_i, [_p, _i, C.POINTER(_p)]),
+    "ahm_hmat_destroy": (_i, [_p]),
+    "ahm_hmat_stats_get": (_i, [_p, C.POINTER(HMatStats)]),
+    "ahm_hmat_download": (_i, [_p, _p, _p]),
+    "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
+    "ahm_hmat_matvec_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
+}
+
+
+def load():
+    """Load libahmed_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AhmError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                           "(make -C ahmed_amd/csrc)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != OK:
+        msg = load().ahm_last_error().decode(errors="replace")
+        raise AhmError(f"{what}: {STATUS.get(rc, rc)}: {msg}")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class Context:
+    """One device + the torch stream current at each call."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        self.device = device
+        h = _p()
+        check(lib.ahm_ctx_create(device, C.byref(h)), "ahm_ctx_create")
+        self.h = h
+
+    def bind_stream(self):
+        torch = _torch()
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        check(load().ahm_ctx_set_stream(self.h, _p(s)), "ahm_ctx_set_stream")
+
+    def sync(self):
+        check(load().ahm_ctx_sync(self.h), "ahm_ctx_sync")
+
+    def close(self):
+        if self.h:
+            load().ahm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+TREE_BBX, TREE_PCA = 0, 1
+
+
+class Tree:
+    """Cluster tree + block cluster tree built by the C++ surface (host)."""
+
+    def __init__(self, pts, bmin: int, eta2: float, kind: int = TREE_BBX):
+        pts = np.ascontiguousarray(pts, np.float64)
+        self.N = len(pts)
+        h = _p()
+        check(load().ahm_tree_build(pts.ctypes.data_as(_p), self.N, bmin, eta2, kind, C.byref(h)),
+              "ahm_tree_build")
+        self.h = h
+        nl, nc = _u32(), _u32()
+        check(load().ahm_tree_info(h, C.byref(nl), C.byref(nc)), "ahm_tree_info")
+        self.n_leaves, self.n_clusters = nl.value, nc.value
+
+    def perm(self):
+        op = np.empty(self.N, np.uint32)
+        po = np.empty(self.N, np.uint32)
+        check(load().ahm_tree_perm(self.h, op.ctypes.data_as(_p), po.ctypes.data_as(_p)), "perm")
+        return op, po
+
+    def leaves(self):
+        n = self.n_leaves
+        b1, n1, b2, n2, i0 = (np.empty(n, np.uint32) for _ in range(5))
+        adm = np.empty(n, np.uint8)
+        check(load().ahm_tree_leaves(self.h, *[a.ctypes.data_as(_p) for a in (b1, n1, b2, n2, adm, i0)]),
+              "leaves")
+        return dict(b1=b1, n1=n1, b2=b2, n2=n2, adm=adm, i0=i0)
+
+    def close(self):
+        if self.h:
+            load().ahm_tree_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def leaf_array(b1, n1, b2, n2, rank, offset) -> np.ndarray:
+    a = np.zeros(len(rank), LEAF_DTYPE)
+    a["b1"], a["n1"], a["b2"], a["n2"] = b1, n1, b2, n2
+    a["rank"], a["offset"] = rank, offset
+    return a
+
+
+class HMatrix:
+    """Device H-matrix (ahm_hmat)."""
+
+    def __init__(self, ctx: Context, handle, n_rows: int, n_cols: int, storage: int):
+        self.ctx, self.h, self.n_rows, self.n_cols, self.storage = ctx, handle, n_rows, n_cols, storage
+
+    @classmethod
+    def upload(cls, ctx: Context, n_rows, n_cols, leaves: np.ndarray, data: np.ndarray,
+               storage: int = F64, keep_leaves: bool = False) -> "HMatrix":
+        leaves = np.ascontiguousarray(leaves, LEAF_DTYPE)
+        ddt = F64 if data.dtype == np.float64 else F32
+        data = np.ascontiguousarray(data)
+        h = _p()
+        ctx.bind_stream()
+        check(load().ahm_hmat_upload(ctx.h, n_rows, n_cols, len(leaves), leaves.ctypes.data_as(_p),
+                                     data.ctypes.data_as(_p), ddt, storage, int(keep_leaves),
+                                     C.byref(h)), "ahm_hmat_upload")
+        return cls(ctx, h, n_rows, n_cols, storage)
+
+    def convert(self, storage: int = F32) -> "HMatrix":
+        h = _p()
+        self.ctx.bind_stream()
+        check(load().ahm_hmat_convert(self.h, storage, C.byref(h)), "ahm_hmat_convert")
+        return HMatrix(self.ctx, h, self.n_rows, self.n_cols, storage)
+
+    def stats(self) -> dict:
+        st = HMatStats()
+        check(load().ahm_hmat_stats_get(self.h, C.byref(st)), "ahm_hmat_stats_get")
+        return {f: getattr(st, f) for f, _ in HMatStats._fields_ if f != "pad"}
+
+    def download(self):
+        st = self.stats()
+        leaves = np.zeros(st["n_leaves"], LEAF_DTYPE)
+        check(load().ahm_hmat_download(self.h, leaves.ctypes.data_as(_p), None), "download")
+        nv = 0
+        for d in leaves:
+            nv = max(nv, int(d["offset"]) + (int(d["n1"]) * int(d["n2"]) if d["rank"] < 0
+                                             else int(d["rank"]) * (int(d["n1"]) + int(d["n2"]))))
+        data = np.zeros(nv, np.float64)
+        check(load().ahm_hmat_download(self.h, None, data.ctypes.data_as(_p)), "download")
+        return leaves, data
+
+    def matvec(self, x, y, alpha: float = 1.0):
+        """y += alpha * H x for torch tensors (1-D, or 2-D column-major as (p, n))."""
+        torch = _torch()
+        vec = F64 if x.dtype == torch.float64 else F32
+        if x.dim() == 1:
+            p, ldx, ldy = 1, x.shape[0], y.shape[0]
+        else:  # (p, n) row-major storage == n x p column-major
+            p, ldx, ldy = x.shape[0], x.stride(0), y.stride(0)
+        self.ctx.bind_stream()
+        check(load().ahm_hmat_matvec(self.h, alpha, _p(x.data_ptr()), ldx, _p(y.data_ptr()), ldy, p,
+                                     vec), "ahm_hmat_matvec")
+        return y
+
+    def matvec_host(self, x: np.ndarray, y: np.ndarray, alpha: float = 1.0):
+        x = np.asfortranarray(x, np.float64)
+        y2 = np.asfortranarray(y, np.float64).copy()
+        p = 1 if x.ndim == 1 else x.shape[1]
+        self.ctx.bind_stream()
+        check(load().ahm_hmat_matvec_host(self.h, alpha, x.ctypes.data_as(_p), self.n_cols,
+                                          y2.ctypes.data_as(_p), self.n_rows, p), "matvec_host")
+        return y2
+
+    def close(self):
+        if self.h:
+            load().ahm_hmat_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

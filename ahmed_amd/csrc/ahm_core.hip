@@ -1,0 +1,106 @@
+// ahm_core.hip -- error state, contexts and device buffers of libahmed_amd.
+#include <cstdarg>
+#include <cstdio>
+
+#include "ahm_internal.h"
+
+namespace ahm {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int DeviceBuf::alloc(size_t b) {
+  release();
+  if (b == 0) return AHM_OK;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return fail(AHM_ERR_NOMEM, "hipMalloc(%zu bytes): %s", b, hipGetErrorString(e));
+  }
+  bytes = b;
+  return AHM_OK;
+}
+
+void DeviceBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+size_t dtype_size(ahm_dtype t) {
+  switch (t) {
+    case AHM_F64: return 8;
+    case AHM_F32: return 4;
+    case AHM_BF16: return 2;
+  }
+  return 0;
+}
+
+}  // namespace ahm
+
+extern "C" {
+
+const char* ahm_last_error(void) { return ahm::g_last_error.c_str(); }
+
+int ahm_ctx_create(int device, ahm_ctx* out) {
+  if (!out) return ahm::fail(AHM_ERR_INVALID, "ahm_ctx_create: out is NULL");
+  int n = 0;
+  AHM_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return ahm::fail(AHM_ERR_INVALID, "ahm_ctx_create: device %d of %d", device, n);
+  AHM_HIP(hipSetDevice(device));
+  ahm_ctx c = new ahm_ctx_s;
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return ahm::fail(AHM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  c->stream = c->own;
+  *out = c;
+  return AHM_OK;
+}
+
+int ahm_ctx_destroy(ahm_ctx c) {
+  if (!c) return AHM_OK;
+  (void)hipSetDevice(c->device);
+  if (c->own) {
+    (void)hipStreamSynchronize(c->own);
+    (void)hipStreamDestroy(c->own);
+  }
+  delete c;
+  return AHM_OK;
+}
+
+int ahm_ctx_set_stream(ahm_ctx c, void* s) {
+  if (!c) return ahm::fail(AHM_ERR_INVALID, "ahm_ctx_set_stream: NULL context");
+  c->stream = s ? (hipStream_t)s : c->own;
+  return AHM_OK;
+}
+
+int ahm_ctx_sync(ahm_ctx c) {
+  if (!c) return ahm::fail(AHM_ERR_INVALID, "ahm_ctx_sync: NULL context");
+  AHM_HIP(hipSetDevice(c->device));
+  AHM_HIP(hipStreamSynchronize(c->stream));
+  return AHM_OK;
+}
+
+}  // extern "C"

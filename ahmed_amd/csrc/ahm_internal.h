@@ -1,0 +1,95 @@
+// ahm_internal.h -- shared declarations of libahmed_amd (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ahm.h"
+
+namespace ahm {
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define AHM_HIP(call)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return ::ahm::fail(AHM_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                         \
+  } while (0)
+
+#define AHM_TRY(expr)          \
+  do {                         \
+    int s_ = (expr);           \
+    if (s_ != AHM_OK) return s_; \
+  } while (0)
+
+// Block size of the panel kernels: a panel tile has at most kPanelThreads rows.
+constexpr int kPanelThreads = 256;
+
+// One GEMV task: out(m) (+)= P (m x W, column-major, ld m) * z, where
+// z[c] = src(zsrc[zoff + c]).  Phase A tasks write V^T x into the t buffer,
+// phase B tasks accumulate alpha * (D x | U t) into y.
+struct GemvTask {
+  uint64_t data;   // element offset of P in the arena
+  uint64_t zoff;   // offset of the task's column-source list
+  uint32_t m, W;
+  uint32_t out;    // first output row (t index for phase A, y row for phase B)
+  uint32_t pad;
+};
+
+// zsrc entry: bit 31 set -> index into t, else index into x
+constexpr uint32_t kFromT = 0x80000000u;
+
+// Copy of a rows x cols block from the mblock-layout buffer into the arena.
+// transpose = 0: dst[r + c*dst_ld] = src[r + c*src_ld]
+// transpose = 1: dst[c + r*dst_ld] = src[r + c*src_ld]   (V -> V^T tile)
+struct CopyJob {
+  uint64_t src, dst;
+  uint32_t rows, cols, src_ld, dst_ld;
+  uint32_t transpose, pad;
+};
+
+struct DeviceBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int alloc(size_t b);
+  void release();
+  ~DeviceBuf() { release(); }
+};
+
+size_t dtype_size(ahm_dtype t);
+
+}  // namespace ahm
+
+struct ahm_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own = nullptr;
+};
+
+struct ahm_hmat_s {
+  ahm_ctx ctx = nullptr;
+  uint32_t n_rows = 0, n_cols = 0;
+  ahm_dtype storage = AHM_F64;
+  ahm_dtype leaf_dtype = AHM_F64;
+  uint64_t nvals = 0;
+  uint32_t t_len = 0;
+  uint32_t max_rank = 0;
+  uint64_t n_dense = 0, n_lowrank = 0;
+  ahm::DeviceBuf arena;       // nvals values in panel layout
+  ahm::DeviceBuf tasks_a, tasks_b, zsrc;
+  uint32_t n_a = 0, n_b = 0;
+  uint64_t n_zsrc = 0;
+  ahm::DeviceBuf tbuf;        // V^T X workspace: t_len * p values
+  size_t tbuf_cap = 0;        // bytes
+  ahm::DeviceBuf leafdata;    // optional mblock-layout copy (keep_leaves)
+  std::vector<ahm_leaf_desc> leaves;
+  // host copies of the plan (used by ahm_hmat_convert)
+  std::vector<ahm::GemvTask> h_tasks_a, h_tasks_b;
+  std::vector<uint32_t> h_zsrc;
+};

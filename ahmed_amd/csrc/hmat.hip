@@ -1,0 +1,531 @@
+// hmat.hip -- device layout of an H-matrix and the H-matvec kernels.
+//
+// Replaces the reference's recursive mltaGeHVec_ (H/mltaGeHVec.cpp:16-35) and
+// its per-leaf BLAS-2 products (mltaLrMVec_ H/mblock.cpp:125-138, mltaGeMVec
+// Include/mblock.h:98-102) by two batched launches over a re-laid-out arena:
+//
+//   phase A  t = V^T x for every low-rank leaf.  Leaves sharing a column
+//            cluster (b2, n2) are stacked into one V^T panel (sum k rows x n2
+//            columns, column-major), so x[b2:b2+n2) is staged once per panel.
+//   phase B  y += alpha * [D pieces | U pieces] z per row segment.  Rows are
+//            cut at every leaf boundary (the leaf clusters of the row tree);
+//            one panel holds, for its m rows, the matching rows of every dense
+//            block and every U that covers them, so each output row is
+//            written by exactly one workgroup: deterministic, no atomics.
+//
+// Every stored value is read exactly once per product; x and t are gathered
+// into LDS per panel through a per-column source list.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "ahm_internal.h"
+
+namespace ahm {
+
+// ------------------------------------------------------------------------
+// kernels
+
+template <typename TD, typename TS>
+__global__ __launch_bounds__(256) void relayout_kernel(const CopyJob* __restrict__ jobs,
+                                                       const TS* __restrict__ src,
+                                                       TD* __restrict__ dst) {
+  const CopyJob j = jobs[blockIdx.x];
+  const uint64_t n = (uint64_t)j.rows * j.cols;
+  for (uint64_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const uint32_t r = (uint32_t)(e % j.rows), c = (uint32_t)(e / j.rows);
+    const TS v = src[j.src + r + (uint64_t)c * j.src_ld];
+    if (j.transpose)
+      dst[j.dst + c + (uint64_t)r * j.dst_ld] = (TD)v;
+    else
+      dst[j.dst + r + (uint64_t)c * j.dst_ld] = (TD)v;
+  }
+}
+
+template <typename TD, typename TS>
+__global__ void cast_kernel(const TS* __restrict__ src, TD* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = (TD)src[i];
+}
+
+// z chunk staged in LDS per pass over a panel
+constexpr int kZChunk = 2048;
+
+// out(m) (+)= P z  for one task per workgroup, blockIdx.y = right-hand side.
+// Thread t < G*m (G = 256/m column groups) owns row t%m and the columns
+// c = t/m, t/m + G, ...; consecutive groups read consecutive columns, so a
+// workgroup's loads cover G*m contiguous values.  Rows are reduced across
+// groups through LDS in a fixed order.
+template <typename TS, typename TV, bool PHASE_B>
+__global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
+    const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
+    const TS* __restrict__ arena, const TV* __restrict__ x, uint32_t ldx, TV* __restrict__ t,
+    uint32_t t_len, TV* __restrict__ y, uint32_t ldy, TV alpha) {
+  __shared__ TV zs[kZChunk];
+  __shared__ TV red[kPanelThreads];
+  const GemvTask tk = tasks[blockIdx.x];
+  const uint32_t rhs = blockIdx.y;
+  const TV* xr = x + (uint64_t)rhs * ldx;
+  TV* tr = t + (uint64_t)rhs * t_len;
+  const int m = (int)tk.m;
+  const int G = kPanelThreads / m;
+  const int tid = threadIdx.x;
+  const int row = tid % m, g = tid / m;
+  const bool active = g < G;
+  const TS* P = arena + tk.data + row;
+  TV a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (uint32_t c0 = 0; c0 < tk.W; c0 += kZChunk) {
+    const int cw = (int)min((uint32_t)kZChunk, tk.W - c0);
+    __syncthreads();
+    for (int c = tid; c < cw; c += kPanelThreads) {
+      const uint32_t s = zsrc[tk.zoff + c0 + c];
+      zs[c] = (s & kFromT) ? tr[s & ~kFromT] : xr[s];
+    }
+    __syncthreads();
+    if (active) {
+      const TS* Pc = P + (uint64_t)c0 * m;
+      int c = g;
+      for (; c + 3 * G < cw; c += 4 * G) {
+        const TV p0 = (TV)Pc[(uint64_t)c * m];
+        const TV p1 = (TV)Pc[(uint64_t)(c + G) * m];
+        const TV p2 = (TV)Pc[(uint64_t)(c + 2 * G) * m];
+        const TV p3 = (TV)Pc[(uint64_t)(c + 3 * G) * m];
+        a0 = fma(p0, zs[c], a0);
+        a1 = fma(p1, zs[c + G], a1);
+        a2 = fma(p2, zs[c + 2 * G], a2);
+        a3 = fma(p3, zs[c + 3 * G], a3);
+      }
+      for (; c < cw; c += G) a0 = fma((TV)Pc[(uint64_t)c * m], zs[c], a0);
+    }
+  }
+  red[tid] = active ? (a0 + a1) + (a2 + a3) : (TV)0;
+  __syncthreads();
+  if (tid < m) {
+    TV s = 0;
+    for (int q = 0; q < G; ++q) s += red[q * m + tid];
+    if (PHASE_B) {
+      TV* yr = y + (uint64_t)rhs * ldy;
+      yr[tk.out + tid] += alpha * s;
+    } else {
+      tr[tk.out + tid] = s;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// plan: host-side construction of the tasks and copy jobs
+
+struct Plan {
+  std::vector<GemvTask> a, b;
+  std::vector<uint32_t> zsrc;
+  std::vector<CopyJob> jobs;
+  uint64_t nvals = 0;
+  uint32_t t_len = 0, max_rank = 0;
+  uint64_t n_dense = 0, n_lowrank = 0;
+};
+
+static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_leaf_desc* L,
+                      Plan& P) {
+  const uint32_t TM = kPanelThreads;
+  // validate
+  for (uint32_t l = 0; l < nl; ++l) {
+    const ahm_leaf_desc& d = L[l];
+    if ((uint64_t)d.b1 + d.n1 > n_rows || (uint64_t)d.b2 + d.n2 > n_cols || d.n1 == 0 ||
+        d.n2 == 0)
+      return fail(AHM_ERR_INVALID, "leaf %u (%u,%u,%u,%u) outside %u x %u", l, d.b1, d.n1,
+                  d.b2, d.n2, n_rows, n_cols);
+    if (d.rank >= 0 && (uint64_t)d.rank > std::min(d.n1, d.n2) + 64u)
+      return fail(AHM_ERR_INVALID, "leaf %u rank %d exceeds its dimensions", l, d.rank);
+  }
+  // t offsets: low-rank leaves grouped by column cluster, in leaf order
+  std::vector<uint32_t> lr;
+  for (uint32_t l = 0; l < nl; ++l)
+    if (L[l].rank > 0) lr.push_back(l);
+  std::stable_sort(lr.begin(), lr.end(), [&](uint32_t a, uint32_t b) {
+    return L[a].b2 != L[b].b2 ? L[a].b2 < L[b].b2 : L[a].n2 < L[b].n2;
+  });
+  std::vector<uint32_t> toff(nl, 0);
+  uint64_t arena = 0, t_len = 0;
+  size_t i = 0;
+  while (i < lr.size()) {
+    size_t j = i;
+    uint64_t K = 0;
+    while (j < lr.size() && L[lr[j]].b2 == L[lr[i]].b2 && L[lr[j]].n2 == L[lr[i]].n2) {
+      toff[lr[j]] = (uint32_t)(t_len + K);
+      K += (uint64_t)L[lr[j]].rank;
+      ++j;
+    }
+    const uint32_t b2 = L[lr[i]].b2, n2 = L[lr[i]].n2;
+    // V^T panel of K rows, tiled into <= TM rows
+    for (uint64_t r0 = 0; r0 < K; r0 += TM) {
+      const uint32_t m = (uint32_t)std::min<uint64_t>(TM, K - r0);
+      GemvTask tk{};
+      tk.data = arena;
+      tk.m = m;
+      tk.W = n2;
+      tk.zoff = P.zsrc.size();
+      tk.out = (uint32_t)(t_len + r0);
+      for (uint32_t c = 0; c < n2; ++c) P.zsrc.push_back(b2 + c);
+      // copy the V columns of the leaves that fall into rows [r0, r0+m)
+      uint64_t rowbase = 0;
+      for (size_t q = i; q < j; ++q) {
+        const ahm_leaf_desc& d = L[lr[q]];
+        const uint64_t k = (uint64_t)d.rank;
+        const uint64_t lo = std::max(rowbase, r0), hi = std::min(rowbase + k, r0 + m);
+        if (lo < hi) {
+          CopyJob cj{};
+          cj.src = d.offset + (uint64_t)d.n1 * k + (lo - rowbase) * d.n2;
+          cj.src_ld = d.n2;
+          cj.rows = d.n2;
+          cj.cols = (uint32_t)(hi - lo);
+          cj.dst = arena + (lo - r0);
+          cj.dst_ld = m;
+          cj.transpose = 1;
+          P.jobs.push_back(cj);
+        }
+        rowbase += k;
+      }
+      P.a.push_back(tk);
+      arena += (uint64_t)m * n2;
+    }
+    t_len += K;
+    i = j;
+  }
+  if (t_len >= kFromT) return fail(AHM_ERR_UNSUPPORTED, "sum of ranks %llu too large", (unsigned long long)t_len);
+  // row segments: cut at every leaf row boundary
+  std::vector<uint32_t> cuts;
+  cuts.reserve(2 * (size_t)nl + 2);
+  cuts.push_back(0);
+  cuts.push_back(n_rows);
+  for (uint32_t l = 0; l < nl; ++l) {
+    cuts.push_back(L[l].b1);
+    cuts.push_back(L[l].b1 + L[l].n1);
+  }
+  std::sort(cuts.begin(), cuts.end());
+  cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+  const size_t nseg = cuts.size() - 1;
+  // leaves covering each segment (leaf order = reference traversal order)
+  std::vector<uint32_t> cnt(nseg + 1, 0);
+  std::vector<std::pair<uint32_t, uint32_t>> span(nl);
+  for (uint32_t l = 0; l < nl; ++l) {
+    if (L[l].rank == 0) { span[l] = {0, 0}; continue; }
+    const uint32_t s0 = (uint32_t)(std::lower_bound(cuts.begin(), cuts.end(), L[l].b1) - cuts.begin());
+    const uint32_t s1 =
+        (uint32_t)(std::lower_bound(cuts.begin(), cuts.end(), L[l].b1 + L[l].n1) - cuts.begin());
+    span[l] = {s0, s1};
+    for (uint32_t s = s0; s < s1; ++s) cnt[s + 1]++;
+  }
+  for (size_t s = 0; s < nseg; ++s) cnt[s + 1] += cnt[s];
+  std::vector<uint32_t> cover(cnt[nseg]);
+  {
+    std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
+    for (uint32_t l = 0; l < nl; ++l)
+      for (uint32_t s = span[l].first; s < span[l].second; ++s) cover[fill[s]++] = l;
+  }
+  for (size_t s = 0; s < nseg; ++s) {
+    const uint32_t r0s = cuts[s], r1s = cuts[s + 1];
+    for (uint32_t r0 = r0s; r0 < r1s; r0 += TM) {
+      const uint32_t m = std::min(TM, r1s - r0);
+      GemvTask tk{};
+      tk.data = arena;
+      tk.m = m;
+      tk.zoff = P.zsrc.size();
+      tk.out = r0;
+      uint32_t W = 0;
+      for (uint32_t q = cnt[s]; q < cnt[s + 1]; ++q) {
+        const ahm_leaf_desc& d = L[cover[q]];
+        const uint32_t cols = d.rank < 0 ? d.n2 : (uint32_t)d.rank;
+        CopyJob cj{};
+        cj.src = d.offset + (r0 - d.b1);
+        cj.src_ld = d.n1;
+        cj.rows = m;
+        cj.cols = cols;
+        cj.dst = arena + (uint64_t)W * m;
+        cj.dst_ld = m;
+        cj.transpose = 0;
+        P.jobs.push_back(cj);
+        if (d.rank < 0)
+          for (uint32_t c = 0; c < cols; ++c) P.zsrc.push_back(d.b2 + c);
+        else
+          for (uint32_t c = 0; c < cols; ++c) P.zsrc.push_back(kFromT | (toff[cover[q]] + c));
+        W += cols;
+      }
+      tk.W = W;
+      if (W) P.b.push_back(tk);
+      arena += (uint64_t)m * W;
+    }
+  }
+  // biggest tasks first: the hardware dispatches workgroups in order
+  auto big = [](const GemvTask& a, const GemvTask& b) {
+    return (uint64_t)a.m * a.W > (uint64_t)b.m * b.W;
+  };
+  std::stable_sort(P.a.begin(), P.a.end(), big);
+  std::stable_sort(P.b.begin(), P.b.end(), big);
+  P.nvals = arena;
+  P.t_len = (uint32_t)t_len;
+  for (uint32_t l = 0; l < nl; ++l) {
+    if (L[l].rank < 0) P.n_dense++;
+    else {
+      P.n_lowrank++;
+      P.max_rank = std::max(P.max_rank, (uint32_t)L[l].rank);
+    }
+  }
+  return AHM_OK;
+}
+
+template <typename T>
+static int upload_vec(DeviceBuf& b, const std::vector<T>& v, hipStream_t s) {
+  AHM_TRY(b.alloc(v.size() * sizeof(T)));
+  if (!v.empty()) AHM_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return AHM_OK;
+}
+
+static int ensure_tbuf(ahm_hmat h, uint32_t p, size_t esize) {
+  const size_t need = (size_t)h->t_len * p * esize;
+  if (need <= h->tbuf_cap) return AHM_OK;
+  AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+  AHM_TRY(h->tbuf.alloc(need));
+  h->tbuf_cap = need;
+  return AHM_OK;
+}
+
+template <typename TS, typename TV>
+static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y, uint32_t ldy,
+                         uint32_t p) {
+  hipStream_t s = h->ctx->stream;
+  TV* t = (TV*)h->tbuf.p;
+  if (h->n_a)
+    hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, false>), dim3(h->n_a, p), dim3(kPanelThreads), 0, s,
+                       (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
+                       (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, alpha);
+  if (h->n_b)
+    hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, true>), dim3(h->n_b, p), dim3(kPanelThreads), 0, s,
+                       (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
+                       (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, alpha);
+  AHM_HIP(hipGetLastError());
+  return AHM_OK;
+}
+
+}  // namespace ahm
+
+using namespace ahm;
+
+extern "C" {
+
+int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_leaves,
+                    const ahm_leaf_desc* leaves, const void* data, ahm_dtype data_dtype,
+                    ahm_dtype storage, int keep_leaves, ahm_hmat* out) {
+  if (!ctx || !out || (!leaves && n_leaves))
+    return fail(AHM_ERR_INVALID, "ahm_hmat_upload: NULL argument");
+  if ((data_dtype != AHM_F64 && data_dtype != AHM_F32) || (storage != AHM_F64 && storage != AHM_F32))
+    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_upload: dtypes %d -> %d", data_dtype, storage);
+  if (data_dtype == AHM_F32 && storage == AHM_F64)
+    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_upload: float leaves cannot be stored as double");
+  AHM_HIP(hipSetDevice(ctx->device));
+  Plan P;
+  AHM_TRY(build_plan(n_rows, n_cols, n_leaves, leaves, P));
+  // leaf values in mblock layout: total = end of the last leaf
+  uint64_t total = 0;
+  for (uint32_t l = 0; l < n_leaves; ++l) {
+    const ahm_leaf_desc& d = leaves[l];
+    const uint64_t nv = d.rank < 0 ? (uint64_t)d.n1 * d.n2 : (uint64_t)d.rank * (d.n1 + d.n2);
+    total = std::max(total, d.offset + nv);
+  }
+  if (total && !data) return fail(AHM_ERR_INVALID, "ahm_hmat_upload: data is NULL");
+  ahm_hmat h = new ahm_hmat_s;
+  h->ctx = ctx;
+  h->n_rows = n_rows;
+  h->n_cols = n_cols;
+  h->storage = storage;
+  h->leaf_dtype = data_dtype;
+  h->nvals = P.nvals;
+  h->t_len = P.t_len;
+  h->max_rank = P.max_rank;
+  h->n_dense = P.n_dense;
+  h->n_lowrank = P.n_lowrank;
+  h->leaves.assign(leaves, leaves + n_leaves);
+  hipStream_t s = ctx->stream;
+  const size_t ds = dtype_size(data_dtype), ss = dtype_size(storage);
+  int st = AHM_OK;
+  DeviceBuf raw, jobs;
+  do {
+    if ((st = raw.alloc(total * ds))) break;
+    if (total && hipMemcpyAsync(raw.p, data, total * ds, hipMemcpyHostToDevice, s) != hipSuccess) {
+      st = fail(AHM_ERR_HIP, "ahm_hmat_upload: copy of %llu values failed", (unsigned long long)total);
+      break;
+    }
+    if ((st = h->arena.alloc(P.nvals * ss))) break;
+    if ((st = upload_vec(jobs, P.jobs, s))) break;
+    if ((st = upload_vec(h->tasks_a, P.a, s))) break;
+    if ((st = upload_vec(h->tasks_b, P.b, s))) break;
+    if ((st = upload_vec(h->zsrc, P.zsrc, s))) break;
+    h->n_a = (uint32_t)P.a.size();
+    h->n_b = (uint32_t)P.b.size();
+    h->n_zsrc = P.zsrc.size();
+    // launch the re-layout in slices (grid.x limit)
+    for (size_t j0 = 0; j0 < P.jobs.size(); j0 += (1u << 30)) {
+      const uint32_t nj = (uint32_t)std::min<size_t>(1u << 30, P.jobs.size() - j0);
+      const CopyJob* J = (const CopyJob*)jobs.p + j0;
+      if (data_dtype == AHM_F64 && storage == AHM_F64)
+        hipLaunchKernelGGL((relayout_kernel<double, double>), dim3(nj), dim3(256), 0, s, J,
+                           (const double*)raw.p, (double*)h->arena.p);
+      else if (data_dtype == AHM_F64)
+        hipLaunchKernelGGL((relayout_kernel<float, double>), dim3(nj), dim3(256), 0, s, J,
+                           (const double*)raw.p, (float*)h->arena.p);
+      else
+        hipLaunchKernelGGL((relayout_kernel<float, float>), dim3(nj), dim3(256), 0, s, J,
+                           (const float*)raw.p, (float*)h->arena.p);
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      st = fail(AHM_ERR_HIP, "ahm_hmat_upload: re-layout kernel failed");
+      break;
+    }
+    h->h_tasks_a = std::move(P.a);
+    h->h_tasks_b = std::move(P.b);
+    h->h_zsrc = std::move(P.zsrc);
+    if (keep_leaves) {
+      h->leafdata.p = raw.p;
+      h->leafdata.bytes = raw.bytes;
+      raw.p = nullptr;
+      raw.bytes = 0;
+    }
+  } while (0);
+  if (st) {
+    delete h;
+    return st;
+  }
+  *out = h;
+  return AHM_OK;
+}
+
+int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
+  if (!src || !out) return fail(AHM_ERR_INVALID, "ahm_hmat_convert: NULL argument");
+  if (src->storage != AHM_F64 || storage != AHM_F32)
+    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_convert: only F64 -> F32");
+  AHM_HIP(hipSetDevice(src->ctx->device));
+  hipStream_t s = src->ctx->stream;
+  ahm_hmat h = new ahm_hmat_s;
+  h->ctx = src->ctx;
+  h->n_rows = src->n_rows;
+  h->n_cols = src->n_cols;
+  h->storage = storage;
+  h->leaf_dtype = src->leaf_dtype;
+  h->nvals = src->nvals;
+  h->t_len = src->t_len;
+  h->max_rank = src->max_rank;
+  h->n_dense = src->n_dense;
+  h->n_lowrank = src->n_lowrank;
+  h->leaves = src->leaves;
+  h->h_tasks_a = src->h_tasks_a;
+  h->h_tasks_b = src->h_tasks_b;
+  h->h_zsrc = src->h_zsrc;
+  int st = AHM_OK;
+  do {
+    if ((st = h->arena.alloc(h->nvals * 4))) break;
+    if ((st = upload_vec(h->tasks_a, h->h_tasks_a, s))) break;
+    if ((st = upload_vec(h->tasks_b, h->h_tasks_b, s))) break;
+    if ((st = upload_vec(h->zsrc, h->h_zsrc, s))) break;
+    h->n_a = src->n_a;
+    h->n_b = src->n_b;
+    h->n_zsrc = src->n_zsrc;
+    if (h->nvals)
+      hipLaunchKernelGGL((cast_kernel<float, double>), dim3(4096), dim3(256), 0, s,
+                         (const double*)src->arena.p, (float*)h->arena.p, h->nvals);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      st = fail(AHM_ERR_HIP, "ahm_hmat_convert: cast kernel failed");
+      break;
+    }
+  } while (0);
+  if (st) {
+    delete h;
+    return st;
+  }
+  *out = h;
+  return AHM_OK;
+}
+
+int ahm_hmat_destroy(ahm_hmat h) {
+  if (!h) return AHM_OK;
+  (void)hipSetDevice(h->ctx->device);
+  (void)hipStreamSynchronize(h->ctx->stream);
+  delete h;
+  return AHM_OK;
+}
+
+int ahm_hmat_stats_get(ahm_hmat h, ahm_hmat_stats* st) {
+  if (!h || !st) return fail(AHM_ERR_INVALID, "ahm_hmat_stats_get: NULL argument");
+  std::memset(st, 0, sizeof *st);
+  st->n_leaves = h->leaves.size();
+  st->n_dense = h->n_dense;
+  st->n_lowrank = h->n_lowrank;
+  st->nvals = h->nvals;
+  st->t_len = h->t_len;
+  st->bytes_per_value = dtype_size(h->storage);
+  st->n_tasks_a = h->n_a;
+  st->n_tasks_b = h->n_b;
+  st->max_rank = h->max_rank;
+  st->n_rows = h->n_rows;
+  st->n_cols = h->n_cols;
+  return AHM_OK;
+}
+
+int ahm_hmat_download(ahm_hmat h, ahm_leaf_desc* leaves, void* data) {
+  if (!h) return fail(AHM_ERR_INVALID, "ahm_hmat_download: NULL handle");
+  if (leaves) std::memcpy(leaves, h->leaves.data(), h->leaves.size() * sizeof(ahm_leaf_desc));
+  if (data) {
+    if (!h->leafdata.p && h->leafdata.bytes == 0 && !h->leaves.empty())
+      return fail(AHM_ERR_INVALID, "ahm_hmat_download: matrix was uploaded without keep_leaves");
+    AHM_HIP(hipSetDevice(h->ctx->device));
+    AHM_HIP(hipMemcpyAsync(data, h->leafdata.p, h->leafdata.bytes, hipMemcpyDeviceToHost,
+                           h->ctx->stream));
+    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+  }
+  return AHM_OK;
+}
+
+int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y, uint32_t ldy,
+                    uint32_t p, ahm_dtype vec) {
+  if (!h || !X || !Y) return fail(AHM_ERR_INVALID, "ahm_hmat_matvec: NULL argument");
+  if (p == 0) return AHM_OK;
+  if (p > 65535) return fail(AHM_ERR_INVALID, "ahm_hmat_matvec: p=%u > 65535", p);
+  if ((p > 1 && (ldx < h->n_cols || ldy < h->n_rows)))
+    return fail(AHM_ERR_INVALID, "ahm_hmat_matvec: ldx=%u ldy=%u for %u x %u", ldx, ldy,
+                h->n_rows, h->n_cols);
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  if (h->storage == AHM_F64 && vec == AHM_F64) {
+    AHM_TRY(ensure_tbuf(h, p, 8));
+    return launch_matvec<double, double>(h, alpha, (const double*)X, ldx, (double*)Y, ldy, p);
+  }
+  if (h->storage == AHM_F32 && vec == AHM_F32) {
+    AHM_TRY(ensure_tbuf(h, p, 4));
+    return launch_matvec<float, float>(h, (float)alpha, (const float*)X, ldx, (float*)Y, ldy, p);
+  }
+  return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec: storage %d with vectors %d", h->storage, vec);
+}
+
+int ahm_hmat_matvec_host(ahm_hmat h, double alpha, const double* X, uint32_t ldx, double* Y,
+                         uint32_t ldy, uint32_t p) {
+  if (!h || !X || !Y) return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_host: NULL argument");
+  if (h->storage != AHM_F64)
+    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec_host: needs an F64 matrix");
+  if (p == 0) return AHM_OK;
+  if (p > 1 && (ldx < h->n_cols || ldy < h->n_rows))
+    return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_host: bad leading dimensions");
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  hipStream_t s = h->ctx->stream;
+  DeviceBuf dx, dy;
+  AHM_TRY(dx.alloc((size_t)h->n_cols * p * 8));
+  AHM_TRY(dy.alloc((size_t)h->n_rows * p * 8));
+  AHM_HIP(hipMemcpy2DAsync(dx.p, (size_t)h->n_cols * 8, X, (size_t)ldx * 8, (size_t)h->n_cols * 8, p,
+                           hipMemcpyHostToDevice, s));
+  AHM_HIP(hipMemcpy2DAsync(dy.p, (size_t)h->n_rows * 8, Y, (size_t)ldy * 8, (size_t)h->n_rows * 8, p,
+                           hipMemcpyHostToDevice, s));
+  AHM_TRY(ahm_hmat_matvec(h, alpha, dx.p, h->n_cols, dy.p, h->n_rows, p, AHM_F64));
+  AHM_HIP(hipMemcpy2DAsync(Y, (size_t)ldy * 8, dy.p, (size_t)h->n_rows * 8, (size_t)h->n_rows * 8, p,
+                           hipMemcpyDeviceToHost, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  return AHM_OK;
+}
+
+}  // extern "C"

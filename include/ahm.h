@@ -1,0 +1,128 @@
+/*
+ * ahm.h -- C ABI of the MI355X H-matrix library (libahmed_amd.so).
+ *
+ * This is the drop-in boundary the AHMED-compatible C++ surface (include/ahmed)
+ * calls into, and what any FFI (ctypes, cgo, JNI) binds.  Plain pointers and
+ * sizes only; every call returns an ahm_status; ahm_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Data conventions (same as the reference):
+ *   - vectors are in cluster-permuted order (op_perm[pos] = original index,
+ *     Include/cluster_bbx.h:106-112);
+ *   - a leaf is (b1, n1, b2, n2): rows [b1, b1+n1), columns [b2, b2+n2);
+ *   - leaf values use mblock<T>'s storage (Include/mblock.h:36-54): a dense
+ *     leaf is n1*n2 column-major; a low-rank leaf U V^T is U (n1 x k,
+ *     column-major) followed by V (n2 x k, column-major);
+ *   - products ACCUMULATE: y += alpha * H x (H/mltaGeHVec.cpp:16-35).
+ * Device pointers are HIP device memory on the context's device; every
+ * device-side call is ordered on the context's stream.  A handle is not
+ * thread-safe: calls on one handle must be serialised by the caller.
+ */
+#ifndef AHM_H
+#define AHM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  AHM_OK = 0,
+  AHM_ERR_INVALID = 1,       /* bad argument / shape                         */
+  AHM_ERR_HIP = 2,           /* HIP runtime error                            */
+  AHM_ERR_NOMEM = 3,         /* device or host allocation failed             */
+  AHM_ERR_UNSUPPORTED = 4,   /* combination not implemented                  */
+  AHM_ERR_NOT_CONVERGED = 5  /* Krylov solver hit nsteps (results still set) */
+} ahm_status;
+
+typedef enum { AHM_F64 = 0, AHM_F32 = 1, AHM_BF16 = 2 } ahm_dtype;
+
+typedef struct ahm_ctx_s* ahm_ctx;
+typedef struct ahm_hmat_s* ahm_hmat;
+typedef struct ahm_tree_s* ahm_tree;
+
+/* cluster tree kinds */
+#define AHM_TREE_BBX 0   /* cluster3d_bbx (Include/cluster_bbx.h:236-302)      */
+#define AHM_TREE_PCA 1   /* bemcluster  (Include/bemcluster.h:17-59)          */
+
+/* One leaf of the block cluster tree with its storage.  rank < 0: dense leaf
+ * (mblock::isGeM), rank >= 0: low-rank leaf (mblock::isLrM, rank 0 = empty).
+ * offset: element offset of the leaf's values in the data array. */
+typedef struct {
+  uint32_t b1, n1, b2, n2;
+  int32_t rank;
+  uint32_t flags;            /* reserved, 0 */
+  uint64_t offset;
+} ahm_leaf_desc;
+
+typedef struct {
+  uint64_t n_leaves, n_dense, n_lowrank;
+  uint64_t nvals;            /* sum of stored values (mblock::nvals)          */
+  uint64_t t_len;            /* sum of low-rank ranks = len(V^T x)            */
+  uint64_t bytes_per_value;  /* storage precision                             */
+  uint64_t n_tasks_a, n_tasks_b;
+  uint32_t max_rank;
+  uint32_t n_rows, n_cols;
+  uint32_t pad;
+} ahm_hmat_stats;
+
+const char* ahm_last_error(void);
+
+/* -------- context: one device, one HIP stream ---------------------------- */
+int ahm_ctx_create(int device, ahm_ctx* out);
+int ahm_ctx_destroy(ahm_ctx ctx);
+/* use an existing hipStream_t (NULL = the context's own stream) */
+int ahm_ctx_set_stream(ahm_ctx ctx, void* hip_stream);
+int ahm_ctx_sync(ahm_ctx ctx);
+
+/* -------- trees (host): the C++ surface's cluster tree + block cluster tree
+ * pts: N x 3 (original order).  Replaces the caller-side sequence
+ * createClusterTree + blcluster::subdivide (Examples/PrecondBEM.cpp:263-277,
+ * Include/cluster_bbx.h:100-128, cluster/blcluster.cpp:40-66).
+ * Leaves come out in idx order; i0[l] is the ACA start row of leaf l relative
+ * to its b1 (bemcluster::icom - b1 as in Include/apprx.h:155; for bbx trees
+ * the row closest to the row cluster's centre of mass). */
+int ahm_tree_build(const double* pts, uint32_t N, uint32_t bmin, double eta2, int kind,
+                   ahm_tree* out);
+int ahm_tree_destroy(ahm_tree t);
+int ahm_tree_info(ahm_tree t, uint32_t* n_leaves, uint32_t* n_clusters);
+int ahm_tree_perm(ahm_tree t, uint32_t* op_perm, uint32_t* po_perm);
+int ahm_tree_leaves(ahm_tree t, uint32_t* b1, uint32_t* n1, uint32_t* b2, uint32_t* n2,
+                    uint8_t* adm, uint32_t* i0);
+
+/* -------- H-matrix upload / download / info -------------------------------
+ * Replaces the per-leaf mblock<T>* array of the reference (Include/H.h:21-45,
+ * allocmbls/freembls) with one device arena in the matvec layout.
+ * data_dtype: element type of `data` (AHM_F64 or AHM_F32, host memory);
+ * storage: AHM_F64 or AHM_F32 (copyH double->float, H/initH.cpp:152-168).
+ * keep_leaves != 0 keeps a device copy in mblock layout for download. */
+int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_leaves,
+                    const ahm_leaf_desc* leaves, const void* data, ahm_dtype data_dtype,
+                    ahm_dtype storage, int keep_leaves, ahm_hmat* out);
+/* Re-layout an existing matrix in another storage precision (FP64 -> FP32). */
+int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out);
+int ahm_hmat_destroy(ahm_hmat h);
+int ahm_hmat_stats_get(ahm_hmat h, ahm_hmat_stats* st);
+/* leaves (n_leaves descs, offsets into data) and values in mblock layout, in
+ * the matrix's storage precision; needs keep_leaves.  data may be NULL to
+ * query descriptors only. */
+int ahm_hmat_download(ahm_hmat h, ahm_leaf_desc* leaves, void* data);
+
+/* -------- products: mltaGeHVec / mltaGeHGeM (H/mltaGeHVec.cpp:567-577) ----
+ * Y += alpha * H * X for p right-hand sides; X is n_cols x p column-major
+ * with leading dimension ldx, Y n_rows x p with ldy (device pointers).
+ * vec: AHM_F64 or AHM_F32 (must match storage unless storage=F32, vec=F64
+ * is asked for, which is refused).  p = 1 is the single-vector path. */
+int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y,
+                    uint32_t ldy, uint32_t p, ahm_dtype vec);
+/* Same with host vectors (the drop-in for mltaGeHVec's host pointers):
+ * copies X in and Y in/out over PCIe around the device product. */
+int ahm_hmat_matvec_host(ahm_hmat h, double alpha, const double* X, uint32_t ldx,
+                         double* Y, uint32_t ldy, uint32_t p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

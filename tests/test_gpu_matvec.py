@@ -1,0 +1,138 @@
+"""GPU H-matvec parity: the HIP library against the reference's own outputs
+(golden fixtures) and against the oracle, on the identical H-matrix.
+
+Bar: FP64 products within 1e-12 relative (max-norm) of the reference/oracle;
+FP32 storage within 2e-6 relative of the FP64 product (the stated
+precision of FP32 leaves with FP32 accumulation).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_names
+from helpers import golden_hmatrix, leaves_of, relerr, sphere_hmatrix
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from ahmed_amd import _lib
+    assert torch.cuda.is_available()
+    return _lib.Context(0)
+
+
+def upload(ctx, H, storage=0, keep=False):
+    from ahmed_amd import _lib
+    leaves, data = leaves_of(H)
+    return _lib.HMatrix.upload(ctx, H.N, H.N, leaves, data, storage=storage, keep_leaves=keep)
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_matvec_matches_reference_golden(ctx, name):
+    g, T, H = golden_hmatrix(name)
+    print(f"[gpu] golden {name} N={H.N}", flush=True)
+    D = upload(ctx, H)
+    y = dev(np.zeros(H.N))
+    D.matvec(dev(g["x"]), y, 1.0)
+    assert relerr(y.cpu().numpy(), g["y"]) <= 1e-12
+    # p = 2, alpha = -0.5, accumulating into zero (mltaGeHGeM)
+    X = dev(g["X"].T.copy())            # (p, n) row-major == n x p column-major
+    Y = dev(np.zeros((2, H.N)))
+    D.matvec(X, Y, -0.5)
+    assert relerr(Y.cpu().numpy().T, g["Y"]) <= 1e-12
+
+
+def test_matvec_accumulates_and_scales(ctx):
+    _, T, H = sphere_hmatrix(6000, 32, 1e-8)
+    D = upload(ctx, H)
+    rng = np.random.default_rng(0)
+    x, y0 = rng.standard_normal(H.N), rng.standard_normal(H.N)
+    ref = H.matvec(x, d=-2.5, y=y0)
+    y = dev(y0)
+    D.matvec(dev(x), y, -2.5)
+    assert relerr(y.cpu().numpy(), ref) <= 1e-12
+
+
+def test_matvec_cfg2_shape_single_and_multi_rhs(ctx):
+    # sphere, leaf size 64, eta 0.8, ACA eps 1e-8 (the one-GPU FP64 config, at N=32768)
+    _, T, H = sphere_hmatrix(32768, 64, 1e-8, weighted=False)
+    print(f"[gpu] cfg2-shape N={H.N}", flush=True)
+    D = upload(ctx, H)
+    st = D.stats()
+    assert st["nvals"] == len(H.layout()["data"])
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(H.N)
+    y = dev(np.zeros(H.N))
+    D.matvec(dev(x), y)
+    assert relerr(y.cpu().numpy(), H.matvec(x)) <= 1e-12
+    X = rng.standard_normal((H.N, 32))
+    Y = dev(np.zeros((32, H.N)))
+    D.matvec(dev(X.T.copy()), Y)
+    assert relerr(Y.cpu().numpy().T, H.matmat(X)) <= 1e-12
+
+
+def test_matvec_fp32_storage_error_is_stated(ctx):
+    _, T, H = sphere_hmatrix(16384, 64, 1e-8)
+    D = upload(ctx, H)
+    D32 = D.convert()
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(H.N)
+    y64 = H.matvec(x)
+    y = dev(np.zeros(H.N), torch.float32)
+    D32.matvec(dev(x, torch.float32), y)
+    err = relerr(y.cpu().numpy(), y64)
+    print(f"[gpu] FP32 leaves + FP32 accumulation vs FP64 H-matvec: {err:.2e}")
+    assert err <= 2e-6
+
+
+def test_matvec_host_drop_in(ctx):
+    g, T, H = golden_hmatrix("pca_yukawa1500")
+    D = upload(ctx, H)
+    y = D.matvec_host(g["x"], np.zeros(H.N))
+    assert relerr(y, g["y"]) <= 1e-12
+
+
+def test_keep_leaves_download_roundtrip(ctx):
+    g, T, H = golden_hmatrix("cube_expr2048")
+    D = upload(ctx, H, keep=True)
+    leaves, data = D.download()
+    L = H.layout()
+    assert np.array_equal(leaves["rank"], L["rank"])
+    assert np.array_equal(data, L["data"])
+
+
+def test_rank_zero_and_all_dense_edge_cases(ctx):
+    from ahmed_amd import _lib
+    rng = np.random.default_rng(3)
+    N = 300
+    # two dense diagonal leaves, one rank-0 leaf, one rank-2 leaf
+    b1 = [0, 150, 0, 150]
+    n1 = [150, 150, 150, 150]
+    b2 = [0, 150, 150, 0]
+    n2 = [150, 150, 150, 150]
+    rank = [-1, -1, 0, 2]
+    sizes = [150 * 150, 150 * 150, 0, 2 * 300]
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    data = rng.standard_normal(int(sum(sizes)))
+    leaves = _lib.leaf_array(b1, n1, b2, n2, rank, off)
+    D = _lib.HMatrix.upload(ctx, N, N, leaves, data)
+    A = np.zeros((N, N))
+    A[:150, :150] = data[:22500].reshape(150, 150).T
+    A[150:, 150:] = data[22500:45000].reshape(150, 150).T
+    U = data[45000:45300].reshape(2, 150).T
+    V = data[45300:45600].reshape(2, 150).T
+    A[150:, :150] = U @ V.T
+    x = rng.standard_normal(N)
+    y = dev(np.zeros(N))
+    D.matvec(dev(x), y)
+    assert relerr(y.cpu().numpy(), A @ x) <= 1e-13
+    with pytest.raises(_lib.AhmError):
+        bad = leaves.copy()
+        bad["n1"][0] = 400
+        _lib.HMatrix.upload(ctx, N, N, bad, data)
