@@ -37,6 +37,18 @@ class HMatStats(C.Structure):
                 ("n_rows", C.c_uint32), ("n_cols", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class Kernel(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_uint32), ("kappa", C.c_double), ("weight", C.c_double)]
+
+
+class AssemblyStats(C.Structure):
+    _fields_ = [("n_candidates", C.c_uint64), ("n_retries", C.c_uint64), ("n_lowrank", C.c_uint64),
+                ("n_dense", C.c_uint64), ("sum_rank", C.c_uint64), ("sum_aca_rank", C.c_uint64)]
+
+
+KERNEL_LAPLACE, KERNEL_YUKAWA, KERNEL_EXPR = 0, 1, 2
+
+
 class AhmError(RuntimeError):
     pass
 
@@ -61,6 +73,10 @@ SIGNATURES = {
     "ahm_hmat_destroy": (_i, [_p]),
     "ahm_hmat_stats_get": (_i, [_p, C.POINTER(HMatStats)]),
     "ahm_hmat_download": (_i, [_p, _p, _p]),
+    "ahm_assemble": (_i, [_p, _p, _p, _u32, _u32, _p, _p, _p, _p, _p, _p, C.POINTER(Kernel), _d, _u32, _p,
+                          _i, _i, C.POINTER(AssemblyStats), C.POINTER(_p)]),
+    "ahm_assemble_tree": (_i, [_p, _p, C.POINTER(Kernel), _d, _u32, _p, _i, _i, C.POINTER(AssemblyStats),
+                               C.POINTER(_p)]),
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_matvec_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
 }
@@ -191,6 +207,24 @@ class HMatrix:
                                      data.ctypes.data_as(_p), ddt, storage, int(keep_leaves),
                                      C.byref(h)), "ahm_hmat_upload")
         return cls(ctx, h, n_rows, n_cols, storage)
+
+    @classmethod
+    def assemble(cls, ctx: Context, tree: "Tree", kernel: int = KERNEL_LAPLACE, kappa: float = 0.0,
+                 weight: float = 1.0, eps: float = 1e-6, rankmax: int = 0xFFFFFFFF, rank_hint=None,
+                 storage: int = F64, keep_leaves: bool = False):
+        """GPU assembly (ACA + recompression + dense leaves) on a tree; returns
+        (HMatrix, stats dict)."""
+        k = Kernel(kernel, 0, kappa, weight)
+        st = AssemblyStats()
+        h = _p()
+        hint = None
+        if rank_hint is not None:
+            hint = np.ascontiguousarray(rank_hint, np.uint32)
+        ctx.bind_stream()
+        check(load().ahm_assemble_tree(ctx.h, tree.h, C.byref(k), eps, rankmax,
+                                       hint.ctypes.data_as(_p) if hint is not None else None, storage,
+                                       int(keep_leaves), C.byref(st), C.byref(h)), "ahm_assemble_tree")
+        return cls(ctx, h, tree.N, tree.N, storage), {f: getattr(st, f) for f, _ in AssemblyStats._fields_}
 
     def convert(self, storage: int = F32) -> "HMatrix":
         h = _p()

@@ -1,0 +1,938 @@
+// aca.hip -- H-matrix assembly on the GPU: batched ACA, batched recompression
+// and dense leaf generation, all leaves in flight at once.
+//
+// Follows the reference's per-leaf rule apprx_unsym (Include/apprx.h:136-172):
+// admissible leaves try ACA (Include/ACA.h:217-350) with
+// maxk = min(n1*n2/(n1+n2)+4, rankmax); on success the factors are
+// recompressed as cpyLrM_cmpr -> addtrll does (H/mblock_D.cpp:357-465): QR of
+// U and of V, R = R1 R2^T with entries below delta*|R|_F/sqrt(mmin*nmin)
+// zeroed, SVD of R, truncation at delta*s_0, singular values folded into U.
+// Otherwise (or on ACA failure) the leaf is generated dense.
+//
+// One workgroup (256 threads, 4 wavefronts) runs one leaf; the sequential
+// pivot logic runs uniformly in every thread, the vector work (kernel rows /
+// columns, the k-term updates, norms, arg-max searches, Householder and
+// Jacobi dot products) is spread over the workgroup.  Reductions use a fixed
+// order, so results are bit-reproducible run to run.
+//
+// The kernel entries (the reference's MATGEN_T plugin, contract
+// Include/matgen_sqntl.h:21-29) are evaluated on the device from the
+// permuted points: entry(i,j) = 1 on i == j, else c/r (Laplace) or
+// exp(-kappa r) c / r (decaying kernels).
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "ahm_internal.h"
+#include "ahm_aca.h"
+
+namespace ahm {
+
+constexpr int kAcaThreads = 256;
+constexpr int kWaves = kAcaThreads / 64;
+
+__device__ __forceinline__ double kentry(const KernelSpec& K, const double* __restrict__ P,
+                                         uint32_t i, uint32_t j) {
+  if (i == j) return 1.0;
+  const double dx = P[3 * (size_t)i] - P[3 * (size_t)j];
+  const double dy = P[3 * (size_t)i + 1] - P[3 * (size_t)j + 1];
+  const double dz = P[3 * (size_t)i + 2] - P[3 * (size_t)j + 2];
+  const double r = sqrt(fma(dz, dz, fma(dy, dy, dx * dx)));
+  if (K.kind == AHM_KERNEL_LAPLACE) return K.c / r;
+  return (exp(-K.kappa * r) * K.c) / r;
+}
+
+// ------------------------------------------------------------------------
+// workgroup reductions (fixed order: lanes by xor butterfly, then waves 0..3)
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double s = sh[0];
+  for (int q = 1; q < kWaves; ++q) s += sh[q];
+  return s;
+}
+
+// arg-max of v (ties -> smallest index); candidates carry v >= 0, others -1
+struct ArgMax {
+  double v;
+  int i;
+};
+__device__ __forceinline__ ArgMax amax(ArgMax a, ArgMax b) {
+  return (b.v > a.v || (b.v == a.v && b.i < a.i)) ? b : a;
+}
+__device__ ArgMax block_argmax(ArgMax a, double* shv, int* shi) {
+  for (int o = 32; o > 0; o >>= 1) {
+    ArgMax b;
+    b.v = __shfl_xor(a.v, o, 64);
+    b.i = __shfl_xor(a.i, o, 64);
+    a = amax(a, b);
+  }
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    shv[w] = a.v;
+    shi[w] = a.i;
+  }
+  __syncthreads();
+  ArgMax r{shv[0], shi[0]};
+  for (int q = 1; q < kWaves; ++q) r = amax(r, ArgMax{shv[q], shi[q]});
+  return r;
+}
+
+// arg-min of integer keys (ties -> smallest index); non-candidates INT_MAX
+__device__ void block_argmin_int(int& key, int& idx, int* shk, int* shi) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const int k2 = __shfl_xor(key, o, 64), i2 = __shfl_xor(idx, o, 64);
+    if (k2 < key || (k2 == key && i2 < idx)) {
+      key = k2;
+      idx = i2;
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    shk[w] = key;
+    shi[w] = idx;
+  }
+  __syncthreads();
+  key = shk[0];
+  idx = shi[0];
+  for (int q = 1; q < kWaves; ++q)
+    if (shk[q] < key || (shk[q] == key && shi[q] < idx)) {
+      key = shk[q];
+      idx = shi[q];
+    }
+}
+
+struct Shared {
+  double dv[kWaves];
+  double av[kWaves];
+  int ai[kWaves];
+  int ki[kWaves];
+  int kk[kWaves];
+};
+
+// ------------------------------------------------------------------------
+// ACA (Include/ACA.h:29-350), one leaf per workgroup
+
+// next index >= j (cyclic) with Z != -1 (nexti, Include/ACA.h:29-36)
+__device__ uint32_t nexti_dev(uint32_t n, const int* Z, uint32_t j) {
+  for (;;) {
+    if (Z[j] != -1) return j;
+    j = (j + 1 < n) ? j + 1 : 0;
+  }
+}
+
+struct AcaState {
+  uint32_t no;
+  double nrmlsk2;
+};
+
+// pv = row i0 of the remainder (Include/ACA.h:57-64); returns arg-max over S>=0
+__device__ ArgMax remainder_row(const KernelSpec& K, const double* P, const AcaBlock& B,
+                                uint32_t i0, uint32_t k, const double* U, const double* V,
+                                double* pv, const int* S, double* ucoef, Shared& sh) {
+  __syncthreads();
+  for (uint32_t l = threadIdx.x; l < k; l += kAcaThreads) ucoef[l] = U[(size_t)l * B.n1 + i0];
+  __syncthreads();
+  ArgMax a{-1.0, 0x7fffffff};
+  for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads) {
+    double v = kentry(K, P, B.b1 + i0, B.b2 + j);
+    for (uint32_t l = 0; l < k; ++l) v = fma(-ucoef[l], V[(size_t)l * B.n2 + j], v);
+    pv[j] = v;
+    if (S[j] >= 0) a = amax(a, ArgMax{fabs(v), (int)j});
+  }
+  return block_argmax(a, sh.av, sh.ai);
+}
+
+// pu = column j0 of the remainder (Include/ACA.h:143-158)
+__device__ ArgMax remainder_col(const KernelSpec& K, const double* P, const AcaBlock& B,
+                                uint32_t j0, uint32_t k, const double* U, const double* V,
+                                double* pu, const int* Z, double* vcoef, Shared& sh) {
+  __syncthreads();
+  for (uint32_t l = threadIdx.x; l < k; l += kAcaThreads) vcoef[l] = V[(size_t)l * B.n2 + j0];
+  __syncthreads();
+  ArgMax a{-1.0, 0x7fffffff};
+  for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads) {
+    double v = kentry(K, P, B.b1 + i, B.b2 + j0);
+    for (uint32_t l = 0; l < k; ++l) v = fma(-vcoef[l], U[(size_t)l * B.n1 + i], v);
+    pu[i] = v;
+    if (Z[i] >= 0) a = amax(a, ArgMax{fabs(v), (int)i});
+  }
+  return block_argmax(a, sh.av, sh.ai);
+}
+
+// ACA_row_step (Include/ACA.h:46-130).  Returns false if no pivot was found.
+__device__ bool aca_row_step(const KernelSpec& K, const double* P, const AcaBlock& B,
+                             uint32_t& klast, uint32_t& i0, uint32_t k, AcaState& st, int* Z,
+                             int* S, double* U, double* V, double scale, double* coef,
+                             Shared& sh) {
+  double* pu = U + (size_t)k * B.n1;
+  double* pv = V + (size_t)k * B.n2;
+  ArgMax m;
+  for (;;) {
+    m = remainder_row(K, P, B, i0, k, U, V, pv, S, coef, sh);
+    if (!(fmax(m.v, 0.0) < 1e-14 * scale)) break;
+    __syncthreads();
+    if (threadIdx.x == 0) Z[i0] = -1;
+    __syncthreads();
+    if (++st.no < B.n1) i0 = nexti_dev(B.n1, Z, i0);
+    if (!(st.no < B.n1)) break;
+  }
+  // m.v < 0: no candidate column at all (the reference reads an unset j0 there)
+  if (m.v < 0.0 || fmax(m.v, 0.0) < 1e-14 * scale) return false;
+  const uint32_t j0 = (uint32_t)m.i;
+  const double sqrtpiv = sqrt(m.v);
+  const double thr = 1e-8 * scale * sqrtpiv;
+  __syncthreads();
+  const double sca = sqrtpiv / pv[j0];
+  __syncthreads();
+  double nv = 0.0;
+  for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads)
+    if (S[j] >= 0) {
+      const double e = (pv[j] *= sca);
+      nv += e * e;
+      if (fabs(e) > thr) ++S[j];
+    }
+  const double nrmv2 = block_sum(nv, sh.dv);
+  // column j0 of the remainder; pivot row/column retired afterwards
+  remainder_col(K, P, B, j0, k, U, V, pu, Z, coef, sh);
+  __syncthreads();
+  klast = (uint32_t)Z[i0];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Z[i0] = -1;
+    S[j0] = -1;
+  }
+  __syncthreads();
+  ++st.no;
+  const double dsca = 1.0 / sqrtpiv;
+  double nu = 0.0;
+  ArgMax a{-1.0, 0x7fffffff};
+  for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads) {
+    const double eabs = fabs(pu[i] *= dsca);
+    if (Z[i] >= 0 && eabs > thr) ++Z[i];
+    nu += eabs * eabs;
+    if (Z[i] >= 0) a = amax(a, ArgMax{eabs, (int)i});
+  }
+  const double nrmu2 = block_sum(nu, sh.dv);
+  a = block_argmax(a, sh.av, sh.ai);
+  if (a.v > 0.0) i0 = (uint32_t)a.i;
+  st.nrmlsk2 = nrmu2 * nrmv2;
+  return true;
+}
+
+// ACA_col_step (Include/ACA.h:134-213)
+__device__ bool aca_col_step(const KernelSpec& K, const double* P, const AcaBlock& B,
+                             uint32_t& j0, uint32_t k, AcaState& st, int* Z, int* S, double* U,
+                             double* V, double scale, double* coef, Shared& sh) {
+  double* pu = U + (size_t)k * B.n1;
+  double* pv = V + (size_t)k * B.n2;
+  ArgMax m;
+  for (;;) {
+    m = remainder_col(K, P, B, j0, k, U, V, pu, Z, coef, sh);
+    if (!(fmax(m.v, 0.0) < 1e-14 * scale)) break;
+    __syncthreads();
+    if (threadIdx.x == 0) S[j0] = -1;
+    __syncthreads();
+    if (++st.no < B.n2) j0 = nexti_dev(B.n2, S, j0);
+    if (!(st.no < B.n2)) break;
+  }
+  if (m.v < 0.0 || fmax(m.v, 0.0) < 1e-14 * scale) return false;
+  const uint32_t i0 = (uint32_t)m.i;
+  const double sqrtpiv = sqrt(m.v);
+  const double thr = 1e-8 * scale * sqrtpiv;
+  __syncthreads();
+  const double sca = sqrtpiv / pu[i0];
+  __syncthreads();
+  double nu = 0.0;
+  for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads)
+    if (Z[i] >= 0) {
+      const double e = (pu[i] *= sca);
+      nu += e * e;
+      if (fabs(e) > thr) ++Z[i];
+    }
+  const double nrmu2 = block_sum(nu, sh.dv);
+  remainder_row(K, P, B, i0, k, U, V, pv, S, coef, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Z[i0] = -1;
+    S[j0] = -1;
+  }
+  __syncthreads();
+  ++st.no;
+  const double dsca = 1.0 / sqrtpiv;
+  double nv = 0.0;
+  ArgMax a{-1.0, 0x7fffffff};
+  for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads) {
+    const double eabs = fabs(pv[j] *= dsca);
+    if (S[j] >= 0 && eabs > thr) ++S[j];
+    nv += eabs * eabs;
+    a = amax(a, ArgMax{eabs, (int)j});
+  }
+  const double nrmv2 = block_sum(nv, sh.dv);
+  a = block_argmax(a, sh.av, sh.ai);
+  if (a.v > 0.0) j0 = (uint32_t)a.i;
+  st.nrmlsk2 = nrmu2 * nrmv2;
+  return true;
+}
+
+// sum_{l<k} (U_l.U_k)(V_l.V_k): one wavefront per term, lanes over rows
+__device__ double cross_sum(const AcaBlock& B, uint32_t k, const double* U, const double* V,
+                            Shared& sh) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double* uk = U + (size_t)k * B.n1;
+  const double* vk = V + (size_t)k * B.n2;
+  double acc = 0.0;  // per-wave sum of its terms, in l order
+  for (uint32_t l = w; l < k; l += kWaves) {
+    double du = 0.0, dv = 0.0;
+    for (uint32_t i = lane; i < B.n1; i += 64) du = fma(U[(size_t)l * B.n1 + i], uk[i], du);
+    for (uint32_t j = lane; j < B.n2; j += 64) dv = fma(V[(size_t)l * B.n2 + j], vk[j], dv);
+    du = wave_sum(du);
+    dv = wave_sum(dv);
+    acc += du * dv;
+  }
+  if (lane != 0) acc = 0.0;
+  return block_sum(acc, sh.dv);
+}
+
+__global__ __launch_bounds__(kAcaThreads) void aca_kernel(const AcaBlock* __restrict__ blocks,
+                                                          AcaResult* __restrict__ res,
+                                                          const double* __restrict__ P,
+                                                          KernelSpec K, double eps,
+                                                          double* __restrict__ slab,
+                                                          int* __restrict__ iscr) {
+  __shared__ Shared sh;
+  __shared__ double coef[kAcaCapMax + 1];
+  const AcaBlock B = blocks[blockIdx.x];
+  double* U = slab + B.uoff;
+  double* V = slab + B.voff;
+  int* Z = iscr + B.zoff;
+  int* S = Z + B.n1;
+  for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads) Z[i] = 0;
+  for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads) S[j] = 0;
+  __syncthreads();
+  AcaState st{0, 0.0};
+  int kmin = -2, keps = -1;
+  uint32_t k = 0, klast = 0, inext, j0 = 0, i0 = B.i0;
+  double nrms2 = 0.0;
+  double scale = fabs(kentry(K, P, B.b1 + B.n1 / 2, B.b2 + B.n2 / 2));
+  int status = 0;  // 0 fail, 1 success, 2 slab capacity reached
+  const uint32_t kmax = B.kmax;
+  do {
+    if (k > B.kcap) {
+      status = 2;
+      goto DONE;
+    }
+    inext = i0;
+    if (!aca_row_step(K, P, B, klast, inext, k, st, Z, S, U, V, scale, coef, sh)) {
+      st.nrmlsk2 = 0.0;
+      goto SUCC;
+    }
+    {
+      const double sum = cross_sum(B, k, U, V, sh);
+      nrms2 += 2.0 * sum + st.nrmlsk2;
+    }
+    if (st.nrmlsk2 < eps * eps * nrms2 && keps == -1) keps = (int)(klast + 1u);
+    ++k;
+    scale = sqrt(st.nrmlsk2 / ((double)B.n1 * B.n2));
+    if (keps > 0) {
+      int key = 0x7fffffff, idx = 0x7fffffff;
+      for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads)
+        if (Z[i] >= 0 && (Z[i] < key || (Z[i] == key && (int)i < idx))) {
+          key = Z[i];
+          idx = (int)i;
+        }
+      block_argmin_int(key, idx, sh.kk, sh.ki);
+      if (key == 0x7fffffff) {
+        st.nrmlsk2 = 0.0;
+        goto SUCC;
+      }
+      kmin = key;
+      i0 = (uint32_t)idx;
+      if ((uint32_t)kmin == k) i0 = inext;
+    } else
+      i0 = inext;
+  } while (st.no < B.n1 && k < kmax && kmin < keps);
+
+  if ((st.no == B.n1 || k == kmax) && kmin < keps) goto NSUCC;
+
+  do {
+    {
+      int key = 0x7fffffff, idx = 0x7fffffff;
+      for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads)
+        if (S[j] >= 0 && (S[j] < key || (S[j] == key && (int)j < idx))) {
+          key = S[j];
+          idx = (int)j;
+        }
+      block_argmin_int(key, idx, sh.kk, sh.ki);
+      if (key == 0x7fffffff) {
+        st.nrmlsk2 = 0.0;
+        goto SUCC;
+      }
+      kmin = key;
+      j0 = (uint32_t)idx;
+    }
+    if (kmin >= keps) goto SUCC;
+    if (k > B.kcap) {
+      status = 2;
+      goto DONE;
+    }
+    if (!aca_col_step(K, P, B, j0, k, st, Z, S, U, V, scale, coef, sh)) {
+      st.nrmlsk2 = 0.0;
+      goto SUCC;
+    }
+    ++k;
+  } while (st.no < B.n2 && k < kmax);
+
+NSUCC:
+  status = 0;
+  goto DONE;
+SUCC : {
+  const double epsa = sqrt(st.nrmlsk2 / nrms2);
+  status = (epsa >= eps) ? 0 : 1;  // NaN (0/0) compares false: success
+}
+DONE:
+  if (threadIdx.x == 0) {
+    res[blockIdx.x].status = status;
+    res[blockIdx.x].k = k;
+    res[blockIdx.x].kt = 0;
+  }
+}
+
+// ------------------------------------------------------------------------
+// recompression (addtrll, H/mblock_D.cpp:357-465)
+
+// Householder QR in place of A (n x k, column-major, ld n): reflector j has
+// v = [1; A[j+1:, j]], tau[j]; R in the upper triangle (LAPACK dgeqr2 form).
+__device__ void house_qr(double* A, uint32_t n, uint32_t k, double* tau, Shared& sh,
+                         double* wbuf) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t r = min(n, k);
+  for (uint32_t j = 0; j < r; ++j) {
+    double* a = A + (size_t)j * n;
+    double s = 0.0;
+    for (uint32_t i = j + 1 + threadIdx.x; i < n; i += kAcaThreads) s = fma(a[i], a[i], s);
+    const double xn2 = block_sum(s, sh.dv);
+    __syncthreads();
+    const double alpha = a[j];
+    double t = 0.0, beta = alpha, scal = 0.0;
+    if (xn2 > 0.0) {
+      beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
+      t = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    __syncthreads();
+    if (xn2 > 0.0)
+      for (uint32_t i = j + 1 + threadIdx.x; i < n; i += kAcaThreads) a[i] *= scal;
+    if (threadIdx.x == 0) {
+      a[j] = beta;
+      tau[j] = t;
+    }
+    __syncthreads();
+    if (t == 0.0) continue;
+    // w_c = v^T A[:, c] for c > j, one wavefront per column
+    for (uint32_t c = j + 1 + w; c < k; c += kWaves) {
+      const double* ac = A + (size_t)c * n;
+      double d = 0.0;
+      for (uint32_t i = j + 1 + lane; i < n; i += 64) d = fma(a[i], ac[i], d);
+      d = wave_sum(d);
+      if (lane == 0) wbuf[c] = d + ac[j];
+    }
+    __syncthreads();
+    const uint32_t rows = n - j, cols = k - j - 1;
+    for (uint32_t e = threadIdx.x; e < rows * cols; e += kAcaThreads) {
+      const uint32_t i = j + e % rows, c = j + 1 + e / rows;
+      const double v = (i == j) ? 1.0 : a[i];
+      A[(size_t)c * n + i] -= t * wbuf[c] * v;
+    }
+    __syncthreads();
+  }
+}
+
+// X (n x m, ld n) <- Q X with Q = H_0 H_1 ... H_{r-1} from house_qr
+__device__ void apply_q(const double* A, uint32_t n, uint32_t r, const double* tau, double* X,
+                        uint32_t m, double* wbuf) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int j = (int)r - 1; j >= 0; --j) {
+    const double t = tau[j];
+    if (t == 0.0) continue;
+    const double* a = A + (size_t)j * n;
+    __syncthreads();
+    for (uint32_t c = w; c < m; c += kWaves) {
+      const double* xc = X + (size_t)c * n;
+      double d = 0.0;
+      for (uint32_t i = j + 1 + lane; i < n; i += 64) d = fma(a[i], xc[i], d);
+      d = wave_sum(d);
+      if (lane == 0) wbuf[c] = d + xc[j];
+    }
+    __syncthreads();
+    const uint32_t rows = n - j;
+    for (uint32_t e = threadIdx.x; e < rows * m; e += kAcaThreads) {
+      const uint32_t i = j + e % rows, c = e / rows;
+      const double v = ((uint32_t)j == i) ? 1.0 : a[i];
+      X[(size_t)c * n + i] -= t * wbuf[c] * v;
+    }
+  }
+  __syncthreads();
+}
+
+// One-sided Jacobi SVD of R (mm x nn, ld mm): R <- R Z with orthogonal Z
+// (nn x nn, ld nn) until the columns of R are mutually orthogonal.  Column
+// pairs are visited in round-robin tournament order, one wavefront per pair.
+__device__ void jacobi_svd(double* R, uint32_t mm, uint32_t nn, double* Zm, Shared& sh) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t e = threadIdx.x; e < nn * nn; e += kAcaThreads) Zm[e] = (e % nn == e / nn) ? 1.0 : 0.0;
+  __syncthreads();
+  if (nn < 2) return;
+  const uint32_t np = (nn + 1) & ~1u;  // even number of players (one bye if odd)
+  __shared__ int rot;
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    if (threadIdx.x == 0) rot = 0;
+    __syncthreads();
+    for (uint32_t round = 0; round < np - 1; ++round) {
+      for (uint32_t pi = w; pi < np / 2; pi += kWaves) {
+        // tournament pairing: player 0 fixed, others rotate
+        auto player = [&](uint32_t s) -> uint32_t {
+          return s == 0 ? 0 : 1 + (s - 1 + round) % (np - 1);
+        };
+        uint32_t p = player(pi), q = player(np - 1 - pi);
+        if (p >= nn || q >= nn) continue;
+        if (p > q) {
+          const uint32_t tt = p;
+          p = q;
+          q = tt;
+        }
+        double* ap = R + (size_t)p * mm;
+        double* aq = R + (size_t)q * mm;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (uint32_t i = lane; i < mm; i += 64) {
+          al = fma(ap[i], ap[i], al);
+          be = fma(aq[i], aq[i], be);
+          ga = fma(ap[i], aq[i], ga);
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        ga = wave_sum(ga);
+        if (fabs(ga) <= 1e-15 * sqrt(al * be) || ga == 0.0) continue;
+        const double zeta = (be - al) / (2.0 * ga);
+        const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(fma(zeta, zeta, 1.0)));
+        const double c = 1.0 / sqrt(fma(t, t, 1.0)), s = c * t;
+        for (uint32_t i = lane; i < mm; i += 64) {
+          const double x = ap[i], y = aq[i];
+          ap[i] = c * x - s * y;
+          aq[i] = s * x + c * y;
+        }
+        double* zp = Zm + (size_t)p * nn;
+        double* zq = Zm + (size_t)q * nn;
+        for (uint32_t i = lane; i < nn; i += 64) {
+          const double x = zp[i], y = zq[i];
+          zp[i] = c * x - s * y;
+          zq[i] = s * x + c * y;
+        }
+        if (lane == 0) rot = 1;
+      }
+      __syncthreads();
+    }
+    const int any = rot;
+    __syncthreads();
+    if (!any) break;
+  }
+}
+
+// QR of U and V, core R = R1 R2^T, Jacobi SVD, truncation.  Leaves the
+// sorted, truncated core factors in the scratch: W = R Z (columns scaled by
+// the singular values, mmin x kt, ld mmin) and Z (nmin x kt, ld nmin).
+__global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock* __restrict__ blocks,
+                                                                 AcaResult* __restrict__ res,
+                                                                 double eps, double* __restrict__ slab) {
+  __shared__ Shared sh;
+  __shared__ double wbuf[kAcaCapMax + 1];
+  __shared__ double sig[kAcaCapMax + 1];
+  __shared__ int order[kAcaCapMax + 1];
+  const AcaBlock B = blocks[blockIdx.x];
+  const AcaResult r0 = res[blockIdx.x];
+  if (r0.status != 1) return;
+  const uint32_t k = r0.k;
+  if (k == 0) return;
+  double* U = slab + B.uoff;
+  double* V = slab + B.voff;
+  const size_t sq = (size_t)(B.kcap + 1) * (B.kcap + 1);
+  double* R = slab + B.roff;
+  double* Zm = R + sq;
+  double* Ws = Zm + sq;
+  double* Zs = Ws + sq;
+  double* tau1 = Zs + sq;
+  double* tau2 = tau1 + (B.kcap + 1);
+  house_qr(U, B.n1, k, tau1, sh, wbuf);
+  house_qr(V, B.n2, k, tau2, sh, wbuf);
+  const uint32_t mmin = min(B.n1, k), nmin = min(B.n2, k), amin = min(mmin, nmin);
+  // R = R1 R2^T (utrmmh, Include/blas.h:2767-2781)
+  double fs = 0.0;
+  for (uint32_t e = threadIdx.x; e < mmin * nmin; e += kAcaThreads) {
+    const uint32_t i = e % mmin, j = e / mmin;
+    double d = 0.0;
+    for (uint32_t l = max(i, j); l < k; ++l) d += U[(size_t)l * B.n1 + i] * V[(size_t)l * B.n2 + j];
+    R[e] = d;
+    fs += d * d;
+  }
+  const double fro = sqrt(block_sum(fs, sh.dv));
+  const double thresh = eps * fro / (double)sqrtf((float)mmin * nmin);
+  for (uint32_t e = threadIdx.x; e < mmin * nmin; e += kAcaThreads)
+    if (fabs(R[e]) < thresh) R[e] = 0.0;
+  __syncthreads();
+  jacobi_svd(R, mmin, nmin, Zm, sh);
+  // singular values = column norms; order descending (ties by index)
+  for (uint32_t c = threadIdx.x >> 6; c < nmin; c += kWaves) {
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x & 63; i < mmin; i += 64) s = fma(R[(size_t)c * mmin + i], R[(size_t)c * mmin + i], s);
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sig[c] = sqrt(s);
+  }
+  __syncthreads();
+  for (uint32_t c = threadIdx.x; c < nmin; c += kAcaThreads) {
+    uint32_t rank = 0;
+    for (uint32_t d = 0; d < nmin; ++d)
+      if (sig[d] > sig[c] || (sig[d] == sig[c] && d < c)) ++rank;
+    order[rank] = (int)c;
+  }
+  __syncthreads();
+  // kt: the reference keeps S_i > delta*S_0 and >= 1e-64 among the first amin
+  uint32_t kt = min(amin, k);
+  const double s0 = sig[order[0]];
+  while (kt > 0 && (sig[order[kt - 1]] <= eps * s0 || sig[order[kt - 1]] < 1e-64)) --kt;
+  // sorted, truncated copies of the core factors
+  for (uint32_t e = threadIdx.x; e < mmin * kt; e += kAcaThreads) {
+    const uint32_t i = e % mmin, c = e / mmin;
+    Ws[e] = R[(size_t)order[c] * mmin + i];
+  }
+  for (uint32_t e = threadIdx.x; e < nmin * kt; e += kAcaThreads) {
+    const uint32_t i = e % nmin, c = e / nmin;
+    Zs[e] = Zm[(size_t)order[c] * nmin + i];
+  }
+  if (threadIdx.x == 0) res[blockIdx.x].kt = kt;
+}
+
+// final U = Q1 [W; 0] (n1 x kt), V = Q2 [Z; 0] (n2 x kt) at the leaf's offset
+__global__ __launch_bounds__(kAcaThreads) void expand_kernel(const AcaBlock* __restrict__ blocks,
+                                                             const AcaResult* __restrict__ res,
+                                                             const uint64_t* __restrict__ out_off,
+                                                             const double* __restrict__ slab_c,
+                                                             double* __restrict__ out) {
+  __shared__ double wbuf[kAcaCapMax + 1];
+  const AcaBlock B = blocks[blockIdx.x];
+  const AcaResult r0 = res[blockIdx.x];
+  if (r0.status != 1 || r0.kt == 0) return;
+  const uint32_t k = r0.k, kt = r0.kt;
+  const uint32_t mmin = min(B.n1, k), nmin = min(B.n2, k);
+  const double* U = slab_c + B.uoff;
+  const double* V = slab_c + B.voff;
+  const size_t sq = (size_t)(B.kcap + 1) * (B.kcap + 1);
+  const double* Ws = slab_c + B.roff + 2 * sq;
+  const double* Zs = Ws + sq;
+  const double* tau1 = Zs + sq;
+  const double* tau2 = tau1 + (B.kcap + 1);
+  double* Uo = out + out_off[blockIdx.x];
+  double* Vo = Uo + (size_t)B.n1 * kt;
+  for (uint32_t e = threadIdx.x; e < B.n1 * kt; e += kAcaThreads) {
+    const uint32_t i = e % B.n1, c = e / B.n1;
+    Uo[e] = i < mmin ? Ws[(size_t)c * mmin + i] : 0.0;
+  }
+  for (uint32_t e = threadIdx.x; e < B.n2 * kt; e += kAcaThreads) {
+    const uint32_t i = e % B.n2, c = e / B.n2;
+    Vo[e] = i < nmin ? Zs[(size_t)c * nmin + i] : 0.0;
+  }
+  __syncthreads();
+  apply_q(U, B.n1, mmin, tau1, Uo, kt, wbuf);
+  apply_q(V, B.n2, nmin, tau2, Vo, kt, wbuf);
+}
+
+// dense leaves (and failed ACA): cmpbl of the whole block, column-major
+__global__ void dense_fill_kernel(const DenseJob* __restrict__ jobs, const double* __restrict__ P,
+                                  KernelSpec K, double* __restrict__ out) {
+  const DenseJob J = jobs[blockIdx.x];
+  const uint64_t n = (uint64_t)J.n1 * J.n2;
+  double* D = out + J.off;
+  for (uint64_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const uint32_t i = (uint32_t)(e % J.n1), j = (uint32_t)(e / J.n1);
+    D[e] = kentry(K, P, J.b1 + i, J.b2 + j);
+  }
+}
+
+}  // namespace ahm
+
+// ------------------------------------------------------------------------
+// host driver
+
+namespace ahm {
+
+namespace {
+
+template <typename T>
+int to_device(DeviceBuf& b, const std::vector<T>& v, hipStream_t s) {
+  AHM_TRY(b.alloc(v.size() * sizeof(T)));
+  if (!v.empty()) AHM_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return AHM_OK;
+}
+
+struct Part {
+  DeviceBuf buf;
+  uint64_t len = 0;
+};
+
+}  // namespace
+
+KernelSpec make_kernel(const ahm_kernel& k) {
+  KernelSpec K;
+  K.kind = k.kind;
+  K.kappa = k.kappa;
+  K.c = k.kind == AHM_KERNEL_EXPR ? k.weight : k.weight * 0.079577471545947667884;
+  return K;
+}
+
+int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uint32_t N,
+                    uint32_t nl, const uint32_t* b1, const uint32_t* n1, const uint32_t* b2,
+                    const uint32_t* n2, const uint8_t* adm, const uint32_t* i0, KernelSpec K,
+                    double eps, uint32_t rankmax, const uint32_t* kcap_hint,
+                    std::vector<ahm_leaf_desc>& leaves, DeviceBuf& data,
+                    ahm_assembly_stats* stats) {
+  hipStream_t s = ctx->stream;
+  std::vector<double> P(3 * (size_t)N);
+  for (uint32_t i = 0; i < N; ++i) {
+    if (op_perm[i] >= N) return fail(AHM_ERR_INVALID, "op_perm[%u] = %u out of range", i, op_perm[i]);
+    for (int j = 0; j < 3; ++j) P[3 * (size_t)i + j] = pts[3 * (size_t)op_perm[i] + j];
+  }
+  DeviceBuf dP;
+  AHM_TRY(to_device(dP, P, s));
+  // ACA candidates and rank limits (Include/apprx.h:143-151).  The product
+  // n1*n2 is taken in 64 bits (the reference's 32-bit product wraps for
+  // leaves with n1*n2 >= 2^32).
+  std::vector<uint32_t> cand, maxk(nl, 0);
+  for (uint32_t l = 0; l < nl; ++l) {
+    if (n1[l] == 0 || n2[l] == 0 || (uint64_t)b1[l] + n1[l] > N || (uint64_t)b2[l] + n2[l] > N ||
+        i0[l] >= n1[l])
+      return fail(AHM_ERR_INVALID, "leaf %u (%u,%u,%u,%u,i0=%u) invalid for N=%u", l, b1[l], n1[l],
+                  b2[l], n2[l], i0[l], N);
+    uint64_t mk = (uint64_t)n1[l] * n2[l] / ((uint64_t)n1[l] + n2[l]) + 4;
+    if (mk > rankmax) mk = rankmax;
+    maxk[l] = (uint32_t)mk;
+    if (adm[l] && mk) cand.push_back(l);
+  }
+  std::vector<int32_t> rank(nl, -1);
+  std::vector<uint32_t> kaca(nl, 0);
+  std::vector<uint32_t> cap(cand.size());
+  for (size_t c = 0; c < cand.size(); ++c) {
+    const uint32_t l = cand[c];
+    const uint32_t lim = std::min(maxk[l], kAcaCapMax);
+    const uint32_t h = kcap_hint ? std::max<uint32_t>(kcap_hint[l], 1) : kAcaCapDefault;
+    cap[c] = std::min(h, lim);
+  }
+  uint64_t retries = 0;
+  std::vector<size_t> todo(cand.size());
+  std::iota(todo.begin(), todo.end(), 0);
+  const uint64_t budget = (uint64_t)24 << 30;  // bytes of ACA scratch per batch
+  std::vector<Part*> parts;
+  std::vector<uint32_t> part_of(nl, 0);
+  std::vector<uint64_t> off_in_part(nl, 0);
+  auto cleanup = [&]() {
+    for (auto* p : parts) delete p;
+    parts.clear();
+  };
+  while (!todo.empty()) {
+    std::vector<size_t> next;
+    size_t pos = 0;
+    while (pos < todo.size()) {
+      std::vector<AcaBlock> blk;
+      uint64_t slab = 0, iscr = 0;
+      size_t end = pos;
+      while (end < todo.size()) {
+        const size_t c = todo[end];
+        const uint32_t l = cand[c];
+        const uint64_t kc = cap[c], sq = (kc + 1) * (kc + 1);
+        const uint64_t need = (kc + 1) * ((uint64_t)n1[l] + n2[l]) + 4 * sq + 2 * (kc + 1);
+        if (end > pos && (slab + need) * 8 > budget) break;
+        AcaBlock B{};
+        B.b1 = b1[l];
+        B.n1 = n1[l];
+        B.b2 = b2[l];
+        B.n2 = n2[l];
+        B.i0 = i0[l];
+        B.kmax = maxk[l];
+        B.kcap = (uint32_t)kc;
+        B.uoff = slab;
+        B.voff = slab + (kc + 1) * n1[l];
+        B.roff = B.voff + (kc + 1) * n2[l];
+        B.zoff = iscr;
+        slab += need;
+        iscr += (uint64_t)n1[l] + n2[l];
+        blk.push_back(B);
+        ++end;
+      }
+      DeviceBuf dblk, dres, dslab, discr;
+      int st = to_device(dblk, blk, s);
+      if (!st) st = dres.alloc(blk.size() * sizeof(AcaResult));
+      if (!st) st = dslab.alloc(slab * 8);
+      if (!st) st = discr.alloc(iscr * 4);
+      if (st) {
+        cleanup();
+        return st;
+      }
+      const uint32_t nb = (uint32_t)blk.size();
+      hipLaunchKernelGGL(aca_kernel, dim3(nb), dim3(kAcaThreads), 0, s, (const AcaBlock*)dblk.p,
+                         (AcaResult*)dres.p, (const double*)dP.p, K, eps, (double*)dslab.p,
+                         (int*)discr.p);
+      hipLaunchKernelGGL(recompress_kernel, dim3(nb), dim3(kAcaThreads), 0, s,
+                         (const AcaBlock*)dblk.p, (AcaResult*)dres.p, eps, (double*)dslab.p);
+      std::vector<AcaResult> rr(nb);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(rr.data(), dres.p, nb * sizeof(AcaResult), hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        cleanup();
+        return fail(AHM_ERR_HIP, "ACA / recompression kernels failed");
+      }
+      std::vector<uint64_t> off(nb, 0);
+      uint64_t tot = 0;
+      for (uint32_t q = 0; q < nb; ++q) {
+        const size_t c = todo[pos + q];
+        const uint32_t l = cand[c];
+        if (rr[q].status == 2) {
+          // slab too small: ACA continues with a larger slab
+          const uint32_t lim = std::min(maxk[l], kAcaCapMax);
+          if (cap[c] >= lim) {
+            rank[l] = -1;  // rank beyond the device ceiling: dense leaf
+            continue;
+          }
+          cap[c] = std::min(std::max(cap[c] * 4, cap[c] + 1), lim);
+          next.push_back(c);
+          ++retries;
+          continue;
+        }
+        kaca[l] = rr[q].k;
+        if (rr[q].status == 1) {
+          rank[l] = (int32_t)rr[q].kt;
+          off[q] = tot;
+          off_in_part[l] = tot;
+          part_of[l] = (uint32_t)parts.size();
+          tot += (uint64_t)rr[q].kt * ((uint64_t)n1[l] + n2[l]);
+        }
+      }
+      Part* part = new Part;
+      parts.push_back(part);
+      part->len = tot;
+      if (tot) {
+        DeviceBuf doff;
+        st = part->buf.alloc(tot * 8);
+        if (!st) st = to_device(doff, off, s);
+        if (st) {
+          cleanup();
+          return st;
+        }
+        hipLaunchKernelGGL(expand_kernel, dim3(nb), dim3(kAcaThreads), 0, s, (const AcaBlock*)dblk.p,
+                           (const AcaResult*)dres.p, (const uint64_t*)doff.p,
+                           (const double*)dslab.p, (double*)part->buf.p);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+          cleanup();
+          return fail(AHM_ERR_HIP, "expand kernel failed");
+        }
+      }
+      pos = end;
+    }
+    todo.swap(next);
+  }
+  // final buffer: the batch parts back to back, then the dense leaves
+  std::vector<uint64_t> part_base(parts.size(), 0);
+  uint64_t total = 0;
+  for (size_t q = 0; q < parts.size(); ++q) {
+    part_base[q] = total;
+    total += parts[q]->len;
+  }
+  leaves.resize(nl);
+  std::vector<DenseJob> dj;
+  for (uint32_t l = 0; l < nl; ++l) {
+    ahm_leaf_desc& d = leaves[l];
+    d.b1 = b1[l];
+    d.n1 = n1[l];
+    d.b2 = b2[l];
+    d.n2 = n2[l];
+    d.rank = rank[l];
+    d.flags = 0;
+    if (rank[l] > 0) d.offset = part_base[part_of[l]] + off_in_part[l];
+    else if (rank[l] == 0) d.offset = 0;
+    else {
+      d.offset = total;
+      dj.push_back(DenseJob{b1[l], n1[l], b2[l], n2[l], total});
+      total += (uint64_t)n1[l] * n2[l];
+    }
+  }
+  int st = data.alloc(total * 8);
+  if (st) {
+    cleanup();
+    return st;
+  }
+  for (size_t q = 0; q < parts.size(); ++q)
+    if (parts[q]->len &&
+        hipMemcpyAsync((double*)data.p + part_base[q], parts[q]->buf.p, parts[q]->len * 8,
+                       hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      cleanup();
+      return fail(AHM_ERR_HIP, "copy of assembled leaves failed");
+    }
+  DeviceBuf ddj;
+  st = to_device(ddj, dj, s);
+  if (st) {
+    cleanup();
+    return st;
+  }
+  for (size_t j0 = 0; j0 < dj.size(); j0 += (1u << 30)) {
+    const uint32_t nj = (uint32_t)std::min<size_t>(1u << 30, dj.size() - j0);
+    hipLaunchKernelGGL(dense_fill_kernel, dim3(nj), dim3(256), 0, s, (const DenseJob*)ddj.p + j0,
+                       (const double*)dP.p, K, (double*)data.p);
+  }
+  const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+  cleanup();
+  if (!ok) return fail(AHM_ERR_HIP, "dense leaf kernel failed");
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->n_candidates = cand.size();
+    stats->n_retries = retries;
+    stats->n_dense = dj.size();
+    for (uint32_t l = 0; l < nl; ++l)
+      if (rank[l] >= 0) {
+        stats->n_lowrank++;
+        stats->sum_rank += (uint64_t)rank[l];
+        stats->sum_aca_rank += kaca[l];
+      }
+  }
+  return AHM_OK;
+}
+
+}  // namespace ahm
+
+extern "C" int ahm_assemble(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uint32_t N,
+                            uint32_t n_leaves, const uint32_t* b1, const uint32_t* n1,
+                            const uint32_t* b2, const uint32_t* n2, const uint8_t* adm,
+                            const uint32_t* i0, const ahm_kernel* kernel, double eps,
+                            uint32_t rankmax, const uint32_t* rank_hint, ahm_dtype storage,
+                            int keep_leaves, ahm_assembly_stats* stats, ahm_hmat* out) {
+  using namespace ahm;
+  if (!ctx || !pts || !op_perm || !kernel || !out || !b1 || !n1 || !b2 || !n2 || !adm || !i0)
+    return fail(AHM_ERR_INVALID, "ahm_assemble: NULL argument");
+  if (kernel->kind < AHM_KERNEL_LAPLACE || kernel->kind > AHM_KERNEL_EXPR)
+    return fail(AHM_ERR_INVALID, "ahm_assemble: unknown kernel %d", kernel->kind);
+  if (!(eps > 0.0)) return fail(AHM_ERR_INVALID, "ahm_assemble: eps must be > 0");
+  AHM_HIP(hipSetDevice(ctx->device));
+  std::vector<ahm_leaf_desc> leaves;
+  DeviceBuf data;
+  AHM_TRY(assemble_leaves(ctx, pts, op_perm, N, n_leaves, b1, n1, b2, n2, adm, i0,
+                          make_kernel(*kernel), eps, rankmax, rank_hint, leaves, data, stats));
+  return hmat_from_device(ctx, N, N, std::move(leaves), data, storage, keep_leaves, out);
+}

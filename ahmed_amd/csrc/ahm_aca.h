@@ -1,0 +1,58 @@
+// ahm_aca.h -- device assembly declarations (aca.hip).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "ahm_internal.h"
+
+namespace ahm {
+
+// ACA slab capacity: default rank columns per leaf and the hard ceiling
+constexpr uint32_t kAcaCapDefault = 48;
+constexpr uint32_t kAcaCapMax = 256;
+
+struct KernelSpec {
+  int kind;      // AHM_KERNEL_*
+  double kappa;
+  double c;      // weight/(4 pi) (Laplace, Yukawa) or weight (EXPR)
+};
+
+struct AcaBlock {
+  uint32_t b1, n1, b2, n2;
+  uint32_t i0;        // ACA start row (relative to b1)
+  uint32_t kmax;      // reference rank limit n1*n2/(n1+n2)+4 (or rankmax)
+  uint32_t kcap;      // slab capacity in rank columns (index 0..kcap)
+  uint32_t pad;
+  uint64_t uoff, voff, roff;  // slab offsets (doubles)
+  uint64_t zoff;              // Z/S offsets (ints)
+};
+
+struct AcaResult {
+  int32_t status;     // 0 ACA failed, 1 success, 2 slab capacity reached
+  uint32_t k;         // ACA rank
+  uint32_t kt;        // rank after recompression
+  uint32_t pad;
+};
+
+struct DenseJob {
+  uint32_t b1, n1, b2, n2;
+  uint64_t off;
+};
+
+KernelSpec make_kernel(const ahm_kernel& k);
+
+int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uint32_t N,
+                    uint32_t nl, const uint32_t* b1, const uint32_t* n1, const uint32_t* b2,
+                    const uint32_t* n2, const uint8_t* adm, const uint32_t* i0, KernelSpec K,
+                    double eps, uint32_t rankmax, const uint32_t* kcap_hint,
+                    std::vector<ahm_leaf_desc>& leaves, DeviceBuf& data,
+                    ahm_assembly_stats* stats);
+
+// builds an H-matrix from leaves whose values are already on the device
+// (mblock layout, F64); takes ownership of `raw`
+int hmat_from_device(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols,
+                     std::vector<ahm_leaf_desc>&& leaves, DeviceBuf& raw, ahm_dtype storage,
+                     int keep_leaves, ahm_hmat* out);
+
+}  // namespace ahm

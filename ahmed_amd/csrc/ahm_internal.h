@@ -64,6 +64,10 @@ struct DeviceBuf {
 
 size_t dtype_size(ahm_dtype t);
 
+int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_leaf_desc>&& leaves,
+               struct DeviceBuf& raw, ahm_dtype data_dtype, ahm_dtype storage, int keep_leaves,
+               ahm_hmat* out);
+
 }  // namespace ahm
 
 struct ahm_ctx_s {

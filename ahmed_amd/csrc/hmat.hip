@@ -20,6 +20,7 @@
 #include <numeric>
 
 #include "ahm_internal.h"
+#include "ahm_aca.h"
 
 namespace ahm {
 
@@ -309,30 +310,18 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
 
 }  // namespace ahm
 
-using namespace ahm;
+namespace ahm {
 
-extern "C" {
-
-int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_leaves,
-                    const ahm_leaf_desc* leaves, const void* data, ahm_dtype data_dtype,
-                    ahm_dtype storage, int keep_leaves, ahm_hmat* out) {
-  if (!ctx || !out || (!leaves && n_leaves))
-    return fail(AHM_ERR_INVALID, "ahm_hmat_upload: NULL argument");
-  if ((data_dtype != AHM_F64 && data_dtype != AHM_F32) || (storage != AHM_F64 && storage != AHM_F32))
-    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_upload: dtypes %d -> %d", data_dtype, storage);
-  if (data_dtype == AHM_F32 && storage == AHM_F64)
-    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_upload: float leaves cannot be stored as double");
-  AHM_HIP(hipSetDevice(ctx->device));
+int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_leaf_desc>&& leaves,
+               DeviceBuf& raw, ahm_dtype data_dtype, ahm_dtype storage, int keep_leaves,
+               ahm_hmat* out) {
   Plan P;
-  AHM_TRY(build_plan(n_rows, n_cols, n_leaves, leaves, P));
-  // leaf values in mblock layout: total = end of the last leaf
-  uint64_t total = 0;
-  for (uint32_t l = 0; l < n_leaves; ++l) {
-    const ahm_leaf_desc& d = leaves[l];
+  AHM_TRY(build_plan(n_rows, n_cols, (uint32_t)leaves.size(), leaves.data(), P));
+  for (const ahm_leaf_desc& d : leaves) {
     const uint64_t nv = d.rank < 0 ? (uint64_t)d.n1 * d.n2 : (uint64_t)d.rank * (d.n1 + d.n2);
-    total = std::max(total, d.offset + nv);
+    if ((d.offset + nv) * dtype_size(data_dtype) > raw.bytes && nv)
+      return fail(AHM_ERR_INVALID, "leaf values beyond the data buffer");
   }
-  if (total && !data) return fail(AHM_ERR_INVALID, "ahm_hmat_upload: data is NULL");
   ahm_hmat h = new ahm_hmat_s;
   h->ctx = ctx;
   h->n_rows = n_rows;
@@ -344,17 +333,12 @@ int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_le
   h->max_rank = P.max_rank;
   h->n_dense = P.n_dense;
   h->n_lowrank = P.n_lowrank;
-  h->leaves.assign(leaves, leaves + n_leaves);
+  h->leaves = std::move(leaves);
   hipStream_t s = ctx->stream;
-  const size_t ds = dtype_size(data_dtype), ss = dtype_size(storage);
+  const size_t ss = dtype_size(storage);
   int st = AHM_OK;
-  DeviceBuf raw, jobs;
+  DeviceBuf jobs;
   do {
-    if ((st = raw.alloc(total * ds))) break;
-    if (total && hipMemcpyAsync(raw.p, data, total * ds, hipMemcpyHostToDevice, s) != hipSuccess) {
-      st = fail(AHM_ERR_HIP, "ahm_hmat_upload: copy of %llu values failed", (unsigned long long)total);
-      break;
-    }
     if ((st = h->arena.alloc(P.nvals * ss))) break;
     if ((st = upload_vec(jobs, P.jobs, s))) break;
     if ((st = upload_vec(h->tasks_a, P.a, s))) break;
@@ -363,7 +347,6 @@ int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_le
     h->n_a = (uint32_t)P.a.size();
     h->n_b = (uint32_t)P.b.size();
     h->n_zsrc = P.zsrc.size();
-    // launch the re-layout in slices (grid.x limit)
     for (size_t j0 = 0; j0 < P.jobs.size(); j0 += (1u << 30)) {
       const uint32_t nj = (uint32_t)std::min<size_t>(1u << 30, P.jobs.size() - j0);
       const CopyJob* J = (const CopyJob*)jobs.p + j0;
@@ -378,7 +361,7 @@ int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_le
                            (const float*)raw.p, (float*)h->arena.p);
     }
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
-      st = fail(AHM_ERR_HIP, "ahm_hmat_upload: re-layout kernel failed");
+      st = fail(AHM_ERR_HIP, "H-matrix re-layout kernel failed");
       break;
     }
     h->h_tasks_a = std::move(P.a);
@@ -397,6 +380,44 @@ int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_le
   }
   *out = h;
   return AHM_OK;
+}
+
+int hmat_from_device(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols,
+                     std::vector<ahm_leaf_desc>&& leaves, DeviceBuf& raw, ahm_dtype storage,
+                     int keep_leaves, ahm_hmat* out) {
+  return hmat_build(ctx, n_rows, n_cols, std::move(leaves), raw, AHM_F64, storage, keep_leaves, out);
+}
+
+}  // namespace ahm
+
+using namespace ahm;
+
+extern "C" {
+
+int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_leaves,
+                    const ahm_leaf_desc* leaves, const void* data, ahm_dtype data_dtype,
+                    ahm_dtype storage, int keep_leaves, ahm_hmat* out) {
+  if (!ctx || !out || (!leaves && n_leaves))
+    return fail(AHM_ERR_INVALID, "ahm_hmat_upload: NULL argument");
+  if ((data_dtype != AHM_F64 && data_dtype != AHM_F32) || (storage != AHM_F64 && storage != AHM_F32))
+    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_upload: dtypes %d -> %d", data_dtype, storage);
+  if (data_dtype == AHM_F32 && storage == AHM_F64)
+    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_upload: float leaves cannot be stored as double");
+  AHM_HIP(hipSetDevice(ctx->device));
+  uint64_t total = 0;
+  for (uint32_t l = 0; l < n_leaves; ++l) {
+    const ahm_leaf_desc& d = leaves[l];
+    const uint64_t nv = d.rank < 0 ? (uint64_t)d.n1 * d.n2 : (uint64_t)d.rank * (d.n1 + d.n2);
+    total = std::max(total, d.offset + nv);
+  }
+  if (total && !data) return fail(AHM_ERR_INVALID, "ahm_hmat_upload: data is NULL");
+  DeviceBuf raw;
+  AHM_TRY(raw.alloc(total * dtype_size(data_dtype)));
+  if (total)
+    AHM_HIP(hipMemcpyAsync(raw.p, data, total * dtype_size(data_dtype), hipMemcpyHostToDevice,
+                           ctx->stream));
+  std::vector<ahm_leaf_desc> L(leaves, leaves + n_leaves);
+  return hmat_build(ctx, n_rows, n_cols, std::move(L), raw, data_dtype, storage, keep_leaves, out);
 }
 
 int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {

@@ -110,6 +110,46 @@ int ahm_hmat_stats_get(ahm_hmat h, ahm_hmat_stats* st);
  * query descriptors only. */
 int ahm_hmat_download(ahm_hmat h, ahm_leaf_desc* leaves, void* data);
 
+/* -------- assembly on the GPU ---------------------------------------------
+ * The reference's matgenGeH_omp / apprx_unsym / ACA / cpyLrM_cmpr pipeline
+ * (Include/matgen_omp.h:59-75, Include/apprx.h:136-172, Include/ACA.h:217-350,
+ * H/mblock_D.cpp:357-465) for kernels the device can evaluate (the MATGEN_T
+ * plugin, Include/matgen_sqntl.h:21-29, becomes an ahm_kernel).  Entry (i,j)
+ * in permuted order is 1.0 on the diagonal, otherwise weight*G(|x_i - x_j|). */
+#define AHM_KERNEL_LAPLACE 0   /* G = 1/(4 pi r)            */
+#define AHM_KERNEL_YUKAWA  1   /* G = exp(-kappa r)/(4 pi r) */
+#define AHM_KERNEL_EXPR    2   /* G = exp(-kappa r)/r        */
+typedef struct {
+  int32_t kind;
+  uint32_t pad;
+  double kappa;
+  double weight;
+} ahm_kernel;
+
+typedef struct {
+  uint64_t n_candidates;    /* admissible leaves that ran ACA              */
+  uint64_t n_retries;       /* ACA reruns after a rank slab was too small   */
+  uint64_t n_lowrank, n_dense;
+  uint64_t sum_rank;        /* sum of ranks after recompression            */
+  uint64_t sum_aca_rank;    /* sum of ACA ranks before recompression       */
+} ahm_assembly_stats;
+
+/* pts: N x 3 in original order, op_perm from the tree; leaves in idx order
+ * with admissibility adm[] and ACA start rows i0[] (relative to b1).
+ * rank_hint (may be NULL): per-leaf rank-slab size; ACA keeps going past an
+ * under-sized slab (the leaf is rerun with a larger one), so hints change
+ * memory and time, never the result.  stats may be NULL. */
+int ahm_assemble(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uint32_t N,
+                 uint32_t n_leaves, const uint32_t* b1, const uint32_t* n1, const uint32_t* b2,
+                 const uint32_t* n2, const uint8_t* adm, const uint32_t* i0,
+                 const ahm_kernel* kernel, double eps, uint32_t rankmax,
+                 const uint32_t* rank_hint, ahm_dtype storage, int keep_leaves,
+                 ahm_assembly_stats* stats, ahm_hmat* out);
+/* same, taking the points, permutation and leaves from a tree */
+int ahm_assemble_tree(ahm_ctx ctx, ahm_tree tree, const ahm_kernel* kernel, double eps,
+                      uint32_t rankmax, const uint32_t* rank_hint, ahm_dtype storage,
+                      int keep_leaves, ahm_assembly_stats* stats, ahm_hmat* out);
+
 /* -------- products: mltaGeHVec / mltaGeHGeM (H/mltaGeHVec.cpp:567-577) ----
  * Y += alpha * H * X for p right-hand sides; X is n_cols x p column-major
  * with leading dimension ldx, Y n_rows x p with ldy (device pointers).

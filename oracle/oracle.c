@@ -617,7 +617,7 @@ static int aca(const matgen* G, unsigned b1, unsigned n1, unsigned b2, unsigned 
     nrms2 += 2.0 * sum + nrmlsk2;
     if (nrmlsk2 < eps * eps * nrms2 && keps == -1) keps = (int)(klast + 1u);
     ++k;
-    scale = sqrt(nrmlsk2 / (n1 * n2));
+    scale = sqrt(nrmlsk2 / ((double)n1 * n2));
     if (keps > 0) {
       for (i0 = 0; i0 < n1 && Z[i0] < 0; ++i0) ;
       if (i0 == n1) { nrmlsk2 = 0.0; goto SUCC; }
@@ -793,7 +793,8 @@ static size_t leaf_nvals(unsigned n1, unsigned n2, int rank)
 static double* apprx_unsym(const matgen* G, unsigned b1, unsigned n1, unsigned b2, unsigned n2,
                            int adm, unsigned i0, double eps, unsigned rankmax, int* rank)
 {
-  unsigned maxk = n1 * n2 / (n1 + n2) + 4;
+  /* 64-bit product: the reference's 32-bit n1*n2 wraps for n1*n2 >= 2^32 */
+  unsigned maxk = (unsigned)((unsigned long long)n1 * n2 / (n1 + n2) + 4);
   if (maxk > rankmax) maxk = rankmax;
   if (adm && maxk) {
     unsigned k;
