@@ -92,7 +92,7 @@ int ahm_ctx_destroy(ahm_ctx c) {
 
 int ahm_ctx_set_stream(ahm_ctx c, void* s) {
   if (!c) return ahm::fail(AHM_ERR_INVALID, "ahm_ctx_set_stream: NULL context");
-  c->stream = s ? (hipStream_t)s : c->own;
+  c->stream = (hipStream_t)s;  // NULL: HIP's default (null) stream
   return AHM_OK;
 }
 

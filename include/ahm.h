@@ -73,7 +73,8 @@ const char* ahm_last_error(void);
 /* -------- context: one device, one HIP stream ---------------------------- */
 int ahm_ctx_create(int device, ahm_ctx* out);
 int ahm_ctx_destroy(ahm_ctx ctx);
-/* use an existing hipStream_t (NULL = the context's own stream) */
+/* run on an existing hipStream_t from now on (NULL = HIP's default stream);
+ * until this is called the context uses a stream of its own */
 int ahm_ctx_set_stream(ahm_ctx ctx, void* hip_stream);
 int ahm_ctx_sync(ahm_ctx ctx);
 
