@@ -32,7 +32,8 @@ assert LEAF_DTYPE.itemsize == C.sizeof(LeafDesc)
 
 class HMatStats(C.Structure):
     _fields_ = [("n_leaves", C.c_uint64), ("n_dense", C.c_uint64), ("n_lowrank", C.c_uint64),
-                ("nvals", C.c_uint64), ("t_len", C.c_uint64), ("bytes_per_value", C.c_uint64),
+                ("nvals", C.c_uint64), ("nvals_dense", C.c_uint64), ("nvals_u", C.c_uint64),
+                ("nvals_v", C.c_uint64), ("t_len", C.c_uint64), ("bytes_per_value", C.c_uint64),
                 ("n_tasks_a", C.c_uint64), ("n_tasks_b", C.c_uint64), ("max_rank", C.c_uint32),
                 ("n_rows", C.c_uint32), ("n_cols", C.c_uint32), ("pad", C.c_uint32)]
 
@@ -79,6 +80,8 @@ SIGNATURES = {
                                C.POINTER(_p)]),
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_matvec_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
+    "ahm_hmat_matvec_profile": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i, _u32, C.POINTER(_d),
+                                     C.POINTER(_d)]),
 }
 
 
@@ -261,6 +264,20 @@ class HMatrix:
         check(load().ahm_hmat_matvec(self.h, alpha, _p(x.data_ptr()), ldx, _p(y.data_ptr()), ldy, p,
                                      vec), "ahm_hmat_matvec")
         return y
+
+    def profile(self, x, y, reps: int = 10, alpha: float = 1.0):
+        """Mean (ms_phase_a, ms_phase_b) of `reps` products timed with HIP events."""
+        torch = _torch()
+        vec = F64 if x.dtype == torch.float64 else F32
+        if x.dim() == 1:
+            p, ldx, ldy = 1, x.shape[0], y.shape[0]
+        else:
+            p, ldx, ldy = x.shape[0], x.stride(0), y.stride(0)
+        a, b = _d(), _d()
+        self.ctx.bind_stream()
+        check(load().ahm_hmat_matvec_profile(self.h, alpha, _p(x.data_ptr()), ldx, _p(y.data_ptr()), ldy,
+                                             p, vec, reps, C.byref(a), C.byref(b)), "matvec_profile")
+        return a.value, b.value
 
     def matvec_host(self, x: np.ndarray, y: np.ndarray, alpha: float = 1.0):
         x = np.asfortranarray(x, np.float64)

@@ -85,6 +85,7 @@ struct ahm_hmat_s {
   uint32_t t_len = 0;
   uint32_t max_rank = 0;
   uint64_t n_dense = 0, n_lowrank = 0;
+  uint64_t nv_dense = 0, nv_u = 0, nv_v = 0;
   ahm::DeviceBuf arena;       // nvals values in panel layout
   ahm::DeviceBuf tasks_a, tasks_b, zsrc;
   uint32_t n_a = 0, n_b = 0;

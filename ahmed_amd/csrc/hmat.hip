@@ -124,6 +124,7 @@ struct Plan {
   uint64_t nvals = 0;
   uint32_t t_len = 0, max_rank = 0;
   uint64_t n_dense = 0, n_lowrank = 0;
+  uint64_t nv_dense = 0, nv_u = 0, nv_v = 0;
 };
 
 static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_leaf_desc* L,
@@ -266,10 +267,14 @@ static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_l
   P.nvals = arena;
   P.t_len = (uint32_t)t_len;
   for (uint32_t l = 0; l < nl; ++l) {
-    if (L[l].rank < 0) P.n_dense++;
-    else {
+    if (L[l].rank < 0) {
+      P.n_dense++;
+      P.nv_dense += (uint64_t)L[l].n1 * L[l].n2;
+    } else {
       P.n_lowrank++;
       P.max_rank = std::max(P.max_rank, (uint32_t)L[l].rank);
+      P.nv_u += (uint64_t)L[l].rank * L[l].n1;
+      P.nv_v += (uint64_t)L[l].rank * L[l].n2;
     }
   }
   return AHM_OK;
@@ -293,13 +298,14 @@ static int ensure_tbuf(ahm_hmat h, uint32_t p, size_t esize) {
 
 template <typename TS, typename TV>
 static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y, uint32_t ldy,
-                         uint32_t p) {
+                         uint32_t p, hipEvent_t ev_mid = nullptr) {
   hipStream_t s = h->ctx->stream;
   TV* t = (TV*)h->tbuf.p;
   if (h->n_a)
     hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, false>), dim3(h->n_a, p), dim3(kPanelThreads), 0, s,
                        (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
                        (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, alpha);
+  if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
   if (h->n_b)
     hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, true>), dim3(h->n_b, p), dim3(kPanelThreads), 0, s,
                        (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
@@ -333,6 +339,9 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
   h->max_rank = P.max_rank;
   h->n_dense = P.n_dense;
   h->n_lowrank = P.n_lowrank;
+  h->nv_dense = P.nv_dense;
+  h->nv_u = P.nv_u;
+  h->nv_v = P.nv_v;
   h->leaves = std::move(leaves);
   hipStream_t s = ctx->stream;
   const size_t ss = dtype_size(storage);
@@ -437,6 +446,9 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
   h->max_rank = src->max_rank;
   h->n_dense = src->n_dense;
   h->n_lowrank = src->n_lowrank;
+  h->nv_dense = src->nv_dense;
+  h->nv_u = src->nv_u;
+  h->nv_v = src->nv_v;
   h->leaves = src->leaves;
   h->h_tasks_a = src->h_tasks_a;
   h->h_tasks_b = src->h_tasks_b;
@@ -481,6 +493,9 @@ int ahm_hmat_stats_get(ahm_hmat h, ahm_hmat_stats* st) {
   st->n_dense = h->n_dense;
   st->n_lowrank = h->n_lowrank;
   st->nvals = h->nvals;
+  st->nvals_dense = h->nv_dense;
+  st->nvals_u = h->nv_u;
+  st->nvals_v = h->nv_v;
   st->t_len = h->t_len;
   st->bytes_per_value = dtype_size(h->storage);
   st->n_tasks_a = h->n_a;
@@ -523,6 +538,49 @@ int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void*
     return launch_matvec<float, float>(h, (float)alpha, (const float*)X, ldx, (float*)Y, ldy, p);
   }
   return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec: storage %d with vectors %d", h->storage, vec);
+}
+
+int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y,
+                            uint32_t ldy, uint32_t p, ahm_dtype vec, uint32_t reps,
+                            double* ms_a, double* ms_b) {
+  if (!h || !X || !Y || !ms_a || !ms_b || reps == 0)
+    return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_profile: bad arguments");
+  if (!((h->storage == AHM_F64 && vec == AHM_F64) || (h->storage == AHM_F32 && vec == AHM_F32)))
+    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec_profile: storage %d with vectors %d",
+                h->storage, vec);
+  if (p == 0 || p > 65535 || (p > 1 && (ldx < h->n_cols || ldy < h->n_rows)))
+    return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_profile: bad p / leading dimensions");
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  AHM_TRY(ensure_tbuf(h, p, dtype_size(vec)));
+  hipEvent_t e0, e1, e2;
+  AHM_HIP(hipEventCreate(&e0));
+  AHM_HIP(hipEventCreate(&e1));
+  AHM_HIP(hipEventCreate(&e2));
+  double ta = 0.0, tb = 0.0;
+  int st = AHM_OK;
+  for (uint32_t r = 0; r < reps && st == AHM_OK; ++r) {
+    if (hipEventRecord(e0, h->ctx->stream) != hipSuccess) { st = fail(AHM_ERR_HIP, "event"); break; }
+    if (vec == AHM_F64)
+      st = launch_matvec<double, double>(h, alpha, (const double*)X, ldx, (double*)Y, ldy, p, e1);
+    else
+      st = launch_matvec<float, float>(h, (float)alpha, (const float*)X, ldx, (float*)Y, ldy, p, e1);
+    if (st) break;
+    float a = 0.f, b = 0.f;
+    if (hipEventRecord(e2, h->ctx->stream) != hipSuccess || hipEventSynchronize(e2) != hipSuccess ||
+        hipEventElapsedTime(&a, e0, e1) != hipSuccess || hipEventElapsedTime(&b, e1, e2) != hipSuccess) {
+      st = fail(AHM_ERR_HIP, "ahm_hmat_matvec_profile: event timing failed");
+      break;
+    }
+    ta += a;
+    tb += b;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  if (st) return st;
+  *ms_a = ta / reps;
+  *ms_b = tb / reps;
+  return AHM_OK;
 }
 
 int ahm_hmat_matvec_host(ahm_hmat h, double alpha, const double* X, uint32_t ldx, double* Y,

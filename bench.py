@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""H-matrix hot-path benchmark on one MI355X.
+
+Workload (the one-GPU FP64 configuration): N = 262144 points uniform on the
+unit sphere (seed 0), bounding-box cluster tree with leaf size 64, eta = 0.8,
+ACA eps = 1e-8, Laplace kernel 1/(4 pi r) with diagonal 1.0 and quadrature
+weight 4 pi / N.  The tree is built by the C++ surface, the H-matrix is
+assembled on the GPU (batched ACA + recompression), then the H-matvec
+y += H x is timed.  A "step" is one H-matvec with 1 right-hand side;
+``value`` is its algorithmic HBM throughput:
+
+    B_alg = 8 * sum_leaves nvals + 8 * (N_cols + 2 N_rows)   bytes per step
+
+(leaf values read once, x read once, y read and written once).  The same
+line also carries the 32-RHS product, FP32 storage, the roofline of the
+dominant kernel (row-panel phase) and the reference's own CPU mltaGeHVec
+timed on this host on the identical H-matrix.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "H-matvec HBM GB/s; GMRES time-to-1e-8 mixed vs FP64 at N=1M, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+FP32_PEAK_TFS = 157.3
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def timed(fn, torch, steps: int, warmup: int):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps  # ms per step
+
+
+def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, budget_s=20.0):
+    """The reference's compiled mltaGeHVec (oracle/_ref) on this host's cores,
+    on the identical leaves; falls back to the oracle restatement."""
+    import oracle as O
+    x = np.random.default_rng(7).standard_normal(len(pts))
+    out = {}
+    try:
+        R = O.Reference()
+        T = R.tree(pts, bmin, eta2, O.TREE_BBX)
+        lv = T.leaves()
+        if not (np.array_equal(lv["n1"], leaves["n1"]) and np.array_equal(lv["b2"], leaves["b2"])):
+            raise RuntimeError("reference tree differs from the product tree")
+        H = R.from_data(T, leaves["rank"], leaves["offset"], data)
+        kind = "reference"
+    except (FileNotFoundError, OSError, RuntimeError) as e:
+        log(f"reference library unavailable ({e}); timing the oracle restatement")
+        o = O.Oracle()
+        H = o.from_leaves(len(pts), leaves["b1"], leaves["n1"], leaves["b2"], leaves["n2"], leaves["rank"],
+                          leaves["offset"], data)
+        kind = "port"
+    H.matvec(x)  # warm-up
+    ts = []
+    t_start = time.perf_counter()
+    while len(ts) < 5 and (time.perf_counter() - t_start) < budget_s:
+        t0 = time.perf_counter()
+        y = H.matvec(x)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    out.update(value=round(B_alg / t / 1e9, 3), unit="GB/s", cores=1, kind=kind,
+               sample=f"{len(ts)} serial mltaGeHVec calls (median {t:.3f} s each) on the identical "
+                      f"H-matrix, N={len(pts)}")
+    # all-core leaf-parallel restatement (per-thread y), for reference
+    try:
+        o = O.Oracle()
+        Ho = o.from_leaves(len(pts), leaves["b1"], leaves["n1"], leaves["b2"], leaves["n2"], leaves["rank"],
+                           leaves["offset"], data)
+        ncore = len(os.sched_getaffinity(0))
+        nthr = min(ncore, 16)
+        o.matvec_omp(Ho, x, nthreads=nthr)
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 3 and time.perf_counter() - t0 < budget_s:
+            yo = o.matvec_omp(Ho, x, nthreads=nthr)
+            reps += 1
+        to = (time.perf_counter() - t0) / reps
+        out["all_cores"] = {"value": round(B_alg / to / 1e9, 3), "unit": "GB/s", "cores": nthr,
+                            "kind": "port", "rel_diff_vs_serial": float(np.max(np.abs(yo - y)) / np.max(np.abs(y)))}
+    except Exception as e:  # report, never hide
+        out["all_cores"] = {"error": str(e)}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--rhs", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding of the H-matvec is a design in DESIGN.md, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ahmed_amd import _lib, geometry
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    N, bmin, eta, eps = args.n, 64, 0.8, 1e-8
+    weight = 4 * np.pi / N
+    ctx = _lib.Context(0)
+    pts = geometry.sphere(N, 0)
+    t0 = time.perf_counter()
+    tree = _lib.Tree(pts, bmin, eta * eta)
+    t_tree = time.perf_counter() - t0
+    log(f"tree: {tree.n_leaves} leaves in {t_tree:.2f} s")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    H, ast = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_LAPLACE, 0.0, weight, eps,
+                                   keep_leaves=not args.no_cpu)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    st = H.stats()
+    log(f"GPU assembly {t_asm:.2f} s: {st['n_lowrank']} low-rank / {st['n_dense']} dense, "
+        f"nvals {st['nvals']:.4g}, ACA retries {ast['n_retries']}")
+
+    # ---- 1 RHS, FP64 ---------------------------------------------------------
+    x = torch.randn(N, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    y = torch.zeros(N, dtype=torch.float64, device="cuda")
+    B_alg = 8.0 * st["nvals"] + 8.0 * (N + 2 * N)
+    ms = timed(lambda: H.matvec(x, y), torch, args.steps, args.warmup)
+    gbs = B_alg / (ms * 1e-3) / 1e9
+    log(f"FP64 H-matvec 1 RHS: {ms:.3f} ms, {gbs:.1f} GB/s")
+    ms_a, ms_b = H.profile(x, y, reps=args.steps)
+    bytes_b = 8.0 * (st["nvals_dense"] + st["nvals_u"]) + 8.0 * (N + st["t_len"] + 2 * N)
+    bytes_a = 8.0 * st["nvals_v"] + 8.0 * (N + st["t_len"])
+    ach_b = bytes_b / (ms_b * 1e-3) / 1e9
+    log(f"phase A (V^T x) {ms_a:.3f} ms {bytes_a / (ms_a * 1e-3) / 1e9:.1f} GB/s; "
+        f"phase B (row panels) {ms_b:.3f} ms {ach_b:.1f} GB/s")
+
+    # ---- p RHS, FP64 ---------------------------------------------------------
+    p = args.rhs
+    X = torch.randn(p, N, dtype=torch.float64, device="cuda")
+    Y = torch.zeros(p, N, dtype=torch.float64, device="cuda")
+    msp = timed(lambda: H.matvec(X, Y), torch, max(3, args.steps // 4), 1)
+    F_alg = 2.0 * p * st["nvals"]
+    Bp = 8.0 * st["nvals"] + 8.0 * p * (N + 2 * N)
+    log(f"FP64 H-matvec {p} RHS: {msp:.3f} ms, {F_alg / (msp * 1e-3) / 1e12:.2f} TF/s")
+
+    # ---- FP32 storage ----------------------------------------------------------
+    H32 = H.convert()
+    x32, y32 = x.float(), torch.zeros(N, dtype=torch.float32, device="cuda")
+    ms32 = timed(lambda: H32.matvec(x32, y32), torch, args.steps, args.warmup)
+    B32 = 4.0 * st["nvals"] + 4.0 * 3 * N
+    yref = torch.zeros(N, dtype=torch.float64, device="cuda")
+    H.matvec(x, yref)
+    y32.zero_()
+    H32.matvec(x32, y32)
+    err32 = float((y32.double() - yref).abs().max() / yref.abs().max())
+    log(f"FP32 H-matvec 1 RHS: {ms32:.3f} ms, {B32 / (ms32 * 1e-3) / 1e9:.1f} GB/s, rel.err vs FP64 {err32:.2e}")
+
+    cpu = None
+    if not args.no_cpu:
+        leaves, data = H.download()
+        log(f"CPU baseline on {len(leaves)} leaves ({data.nbytes / 1e9:.1f} GB)")
+        cpu = cpu_baseline(pts, leaves, data, bmin, eta * eta, B_alg)
+        log(f"CPU: {cpu}")
+        del data
+
+    line = {
+        "metric": METRIC,
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded points on the unit sphere; H-matrix assembled on the GPU)",
+        "config": {"workload": f"FP64 H-matvec, 1 RHS, N={N} unit sphere, leaf size {bmin}, eta {eta}, "
+                               f"ACA eps {eps}, Laplace kernel (weight 4pi/N)",
+                   "N": N, "leaf_size": bmin, "eta": eta, "aca_eps": eps, "rhs": 1,
+                   "leaves": st["n_leaves"], "lowrank_leaves": st["n_lowrank"], "dense_leaves": st["n_dense"],
+                   "nvals": st["nvals"], "bytes_per_step": B_alg},
+        "roofline": {"bound": "hbm", "kernel": "panel_gemv_kernel<double,double,PHASE_B> (row panels)",
+                     "achieved": round(ach_b, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach_b / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": bytes_b, "ms_per_launch": round(ms_b, 4),
+                     "phase_a": {"ms": round(ms_a, 4), "GB/s": round(bytes_a / (ms_a * 1e-3) / 1e9, 2)}},
+        "cpu_baseline": cpu,
+        "matvec_multi_rhs": {"rhs": p, "ms": round(msp, 4),
+                             "TF/s": round(F_alg / (msp * 1e-3) / 1e12, 3),
+                             "mfma_frac_fp64": round(F_alg / (msp * 1e-3) / 1e12 / FP64_PEAK_TFS, 4),
+                             "GB/s_alg": round(Bp / (msp * 1e-3) / 1e9, 2)},
+        "fp32_storage": {"ms": round(ms32, 4), "GB/s": round(B32 / (ms32 * 1e-3) / 1e9, 2),
+                         "rel_err_vs_fp64": err32},
+        "assembly": {"tree_s": round(t_tree, 3), "gpu_assembly_s": round(t_asm, 3),
+                     "aca_retries": ast["n_retries"], "mean_rank": ast["sum_rank"] / max(1, ast["n_lowrank"])},
+        "gmres": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -60,6 +60,9 @@ typedef struct {
 typedef struct {
   uint64_t n_leaves, n_dense, n_lowrank;
   uint64_t nvals;            /* sum of stored values (mblock::nvals)          */
+  uint64_t nvals_dense;      /* values in dense leaves (sum n1*n2)            */
+  uint64_t nvals_u;          /* values in U factors (sum k*n1)                */
+  uint64_t nvals_v;          /* values in V factors (sum k*n2)                */
   uint64_t t_len;            /* sum of low-rank ranks = len(V^T x)            */
   uint64_t bytes_per_value;  /* storage precision                             */
   uint64_t n_tasks_a, n_tasks_b;
@@ -158,6 +161,12 @@ int ahm_assemble_tree(ahm_ctx ctx, ahm_tree tree, const ahm_kernel* kernel, doub
  * is asked for, which is refused).  p = 1 is the single-vector path. */
 int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y,
                     uint32_t ldy, uint32_t p, ahm_dtype vec);
+/* Runs the product `reps` times with HIP events around each of the two
+ * launches (V^T x, then the row panels) and returns their mean durations in
+ * milliseconds (instrumentation; same arguments as ahm_hmat_matvec). */
+int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y,
+                            uint32_t ldy, uint32_t p, ahm_dtype vec, uint32_t reps,
+                            double* ms_phase_a, double* ms_phase_b);
 /* Same with host vectors (the drop-in for mltaGeHVec's host pointers):
  * copies X in and Y in/out over PCIe around the device product. */
 int ahm_hmat_matvec_host(ahm_hmat h, double alpha, const double* X, uint32_t ldx,

@@ -307,6 +307,8 @@ class Reference:
         L.ref_hmat_assemble.restype = _p
         L.ref_hmat_assemble.argtypes = [_p, C.c_int, _d, _d, _d, _u]
         L.ref_hmat_free.argtypes = [_p]
+        L.ref_hmat_from_data.restype = _p
+        L.ref_hmat_from_data.argtypes = [_p, _p, _p, _p]
         L.ref_hmat_layout.restype = C.c_size_t
         L.ref_hmat_layout.argtypes = [_p, _p, _p]
         L.ref_hmat_data.argtypes = [_p, _p]
@@ -339,6 +341,14 @@ class Reference:
     def assemble(self, tree, pts=None, kernel=KERNEL_LAPLACE, kappa=0.0, eps=1e-6, rankmax=1 << 30, nthreads=0,
                  weight=1.0):
         h = self.lib.ref_hmat_assemble(tree.h, kernel, _d(kappa), _d(weight), _d(eps), _u(rankmax))
+        return _HMat(self, h, tree.N, tree)
+
+    def from_data(self, tree, rank, offset, data):
+        """The reference's mblock array filled with given leaf values (idx order)."""
+        rank = np.ascontiguousarray(rank, np.int32)
+        offset = np.ascontiguousarray(offset, np.uint64)
+        data = np.ascontiguousarray(data, np.float64)
+        h = self.lib.ref_hmat_from_data(tree.h, _ptr(rank), _ptr(offset), _ptr(data))
         return _HMat(self, h, tree.N, tree)
 
     def _layout(self, H):

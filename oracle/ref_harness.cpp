@@ -233,6 +233,32 @@ void* ref_hmat_assemble(void* tp, int kernel, double kappa, double weight, doubl
   return h;
 }
 
+// an mblock array holding given leaf values (mblock layout, leaves in idx
+// order of this tree): lets the reference's own mltaGeHVec run on a matrix
+// assembled elsewhere
+void* ref_hmat_from_data(void* tp, const int* rank, const size_t* offset, const double* data) {
+  init();
+  Tree* t = (Tree*)tp;
+  HMat* h = new HMat;
+  h->t = t;
+  allocmbls(t->bl, h->A);
+  for (unsigned i = 0; i < t->nblcks; ++i) {
+    blcluster* b = t->leaves[i];
+    const unsigned n1 = b->getn1(), n2 = b->getn2();
+    mblock<double>* m = new mblock<double>(n1, n2);
+    if (rank[i] < 0) {
+      m->setGeM();
+      std::memcpy(m->getdata(), data + offset[i], sizeof(double) * (size_t)n1 * n2);
+    } else {
+      m->setrank((unsigned)rank[i]);
+      if (rank[i] > 0)
+        std::memcpy(m->getdata(), data + offset[i], sizeof(double) * (size_t)rank[i] * (n1 + n2));
+    }
+    h->A[i] = m;
+  }
+  return h;
+}
+
 void ref_hmat_free(void* p) {
   HMat* h = (HMat*)p;
   freembls(h->t->bl, h->A);
