@@ -38,8 +38,20 @@ struct GemvTask {
   uint64_t data;   // element offset of P in the arena
   uint64_t zoff;   // offset of the task's column-source list
   uint32_t m, W;
-  uint32_t out;    // first output row (t index for phase A, y row for phase B)
-  uint32_t pad;
+  uint32_t out;    // first output row: t index (phase A), y row (phase B, whole
+                   // panel) or partial-sum slot (phase B, column chunk)
+  uint32_t partial;  // 1: write the chunk's row sums to the partial buffer
+};
+
+// Row panels wider than one chunk are cut into column chunks of about
+// kChunkValues values; their row sums meet in a fixed-order reduction.
+constexpr uint64_t kChunkValues = 48 * 1024;
+
+struct ReduceTask {
+  uint32_t row;      // first y row
+  uint32_t m;        // rows
+  uint32_t part;     // first partial slot (chunk c, row r at part + c*m + r)
+  uint32_t nchunks;
 };
 
 // zsrc entry: bit 31 set -> index into t, else index into x
@@ -87,8 +99,12 @@ struct ahm_hmat_s {
   uint64_t n_dense = 0, n_lowrank = 0;
   uint64_t nv_dense = 0, nv_u = 0, nv_v = 0;
   ahm::DeviceBuf arena;       // nvals values in panel layout
-  ahm::DeviceBuf tasks_a, tasks_b, zsrc;
-  uint32_t n_a = 0, n_b = 0;
+  ahm::DeviceBuf tasks_a, tasks_b, zsrc, tasks_r;
+  uint32_t n_a = 0, n_b = 0, n_r = 0;
+  uint32_t part_len = 0;      // partial-sum slots per right-hand side
+  ahm::DeviceBuf pbuf;        // partial sums: part_len * p values
+  size_t pbuf_cap = 0;
+  std::vector<ahm::ReduceTask> h_tasks_r;
   uint64_t n_zsrc = 0;
   ahm::DeviceBuf tbuf;        // V^T X workspace: t_len * p values
   size_t tbuf_cap = 0;        // bytes

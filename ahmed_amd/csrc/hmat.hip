@@ -57,12 +57,15 @@ constexpr int kZChunk = 2048;
 // Thread t < G*m (G = 256/m column groups) owns row t%m and the columns
 // c = t/m, t/m + G, ...; consecutive groups read consecutive columns, so a
 // workgroup's loads cover G*m contiguous values.  Rows are reduced across
-// groups through LDS in a fixed order.
+// groups through LDS in a fixed order.  Phase B tasks either accumulate
+// alpha * sum into y or, for a column chunk of a wide panel, store the row
+// sums in the partial buffer.
 template <typename TS, typename TV, bool PHASE_B>
 __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
     const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
     const TS* __restrict__ arena, const TV* __restrict__ x, uint32_t ldx, TV* __restrict__ t,
-    uint32_t t_len, TV* __restrict__ y, uint32_t ldy, TV alpha) {
+    uint32_t t_len, TV* __restrict__ y, uint32_t ldy, TV* __restrict__ part, uint32_t part_len,
+    TV alpha) {
   __shared__ TV zs[kZChunk];
   __shared__ TV red[kPanelThreads];
   const GemvTask tk = tasks[blockIdx.x];
@@ -86,18 +89,28 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
     __syncthreads();
     if (active) {
       const TS* Pc = P + (uint64_t)c0 * m;
+      const uint32_t step = (uint32_t)(G * m);
+      uint32_t off = (uint32_t)g * m;
       int c = g;
-      for (; c + 3 * G < cw; c += 4 * G) {
-        const TV p0 = (TV)Pc[(uint64_t)c * m];
-        const TV p1 = (TV)Pc[(uint64_t)(c + G) * m];
-        const TV p2 = (TV)Pc[(uint64_t)(c + 2 * G) * m];
-        const TV p3 = (TV)Pc[(uint64_t)(c + 3 * G) * m];
+      for (; c + 7 * G < cw; c += 8 * G, off += 8 * step) {
+        const TV p0 = (TV)Pc[off];
+        const TV p1 = (TV)Pc[off + step];
+        const TV p2 = (TV)Pc[off + 2 * step];
+        const TV p3 = (TV)Pc[off + 3 * step];
+        const TV p4 = (TV)Pc[off + 4 * step];
+        const TV p5 = (TV)Pc[off + 5 * step];
+        const TV p6 = (TV)Pc[off + 6 * step];
+        const TV p7 = (TV)Pc[off + 7 * step];
         a0 = fma(p0, zs[c], a0);
         a1 = fma(p1, zs[c + G], a1);
         a2 = fma(p2, zs[c + 2 * G], a2);
         a3 = fma(p3, zs[c + 3 * G], a3);
+        a0 = fma(p4, zs[c + 4 * G], a0);
+        a1 = fma(p5, zs[c + 5 * G], a1);
+        a2 = fma(p6, zs[c + 6 * G], a2);
+        a3 = fma(p7, zs[c + 7 * G], a3);
       }
-      for (; c < cw; c += G) a0 = fma((TV)Pc[(uint64_t)c * m], zs[c], a0);
+      for (; c < cw; c += G, off += step) a0 = fma((TV)Pc[off], zs[c], a0);
     }
   }
   red[tid] = active ? (a0 + a1) + (a2 + a3) : (TV)0;
@@ -106,11 +119,28 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
     TV s = 0;
     for (int q = 0; q < G; ++q) s += red[q * m + tid];
     if (PHASE_B) {
-      TV* yr = y + (uint64_t)rhs * ldy;
-      yr[tk.out + tid] += alpha * s;
+      if (tk.partial)
+        part[(uint64_t)rhs * part_len + tk.out + tid] = s;
+      else
+        y[(uint64_t)rhs * ldy + tk.out + tid] += alpha * s;
     } else {
       tr[tk.out + tid] = s;
     }
+  }
+}
+
+// y rows += alpha * sum over chunks (chunk order fixed) for chunked panels
+template <typename TV>
+__global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
+    const ReduceTask* __restrict__ tasks, const TV* __restrict__ part, uint32_t part_len,
+    TV* __restrict__ y, uint32_t ldy, TV alpha) {
+  const ReduceTask r = tasks[blockIdx.x];
+  const uint32_t rhs = blockIdx.y;
+  const TV* pr = part + (uint64_t)rhs * part_len + r.part;
+  for (uint32_t i = threadIdx.x; i < r.m; i += kPanelThreads) {
+    TV s = 0;
+    for (uint32_t c = 0; c < r.nchunks; ++c) s += pr[(uint64_t)c * r.m + i];
+    y[(uint64_t)rhs * ldy + r.row + i] += alpha * s;
   }
 }
 
@@ -119,6 +149,8 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
 
 struct Plan {
   std::vector<GemvTask> a, b;
+  std::vector<ReduceTask> r;
+  uint32_t part_len = 0;
   std::vector<uint32_t> zsrc;
   std::vector<CopyJob> jobs;
   uint64_t nvals = 0;
@@ -254,7 +286,30 @@ static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_l
         W += cols;
       }
       tk.W = W;
-      if (W) P.b.push_back(tk);
+      if (W) {
+        // cut wide panels into column chunks of about kChunkValues values
+        const uint64_t vals = (uint64_t)m * W;
+        const uint32_t nch = (uint32_t)std::min<uint64_t>(W, (vals + kChunkValues - 1) / kChunkValues);
+        if (nch <= 1) {
+          tk.partial = 0;
+          P.b.push_back(tk);
+        } else {
+          ReduceTask rt{r0, m, P.part_len, nch};
+          for (uint32_t c = 0; c < nch; ++c) {
+            const uint32_t c0 = (uint32_t)((uint64_t)W * c / nch);
+            const uint32_t c1 = (uint32_t)((uint64_t)W * (c + 1) / nch);
+            GemvTask ck = tk;
+            ck.data = tk.data + (uint64_t)c0 * m;
+            ck.zoff = tk.zoff + c0;
+            ck.W = c1 - c0;
+            ck.out = P.part_len + c * m;
+            ck.partial = 1;
+            P.b.push_back(ck);
+          }
+          P.part_len += nch * m;
+          P.r.push_back(rt);
+        }
+      }
       arena += (uint64_t)m * W;
     }
   }
@@ -296,20 +351,33 @@ static int ensure_tbuf(ahm_hmat h, uint32_t p, size_t esize) {
   return AHM_OK;
 }
 
+static int ensure_pbuf(ahm_hmat h, uint32_t p, size_t esize) {
+  const size_t need = (size_t)h->part_len * p * esize;
+  if (need <= h->pbuf_cap) return AHM_OK;
+  AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+  AHM_TRY(h->pbuf.alloc(need));
+  h->pbuf_cap = need;
+  return AHM_OK;
+}
+
 template <typename TS, typename TV>
 static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y, uint32_t ldy,
                          uint32_t p, hipEvent_t ev_mid = nullptr) {
   hipStream_t s = h->ctx->stream;
   TV* t = (TV*)h->tbuf.p;
+  TV* part = (TV*)h->pbuf.p;
   if (h->n_a)
     hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, false>), dim3(h->n_a, p), dim3(kPanelThreads), 0, s,
                        (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
-                       (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, alpha);
+                       (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
   if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
   if (h->n_b)
     hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, true>), dim3(h->n_b, p), dim3(kPanelThreads), 0, s,
                        (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
-                       (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, alpha);
+                       (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
+  if (h->n_r)
+    hipLaunchKernelGGL((chunk_reduce_kernel<TV>), dim3(h->n_r, p), dim3(kPanelThreads), 0, s,
+                       (const ReduceTask*)h->tasks_r.p, (const TV*)part, h->part_len, Y, ldy, alpha);
   AHM_HIP(hipGetLastError());
   return AHM_OK;
 }
@@ -353,8 +421,11 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
     if ((st = upload_vec(h->tasks_a, P.a, s))) break;
     if ((st = upload_vec(h->tasks_b, P.b, s))) break;
     if ((st = upload_vec(h->zsrc, P.zsrc, s))) break;
+    if ((st = upload_vec(h->tasks_r, P.r, s))) break;
     h->n_a = (uint32_t)P.a.size();
     h->n_b = (uint32_t)P.b.size();
+    h->n_r = (uint32_t)P.r.size();
+    h->part_len = P.part_len;
     h->n_zsrc = P.zsrc.size();
     for (size_t j0 = 0; j0 < P.jobs.size(); j0 += (1u << 30)) {
       const uint32_t nj = (uint32_t)std::min<size_t>(1u << 30, P.jobs.size() - j0);
@@ -375,6 +446,7 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
     }
     h->h_tasks_a = std::move(P.a);
     h->h_tasks_b = std::move(P.b);
+    h->h_tasks_r = std::move(P.r);
     h->h_zsrc = std::move(P.zsrc);
     if (keep_leaves) {
       h->leafdata.p = raw.p;
@@ -452,15 +524,19 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
   h->leaves = src->leaves;
   h->h_tasks_a = src->h_tasks_a;
   h->h_tasks_b = src->h_tasks_b;
+  h->h_tasks_r = src->h_tasks_r;
   h->h_zsrc = src->h_zsrc;
+  h->part_len = src->part_len;
   int st = AHM_OK;
   do {
     if ((st = h->arena.alloc(h->nvals * 4))) break;
     if ((st = upload_vec(h->tasks_a, h->h_tasks_a, s))) break;
     if ((st = upload_vec(h->tasks_b, h->h_tasks_b, s))) break;
     if ((st = upload_vec(h->zsrc, h->h_zsrc, s))) break;
+    if ((st = upload_vec(h->tasks_r, h->h_tasks_r, s))) break;
     h->n_a = src->n_a;
     h->n_b = src->n_b;
+    h->n_r = src->n_r;
     h->n_zsrc = src->n_zsrc;
     if (h->nvals)
       hipLaunchKernelGGL((cast_kernel<float, double>), dim3(4096), dim3(256), 0, s,
@@ -531,10 +607,12 @@ int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void*
   AHM_HIP(hipSetDevice(h->ctx->device));
   if (h->storage == AHM_F64 && vec == AHM_F64) {
     AHM_TRY(ensure_tbuf(h, p, 8));
+    AHM_TRY(ensure_pbuf(h, p, 8));
     return launch_matvec<double, double>(h, alpha, (const double*)X, ldx, (double*)Y, ldy, p);
   }
   if (h->storage == AHM_F32 && vec == AHM_F32) {
     AHM_TRY(ensure_tbuf(h, p, 4));
+    AHM_TRY(ensure_pbuf(h, p, 4));
     return launch_matvec<float, float>(h, (float)alpha, (const float*)X, ldx, (float*)Y, ldy, p);
   }
   return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec: storage %d with vectors %d", h->storage, vec);
@@ -552,6 +630,7 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
     return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_profile: bad p / leading dimensions");
   AHM_HIP(hipSetDevice(h->ctx->device));
   AHM_TRY(ensure_tbuf(h, p, dtype_size(vec)));
+  AHM_TRY(ensure_pbuf(h, p, dtype_size(vec)));
   hipEvent_t e0, e1, e2;
   AHM_HIP(hipEventCreate(&e0));
   AHM_HIP(hipEventCreate(&e1));
