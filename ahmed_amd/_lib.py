@@ -78,6 +78,10 @@ SIGNATURES = {
                           _i, _i, C.POINTER(AssemblyStats), C.POINTER(_p)]),
     "ahm_assemble_tree": (_i, [_p, _p, C.POINTER(Kernel), _d, _u32, _p, _i, _i, C.POINTER(AssemblyStats),
                                C.POINTER(_p)]),
+    "ahm_gmres": (_i, [_p, _p, _p, C.POINTER(_d), _u32, C.POINTER(_u32), _p, _u32, C.POINTER(_u32)]),
+    "ahm_cg": (_i, [_p, _p, _p, C.POINTER(_d), C.POINTER(_u32)]),
+    "ahm_gmres_ir": (_i, [_p, _p, _p, _p, C.POINTER(_d), _u32, _d, _u32, _u32, C.POINTER(_u32),
+                          C.POINTER(_u32), _p, _u32]),
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_matvec_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
     "ahm_hmat_matvec_profile": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i, _u32, C.POINTER(_d),
@@ -278,6 +282,42 @@ class HMatrix:
         check(load().ahm_hmat_matvec_profile(self.h, alpha, _p(x.data_ptr()), ldx, _p(y.data_ptr()), ldy,
                                              p, vec, reps, C.byref(a), C.byref(b)), "matvec_profile")
         return a.value, b.value
+
+    def gmres(self, b, x, eps: float = 1e-8, m: int = 30, nsteps: int = 10000, nhist: int = 0):
+        """Device GMRES(m) (solvers/GMRES.cpp semantics); b, x FP64 torch
+        tensors, x updated in place.  Returns dict(converged, eps, nsteps,
+        nmatvec, hist)."""
+        e, ns, mv = _d(eps), _u32(nsteps), _u32(0)
+        hist = np.zeros(max(nhist, 1))
+        self.ctx.bind_stream()
+        rc = load().ahm_gmres(self.h, _p(b.data_ptr()), _p(x.data_ptr()), C.byref(e), m, C.byref(ns),
+                              hist.ctypes.data_as(_p), nhist, C.byref(mv))
+        if rc not in (OK, 5):
+            check(rc, "ahm_gmres")
+        return dict(converged=rc == OK, eps=e.value, nsteps=ns.value, nmatvec=mv.value,
+                    hist=hist[:nhist].copy())
+
+    def cg(self, b, x, eps: float = 1e-8, nsteps: int = 10000):
+        e, ns = _d(eps), _u32(nsteps)
+        self.ctx.bind_stream()
+        rc = load().ahm_cg(self.h, _p(b.data_ptr()), _p(x.data_ptr()), C.byref(e), C.byref(ns))
+        if rc not in (OK, 5):
+            check(rc, "ahm_cg")
+        return dict(converged=rc == OK, eps=e.value, nsteps=ns.value)
+
+    def gmres_ir(self, lo: "HMatrix", b, x, eps: float = 1e-8, m: int = 50, inner_eps: float = 1e-5,
+                 inner_steps: int = 1000, max_outer: int = 20, nhist: int = 32):
+        """FP64 iterative refinement with inner GMRES on `lo` (e.g. FP32 copy)."""
+        e, outer, inner = _d(eps), _u32(0), _u32(0)
+        hist = np.zeros(max(nhist, 1))
+        self.ctx.bind_stream()
+        rc = load().ahm_gmres_ir(self.h, lo.h, _p(b.data_ptr()), _p(x.data_ptr()), C.byref(e), m, inner_eps,
+                                 inner_steps, max_outer, C.byref(outer), C.byref(inner),
+                                 hist.ctypes.data_as(_p), nhist)
+        if rc not in (OK, 5):
+            check(rc, "ahm_gmres_ir")
+        return dict(converged=rc == OK, eps=e.value, outer=outer.value, inner=inner.value,
+                    hist=hist[:min(nhist, outer.value + 1)].copy())
 
     def matvec_host(self, x: np.ndarray, y: np.ndarray, alpha: float = 1.0):
         x = np.asfortranarray(x, np.float64)

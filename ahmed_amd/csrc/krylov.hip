@@ -1,0 +1,496 @@
+// krylov.hip -- GMRES(m), CG and FP64 iterative refinement on the GPU.
+//
+// GMRes follows solvers/GMRES.cpp:80-181 step for step: restarted,
+// right-preconditioned (no preconditioner here), modified Gram-Schmidt,
+// Givens rotations, convergence on the implicit residual |s_{i+1}|/|b|, x
+// updated at convergence or restart, true residual recomputed per restart.
+// CG follows solvers/CG.cpp:31-113.
+//
+// Vectors, the orthogonalisation and the rotations stay FP64 on the device
+// for every matrix precision; with an FP32 matrix only the H-matvec runs in
+// FP32 (the Krylov vector is rounded to FP32 on the way in, the product
+// widened to FP64 on the way out).  The (m+1) x m Hessenberg column and the
+// Givens rotations live on the host (O(m) work per step; one device->host
+// copy per Arnoldi step carries the new column and the residual estimate).
+//
+// Reductions are deterministic: a fixed grid of kRedBlocks workgroups, each
+// with a fixed element set, writes one partial per workgroup; consumers sum
+// the partials in a fixed order.  The MGS step k fuses "h_k = <w, v_k>
+// (from partials), w -= h_k v_k, partials of <w, v_{k+1}>" in one launch.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ahm_internal.h"
+
+namespace ahm {
+
+constexpr int kRedBlocks = 512;
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// sum of a kRedThreads-wide workgroup, fixed order; valid in every thread
+__device__ double group_sum(double v) {
+  __shared__ double sh[kRedThreads / 64];
+  v = wsum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < kRedThreads / 64; ++q) s += sh[q];
+  return s;
+}
+
+// sum of the kRedBlocks partials, fixed order, in every thread
+__device__ double sum_partials(const double* __restrict__ part) {
+  __shared__ double sp;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) v += part[i];
+  v = group_sum(v);
+  if (threadIdx.x == 0) sp = v;
+  __syncthreads();
+  return sp;
+}
+
+__global__ __launch_bounds__(kRedThreads) void dot_kernel(const double* __restrict__ a,
+                                                          const double* __restrict__ b, uint32_t n,
+                                                          double* __restrict__ part) {
+  double s = 0.0;
+  for (uint32_t i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += kRedBlocks * kRedThreads)
+    s = fma(a[i], b[i], s);
+  s = group_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// h = sum(part_in); out[0] = h (block 0); w -= h v; part_out = <w, next> (or <w,w>)
+__global__ __launch_bounds__(kRedThreads) void mgs_kernel(double* __restrict__ w,
+                                                          const double* __restrict__ v,
+                                                          const double* __restrict__ next,
+                                                          uint32_t n,
+                                                          const double* __restrict__ part_in,
+                                                          double* __restrict__ hout,
+                                                          double* __restrict__ part_out) {
+  const double h = sum_partials(part_in);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
+  double s = 0.0;
+  for (uint32_t i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += kRedBlocks * kRedThreads) {
+    const double wi = fma(-h, v[i], w[i]);
+    w[i] = wi;
+    s = fma(wi, next ? next[i] : wi, s);
+  }
+  s = group_sum(s);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = s;
+}
+
+// h = sqrt(sum(part)); out = h; w *= 1/h  (Arnoldi normalisation)
+__global__ __launch_bounds__(kRedThreads) void normalize_kernel(double* __restrict__ w, uint32_t n,
+                                                                const double* __restrict__ part,
+                                                                double* __restrict__ hout) {
+  const double h = sqrt(sum_partials(part));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
+  const double inv = 1.0 / h;
+  for (uint32_t i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += kRedBlocks * kRedThreads)
+    w[i] *= inv;
+}
+
+// result = sum(part) into out (single workgroup)
+__global__ __launch_bounds__(kRedThreads) void finish_kernel(const double* __restrict__ part,
+                                                             double* __restrict__ out) {
+  const double h = sum_partials(part);
+  if (threadIdx.x == 0) *out = h;
+}
+
+__global__ void scale_copy_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                  uint32_t n, double a) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[i] = a * src[i];
+}
+
+__global__ void fill_kernel(double* __restrict__ d, uint32_t n, double v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    d[i] = v;
+}
+
+__global__ void to_f32_kernel(const double* __restrict__ s, float* __restrict__ d, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    d[i] = (float)s[i];
+}
+
+__global__ void add_f32_kernel(const float* __restrict__ s, double* __restrict__ d, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    d[i] += (double)s[i];
+}
+
+// x += V y, V n x k column-major (ld n), y in device memory (GMRES update)
+__global__ void gemv_update_kernel(const double* __restrict__ V, uint32_t n, uint32_t k,
+                                   const double* __restrict__ y, double* __restrict__ x) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (uint32_t l = 0; l < k; ++l) s = fma(V[(uint64_t)l * n + i], y[l], s);
+    x[i] += s;
+  }
+}
+
+// y = a*x + y
+__global__ void axpy_kernel(double a, const double* __restrict__ x, double* __restrict__ y,
+                            uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = fma(a, x[i], y[i]);
+}
+
+// p = rhat + beta*p (CG direction update, solvers/CG.cpp:80-83: scal then axpy)
+__global__ void cg_dir_kernel(double beta, const double* __restrict__ r, double* __restrict__ p,
+                              uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = r[i] + beta * p[i];
+}
+
+namespace {
+
+constexpr int kEwBlocks = 1024;
+
+// The operator y += alpha A x with FP64 vectors, for either matrix precision.
+struct Op {
+  ahm_hmat h;
+  uint32_t n;
+  DeviceBuf x32, y32;
+  int init() {
+    if (h->storage == AHM_F32) {
+      AHM_TRY(x32.alloc((size_t)n * 4));
+      AHM_TRY(y32.alloc((size_t)n * 4));
+    }
+    return AHM_OK;
+  }
+  int amux(double alpha, const double* x, double* y) {
+    hipStream_t s = h->ctx->stream;
+    if (h->storage == AHM_F64) return ahm_hmat_matvec(h, alpha, x, n, y, n, 1, AHM_F64);
+    hipLaunchKernelGGL(to_f32_kernel, dim3(kEwBlocks), dim3(256), 0, s, x, (float*)x32.p, n);
+    AHM_HIP(hipMemsetAsync(y32.p, 0, (size_t)n * 4, s));
+    AHM_TRY(ahm_hmat_matvec(h, alpha, x32.p, n, y32.p, n, 1, AHM_F32));
+    hipLaunchKernelGGL(add_f32_kernel, dim3(kEwBlocks), dim3(256), 0, s, (const float*)y32.p, y, n);
+    return AHM_OK;
+  }
+};
+
+struct Red {
+  DeviceBuf part, scal;   // kRedBlocks partials (x2), scalars
+  double* host = nullptr;
+  int init(uint32_t nscal) {
+    AHM_TRY(part.alloc(2 * kRedBlocks * sizeof(double)));
+    AHM_TRY(scal.alloc((size_t)nscal * sizeof(double)));
+    return AHM_OK;
+  }
+  double* p0() { return (double*)part.p; }
+  double* p1() { return (double*)part.p + kRedBlocks; }
+  double* s() { return (double*)scal.p; }
+};
+
+// |v|_2 on the host (one sync)
+int norm2(hipStream_t s, Red& R, const double* v, uint32_t n, double* out) {
+  hipLaunchKernelGGL(dot_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, v, v, n, R.p0());
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kRedThreads), 0, s, R.p0(), R.s());
+  AHM_HIP(hipMemcpyAsync(out, R.s(), sizeof(double), hipMemcpyDeviceToHost, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  *out = std::sqrt(*out);
+  return AHM_OK;
+}
+
+int dot(hipStream_t s, Red& R, const double* a, const double* b, uint32_t n, double* out) {
+  hipLaunchKernelGGL(dot_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, a, b, n, R.p0());
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kRedThreads), 0, s, R.p0(), R.s());
+  AHM_HIP(hipMemcpyAsync(out, R.s(), sizeof(double), hipMemcpyDeviceToHost, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  return AHM_OK;
+}
+
+// genPlRot / applPlRot, solvers/GMRES.cpp:33-55
+void gen_rot(double dx, double dy, double& cs, double& sn) {
+  if (dy == 0.0) {
+    cs = 1.0;
+    sn = 0.0;
+  } else if (std::fabs(dy) > std::fabs(dx)) {
+    const double t = dx / dy;
+    sn = 1.0 / std::sqrt(1.0 + t * t);
+    cs = t * sn;
+  } else {
+    const double t = dy / dx;
+    cs = 1.0 / std::sqrt(1.0 + t * t);
+    sn = t * cs;
+  }
+}
+void app_rot(double& dx, double& dy, double cs, double sn) {
+  const double t = cs * dx + sn * dy;
+  dy = cs * dy - sn * dx;
+  dx = t;
+}
+
+}  // namespace
+
+// Restarted MGS-GMRES(m) on the device.  b, x: FP64 device vectors (x in/out).
+// Returns AHM_OK on convergence, AHM_ERR_NOT_CONVERGED otherwise (eps, nsteps
+// then hold the achieved residual and the steps taken, as the reference's
+// return value 1 does).
+int gmres_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m,
+                 uint32_t* nsteps, double* hist, uint32_t nhist, uint32_t* nmatvec) {
+  const uint32_t n = h->n_rows;
+  hipStream_t s = h->ctx->stream;
+  Op A{h, n};
+  AHM_TRY(A.init());
+  Red R;
+  AHM_TRY(R.init(m + 2));
+  DeviceBuf dV, dr, dy;
+  AHM_TRY(dV.alloc((size_t)n * (m + 1) * 8));
+  AHM_TRY(dr.alloc((size_t)n * 8));
+  AHM_TRY(dy.alloc((size_t)(m + 1) * 8));
+  double* V = (double*)dV.p;
+  double* r = (double*)dr.p;
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m + 1), sn(m + 1), sv(m + 1), col(m + 2);
+  uint32_t nh = 0, mv = 0;
+  auto update = [&](uint32_t k) -> int {
+    // solve the k x k upper triangle H y = s (dtrtrs 'U','N','N'), x += V y
+    std::vector<double> y(sv.begin(), sv.begin() + k);
+    for (int i = (int)k - 1; i >= 0; --i) {
+      double t = y[i];
+      for (uint32_t l = i + 1; l < k; ++l) t -= H[i + (size_t)l * (m + 1)] * y[l];
+      y[i] = t / H[i + (size_t)i * (m + 1)];
+    }
+    AHM_HIP(hipMemcpyAsync(dy.p, y.data(), k * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(gemv_update_kernel, dim3(kEwBlocks), dim3(256), 0, s, V, n, k,
+                       (const double*)dy.p, x);
+    AHM_HIP(hipStreamSynchronize(s));  // y lives on the host stack
+    return AHM_OK;
+  };
+  double normb;
+  AHM_TRY(norm2(s, R, b, n, &normb));
+  if (normb == 0.0) {
+    hipLaunchKernelGGL(fill_kernel, dim3(kEwBlocks), dim3(256), 0, s, x, n, 0.0);
+    *eps = 0.0;
+    *nsteps = 0;
+    AHM_HIP(hipStreamSynchronize(s));
+    return AHM_OK;
+  }
+  // r = b - A x
+  AHM_HIP(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  AHM_TRY(A.amux(-1.0, x, r));
+  ++mv;
+  double beta, resid;
+  AHM_TRY(norm2(s, R, r, n, &beta));
+  if ((resid = beta / normb) <= *eps) {
+    *eps = resid;
+    *nsteps = 0;
+    if (nmatvec) *nmatvec = mv;
+    return AHM_OK;
+  }
+  uint32_t j = 1;
+  const uint32_t maxsteps = *nsteps;
+  while (j <= maxsteps) {
+    hipLaunchKernelGGL(scale_copy_kernel, dim3(kEwBlocks), dim3(256), 0, s, r, V, n, 1.0 / beta);
+    sv[0] = beta;
+    for (uint32_t q = 1; q <= m; ++q) sv[q] = 0.0;
+    uint32_t i;
+    for (i = 0; i < m && j <= maxsteps; i++, j++) {
+      double* vi = V + (size_t)i * n;
+      double* w = V + (size_t)(i + 1) * n;
+      hipLaunchKernelGGL(fill_kernel, dim3(kEwBlocks), dim3(256), 0, s, w, n, 0.0);
+      AHM_TRY(A.amux(1.0, vi, w));
+      ++mv;
+      // MGS: k = 0..i, fused (dot from partials, axpy, next dot)
+      hipLaunchKernelGGL(dot_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, w, V, n, R.p0());
+      double* pin = R.p0();
+      double* pout = R.p1();
+      for (uint32_t k = 0; k <= i; ++k) {
+        const double* next = (k < i) ? V + (size_t)(k + 1) * n : nullptr;
+        hipLaunchKernelGGL(mgs_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, w,
+                           V + (size_t)k * n, next, n, pin, R.s() + k, pout);
+        std::swap(pin, pout);
+      }
+      hipLaunchKernelGGL(normalize_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, w, n, pin,
+                         R.s() + i + 1);
+      AHM_HIP(hipMemcpyAsync(col.data(), R.s(), (i + 2) * 8, hipMemcpyDeviceToHost, s));
+      AHM_HIP(hipStreamSynchronize(s));
+      double* Hc = H.data() + (size_t)i * (m + 1);
+      for (uint32_t k = 0; k <= i + 1; ++k) Hc[k] = col[k];
+      for (uint32_t k = 0; k < i; ++k) app_rot(Hc[k], Hc[k + 1], cs[k], sn[k]);
+      gen_rot(Hc[i], Hc[i + 1], cs[i], sn[i]);
+      app_rot(Hc[i], Hc[i + 1], cs[i], sn[i]);
+      app_rot(sv[i], sv[i + 1], cs[i], sn[i]);
+      resid = std::fabs(sv[i + 1] / normb);
+      if (hist && nh < nhist) hist[nh++] = resid;
+      if (resid < *eps) {
+        AHM_TRY(update(i + 1));
+        *eps = resid;
+        *nsteps = j;
+        if (nmatvec) *nmatvec = mv;
+        return AHM_OK;
+      }
+    }
+    // the reference passes m here; i < m only when nsteps ran out mid-cycle
+    AHM_TRY(update(i));
+    AHM_HIP(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    AHM_TRY(A.amux(-1.0, x, r));
+    ++mv;
+    AHM_TRY(norm2(s, R, r, n, &beta));
+    if ((resid = beta / normb) < *eps) {
+      *eps = resid;
+      *nsteps = j;
+      if (nmatvec) *nmatvec = mv;
+      return AHM_OK;
+    }
+  }
+  *eps = resid;  // nsteps keeps its input value, as in the reference
+  if (nmatvec) *nmatvec = mv;
+  return AHM_ERR_NOT_CONVERGED;
+}
+
+// CG, solvers/CG.cpp:31-113 (no preconditioner).
+int cg_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nsteps) {
+  const uint32_t n = h->n_rows;
+  hipStream_t s = h->ctx->stream;
+  Op A{h, n};
+  AHM_TRY(A.init());
+  Red R;
+  AHM_TRY(R.init(4));
+  DeviceBuf buf;
+  AHM_TRY(buf.alloc((size_t)n * 3 * 8));
+  double* p = (double*)buf.p;
+  double* q = p + n;
+  double* r = q + n;
+  double nrmb;
+  AHM_TRY(norm2(s, R, b, n, &nrmb));
+  if (nrmb < 1e-16) {
+    hipLaunchKernelGGL(fill_kernel, dim3(kEwBlocks), dim3(256), 0, s, x, n, 0.0);
+    AHM_HIP(hipStreamSynchronize(s));
+    *eps = 0.0;
+    *nsteps = 0;
+    return AHM_OK;
+  }
+  AHM_HIP(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  AHM_TRY(A.amux(-1.0, x, r));
+  double resid;
+  AHM_TRY(norm2(s, R, r, n, &resid));
+  if (resid <= *eps * nrmb) {
+    *eps = resid / nrmb;
+    *nsteps = 0;
+    return AHM_OK;
+  }
+  double rho, rho1 = 0.0;
+  const uint32_t maxsteps = *nsteps;
+  for (uint32_t l = 1; l <= maxsteps; ++l) {
+    AHM_TRY(dot(s, R, r, r, n, &rho));  // rhat = r (no preconditioner)
+    if (l > 1)
+      hipLaunchKernelGGL(cg_dir_kernel, dim3(kEwBlocks), dim3(256), 0, s, rho / rho1, r, p, n);
+    else
+      AHM_HIP(hipMemcpyAsync(p, r, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(fill_kernel, dim3(kEwBlocks), dim3(256), 0, s, q, n, 0.0);
+    AHM_TRY(A.amux(1.0, p, q));
+    double pq;
+    AHM_TRY(dot(s, R, p, q, n, &pq));
+    const double alpha = rho / pq;
+    hipLaunchKernelGGL(axpy_kernel, dim3(kEwBlocks), dim3(256), 0, s, alpha, p, x, n);
+    hipLaunchKernelGGL(axpy_kernel, dim3(kEwBlocks), dim3(256), 0, s, -alpha, q, r, n);
+    AHM_TRY(norm2(s, R, r, n, &resid));
+    if (resid <= *eps * nrmb) {
+      *eps = resid / nrmb;
+      *nsteps = l;
+      return AHM_OK;
+    }
+    rho1 = rho;
+  }
+  *eps = resid / nrmb;
+  return AHM_ERR_NOT_CONVERGED;
+}
+
+}  // namespace ahm
+
+using namespace ahm;
+
+extern "C" {
+
+int ahm_gmres(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m, uint32_t* nsteps,
+              double* hist, uint32_t nhist, uint32_t* nmatvec) {
+  if (!h || !b || !x || !eps || !nsteps || m == 0)
+    return fail(AHM_ERR_INVALID, "ahm_gmres: bad arguments");
+  if (h->n_rows != h->n_cols) return fail(AHM_ERR_INVALID, "ahm_gmres: matrix is not square");
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  const int st = gmres_device(h, b, x, eps, m, nsteps, hist, nhist, nmatvec);
+  if (st == AHM_ERR_NOT_CONVERGED)
+    set_error("ahm_gmres: no convergence in %u steps (residual %.3e)", *nsteps, *eps);
+  return st;
+}
+
+int ahm_cg(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nsteps) {
+  if (!h || !b || !x || !eps || !nsteps) return fail(AHM_ERR_INVALID, "ahm_cg: bad arguments");
+  if (h->n_rows != h->n_cols) return fail(AHM_ERR_INVALID, "ahm_cg: matrix is not square");
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  const int st = cg_device(h, b, x, eps, nsteps);
+  if (st == AHM_ERR_NOT_CONVERGED)
+    set_error("ahm_cg: no convergence in %u steps (residual %.3e)", *nsteps, *eps);
+  return st;
+}
+
+// FP64 iterative refinement around GMRES on a lower-precision matrix:
+//   r = b - A64 x;  solve A_lo d = r by GMRES(m) to inner_eps;  x += d
+// until |r| / |b| < eps.  The residual uses the FP64 H-matvec.
+int ahm_gmres_ir(ahm_hmat hi, ahm_hmat lo, const double* b, double* x, double* eps, uint32_t m,
+                 double inner_eps, uint32_t inner_steps, uint32_t max_outer, uint32_t* outer,
+                 uint32_t* total_inner, double* hist, uint32_t nhist) {
+  if (!hi || !lo || !b || !x || !eps || m == 0 || hi->storage != AHM_F64 ||
+      hi->n_rows != lo->n_rows || hi->n_rows != hi->n_cols)
+    return fail(AHM_ERR_INVALID, "ahm_gmres_ir: bad arguments");
+  AHM_HIP(hipSetDevice(hi->ctx->device));
+  const uint32_t n = hi->n_rows;
+  hipStream_t s = hi->ctx->stream;
+  Red R;
+  AHM_TRY(R.init(2));
+  DeviceBuf dr, dd;
+  AHM_TRY(dr.alloc((size_t)n * 8));
+  AHM_TRY(dd.alloc((size_t)n * 8));
+  double* r = (double*)dr.p;
+  double* d = (double*)dd.p;
+  double normb, resid = 1.0;
+  AHM_TRY(norm2(s, R, b, n, &normb));
+  uint32_t it = 0, inner_total = 0, nh = 0;
+  if (normb == 0.0) {
+    hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, s, x, n, 0.0);
+    AHM_HIP(hipStreamSynchronize(s));
+    *eps = 0.0;
+    if (outer) *outer = 0;
+    if (total_inner) *total_inner = 0;
+    return AHM_OK;
+  }
+  int st = AHM_ERR_NOT_CONVERGED;
+  for (;;) {
+    AHM_HIP(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    AHM_TRY(ahm_hmat_matvec(hi, -1.0, x, n, r, n, 1, AHM_F64));
+    double nr;
+    AHM_TRY(norm2(s, R, r, n, &nr));
+    resid = nr / normb;
+    if (hist && nh < nhist) hist[nh++] = resid;
+    if (resid < *eps) {
+      st = AHM_OK;
+      break;
+    }
+    if (it >= max_outer) break;
+    // inner solve A_lo d = r, d0 = 0, relative tolerance inner_eps
+    hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, s, d, n, 0.0);
+    double ie = std::max(inner_eps, 0.5 * (*eps) / resid);
+    uint32_t ns = inner_steps, mvs = 0;
+    const int gs = gmres_device(lo, r, d, &ie, m, &ns, nullptr, 0, &mvs);
+    if (gs != AHM_OK && gs != AHM_ERR_NOT_CONVERGED) return gs;
+    inner_total += ns;
+    hipLaunchKernelGGL(axpy_kernel, dim3(1024), dim3(256), 0, s, 1.0, (const double*)d, x, n);
+    ++it;
+  }
+  AHM_HIP(hipStreamSynchronize(s));
+  *eps = resid;
+  if (outer) *outer = it;
+  if (total_inner) *total_inner = inner_total;
+  if (st) set_error("ahm_gmres_ir: no convergence after %u refinements (residual %.3e)", it, resid);
+  return st;
+}
+
+}  // extern "C"

@@ -172,6 +172,29 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
 int ahm_hmat_matvec_host(ahm_hmat h, double alpha, const double* X, uint32_t ldx,
                          double* Y, uint32_t ldy, uint32_t p);
 
+/* -------- Krylov solvers on the device --------------------------------------
+ * b, x: FP64 device vectors of length n_rows (x is the initial guess and the
+ * result).  eps: in = tolerance, out = achieved relative residual; nsteps:
+ * in = step limit, out = steps taken on convergence (unchanged otherwise).
+ * Returns AHM_OK on convergence and AHM_ERR_NOT_CONVERGED otherwise (the
+ * reference's 0 / 1).  With an FP32 matrix only the H-matvec runs in FP32;
+ * Krylov vectors, orthogonalisation and rotations stay FP64.
+ * hist (host, may be NULL) receives up to nhist residual estimates;
+ * nmatvec (may be NULL) the number of H-matvecs. */
+/* GMRes, solvers/GMRES.cpp:80-181 (MGS + Givens, restart length m) */
+int ahm_gmres(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m,
+              uint32_t* nsteps, double* hist, uint32_t nhist, uint32_t* nmatvec);
+/* CG, solvers/CG.cpp:31-113 */
+int ahm_cg(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nsteps);
+/* FP64 iterative refinement: r = b - A64 x (FP64 H-matvec), inner GMRES(m)
+ * on `lo` (typically the FP32 copy of the same matrix) to inner_eps relative,
+ * x += d, until |r|/|b| < eps or max_outer refinements.  No counterpart in
+ * the reference; its FEMMatrix (Include/matrix.h:489-507) is the closest
+ * mixed-precision precedent. */
+int ahm_gmres_ir(ahm_hmat hi, ahm_hmat lo, const double* b, double* x, double* eps, uint32_t m,
+                 double inner_eps, uint32_t inner_steps, uint32_t max_outer, uint32_t* outer,
+                 uint32_t* total_inner, double* hist, uint32_t nhist);
+
 #ifdef __cplusplus
 }
 #endif
