@@ -1,22 +1,27 @@
 #!/usr/bin/env python3
 """H-matrix hot-path benchmark on one MI355X.
 
-Workload (the one-GPU FP64 configuration): N = 262144 points uniform on the
-unit sphere (seed 0), bounding-box cluster tree with leaf size 64, eta = 0.8,
-ACA eps = 1e-8, Laplace kernel 1/(4 pi r) with diagonal 1.0 and quadrature
-weight 4 pi / N.  The tree is built by the C++ surface, the H-matrix is
-assembled on the GPU (batched ACA + recompression), then the H-matvec
-y += H x is timed.  A "step" is one H-matvec with 1 right-hand side;
-``value`` is its algorithmic HBM throughput:
+Default workload (the N = 1M mixed-precision configuration, which fits one
+GPU): N = 1048576 points uniform on the surface of the unit cube (face chosen
+uniformly, uniform on the face; seed 0), bounding-box cluster tree with leaf
+size 64, eta = 0.8, ACA eps = 1e-5, Laplace kernel 1/(4 pi r) with diagonal
+1.0 and quadrature weight 6/N.  ``--config cfg2`` selects the one-GPU FP64
+configuration instead (N = 262144 on the unit sphere, ACA eps 1e-8, GMRES to
+1e-10, weight 4 pi/N).
+
+The tree is built by the C++ surface, the H-matrix is assembled on the GPU
+(batched ACA + recompression).  A "step" is one FP64 H-matvec y += H x with
+one right-hand side; ``value`` is its algorithmic HBM throughput
 
     B_alg = 8 * sum_leaves nvals + 8 * (N_cols + 2 N_rows)   bytes per step
 
 (leaf values read once, x read once, y read and written once).  The same
-line also carries the 32-RHS product, FP32 storage, the roofline of the
-dominant kernel (row-panel phase) and the reference's own CPU mltaGeHVec
-timed on this host on the identical H-matrix.
+line carries the 32-RHS product, FP32 storage, GMRES(50) time-to-tolerance
+in plain FP64 and with FP64 iterative refinement around FP32 GMRES, the
+roofline of the dominant kernel (row-panel phase) and the reference's own
+CPU mltaGeHVec timed on this host on the identical H-matrix.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg3|cfg2] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -112,7 +117,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--config", choices=["cfg3", "cfg2"], default="cfg3")
+    ap.add_argument("--n", type=int, default=0, help="override N")
     ap.add_argument("--rhs", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -125,10 +131,16 @@ def main():
     from ahmed_amd import _lib, geometry
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    N, bmin, eta, eps = args.n, 64, 0.8, 1e-8
-    weight = 4 * np.pi / N
+    if args.config == "cfg2":
+        N, eps, tol, geo = args.n or 262144, 1e-8, 1e-10, "unit sphere"
+        pts = geometry.sphere(N, 0)
+        weight = 4 * np.pi / N
+    else:
+        N, eps, tol, geo = args.n or 1048576, 1e-5, 1e-8, "unit-cube surface"
+        pts = geometry.cube_surface(N, 0)
+        weight = 6.0 / N
+    bmin, eta = 64, 0.8
     ctx = _lib.Context(0)
-    pts = geometry.sphere(N, 0)
     t0 = time.perf_counter()
     tree = _lib.Tree(pts, bmin, eta * eta)
     t_tree = time.perf_counter() - t0
@@ -178,6 +190,33 @@ def main():
     err32 = float((y32.double() - yref).abs().max() / yref.abs().max())
     log(f"FP32 H-matvec 1 RHS: {ms32:.3f} ms, {B32 / (ms32 * 1e-3) / 1e9:.1f} GB/s, rel.err vs FP64 {err32:.2e}")
 
+    # ---- GMRES(50) time-to-tolerance: FP64 vs FP64 refinement around FP32 ----
+    b = torch.zeros(N, dtype=torch.float64, device="cuda")
+    H.matvec(torch.ones(N, dtype=torch.float64, device="cuda"), b)   # b = H * 1
+    gm = {}
+    for mode in ("fp64", "mixed"):
+        xs = torch.zeros(N, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if mode == "fp64":
+            r = H.gmres(b, xs, eps=tol, m=50, nsteps=2000)
+            info = {"iterations": r["nsteps"], "matvecs": r["nmatvec"]}
+        else:
+            r = H.gmres_ir(H32, b, xs, eps=tol, m=50, inner_eps=1e-5, inner_steps=2000)
+            info = {"refinements": r["outer"], "inner_iterations": r["inner"]}
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res = torch.zeros(N, dtype=torch.float64, device="cuda")
+        res.copy_(b)
+        H.matvec(xs, res, -1.0)
+        true = float(res.norm() / b.norm())
+        err = float((xs - 1.0).abs().max())
+        gm[mode] = dict(seconds=round(dt, 4), converged=r["converged"], reported_residual=r["eps"],
+                        true_residual_fp64=true, max_err_vs_ones=err, **info)
+        log(f"GMRES(50) {mode}: {dt:.3f} s, residual {true:.2e}, {info}")
+    gm["tolerance"] = tol
+    gm["speedup_mixed_vs_fp64"] = round(gm["fp64"]["seconds"] / gm["mixed"]["seconds"], 3)
+
     cpu = None
     if not args.no_cpu:
         leaves, data = H.download()
@@ -198,9 +237,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded points on the unit sphere; H-matrix assembled on the GPU)",
-        "config": {"workload": f"FP64 H-matvec, 1 RHS, N={N} unit sphere, leaf size {bmin}, eta {eta}, "
-                               f"ACA eps {eps}, Laplace kernel (weight 4pi/N)",
+        "data": f"synthetic (seeded points on the {geo}; H-matrix assembled on the GPU)",
+        "config": {"workload": f"FP64 H-matvec, 1 RHS, N={N} {geo}, leaf size {bmin}, eta {eta}, "
+                               f"ACA eps {eps}, Laplace kernel (quadrature weight {weight:.4g})",
+                   "config": args.config,
                    "N": N, "leaf_size": bmin, "eta": eta, "aca_eps": eps, "rhs": 1,
                    "leaves": st["n_leaves"], "lowrank_leaves": st["n_lowrank"], "dense_leaves": st["n_dense"],
                    "nvals": st["nvals"], "bytes_per_step": B_alg},
@@ -218,7 +258,7 @@ def main():
                          "rel_err_vs_fp64": err32},
         "assembly": {"tree_s": round(t_tree, 3), "gpu_assembly_s": round(t_asm, 3),
                      "aca_retries": ast["n_retries"], "mean_rank": ast["sum_rank"] / max(1, ast["n_lowrank"])},
-        "gmres": None,
+        "gmres": gm,
     }
     print(json.dumps(line), flush=True)
 
