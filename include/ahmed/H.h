@@ -1,0 +1,51 @@
+// H.h -- H-matrix functions of the drop-in surface (the subset of the
+// reference's Include/H.h on the matvec / assembly / Krylov path).
+//
+// The mblock<T>* array indexed by leaf idx is the reference's H-matrix
+// (Include/H.h:21-45).  The products below run on the GPU: the first call on a
+// (tree, array) pair uploads the leaves into a device arena that is reused
+// until an mblock changes (see mblock.h).
+#ifndef AHMED_H_H
+#define AHMED_H_H
+
+#include "blcluster.h"
+#include "mblock.h"
+
+template <class T>
+void allocmbls(unsigned n, mblock<T>**& A) {
+  A = new mblock<T>*[n];
+  for (unsigned i = 0; i < n; ++i) A[i] = nullptr;
+}
+template <class T>
+void allocmbls(blcluster* bl, mblock<T>**& A) {
+  allocmbls((unsigned)bl->nleaves(), A);
+}
+template <class T>
+void freembls(unsigned n, mblock<T>**& A) {
+  if (A) {
+    for (unsigned i = 0; i < n; ++i) delete A[i];
+    delete[] A;
+    A = nullptr;
+  }
+}
+template <class T>
+void freembls(blcluster* bl, mblock<T>**& A) {
+  freembls((unsigned)bl->nleaves(), A);
+}
+
+// y += d A x (H/mltaGeHVec.cpp:567-571).  x, y are host vectors offset so
+// that x[0] is column bl->getb2() and y[0] row bl->getb1().  Returns true
+// if some leaf contributed.
+bool mltaGeHVec(double d, blcluster* bl, mblock<double>** A, double* x, double* y);
+bool mltaGeHVec(float d, blcluster* bl, mblock<float>** A, float* x, float* y);
+// Y += d A X for p columns (H/mltaGeHVec.cpp:573-577)
+bool mltaGeHGeM(double d, blcluster* bl, mblock<double>** A, unsigned p, double* X,
+                unsigned ldX, double* Y, unsigned ldY);
+
+// storage in bytes and largest rank (H/initH.cpp:16-78)
+unsigned long sizeH(blcluster* bl, mblock<double>** A);
+unsigned Hmax_rank(blcluster* bl, mblock<double>** A);
+// FP64 -> FP32 copy of every leaf (H/initH.cpp:152-168)
+void copyH(blcluster* bl, mblock<double>** A, mblock<float>** B);
+
+#endif

@@ -18,6 +18,11 @@ class bemblcluster : public blcluster {
   blcluster* clone(cluster* p1, cluster* p2) const override {
     return new bemblcluster(static_cast<bemcluster<T1>*>(p1), static_cast<bemcluster<T2>*>(p2));
   }
+  bool aca_start_row(unsigned& icom) const override {
+    if (!cl1) return false;
+    icom = cl1->geticom();
+    return true;
+  }
   bemcluster<T1>* getcl1() const { return cl1; }
   bemcluster<T2>* getcl2() const { return cl2; }
 };

@@ -110,6 +110,13 @@ class blcluster {
 
   virtual blcluster* clone(cluster* c1, cluster* c2) const { return new blcluster(c1, c2); }
 
+  // ACA start row of this leaf's row cluster when the tree carries one
+  // (bemblcluster: bemcluster::icom, Include/apprx.h:155); false otherwise.
+  virtual bool aca_start_row(unsigned& icom) const {
+    (void)icom;
+    return false;
+  }
+
   virtual void subdivide(cluster* cl1, cluster* cl2, double eta2, unsigned& nblcks,
                          unsigned maxdepth = 0) {
     nblcks = 0;
