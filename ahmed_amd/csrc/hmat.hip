@@ -129,6 +129,114 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
   }
 }
 
+// ------------------------------------------------------------------------
+// p right-hand sides: the same panels as a matrix product on the MFMA units.
+//
+// out(m x p) (+)= P (m x W) * Z (W x p), one task per workgroup and 32
+// right-hand sides per blockIdx.y.  The output is tiled 16 x 16; wavefront w
+// owns the row tiles w, w+4, w+8, w+12 and both 16-column tiles of its RHS
+// group, so every panel value is loaded once per 32 right-hand sides.  Z is
+// gathered into LDS in chunks of kMmChunk columns.  Per k-step of 4:
+//   A (16 x 4 of P):  lane l holds P[row0 + l%16][k0 + l/16]
+//   B (4 x 16 of Z):  lane l holds Z[k0 + l/16][col0 + l%16]
+//   C (16 x 16):      f64: lane l, value r -> (l/16 + 4r, l%16)
+//                     f32: lane l, value r -> (4(l/16) + r, l%16)
+// (lane layouts of v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 on gfx950).
+constexpr int kMmChunk = 64;
+constexpr int kMmRhs = 32;
+
+typedef double mm_f64x4 __attribute__((ext_vector_type(4)));
+typedef float mm_f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MmAcc;
+template <> struct MmAcc<double> {
+  typedef mm_f64x4 type;
+  static __device__ __forceinline__ type mma(double a, double b, type c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return lane / 16 + 4 * r; }
+};
+template <> struct MmAcc<float> {
+  typedef mm_f32x4 type;
+  static __device__ __forceinline__ type mma(float a, float b, type c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane / 16) + r; }
+};
+
+template <typename T, bool PHASE_B>
+__global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
+    const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
+    const T* __restrict__ arena, const T* __restrict__ x, uint32_t ldx, T* __restrict__ t,
+    uint32_t t_len, T* __restrict__ y, uint32_t ldy, T* __restrict__ part, uint32_t part_len,
+    T alpha, uint32_t p) {
+  typedef typename MmAcc<T>::type acc_t;
+  __shared__ T zs[kMmChunk][kMmRhs + 1];
+  const GemvTask tk = tasks[blockIdx.x];
+  const uint32_t rhs0 = blockIdx.y * kMmRhs;
+  const int m = (int)tk.m;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nrt = (m + 15) / 16;
+  acc_t acc[4][2];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+  const T* P = arena + tk.data;
+  const int li = lane & 15, lk = lane >> 4;
+  for (uint32_t c0 = 0; c0 < tk.W; c0 += kMmChunk) {
+    const int cw = (int)min((uint32_t)kMmChunk, tk.W - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < kMmChunk * kMmRhs; e += kPanelThreads) {
+      const int c = e / kMmRhs, j = e % kMmRhs;
+      T v = 0;
+      if (c < cw && rhs0 + j < p) {
+        const uint32_t s = zsrc[tk.zoff + c0 + c];
+        v = (s & kFromT) ? t[(uint64_t)(rhs0 + j) * t_len + (s & ~kFromT)]
+                         : x[(uint64_t)(rhs0 + j) * ldx + s];
+      }
+      zs[c][j] = v;
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < cw; k0 += 4) {
+      const int k = k0 + lk;
+      const T b0 = zs[k][li];        // zero-filled beyond cw
+      const T b1 = zs[k][16 + li];
+      const T* Pk = P + (uint64_t)(c0 + k) * m;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rt = w + 4 * q;
+        if (rt < nrt) {
+          const int row = rt * 16 + li;
+          const T a = (row < m && k < cw) ? Pk[row] : (T)0;
+          acc[q][0] = MmAcc<T>::mma(a, b0, acc[q][0]);
+          acc[q][1] = MmAcc<T>::mma(a, b1, acc[q][1]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int rt = w + 4 * q;
+    if (rt >= nrt) continue;
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rt * 16 + MmAcc<T>::row(lane, r);
+        const uint32_t j = rhs0 + ct * 16 + li;
+        if (row >= m || j >= p) continue;
+        const T v = acc[q][ct][r];
+        if (PHASE_B) {
+          if (tk.partial)
+            part[(uint64_t)j * part_len + tk.out + row] = v;
+          else
+            y[(uint64_t)j * ldy + tk.out + row] += alpha * v;
+        } else {
+          t[(uint64_t)j * t_len + tk.out + row] = v;
+        }
+      }
+  }
+}
+
 // y rows += alpha * sum over chunks (chunk order fixed) for chunked panels
 template <typename TV>
 __global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
@@ -366,15 +474,29 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
   hipStream_t s = h->ctx->stream;
   TV* t = (TV*)h->tbuf.p;
   TV* part = (TV*)h->pbuf.p;
-  if (h->n_a)
-    hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, false>), dim3(h->n_a, p), dim3(kPanelThreads), 0, s,
-                       (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
-                       (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
-  if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
-  if (h->n_b)
-    hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, true>), dim3(h->n_b, p), dim3(kPanelThreads), 0, s,
-                       (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
-                       (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
+  if (p == 1) {
+    if (h->n_a)
+      hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, false>), dim3(h->n_a, 1), dim3(kPanelThreads), 0, s,
+                         (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
+                         (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
+    if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
+    if (h->n_b)
+      hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, true>), dim3(h->n_b, 1), dim3(kPanelThreads), 0, s,
+                         (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
+                         (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
+  } else {
+    // p >= 2: MFMA panel products, 32 right-hand sides per workgroup
+    const uint32_t ng = (p + kMmRhs - 1) / kMmRhs;
+    if (h->n_a)
+      hipLaunchKernelGGL((panel_mm_kernel<TV, false>), dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,
+                         (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
+                         (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha, p);
+    if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
+    if (h->n_b)
+      hipLaunchKernelGGL((panel_mm_kernel<TV, true>), dim3(h->n_b, ng), dim3(kPanelThreads), 0, s,
+                         (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
+                         (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha, p);
+  }
   if (h->n_r)
     hipLaunchKernelGGL((chunk_reduce_kernel<TV>), dim3(h->n_r, p), dim3(kPanelThreads), 0, s,
                        (const ReduceTask*)h->tasks_r.p, (const TV*)part, h->part_len, Y, ldy, alpha);

@@ -88,7 +88,7 @@ class _HMat:
     def matmat(self, X, d=1.0, Y=None):
         X = np.asfortranarray(X, np.float64)
         p = X.shape[1]
-        Y = np.zeros((self.N, p), order="F") if Y is None else np.asfortranarray(Y, np.float64).copy()
+        Y = np.zeros((self.N, p), order="F") if Y is None else np.array(Y, np.float64, order="F", copy=True)
         self.api._matmat(self.h, _d(d), _u(p), _ptr(X), _u(self.N), _ptr(Y), _u(self.N))
         return Y
 

@@ -136,3 +136,22 @@ def test_rank_zero_and_all_dense_edge_cases(ctx):
         bad = leaves.copy()
         bad["n1"][0] = 400
         _lib.HMatrix.upload(ctx, N, N, bad, data)
+
+
+@pytest.mark.parametrize("p", [3, 17, 40])
+def test_multi_rhs_mfma_odd_counts_fp64_and_fp32(ctx, p):
+    _, T, H = sphere_hmatrix(16384, 64, 1e-8)
+    D = upload(ctx, H)
+    rng = np.random.default_rng(p)
+    X = rng.standard_normal((H.N, p))
+    Y0 = rng.standard_normal((H.N, p))
+    ref = H.matmat(X, d=0.75, Y=Y0)
+    Y = dev(Y0.T.copy())
+    D.matvec(dev(X.T.copy()), Y, 0.75)
+    assert relerr(Y.cpu().numpy().T, ref) <= 1e-12
+    D32 = D.convert()
+    Y32 = dev(np.zeros((p, H.N)), torch.float32)
+    D32.matvec(dev(X.T.copy(), torch.float32), Y32, 1.0)
+    err = relerr(Y32.cpu().numpy().T.astype(np.float64), H.matmat(X))
+    print(f"[gpu] FP32 MFMA p={p}: rel.err vs FP64 {err:.2e}")
+    assert err <= 2e-6
