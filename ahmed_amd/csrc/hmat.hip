@@ -136,13 +136,14 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
 // right-hand sides per blockIdx.y.  The output is tiled 16 x 16; wavefront w
 // owns the row tiles w, w+4, w+8, w+12 and both 16-column tiles of its RHS
 // group, so every panel value is loaded once per 32 right-hand sides.  Z is
-// gathered into LDS in chunks of kMmChunk columns.  Per k-step of 4:
+// gathered into LDS in chunks of kMmChunk columns; the panel loads of four
+// k-steps are issued before their MFMAs.  Per k-step of 4:
 //   A (16 x 4 of P):  lane l holds P[row0 + l%16][k0 + l/16]
 //   B (4 x 16 of Z):  lane l holds Z[k0 + l/16][col0 + l%16]
 //   C (16 x 16):      f64: lane l, value r -> (l/16 + 4r, l%16)
 //                     f32: lane l, value r -> (4(l/16) + r, l%16)
 // (lane layouts of v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 on gfx950).
-constexpr int kMmChunk = 64;
+constexpr int kMmChunk = 128;
 constexpr int kMmRhs = 32;
 
 typedef double mm_f64x4 __attribute__((ext_vector_type(4)));
@@ -185,8 +186,9 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
   for (uint32_t c0 = 0; c0 < tk.W; c0 += kMmChunk) {
     const int cw = (int)min((uint32_t)kMmChunk, tk.W - c0);
     __syncthreads();
+    // gather: consecutive threads take consecutive columns of one RHS
     for (int e = threadIdx.x; e < kMmChunk * kMmRhs; e += kPanelThreads) {
-      const int c = e / kMmRhs, j = e % kMmRhs;
+      const int j = e / kMmChunk, c = e % kMmChunk;
       T v = 0;
       if (c < cw && rhs0 + j < p) {
         const uint32_t s = zsrc[tk.zoff + c0 + c];
@@ -196,20 +198,30 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
       zs[c][j] = v;
     }
     __syncthreads();
-    for (int k0 = 0; k0 < cw; k0 += 4) {
-      const int k = k0 + lk;
-      const T b0 = zs[k][li];        // zero-filled beyond cw
-      const T b1 = zs[k][16 + li];
-      const T* Pk = P + (uint64_t)(c0 + k) * m;
+    // 4 k-steps (16 columns) per round: all panel loads first, then MFMAs
+    for (int k0 = 0; k0 < cw; k0 += 16) {
+      T a[4][4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int rt = w + 4 * q;
-        if (rt < nrt) {
-          const int row = rt * 16 + li;
-          const T a = (row < m && k < cw) ? Pk[row] : (T)0;
-          acc[q][0] = MmAcc<T>::mma(a, b0, acc[q][0]);
-          acc[q][1] = MmAcc<T>::mma(a, b1, acc[q][1]);
+      for (int st = 0; st < 4; ++st) {
+        const int k = k0 + 4 * st + lk;
+        const T* Pk = P + (uint64_t)(c0 + k) * m;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = (w + 4 * q) * 16 + li;
+          a[st][q] = (row < m && k < cw) ? Pk[row] : (T)0;
         }
+      }
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const int k = k0 + 4 * st + lk;   // < kMmChunk; zero-filled beyond cw
+        const T b0 = zs[k][li];
+        const T b1 = zs[k][16 + li];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (w + 4 * q < nrt) {
+            acc[q][0] = MmAcc<T>::mma(a[st][q], b0, acc[q][0]);
+            acc[q][1] = MmAcc<T>::mma(a[st][q], b1, acc[q][1]);
+          }
       }
     }
   }
