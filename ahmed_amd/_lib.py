@@ -82,6 +82,9 @@ SIGNATURES = {
     "ahm_cg": (_i, [_p, _p, _p, C.POINTER(_d), C.POINTER(_u32)]),
     "ahm_gmres_ir": (_i, [_p, _p, _p, _p, C.POINTER(_d), _u32, _d, _u32, _u32, C.POINTER(_u32),
                           C.POINTER(_u32), _p, _u32]),
+    "ahm_assemble_batch": (_i, [_p, _u32, _p, _p, _d, _u32, _i, _i, C.POINTER(AssemblyStats), C.POINTER(_p)]),
+    "ahm_hmat_segments": (_i, [_p, C.POINTER(_u32), _p]),
+    "ahm_gmres_batch": (_i, [_p, _p, _p, _d, _u32, _u32, _i, _p, _p, _p]),
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_matvec_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
     "ahm_hmat_matvec_profile": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i, _u32, C.POINTER(_d),
@@ -232,6 +235,41 @@ class HMatrix:
                                        hint.ctypes.data_as(_p) if hint is not None else None, storage,
                                        int(keep_leaves), C.byref(st), C.byref(h)), "ahm_assemble_tree")
         return cls(ctx, h, tree.N, tree.N, storage), {f: getattr(st, f) for f, _ in AssemblyStats._fields_}
+
+    @classmethod
+    def assemble_batch(cls, ctx: Context, trees, kernels, eps: float = 1e-6, rankmax: int = 0xFFFFFFFF,
+                       storage: int = F64, keep_leaves: bool = False):
+        """Independent H-matrices (trees[s] with kernels[s] = (kind, kappa,
+        weight)) assembled in one batch and stacked block-diagonally."""
+        n = len(trees)
+        th = (_p * n)(*[t.h for t in trees])
+        ks = (Kernel * n)(*[Kernel(k[0], 0, k[1], k[2]) for k in kernels])
+        st = AssemblyStats()
+        h = _p()
+        ctx.bind_stream()
+        check(load().ahm_assemble_batch(ctx.h, n, C.cast(th, _p), C.cast(ks, _p), eps, rankmax, storage,
+                                        int(keep_leaves), C.byref(st), C.byref(h)), "ahm_assemble_batch")
+        N = sum(t.N for t in trees)
+        return cls(ctx, h, N, N, storage), {f: getattr(st, f) for f, _ in AssemblyStats._fields_}
+
+    def segments(self):
+        ns = _u32()
+        check(load().ahm_hmat_segments(self.h, C.byref(ns), None), "segments")
+        off = np.zeros(ns.value + 1, np.uint32)
+        check(load().ahm_hmat_segments(self.h, C.byref(ns), off.ctypes.data_as(_p)), "segments")
+        return off
+
+    def gmres_batch(self, b, x, eps: float = 1e-8, m: int = 50, nsteps: int = 10000, use_graph: bool = True):
+        S = len(self.segments()) - 1
+        e = np.zeros(S)
+        st = np.zeros(S, np.uint32)
+        cv = np.zeros(S, np.int32)
+        self.ctx.bind_stream()
+        rc = load().ahm_gmres_batch(self.h, _p(b.data_ptr()), _p(x.data_ptr()), eps, m, nsteps, int(use_graph),
+                                    e.ctypes.data_as(_p), st.ctypes.data_as(_p), cv.ctypes.data_as(_p))
+        if rc not in (OK, 5):
+            check(rc, "ahm_gmres_batch")
+        return dict(converged=cv.astype(bool), eps=e, nsteps=st)
 
     def convert(self, storage: int = F32) -> "HMatrix":
         h = _p()

@@ -308,12 +308,14 @@ __device__ double cross_sum(const AcaBlock& B, uint32_t k, const double* U, cons
 __global__ __launch_bounds__(kAcaThreads) void aca_kernel(const AcaBlock* __restrict__ blocks,
                                                           AcaResult* __restrict__ res,
                                                           const double* __restrict__ P,
-                                                          KernelSpec K, double eps,
+                                                          const KernelSpec* __restrict__ Ks,
+                                                          double eps,
                                                           double* __restrict__ slab,
                                                           int* __restrict__ iscr) {
   __shared__ Shared sh;
   __shared__ double coef[kAcaCapMax + 1];
   const AcaBlock B = blocks[blockIdx.x];
+  const KernelSpec K = Ks[B.kidx];
   double* U = slab + B.uoff;
   double* V = slab + B.voff;
   int* Z = iscr + B.zoff;
@@ -658,8 +660,9 @@ __global__ __launch_bounds__(kAcaThreads) void expand_kernel(const AcaBlock* __r
 
 // dense leaves (and failed ACA): cmpbl of the whole block, column-major
 __global__ void dense_fill_kernel(const DenseJob* __restrict__ jobs, const double* __restrict__ P,
-                                  KernelSpec K, double* __restrict__ out) {
+                                  const KernelSpec* __restrict__ Ks, double* __restrict__ out) {
   const DenseJob J = jobs[blockIdx.x];
+  const KernelSpec K = Ks[J.kidx];
   const uint64_t n = (uint64_t)J.n1 * J.n2;
   double* D = out + J.off;
   for (uint64_t e = threadIdx.x; e < n; e += blockDim.x) {
@@ -701,11 +704,20 @@ KernelSpec make_kernel(const ahm_kernel& k) {
 
 int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uint32_t N,
                     uint32_t nl, const uint32_t* b1, const uint32_t* n1, const uint32_t* b2,
-                    const uint32_t* n2, const uint8_t* adm, const uint32_t* i0, KernelSpec K,
+                    const uint32_t* n2, const uint8_t* adm, const uint32_t* i0,
+                    const KernelSpec* Ks, uint32_t nk, const uint32_t* leaf_kernel,
                     double eps, uint32_t rankmax, const uint32_t* kcap_hint,
                     std::vector<ahm_leaf_desc>& leaves, DeviceBuf& data,
                     ahm_assembly_stats* stats) {
   hipStream_t s = ctx->stream;
+  if (nk == 0) return fail(AHM_ERR_INVALID, "no kernel");
+  for (uint32_t l = 0; leaf_kernel && l < nl; ++l)
+    if (leaf_kernel[l] >= nk) return fail(AHM_ERR_INVALID, "leaf %u: kernel %u of %u", l, leaf_kernel[l], nk);
+  DeviceBuf dK;
+  {
+    std::vector<KernelSpec> kv(Ks, Ks + nk);
+    AHM_TRY(to_device(dK, kv, s));
+  }
   std::vector<double> P(3 * (size_t)N);
   for (uint32_t i = 0; i < N; ++i) {
     if (op_perm[i] >= N) return fail(AHM_ERR_INVALID, "op_perm[%u] = %u out of range", i, op_perm[i]);
@@ -768,6 +780,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
         B.i0 = i0[l];
         B.kmax = maxk[l];
         B.kcap = (uint32_t)kc;
+        B.kidx = leaf_kernel ? leaf_kernel[l] : 0;
         B.uoff = slab;
         B.voff = slab + (kc + 1) * n1[l];
         B.roff = B.voff + (kc + 1) * n2[l];
@@ -788,8 +801,8 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       }
       const uint32_t nb = (uint32_t)blk.size();
       hipLaunchKernelGGL(aca_kernel, dim3(nb), dim3(kAcaThreads), 0, s, (const AcaBlock*)dblk.p,
-                         (AcaResult*)dres.p, (const double*)dP.p, K, eps, (double*)dslab.p,
-                         (int*)discr.p);
+                         (AcaResult*)dres.p, (const double*)dP.p, (const KernelSpec*)dK.p, eps,
+                         (double*)dslab.p, (int*)discr.p);
       hipLaunchKernelGGL(recompress_kernel, dim3(nb), dim3(kAcaThreads), 0, s,
                          (const AcaBlock*)dblk.p, (AcaResult*)dres.p, eps, (double*)dslab.p);
       std::vector<AcaResult> rr(nb);
@@ -870,7 +883,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     else if (rank[l] == 0) d.offset = 0;
     else {
       d.offset = total;
-      dj.push_back(DenseJob{b1[l], n1[l], b2[l], n2[l], total});
+      dj.push_back(DenseJob{b1[l], n1[l], b2[l], n2[l], total, leaf_kernel ? leaf_kernel[l] : 0u, 0u});
       total += (uint64_t)n1[l] * n2[l];
     }
   }
@@ -895,7 +908,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
   for (size_t j0 = 0; j0 < dj.size(); j0 += (1u << 30)) {
     const uint32_t nj = (uint32_t)std::min<size_t>(1u << 30, dj.size() - j0);
     hipLaunchKernelGGL(dense_fill_kernel, dim3(nj), dim3(256), 0, s, (const DenseJob*)ddj.p + j0,
-                       (const double*)dP.p, K, (double*)data.p);
+                       (const double*)dP.p, (const KernelSpec*)dK.p, (double*)data.p);
   }
   const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
   cleanup();
@@ -932,7 +945,8 @@ extern "C" int ahm_assemble(ahm_ctx ctx, const double* pts, const uint32_t* op_p
   AHM_HIP(hipSetDevice(ctx->device));
   std::vector<ahm_leaf_desc> leaves;
   DeviceBuf data;
-  AHM_TRY(assemble_leaves(ctx, pts, op_perm, N, n_leaves, b1, n1, b2, n2, adm, i0,
-                          make_kernel(*kernel), eps, rankmax, rank_hint, leaves, data, stats));
+  const KernelSpec K = make_kernel(*kernel);
+  AHM_TRY(assemble_leaves(ctx, pts, op_perm, N, n_leaves, b1, n1, b2, n2, adm, i0, &K, 1, nullptr,
+                          eps, rankmax, rank_hint, leaves, data, stats));
   return hmat_from_device(ctx, N, N, std::move(leaves), data, storage, keep_leaves, out);
 }

@@ -23,7 +23,7 @@ struct AcaBlock {
   uint32_t i0;        // ACA start row (relative to b1)
   uint32_t kmax;      // reference rank limit n1*n2/(n1+n2)+4 (or rankmax)
   uint32_t kcap;      // slab capacity in rank columns (index 0..kcap)
-  uint32_t pad;
+  uint32_t kidx;      // kernel of this leaf (index into the kernel table)
   uint64_t uoff, voff, roff;  // slab offsets (doubles)
   uint64_t zoff;              // Z/S offsets (ints)
 };
@@ -38,13 +38,16 @@ struct AcaResult {
 struct DenseJob {
   uint32_t b1, n1, b2, n2;
   uint64_t off;
+  uint32_t kidx, pad;
 };
 
 KernelSpec make_kernel(const ahm_kernel& k);
 
+// Ks[0..nk) kernel table; leaf_kernel (may be NULL: all 0) selects per leaf
 int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uint32_t N,
                     uint32_t nl, const uint32_t* b1, const uint32_t* n1, const uint32_t* b2,
-                    const uint32_t* n2, const uint8_t* adm, const uint32_t* i0, KernelSpec K,
+                    const uint32_t* n2, const uint8_t* adm, const uint32_t* i0,
+                    const KernelSpec* Ks, uint32_t nk, const uint32_t* leaf_kernel,
                     double eps, uint32_t rankmax, const uint32_t* kcap_hint,
                     std::vector<ahm_leaf_desc>& leaves, DeviceBuf& data,
                     ahm_assembly_stats* stats);

@@ -105,6 +105,9 @@ struct ahm_hmat_s {
   ahm::DeviceBuf pbuf;        // partial sums: part_len * p values
   size_t pbuf_cap = 0;
   std::vector<ahm::ReduceTask> h_tasks_r;
+  // independent systems stacked block-diagonally (batched assembly):
+  // system s owns rows/columns [seg_off[s], seg_off[s+1])
+  std::vector<uint32_t> seg_off;
   uint64_t n_zsrc = 0;
   ahm::DeviceBuf tbuf;        // V^T X workspace: t_len * p values
   size_t tbuf_cap = 0;        // bytes

@@ -659,6 +659,7 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
   h->h_tasks_a = src->h_tasks_a;
   h->h_tasks_b = src->h_tasks_b;
   h->h_tasks_r = src->h_tasks_r;
+  h->seg_off = src->seg_off;
   h->h_zsrc = src->h_zsrc;
   h->part_len = src->part_len;
   int st = AHM_OK;
@@ -713,6 +714,21 @@ int ahm_hmat_stats_get(ahm_hmat h, ahm_hmat_stats* st) {
   st->max_rank = h->max_rank;
   st->n_rows = h->n_rows;
   st->n_cols = h->n_cols;
+  return AHM_OK;
+}
+
+int ahm_hmat_segments(ahm_hmat h, uint32_t* n_seg, uint32_t* offsets) {
+  if (!h || !n_seg) return fail(AHM_ERR_INVALID, "ahm_hmat_segments: NULL argument");
+  if (h->seg_off.empty()) {
+    *n_seg = 1;
+    if (offsets) {
+      offsets[0] = 0;
+      offsets[1] = h->n_rows;
+    }
+    return AHM_OK;
+  }
+  *n_seg = (uint32_t)h->seg_off.size() - 1;
+  if (offsets) std::memcpy(offsets, h->seg_off.data(), h->seg_off.size() * sizeof(uint32_t));
   return AHM_OK;
 }
 

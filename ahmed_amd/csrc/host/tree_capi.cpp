@@ -7,6 +7,7 @@
 
 #include "ahm.h"
 #include "ahm_internal.h"
+#include "ahm_aca.h"
 #include "ahmed/bemblcluster.h"
 #include "ahmed/blcluster.h"
 #include "ahmed/cluster_bbx.h"
@@ -156,4 +157,54 @@ extern "C" int ahm_assemble_tree(ahm_ctx ctx, ahm_tree t, const ahm_kernel* kern
   return ahm_assemble(ctx, pts.data(), t->op.data(), t->N, (uint32_t)nl, b1.data(), n1.data(),
                       b2.data(), n2.data(), adm.data(), t->i0.data(), kernel, eps, rankmax,
                       rank_hint, storage, keep_leaves, stats, out);
+}
+
+// n_mat independent H-matrices (own points, tree and kernel) stacked into one
+// block-diagonal H-matrix: one batched ACA / recompression launch for all
+// leaves, one launch per phase for every product afterwards.
+extern "C" int ahm_assemble_batch(ahm_ctx ctx, uint32_t n_mat, const ahm_tree* trees,
+                                  const ahm_kernel* kernels, double eps, uint32_t rankmax,
+                                  ahm_dtype storage, int keep_leaves, ahm_assembly_stats* stats,
+                                  ahm_hmat* out) {
+  if (!ctx || !trees || !kernels || !out || n_mat == 0)
+    return ahm::fail(AHM_ERR_INVALID, "ahm_assemble_batch: bad arguments");
+  std::vector<uint32_t> off(n_mat + 1, 0);
+  for (uint32_t s = 0; s < n_mat; ++s) {
+    if (!trees[s]) return ahm::fail(AHM_ERR_INVALID, "ahm_assemble_batch: tree %u is NULL", s);
+    if ((uint64_t)off[s] + trees[s]->N >= 0x7fffffffu)
+      return ahm::fail(AHM_ERR_UNSUPPORTED, "ahm_assemble_batch: too many points");
+    off[s + 1] = off[s] + trees[s]->N;
+  }
+  const uint32_t N = off[n_mat];
+  std::vector<double> pts(3 * (size_t)N);
+  std::vector<uint32_t> op(N), b1, n1, b2, n2, i0, lk;
+  std::vector<uint8_t> adm;
+  std::vector<ahm::KernelSpec> K(n_mat);
+  for (uint32_t s = 0; s < n_mat; ++s) {
+    const ahm_tree t = trees[s];
+    for (uint32_t i = 0; i < t->N; ++i) {
+      for (int j = 0; j < 3; ++j) pts[3 * ((size_t)off[s] + i) + j] = t->pts[i].x[j];
+      op[off[s] + i] = off[s] + t->op[i];
+    }
+    for (size_t l = 0; l < t->leaves.size(); ++l) {
+      const blcluster* b = t->leaves[l];
+      b1.push_back(off[s] + b->getb1());
+      n1.push_back(b->getn1());
+      b2.push_back(off[s] + b->getb2());
+      n2.push_back(b->getn2());
+      adm.push_back(b->isadm() ? 1 : 0);
+      i0.push_back(t->i0[l]);
+      lk.push_back(s);
+    }
+    K[s] = ahm::make_kernel(kernels[s]);
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) return ahm::fail(AHM_ERR_HIP, "hipSetDevice");
+  std::vector<ahm_leaf_desc> leaves;
+  ahm::DeviceBuf data;
+  AHM_TRY(ahm::assemble_leaves(ctx, pts.data(), op.data(), N, (uint32_t)b1.size(), b1.data(),
+                               n1.data(), b2.data(), n2.data(), adm.data(), i0.data(), K.data(),
+                               n_mat, lk.data(), eps, rankmax, nullptr, leaves, data, stats));
+  AHM_TRY(ahm::hmat_from_device(ctx, N, N, std::move(leaves), data, storage, keep_leaves, out));
+  (*out)->seg_off = off;
+  return AHM_OK;
 }

@@ -154,6 +154,19 @@ int ahm_assemble_tree(ahm_ctx ctx, ahm_tree tree, const ahm_kernel* kernel, doub
                       uint32_t rankmax, const uint32_t* rank_hint, ahm_dtype storage,
                       int keep_leaves, ahm_assembly_stats* stats, ahm_hmat* out);
 
+/* n_mat independent H-matrices (each with its own points, tree and kernel)
+ * assembled in one batched launch and stacked block-diagonally: system s
+ * owns rows/columns [offsets[s], offsets[s+1]) of the result, with offsets
+ * the running sums of the trees' point counts.  Every product of the result
+ * is one launch per phase for all systems. */
+int ahm_assemble_batch(ahm_ctx ctx, uint32_t n_mat, const ahm_tree* trees,
+                       const ahm_kernel* kernels, double eps, uint32_t rankmax,
+                       ahm_dtype storage, int keep_leaves, ahm_assembly_stats* stats,
+                       ahm_hmat* out);
+/* number of independent systems and their offsets (n_seg + 1 values);
+ * a matrix not built by ahm_assemble_batch is one system. */
+int ahm_hmat_segments(ahm_hmat h, uint32_t* n_seg, uint32_t* offsets);
+
 /* -------- products: mltaGeHVec / mltaGeHGeM (H/mltaGeHVec.cpp:567-577) ----
  * Y += alpha * H * X for p right-hand sides; X is n_cols x p column-major
  * with leading dimension ldx, Y n_rows x p with ldy (device pointers).
@@ -194,6 +207,18 @@ int ahm_cg(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nsteps
 int ahm_gmres_ir(ahm_hmat hi, ahm_hmat lo, const double* b, double* x, double* eps, uint32_t m,
                  double inner_eps, uint32_t inner_steps, uint32_t max_outer, uint32_t* outer,
                  uint32_t* total_inner, double* hist, uint32_t nhist);
+
+/* GMRes(m) on every independent system of a batched matrix
+ * (ahm_assemble_batch) at once: per-system Krylov spaces, rotations and
+ * convergence tests (solvers/GMRES.cpp:80-181 each), Arnoldi steps in
+ * lockstep so one H-matvec launch per phase serves all systems; with
+ * use_graph != 0 each Arnoldi step is captured once into a hipGraph and
+ * replayed.  b, x: FP64 device vectors over all systems.  Per system
+ * (may be NULL): achieved residual, steps (the reference's nsteps output),
+ * converged flag.  AHM_OK if every system converged. */
+int ahm_gmres_batch(ahm_hmat h, const double* b, double* x, double eps, uint32_t m,
+                    uint32_t nsteps, int use_graph, double* eps_out, uint32_t* steps_out,
+                    int* converged);
 
 #ifdef __cplusplus
 }
