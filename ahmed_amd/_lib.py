@@ -86,6 +86,7 @@ SIGNATURES = {
     "ahm_hmat_segments": (_i, [_p, C.POINTER(_u32), _p]),
     "ahm_gmres_batch": (_i, [_p, _p, _p, _d, _u32, _u32, _i, _p, _p, _p]),
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
+    "ahm_hmat_autotune": (_i, [_p, _u32, _i, _u32, C.POINTER(C.c_uint64), C.POINTER(_d)]),
     "ahm_hmat_matvec_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
     "ahm_hmat_matvec_profile": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i, _u32, C.POINTER(_d),
                                      C.POINTER(_d)]),
@@ -356,6 +357,13 @@ class HMatrix:
             check(rc, "ahm_gmres_ir")
         return dict(converged=rc == OK, eps=e.value, outer=outer.value, inner=inner.value,
                     hist=hist[:min(nhist, outer.value + 1)].copy())
+
+    def autotune(self, p: int = 1, reps: int = 5):
+        """Pick the fastest row-panel chunk size for p right-hand sides."""
+        cv, ms = C.c_uint64(), _d()
+        self.ctx.bind_stream()
+        check(load().ahm_hmat_autotune(self.h, p, self.storage, reps, C.byref(cv), C.byref(ms)), "autotune")
+        return cv.value, ms.value
 
     def matvec_host(self, x: np.ndarray, y: np.ndarray, alpha: float = 1.0):
         x = np.asfortranarray(x, np.float64)

@@ -108,6 +108,9 @@ struct ahm_hmat_s {
   // independent systems stacked block-diagonally (batched assembly):
   // system s owns rows/columns [seg_off[s], seg_off[s+1])
   std::vector<uint32_t> seg_off;
+  // phase-B row panels before chunking, and the chunk size in use
+  std::vector<ahm::GemvTask> h_panels;
+  uint64_t chunk_values = ahm::kChunkValues;
   uint64_t n_zsrc = 0;
   ahm::DeviceBuf tbuf;        // V^T X workspace: t_len * p values
   size_t tbuf_cap = 0;        // bytes

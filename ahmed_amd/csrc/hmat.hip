@@ -178,6 +178,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
   const int m = (int)tk.m;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nrt = (m + 15) / 16;
+  const bool two = p - rhs0 > 16;  // second 16-column tile needed
   acc_t acc[4][2];
   for (int a = 0; a < 4; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
@@ -220,7 +221,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
         for (int q = 0; q < 4; ++q)
           if (w + 4 * q < nrt) {
             acc[q][0] = MmAcc<T>::mma(a[st][q], b0, acc[q][0]);
-            acc[q][1] = MmAcc<T>::mma(a[st][q], b1, acc[q][1]);
+            if (two) acc[q][1] = MmAcc<T>::mma(a[st][q], b1, acc[q][1]);
           }
       }
     }
@@ -233,6 +234,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
     for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
+        if (ct == 1 && !two) continue;
         const int row = rt * 16 + MmAcc<T>::row(lane, r);
         const uint32_t j = rhs0 + ct * 16 + li;
         if (row >= m || j >= p) continue;
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
 // plan: host-side construction of the tasks and copy jobs
 
 struct Plan {
-  std::vector<GemvTask> a, b;
+  std::vector<GemvTask> a, b, panels;  // panels: phase-B row panels before chunking
   std::vector<ReduceTask> r;
   uint32_t part_len = 0;
   std::vector<uint32_t> zsrc;
@@ -278,6 +280,40 @@ struct Plan {
   uint64_t n_dense = 0, n_lowrank = 0;
   uint64_t nv_dense = 0, nv_u = 0, nv_v = 0;
 };
+
+// Phase-B tasks from the row panels: panels wider than about chunk_values
+// values are cut into equal column chunks whose row sums meet in a reduction
+// task; tasks sorted largest first.
+static void chunk_panels(const std::vector<GemvTask>& panels, uint64_t chunk_values,
+                         std::vector<GemvTask>& b, std::vector<ReduceTask>& r, uint32_t& part_len) {
+  b.clear();
+  r.clear();
+  part_len = 0;
+  for (const GemvTask& tk : panels) {
+    const uint64_t vals = (uint64_t)tk.m * tk.W;
+    const uint32_t nch = (uint32_t)std::min<uint64_t>(tk.W, (vals + chunk_values - 1) / chunk_values);
+    if (nch <= 1) {
+      b.push_back(tk);
+      continue;
+    }
+    for (uint32_t c = 0; c < nch; ++c) {
+      const uint32_t c0 = (uint32_t)((uint64_t)tk.W * c / nch);
+      const uint32_t c1 = (uint32_t)((uint64_t)tk.W * (c + 1) / nch);
+      GemvTask ck = tk;
+      ck.data = tk.data + (uint64_t)c0 * tk.m;
+      ck.zoff = tk.zoff + c0;
+      ck.W = c1 - c0;
+      ck.out = part_len + c * tk.m;
+      ck.partial = 1;
+      b.push_back(ck);
+    }
+    r.push_back(ReduceTask{tk.out, tk.m, part_len, nch});
+    part_len += nch * tk.m;
+  }
+  std::stable_sort(b.begin(), b.end(), [](const GemvTask& x, const GemvTask& y) {
+    return (uint64_t)x.m * x.W > (uint64_t)y.m * y.W;
+  });
+}
 
 static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_leaf_desc* L,
                       Plan& P) {
@@ -406,30 +442,8 @@ static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_l
         W += cols;
       }
       tk.W = W;
-      if (W) {
-        // cut wide panels into column chunks of about kChunkValues values
-        const uint64_t vals = (uint64_t)m * W;
-        const uint32_t nch = (uint32_t)std::min<uint64_t>(W, (vals + kChunkValues - 1) / kChunkValues);
-        if (nch <= 1) {
-          tk.partial = 0;
-          P.b.push_back(tk);
-        } else {
-          ReduceTask rt{r0, m, P.part_len, nch};
-          for (uint32_t c = 0; c < nch; ++c) {
-            const uint32_t c0 = (uint32_t)((uint64_t)W * c / nch);
-            const uint32_t c1 = (uint32_t)((uint64_t)W * (c + 1) / nch);
-            GemvTask ck = tk;
-            ck.data = tk.data + (uint64_t)c0 * m;
-            ck.zoff = tk.zoff + c0;
-            ck.W = c1 - c0;
-            ck.out = P.part_len + c * m;
-            ck.partial = 1;
-            P.b.push_back(ck);
-          }
-          P.part_len += nch * m;
-          P.r.push_back(rt);
-        }
-      }
+      tk.partial = 0;
+      if (W) P.panels.push_back(tk);
       arena += (uint64_t)m * W;
     }
   }
@@ -438,7 +452,7 @@ static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_l
     return (uint64_t)a.m * a.W > (uint64_t)b.m * b.W;
   };
   std::stable_sort(P.a.begin(), P.a.end(), big);
-  std::stable_sort(P.b.begin(), P.b.end(), big);
+  chunk_panels(P.panels, kChunkValues, P.b, P.r, P.part_len);
   P.nvals = arena;
   P.t_len = (uint32_t)t_len;
   for (uint32_t l = 0; l < nl; ++l) {
@@ -520,6 +534,29 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
 
 namespace ahm {
 
+// re-cut the phase-B row panels into chunks of about cv values (the arena is
+// unchanged: chunks are column ranges of the same panels)
+int set_chunking(ahm_hmat h, uint64_t cv) {
+  std::vector<GemvTask> b;
+  std::vector<ReduceTask> r;
+  uint32_t part_len = 0;
+  chunk_panels(h->h_panels, cv, b, r, part_len);
+  hipStream_t s = h->ctx->stream;
+  AHM_HIP(hipStreamSynchronize(s));
+  AHM_TRY(upload_vec(h->tasks_b, b, s));
+  AHM_TRY(upload_vec(h->tasks_r, r, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  h->n_b = (uint32_t)b.size();
+  h->n_r = (uint32_t)r.size();
+  h->part_len = part_len;
+  h->pbuf_cap = 0;  // partial-sum slots change with the chunking
+  h->pbuf.release();
+  h->h_tasks_b = std::move(b);
+  h->h_tasks_r = std::move(r);
+  h->chunk_values = cv;
+  return AHM_OK;
+}
+
 int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_leaf_desc>&& leaves,
                DeviceBuf& raw, ahm_dtype data_dtype, ahm_dtype storage, int keep_leaves,
                ahm_hmat* out) {
@@ -581,6 +618,8 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
     h->h_tasks_a = std::move(P.a);
     h->h_tasks_b = std::move(P.b);
     h->h_tasks_r = std::move(P.r);
+    h->h_panels = std::move(P.panels);
+    h->chunk_values = kChunkValues;
     h->h_zsrc = std::move(P.zsrc);
     if (keep_leaves) {
       h->leafdata.p = raw.p;
@@ -659,6 +698,8 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
   h->h_tasks_a = src->h_tasks_a;
   h->h_tasks_b = src->h_tasks_b;
   h->h_tasks_r = src->h_tasks_r;
+  h->h_panels = src->h_panels;
+  h->chunk_values = src->chunk_values;
   h->seg_off = src->seg_off;
   h->h_zsrc = src->h_zsrc;
   h->part_len = src->part_len;
@@ -809,6 +850,58 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
   if (st) return st;
   *ms_a = ta / reps;
   *ms_b = tb / reps;
+  return AHM_OK;
+}
+
+int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint64_t* chunk_values,
+                      double* best_ms) {
+  if (!h || p == 0 || reps == 0) return fail(AHM_ERR_INVALID, "ahm_hmat_autotune: bad arguments");
+  if (!((h->storage == AHM_F64 && vec == AHM_F64) || (h->storage == AHM_F32 && vec == AHM_F32)))
+    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_autotune: storage %d with vectors %d", h->storage, vec);
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  hipStream_t s = h->ctx->stream;
+  const size_t es = dtype_size(vec);
+  DeviceBuf dx, dy;
+  AHM_TRY(dx.alloc((size_t)h->n_cols * p * es));
+  AHM_TRY(dy.alloc((size_t)h->n_rows * p * es));
+  AHM_HIP(hipMemsetAsync(dx.p, 0, (size_t)h->n_cols * p * es, s));
+  AHM_HIP(hipMemsetAsync(dy.p, 0, (size_t)h->n_rows * p * es, s));
+  const uint64_t cand[] = {16u << 10, 32u << 10, 48u << 10, 96u << 10, 192u << 10, 1ull << 40};
+  double best = 1e300;
+  uint64_t arg = h->chunk_values;
+  hipEvent_t e0, e1;
+  AHM_HIP(hipEventCreate(&e0));
+  AHM_HIP(hipEventCreate(&e1));
+  int st = AHM_OK;
+  for (uint64_t cv : cand) {
+    if ((st = set_chunking(h, cv))) break;
+    AHM_TRY(ensure_tbuf(h, p, es));
+    AHM_TRY(ensure_pbuf(h, p, es));
+    auto run = [&]() {
+      return vec == AHM_F64
+                 ? launch_matvec<double, double>(h, 1.0, (const double*)dx.p, h->n_cols, (double*)dy.p,
+                                                 h->n_rows, p)
+                 : launch_matvec<float, float>(h, 1.0f, (const float*)dx.p, h->n_cols, (float*)dy.p,
+                                               h->n_rows, p);
+    };
+    if ((st = run())) break;  // warm-up
+    AHM_HIP(hipEventRecord(e0, s));
+    for (uint32_t r = 0; r < reps && !st; ++r) st = run();
+    AHM_HIP(hipEventRecord(e1, s));
+    AHM_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    AHM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms / reps < best) {
+      best = ms / reps;
+      arg = cv;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  AHM_TRY(set_chunking(h, arg));
+  if (chunk_values) *chunk_values = arg;
+  if (best_ms) *best_ms = best;
   return AHM_OK;
 }
 

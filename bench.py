@@ -112,12 +112,82 @@ def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, budget_s=20.0):
     return out
 
 
+def bench_batched(args):
+    """64 independent H-matrices, N log-uniform in [4096, 65536] on seeded
+    ellipsoids with axis ratios U(1,4), kernel exp(-k r)/(4 pi r) with
+    k ~ U(0,5), ACA eps 1e-6, 8 RHS each: one batched assembly, one launch per
+    phase for every product, batched GMRES(50) with hipGraph-captured steps."""
+    import torch
+    from ahmed_amd import _lib, geometry
+    rng = np.random.default_rng(2024)
+    n_mat = 64
+    trees, kernels, sizes = [], [], []
+    for s in range(n_mat):
+        N = int(np.exp(rng.uniform(np.log(4096), np.log(65536))))
+        a, b = rng.uniform(1, 4, 2)
+        kappa = rng.uniform(0, 5)
+        pts = geometry.ellipsoid(N, (1.0, a, b), seed=1000 + s)
+        p = 1.6075
+        area = 4 * np.pi * ((a ** p + b ** p + (a * b) ** p) / 3) ** (1 / p)
+        trees.append(_lib.Tree(pts, 64, 0.64))
+        kernels.append((_lib.KERNEL_YUKAWA, kappa, area / N))
+        sizes.append(N)
+    ctx = _lib.Context(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    H, ast = _lib.HMatrix.assemble_batch(ctx, trees, kernels, eps=1e-6)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    st = H.stats()
+    Ntot = H.n_rows
+    log(f"batched assembly of {n_mat} matrices (N total {Ntot}): {t_asm:.2f} s, nvals {st['nvals']:.4g}")
+    p = 8
+    X = torch.randn(p, Ntot, dtype=torch.float64, device="cuda")
+    Y = torch.zeros(p, Ntot, dtype=torch.float64, device="cuda")
+    ms = timed(lambda: H.matvec(X, Y), torch, args.steps, args.warmup)
+    B_alg = 8.0 * st["nvals"] + 8.0 * p * 3 * Ntot
+    F_alg = 2.0 * p * st["nvals"]
+    x1 = torch.randn(Ntot, dtype=torch.float64, device="cuda")
+    y1 = torch.zeros(Ntot, dtype=torch.float64, device="cuda")
+    ms1 = timed(lambda: H.matvec(x1, y1), torch, args.steps, args.warmup)
+    log(f"batched H-matvec: 8 RHS {ms:.3f} ms ({B_alg / ms / 1e6:.1f} GB/s, {F_alg / ms / 1e9:.2f} TF/s); "
+        f"1 RHS {ms1:.3f} ms ({(8.0 * st['nvals'] + 24.0 * Ntot) / ms1 / 1e6:.1f} GB/s)")
+    b = torch.zeros(Ntot, dtype=torch.float64, device="cuda")
+    H.matvec(torch.ones(Ntot, dtype=torch.float64, device="cuda"), b)
+    res = {}
+    for graph in (True, False):
+        xs = torch.zeros(Ntot, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = H.gmres_batch(b, xs, eps=1e-8, m=50, nsteps=2000, use_graph=graph)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res["graph" if graph else "eager"] = dict(seconds=round(dt, 4), all_converged=bool(r["converged"].all()),
+                                                  max_steps=int(r["nsteps"].max()),
+                                                  mean_steps=float(r["nsteps"].mean()),
+                                                  max_err_vs_ones=float((xs - 1).abs().max()))
+        log(f"batched GMRES(50) {'graph' if graph else 'eager'}: {dt:.3f} s, steps max {r['nsteps'].max()}")
+    line = {
+        "metric": "batched H-matvec HBM GB/s (64 independent H-matrices, 8 RHS)",
+        "value": round(B_alg / ms / 1e6, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded ellipsoids)",
+        "config": {"workload": "64 H-matrices, N log-uniform in [4096, 65536], ellipsoids axis ratios U(1,4), "
+                               "exp(-k r)/(4 pi r) with k ~ U(0,5), ACA eps 1e-6, 8 RHS, leaf 64, eta 0.8",
+                   "config": "cfg4", "N_total": Ntot, "sizes_min_max": [min(sizes), max(sizes)],
+                   "nvals": st["nvals"]},
+        "TF/s": round(F_alg / ms / 1e9, 3), "matvec_1rhs_ms": round(ms1, 4),
+        "assembly_s": round(t_asm, 3), "aca_retries": ast["n_retries"], "gmres_batch": res,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", choices=["cfg3", "cfg2"], default="cfg3")
+    ap.add_argument("--config", choices=["cfg3", "cfg2", "cfg4"], default="cfg3")
     ap.add_argument("--n", type=int, default=0, help="override N")
     ap.add_argument("--rhs", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
@@ -131,6 +201,8 @@ def main():
     from ahmed_amd import _lib, geometry
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.config == "cfg4":
+        return bench_batched(args)
     if args.config == "cfg2":
         N, eps, tol, geo = args.n or 262144, 1e-8, 1e-10, "unit sphere"
         pts = geometry.sphere(N, 0)

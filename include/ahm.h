@@ -180,6 +180,11 @@ int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void*
 int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y,
                             uint32_t ldy, uint32_t p, ahm_dtype vec, uint32_t reps,
                             double* ms_phase_a, double* ms_phase_b);
+/* Autotuner: times `reps` products with p right-hand sides for each
+ * candidate column-chunk size of the row panels (16K .. 192K values, or
+ * none) and keeps the fastest; results are unchanged by the choice. */
+int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps,
+                      uint64_t* chunk_values, double* best_ms);
 /* Same with host vectors (the drop-in for mltaGeHVec's host pointers):
  * copies X in and Y in/out over PCIe around the device product. */
 int ahm_hmat_matvec_host(ahm_hmat h, double alpha, const double* X, uint32_t ldx,

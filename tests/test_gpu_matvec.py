@@ -155,3 +155,26 @@ def test_multi_rhs_mfma_odd_counts_fp64_and_fp32(ctx, p):
     err = relerr(Y32.cpu().numpy().T.astype(np.float64), H.matmat(X))
     print(f"[gpu] FP32 MFMA p={p}: rel.err vs FP64 {err:.2e}")
     assert err <= 2e-6
+
+
+def test_autotuned_chunking_keeps_results(ctx):
+    _, T, H = sphere_hmatrix(32768, 64, 1e-8, weighted=False)
+    D = upload(ctx, H)
+    rng = np.random.default_rng(9)
+    x = dev(rng.standard_normal(H.N))
+    y0 = dev(np.zeros(H.N))
+    D.matvec(x, y0)
+    X = dev(rng.standard_normal((8, H.N)))
+    Y0 = dev(np.zeros((8, H.N)))
+    D.matvec(X, Y0)
+    for p in (1, 8):
+        cv, ms = D.autotune(p=p, reps=3)
+        print(f"[gpu] autotune p={p}: chunk {cv} values, {ms:.3f} ms")
+        y = dev(np.zeros(H.N))
+        D.matvec(x, y)
+        Y = dev(np.zeros((8, H.N)))
+        D.matvec(X, Y)
+        # chunking changes the summation order across chunks only
+        assert relerr(y.cpu().numpy(), y0.cpu().numpy()) <= 1e-14
+        assert relerr(Y.cpu().numpy(), Y0.cpu().numpy()) <= 1e-14
+    assert relerr(y0.cpu().numpy(), H.matvec(x.cpu().numpy())) <= 1e-12
