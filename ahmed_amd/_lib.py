@@ -44,7 +44,8 @@ class Kernel(C.Structure):
 
 class AssemblyStats(C.Structure):
     _fields_ = [("n_candidates", C.c_uint64), ("n_retries", C.c_uint64), ("n_lowrank", C.c_uint64),
-                ("n_dense", C.c_uint64), ("sum_rank", C.c_uint64), ("sum_aca_rank", C.c_uint64)]
+                ("n_dense", C.c_uint64), ("sum_rank", C.c_uint64), ("sum_aca_rank", C.c_uint64),
+                ("slab_bytes", C.c_uint64), ("peak_slab_bytes", C.c_uint64)]
 
 
 KERNEL_LAPLACE, KERNEL_YUKAWA, KERNEL_EXPR = 0, 1, 2
@@ -75,9 +76,9 @@ SIGNATURES = {
     "ahm_hmat_stats_get": (_i, [_p, C.POINTER(HMatStats)]),
     "ahm_hmat_download": (_i, [_p, _p, _p]),
     "ahm_assemble": (_i, [_p, _p, _p, _u32, _u32, _p, _p, _p, _p, _p, _p, C.POINTER(Kernel), _d, _u32, _p,
-                          _i, _i, C.POINTER(AssemblyStats), C.POINTER(_p)]),
+                          _i, _i, C.POINTER(AssemblyStats), _p, C.POINTER(_p)]),
     "ahm_assemble_tree": (_i, [_p, _p, C.POINTER(Kernel), _d, _u32, _p, _i, _i, C.POINTER(AssemblyStats),
-                               C.POINTER(_p)]),
+                               _p, C.POINTER(_p)]),
     "ahm_gmres": (_i, [_p, _p, _p, C.POINTER(_d), _u32, C.POINTER(_u32), _p, _u32, C.POINTER(_u32)]),
     "ahm_cg": (_i, [_p, _p, _p, C.POINTER(_d), C.POINTER(_u32)]),
     "ahm_gmres_ir": (_i, [_p, _p, _p, _p, C.POINTER(_d), _u32, _d, _u32, _u32, C.POINTER(_u32),
@@ -85,6 +86,7 @@ SIGNATURES = {
     "ahm_assemble_batch": (_i, [_p, _u32, _p, _p, _d, _u32, _i, _i, C.POINTER(AssemblyStats), C.POINTER(_p)]),
     "ahm_hmat_segments": (_i, [_p, C.POINTER(_u32), _p]),
     "ahm_gmres_batch": (_i, [_p, _p, _p, _d, _u32, _u32, _i, _p, _p, _p]),
+    "ahm_rank_predict": (_i, [_p, _p, _u32, _p, C.c_float, _u32, _p, _p, _p]),
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_autotune": (_i, [_p, _u32, _i, _u32, C.POINTER(C.c_uint64), C.POINTER(_d)]),
     "ahm_hmat_matvec_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
@@ -222,20 +224,27 @@ class HMatrix:
     @classmethod
     def assemble(cls, ctx: Context, tree: "Tree", kernel: int = KERNEL_LAPLACE, kappa: float = 0.0,
                  weight: float = 1.0, eps: float = 1e-6, rankmax: int = 0xFFFFFFFF, rank_hint=None,
-                 storage: int = F64, keep_leaves: bool = False):
+                 storage: int = F64, keep_leaves: bool = False, aca_ranks: bool = False):
         """GPU assembly (ACA + recompression + dense leaves) on a tree; returns
-        (HMatrix, stats dict)."""
+        (HMatrix, stats dict); stats["aca_rank"] holds per-leaf ACA ranks when
+        aca_ranks is set."""
         k = Kernel(kernel, 0, kappa, weight)
         st = AssemblyStats()
         h = _p()
         hint = None
         if rank_hint is not None:
             hint = np.ascontiguousarray(rank_hint, np.uint32)
+        ar = np.zeros(tree.n_leaves, np.uint32) if aca_ranks else None
         ctx.bind_stream()
         check(load().ahm_assemble_tree(ctx.h, tree.h, C.byref(k), eps, rankmax,
                                        hint.ctypes.data_as(_p) if hint is not None else None, storage,
-                                       int(keep_leaves), C.byref(st), C.byref(h)), "ahm_assemble_tree")
-        return cls(ctx, h, tree.N, tree.N, storage), {f: getattr(st, f) for f, _ in AssemblyStats._fields_}
+                                       int(keep_leaves), C.byref(st),
+                                       ar.ctypes.data_as(_p) if ar is not None else None, C.byref(h)),
+              "ahm_assemble_tree")
+        out = {f: getattr(st, f) for f, _ in AssemblyStats._fields_}
+        if ar is not None:
+            out["aca_rank"] = ar
+        return cls(ctx, h, tree.N, tree.N, storage), out
 
     @classmethod
     def assemble_batch(cls, ctx: Context, trees, kernels, eps: float = 1e-6, rankmax: int = 0xFFFFFFFF,

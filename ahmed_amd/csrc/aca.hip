@@ -708,7 +708,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
                     const KernelSpec* Ks, uint32_t nk, const uint32_t* leaf_kernel,
                     double eps, uint32_t rankmax, const uint32_t* kcap_hint,
                     std::vector<ahm_leaf_desc>& leaves, DeviceBuf& data,
-                    ahm_assembly_stats* stats) {
+                    ahm_assembly_stats* stats, uint32_t* aca_rank_out) {
   hipStream_t s = ctx->stream;
   if (nk == 0) return fail(AHM_ERR_INVALID, "no kernel");
   for (uint32_t l = 0; leaf_kernel && l < nl; ++l)
@@ -748,7 +748,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     const uint32_t h = kcap_hint ? std::max<uint32_t>(kcap_hint[l], 1) : kAcaCapDefault;
     cap[c] = std::min(h, lim);
   }
-  uint64_t retries = 0;
+  uint64_t retries = 0, slab_total = 0, slab_peak = 0;
   std::vector<size_t> todo(cand.size());
   std::iota(todo.begin(), todo.end(), 0);
   const uint64_t budget = (uint64_t)24 << 30;  // bytes of ACA scratch per batch
@@ -790,6 +790,8 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
         blk.push_back(B);
         ++end;
       }
+      slab_total += slab * 8;
+      slab_peak = std::max<uint64_t>(slab_peak, slab * 8);
       DeviceBuf dblk, dres, dslab, discr;
       int st = to_device(dblk, blk, s);
       if (!st) st = dres.alloc(blk.size() * sizeof(AcaResult));
@@ -913,10 +915,14 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
   const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
   cleanup();
   if (!ok) return fail(AHM_ERR_HIP, "dense leaf kernel failed");
+  if (aca_rank_out)
+    for (uint32_t l = 0; l < nl; ++l) aca_rank_out[l] = kaca[l];
   if (stats) {
     std::memset(stats, 0, sizeof *stats);
     stats->n_candidates = cand.size();
     stats->n_retries = retries;
+    stats->slab_bytes = slab_total;
+    stats->peak_slab_bytes = slab_peak;
     stats->n_dense = dj.size();
     for (uint32_t l = 0; l < nl; ++l)
       if (rank[l] >= 0) {
@@ -935,7 +941,8 @@ extern "C" int ahm_assemble(ahm_ctx ctx, const double* pts, const uint32_t* op_p
                             const uint32_t* b2, const uint32_t* n2, const uint8_t* adm,
                             const uint32_t* i0, const ahm_kernel* kernel, double eps,
                             uint32_t rankmax, const uint32_t* rank_hint, ahm_dtype storage,
-                            int keep_leaves, ahm_assembly_stats* stats, ahm_hmat* out) {
+                            int keep_leaves, ahm_assembly_stats* stats, uint32_t* aca_rank,
+                            ahm_hmat* out) {
   using namespace ahm;
   if (!ctx || !pts || !op_perm || !kernel || !out || !b1 || !n1 || !b2 || !n2 || !adm || !i0)
     return fail(AHM_ERR_INVALID, "ahm_assemble: NULL argument");
@@ -947,6 +954,6 @@ extern "C" int ahm_assemble(ahm_ctx ctx, const double* pts, const uint32_t* op_p
   DeviceBuf data;
   const KernelSpec K = make_kernel(*kernel);
   AHM_TRY(assemble_leaves(ctx, pts, op_perm, N, n_leaves, b1, n1, b2, n2, adm, i0, &K, 1, nullptr,
-                          eps, rankmax, rank_hint, leaves, data, stats));
+                          eps, rankmax, rank_hint, leaves, data, stats, aca_rank));
   return hmat_from_device(ctx, N, N, std::move(leaves), data, storage, keep_leaves, out);
 }

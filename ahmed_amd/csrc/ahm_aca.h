@@ -50,7 +50,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
                     const KernelSpec* Ks, uint32_t nk, const uint32_t* leaf_kernel,
                     double eps, uint32_t rankmax, const uint32_t* kcap_hint,
                     std::vector<ahm_leaf_desc>& leaves, DeviceBuf& data,
-                    ahm_assembly_stats* stats);
+                    ahm_assembly_stats* stats, uint32_t* aca_rank_out = nullptr);
 
 // builds an H-matrix from leaves whose values are already on the device
 // (mblock layout, F64); takes ownership of `raw`

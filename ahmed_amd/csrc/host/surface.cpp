@@ -242,7 +242,7 @@ void matgenGeH_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double ep
     if (!context()) return;
     if (ahm_assemble(g_ctx, G.pts, G.op_perm, G.N, (uint32_t)nl, b1.data(), n1.data(), b2.data(),
                      n2.data(), adm.data(), i0.data(), &k, eps, rankmax, nullptr, AHM_F64, 1,
-                     nullptr, &h) != AHM_OK) {
+                     nullptr, nullptr, &h) != AHM_OK) {
       std::fprintf(stderr, "ahmed: matgenGeH_gpu failed: %s\n", ahm_last_error());
       return;
     }

@@ -145,7 +145,8 @@ int ahm_tree_leaves(ahm_tree t, uint32_t* b1, uint32_t* n1, uint32_t* b2, uint32
 
 extern "C" int ahm_assemble_tree(ahm_ctx ctx, ahm_tree t, const ahm_kernel* kernel, double eps,
                                  uint32_t rankmax, const uint32_t* rank_hint, ahm_dtype storage,
-                                 int keep_leaves, ahm_assembly_stats* stats, ahm_hmat* out) {
+                                 int keep_leaves, ahm_assembly_stats* stats, uint32_t* aca_rank,
+                                 ahm_hmat* out) {
   if (!t) return ahm::fail(AHM_ERR_INVALID, "ahm_assemble_tree: NULL tree");
   const size_t nl = t->leaves.size();
   std::vector<uint32_t> b1(nl), n1(nl), b2(nl), n2(nl);
@@ -156,7 +157,7 @@ extern "C" int ahm_assemble_tree(ahm_ctx ctx, ahm_tree t, const ahm_kernel* kern
     for (int j = 0; j < 3; ++j) pts[3 * (size_t)i + j] = t->pts[i].x[j];
   return ahm_assemble(ctx, pts.data(), t->op.data(), t->N, (uint32_t)nl, b1.data(), n1.data(),
                       b2.data(), n2.data(), adm.data(), t->i0.data(), kernel, eps, rankmax,
-                      rank_hint, storage, keep_leaves, stats, out);
+                      rank_hint, storage, keep_leaves, stats, aca_rank, out);
 }
 
 // n_mat independent H-matrices (own points, tree and kernel) stacked into one

@@ -136,23 +136,28 @@ typedef struct {
   uint64_t n_lowrank, n_dense;
   uint64_t sum_rank;        /* sum of ranks after recompression            */
   uint64_t sum_aca_rank;    /* sum of ACA ranks before recompression       */
+  uint64_t slab_bytes;      /* ACA scratch allocated over all batches/reruns */
+  uint64_t peak_slab_bytes; /* largest single batch                          */
 } ahm_assembly_stats;
 
 /* pts: N x 3 in original order, op_perm from the tree; leaves in idx order
  * with admissibility adm[] and ACA start rows i0[] (relative to b1).
  * rank_hint (may be NULL): per-leaf rank-slab size; ACA keeps going past an
  * under-sized slab (the leaf is rerun with a larger one), so hints change
- * memory and time, never the result.  stats may be NULL. */
+ * memory and time, never the result.  stats may be NULL; aca_rank (may be
+ * NULL) receives each leaf's ACA rank before recompression (0 for leaves
+ * stored dense). */
 int ahm_assemble(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uint32_t N,
                  uint32_t n_leaves, const uint32_t* b1, const uint32_t* n1, const uint32_t* b2,
                  const uint32_t* n2, const uint8_t* adm, const uint32_t* i0,
                  const ahm_kernel* kernel, double eps, uint32_t rankmax,
                  const uint32_t* rank_hint, ahm_dtype storage, int keep_leaves,
-                 ahm_assembly_stats* stats, ahm_hmat* out);
+                 ahm_assembly_stats* stats, uint32_t* aca_rank, ahm_hmat* out);
 /* same, taking the points, permutation and leaves from a tree */
 int ahm_assemble_tree(ahm_ctx ctx, ahm_tree tree, const ahm_kernel* kernel, double eps,
                       uint32_t rankmax, const uint32_t* rank_hint, ahm_dtype storage,
-                      int keep_leaves, ahm_assembly_stats* stats, ahm_hmat* out);
+                      int keep_leaves, ahm_assembly_stats* stats, uint32_t* aca_rank,
+                      ahm_hmat* out);
 
 /* n_mat independent H-matrices (each with its own points, tree and kernel)
  * assembled in one batched launch and stacked block-diagonally: system s
@@ -166,6 +171,16 @@ int ahm_assemble_batch(ahm_ctx ctx, uint32_t n_mat, const ahm_tree* trees,
 /* number of independent systems and their offsets (n_seg + 1 values);
  * a matrix not built by ahm_assemble_batch is one system. */
 int ahm_hmat_segments(ahm_hmat h, uint32_t* n_seg, uint32_t* offsets);
+
+/* Rank-slab sizes for the ACA of each leaf from an 8-64-64-1 MLP (BF16
+ * inference, FP32 accumulation).  feats: n x 8 host floats (tree level,
+ * log n1, log n2, diam_row/diam_col, distance/max diam, log10 eps, kappa,
+ * log10 max diam); params: packed FP32 W1(8x64) b1 W2(64x64) b2 W3(64) b3
+ * mean(8) std(8); hint[l] = clamp(ceil(pad*(exp(pred)-1)) + margin, 1,
+ * maxk[l]) (maxk may be NULL); pred (may be NULL) receives log(1+rank).
+ * Feed hint to ahm_assemble's rank_hint.  No counterpart in the reference. */
+int ahm_rank_predict(ahm_ctx ctx, const float* feats, uint32_t n, const float* params, float pad,
+                     uint32_t margin, const uint32_t* maxk, uint32_t* hint, float* pred);
 
 /* -------- products: mltaGeHVec / mltaGeHGeM (H/mltaGeHVec.cpp:567-577) ----
  * Y += alpha * H * X for p right-hand sides; X is n_cols x p column-major
