@@ -102,6 +102,13 @@ def load():
         if not os.path.exists(LIB_PATH):
             raise AhmError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
                            "(make -C ahmed_amd/csrc)")
+        # torch ships its own HIP runtime; loading it first makes this library
+        # bind to the same runtime instance (same SONAME), so device pointers
+        # and streams from torch are valid here
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(lib, name)
