@@ -108,7 +108,7 @@ def train(X: np.ndarray, Y: np.ndarray, epochs: int = 6, seed: int = 0) -> np.nd
             loss = torch.mean((net(Xt[idx]) - Yt[idx]) ** 2)
             loss.backward()
             opt.step()
-            tot += float(loss) * len(idx)
+            tot += float(loss.detach()) * len(idx)
         print(f"epoch {ep}: mse {tot / n:.4f}", flush=True)
     W1 = net[0].weight.detach().numpy().T    # (NF, NH): row = input
     W2 = net[2].weight.detach().numpy().T

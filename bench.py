@@ -182,12 +182,72 @@ def bench_batched(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_learned_slabs(args):
+    """Batched GPU ACA on N = 1M unit-cube points, kernel exp(-2 r)/r, eps 1e-6,
+    with rank slabs sized three ways: the reference's worst case
+    (n1*n2/(n1+n2)+4, capped at 256), a fixed 48 columns, and the MLP
+    prediction.  ACA continues past under-sized slabs, so the three final
+    matrices are identical; time, reruns and slab memory differ."""
+    import torch
+    from ahmed_amd import _lib, geometry
+    from ahmed_amd.rank_mlp import leaf_features, load_params, predict_hints
+    N, eps, kappa = args.n or 1048576, 1e-6, 2.0
+    pts = geometry.cube_surface(N, 0)
+    tree = _lib.Tree(pts, 64, 0.64)
+    ctx = _lib.Context(0)
+    t0 = time.perf_counter()
+    feats, adm, maxk = leaf_features(pts, tree, eps, kappa)
+    t_feat = time.perf_counter() - t0
+    params = load_params()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hint_adm, _ = predict_hints(ctx, feats, params, maxk=maxk)
+    t_inf = time.perf_counter() - t0
+    log(f"features for {len(adm)} admissible leaves {t_feat:.2f} s (host), MLP inference {t_inf * 1e3:.1f} ms")
+    hints = {"worst_case": np.minimum(maxk, 256), "fixed48": None, "learned": hint_adm}
+    out, ranks = {}, {}
+    for name, h in hints.items():
+        full = None
+        if h is not None:
+            full = np.full(tree.n_leaves, 48, np.uint32)
+            full[adm] = h
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        H, st = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_EXPR, kappa, 6.0 / N, eps, rank_hint=full,
+                                      aca_ranks=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ranks[name] = (st["aca_rank"], H.stats()["nvals"], st["sum_rank"])
+        out[name] = dict(seconds=round(dt, 3), aca_reruns=st["n_retries"], slab_GB=round(st["slab_bytes"] / 1e9, 3),
+                         peak_batch_slab_GB=round(st["peak_slab_bytes"] / 1e9, 3), sum_rank=st["sum_rank"],
+                         nvals=H.stats()["nvals"])
+        log(f"{name}: {dt:.3f} s, reruns {st['n_retries']}, slab {st['slab_bytes'] / 1e9:.2f} GB")
+        H.close()
+    same = all(np.array_equal(ranks[k][0], ranks["fixed48"][0]) and ranks[k][1] == ranks["fixed48"][1]
+               for k in ranks)
+    actual = ranks["fixed48"][0][adm]
+    pred_q = dict(under_predicted=float(np.mean(hint_adm < actual)),
+                  mean_slab_over_rank=float(np.mean(hint_adm / np.maximum(actual, 1))))
+    line = {
+        "metric": "GPU ACA assembly seconds with learned rank slabs", "value": out["learned"]["seconds"],
+        "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": out["learned"]["seconds"] * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded points on the unit-cube surface)",
+        "config": {"workload": f"GPU ACA + recompression, N={N} unit-cube surface, exp(-2r)/r, eps 1e-6, leaf 64, "
+                               "eta 0.8; MLP 8-64-64-1 BF16 rank slabs", "config": "cfg5", "N": N,
+                   "admissible_leaves": int(len(adm))},
+        "variants": out, "identical_matrices": bool(same), "prediction": pred_q,
+        "feature_seconds_host": round(t_feat, 3), "inference_ms": round(t_inf * 1e3, 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", choices=["cfg3", "cfg2", "cfg4"], default="cfg3")
+    ap.add_argument("--config", choices=["cfg3", "cfg2", "cfg4", "cfg5"], default="cfg3")
     ap.add_argument("--n", type=int, default=0, help="override N")
     ap.add_argument("--rhs", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
@@ -203,6 +263,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.config == "cfg4":
         return bench_batched(args)
+    if args.config == "cfg5":
+        return bench_learned_slabs(args)
     if args.config == "cfg2":
         N, eps, tol, geo = args.n or 262144, 1e-8, 1e-10, "unit sphere"
         pts = geometry.sphere(N, 0)
