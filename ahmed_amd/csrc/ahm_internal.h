@@ -38,8 +38,8 @@ struct GemvTask {
   uint64_t data;   // element offset of P in the arena
   uint64_t zoff;   // offset of the task's column-source list
   uint32_t m, W;
-  uint32_t out;    // first output row: t index (phase A), y row (phase B, whole
-                   // panel) or partial-sum slot (phase B, column chunk)
+  uint32_t out;    // first output row: t index (phase A), y row (phase B), or
+                   // partial-sum slot (column chunk of either phase)
   uint32_t partial;  // 1: write the chunk's row sums to the partial buffer
 };
 
@@ -48,7 +48,7 @@ struct GemvTask {
 constexpr uint64_t kChunkValues = 48 * 1024;
 
 struct ReduceTask {
-  uint32_t row;      // first y row
+  uint32_t row;      // first output row (t index in phase A, y row in phase B)
   uint32_t m;        // rows
   uint32_t part;     // first partial slot (chunk c, row r at part + c*m + r)
   uint32_t nchunks;
@@ -99,17 +99,18 @@ struct ahm_hmat_s {
   uint64_t n_dense = 0, n_lowrank = 0;
   uint64_t nv_dense = 0, nv_u = 0, nv_v = 0;
   ahm::DeviceBuf arena;       // nvals values in panel layout
-  ahm::DeviceBuf tasks_a, tasks_b, zsrc, tasks_r;
-  uint32_t n_a = 0, n_b = 0, n_r = 0;
-  uint32_t part_len = 0;      // partial-sum slots per right-hand side
+  ahm::DeviceBuf tasks_a, tasks_b, zsrc, tasks_r, tasks_ra;
+  uint32_t n_a = 0, n_b = 0, n_r = 0, n_ra = 0;
+  uint32_t part_len = 0;      // partial-sum slots per right-hand side (shared by
+                              // the phase-A and phase-B chunk sums, used in turn)
   ahm::DeviceBuf pbuf;        // partial sums: part_len * p values
   size_t pbuf_cap = 0;
-  std::vector<ahm::ReduceTask> h_tasks_r;
+  std::vector<ahm::ReduceTask> h_tasks_r, h_tasks_ra;
   // independent systems stacked block-diagonally (batched assembly):
   // system s owns rows/columns [seg_off[s], seg_off[s+1])
   std::vector<uint32_t> seg_off;
-  // phase-B row panels before chunking, and the chunk size in use
-  std::vector<ahm::GemvTask> h_panels;
+  // V^T panels and row panels before chunking, and the chunk size in use
+  std::vector<ahm::GemvTask> h_panels, h_panels_a;
   uint64_t chunk_values = ahm::kChunkValues;
   uint64_t n_zsrc = 0;
   ahm::DeviceBuf tbuf;        // V^T X workspace: t_len * p values

@@ -13,8 +13,12 @@
 //            block and every U that covers them, so each output row is
 //            written by exactly one workgroup: deterministic, no atomics.
 //
-// Every stored value is read exactly once per product; x and t are gathered
-// into LDS per panel through a per-column source list.
+// Panels of either phase wider than about kChunkValues values are cut into
+// column chunks (one workgroup each) whose row sums meet in a fixed-order
+// reduction, so a 32K-column V^T panel does not leave one CU streaming it
+// alone at the end of the launch.  Every stored value is read exactly once
+// per product; x and t are gathered into LDS per panel through a per-column
+// source list.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -123,6 +127,8 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
         part[(uint64_t)rhs * part_len + tk.out + tid] = s;
       else
         y[(uint64_t)rhs * ldy + tk.out + tid] += alpha * s;
+    } else if (tk.partial) {
+      part[(uint64_t)rhs * part_len + tk.out + tid] = s;
     } else {
       tr[tk.out + tid] = s;
     }
@@ -239,20 +245,19 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
         const uint32_t j = rhs0 + ct * 16 + li;
         if (row >= m || j >= p) continue;
         const T v = acc[q][ct][r];
-        if (PHASE_B) {
-          if (tk.partial)
-            part[(uint64_t)j * part_len + tk.out + row] = v;
-          else
-            y[(uint64_t)j * ldy + tk.out + row] += alpha * v;
-        } else {
+        if (tk.partial)
+          part[(uint64_t)j * part_len + tk.out + row] = v;
+        else if (PHASE_B)
+          y[(uint64_t)j * ldy + tk.out + row] += alpha * v;
+        else
           t[(uint64_t)j * t_len + tk.out + row] = v;
-        }
       }
   }
 }
 
-// y rows += alpha * sum over chunks (chunk order fixed) for chunked panels
-template <typename TV>
+// Sum over chunks (chunk order fixed) for chunked panels: phase B adds
+// alpha * sum to y rows, phase A (ASSIGN) stores the sum into t.
+template <typename TV, bool ASSIGN>
 __global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
     const ReduceTask* __restrict__ tasks, const TV* __restrict__ part, uint32_t part_len,
     TV* __restrict__ y, uint32_t ldy, TV alpha) {
@@ -262,7 +267,10 @@ __global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
   for (uint32_t i = threadIdx.x; i < r.m; i += kPanelThreads) {
     TV s = 0;
     for (uint32_t c = 0; c < r.nchunks; ++c) s += pr[(uint64_t)c * r.m + i];
-    y[(uint64_t)rhs * ldy + r.row + i] += alpha * s;
+    if (ASSIGN)
+      y[(uint64_t)rhs * ldy + r.row + i] = s;
+    else
+      y[(uint64_t)rhs * ldy + r.row + i] += alpha * s;
   }
 }
 
@@ -270,8 +278,9 @@ __global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
 // plan: host-side construction of the tasks and copy jobs
 
 struct Plan {
-  std::vector<GemvTask> a, b, panels;  // panels: phase-B row panels before chunking
-  std::vector<ReduceTask> r;
+  // a_panels / panels: V^T panels and row panels before chunking
+  std::vector<GemvTask> a, b, a_panels, panels;
+  std::vector<ReduceTask> r, ra;
   uint32_t part_len = 0;
   std::vector<uint32_t> zsrc;
   std::vector<CopyJob> jobs;
@@ -281,7 +290,7 @@ struct Plan {
   uint64_t nv_dense = 0, nv_u = 0, nv_v = 0;
 };
 
-// Phase-B tasks from the row panels: panels wider than about chunk_values
+// Tasks from the panels of one phase: panels wider than about chunk_values
 // values are cut into equal column chunks whose row sums meet in a reduction
 // task; tasks sorted largest first.
 static void chunk_panels(const std::vector<GemvTask>& panels, uint64_t chunk_values,
@@ -376,7 +385,7 @@ static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_l
         }
         rowbase += k;
       }
-      P.a.push_back(tk);
+      P.a_panels.push_back(tk);
       arena += (uint64_t)m * n2;
     }
     t_len += K;
@@ -447,12 +456,13 @@ static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_l
       arena += (uint64_t)m * W;
     }
   }
-  // biggest tasks first: the hardware dispatches workgroups in order
-  auto big = [](const GemvTask& a, const GemvTask& b) {
-    return (uint64_t)a.m * a.W > (uint64_t)b.m * b.W;
-  };
-  std::stable_sort(P.a.begin(), P.a.end(), big);
-  chunk_panels(P.panels, kChunkValues, P.b, P.r, P.part_len);
+  // wide panels of both phases cut into column chunks, biggest tasks first
+  // (the hardware dispatches workgroups in order); the two phases use the
+  // partial-sum slots in turn
+  uint32_t part_a = 0, part_b = 0;
+  chunk_panels(P.a_panels, kChunkValues, P.a, P.ra, part_a);
+  chunk_panels(P.panels, kChunkValues, P.b, P.r, part_b);
+  P.part_len = std::max(part_a, part_b);
   P.nvals = arena;
   P.t_len = (uint32_t)t_len;
   for (uint32_t l = 0; l < nl; ++l) {
@@ -505,6 +515,9 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
       hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, false>), dim3(h->n_a, 1), dim3(kPanelThreads), 0, s,
                          (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
                          (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
+    if (h->n_ra)
+      hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, 1), dim3(kPanelThreads), 0, s,
+                         (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha);
     if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
     if (h->n_b)
       hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, true>), dim3(h->n_b, 1), dim3(kPanelThreads), 0, s,
@@ -517,6 +530,9 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
       hipLaunchKernelGGL((panel_mm_kernel<TV, false>), dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,
                          (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
                          (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha, p);
+    if (h->n_ra)
+      hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, p), dim3(kPanelThreads), 0, s,
+                         (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha);
     if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
     if (h->n_b)
       hipLaunchKernelGGL((panel_mm_kernel<TV, true>), dim3(h->n_b, ng), dim3(kPanelThreads), 0, s,
@@ -524,7 +540,7 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
                          (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha, p);
   }
   if (h->n_r)
-    hipLaunchKernelGGL((chunk_reduce_kernel<TV>), dim3(h->n_r, p), dim3(kPanelThreads), 0, s,
+    hipLaunchKernelGGL((chunk_reduce_kernel<TV, false>), dim3(h->n_r, p), dim3(kPanelThreads), 0, s,
                        (const ReduceTask*)h->tasks_r.p, (const TV*)part, h->part_len, Y, ldy, alpha);
   AHM_HIP(hipGetLastError());
   return AHM_OK;
@@ -534,23 +550,30 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
 
 namespace ahm {
 
-// re-cut the phase-B row panels into chunks of about cv values (the arena is
-// unchanged: chunks are column ranges of the same panels)
+// re-cut the panels of both phases into chunks of about cv values (the arena
+// is unchanged: chunks are column ranges of the same panels)
 int set_chunking(ahm_hmat h, uint64_t cv) {
-  std::vector<GemvTask> b;
-  std::vector<ReduceTask> r;
-  uint32_t part_len = 0;
-  chunk_panels(h->h_panels, cv, b, r, part_len);
+  std::vector<GemvTask> a, b;
+  std::vector<ReduceTask> ra, r;
+  uint32_t part_a = 0, part_b = 0;
+  chunk_panels(h->h_panels_a, cv, a, ra, part_a);
+  chunk_panels(h->h_panels, cv, b, r, part_b);
   hipStream_t s = h->ctx->stream;
   AHM_HIP(hipStreamSynchronize(s));
+  AHM_TRY(upload_vec(h->tasks_a, a, s));
+  AHM_TRY(upload_vec(h->tasks_ra, ra, s));
   AHM_TRY(upload_vec(h->tasks_b, b, s));
   AHM_TRY(upload_vec(h->tasks_r, r, s));
   AHM_HIP(hipStreamSynchronize(s));
+  h->n_a = (uint32_t)a.size();
+  h->n_ra = (uint32_t)ra.size();
   h->n_b = (uint32_t)b.size();
   h->n_r = (uint32_t)r.size();
-  h->part_len = part_len;
+  h->part_len = std::max(part_a, part_b);
   h->pbuf_cap = 0;  // partial-sum slots change with the chunking
   h->pbuf.release();
+  h->h_tasks_a = std::move(a);
+  h->h_tasks_ra = std::move(ra);
   h->h_tasks_b = std::move(b);
   h->h_tasks_r = std::move(r);
   h->chunk_values = cv;
@@ -593,9 +616,11 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
     if ((st = upload_vec(h->tasks_b, P.b, s))) break;
     if ((st = upload_vec(h->zsrc, P.zsrc, s))) break;
     if ((st = upload_vec(h->tasks_r, P.r, s))) break;
+    if ((st = upload_vec(h->tasks_ra, P.ra, s))) break;
     h->n_a = (uint32_t)P.a.size();
     h->n_b = (uint32_t)P.b.size();
     h->n_r = (uint32_t)P.r.size();
+    h->n_ra = (uint32_t)P.ra.size();
     h->part_len = P.part_len;
     h->n_zsrc = P.zsrc.size();
     for (size_t j0 = 0; j0 < P.jobs.size(); j0 += (1u << 30)) {
@@ -618,7 +643,9 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
     h->h_tasks_a = std::move(P.a);
     h->h_tasks_b = std::move(P.b);
     h->h_tasks_r = std::move(P.r);
+    h->h_tasks_ra = std::move(P.ra);
     h->h_panels = std::move(P.panels);
+    h->h_panels_a = std::move(P.a_panels);
     h->chunk_values = kChunkValues;
     h->h_zsrc = std::move(P.zsrc);
     if (keep_leaves) {
@@ -698,7 +725,9 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
   h->h_tasks_a = src->h_tasks_a;
   h->h_tasks_b = src->h_tasks_b;
   h->h_tasks_r = src->h_tasks_r;
+  h->h_tasks_ra = src->h_tasks_ra;
   h->h_panels = src->h_panels;
+  h->h_panels_a = src->h_panels_a;
   h->chunk_values = src->chunk_values;
   h->seg_off = src->seg_off;
   h->h_zsrc = src->h_zsrc;
@@ -710,9 +739,11 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
     if ((st = upload_vec(h->tasks_b, h->h_tasks_b, s))) break;
     if ((st = upload_vec(h->zsrc, h->h_zsrc, s))) break;
     if ((st = upload_vec(h->tasks_r, h->h_tasks_r, s))) break;
+    if ((st = upload_vec(h->tasks_ra, h->h_tasks_ra, s))) break;
     h->n_a = src->n_a;
     h->n_b = src->n_b;
     h->n_r = src->n_r;
+    h->n_ra = src->n_ra;
     h->n_zsrc = src->n_zsrc;
     if (h->nvals)
       hipLaunchKernelGGL((cast_kernel<float, double>), dim3(4096), dim3(256), 0, s,

@@ -251,6 +251,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override N")
     ap.add_argument("--rhs", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="end with a plain 2 GiB read+write kernel (scale for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -350,6 +352,14 @@ def main():
         log(f"GMRES(50) {mode}: {dt:.3f} s, residual {true:.2e}, {info}")
     gm["tolerance"] = tol
     gm["speedup_mixed_vs_fp64"] = round(gm["fp64"]["seconds"] / gm["mixed"]["seconds"], 3)
+
+    if args.copy_probe:
+        a = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+        c = torch.empty_like(a)
+        torch.neg(a, out=c)
+        torch.cuda.synchronize()
+        log("copy probe: neg over 2 GiB (reads 2 GiB, writes 2 GiB)")
+        del a, c
 
     cpu = None
     if not args.no_cpu:
