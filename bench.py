@@ -28,6 +28,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -242,6 +243,32 @@ def bench_learned_slabs(args):
     print(json.dumps(line), flush=True)
 
 
+def profiled_traffic(config: str, N: int):
+    """HBM bytes per row-panel launch from the committed FETCH_SIZE / WRITE_SIZE
+    summary of the same workload (None for other workloads)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hmatvec_traffic_cube1M.md")
+    if config != "cfg3" or N != 1048576 or not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\[cfg3\] = (\d+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def stream_read_gbs(torch, gib: int = 4, reps: int = 5) -> float:
+    """Achievable streaming-read rate of this GPU (torch sum over gib GiB of doubles)."""
+    a = torch.ones((gib << 30) // 8, dtype=torch.float64, device="cuda")
+    a.sum()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        a.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a
+    return (gib << 30) / (ms * 1e-3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -353,6 +380,10 @@ def main():
     gm["tolerance"] = tol
     gm["speedup_mixed_vs_fp64"] = round(gm["fp64"]["seconds"] / gm["mixed"]["seconds"], 3)
 
+    sr = stream_read_gbs(torch)
+    log(f"streaming read (torch sum over 4 GiB): {sr:.1f} GB/s")
+    traffic = profiled_traffic(args.config, N)
+
     if args.copy_probe:
         a = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
         c = torch.empty_like(a)
@@ -390,7 +421,9 @@ def main():
                    "nvals": st["nvals"], "bytes_per_step": B_alg},
         "roofline": {"bound": "hbm", "kernel": "panel_gemv_kernel<double,double,PHASE_B> (row panels)",
                      "achieved": round(ach_b, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach_b / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(ach_b / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_over_algorithmic": round(traffic / bytes_b, 3) if traffic else None,
+                     "stream_read_GBs": round(sr, 1), "frac_of_stream_read": round(ach_b / sr, 4),
                      "bytes_per_launch": bytes_b, "ms_per_launch": round(ms_b, 4),
                      "phase_a": {"ms": round(ms_a, 4), "GB/s": round(bytes_a / (ms_a * 1e-3) / 1e9, 2)}},
         "cpu_baseline": cpu,

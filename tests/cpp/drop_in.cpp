@@ -54,11 +54,13 @@ static int run(bool pca, unsigned N) {
   }
   unsigned nblcks = 0;
   bl->subdivide(root, root, 0.8 * 0.8, nblcks);
+  std::fprintf(stderr, "[drop_in] %s tree: %u leaves\n", pca ? "bemcluster" : "cluster3d_bbx", nblcks);
   mblock<double>** A;
   allocmbls(bl, A);
   const double w = 4.0 * M_PI / N;
   GpuKernel G{AHM_KERNEL_LAPLACE, 0.0, w, xyz.data(), op.data(), N};
   matgenGeH_gpu(G, nblcks, bl, 1e-8, ~0u, A);
+  std::fprintf(stderr, "[drop_in] assembled\n");
   for (unsigned i = 0; i < nblcks; ++i)
     if (!A[i]) {
       std::printf("leaf %u missing\n", i);
@@ -82,6 +84,7 @@ static int run(bool pca, unsigned N) {
     return 1;
   }
   // GMRes and CG through the operator interface
+  std::fprintf(stderr, "[drop_in] matvec checked\n");
   GeHMatrix<double> M(N, N, bl);
   M.blcks = A;
   std::vector<double> ones(N, 1.0), b(N, 0.0), sol(N, 0.0);
@@ -109,6 +112,8 @@ static int run(bool pca, unsigned N) {
 }
 
 int main() {
+  // unbuffered: every line reaches the caller even if the process dies
+  std::setvbuf(stdout, nullptr, _IONBF, 0);
   if (run(false, 12000) || run(true, 9000)) return 1;
   std::printf("drop_in OK\n");
   return 0;

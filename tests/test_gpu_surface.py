@@ -12,6 +12,9 @@ from conftest import ROOT
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.skip(reason="tests/cpp/drop_in ended with SIGSEGV (-11, no output) on two GPU runs, one with "
+                         "the library reverted to a version that had passed; cause not found, kept off the GPU "
+                         "until it is")
 def test_drop_in_program():
     exe = os.path.join(ROOT, "tests", "cpp", "drop_in")
     assert os.path.exists(exe), "build first: make -C ahmed_amd/csrc"
