@@ -113,8 +113,10 @@ struct ahm_hmat_s {
   std::vector<ahm::GemvTask> h_panels, h_panels_a;
   uint64_t chunk_values = ahm::kChunkValues;
   uint64_t n_zsrc = 0;
-  ahm::DeviceBuf tbuf;        // V^T X workspace: t_len * p values
+  ahm::DeviceBuf tbuf;        // V^T X workspace: t_len values per right-hand side slot
   size_t tbuf_cap = 0;        // bytes
+  ahm::DeviceBuf xbuf;        // interleaved copy of X for multi-RHS products
+  size_t xbuf_cap = 0;        // bytes
   ahm::DeviceBuf leafdata;    // optional mblock-layout copy (keep_leaves)
   std::vector<ahm_leaf_desc> leaves;
   // host copies of the plan (used by ahm_hmat_convert)

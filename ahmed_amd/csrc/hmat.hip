@@ -142,8 +142,9 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
 // right-hand sides per blockIdx.y.  The output is tiled 16 x 16; wavefront w
 // owns the row tiles w, w+4, w+8, w+12 and both 16-column tiles of its RHS
 // group, so every panel value is loaded once per 32 right-hand sides.  Z is
-// gathered into LDS in chunks of kMmChunk columns; the panel loads of four
-// k-steps are issued before their MFMAs.  Per k-step of 4:
+// gathered into LDS in chunks of kMmChunk columns from interleaved copies of
+// x and t (the right-hand sides of one source row contiguous); the panel
+// loads of four k-steps are issued before their MFMAs.  Per k-step of 4:
 //   A (16 x 4 of P):  lane l holds P[row0 + l%16][k0 + l/16]
 //   B (4 x 16 of Z):  lane l holds Z[k0 + l/16][col0 + l%16]
 //   C (16 x 16):      f64: lane l, value r -> (l/16 + 4r, l%16)
@@ -174,17 +175,20 @@ template <> struct MmAcc<float> {
 template <typename T, bool PHASE_B>
 __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
     const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
-    const T* __restrict__ arena, const T* __restrict__ x, uint32_t ldx, T* __restrict__ t,
-    uint32_t t_len, T* __restrict__ y, uint32_t ldy, T* __restrict__ part, uint32_t part_len,
-    T alpha, uint32_t p) {
+    const T* __restrict__ arena, const T* __restrict__ xi, uint32_t n_x, T* __restrict__ ti,
+    uint32_t t_len, int pw, T* __restrict__ y, uint32_t ldy, T* __restrict__ part,
+    uint32_t part_len, T alpha, uint32_t p) {
   typedef typename MmAcc<T>::type acc_t;
   __shared__ T zs[kMmChunk][kMmRhs + 1];
   const GemvTask tk = tasks[blockIdx.x];
-  const uint32_t rhs0 = blockIdx.y * kMmRhs;
+  const uint32_t g = blockIdx.y, rhs0 = g * kMmRhs;
+  const T* xg = xi + (uint64_t)g * n_x * pw;
+  T* tg = ti + (uint64_t)g * t_len * pw;
   const int m = (int)tk.m;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nrt = (m + 15) / 16;
-  const bool two = p - rhs0 > 16;  // second 16-column tile needed
+  const bool two = pw > 16 && p - rhs0 > 16;  // second 16-column tile needed
+  const int lpw = pw > 16 ? 5 : 4;
   acc_t acc[4][2];
   for (int a = 0; a < 4; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
@@ -193,14 +197,13 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
   for (uint32_t c0 = 0; c0 < tk.W; c0 += kMmChunk) {
     const int cw = (int)min((uint32_t)kMmChunk, tk.W - c0);
     __syncthreads();
-    // gather: consecutive threads take consecutive columns of one RHS
-    for (int e = threadIdx.x; e < kMmChunk * kMmRhs; e += kPanelThreads) {
-      const int j = e / kMmChunk, c = e % kMmChunk;
+    // gather: pw consecutive threads read the contiguous run of one source row
+    for (int e = threadIdx.x; e < (kMmChunk << lpw); e += kPanelThreads) {
+      const int c = e >> lpw, j = e & (pw - 1);
       T v = 0;
-      if (c < cw && rhs0 + j < p) {
+      if (c < cw) {
         const uint32_t s = zsrc[tk.zoff + c0 + c];
-        v = (s & kFromT) ? t[(uint64_t)(rhs0 + j) * t_len + (s & ~kFromT)]
-                         : x[(uint64_t)(rhs0 + j) * ldx + s];
+        v = (s & kFromT) ? tg[(uint64_t)(s & ~kFromT) * pw + j] : xg[(uint64_t)s * pw + j];
       }
       zs[c][j] = v;
     }
@@ -242,7 +245,8 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
       for (int r = 0; r < 4; ++r) {
         if (ct == 1 && !two) continue;
         const int row = rt * 16 + MmAcc<T>::row(lane, r);
-        const uint32_t j = rhs0 + ct * 16 + li;
+        const int jj = ct * 16 + li;
+        const uint32_t j = rhs0 + jj;
         if (row >= m || j >= p) continue;
         const T v = acc[q][ct][r];
         if (tk.partial)
@@ -250,27 +254,55 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
         else if (PHASE_B)
           y[(uint64_t)j * ldy + tk.out + row] += alpha * v;
         else
-          t[(uint64_t)j * t_len + tk.out + row] = v;
+          tg[(uint64_t)(tk.out + row) * pw + jj] = v;
       }
+  }
+}
+
+// X (n x p, right-hand sides ldx apart) -> Xi: groups of 32 right-hand sides,
+// row i of group g holding its pw values contiguously at (g*n + i)*pw
+// (zero beyond p).  64 rows x 32 right-hand sides per block through LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void interleave_kernel(const T* __restrict__ X, uint32_t ldx,
+                                                         uint32_t n, uint32_t p, int pw,
+                                                         T* __restrict__ Xi) {
+  __shared__ T tile[32][65];
+  const uint32_t i0 = blockIdx.x * 64u, g = blockIdx.y;
+  for (int e = threadIdx.x; e < 32 * 64; e += 256) {
+    const int jj = e >> 6, ii = e & 63;
+    const uint32_t rhs = g * 32u + jj, i = i0 + ii;
+    tile[jj][ii] = (jj < pw && rhs < p && i < n) ? X[(uint64_t)rhs * ldx + i] : (T)0;
+  }
+  __syncthreads();
+  const int lpw = pw > 16 ? 5 : 4;
+  for (int e = threadIdx.x; e < (64 << lpw); e += 256) {
+    const int ii = e >> lpw, jj = e & (pw - 1);
+    const uint32_t i = i0 + ii;
+    if (i < n) Xi[((uint64_t)g * n + i) * pw + jj] = tile[jj][ii];
   }
 }
 
 // Sum over chunks (chunk order fixed) for chunked panels: phase B adds
 // alpha * sum to y rows, phase A (ASSIGN) stores the sum into t.
+// pw > 0: y is interleaved (element (row, rhs) at ((rhs/32)*ldy + row)*pw +
+// rhs%32, the layout of t in multi-RHS products), else y[rhs*ldy + row].
 template <typename TV, bool ASSIGN>
 __global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
     const ReduceTask* __restrict__ tasks, const TV* __restrict__ part, uint32_t part_len,
-    TV* __restrict__ y, uint32_t ldy, TV alpha) {
+    TV* __restrict__ y, uint32_t ldy, TV alpha, int pw) {
   const ReduceTask r = tasks[blockIdx.x];
   const uint32_t rhs = blockIdx.y;
   const TV* pr = part + (uint64_t)rhs * part_len + r.part;
+  const uint64_t base = pw ? (uint64_t)(rhs >> 5) * ldy * pw + (rhs & 31) : (uint64_t)rhs * ldy;
+  const uint64_t stride = pw ? (uint64_t)pw : 1;
   for (uint32_t i = threadIdx.x; i < r.m; i += kPanelThreads) {
     TV s = 0;
     for (uint32_t c = 0; c < r.nchunks; ++c) s += pr[(uint64_t)c * r.m + i];
+    TV* o = y + base + (r.row + i) * stride;
     if (ASSIGN)
-      y[(uint64_t)rhs * ldy + r.row + i] = s;
+      *o = s;
     else
-      y[(uint64_t)rhs * ldy + r.row + i] += alpha * s;
+      *o += alpha * s;
   }
 }
 
@@ -486,12 +518,26 @@ static int upload_vec(DeviceBuf& b, const std::vector<T>& v, hipStream_t s) {
   return AHM_OK;
 }
 
+// multi-RHS products keep x and t interleaved in groups of 32 right-hand
+// sides (16 when p <= 16): rhs_width values per row and group
+static uint32_t rhs_width(uint32_t p) { return p <= 16 ? 16u : (uint32_t)kMmRhs; }
+static uint32_t rhs_slots(uint32_t p) {
+  return p == 1 ? 1u : ((p + kMmRhs - 1) / kMmRhs) * rhs_width(p);
+}
+
 static int ensure_tbuf(ahm_hmat h, uint32_t p, size_t esize) {
-  const size_t need = (size_t)h->t_len * p * esize;
-  if (need <= h->tbuf_cap) return AHM_OK;
-  AHM_HIP(hipStreamSynchronize(h->ctx->stream));
-  AHM_TRY(h->tbuf.alloc(need));
-  h->tbuf_cap = need;
+  const size_t need = (size_t)h->t_len * rhs_slots(p) * esize;
+  if (need > h->tbuf_cap) {
+    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+    AHM_TRY(h->tbuf.alloc(need));
+    h->tbuf_cap = need;
+  }
+  const size_t xneed = p == 1 ? 0 : (size_t)h->n_cols * rhs_slots(p) * esize;
+  if (xneed > h->xbuf_cap) {
+    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+    AHM_TRY(h->xbuf.alloc(xneed));
+    h->xbuf_cap = xneed;
+  }
   return AHM_OK;
 }
 
@@ -517,31 +563,39 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
                          (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
     if (h->n_ra)
       hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, 1), dim3(kPanelThreads), 0, s,
-                         (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha);
+                         (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha, 0);
     if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
     if (h->n_b)
       hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, true>), dim3(h->n_b, 1), dim3(kPanelThreads), 0, s,
                          (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
                          (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
   } else {
-    // p >= 2: MFMA panel products, 32 right-hand sides per workgroup
+    // p >= 2: MFMA panel products, 32 right-hand sides per workgroup, on
+    // interleaved copies of x and t
     const uint32_t ng = (p + kMmRhs - 1) / kMmRhs;
+    const int pw = (int)rhs_width(p);
+    TV* xi = (TV*)h->xbuf.p;
+    hipLaunchKernelGGL((interleave_kernel<TV>), dim3((h->n_cols + 63) / 64, ng), dim3(256), 0, s, X,
+                       ldx, h->n_cols, p, pw, xi);
     if (h->n_a)
       hipLaunchKernelGGL((panel_mm_kernel<TV, false>), dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,
                          (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
-                         (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha, p);
+                         (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, pw, Y, ldy, part,
+                         h->part_len, alpha, p);
     if (h->n_ra)
       hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, p), dim3(kPanelThreads), 0, s,
-                         (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha);
+                         (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha,
+                         pw);
     if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
     if (h->n_b)
       hipLaunchKernelGGL((panel_mm_kernel<TV, true>), dim3(h->n_b, ng), dim3(kPanelThreads), 0, s,
                          (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
-                         (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha, p);
+                         (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, pw, Y, ldy, part,
+                         h->part_len, alpha, p);
   }
   if (h->n_r)
     hipLaunchKernelGGL((chunk_reduce_kernel<TV, false>), dim3(h->n_r, p), dim3(kPanelThreads), 0, s,
-                       (const ReduceTask*)h->tasks_r.p, (const TV*)part, h->part_len, Y, ldy, alpha);
+                       (const ReduceTask*)h->tasks_r.p, (const TV*)part, h->part_len, Y, ldy, alpha, 0);
   AHM_HIP(hipGetLastError());
   return AHM_OK;
 }

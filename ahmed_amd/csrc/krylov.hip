@@ -66,6 +66,16 @@ __global__ __launch_bounds__(kRedThreads) void dot_kernel(const double* __restri
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// partials of sum |a_i| (zero exactly when every a_i is zero)
+__global__ __launch_bounds__(kRedThreads) void asum_kernel(const double* __restrict__ a, uint32_t n,
+                                                           double* __restrict__ part) {
+  double s = 0.0;
+  for (uint32_t i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += kRedBlocks * kRedThreads)
+    s += fabs(a[i]);
+  s = group_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 // h = sum(part_in); out[0] = h (block 0); w -= h v; part_out = <w, next> (or <w,w>)
 __global__ __launch_bounds__(kRedThreads) void mgs_kernel(double* __restrict__ w,
                                                           const double* __restrict__ v,
@@ -199,6 +209,22 @@ int norm2(hipStream_t s, Red& R, const double* v, uint32_t n, double* out) {
   return AHM_OK;
 }
 
+// r = b - A x.  When x is zero on entry (every element exactly 0) the product
+// is skipped: A*0 contributes exact zeros, so r = b bit for bit.
+int initial_residual(hipStream_t s, Red& R, Op& A, const double* b, const double* x, double* r,
+                     uint32_t n, uint32_t& mv) {
+  hipLaunchKernelGGL(asum_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, x, n, R.p0());
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kRedThreads), 0, s, R.p0(), R.s());
+  double ax = 0.0;
+  AHM_HIP(hipMemcpyAsync(&ax, R.s(), sizeof(double), hipMemcpyDeviceToHost, s));
+  AHM_HIP(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  if (ax == 0.0) return AHM_OK;
+  AHM_TRY(A.amux(-1.0, x, r));
+  ++mv;
+  return AHM_OK;
+}
+
 int dot(hipStream_t s, Red& R, const double* a, const double* b, uint32_t n, double* out) {
   hipLaunchKernelGGL(dot_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, a, b, n, R.p0());
   hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kRedThreads), 0, s, R.p0(), R.s());
@@ -274,9 +300,7 @@ int gmres_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m
     return AHM_OK;
   }
   // r = b - A x
-  AHM_HIP(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-  AHM_TRY(A.amux(-1.0, x, r));
-  ++mv;
+  AHM_TRY(initial_residual(s, R, A, b, x, r, n, mv));
   double beta, resid;
   AHM_TRY(norm2(s, R, r, n, &beta));
   if ((resid = beta / normb) <= *eps) {
@@ -368,8 +392,8 @@ int cg_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nst
     *nsteps = 0;
     return AHM_OK;
   }
-  AHM_HIP(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-  AHM_TRY(A.amux(-1.0, x, r));
+  uint32_t mv0 = 0;
+  AHM_TRY(initial_residual(s, R, A, b, x, r, n, mv0));
   double resid;
   AHM_TRY(norm2(s, R, r, n, &resid));
   if (resid <= *eps * nrmb) {
@@ -463,9 +487,16 @@ int ahm_gmres_ir(ahm_hmat hi, ahm_hmat lo, const double* b, double* x, double* e
     return AHM_OK;
   }
   int st = AHM_ERR_NOT_CONVERGED;
+  Op Ahi{hi, n};
   for (;;) {
-    AHM_HIP(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-    AHM_TRY(ahm_hmat_matvec(hi, -1.0, x, n, r, n, 1, AHM_F64));
+    // FP64 residual r = b - A x (skipped product when x starts at zero)
+    uint32_t mvh = 0;
+    if (it == 0) {
+      AHM_TRY(initial_residual(s, R, Ahi, b, x, r, n, mvh));
+    } else {
+      AHM_HIP(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+      AHM_TRY(ahm_hmat_matvec(hi, -1.0, x, n, r, n, 1, AHM_F64));
+    }
     double nr;
     AHM_TRY(norm2(s, R, r, n, &nr));
     resid = nr / normb;

@@ -61,9 +61,10 @@ def timed(fn, torch, steps: int, warmup: int):
     return e0.elapsed_time(e1) / steps  # ms per step
 
 
-def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, budget_s=20.0):
+def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, budget_s=20.0, gmres=None):
     """The reference's compiled mltaGeHVec (oracle/_ref) on this host's cores,
-    on the identical leaves; falls back to the oracle restatement."""
+    on the identical leaves; falls back to the oracle restatement.  gmres =
+    (b, tol, m): also time one serial GMRes solve of H x = b (x0 = 0)."""
     import oracle as O
     x = np.random.default_rng(7).standard_normal(len(pts))
     out = {}
@@ -92,6 +93,14 @@ def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, budget_s=20.0):
     out.update(value=round(B_alg / t / 1e9, 3), unit="GB/s", cores=1, kind=kind,
                sample=f"{len(ts)} serial mltaGeHVec calls (median {t:.3f} s each) on the identical "
                       f"H-matrix, N={len(pts)}")
+    if gmres is not None:
+        b, tol, m = gmres
+        t0 = time.perf_counter()
+        xs, rc, res, steps = H.gmres(b, eps=tol, m=m, nsteps=2000)
+        tg = time.perf_counter() - t0
+        out["gmres"] = {"seconds": round(tg, 3), "rc": rc, "residual": res, "iterations": steps, "cores": 1,
+                        "kind": kind, "sample": f"one serial GMRes({m}) solve to {tol:g} on the identical H-matrix"}
+        log(f"CPU GMRes({m}): {tg:.2f} s, {steps} iterations, residual {res:.2e}")
     # all-core leaf-parallel restatement (per-thread y), for reference
     try:
         o = O.Oracle()
@@ -396,7 +405,10 @@ def main():
     if not args.no_cpu:
         leaves, data = H.download()
         log(f"CPU baseline on {len(leaves)} leaves ({data.nbytes / 1e9:.1f} GB)")
-        cpu = cpu_baseline(pts, leaves, data, bmin, eta * eta, B_alg)
+        cpu = cpu_baseline(pts, leaves, data, bmin, eta * eta, B_alg, gmres=(b.cpu().numpy(), tol, 50))
+        if "gmres" in cpu:
+            gm["speedup_mixed_vs_cpu_serial"] = round(cpu["gmres"]["seconds"] / gm["mixed"]["seconds"], 1)
+            gm["speedup_fp64_vs_cpu_serial"] = round(cpu["gmres"]["seconds"] / gm["fp64"]["seconds"], 1)
         log(f"CPU: {cpu}")
         del data
 
