@@ -292,7 +292,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "sharding of the H-matvec is a design in DESIGN.md, not a measured path.", file=sys.stderr)
+              "sharding of the H-matvec (row-cluster intervals, all-gather of x) is designed, not built.", file=sys.stderr)
         sys.exit(2)
 
     import torch
