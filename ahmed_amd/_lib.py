@@ -75,6 +75,10 @@ SIGNATURES = {
     "ahm_hmat_destroy": (_i, [_p]),
     "ahm_hmat_stats_get": (_i, [_p, C.POINTER(HMatStats)]),
     "ahm_hmat_download": (_i, [_p, _p, _p]),
+    "ahm_mbls_save": (_i, [C.c_char_p, _d, _u32, _u32, _p, _p, _i]),
+    "ahm_mbls_info": (_i, [C.c_char_p, _i, C.POINTER(_d), C.POINTER(_u32), C.POINTER(_u32),
+                           C.POINTER(C.c_uint64)]),
+    "ahm_mbls_read": (_i, [C.c_char_p, _i, _u32, _p, _p]),
     "ahm_assemble": (_i, [_p, _p, _p, _u32, _u32, _p, _p, _p, _p, _p, _p, C.POINTER(Kernel), _d, _u32, _p,
                           _i, _i, C.POINTER(AssemblyStats), _p, C.POINTER(_p)]),
     "ahm_assemble_tree": (_i, [_p, _p, C.POINTER(Kernel), _d, _u32, _p, _i, _i, C.POINTER(AssemblyStats),
@@ -160,6 +164,29 @@ class Context:
 
 
 TREE_BBX, TREE_PCA = 0, 1
+
+
+def mbls_save(path, eta: float, bmin: int, leaves: np.ndarray, data: np.ndarray) -> None:
+    """Write leaves (LEAF_DTYPE, idx order) and their mblock-layout values in
+    the reference's savembls format (Include/H.h:90-103)."""
+    leaves = np.ascontiguousarray(leaves, LEAF_DTYPE)
+    dt = F64 if data.dtype == np.float64 else F32
+    data = np.ascontiguousarray(data)
+    check(load().ahm_mbls_save(os.fsencode(path), eta, bmin, len(leaves), leaves.ctypes.data_as(_p),
+                               data.ctypes.data_as(_p), dt), "ahm_mbls_save")
+
+
+def mbls_read(path, dtype=np.float64):
+    """Read a savembls file: (eta, bmin, leaves with n1/n2/rank/offset, values)."""
+    dt = F64 if np.dtype(dtype) == np.float64 else F32
+    eta, bmin, nl, nv = _d(), _u32(), _u32(), C.c_uint64()
+    check(load().ahm_mbls_info(os.fsencode(path), dt, C.byref(eta), C.byref(bmin), C.byref(nl),
+                               C.byref(nv)), "ahm_mbls_info")
+    leaves = np.zeros(nl.value, LEAF_DTYPE)
+    data = np.empty(nv.value, dtype)
+    check(load().ahm_mbls_read(os.fsencode(path), dt, nl.value, leaves.ctypes.data_as(_p),
+                               data.ctypes.data_as(_p)), "ahm_mbls_read")
+    return eta.value, bmin.value, leaves, data
 
 
 class Tree:
@@ -298,6 +325,27 @@ class HMatrix:
         st = HMatStats()
         check(load().ahm_hmat_stats_get(self.h, C.byref(st)), "ahm_hmat_stats_get")
         return {f: getattr(st, f) for f, _ in HMatStats._fields_ if f != "pad"}
+
+    def save(self, path, eta: float, bmin: int) -> None:
+        """savembls of this matrix (needs keep_leaves): the reference's format."""
+        leaves, data = self.download()
+        mbls_save(path, eta, bmin, leaves, data)
+
+    @classmethod
+    def load(cls, ctx: Context, path, pts, kind: int = TREE_BBX, storage: int = F64,
+             keep_leaves: bool = False):
+        """loadmbls (Include/H.h:60-75) onto the block cluster tree rebuilt from
+        pts with the file's bmin and eta; returns (HMatrix, Tree)."""
+        eta, bmin, leaves, data = mbls_read(path)
+        tree = Tree(pts, bmin, eta * eta, kind)
+        lv = tree.leaves()
+        if tree.n_leaves != len(leaves) or not (np.array_equal(lv["n1"], leaves["n1"])
+                                                and np.array_equal(lv["n2"], leaves["n2"])):
+            tree.close()
+            raise AhmError(f"{path}: leaves do not match the tree of these points "
+                           f"(bmin {bmin}, eta {eta})")
+        leaves["b1"], leaves["b2"] = lv["b1"], lv["b2"]
+        return cls.upload(ctx, tree.N, tree.N, leaves, data, storage, keep_leaves), tree
 
     def download(self):
         st = self.stats()

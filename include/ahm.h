@@ -114,6 +114,28 @@ int ahm_hmat_stats_get(ahm_hmat h, ahm_hmat_stats* st);
  * query descriptors only. */
 int ahm_hmat_download(ahm_hmat h, ahm_leaf_desc* leaves, void* data);
 
+/* -------- on-disk H-matrix: the reference's savembls / loadmbls format -----
+ * (Include/H.h:47-103; mblock::save/load Include/mblock.h:737-804), so files
+ * written by either side load in the other.  Layout, native byte order:
+ *   double eta, uint32 bmin, uint32 n_leaves, then per leaf in idx order a
+ *   one-byte presence flag (1) followed by uint32 n1, n2, status; status 0
+ *   (low rank): uint32 rank, rank*(n1+n2) values (U then V, column-major);
+ *   status 4 (dense): n1*n2 values column-major.  Values are double for
+ *   AHM_F64 files (mblock<double>), float for AHM_F32.
+ * The file holds no geometry: b1/b2 of the leaves come from the block cluster
+ * tree rebuilt from the points with the stored bmin and eta.  The reference's
+ * Hermitian / triangular / symmetric dense leaves (status 1, 2, 3, 5) are
+ * outside the GeH path and reported as AHM_ERR_UNSUPPORTED. */
+int ahm_mbls_save(const char* path, double eta, uint32_t bmin, uint32_t n_leaves,
+                  const ahm_leaf_desc* leaves, const void* data, ahm_dtype dtype);
+/* header of a file and the number of values it holds (to size ahm_mbls_read) */
+int ahm_mbls_info(const char* path, ahm_dtype dtype, double* eta, uint32_t* bmin,
+                  uint32_t* n_leaves, uint64_t* n_values);
+/* leaves[i]: n1, n2, rank (-1 dense), offset (values packed in leaf order);
+ * b1 = b2 = 0.  n_leaves must equal the file's count. */
+int ahm_mbls_read(const char* path, ahm_dtype dtype, uint32_t n_leaves, ahm_leaf_desc* leaves,
+                  void* data);
+
 /* -------- assembly on the GPU ---------------------------------------------
  * The reference's matgenGeH_omp / apprx_unsym / ACA / cpyLrM_cmpr pipeline
  * (Include/matgen_omp.h:59-75, Include/apprx.h:136-172, Include/ACA.h:217-350,

@@ -33,6 +33,47 @@ void freembls(blcluster* bl, mblock<T>**& A) {
   freembls((unsigned)bl->nleaves(), A);
 }
 
+// The reference's H-matrix file (Include/H.h:47-103): eta, bmin, the leaf
+// count, then per leaf a presence byte and the mblock record (mblock::save).
+// loadmbls allocates the array itself (no allocmbls needed), as there.
+template <class T>
+void savembls(const double eta, const unsigned bmin, blcluster* bl, mblock<T>**& A,
+              std::ofstream& os) {
+  if (!os) {
+    std::cerr << "savembls: output stream not open" << std::endl;
+    std::exit(1);
+  }
+  const unsigned n = (unsigned)bl->nleaves();
+  os.write((const char*)&eta, sizeof(double));
+  os.write((const char*)&bmin, sizeof(unsigned));
+  os.write((const char*)&n, sizeof(unsigned));
+  for (unsigned i = 0; i < n; ++i) {
+    const bool present = A[i] != nullptr;
+    os.write((const char*)&present, sizeof(bool));
+    if (present) A[i]->save(os);
+  }
+}
+template <class T>
+void loadmbls(double& eta, unsigned& bmin, mblock<T>**& A, std::ifstream& is) {
+  if (!is) {
+    std::cerr << "loadmbls: input stream not open" << std::endl;
+    std::exit(1);
+  }
+  unsigned n = 0;
+  is.read((char*)&eta, sizeof(double));
+  is.read((char*)&bmin, sizeof(unsigned));
+  is.read((char*)&n, sizeof(unsigned));
+  allocmbls(n, A);
+  for (unsigned i = 0; i < n; ++i) {
+    bool present = false;
+    is.read((char*)&present, sizeof(bool));
+    if (present) {
+      A[i] = new mblock<T>(0, 0);
+      A[i]->load(is);
+    }
+  }
+}
+
 // y += d A x (H/mltaGeHVec.cpp:567-571).  x, y are host vectors offset so
 // that x[0] is column bl->getb2() and y[0] row bl->getb1().  Returns true
 // if some leaf contributed.

@@ -11,7 +11,10 @@
 #define AHMED_MBLOCK_H
 
 #include <cassert>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iostream>
 
 extern "C" void ahm_surface_invalidate(void);
 
@@ -77,6 +80,36 @@ class mblock {
   void cpyGeM(const T* A) {
     setGeM();
     std::memcpy(data, A, sizeof(T) * (size_t)n1 * n2);
+  }
+
+  // The reference's on-disk leaf record (Include/mblock.h:737-804): n1, n2,
+  // status (0 low rank, 4 dense), for low rank the rank, then the values.
+  // Only the two storage kinds of the GeH path exist here; a record of the
+  // reference's Hermitian / triangular / symmetric kinds stops the program
+  // with a message, as the reference does for an unknown status.
+  void save(std::ofstream& os) const {
+    const unsigned status = isLrM() ? 0u : 4u;
+    os.write((const char*)&n1, sizeof(unsigned));
+    os.write((const char*)&n2, sizeof(unsigned));
+    os.write((const char*)&status, sizeof(unsigned));
+    if (isLrM()) os.write((const char*)&bl_rank, sizeof(unsigned));
+    if (nvals()) os.write((const char*)data, (std::streamsize)(sizeof(T) * nvals()));
+  }
+  void load(std::ifstream& is) {
+    unsigned status = 0, k = 0;
+    is.read((char*)&n1, sizeof(unsigned));
+    is.read((char*)&n2, sizeof(unsigned));
+    is.read((char*)&status, sizeof(unsigned));
+    if (status == 0) {
+      is.read((char*)&k, sizeof(unsigned));
+      setrank(k);
+    } else if (status == 4) {
+      setGeM();
+    } else {
+      std::cerr << "mblock::load: leaf storage kind " << status << " is not supported" << std::endl;
+      std::exit(1);
+    }
+    if (nvals()) is.read((char*)data, (std::streamsize)(sizeof(T) * nvals()));
   }
 };
 

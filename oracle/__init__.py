@@ -254,6 +254,19 @@ class Oracle:
         h = self.lib.or_hmat_from_leaves(N, len(rank), *[_ptr(a) for a in arrs], _ptr(rank), _ptr(offset), _ptr(data))
         return _HMat(self, h, N)
 
+    def save(self, H, path, eta, bmin):
+        """The reference's savembls (Include/H.h:90-103) of H to path."""
+        if not self.lib.ref_hmat_save(H.h, str(path).encode(), _d(eta), _u(bmin)):
+            raise OSError(f"ref_hmat_save({path}) failed")
+
+    def load(self, tree, path):
+        """The reference's loadmbls (Include/H.h:60-75) onto tree's leaves; (H, eta, bmin)."""
+        eta, bmin = _d(0.0), _u(0)
+        h = self.lib.ref_hmat_load(tree.h, str(path).encode(), C.byref(eta), C.byref(bmin))
+        if not h:
+            raise OSError(f"ref_hmat_load({path}) failed")
+        return _HMat(self, h, tree.N, tree), eta.value, bmin.value
+
     def _layout(self, H):
         nl = self.lib.or_hmat_nleaves(H.h)
         b1, n1, b2, n2 = (np.empty(nl, np.uint32) for _ in range(4))
@@ -323,6 +336,10 @@ class Reference:
         L.ref_aca_dense.restype = C.c_int
         L.ref_aca_dense.argtypes = [_p, _u, _u, _d, _u, _u, _d, C.POINTER(_u), _p, _p]
         L.ref_addtrll.argtypes = [_u, _u, _u, _p, _p, _d, _u, C.POINTER(_u), _p, _p]
+        L.ref_hmat_save.restype = C.c_int
+        L.ref_hmat_save.argtypes = [_p, C.c_char_p, _d, _u]
+        L.ref_hmat_load.restype = _p
+        L.ref_hmat_load.argtypes = [_p, C.c_char_p, C.POINTER(_d), C.POINTER(_u)]
         self._tree_free = L.ref_tree_free
         self._tree_perm = L.ref_tree_perm
         self._tree_nleaves = L.ref_tree_nleaves
@@ -350,6 +367,19 @@ class Reference:
         data = np.ascontiguousarray(data, np.float64)
         h = self.lib.ref_hmat_from_data(tree.h, _ptr(rank), _ptr(offset), _ptr(data))
         return _HMat(self, h, tree.N, tree)
+
+    def save(self, H, path, eta, bmin):
+        """The reference's savembls (Include/H.h:90-103) of H to path."""
+        if not self.lib.ref_hmat_save(H.h, str(path).encode(), _d(eta), _u(bmin)):
+            raise OSError(f"ref_hmat_save({path}) failed")
+
+    def load(self, tree, path):
+        """The reference's loadmbls (Include/H.h:60-75) onto tree's leaves; (H, eta, bmin)."""
+        eta, bmin = _d(0.0), _u(0)
+        h = self.lib.ref_hmat_load(tree.h, str(path).encode(), C.byref(eta), C.byref(bmin))
+        if not h:
+            raise OSError(f"ref_hmat_load({path}) failed")
+        return _HMat(self, h, tree.N, tree), eta.value, bmin.value
 
     def _layout(self, H):
         lv = H.tree.leaves()

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <sstream>
 #include <vector>
@@ -350,6 +351,35 @@ void ref_addtrll(unsigned n1, unsigned n2, unsigned k, double* U, double* V, dou
     std::memcpy(Uo, m.getdata(), sizeof(double) * (size_t)n1 * *kt);
     std::memcpy(Vo, m.getdata() + (size_t)n1 * *kt, sizeof(double) * (size_t)n2 * *kt);
   }
+}
+
+// the reference's own on-disk format: savembls / loadmbls (Include/H.h:47-103,
+// mblock::save/load Include/mblock.h:737-804)
+int ref_hmat_save(void* hp, const char* path, double eta, unsigned bmin) {
+  init();
+  HMat* h = (HMat*)hp;
+  std::ofstream os(path, std::ios::binary);
+  if (!os) return 0;
+  savembls(eta, bmin, h->t->bl, h->A, os);
+  return os.good() ? 1 : 0;
+}
+
+// loadmbls into the leaves of a tree rebuilt from the points (the file holds
+// no geometry); returns NULL when the leaf count or a leaf's size differs
+void* ref_hmat_load(void* tp, const char* path, double* eta, unsigned* bmin) {
+  init();
+  Tree* t = (Tree*)tp;
+  std::ifstream is(path, std::ios::binary);
+  if (!is) return nullptr;
+  mblock<double>** A = nullptr;
+  loadmbls(*eta, *bmin, A, is);
+  for (unsigned i = 0; i < t->nblcks; ++i)
+    if (!A[i] || A[i]->getn1() != t->leaves[i]->getn1() || A[i]->getn2() != t->leaves[i]->getn2())
+      return nullptr;
+  HMat* h = new HMat;
+  h->t = t;
+  h->A = A;
+  return h;
 }
 
 }  // extern "C"

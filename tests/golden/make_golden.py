@@ -1,12 +1,12 @@
 """Write golden vectors from the COMPILED REFERENCE (run in the build container).
 
-Usage:  make -C oracle ref && python tests/golden/make_golden.py
+Usage:  make -C oracle ref && python tests/golden/make_golden.py [mbls]
 
 The reference's own sources (read in place under /root/reference, compiled by
 oracle/Makefile into oracle/_ref/libahmedref.so) produce every expected value
 here: cluster permutation, block-tree leaves, ACA ranks, a SHA-256 of all
 stored leaf values plus each leaf's Frobenius norm, H*x, H*X, GMRes and CG results, plus ACA / addtrll on
-explicit blocks.  Only data is written (golden_*.npz); nothing from the
+explicit blocks, plus SHA-256 digests of the reference's savembls files.  Only data is written (golden_*.npz); nothing from the
 reference travels in any other form.
 """
 from __future__ import annotations
@@ -80,8 +80,39 @@ def leaf_fro(L):
     return fro
 
 
+MBLS_CASES = ["ellip_yukawa1800", "pca_yukawa1500"]
+
+
+def write_mbls(R):
+    """SHA-256 of the files the reference's savembls writes for two golden
+    matrices (Include/H.h:90-103), checked by reloading them with its loadmbls."""
+    import tempfile
+    rec = {}
+    for name in MBLS_CASES:
+        c = next(c for c in CASES if c[0] == name)
+        _, geo, N, seed, bmin, eta, kind, kern, kappa, w, eps = c[:11]
+        pts = points(geo, N, seed)
+        T = R.tree(pts, bmin, eta * eta, kind)
+        H = R.assemble(T, kernel=kern, kappa=kappa, eps=eps, weight=weight_of(w, geo, N))
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "h.mbls")
+            R.save(H, path, eta, bmin)
+            raw = open(path, "rb").read()
+            H2, eta2, bmin2 = R.load(T, path)
+            x = np.random.default_rng(1).standard_normal(N)
+            assert eta2 == eta and bmin2 == bmin and np.array_equal(H2.matvec(x), H.matvec(x))
+        rec[f"{name}_sha256"] = hashlib.sha256(raw).hexdigest()
+        rec[f"{name}_bytes"] = len(raw)
+        rec[f"{name}_eta"] = eta
+        rec[f"{name}_bmin"] = bmin
+        print("mbls", name, len(raw), rec[f"{name}_sha256"][:16])
+    np.savez_compressed(os.path.join(OUT, "golden_mbls.npz"), **rec)
+
+
 def main():
     R = O.Reference()
+    if sys.argv[1:] == ["mbls"]:
+        return write_mbls(R)
     ver = O.Oracle().blas_version()
     for (name, geo, N, seed, bmin, eta, kind, kern, kappa, w, eps, gm, cgp) in CASES:
         pts = points(geo, N, seed)
@@ -138,6 +169,7 @@ def main():
         print("aca", c, ok, U.shape[1], "->", Uo.shape[1])
     aca["blas"] = ver
     np.savez_compressed(os.path.join(OUT, "golden_aca.npz"), **aca)
+    write_mbls(R)
 
 
 if __name__ == "__main__":
