@@ -94,6 +94,8 @@ SIGNATURES = {
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_autotune": (_i, [_p, _u32, _i, _u32, C.POINTER(C.c_uint64), C.POINTER(_d)]),
     "ahm_hmat_matvec_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
+    "ahm_hmat_matvec_t": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
+    "ahm_hmat_matvec_t_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
     "ahm_hmat_matvec_profile": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i, _u32, C.POINTER(_d),
                                      C.POINTER(_d)]),
 }
@@ -370,6 +372,20 @@ class HMatrix:
         self.ctx.bind_stream()
         check(load().ahm_hmat_matvec(self.h, alpha, _p(x.data_ptr()), ldx, _p(y.data_ptr()), ldy, p,
                                      vec), "ahm_hmat_matvec")
+        return y
+
+    def matvec_t(self, x, y, alpha: float = 1.0):
+        """y += alpha * H^T x (mltaGeHhVec, real case): x has n_rows values per
+        right-hand side, y n_cols; 1-D tensors or (p, n) as in matvec."""
+        torch = _torch()
+        vec = F64 if x.dtype == torch.float64 else F32
+        if x.dim() == 1:
+            p, ldx, ldy = 1, x.shape[0], y.shape[0]
+        else:
+            p, ldx, ldy = x.shape[0], x.stride(0), y.stride(0)
+        self.ctx.bind_stream()
+        check(load().ahm_hmat_matvec_t(self.h, alpha, _p(x.data_ptr()), ldx, _p(y.data_ptr()), ldy, p,
+                                       vec), "ahm_hmat_matvec_t")
         return y
 
     def profile(self, x, y, reps: int = 10, alpha: float = 1.0):

@@ -122,4 +122,14 @@ struct ahm_hmat_s {
   // host copies of the plan (used by ahm_hmat_convert)
   std::vector<ahm::GemvTask> h_tasks_a, h_tasks_b;
   std::vector<uint32_t> h_zsrc;
+  // transposed product (built on first use, hmat_t.hip): column-dot tasks of
+  // both phases and, per destination (t index / column), the panel columns
+  // that contribute to it (CSR, fixed order)
+  struct Transposed {
+    bool built = false;
+    ahm::DeviceBuf tasks_a, tasks_b, off_t, pos_t, off_y, pos_y;
+    uint32_t n_a = 0, n_b = 0;
+    ahm::DeviceBuf contrib, tprime;   // one value per panel column; t' = U^T x
+    size_t contrib_cap = 0, tprime_cap = 0;
+  } tr;
 };

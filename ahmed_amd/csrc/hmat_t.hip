@@ -1,0 +1,271 @@
+// hmat_t.hip -- the transposed H-matvec y += alpha A^T x on the same arena.
+//
+// Replaces the reference's mltaGeHhVec_ / mltaGeHhGeM_ (H/mltaGeHVec.cpp:
+// 61-104) with its leaf products mltaLrMhVec_ (H/mblock.cpp:159-170) and
+// mltaGeMhVec (Include/mblock.h:112-115), real case (A^H = A^T).
+//
+// No second copy of the matrix: every panel column of the forward layout
+// (hmat.hip) becomes one dot product of that column with the vector slice of
+// its rows, written to its own slot (the column's position in the source
+// list).  Four launches per right-hand side:
+//   1. dots of the row panels' columns with x  (dense columns -> y, U columns -> t')
+//   2. t'[i] = sum of the U-column dots of low-rank block i      (CSR, fixed order)
+//   3. dots of the V^T panels' columns with t'                   (-> y)
+//   4. y[j] += alpha * sum of the dots that belong to column j   (CSR, fixed order)
+// Each output is summed by one thread in a fixed order: deterministic, no
+// atomics.  The panels are read once; the slots cost 8 bytes per panel
+// column (one per 65-331 values at the benchmark sizes).
+#include <algorithm>
+#include <vector>
+
+#include "ahm_internal.h"
+
+namespace ahm {
+
+namespace {
+
+struct ColTask {
+  uint64_t data;   // element offset of the panel piece in the arena
+  uint64_t zoff;   // slot (source-list position) of its first column
+  uint32_t m, W;   // rows, columns
+  uint32_t base;   // index of row 0 in the vector (x row, or t index)
+  uint32_t pad;
+};
+
+// slots[zoff + c] = sum_r P[r][c] v[base + r].  Lanes form groups of gs
+// (a power of two >= m, at most 64); a group takes one column at a time and
+// its lanes hold rows gl, gl + 64, ... (m <= 256); the group sum is a fixed
+// butterfly of shuffles.
+template <typename TS, typename TV>
+__global__ __launch_bounds__(256) void coldot_kernel(const ColTask* __restrict__ tasks,
+                                                     const TS* __restrict__ arena,
+                                                     const TV* __restrict__ v,
+                                                     TV* __restrict__ slots) {
+  const ColTask tk = tasks[blockIdx.x];
+  const int m = (int)tk.m;
+  int gs = 64;
+  if (m <= 32) {
+    gs = 1;
+    while (gs < m) gs <<= 1;
+  }
+  const int lane = threadIdx.x & 63;
+  const int gl = lane & (gs - 1);
+  const int per_wave = 64 / gs;
+  const int grp = (int)(threadIdx.x >> 6) * per_wave + lane / gs;
+  const int ngrp = 4 * per_wave;
+  TV vr[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = gl + 64 * q;
+    vr[q] = r < m ? v[tk.base + r] : (TV)0;
+  }
+  const TS* P = arena + tk.data;
+  // four columns per group and pass: their loads are in flight together
+  for (uint32_t c0 = grp; c0 < tk.W; c0 += 4u * ngrp) {
+    TV s[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t c = c0 + (uint32_t)u * ngrp;
+      s[u] = 0;
+      if (c < tk.W) {
+        const TS* Pc = P + (uint64_t)c * m;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = gl + 64 * q;
+          if (r < m) s[u] = fma((TV)Pc[r], vr[q], s[u]);
+        }
+      }
+    }
+    for (int o = gs >> 1; o > 0; o >>= 1)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s[u] += __shfl_xor(s[u], o, 64);
+    if (gl == 0)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t c = c0 + (uint32_t)u * ngrp;
+        if (c < tk.W) slots[tk.zoff + c] = s[u];
+      }
+  }
+}
+
+// out[i] (=|+= alpha *) sum_{j in [off[i], off[i+1])} slots[pos[j]]
+template <typename TV, bool ASSIGN>
+__global__ __launch_bounds__(256) void csr_sum_kernel(const uint32_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ pos,
+                                                      const TV* __restrict__ slots, uint32_t n,
+                                                      TV* __restrict__ out, TV alpha) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  TV s = 0;
+  for (uint32_t j = off[i]; j < off[i + 1]; ++j) s += slots[pos[j]];
+  if (ASSIGN)
+    out[i] = s;
+  else
+    out[i] += alpha * s;
+}
+
+// column pieces of about kChunkValues values per task
+void col_tasks(const std::vector<GemvTask>& panels, std::vector<ColTask>& out) {
+  for (const GemvTask& p : panels) {
+    const uint32_t cols = (uint32_t)std::max<uint64_t>(1, kChunkValues / std::max(1u, p.m));
+    for (uint32_t c0 = 0; c0 < p.W; c0 += cols) {
+      ColTask t{};
+      t.data = p.data + (uint64_t)c0 * p.m;
+      t.zoff = p.zoff + c0;
+      t.m = p.m;
+      t.W = std::min(cols, p.W - c0);
+      t.base = p.out;
+      out.push_back(t);
+    }
+  }
+  std::stable_sort(out.begin(), out.end(), [](const ColTask& a, const ColTask& b) {
+    return (uint64_t)a.m * a.W > (uint64_t)b.m * b.W;
+  });
+}
+
+// CSR of the slots by destination, slots in increasing position
+void csr(const std::vector<uint32_t>& dest, const std::vector<uint32_t>& slot, uint32_t n,
+         std::vector<uint32_t>& off, std::vector<uint32_t>& pos) {
+  off.assign((size_t)n + 1, 0);
+  for (uint32_t d : dest) off[d + 1]++;
+  for (uint32_t i = 0; i < n; ++i) off[i + 1] += off[i];
+  pos.resize(dest.size());
+  std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+  for (size_t k = 0; k < dest.size(); ++k) pos[fill[dest[k]]++] = slot[k];
+}
+
+template <typename T>
+int upload(DeviceBuf& b, const std::vector<T>& v, hipStream_t s) {
+  AHM_TRY(b.alloc(std::max<size_t>(1, v.size()) * sizeof(T)));
+  if (!v.empty()) AHM_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return AHM_OK;
+}
+
+int build_transposed(ahm_hmat h) {
+  auto& T = h->tr;
+  if (T.built) return AHM_OK;
+  if (h->n_zsrc >= 0xFFFFFFFFull) return fail(AHM_ERR_UNSUPPORTED, "transposed product: too many panel columns");
+  std::vector<ColTask> ta, tb;
+  col_tasks(h->h_panels_a, ta);
+  col_tasks(h->h_panels, tb);
+  std::vector<uint32_t> dt, st, dy, sy;
+  for (uint32_t q = 0; q < (uint32_t)h->h_zsrc.size(); ++q) {
+    const uint32_t s = h->h_zsrc[q];
+    if (s & kFromT) {
+      dt.push_back(s & ~kFromT);
+      st.push_back(q);
+    } else {
+      dy.push_back(s);
+      sy.push_back(q);
+    }
+  }
+  std::vector<uint32_t> off_t, pos_t, off_y, pos_y;
+  csr(dt, st, h->t_len, off_t, pos_t);
+  csr(dy, sy, h->n_cols, off_y, pos_y);
+  hipStream_t s = h->ctx->stream;
+  AHM_TRY(upload(T.tasks_a, ta, s));
+  AHM_TRY(upload(T.tasks_b, tb, s));
+  AHM_TRY(upload(T.off_t, off_t, s));
+  AHM_TRY(upload(T.pos_t, pos_t, s));
+  AHM_TRY(upload(T.off_y, off_y, s));
+  AHM_TRY(upload(T.pos_y, pos_y, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  T.n_a = (uint32_t)ta.size();
+  T.n_b = (uint32_t)tb.size();
+  T.built = true;
+  return AHM_OK;
+}
+
+template <typename TS, typename TV>
+int launch_t(ahm_hmat h, TV alpha, const TV* x, TV* y) {
+  auto& T = h->tr;
+  hipStream_t s = h->ctx->stream;
+  TV* slots = (TV*)T.contrib.p;
+  TV* tp = (TV*)T.tprime.p;
+  const TS* arena = (const TS*)h->arena.p;
+  if (T.n_b)
+    hipLaunchKernelGGL((coldot_kernel<TS, TV>), dim3(T.n_b), dim3(256), 0, s,
+                       (const ColTask*)T.tasks_b.p, arena, x, slots);
+  if (h->t_len)
+    hipLaunchKernelGGL((csr_sum_kernel<TV, true>), dim3((h->t_len + 255) / 256), dim3(256), 0, s,
+                       (const uint32_t*)T.off_t.p, (const uint32_t*)T.pos_t.p, (const TV*)slots,
+                       h->t_len, tp, (TV)1);
+  if (T.n_a)
+    hipLaunchKernelGGL((coldot_kernel<TS, TV>), dim3(T.n_a), dim3(256), 0, s,
+                       (const ColTask*)T.tasks_a.p, arena, (const TV*)tp, slots);
+  if (h->n_cols)
+    hipLaunchKernelGGL((csr_sum_kernel<TV, false>), dim3((h->n_cols + 255) / 256), dim3(256), 0, s,
+                       (const uint32_t*)T.off_y.p, (const uint32_t*)T.pos_y.p, (const TV*)slots,
+                       h->n_cols, y, alpha);
+  AHM_HIP(hipGetLastError());
+  return AHM_OK;
+}
+
+}  // namespace
+
+}  // namespace ahm
+
+extern "C" {
+
+int ahm_hmat_matvec_t(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y, uint32_t ldy,
+                      uint32_t p, ahm_dtype vec) {
+  using namespace ahm;
+  if (!h || !X || !Y) return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_t: NULL argument");
+  if (p == 0) return AHM_OK;
+  if (p > 1 && (ldx < h->n_rows || ldy < h->n_cols))
+    return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_t: ldx=%u ldy=%u for %u x %u", ldx, ldy, h->n_rows,
+                h->n_cols);
+  const bool f64 = h->storage == AHM_F64 && vec == AHM_F64;
+  const bool f32 = h->storage == AHM_F32 && vec == AHM_F32;
+  if (!f64 && !f32)
+    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec_t: storage %d with vectors %d", h->storage, vec);
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  AHM_TRY(build_transposed(h));
+  auto& T = h->tr;
+  const size_t es = f64 ? 8 : 4;
+  if (h->n_zsrc * es > T.contrib_cap) {
+    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+    AHM_TRY(T.contrib.alloc(std::max<size_t>(1, h->n_zsrc) * es));
+    T.contrib_cap = h->n_zsrc * es;
+  }
+  if ((size_t)h->t_len * es > T.tprime_cap) {
+    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+    AHM_TRY(T.tprime.alloc(std::max<size_t>(1, h->t_len) * es));
+    T.tprime_cap = (size_t)h->t_len * es;
+  }
+  for (uint32_t j = 0; j < p; ++j) {
+    if (f64)
+      AHM_TRY((launch_t<double, double>(h, alpha, (const double*)X + (size_t)j * ldx,
+                                        (double*)Y + (size_t)j * ldy)));
+    else
+      AHM_TRY((launch_t<float, float>(h, (float)alpha, (const float*)X + (size_t)j * ldx,
+                                      (float*)Y + (size_t)j * ldy)));
+  }
+  return AHM_OK;
+}
+
+int ahm_hmat_matvec_t_host(ahm_hmat h, double alpha, const double* X, uint32_t ldx, double* Y,
+                           uint32_t ldy, uint32_t p) {
+  using namespace ahm;
+  if (!h || !X || !Y) return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_t_host: NULL argument");
+  if (h->storage != AHM_F64) return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec_t_host: needs an F64 matrix");
+  if (p == 0) return AHM_OK;
+  if (p > 1 && (ldx < h->n_rows || ldy < h->n_cols))
+    return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_t_host: bad leading dimensions");
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  hipStream_t s = h->ctx->stream;
+  DeviceBuf dx, dy;
+  AHM_TRY(dx.alloc((size_t)h->n_rows * p * 8));
+  AHM_TRY(dy.alloc((size_t)h->n_cols * p * 8));
+  AHM_HIP(hipMemcpy2DAsync(dx.p, (size_t)h->n_rows * 8, X, (size_t)ldx * 8, (size_t)h->n_rows * 8, p,
+                           hipMemcpyHostToDevice, s));
+  AHM_HIP(hipMemcpy2DAsync(dy.p, (size_t)h->n_cols * 8, Y, (size_t)ldy * 8, (size_t)h->n_cols * 8, p,
+                           hipMemcpyHostToDevice, s));
+  AHM_TRY(ahm_hmat_matvec_t(h, alpha, dx.p, h->n_rows, dy.p, h->n_cols, p, AHM_F64));
+  AHM_HIP(hipMemcpy2DAsync(Y, (size_t)ldy * 8, dy.p, (size_t)h->n_cols * 8, (size_t)h->n_cols * 8, p,
+                           hipMemcpyDeviceToHost, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  return AHM_OK;
+}
+
+}  // extern "C"

@@ -158,6 +158,29 @@ bool mltaGeHGeM(double d, blcluster* bl, mblock<double>** A, unsigned p, double*
   return changed;
 }
 
+bool mltaGeHhVec(double d, blcluster* bl, mblock<double>** A, double* x, double* y) {
+  bool changed = false;
+  ahm_hmat h = device_matrix(bl, A, changed);
+  if (!h) return false;
+  if (ahm_hmat_matvec_t_host(h, d, x, bl->getn1(), y, bl->getn2(), 1) != AHM_OK) {
+    std::fprintf(stderr, "ahmed: mltaGeHhVec failed: %s\n", ahm_last_error());
+    return false;
+  }
+  return changed;
+}
+
+bool mltaGeHhGeM(double d, blcluster* bl, mblock<double>** A, unsigned p, double* X, unsigned ldX,
+                 double* Y, unsigned ldY) {
+  bool changed = false;
+  ahm_hmat h = device_matrix(bl, A, changed);
+  if (!h) return false;
+  if (ahm_hmat_matvec_t_host(h, d, X, ldX, Y, ldY, p) != AHM_OK) {
+    std::fprintf(stderr, "ahmed: mltaGeHhGeM failed: %s\n", ahm_last_error());
+    return false;
+  }
+  return changed;
+}
+
 bool mltaGeHVec(float d, blcluster* bl, mblock<float>** A, float* x, float* y) {
   bool changed = false;
   ahm_hmat h = device_matrix(bl, A, changed);

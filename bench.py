@@ -341,6 +341,12 @@ def main():
     log(f"phase A (V^T x) {ms_a:.3f} ms {bytes_a / (ms_a * 1e-3) / 1e9:.1f} GB/s; "
         f"phase B (row panels) {ms_b:.3f} ms {ach_b:.1f} GB/s")
 
+    # ---- transposed product y += H^T x (mltaGeHhVec), FP64 ---------------------
+    yt = torch.zeros(N, dtype=torch.float64, device="cuda")
+    H.matvec_t(x, yt)          # builds the column index once
+    mst = timed(lambda: H.matvec_t(x, yt), torch, args.steps, args.warmup)
+    log(f"FP64 transposed H-matvec 1 RHS: {mst:.3f} ms, {B_alg / (mst * 1e-3) / 1e9:.1f} GB/s")
+
     # ---- p RHS, FP64 ---------------------------------------------------------
     p = args.rhs
     X = torch.randn(p, N, dtype=torch.float64, device="cuda")
@@ -443,6 +449,7 @@ def main():
                              "TF/s": round(F_alg / (msp * 1e-3) / 1e12, 3),
                              "mfma_frac_fp64": round(F_alg / (msp * 1e-3) / 1e12 / FP64_PEAK_TFS, 4),
                              "GB/s_alg": round(Bp / (msp * 1e-3) / 1e9, 2)},
+        "matvec_transposed": {"ms": round(mst, 4), "GB/s": round(B_alg / (mst * 1e-3) / 1e9, 2)},
         "fp32_storage": {"ms": round(ms32, 4), "GB/s": round(B32 / (ms32 * 1e-3) / 1e9, 2),
                          "rel_err_vs_fp64": err32},
         "assembly": {"tree_s": round(t_tree, 3), "gpu_assembly_s": round(t_asm, 3),

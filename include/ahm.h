@@ -227,6 +227,18 @@ int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps,
 int ahm_hmat_matvec_host(ahm_hmat h, double alpha, const double* X, uint32_t ldx,
                          double* Y, uint32_t ldy, uint32_t p);
 
+/* Y += alpha A^T X: the real case of mltaGeHhVec / mltaGeHhGeM
+ * (H/mltaGeHVec.cpp:61-104, leaf products H/mblock.cpp:159-170 and
+ * Include/mblock.h:112-115).  X: p columns of n_rows values (ldx apart),
+ * Y: p columns of n_cols values (ldy apart), cluster-permuted as in
+ * ahm_hmat_matvec; runs on the forward layout (no second copy of the leaves),
+ * deterministic.  The first call on a matrix builds its column index. */
+int ahm_hmat_matvec_t(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y,
+                      uint32_t ldy, uint32_t p, ahm_dtype vec);
+/* the same with host pointers (FP64 matrices), as ahm_hmat_matvec_host */
+int ahm_hmat_matvec_t_host(ahm_hmat h, double alpha, const double* X, uint32_t ldx, double* Y,
+                           uint32_t ldy, uint32_t p);
+
 /* -------- Krylov solvers on the device --------------------------------------
  * b, x: FP64 device vectors of length n_rows (x is the initial guess and the
  * result).  eps: in = tolerance, out = achieved relative residual; nsteps:

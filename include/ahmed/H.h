@@ -83,6 +83,13 @@ bool mltaGeHVec(float d, blcluster* bl, mblock<float>** A, float* x, float* y);
 bool mltaGeHGeM(double d, blcluster* bl, mblock<double>** A, unsigned p, double* X,
                 unsigned ldX, double* Y, unsigned ldY);
 
+// y += d A^T x and Y += d A^T X (the real case of mltaGeHhVec / mltaGeHhGeM,
+// H/mltaGeHVec.cpp:61-104): x, X offset to row bl->getb1(), y, Y to column
+// bl->getb2().  Returns true if some leaf contributed.
+bool mltaGeHhVec(double d, blcluster* bl, mblock<double>** A, double* x, double* y);
+bool mltaGeHhGeM(double d, blcluster* bl, mblock<double>** A, unsigned p, double* X,
+                 unsigned ldX, double* Y, unsigned ldY);
+
 // storage in bytes and largest rank (H/initH.cpp:16-78)
 unsigned long sizeH(blcluster* bl, mblock<double>** A);
 unsigned Hmax_rank(blcluster* bl, mblock<double>** A);

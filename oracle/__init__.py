@@ -85,6 +85,13 @@ class _HMat:
         self.api._matvec(self.h, _d(d), _ptr(x), _ptr(y))
         return y
 
+    def matvec_t(self, x, d=1.0, y=None):
+        """y += d A^T x (mltaGeHhVec); x has the row length, y the column length."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.N) if y is None else np.ascontiguousarray(y, np.float64).copy()
+        self.api._matvec_t(self.h, _d(d), _ptr(x), _ptr(y))
+        return y
+
     def matmat(self, X, d=1.0, Y=None):
         X = np.asfortranarray(X, np.float64)
         p = X.shape[1]
@@ -161,6 +168,8 @@ class Oracle:
         L.or_hmat_data.argtypes = [_p]
         L.or_hmat_matvec.restype = C.c_int
         L.or_hmat_matvec.argtypes = [_p, _d, _p, _p]
+        L.or_hmat_matvec_t.restype = C.c_int
+        L.or_hmat_matvec_t.argtypes = [_p, _d, _p, _p]
         L.or_hmat_matvec_omp.restype = C.c_int
         L.or_hmat_matvec_omp.argtypes = [_p, _d, _p, _p, C.c_int]
         L.or_hmat_matmat.restype = C.c_int
@@ -180,6 +189,7 @@ class Oracle:
         self._tree_leaves = L.or_tree_leaves
         self._hmat_free = L.or_hmat_free
         self._matvec = L.or_hmat_matvec
+        self._matvec_t = L.or_hmat_matvec_t
         self._matmat = L.or_hmat_matmat
         self._cg = L.or_cg
 
@@ -327,6 +337,8 @@ class Reference:
         L.ref_hmat_data.argtypes = [_p, _p]
         L.ref_hmat_matvec.restype = C.c_int
         L.ref_hmat_matvec.argtypes = [_p, _d, _p, _p]
+        L.ref_hmat_matvec_t.restype = C.c_int
+        L.ref_hmat_matvec_t.argtypes = [_p, _d, _p, _p]
         L.ref_hmat_matmat.restype = C.c_int
         L.ref_hmat_matmat.argtypes = [_p, _d, _u, _p, _u, _p, _u]
         L.ref_gmres.restype = _u
@@ -346,6 +358,7 @@ class Reference:
         self._tree_leaves = L.ref_tree_leaves
         self._hmat_free = L.ref_hmat_free
         self._matvec = L.ref_hmat_matvec
+        self._matvec_t = L.ref_hmat_matvec_t
         self._matmat = L.ref_hmat_matmat
         self._gmres = L.ref_gmres
         self._cg = L.ref_cg

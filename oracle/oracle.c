@@ -927,6 +927,35 @@ int or_hmat_matvec(const or_hmat* h, double d, const double* x, double* y)
   return changed;
 }
 
+/* y += d A^T x, the real case of mltaGeHhVec_ (H/mltaGeHVec.cpp:61-81):
+ * leaves in idx order; low rank: e = d <U_l, x> then y += e V_l
+ * (mltaLrMhVec_, H/mblock.cpp:159-170); dense: dgemv 'T' (mltaGeMhVec,
+ * Include/mblock.h:112-115).  x is indexed by rows, y by columns. */
+int or_hmat_matvec_t(const or_hmat* h, double d, const double* x, double* y)
+{
+  blas_init();
+  int changed = 0;
+  for (unsigned l = 0; l < h->nl; ++l) {
+    const unsigned n1 = h->n1[l], n2 = h->n2[l];
+    const double* data = h->data + h->off[l];
+    const double* xp = x + h->b1[l];
+    double* yp = y + h->b2[l];
+    if (h->rank[l] >= 0) {
+      const unsigned k = (unsigned)h->rank[l];
+      if (!k) continue;
+      for (unsigned q = 0; q < k; ++q) {
+        const double e = d * scpr(n1, data + (size_t)q * n1, xp);
+        axpy(n2, e, data + (size_t)k * n1 + (size_t)q * n2, yp);
+      }
+    } else {
+      int mm = (int)n1, nn = (int)n2;
+      dgemv_("T", &mm, &nn, &d, data, &mm, xp, &I1, &DONE, yp, &I1);
+    }
+    changed = 1;
+  }
+  return changed;
+}
+
 /* mltaGeHGeM_ (H/mltaGeHVec.cpp:38-59), mltaLrMGeM_ (H/mblock.cpp:140-154),
  * mltaGeMGeM (Include/mblock.h:104-109) */
 int or_hmat_matmat(const or_hmat* h, double d, unsigned p, const double* X, unsigned ldX,

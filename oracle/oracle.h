@@ -97,6 +97,7 @@ const double* or_hmat_data(const or_hmat* h);
 /* ---- matvec: mltaGeHVec / mltaGeHGeM (H/mltaGeHVec.cpp:16-59,567-577),
  * leaf products H/mblock.cpp:125-154, Include/mblock.h:98-109     [golden] */
 int or_hmat_matvec(const or_hmat* h, double d, const double* x, double* y);
+int or_hmat_matvec_t(const or_hmat* h, double d, const double* x, double* y);
 int or_hmat_matmat(const or_hmat* h, double d, unsigned p, const double* X,
                    unsigned ldX, double* Y, unsigned ldY);
 /* leaf-parallel all-core variant with per-thread y (CPU baseline only) */

@@ -295,6 +295,12 @@ int ref_hmat_matvec(void* p, double d, double* x, double* y) {
   return mltaGeHVec(d, h->t->bl, h->A, x, y) ? 1 : 0;
 }
 
+// y += d A^H x (H/mltaGeHVec.cpp:61-81); x indexed by rows, y by columns
+int ref_hmat_matvec_t(void* p, double d, double* x, double* y) {
+  HMat* h = (HMat*)p;
+  return mltaGeHhVec(d, h->t->bl, h->A, x, y) ? 1 : 0;
+}
+
 int ref_hmat_matmat(void* p, double d, unsigned pp, double* X, unsigned ldX, double* Y,
                     unsigned ldY) {
   HMat* h = (HMat*)p;

@@ -1,6 +1,6 @@
 """Write golden vectors from the COMPILED REFERENCE (run in the build container).
 
-Usage:  make -C oracle ref && python tests/golden/make_golden.py [mbls]
+Usage:  make -C oracle ref && python tests/golden/make_golden.py [mbls | transposed]
 
 The reference's own sources (read in place under /root/reference, compiled by
 oracle/Makefile into oracle/_ref/libahmedref.so) produce every expected value
@@ -109,10 +109,29 @@ def write_mbls(R):
     np.savez_compressed(os.path.join(OUT, "golden_mbls.npz"), **rec)
 
 
+def write_transposed(R):
+    """The reference's mltaGeHhVec (y += d A^H x, H/mltaGeHVec.cpp:61-81) on
+    every golden matrix: golden_transposed.npz holds <case>_x, <case>_y, <case>_d."""
+    rec = {}
+    for (name, geo, N, seed, bmin, eta, kind, kern, kappa, w, eps, _, _) in CASES:
+        pts = points(geo, N, seed)
+        T = R.tree(pts, bmin, eta * eta, kind)
+        H = R.assemble(T, kernel=kern, kappa=kappa, eps=eps, weight=weight_of(w, geo, N))
+        rng = np.random.default_rng(500 + seed)
+        x = rng.standard_normal(N)
+        y0 = rng.standard_normal(N)
+        rec[f"{name}_x"], rec[f"{name}_y0"], rec[f"{name}_d"] = x, y0, -0.75
+        rec[f"{name}_y"] = H.matvec_t(x, d=-0.75, y=y0)
+        print("transposed", name)
+    np.savez_compressed(os.path.join(OUT, "golden_transposed.npz"), **rec)
+
+
 def main():
     R = O.Reference()
     if sys.argv[1:] == ["mbls"]:
         return write_mbls(R)
+    if sys.argv[1:] == ["transposed"]:
+        return write_transposed(R)
     ver = O.Oracle().blas_version()
     for (name, geo, N, seed, bmin, eta, kind, kern, kappa, w, eps, gm, cgp) in CASES:
         pts = points(geo, N, seed)
@@ -170,6 +189,7 @@ def main():
     aca["blas"] = ver
     np.savez_compressed(os.path.join(OUT, "golden_aca.npz"), **aca)
     write_mbls(R)
+    write_transposed(R)
 
 
 if __name__ == "__main__":

@@ -100,6 +100,9 @@ def test_matvec_equals_dense_product(small_h):
     X = rng.standard_normal((H.N, 3))
     Y = H.matmat(X, d=1.3)
     np.testing.assert_allclose(Y, 1.3 * A @ X, rtol=0, atol=1e-13 * np.abs(A @ X).max())
+    # transposed product: y += d A^T x against the dense transpose
+    np.testing.assert_allclose(H.matvec_t(x, d=-0.7, y=y0), y0 - 0.7 * A.T @ x, rtol=0,
+                               atol=1e-13 * np.abs(A.T @ x).max())
     Y0 = rng.standard_normal((H.N, 3))   # accumulation into a C-ordered start value
     Y = H.matmat(X, d=-0.4, Y=Y0)
     np.testing.assert_allclose(Y, Y0 - 0.4 * A @ X, rtol=0, atol=1e-13 * np.abs(A @ X).max())

@@ -2,15 +2,16 @@
 
 Fixtures: tests/golden/golden_*.npz (tests/golden/make_golden.py).  Trees,
 leaves and ACA ranks must match exactly; stored leaf values bit-for-bit
-(SHA-256); matvec / GMRes / CG to 1e-13 (bit-equal in the build container,
+(SHA-256); matvec, transposed matvec (golden_transposed.npz), GMRes and CG to 1e-13 (bit-equal in the build container,
 tolerance covers a different MKL code path elsewhere).
 """
 import hashlib
+import os
 
 import numpy as np
 import pytest
 
-from conftest import golden_names, golden_points, load_golden
+from conftest import GOLDEN, golden_names, golden_points, load_golden
 
 NAMES = golden_names()
 
@@ -18,6 +19,7 @@ NAMES = golden_names()
 @pytest.fixture(scope="module", params=NAMES)
 def case(request, oracle_api):
     g = load_golden(request.param)
+    g["name"] = request.param
     pts = golden_points(g)
     T = oracle_api.tree(pts, int(g["bmin"]), float(g["eta2"]), int(g["kind"]))
     H = oracle_api.assemble(T, pts, int(g["kernel"]), float(g["kappa"]), float(g["eps"]),
@@ -61,6 +63,14 @@ def test_matvec_matches_reference(case):
     assert np.max(np.abs(y - g["y"])) <= 1e-13 * np.max(np.abs(g["y"]))
     Y = H.matmat(g["X"], d=-0.5)
     assert np.max(np.abs(Y - g["Y"])) <= 1e-13 * np.max(np.abs(g["Y"]))
+
+
+def test_transposed_matvec_matches_reference(case):
+    g, _, H = case
+    name = g["name"]
+    t = np.load(os.path.join(GOLDEN, "golden_transposed.npz"))
+    y = H.matvec_t(t[f"{name}_x"], d=float(t[f"{name}_d"]), y=t[f"{name}_y0"])
+    assert np.max(np.abs(y - t[f"{name}_y"])) <= 1e-13 * np.max(np.abs(t[f"{name}_y"]))
 
 
 def test_gmres_matches_reference(case):
