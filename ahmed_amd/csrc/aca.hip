@@ -692,6 +692,26 @@ struct Part {
   uint64_t len = 0;
 };
 
+// device scratch reused across batches: reallocated only when a batch needs
+// more (allocating and freeing tens of GB per batch costs more than the
+// batch's kernels)
+struct Grow {
+  DeviceBuf buf;
+  size_t cap = 0;
+  int need(size_t bytes) {
+    if (bytes <= cap && buf.p) return AHM_OK;
+    AHM_TRY(buf.alloc(std::max<size_t>(bytes, 1)));
+    cap = std::max<size_t>(bytes, 1);
+    return AHM_OK;
+  }
+  template <typename T>
+  int put(const std::vector<T>& v, hipStream_t s) {
+    AHM_TRY(need(v.size() * sizeof(T)));
+    if (!v.empty()) AHM_HIP(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return AHM_OK;
+  }
+};
+
 }  // namespace
 
 KernelSpec make_kernel(const ahm_kernel& k) {
@@ -759,8 +779,11 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     for (auto* p : parts) delete p;
     parts.clear();
   };
-  while (!todo.empty()) {
-    std::vector<size_t> next;
+  Grow gblk, gres, gslab, gscr, goff;
+  // one pass over a growing list: a leaf whose slab was too small goes back
+  // to the end of the list with a larger slab, so reruns share later batches
+  // with first attempts instead of forming a thin pass of their own
+  {
     size_t pos = 0;
     while (pos < todo.size()) {
       std::vector<AcaBlock> blk;
@@ -792,11 +815,16 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       }
       slab_total += slab * 8;
       slab_peak = std::max<uint64_t>(slab_peak, slab * 8);
-      DeviceBuf dblk, dres, dslab, discr;
-      int st = to_device(dblk, blk, s);
-      if (!st) st = dres.alloc(blk.size() * sizeof(AcaResult));
-      if (!st) st = dslab.alloc(slab * 8);
-      if (!st) st = discr.alloc(iscr * 4);
+      // the previous batch's kernels finished before its results were read,
+      // so the scratch can be reused
+      int st = gblk.put(blk, s);
+      if (!st) st = gres.need(blk.size() * sizeof(AcaResult));
+      if (!st) st = gslab.need(slab * 8);
+      if (!st) st = gscr.need(iscr * 4);
+      DeviceBuf& dblk = gblk.buf;
+      DeviceBuf& dres = gres.buf;
+      DeviceBuf& dslab = gslab.buf;
+      DeviceBuf& discr = gscr.buf;
       if (st) {
         cleanup();
         return st;
@@ -828,7 +856,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
             continue;
           }
           cap[c] = std::min(std::max(cap[c] * 4, cap[c] + 1), lim);
-          next.push_back(c);
+          todo.push_back(c);
           ++retries;
           continue;
         }
@@ -845,9 +873,9 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       parts.push_back(part);
       part->len = tot;
       if (tot) {
-        DeviceBuf doff;
+        DeviceBuf& doff = goff.buf;
         st = part->buf.alloc(tot * 8);
-        if (!st) st = to_device(doff, off, s);
+        if (!st) st = goff.put(off, s);
         if (st) {
           cleanup();
           return st;
@@ -862,7 +890,6 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       }
       pos = end;
     }
-    todo.swap(next);
   }
   // final buffer: the batch parts back to back, then the dense leaves
   std::vector<uint64_t> part_base(parts.size(), 0);
