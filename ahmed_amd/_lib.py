@@ -45,7 +45,8 @@ class Kernel(C.Structure):
 class AssemblyStats(C.Structure):
     _fields_ = [("n_candidates", C.c_uint64), ("n_retries", C.c_uint64), ("n_lowrank", C.c_uint64),
                 ("n_dense", C.c_uint64), ("sum_rank", C.c_uint64), ("sum_aca_rank", C.c_uint64),
-                ("slab_bytes", C.c_uint64), ("peak_slab_bytes", C.c_uint64)]
+                ("slab_bytes", C.c_uint64), ("peak_slab_bytes", C.c_uint64),
+                ("aca_ms", C.c_double)]
 
 
 KERNEL_LAPLACE, KERNEL_YUKAWA, KERNEL_EXPR = 0, 1, 2

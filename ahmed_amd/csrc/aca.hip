@@ -20,6 +20,7 @@
 // permuted points: entry(i,j) = 1 on i == j, else c/r (Laplace) or
 // exp(-kappa r) c / r (decaying kernels).
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <numeric>
 
@@ -730,6 +731,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
                     std::vector<ahm_leaf_desc>& leaves, DeviceBuf& data,
                     ahm_assembly_stats* stats, uint32_t* aca_rank_out) {
   hipStream_t s = ctx->stream;
+  PhaseTimer timer("assembly");
   if (nk == 0) return fail(AHM_ERR_INVALID, "no kernel");
   for (uint32_t l = 0; leaf_kernel && l < nl; ++l)
     if (leaf_kernel[l] >= nk) return fail(AHM_ERR_INVALID, "leaf %u: kernel %u of %u", l, leaf_kernel[l], nk);
@@ -769,8 +771,11 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     cap[c] = std::min(h, lim);
   }
   uint64_t retries = 0, slab_total = 0, slab_peak = 0;
+  // largest slabs first: the leaves most likely to need a rerun go early, so
+  // their reruns share the following batches with many small first attempts
   std::vector<size_t> todo(cand.size());
   std::iota(todo.begin(), todo.end(), 0);
+  std::stable_sort(todo.begin(), todo.end(), [&](size_t a, size_t b) { return cap[a] > cap[b]; });
   const uint64_t budget = (uint64_t)24 << 30;  // bytes of ACA scratch per batch
   std::vector<Part*> parts;
   std::vector<uint32_t> part_of(nl, 0);
@@ -780,9 +785,12 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     parts.clear();
   };
   Grow gblk, gres, gslab, gscr, goff;
+  timer.mark("set-up (points, candidates)");
+  const auto t_aca0 = std::chrono::steady_clock::now();
   // one pass over a growing list: a leaf whose slab was too small goes back
-  // to the end of the list with a larger slab, so reruns share later batches
-  // with first attempts instead of forming a thin pass of their own
+  // into the list, just ahead of the next batch, with a larger slab, so reruns
+  // share batches with first attempts instead of forming a thin pass of their
+  // own
   {
     size_t pos = 0;
     while (pos < todo.size()) {
@@ -845,6 +853,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       }
       std::vector<uint64_t> off(nb, 0);
       uint64_t tot = 0;
+      std::vector<size_t> rerun;
       for (uint32_t q = 0; q < nb; ++q) {
         const size_t c = todo[pos + q];
         const uint32_t l = cand[c];
@@ -856,7 +865,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
             continue;
           }
           cap[c] = std::min(std::max(cap[c] * 4, cap[c] + 1), lim);
-          todo.push_back(c);
+          rerun.push_back(c);
           ++retries;
           continue;
         }
@@ -889,8 +898,13 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
         }
       }
       pos = end;
+      // reruns join the next batch
+      todo.insert(todo.begin() + (ptrdiff_t)pos, rerun.begin(), rerun.end());
     }
   }
+  timer.mark("ACA batches (kernels, result copies, slab decisions)");
+  const double aca_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_aca0).count();
   // final buffer: the batch parts back to back, then the dense leaves
   std::vector<uint64_t> part_base(parts.size(), 0);
   uint64_t total = 0;
@@ -916,7 +930,9 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       total += (uint64_t)n1[l] * n2[l];
     }
   }
+  timer.mark("leaf list");
   int st = data.alloc(total * 8);
+  timer.mark("leaf buffer allocation");
   if (st) {
     cleanup();
     return st;
@@ -942,6 +958,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
   const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
   cleanup();
   if (!ok) return fail(AHM_ERR_HIP, "dense leaf kernel failed");
+  timer.mark("leaf buffer (low-rank parts, dense leaves)");
   if (aca_rank_out)
     for (uint32_t l = 0; l < nl; ++l) aca_rank_out[l] = kaca[l];
   if (stats) {
@@ -950,6 +967,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     stats->n_retries = retries;
     stats->slab_bytes = slab_total;
     stats->peak_slab_bytes = slab_peak;
+    stats->aca_ms = aca_ms;
     stats->n_dense = dj.size();
     for (uint32_t l = 0; l < nl; ++l)
       if (rank[l] >= 0) {

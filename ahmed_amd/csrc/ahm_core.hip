@@ -1,6 +1,8 @@
 // ahm_core.hip -- error state, contexts and device buffers of libahmed_amd.
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 
 #include "ahm_internal.h"
 
@@ -43,6 +45,20 @@ void DeviceBuf::release() {
   if (p) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
+}
+
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+PhaseTimer::PhaseTimer(const char* w) : on(std::getenv("AHM_TIMING") != nullptr), what(w), t_last(now_ms()) {}
+
+void PhaseTimer::mark(const char* phase) {
+  if (!on) return;
+  const double t = now_ms();
+  std::fprintf(stderr, "[ahm timing] %s: %s %.1f ms\n", what, phase, t - t_last);
+  t_last = t;
 }
 
 size_t dtype_size(ahm_dtype t) {

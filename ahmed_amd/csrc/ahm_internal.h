@@ -76,6 +76,17 @@ struct DeviceBuf {
 
 size_t dtype_size(ahm_dtype t);
 
+// Host-side phase timing of the set-up paths (assembly, layout), printed to
+// stderr when the environment variable AHM_TIMING is set: one line per
+// mark with the milliseconds since the previous mark.
+struct PhaseTimer {
+  explicit PhaseTimer(const char* what);
+  void mark(const char* phase);
+  bool on;
+  const char* what;
+  double t_last;
+};
+
 int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_leaf_desc>&& leaves,
                struct DeviceBuf& raw, ahm_dtype data_dtype, ahm_dtype storage, int keep_leaves,
                ahm_hmat* out);

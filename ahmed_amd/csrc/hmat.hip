@@ -637,8 +637,10 @@ int set_chunking(ahm_hmat h, uint64_t cv) {
 int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_leaf_desc>&& leaves,
                DeviceBuf& raw, ahm_dtype data_dtype, ahm_dtype storage, int keep_leaves,
                ahm_hmat* out) {
+  PhaseTimer timer("layout");
   Plan P;
   AHM_TRY(build_plan(n_rows, n_cols, (uint32_t)leaves.size(), leaves.data(), P));
+  timer.mark("plan (panels, chunks, copy jobs)");
   for (const ahm_leaf_desc& d : leaves) {
     const uint64_t nv = d.rank < 0 ? (uint64_t)d.n1 * d.n2 : (uint64_t)d.rank * (d.n1 + d.n2);
     if ((d.offset + nv) * dtype_size(data_dtype) > raw.bytes && nv)
@@ -665,6 +667,7 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
   DeviceBuf jobs;
   do {
     if ((st = h->arena.alloc(P.nvals * ss))) break;
+    timer.mark("arena allocation");
     if ((st = upload_vec(jobs, P.jobs, s))) break;
     if ((st = upload_vec(h->tasks_a, P.a, s))) break;
     if ((st = upload_vec(h->tasks_b, P.b, s))) break;
@@ -677,6 +680,7 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
     h->n_ra = (uint32_t)P.ra.size();
     h->part_len = P.part_len;
     h->n_zsrc = P.zsrc.size();
+    if (timer.on && hipStreamSynchronize(s) == hipSuccess) timer.mark("task, source-list and job uploads");
     for (size_t j0 = 0; j0 < P.jobs.size(); j0 += (1u << 30)) {
       const uint32_t nj = (uint32_t)std::min<size_t>(1u << 30, P.jobs.size() - j0);
       const CopyJob* J = (const CopyJob*)jobs.p + j0;
@@ -694,6 +698,7 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
       st = fail(AHM_ERR_HIP, "H-matrix re-layout kernel failed");
       break;
     }
+    timer.mark("re-layout kernel");
     h->h_tasks_a = std::move(P.a);
     h->h_tasks_b = std::move(P.b);
     h->h_tasks_r = std::move(P.r);

@@ -228,10 +228,12 @@ def bench_learned_slabs(args):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         ranks[name] = (st["aca_rank"], H.stats()["nvals"], st["sum_rank"])
-        out[name] = dict(seconds=round(dt, 3), aca_reruns=st["n_retries"], slab_GB=round(st["slab_bytes"] / 1e9, 3),
+        out[name] = dict(seconds=round(dt, 3), aca_batches_s=round(st["aca_ms"] / 1e3, 3),
+                         aca_reruns=st["n_retries"], slab_GB=round(st["slab_bytes"] / 1e9, 3),
                          peak_batch_slab_GB=round(st["peak_slab_bytes"] / 1e9, 3), sum_rank=st["sum_rank"],
                          nvals=H.stats()["nvals"])
-        log(f"{name}: {dt:.3f} s, reruns {st['n_retries']}, slab {st['slab_bytes'] / 1e9:.2f} GB")
+        log(f"{name}: {dt:.3f} s (ACA batches {st['aca_ms'] / 1e3:.3f} s), reruns {st['n_retries']}, "
+            f"slab {st['slab_bytes'] / 1e9:.2f} GB")
         H.close()
     same = all(np.array_equal(ranks[k][0], ranks["fixed48"][0]) and ranks[k][1] == ranks["fixed48"][1]
                for k in ranks)

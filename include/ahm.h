@@ -160,6 +160,7 @@ typedef struct {
   uint64_t sum_aca_rank;    /* sum of ACA ranks before recompression       */
   uint64_t slab_bytes;      /* ACA scratch allocated over all batches/reruns */
   uint64_t peak_slab_bytes; /* largest single batch                          */
+  double aca_ms;            /* wall time of the ACA batches (host clock)     */
 } ahm_assembly_stats;
 
 /* pts: N x 3 in original order, op_perm from the tree; leaves in idx order
