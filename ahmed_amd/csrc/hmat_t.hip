@@ -32,60 +32,82 @@ struct ColTask {
   uint32_t pad;
 };
 
-// slots[zoff + c] = sum_r P[r][c] v[base + r].  Lanes form groups of gs
-// (a power of two >= m, at most 64); a group takes one column at a time and
-// its lanes hold rows gl, gl + 64, ... (m <= 256); the group sum is a fixed
-// butterfly of shuffles.
+// slots[zoff + c] = sum_r P[r][c] v[base + r].  Lanes form groups of GS
+// (32 for m > 128, 16 for m > 8, else the power of two >= m); a group takes
+// four columns at a time and its lanes hold rows gl, gl + GS, ... (at most 8
+// passes, m <= 256); the group sums
+// are fixed butterflies of shuffles.  Out-of-range rows and columns read a
+// clamped, valid address (row 0 / the group's first column) and are weighted
+// by zero or not stored, so every load is unconditional and a group's loads
+// are in flight together.
+template <typename TS, typename TV, int GS, int NQ>
+__device__ __forceinline__ void coldot_body(const ColTask& tk, const TS* __restrict__ arena,
+                                            const TV* __restrict__ v, TV* __restrict__ slots,
+                                            TV* __restrict__ out) {
+  constexpr int kPerWave = 64 / GS;
+  constexpr int kGroups = 4 * kPerWave;
+  const int m = (int)tk.m;
+  const int lane = threadIdx.x & 63;
+  const int gl = lane & (GS - 1);
+  const int grp = (int)(threadIdx.x >> 6) * kPerWave + lane / GS;
+  const int nq = (m + GS - 1) / GS;  // row passes, uniform over the workgroup
+  TV vr[NQ];
+  int roff[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int r = gl + GS * q;
+    const bool in = r < m;
+    roff[q] = in ? r : 0;
+    vr[q] = in ? v[tk.base + r] : (TV)0;
+  }
+  const TS* P = arena + tk.data;
+  // the workgroup takes 4 * kGroups consecutive columns per pass; their sums
+  // meet in LDS (out, 4 * kGroups <= 1024 values) and leave in coalesced stores
+  for (uint32_t base = 0; base < tk.W; base += 4u * kGroups) {
+    TV s[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t c = base + (uint32_t)(u * kGroups + grp);
+      const TS* Pc = P + (uint64_t)(c < tk.W ? c : base) * m;
+      s[u] = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (q < nq) s[u] = fma((TV)Pc[roff[q]], vr[q], s[u]);
+    }
+#pragma unroll
+    for (int o = GS / 2; o > 0; o >>= 1)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s[u] += __shfl_xor(s[u], o, 64);
+    if (gl == 0)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) out[u * kGroups + grp] = s[u];
+    __syncthreads();
+    const uint32_t n = min(4u * kGroups, tk.W - base);
+    for (uint32_t t = threadIdx.x; t < n; t += 256) slots[tk.zoff + base + t] = out[t];
+    __syncthreads();
+  }
+}
+
 template <typename TS, typename TV>
 __global__ __launch_bounds__(256) void coldot_kernel(const ColTask* __restrict__ tasks,
                                                      const TS* __restrict__ arena,
                                                      const TV* __restrict__ v,
                                                      TV* __restrict__ slots) {
+  __shared__ TV out[1024];
   const ColTask tk = tasks[blockIdx.x];
-  const int m = (int)tk.m;
-  int gs = 64;
-  if (m <= 32) {
-    gs = 1;
-    while (gs < m) gs <<= 1;
-  }
-  const int lane = threadIdx.x & 63;
-  const int gl = lane & (gs - 1);
-  const int per_wave = 64 / gs;
-  const int grp = (int)(threadIdx.x >> 6) * per_wave + lane / gs;
-  const int ngrp = 4 * per_wave;
-  TV vr[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int r = gl + 64 * q;
-    vr[q] = r < m ? v[tk.base + r] : (TV)0;
-  }
-  const TS* P = arena + tk.data;
-  // four columns per group and pass: their loads are in flight together
-  for (uint32_t c0 = grp; c0 < tk.W; c0 += 4u * ngrp) {
-    TV s[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t c = c0 + (uint32_t)u * ngrp;
-      s[u] = 0;
-      if (c < tk.W) {
-        const TS* Pc = P + (uint64_t)c * m;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int r = gl + 64 * q;
-          if (r < m) s[u] = fma((TV)Pc[r], vr[q], s[u]);
-        }
-      }
-    }
-    for (int o = gs >> 1; o > 0; o >>= 1)
-#pragma unroll
-      for (int u = 0; u < 4; ++u) s[u] += __shfl_xor(s[u], o, 64);
-    if (gl == 0)
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t c = c0 + (uint32_t)u * ngrp;
-        if (c < tk.W) slots[tk.zoff + c] = s[u];
-      }
-  }
+  const uint32_t m = tk.m;
+  if (m > 128)
+    coldot_body<TS, TV, 32, 8>(tk, arena, v, slots, out);
+  else if (m > 8)
+    coldot_body<TS, TV, 16, 8>(tk, arena, v, slots, out);
+  else if (m > 4)
+    coldot_body<TS, TV, 8, 1>(tk, arena, v, slots, out);
+  else if (m > 2)
+    coldot_body<TS, TV, 4, 1>(tk, arena, v, slots, out);
+  else if (m > 1)
+    coldot_body<TS, TV, 2, 1>(tk, arena, v, slots, out);
+  else
+    coldot_body<TS, TV, 1, 1>(tk, arena, v, slots, out);
 }
 
 // out[i] (=|+= alpha *) sum_{j in [off[i], off[i+1])} slots[pos[j]]
