@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "ahm_internal.h"
 #include "ahm_aca.h"
@@ -136,6 +137,99 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
 }
 
 // ------------------------------------------------------------------------
+// 2 <= p <= 16 right-hand sides: the single-RHS kernel's thread mapping and
+// streaming with NP = 8 or 16 accumulators per thread.  At these widths the
+// products are bound by the panel stream, not by arithmetic, and the flat
+// per-row loads of the single-RHS kernel stream at the full rate (the MFMA
+// fragments' 16-row pieces do not); Z comes from the NP-wide interleaved
+// copies of x and t (one NP*8-byte row per column), staged in LDS and read
+// as broadcasts.
+constexpr int kZChunkP = 256;   // columns of Z staged per pass
+
+template <typename TS, typename TV, bool PHASE_B, int NP>
+__global__ __launch_bounds__(kPanelThreads) void panel_gemvp_kernel(
+    const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
+    const TS* __restrict__ arena, const TV* __restrict__ xi, TV* __restrict__ ti, uint32_t t_len,
+    TV* __restrict__ y, uint32_t ldy, TV* __restrict__ part, uint32_t part_len, TV alpha,
+    uint32_t p) {
+  constexpr int LNP = NP == 16 ? 4 : 3;
+  __shared__ TV zs[kZChunkP * NP];   // also the cross-group reduction buffer (256 x NP)
+  const GemvTask tk = tasks[blockIdx.x];
+  const int m = (int)tk.m;
+  const int G = kPanelThreads / m;
+  const int tid = threadIdx.x;
+  const int row = tid % m, g = tid / m;
+  const bool active = g < G;
+  const TS* P = arena + tk.data + row;
+  TV a[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) a[j] = 0;
+  for (uint32_t c0 = 0; c0 < tk.W; c0 += kZChunkP) {
+    const int cw = (int)min((uint32_t)kZChunkP, tk.W - c0);
+    __syncthreads();
+    {
+      // source-list entries first, then the rows (NP threads per row)
+      constexpr int kG = kZChunkP * NP / kPanelThreads;
+      uint32_t src[kG];
+#pragma unroll
+      for (int i = 0; i < kG; ++i) {
+        const int c = (tid + i * kPanelThreads) >> LNP;
+        src[i] = zsrc[tk.zoff + c0 + (c < cw ? c : 0)];
+      }
+#pragma unroll
+      for (int i = 0; i < kG; ++i) {
+        const int e = tid + i * kPanelThreads;
+        const uint32_t sv = src[i];
+        const TV* r = (sv & kFromT) ? ti + (uint64_t)(sv & ~kFromT) * NP : xi + (uint64_t)sv * NP;
+        const TV v = r[e & (NP - 1)];
+        zs[e] = (e >> LNP) < cw ? v : (TV)0;
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const TS* Pc = P + (uint64_t)c0 * m;
+      const uint32_t step = (uint32_t)(G * m);
+      uint32_t off = (uint32_t)g * m;
+      int c = g;
+      for (; c + 3 * G < cw; c += 4 * G, off += 4 * step) {
+        const TV p0 = (TV)Pc[off];
+        const TV p1 = (TV)Pc[off + step];
+        const TV p2 = (TV)Pc[off + 2 * step];
+        const TV p3 = (TV)Pc[off + 3 * step];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+          a[j] = fma(p0, zs[NP * c + j], a[j]);
+          a[j] = fma(p1, zs[NP * (c + G) + j], a[j]);
+          a[j] = fma(p2, zs[NP * (c + 2 * G) + j], a[j]);
+          a[j] = fma(p3, zs[NP * (c + 3 * G) + j], a[j]);
+        }
+      }
+      for (; c < cw; c += G, off += step) {
+        const TV p0 = (TV)Pc[off];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) a[j] = fma(p0, zs[NP * c + j], a[j]);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NP; ++j) zs[j * kPanelThreads + tid] = active ? a[j] : (TV)0;
+  __syncthreads();
+  if (tid < m) {
+    for (uint32_t j = 0; j < p; ++j) {
+      TV s = 0;
+      for (int q = 0; q < G; ++q) s += zs[j * kPanelThreads + q * m + tid];
+      if (tk.partial)
+        part[(uint64_t)j * part_len + tk.out + tid] = s;
+      else if (PHASE_B)
+        y[(uint64_t)j * ldy + tk.out + tid] += alpha * s;
+      else
+        ti[(uint64_t)(tk.out + tid) * NP + j] = s;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
 // p right-hand sides: the same panels as a matrix product on the MFMA units.
 //
 // out(m x p) (+)= P (m x W) * Z (W x p), one task per workgroup and 32
@@ -172,6 +266,35 @@ template <> struct MmAcc<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane / 16) + r; }
 };
 
+// Z chunk into LDS: zs[c][j] = source row of column c (value j of 2^LPW),
+// zero for c >= cw.  Each thread loads its source-list entries, then its values.
+template <typename T, int LPW>
+__device__ __forceinline__ void mm_gather(T (&zs)[kMmChunk][kMmRhs + 1], const uint32_t* __restrict__ zsrc,
+                                          int cw, const T* __restrict__ xg, const T* __restrict__ tg) {
+  constexpr int kPw = 1 << LPW;
+  constexpr int kG = (kMmChunk << LPW) / kPanelThreads;
+  uint32_t src[kG];
+#pragma unroll
+  for (int i = 0; i < kG; ++i) {
+    const int c = (int)(threadIdx.x + i * kPanelThreads) >> LPW;
+    src[i] = zsrc[c < cw ? c : 0];
+  }
+  T val[kG];
+#pragma unroll
+  for (int i = 0; i < kG; ++i) {
+    const int j = (int)(threadIdx.x + i * kPanelThreads) & (kPw - 1);
+    const uint32_t sv = src[i];
+    const T* row = (sv & kFromT) ? tg + (uint64_t)(sv & ~kFromT) * kPw : xg + (uint64_t)sv * kPw;
+    val[i] = row[j];
+  }
+#pragma unroll
+  for (int i = 0; i < kG; ++i) {
+    const int e = threadIdx.x + i * kPanelThreads;
+    const int c = e >> LPW;
+    zs[c][e & (kPw - 1)] = c < cw ? val[i] : (T)0;
+  }
+}
+
 template <typename T, bool PHASE_B>
 __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
     const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
@@ -188,7 +311,6 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nrt = (m + 15) / 16;
   const bool two = pw > 16 && p - rhs0 > 16;  // second 16-column tile needed
-  const int lpw = pw > 16 ? 5 : 4;
   acc_t acc[4][2];
   for (int a = 0; a < 4; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
@@ -197,16 +319,16 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
   for (uint32_t c0 = 0; c0 < tk.W; c0 += kMmChunk) {
     const int cw = (int)min((uint32_t)kMmChunk, tk.W - c0);
     __syncthreads();
-    // gather: pw consecutive threads read the contiguous run of one source row
-    for (int e = threadIdx.x; e < (kMmChunk << lpw); e += kPanelThreads) {
-      const int c = e >> lpw, j = e & (pw - 1);
-      T v = 0;
-      if (c < cw) {
-        const uint32_t s = zsrc[tk.zoff + c0 + c];
-        v = (s & kFromT) ? tg[(uint64_t)(s & ~kFromT) * pw + j] : xg[(uint64_t)s * pw + j];
-      }
-      zs[c][j] = v;
-    }
+    // gather: pw consecutive threads read the contiguous run of one source
+    // row.  All source-list entries of the thread first, then all values, so
+    // the two rounds of loads are each in flight together; columns beyond cw
+    // read column 0's (valid) source and store zero.
+    if (pw > 16)
+      mm_gather<T, 5>(zs, zsrc + tk.zoff + c0, cw, xg, tg);
+    else if (pw > 8)
+      mm_gather<T, 4>(zs, zsrc + tk.zoff + c0, cw, xg, tg);
+    else
+      mm_gather<T, 3>(zs, zsrc + tk.zoff + c0, cw, xg, tg);
     __syncthreads();
     // 4 k-steps (16 columns) per round: all panel loads first, then MFMAs
     for (int k0 = 0; k0 < cw; k0 += 16) {
@@ -217,14 +339,18 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
         const T* Pk = P + (uint64_t)(c0 + k) * m;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
+          // unconditional load from a valid address (row 0 of column c0),
+          // zero outside the task
           const int row = (w + 4 * q) * 16 + li;
-          a[st][q] = (row < m && k < cw) ? Pk[row] : (T)0;
+          const bool in = row < m && k < cw;
+          const T v = (in ? Pk : P + (uint64_t)c0 * m)[in ? row : 0];
+          a[st][q] = in ? v : (T)0;
         }
       }
 #pragma unroll
       for (int st = 0; st < 4; ++st) {
         const int k = k0 + 4 * st + lk;   // < kMmChunk; zero-filled beyond cw
-        const T b0 = zs[k][li];
+        const T b0 = li < pw ? zs[k][li] : (T)0;   // pw = 8: columns 8-15 unused
         const T b1 = zs[k][16 + li];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -274,7 +400,7 @@ __global__ __launch_bounds__(256) void interleave_kernel(const T* __restrict__ X
     tile[jj][ii] = (jj < pw && rhs < p && i < n) ? X[(uint64_t)rhs * ldx + i] : (T)0;
   }
   __syncthreads();
-  const int lpw = pw > 16 ? 5 : 4;
+  const int lpw = pw > 16 ? 5 : pw > 8 ? 4 : 3;
   for (int e = threadIdx.x; e < (64 << lpw); e += 256) {
     const int ii = e >> lpw, jj = e & (pw - 1);
     const uint32_t i = i0 + ii;
@@ -519,8 +645,8 @@ static int upload_vec(DeviceBuf& b, const std::vector<T>& v, hipStream_t s) {
 }
 
 // multi-RHS products keep x and t interleaved in groups of 32 right-hand
-// sides (16 when p <= 16): rhs_width values per row and group
-static uint32_t rhs_width(uint32_t p) { return p <= 16 ? 16u : (uint32_t)kMmRhs; }
+// sides (16 when p <= 16, 8 when p <= 8): rhs_width values per row and group
+static uint32_t rhs_width(uint32_t p) { return p <= 8 ? 8u : p <= 16 ? 16u : (uint32_t)kMmRhs; }
 static uint32_t rhs_slots(uint32_t p) {
   return p == 1 ? 1u : ((p + kMmRhs - 1) / kMmRhs) * rhs_width(p);
 }
@@ -570,28 +696,54 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
                          (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
                          (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
   } else {
-    // p >= 2: MFMA panel products, 32 right-hand sides per workgroup, on
-    // interleaved copies of x and t
+    // p >= 2 on interleaved copies of x and t: p <= 16 with the streaming
+    // kernel, wider with MFMA panel products (32 right-hand sides per
+    // workgroup)
     const uint32_t ng = (p + kMmRhs - 1) / kMmRhs;
     const int pw = (int)rhs_width(p);
     TV* xi = (TV*)h->xbuf.p;
     hipLaunchKernelGGL((interleave_kernel<TV>), dim3((h->n_cols + 63) / 64, ng), dim3(256), 0, s, X,
                        ldx, h->n_cols, p, pw, xi);
-    if (h->n_a)
-      hipLaunchKernelGGL((panel_mm_kernel<TV, false>), dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,
-                         (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
-                         (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, pw, Y, ldy, part,
-                         h->part_len, alpha, p);
-    if (h->n_ra)
-      hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, p), dim3(kPanelThreads), 0, s,
-                         (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha,
-                         pw);
-    if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
-    if (h->n_b)
-      hipLaunchKernelGGL((panel_mm_kernel<TV, true>), dim3(h->n_b, ng), dim3(kPanelThreads), 0, s,
-                         (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
-                         (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, pw, Y, ldy, part,
-                         h->part_len, alpha, p);
+    if (pw <= 16) {
+      auto run = [&](auto np) {
+        constexpr int NP = decltype(np)::value;
+        if (h->n_a)
+          hipLaunchKernelGGL((panel_gemvp_kernel<TS, TV, false, NP>), dim3(h->n_a), dim3(kPanelThreads), 0,
+                             s, (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
+                             (const TS*)h->arena.p, (const TV*)xi, t, h->t_len, Y, ldy, part,
+                             h->part_len, alpha, p);
+        if (h->n_ra)
+          hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, p), dim3(kPanelThreads), 0, s,
+                             (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t,
+                             h->t_len, alpha, pw);
+        if (ev_mid) (void)hipEventRecord(ev_mid, s);
+        if (h->n_b)
+          hipLaunchKernelGGL((panel_gemvp_kernel<TS, TV, true, NP>), dim3(h->n_b), dim3(kPanelThreads), 0,
+                             s, (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
+                             (const TS*)h->arena.p, (const TV*)xi, t, h->t_len, Y, ldy, part,
+                             h->part_len, alpha, p);
+      };
+      if (pw == 8)
+        run(std::integral_constant<int, 8>());
+      else
+        run(std::integral_constant<int, 16>());
+    } else {
+      if (h->n_a)
+        hipLaunchKernelGGL((panel_mm_kernel<TV, false>), dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,
+                           (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
+                           (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, pw, Y, ldy, part,
+                           h->part_len, alpha, p);
+      if (h->n_ra)
+        hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, p), dim3(kPanelThreads), 0, s,
+                           (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha,
+                           pw);
+      if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
+      if (h->n_b)
+        hipLaunchKernelGGL((panel_mm_kernel<TV, true>), dim3(h->n_b, ng), dim3(kPanelThreads), 0, s,
+                           (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
+                           (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, pw, Y, ldy, part,
+                           h->part_len, alpha, p);
+    }
   }
   if (h->n_r)
     hipLaunchKernelGGL((chunk_reduce_kernel<TV, false>), dim3(h->n_r, p), dim3(kPanelThreads), 0, s,

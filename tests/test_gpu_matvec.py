@@ -138,8 +138,9 @@ def test_rank_zero_and_all_dense_edge_cases(ctx):
         _lib.HMatrix.upload(ctx, N, N, bad, data)
 
 
-@pytest.mark.parametrize("p", [3, 17, 40])
-def test_multi_rhs_mfma_odd_counts_fp64_and_fp32(ctx, p):
+# 3, 8: 8-wide streaming kernel; 12, 16: 16-wide; 17, 40: MFMA panel products
+@pytest.mark.parametrize("p", [3, 8, 12, 16, 17, 40])
+def test_multi_rhs_odd_counts_fp64_and_fp32(ctx, p):
     _, T, H = sphere_hmatrix(16384, 64, 1e-8)
     D = upload(ctx, H)
     rng = np.random.default_rng(p)
@@ -153,7 +154,7 @@ def test_multi_rhs_mfma_odd_counts_fp64_and_fp32(ctx, p):
     Y32 = dev(np.zeros((p, H.N)), torch.float32)
     D32.matvec(dev(X.T.copy(), torch.float32), Y32, 1.0)
     err = relerr(Y32.cpu().numpy().T.astype(np.float64), H.matmat(X))
-    print(f"[gpu] FP32 MFMA p={p}: rel.err vs FP64 {err:.2e}")
+    print(f"[gpu] FP32 p={p}: rel.err vs FP64 {err:.2e}")
     # FP32 values, right-hand sides and accumulation: 1e-6..2e-6 (max-norm)
     # measured; the sum order follows the task shapes, hence the margin
     assert err <= 5e-6
