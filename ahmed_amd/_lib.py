@@ -99,7 +99,30 @@ SIGNATURES = {
     "ahm_hmat_matvec_t_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
     "ahm_hmat_matvec_profile": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i, _u32, C.POINTER(_d),
                                      C.POINTER(_d)]),
+    "ahm_trih_upload": (_i, [_p, _u32, C.c_char, _u32, _p, _p, _i, C.POINTER(_p)]),
+    "ahm_trih_destroy": (_i, [_p]),
+    "ahm_trih_solve": (_i, [_p, _p, _u32, _u32]),
+    "ahm_trih_solve_host": (_i, [_p, _p, _u32, _u32]),
+    "ahm_trih_info": (_i, [_p, C.POINTER(_u32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
 }
+
+LEAF_LTM, LEAF_UTM = 1, 2
+
+
+def factor_leaves(layout) -> tuple:
+    """(leaves, data) for ahm_trih_upload from a factor layout: per leaf b1,
+    n1, b2, n2, offset and kind (-2 absent, -1 dense, k >= 0 low rank, -3
+    packed unit lower with the row permutation on its diagonal, -4 packed
+    upper), the mblock storage of the reference's L and U arrays."""
+    kind = np.asarray(layout["kind"])
+    keep = kind != -2
+    L = np.zeros(int(keep.sum()), LEAF_DTYPE)
+    for f in ("b1", "n1", "b2", "n2", "offset"):
+        L[f] = np.asarray(layout[f])[keep]
+    k = kind[keep]
+    L["rank"] = np.where(k >= 0, k, -1)
+    L["flags"] = np.where(k == -3, LEAF_LTM, np.where(k == -4, LEAF_UTM, 0))
+    return L, np.ascontiguousarray(layout["data"], np.float64)
 
 
 def load():
@@ -237,6 +260,53 @@ def leaf_array(b1, n1, b2, n2, rank, offset) -> np.ndarray:
     a["b1"], a["n1"], a["b2"], a["n2"] = b1, n1, b2, n2
     a["rank"], a["offset"] = rank, offset
     return a
+
+
+class TriFactor:
+    """Triangular H-matrix factor on the device (ahm_trih): uplo 'L' (unit
+    lower, LtHGeM_solve) or 'U' (upper, UtHGeM_solve)."""
+
+    def __init__(self, ctx: Context, handle, n: int, uplo: str):
+        self.ctx, self.h, self.n, self.uplo = ctx, handle, n, uplo
+
+    @classmethod
+    def upload(cls, ctx: Context, n: int, uplo: str, leaves: np.ndarray, data: np.ndarray) -> "TriFactor":
+        leaves = np.ascontiguousarray(leaves, LEAF_DTYPE)
+        data = np.ascontiguousarray(data, np.float64)
+        h = _p()
+        ctx.bind_stream()
+        check(load().ahm_trih_upload(ctx.h, n, uplo.encode(), len(leaves), leaves.ctypes.data_as(_p),
+                                     data.ctypes.data_as(_p), F64, C.byref(h)), "ahm_trih_upload")
+        return cls(ctx, h, n, uplo)
+
+    @classmethod
+    def from_layout(cls, ctx: Context, n: int, uplo: str, layout) -> "TriFactor":
+        leaves, data = factor_leaves(layout)
+        return cls.upload(ctx, n, uplo, leaves, data)
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().ahm_trih_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        nd, ni, nv = _u32(), C.c_uint64(), C.c_uint64()
+        check(load().ahm_trih_info(self.h, C.byref(nd), C.byref(ni), C.byref(nv)), "ahm_trih_info")
+        return {"n_diag": nd.value, "n_items": ni.value, "nvals": nv.value}
+
+    def solve(self, B):
+        """B <- op^{-1} B in place: a float64 device tensor of n values, or
+        (p, n) for p right-hand sides (row j = right-hand side j)."""
+        if B.dim() == 1:
+            p, ldb = 1, B.shape[0]
+        else:
+            p, ldb = B.shape[0], B.stride(0)
+        self.ctx.bind_stream()
+        check(load().ahm_trih_solve(self.h, _p(B.data_ptr()), ldb, p), "ahm_trih_solve")
+        return B
 
 
 class HMatrix:

@@ -1,0 +1,712 @@
+// trih.hip -- forward / backward substitution with triangular H-matrix factors.
+//
+// Replaces the reference's LtHGeM_solve_ / UtHGeM_solve_ (H/TH_solve.cpp:
+// 18-91) with the leaf solves mblock::ltr_solve (H/mblock_D.cpp:1183-1205:
+// gather by the row permutation, then dtpsv 'L','N','U', Include/blas.h:
+// 3749-3755) and mblock::utr_solve (Include/mblock.h:694-697: dtpsv 'U','N',
+// 'N', Include/blas.h:3930-3936).
+//
+// The reference recurses over the block tree: solve the first diagonal son,
+// subtract the off-diagonal son times the solved part from the rest, solve
+// the second diagonal son.  Only the diagonal leaves form a chain; an
+// off-diagonal leaf can run as soon as the diagonal leaves covering its
+// columns are solved, and must finish before the diagonal leaves covering
+// its rows start.  Here that is one persistent launch over a work queue:
+//   S  solve one diagonal leaf (one workgroup; a wavefront substitutes 64
+//      rows at a time from a 64x64 triangle staged in LDS, the workgroup
+//      applies the rectangle below (above, for 'U') the triangle);
+//   D  a tile (<= 256 rows) of a dense off-diagonal leaf: acc[rows] -= D x;
+//   A  a column chunk of V^T x of a low-rank leaf, into its own slot;
+//   B  a row tile of a low-rank leaf: acc[rows] -= U t, t = fixed-order sum
+//      of the leaf's A slots.
+// Workgroups take items in queue order (one atomic counter).  An item only
+// waits for items earlier in the queue (the host builds the order and checks
+// it by running the queue sequentially), so every wait ends: the items it
+// waits for were taken by workgroups that are already running.  Waits are
+// bounded anyway; a wait that runs out sets an abort flag that ends every
+// workgroup and the call returns an error.
+//
+// acc is the right-hand side vector itself (solved in place).  D and B items
+// subtract with FP64 atomics, so the order of the contributions to one row
+// (not their values) depends on timing; S items, A slots and t sums are in a
+// fixed order.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "ahm_internal.h"
+
+struct ahm_trih_s {
+  ahm_ctx ctx = nullptr;
+  uint32_t n = 0;
+  bool lower = true;
+  uint32_t n_diag = 0, n_lr = 0;
+  uint64_t n_items = 0, nvals = 0, arena_vals = 0;
+  ahm::DeviceBuf arena, items, perm, pend0, counters, tpart;
+  uint64_t t_len = 0;
+  uint32_t grid = 0;
+};
+
+namespace ahm {
+
+namespace {
+
+constexpr uint32_t kItemS = 0, kItemA = 1, kItemB = 2, kItemD = 3;
+constexpr int kTri = 64;             // rows substituted per wavefront pass
+constexpr uint32_t kMaxDiag = 2048;  // largest diagonal leaf (LDS vector)
+constexpr uint32_t kMaxW = 2048;     // columns per gemv item (LDS vector)
+constexpr uint64_t kItemValues = 48 * 1024;
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+struct TriItem {
+  uint64_t data;   // arena offset: panel (A/B/D) or the diagonal block (S)
+  uint32_t type;
+  uint32_t m;      // rows (A/B/D), or the diagonal leaf's size (S)
+  uint32_t W;      // columns (A/B/D)
+  uint32_t ld;     // column stride of the panel (A/B/D)
+  uint32_t in;     // A/D: first column in acc; B: slot of chunk 0, rank 0
+  uint32_t out;    // A: first slot; B/D: first row in acc; S: first row
+  uint32_t wait;   // S: diagonal index; A/D: diagonal that must be solved; B: low-rank leaf
+  uint32_t wait2;  // S: previous diagonal in solve order (kNone for the first); B: A-item count
+  uint32_t sig0;   // B/D: first diagonal whose pending count drops; A: low-rank leaf
+  uint32_t sig1;   // B/D: last such diagonal
+  uint32_t aux;    // B: rank k (slot stride); S: permutation offset (lower)
+  uint32_t parts;  // B: slots summed per rank entry (the leaf's column chunks)
+};
+
+// counters layout: [0] queue head, [1] abort flag, then solved[n_diag],
+// tdone[n_lr], pending[n_diag]
+struct Counters {
+  uint32_t* head;
+  uint32_t* abort;
+  uint32_t* solved;
+  uint32_t* tdone;
+  uint32_t* pending;
+};
+
+__device__ __forceinline__ uint32_t load_acq(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// spin until *p == v (eq) or *p >= v; false if the abort flag is (or gets) set
+__device__ bool spin_until(const uint32_t* p, uint32_t v, bool eq, uint32_t* abort) {
+  for (uint32_t it = 0;; ++it) {
+    const uint32_t c = load_acq(p);
+    if (eq ? c == v : c >= v) return true;
+    if ((it & 31) == 31 && load_acq(abort)) return false;
+    if (it > kSpinLimit) {
+      __hip_atomic_store(abort, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const unsigned long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+struct TriSmem {
+  double z[kMaxDiag];          // S: the leaf's vector; A/B/D: the input slice
+  double tri[kTri * kTri];     // S: one triangle; A/B/D: cross-group partial sums
+  uint32_t item;
+  uint32_t ok;
+};
+
+// out(r) = sum_c P[r + c*ld] * z[c] for r < m (m <= 256, W <= kMaxW), z in
+// LDS; thread t < G*m owns row t % m and columns g, g+G, ... (G = 256/m);
+// returns the row sum on threads t < m (fixed-order reduction)
+__device__ __forceinline__ double tile_gemv(const double* __restrict__ P, uint32_t m, uint32_t W,
+                                            uint32_t ld, const double* z, double* red) {
+  const int tid = threadIdx.x;
+  const int G = kPanelThreads / (int)m;
+  const int row = tid % (int)m, g = tid / (int)m;
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  if (g < G) {
+    const double* Pr = P + row;
+    int c = g;
+    for (; c + 3 * G < (int)W; c += 4 * G) {
+      a0 = fma(Pr[(uint64_t)c * ld], z[c], a0);
+      a1 = fma(Pr[(uint64_t)(c + G) * ld], z[c + G], a1);
+      a2 = fma(Pr[(uint64_t)(c + 2 * G) * ld], z[c + 2 * G], a2);
+      a3 = fma(Pr[(uint64_t)(c + 3 * G) * ld], z[c + 3 * G], a3);
+    }
+    for (; c < (int)W; c += G) a0 = fma(Pr[(uint64_t)c * ld], z[c], a0);
+  }
+  red[tid] = g < G ? (a0 + a1) + (a2 + a3) : 0.0;
+  __syncthreads();
+  double s = 0;
+  if (tid < (int)m)
+    for (int q = 0; q < G; ++q) s += red[q * (int)m + tid];
+  return s;
+}
+
+// one diagonal leaf, lower: z = P^T-gathered slice, unit lower substitution
+__device__ void solve_lower(const TriItem& it, const double* __restrict__ arena,
+                            const uint32_t* __restrict__ perm, double* acc, TriSmem& sm) {
+  const int tid = threadIdx.x;
+  const uint32_t n = it.m, s0 = it.out;
+  const uint32_t np = (n + kTri - 1) / kTri;
+  for (uint32_t r = tid; r < np * kTri; r += kPanelThreads)
+    sm.z[r] = r < n ? acc[s0 + perm[it.aux + r]] : 0.0;
+  const double* blk = arena + it.data;
+  for (uint32_t p = 0; p < np; ++p) {
+    const uint32_t c0 = p * kTri;
+    const uint32_t below = n > c0 + kTri ? n - c0 - kTri : 0;
+    __syncthreads();
+    for (int e = tid; e < kTri * kTri; e += kPanelThreads) sm.tri[e] = blk[e];
+    __syncthreads();
+    if (tid < kTri) {
+      double zr = sm.z[c0 + tid];
+#pragma unroll
+      for (int j = 0; j < kTri; ++j) {
+        const double xj = readlane_d(zr, j);
+        if (tid > j) zr = fma(-sm.tri[j * kTri + tid], xj, zr);
+      }
+      sm.z[c0 + tid] = zr;
+    }
+    __syncthreads();
+    const double* rect = blk + kTri * kTri;
+    for (uint32_t r0 = 0; r0 < below; r0 += kPanelThreads) {
+      const uint32_t m = min(below - r0, (uint32_t)kPanelThreads);
+      const double s = tile_gemv(rect + r0, m, kTri, below, sm.z + c0, sm.tri);
+      __syncthreads();
+      if (tid < (int)m) sm.z[c0 + kTri + r0 + tid] -= s;
+      __syncthreads();
+    }
+    blk = rect + (uint64_t)below * kTri;
+  }
+  __syncthreads();
+  for (uint32_t r = tid; r < n; r += kPanelThreads) acc[s0 + r] = sm.z[r];
+}
+
+// one diagonal leaf, upper: backward substitution, reciprocal diagonal
+__device__ void solve_upper(const TriItem& it, const double* __restrict__ arena, double* acc,
+                            TriSmem& sm) {
+  const int tid = threadIdx.x;
+  const uint32_t n = it.m, s0 = it.out;
+  const uint32_t np = (n + kTri - 1) / kTri;
+  for (uint32_t r = tid; r < np * kTri; r += kPanelThreads) sm.z[r] = r < n ? acc[s0 + r] : 0.0;
+  // panels are stored last first: triangle, then the rectangle above it
+  const double* blk = arena + it.data;
+  for (uint32_t pp = np; pp-- > 0;) {
+    const uint32_t c0 = pp * kTri;
+    __syncthreads();
+    for (int e = tid; e < kTri * kTri; e += kPanelThreads) sm.tri[e] = blk[e];
+    __syncthreads();
+    if (tid < kTri) {
+      double zr = sm.z[c0 + tid];
+#pragma unroll
+      for (int j = kTri - 1; j >= 0; --j) {
+        if (tid == j) zr *= sm.tri[j * kTri + j];
+        const double xj = readlane_d(zr, j);
+        if (tid < j) zr = fma(-sm.tri[j * kTri + tid], xj, zr);
+      }
+      sm.z[c0 + tid] = zr;
+    }
+    __syncthreads();
+    const double* rect = blk + kTri * kTri;
+    for (uint32_t r0 = 0; r0 < c0; r0 += kPanelThreads) {
+      const uint32_t m = min(c0 - r0, (uint32_t)kPanelThreads);
+      const double s = tile_gemv(rect + r0, m, kTri, c0, sm.z + c0, sm.tri);
+      __syncthreads();
+      if (tid < (int)m) sm.z[r0 + tid] -= s;
+      __syncthreads();
+    }
+    blk = rect + (uint64_t)c0 * kTri;
+  }
+  __syncthreads();
+  for (uint32_t r = tid; r < n; r += kPanelThreads) acc[s0 + r] = sm.z[r];
+}
+
+__global__ __launch_bounds__(kPanelThreads) void trih_kernel(
+    const TriItem* __restrict__ items, uint32_t n_items, const double* __restrict__ arena,
+    const uint32_t* __restrict__ perm, double* acc, double* tpart, Counters cn, int lower) {
+  __shared__ TriSmem sm;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) {
+      const uint32_t i = __hip_atomic_fetch_add(cn.head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      sm.item = (i < n_items && !load_acq(cn.abort)) ? i : kNone;
+    }
+    __syncthreads();
+    const uint32_t idx = sm.item;
+    __syncthreads();
+    if (idx == kNone) return;
+    const TriItem it = items[idx];
+    if (tid == 0) {
+      bool ok;
+      if (it.type == kItemS) {
+        ok = spin_until(cn.pending + it.wait, 0u, true, cn.abort);
+        if (ok && it.wait2 != kNone) ok = spin_until(cn.solved + it.wait2, 1u, false, cn.abort);
+      } else if (it.type == kItemB) {
+        ok = spin_until(cn.tdone + it.wait, it.wait2, false, cn.abort);
+      } else {
+        ok = spin_until(cn.solved + it.wait, 1u, false, cn.abort);
+      }
+      sm.ok = ok ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!sm.ok) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (it.type == kItemS) {
+      if (lower)
+        solve_lower(it, arena, perm, acc, sm);
+      else
+        solve_upper(it, arena, acc, sm);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      if (tid == 0) __hip_atomic_store(cn.solved + it.wait, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      continue;
+    }
+    // stage the input slice
+    if (it.type == kItemB) {
+      const uint32_t k = it.aux, nparts = it.parts;
+      for (uint32_t l = tid; l < it.W; l += kPanelThreads) {
+        double t = 0;
+        for (uint32_t c = 0; c < nparts; ++c) t += tpart[it.in + (uint64_t)c * k + l];
+        sm.z[l] = t;
+      }
+    } else {
+      for (uint32_t c = tid; c < it.W; c += kPanelThreads) sm.z[c] = acc[it.in + c];
+    }
+    __syncthreads();
+    const double s = tile_gemv(arena + it.data, it.m, it.W, it.ld, sm.z, sm.tri);
+    if (tid < (int)it.m) {
+      if (it.type == kItemA)
+        tpart[it.out + tid] = s;
+      else
+        unsafeAtomicAdd(acc + it.out + tid, -s);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) {
+      if (it.type == kItemA) {
+        __hip_atomic_fetch_add(cn.tdone + it.sig0, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        for (uint32_t d = it.sig0; d <= it.sig1; ++d)
+          __hip_atomic_fetch_sub(cn.pending + d, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+}
+
+// counters <- (0, 0, solved = 0, tdone = 0, pending = pend0)
+__global__ void trih_reset_kernel(uint32_t* counters, uint32_t n_zero, const uint32_t* pend0,
+                                  uint32_t n_diag) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_zero) counters[i] = 0;
+  else if (i < n_zero + n_diag) counters[i] = pend0[i - n_zero];
+}
+
+// ---------------------------------------------------------------- host plan
+
+struct Diag {
+  uint32_t leaf, s, n;
+};
+
+}  // namespace
+
+// The work queue of one factor, built and checked on the host.
+struct TriPlan {
+  std::vector<double> arena;
+  std::vector<uint32_t> perm;
+  std::vector<TriItem> queue;
+  std::vector<uint32_t> pend0;   // per diagonal: B/D items touching its rows
+  uint32_t n_diag = 0, n_lr = 0;
+  uint64_t t_len = 0, nvals = 0;
+};
+
+int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* leaves, const double* src,
+               TriPlan& P) {
+  const char uplo = lower ? 'L' : 'U';
+  const uint32_t dflag = lower ? AHM_LEAF_LTM : AHM_LEAF_UTM;
+
+  // diagonal leaves, sorted by first row; they must tile [0, n)
+  std::vector<Diag> dg;
+  for (uint32_t l = 0; l < n_leaves; ++l) {
+    const ahm_leaf_desc& d = leaves[l];
+    if (d.flags == dflag) {
+      if (d.b1 != d.b2 || d.n1 != d.n2 || d.n1 == 0)
+        return fail(AHM_ERR_INVALID, "ahm_trih_upload: diagonal leaf %u is not square on the diagonal", l);
+      if (d.n1 > kMaxDiag)
+        return fail(AHM_ERR_UNSUPPORTED, "ahm_trih_upload: diagonal leaf of %u rows (at most %u)", d.n1,
+                    kMaxDiag);
+      dg.push_back({l, d.b1, d.n1});
+    } else if (d.flags != 0) {
+      return fail(AHM_ERR_INVALID, "ahm_trih_upload: leaf %u has flags %u (expected 0 or %u for '%c')", l,
+                  d.flags, dflag, uplo);
+    }
+  }
+  std::sort(dg.begin(), dg.end(), [](const Diag& a, const Diag& b) { return a.s < b.s; });
+  {
+    uint32_t pos = 0;
+    for (const Diag& d : dg) {
+      if (d.s != pos) return fail(AHM_ERR_INVALID, "ahm_trih_upload: diagonal leaves do not tile [0, %u)", n);
+      pos += d.n;
+    }
+    if (pos != n) return fail(AHM_ERR_INVALID, "ahm_trih_upload: diagonal leaves do not tile [0, %u)", n);
+  }
+  const uint32_t D = (uint32_t)dg.size();
+  std::vector<uint32_t> row_diag(n);
+  for (uint32_t i = 0; i < D; ++i)
+    for (uint32_t r = 0; r < dg[i].n; ++r) row_diag[dg[i].s + r] = i;
+  // solve order: step(d) = d (lower) or D-1-d (upper)
+  auto step = [&](uint32_t d) { return lower ? d : D - 1 - d; };
+
+  std::vector<double>& ar = P.arena;
+  std::vector<uint32_t>& perm = P.perm;
+  std::vector<TriItem> its;
+  std::vector<uint32_t> need;   // step that needs the item (non-S items)
+  std::vector<uint32_t> cls;    // ordering class: A 0, D 1, B 2
+  uint64_t nvals = 0;
+
+  // diagonal leaves: 64-row panels, each a 64x64 triangle followed by the
+  // rectangle below it (lower) or above it (upper, panels last first)
+  std::vector<TriItem> sitems(D);
+  for (uint32_t i = 0; i < D; ++i) {
+    const ahm_leaf_desc& d = leaves[dg[i].leaf];
+    const uint32_t nn = d.n1;
+    const double* pk = src + d.offset;
+    nvals += (uint64_t)nn * (nn + 1) / 2;
+    TriItem it{};
+    it.type = kItemS;
+    it.m = nn;
+    it.out = d.b1;
+    it.data = ar.size();
+    it.wait = i;
+    const uint32_t np = (nn + kTri - 1) / kTri;
+    // packed column j starts at: lower sum_{q<j} (n-q); upper j(j+1)/2
+    auto lo_at = [&](uint32_t r, uint32_t c) -> double {  // r > c
+      const uint64_t o = (uint64_t)c * nn - (uint64_t)c * (c - 1) / 2;
+      return pk[o + (r - c)];
+    };
+    auto up_at = [&](uint32_t r, uint32_t c) -> double {  // r <= c
+      return pk[(uint64_t)c * (c + 1) / 2 + r];
+    };
+    if (lower) {
+      it.aux = (uint32_t)perm.size();
+      std::vector<char> seen(nn, 0);
+      for (uint32_t j = 0; j < nn; ++j) {
+        const double v = pk[(uint64_t)j * nn - (uint64_t)j * (j - 1) / 2];
+        const uint32_t ip = (uint32_t)v;
+        if (!(v >= 0.0) || (double)ip != v || ip >= nn || seen[ip])
+          return fail(AHM_ERR_INVALID, "ahm_trih_upload: diagonal leaf at row %u: entry %u is not a permutation index",
+                      d.b1, j);
+        seen[ip] = 1;
+        perm.push_back(ip);
+      }
+      for (uint32_t p = 0; p < np; ++p) {
+        const uint32_t c0 = p * kTri;
+        const size_t base = ar.size();
+        ar.resize(base + kTri * kTri, 0.0);
+        for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j)
+          for (uint32_t r = j + 1; r < kTri && c0 + r < nn; ++r) ar[base + j * kTri + r] = lo_at(c0 + r, c0 + j);
+        const uint32_t below = nn > c0 + kTri ? nn - c0 - kTri : 0;
+        const size_t rb = ar.size();
+        ar.resize(rb + (size_t)below * kTri, 0.0);
+        for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j)
+          for (uint32_t r = 0; r < below; ++r) ar[rb + (size_t)j * below + r] = lo_at(c0 + kTri + r, c0 + j);
+      }
+    } else {
+      for (uint32_t pp = np; pp-- > 0;) {
+        const uint32_t c0 = pp * kTri;
+        const size_t base = ar.size();
+        ar.resize(base + kTri * kTri, 0.0);
+        for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j) {
+          for (uint32_t r = 0; r < j; ++r) ar[base + j * kTri + r] = up_at(c0 + r, c0 + j);
+          const double dj = up_at(c0 + j, c0 + j);
+          if (dj == 0.0)
+            return fail(AHM_ERR_INVALID, "ahm_trih_upload: zero on the diagonal of the leaf at row %u", d.b1);
+          ar[base + j * kTri + j] = 1.0 / dj;
+        }
+        const size_t rb = ar.size();
+        ar.resize(rb + (size_t)c0 * kTri, 0.0);
+        for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j)
+          for (uint32_t r = 0; r < c0; ++r) ar[rb + (size_t)j * c0 + r] = up_at(r, c0 + j);
+      }
+    }
+    it.wait2 = kNone;
+    sitems[i] = it;
+  }
+  for (uint32_t i = 0; i < D; ++i) {
+    // previous diagonal in solve order
+    if (lower && i > 0) sitems[i].wait2 = i - 1;
+    if (!lower && i + 1 < D) sitems[i].wait2 = i + 1;
+  }
+
+  // off-diagonal leaves
+  std::vector<uint32_t> a_count;  // per low-rank leaf
+  uint64_t t_len = 0;
+  for (uint32_t l = 0; l < n_leaves; ++l) {
+    const ahm_leaf_desc& d = leaves[l];
+    if (d.flags) continue;
+    const uint64_t r_end = (uint64_t)d.b1 + d.n1, c_end = (uint64_t)d.b2 + d.n2;
+    if (r_end > n || c_end > n) return fail(AHM_ERR_INVALID, "ahm_trih_upload: leaf %u outside [0, %u)", l, n);
+    if (d.n1 == 0 || d.n2 == 0) continue;
+    if (lower ? c_end > d.b1 : r_end > d.b2)
+      return fail(AHM_ERR_INVALID, "ahm_trih_upload: leaf %u is not strictly %s the diagonal", l,
+                  lower ? "below" : "above");
+    // diagonal whose solve completes a column range [c0, c1)
+    auto ready_of = [&](uint32_t c0, uint32_t c1) { return lower ? row_diag[c1 - 1] : row_diag[c0]; };
+    // diagonals touched by rows [r0, r1), and the step needing them first
+    auto rows_of = [&](uint32_t r0, uint32_t r1, uint32_t& d0, uint32_t& d1) {
+      d0 = row_diag[r0];
+      d1 = row_diag[r1 - 1];
+      return lower ? step(d0) : step(d1);
+    };
+    if (d.rank < 0) {
+      nvals += (uint64_t)d.n1 * d.n2;
+      const size_t base = ar.size();
+      ar.insert(ar.end(), src + d.offset, src + d.offset + (size_t)d.n1 * d.n2);
+      const uint32_t mrow = std::min<uint32_t>(d.n1, kPanelThreads);
+      const uint32_t wc = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(kMaxW, kItemValues / mrow));
+      for (uint32_t r0 = 0; r0 < d.n1; r0 += kPanelThreads) {
+        const uint32_t m = std::min<uint32_t>(kPanelThreads, d.n1 - r0);
+        for (uint32_t c0 = 0; c0 < d.n2; c0 += wc) {
+          const uint32_t W = std::min(wc, d.n2 - c0);
+          TriItem it{};
+          it.type = kItemD;
+          it.data = base + r0 + (uint64_t)c0 * d.n1;
+          it.m = m;
+          it.W = W;
+          it.ld = d.n1;
+          it.in = d.b2 + c0;
+          it.out = d.b1 + r0;
+          it.wait = ready_of(d.b2 + c0, d.b2 + c0 + W);
+          need.push_back(rows_of(d.b1 + r0, d.b1 + r0 + m, it.sig0, it.sig1));
+          cls.push_back(1);
+          its.push_back(it);
+        }
+      }
+    } else if (d.rank > 0) {
+      const uint32_t k = (uint32_t)d.rank;
+      if (k > kMaxW) return fail(AHM_ERR_UNSUPPORTED, "ahm_trih_upload: rank %u above %u", k, kMaxW);
+      nvals += (uint64_t)k * (d.n1 + d.n2);
+      const uint32_t lr = (uint32_t)a_count.size();
+      const double* U = src + d.offset;
+      const double* V = U + (size_t)k * d.n1;
+      const size_t ub = ar.size();
+      ar.insert(ar.end(), U, U + (size_t)k * d.n1);
+      const size_t vb = ar.size();
+      ar.resize(vb + (size_t)k * d.n2);
+      for (uint32_t c = 0; c < d.n2; ++c)
+        for (uint32_t q = 0; q < k; ++q) ar[vb + (size_t)c * k + q] = V[(size_t)q * d.n2 + c];
+      // the leaf's first need: its first (lower) / last (upper) row diagonal
+      uint32_t e0, e1;
+      const uint32_t leaf_need = rows_of(d.b1, d.b1 + d.n1, e0, e1);
+      // A items: rank slices of <= 256 x column chunks
+      const uint32_t mk = std::min<uint32_t>(k, kPanelThreads);
+      const uint32_t wc = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(kMaxW, kItemValues / mk));
+      const uint32_t nchunks = (d.n2 + wc - 1) / wc;
+      uint32_t na = 0;
+      for (uint32_t ci = 0; ci < nchunks; ++ci) {
+        const uint32_t c0 = ci * wc, W = std::min(wc, d.n2 - c0);
+        for (uint32_t q0 = 0; q0 < k; q0 += kPanelThreads) {
+          TriItem it{};
+          it.type = kItemA;
+          it.data = vb + (uint64_t)c0 * k + q0;
+          it.m = std::min<uint32_t>(kPanelThreads, k - q0);
+          it.W = W;
+          it.ld = k;
+          it.in = d.b2 + c0;
+          it.out = (uint32_t)(t_len + (uint64_t)ci * k + q0);
+          it.wait = ready_of(d.b2 + c0, d.b2 + c0 + W);
+          it.sig0 = lr;
+          need.push_back(leaf_need);
+          cls.push_back(0);
+          its.push_back(it);
+          ++na;
+        }
+      }
+      a_count.push_back(na);
+      // B items: row tiles of U times the summed t
+      for (uint32_t r0 = 0; r0 < d.n1; r0 += kPanelThreads) {
+        const uint32_t m = std::min<uint32_t>(kPanelThreads, d.n1 - r0);
+        TriItem it{};
+        it.type = kItemB;
+        it.data = ub + r0;
+        it.m = m;
+        it.W = k;
+        it.ld = d.n1;
+        it.in = (uint32_t)t_len;
+        it.out = d.b1 + r0;
+        it.wait = lr;
+        it.wait2 = na;        // A items of the leaf
+        it.aux = k;
+        it.parts = nchunks;   // slots summed per rank entry
+        need.push_back(rows_of(d.b1 + r0, d.b1 + r0 + m, it.sig0, it.sig1));
+        cls.push_back(2);
+        its.push_back(it);
+      }
+      t_len += (uint64_t)nchunks * k;
+      if (t_len > 0xffffffffull) return fail(AHM_ERR_UNSUPPORTED, "ahm_trih_upload: too many low-rank slots");
+    }
+  }
+  // queue: for each step, the items needed there (A, D, B), then its S item
+  const uint32_t n_off = (uint32_t)its.size();
+  std::vector<uint32_t> ord(n_off);
+  for (uint32_t i = 0; i < n_off; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
+    if (need[a] != need[b]) return need[a] < need[b];
+    return cls[a] < cls[b];
+  });
+  std::vector<TriItem>& q = P.queue;
+  q.reserve(n_off + D);
+  std::vector<uint32_t>& pend0 = P.pend0;
+  pend0.assign(D, 0);
+  {
+    size_t k = 0;
+    for (uint32_t st = 0; st < D; ++st) {
+      for (; k < n_off && need[ord[k]] == st; ++k) q.push_back(its[ord[k]]);
+      q.push_back(sitems[lower ? st : D - 1 - st]);
+    }
+    if (k != n_off) return fail(AHM_ERR_INVALID, "ahm_trih_upload: internal: unplaced items");
+  }
+  for (const TriItem& it : q)
+    if (it.type == kItemB || it.type == kItemD)
+      for (uint32_t d = it.sig0; d <= it.sig1; ++d) ++pend0[d];
+
+  // run the queue once on the host: every wait must hold when reached
+  {
+    std::vector<uint32_t> solved(D, 0), tdone(a_count.size(), 0), pend(pend0);
+    for (const TriItem& it : q) {
+      bool ok;
+      if (it.type == kItemS)
+        ok = pend[it.wait] == 0 && (it.wait2 == kNone || solved[it.wait2]);
+      else if (it.type == kItemB)
+        ok = tdone[it.wait] >= it.wait2;
+      else
+        ok = solved[it.wait] != 0;
+      if (!ok) return fail(AHM_ERR_INVALID, "ahm_trih_upload: internal: queue order violates a dependency");
+      if (it.type == kItemS) solved[it.wait] = 1;
+      else if (it.type == kItemA) ++tdone[it.sig0];
+      else
+        for (uint32_t d = it.sig0; d <= it.sig1; ++d) --pend[d];
+    }
+  }
+  P.n_diag = D;
+  P.n_lr = (uint32_t)a_count.size();
+  P.t_len = t_len;
+  P.nvals = nvals;
+  return AHM_OK;
+}
+
+}  // namespace ahm
+
+extern "C" {
+
+int ahm_trih_upload(ahm_ctx ctx, uint32_t n, char uplo, uint32_t n_leaves, const ahm_leaf_desc* leaves,
+                    const void* data, ahm_dtype data_dtype, ahm_trih* out) {
+  using namespace ahm;
+  if (!ctx || !out || (n_leaves && (!leaves || !data)))
+    return fail(AHM_ERR_INVALID, "ahm_trih_upload: NULL argument");
+  if (uplo != 'L' && uplo != 'U') return fail(AHM_ERR_INVALID, "ahm_trih_upload: uplo must be 'L' or 'U'");
+  if (data_dtype != AHM_F64) return fail(AHM_ERR_UNSUPPORTED, "ahm_trih_upload: leaf values must be AHM_F64");
+  *out = nullptr;
+  TriPlan P;
+  AHM_TRY(build_plan(n, uplo == 'L', n_leaves, leaves, (const double*)data, P));
+  const std::vector<double>& ar = P.arena;
+  const std::vector<uint32_t>& perm = P.perm;
+  const std::vector<TriItem>& q = P.queue;
+  const std::vector<uint32_t>& pend0 = P.pend0;
+  const uint32_t D = P.n_diag;
+
+  std::unique_ptr<ahm_trih_s> h(new ahm_trih_s);
+  h->ctx = ctx;
+  h->n = n;
+  h->lower = uplo == 'L';
+  h->n_diag = D;
+  h->n_lr = P.n_lr;
+  h->n_items = q.size();
+  h->nvals = P.nvals;
+  h->arena_vals = ar.size();
+  h->t_len = P.t_len;
+  AHM_HIP(hipSetDevice(ctx->device));
+  AHM_TRY(h->arena.alloc(std::max<size_t>(ar.size(), 1) * sizeof(double)));
+  AHM_TRY(h->items.alloc(std::max<size_t>(q.size(), 1) * sizeof(TriItem)));
+  AHM_TRY(h->perm.alloc(std::max<size_t>(perm.size(), 1) * sizeof(uint32_t)));
+  AHM_TRY(h->pend0.alloc(std::max<size_t>(D, 1) * sizeof(uint32_t)));
+  AHM_TRY(h->counters.alloc((2 + 2 * (size_t)D + P.n_lr) * sizeof(uint32_t)));
+  AHM_TRY(h->tpart.alloc(std::max<uint64_t>(P.t_len, 1) * sizeof(double)));
+  if (!ar.empty()) AHM_HIP(hipMemcpy(h->arena.p, ar.data(), ar.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (!q.empty()) AHM_HIP(hipMemcpy(h->items.p, q.data(), q.size() * sizeof(TriItem), hipMemcpyHostToDevice));
+  if (!perm.empty())
+    AHM_HIP(hipMemcpy(h->perm.p, perm.data(), perm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (D) AHM_HIP(hipMemcpy(h->pend0.p, pend0.data(), D * sizeof(uint32_t), hipMemcpyHostToDevice));
+  int per_cu = 0, n_cu = 0;
+  AHM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trih_kernel, kPanelThreads, 0));
+  AHM_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  h->grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(q.size(), (uint64_t)std::max(per_cu, 1) * n_cu));
+  *out = h.release();
+  return AHM_OK;
+}
+
+int ahm_trih_destroy(ahm_trih t) {
+  if (!t) return AHM_OK;
+  (void)hipSetDevice(t->ctx->device);
+  delete t;
+  return AHM_OK;
+}
+
+int ahm_trih_info(ahm_trih t, uint32_t* n_diag, uint64_t* n_items, uint64_t* nvals) {
+  using namespace ahm;
+  if (!t) return fail(AHM_ERR_INVALID, "ahm_trih_info: NULL handle");
+  if (n_diag) *n_diag = t->n_diag;
+  if (n_items) *n_items = t->n_items;
+  if (nvals) *nvals = t->nvals;
+  return AHM_OK;
+}
+
+int ahm_trih_solve(ahm_trih t, void* B, uint32_t ldb, uint32_t p) {
+  using namespace ahm;
+  if (!t || (!B && p)) return fail(AHM_ERR_INVALID, "ahm_trih_solve: NULL argument");
+  if (p > 1 && ldb < t->n) return fail(AHM_ERR_INVALID, "ahm_trih_solve: ldb %u < n %u", ldb, t->n);
+  if (p == 0 || t->n == 0) return AHM_OK;
+  AHM_HIP(hipSetDevice(t->ctx->device));
+  hipStream_t s = t->ctx->stream;
+  uint32_t* cnt = (uint32_t*)t->counters.p;
+  const uint32_t D = t->n_diag;
+  const uint32_t n_zero = 2 + D + t->n_lr;
+  Counters cn{cnt, cnt + 1, cnt + 2, cnt + 2 + D, cnt + 2 + D + t->n_lr};
+  for (uint32_t j = 0; j < p; ++j) {
+    double* x = (double*)B + (uint64_t)j * (p > 1 ? ldb : 0);
+    const uint32_t tot = n_zero + D;
+    hipLaunchKernelGGL(trih_reset_kernel, dim3((tot + 255) / 256), dim3(256), 0, s, cnt, n_zero,
+                       (const uint32_t*)t->pend0.p, D);
+    hipLaunchKernelGGL(trih_kernel, dim3(t->grid), dim3(kPanelThreads), 0, s, (const TriItem*)t->items.p,
+                       (uint32_t)t->n_items, (const double*)t->arena.p, (const uint32_t*)t->perm.p, x,
+                       (double*)t->tpart.p, cn, t->lower ? 1 : 0);
+    AHM_HIP(hipGetLastError());
+  }
+  uint32_t ab = 0;
+  AHM_HIP(hipMemcpyAsync(&ab, cnt + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  if (ab) return fail(AHM_ERR_HIP, "ahm_trih_solve: a dependency wait ran out (internal error)");
+  return AHM_OK;
+}
+
+int ahm_trih_solve_host(ahm_trih t, double* B, uint32_t ldb, uint32_t p) {
+  using namespace ahm;
+  if (!t || (!B && p)) return fail(AHM_ERR_INVALID, "ahm_trih_solve_host: NULL argument");
+  if (p == 0) return AHM_OK;
+  if (p > 1 && ldb < t->n) return fail(AHM_ERR_INVALID, "ahm_trih_solve_host: ldb %u < n %u", ldb, t->n);
+  AHM_HIP(hipSetDevice(t->ctx->device));
+  hipStream_t s = t->ctx->stream;
+  DeviceBuf db;
+  AHM_TRY(db.alloc((size_t)t->n * p * 8));
+  AHM_HIP(hipMemcpy2DAsync(db.p, (size_t)t->n * 8, B, (size_t)(p > 1 ? ldb : t->n) * 8, (size_t)t->n * 8, p,
+                           hipMemcpyHostToDevice, s));
+  AHM_TRY(ahm_trih_solve(t, db.p, t->n, p));
+  AHM_HIP(hipMemcpy2DAsync(B, (size_t)(p > 1 ? ldb : t->n) * 8, db.p, (size_t)t->n * 8, (size_t)t->n * 8, p,
+                           hipMemcpyDeviceToHost, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  return AHM_OK;
+}
+
+}  // extern "C"

@@ -42,6 +42,7 @@ typedef enum { AHM_F64 = 0, AHM_F32 = 1, AHM_BF16 = 2 } ahm_dtype;
 typedef struct ahm_ctx_s* ahm_ctx;
 typedef struct ahm_hmat_s* ahm_hmat;
 typedef struct ahm_tree_s* ahm_tree;
+typedef struct ahm_trih_s* ahm_trih;
 
 /* cluster tree kinds */
 #define AHM_TREE_BBX 0   /* cluster3d_bbx (Include/cluster_bbx.h:236-302)      */
@@ -53,9 +54,19 @@ typedef struct ahm_tree_s* ahm_tree;
 typedef struct {
   uint32_t b1, n1, b2, n2;
   int32_t rank;
-  uint32_t flags;            /* reserved, 0 */
+  uint32_t flags;            /* 0, or AHM_LEAF_LTM / AHM_LEAF_UTM (triangular factors) */
   uint64_t offset;
 } ahm_leaf_desc;
+
+/* Diagonal leaves of triangular factors (ahm_trih_upload), n1 == n2 == n,
+ * n(n+1)/2 packed values as the reference's mblock stores them:
+ * AHM_LEAF_LTM  unit lower triangle (mblock::isLtM), column j holds rows
+ *               j..n-1 and its entry for row j is the row permutation
+ *               index ip_j as a value (mblock::decomp_LU, H/mblock_D.cpp:
+ *               1078-1107): the leaf represents P^T L with (P b)_i = b[ip_i];
+ * AHM_LEAF_UTM  upper triangle (mblock::isUtM), column j holds rows 0..j. */
+#define AHM_LEAF_LTM 1u
+#define AHM_LEAF_UTM 2u
 
 typedef struct {
   uint64_t n_leaves, n_dense, n_lowrank;
@@ -274,6 +285,32 @@ int ahm_gmres_ir(ahm_hmat hi, ahm_hmat lo, const double* b, double* x, double* e
 int ahm_gmres_batch(ahm_hmat h, const double* b, double* x, double eps, uint32_t m,
                     uint32_t nsteps, int use_graph, double* eps_out, uint32_t* steps_out,
                     int* converged);
+
+/* -------- triangular H-matrix factors: forward / backward substitution ----
+ * The reference's LtHGeM_solve / UtHGeM_solve (H/TH_solve.cpp:18-91,
+ * 475-535) with the leaf solves mblock::ltr_solve (H/mblock_D.cpp:
+ * 1183-1205) and mblock::utr_solve (Include/mblock.h:694-697); HLU_solve
+ * (Include/H.h:1383-1388) is an 'L' solve followed by a 'U' solve.
+ * uplo 'L': the unit lower factor (diagonal leaves AHM_LEAF_LTM, the other
+ * leaves strictly below the diagonal); 'U': the upper factor (AHM_LEAF_UTM,
+ * other leaves strictly above).  Off-diagonal leaves are dense (rank < 0) or
+ * low rank (mblock layout, U then V); the diagonal leaves must tile [0, n).
+ * data: host values of data_dtype (AHM_F64); the factor is stored in FP64.
+ * One persistent launch per right-hand side: the diagonal leaves are solved
+ * in order while the off-diagonal leaf products run as soon as the columns
+ * they read are solved (dependency counters in device memory, a work queue
+ * in an order the host has checked to be free of cyclic waits). */
+int ahm_trih_upload(ahm_ctx ctx, uint32_t n, char uplo, uint32_t n_leaves,
+                    const ahm_leaf_desc* leaves, const void* data, ahm_dtype data_dtype,
+                    ahm_trih* out);
+int ahm_trih_destroy(ahm_trih t);
+/* B (n x p, column stride ldb, FP64 device memory) <- op^{-1} B, in place,
+ * on the context stream (the reference's "B is destroyed, X stored in B"). */
+int ahm_trih_solve(ahm_trih t, void* B, uint32_t ldb, uint32_t p);
+/* the same on host memory (copies in and out) */
+int ahm_trih_solve_host(ahm_trih t, double* B, uint32_t ldb, uint32_t p);
+/* diagonal leaves, off-diagonal items and stored values of a factor */
+int ahm_trih_info(ahm_trih t, uint32_t* n_diag, uint64_t* n_items, uint64_t* nvals);
 
 #ifdef __cplusplus
 }

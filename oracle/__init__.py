@@ -131,6 +131,56 @@ class _HMat:
         return A
 
 
+# factor leaf kinds (oracle/ref_harness.cpp factor_kind): >= 0 low rank
+KIND_ABSENT, KIND_DENSE, KIND_LTM, KIND_UTM = -2, -1, -3, -4
+
+
+class _Factors:
+    """L and U of the reference's HLU on a tree (leaves in idx order)."""
+
+    def __init__(self, api, handle, tree):
+        self.api, self.h, self.tree, self.N = api, handle, tree, tree.N
+
+    def __del__(self):
+        try:
+            self.api.lib.ref_factors_free(self.h)
+        except Exception:
+            pass
+
+    def layout(self, which):
+        """which 'L' or 'U': dict(b1, n1, b2, n2, kind, offset, data); kind per
+        leaf: -2 absent, -1 dense, -3 packed unit lower with the row permutation
+        on its diagonal, -4 packed upper, k >= 0 low rank."""
+        lv = self.tree.leaves()
+        nl = len(lv["b1"])
+        kind = np.empty(nl, np.int32)
+        off = np.empty(nl, np.uint64)
+        w = 0 if which == "L" else 1
+        nv = self.api.lib.ref_factor_layout(self.h, w, _ptr(kind), _ptr(off))
+        data = np.empty(nv)
+        self.api.lib.ref_factor_data(self.h, w, _ptr(data))
+        return dict(b1=lv["b1"], n1=lv["n1"], b2=lv["b2"], n2=lv["n2"], kind=kind, offset=off, data=data)
+
+    def solve(self, which, B):
+        """which 'L' (LtHGeM_solve), 'U' (UtHGeM_solve) or 'LU' (HLU_solve per
+        column): returns X with op X = B; B is a vector or N x p."""
+        B = np.asarray(B, np.float64)
+        vec = B.ndim == 1
+        X = np.array(B.reshape(self.N, -1), order="F", copy=True)
+        w = {"L": 0, "U": 1, "LU": 2}[which]
+        self.api.lib.ref_trisolve(self.h, w, _u(X.shape[1]), _ptr(X), _u(self.N))
+        return X[:, 0].copy() if vec else X
+
+    def gmres(self, H, b, x0=None, eps=1e-8, m=30, nsteps=10000):
+        """GMRes with HLU_solve as right preconditioner (solvers/GMRES.cpp)."""
+        b = np.ascontiguousarray(b, np.float64).copy()
+        x = np.zeros(self.N) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+        e = _d(eps)
+        ns = _u(nsteps)
+        rc = self.api.lib.ref_gmres_hlu(H.h, self.h, _ptr(b), _ptr(x), C.byref(e), _u(m), C.byref(ns))
+        return x, int(rc), e.value, ns.value
+
+
 class Oracle:
     """ctypes binding of liboracle.so."""
 
@@ -183,6 +233,8 @@ class Oracle:
         L.or_cg_dense.restype = _u
         L.or_cg_dense.argtypes = [_u, _p, _p, _p, C.POINTER(_d), C.POINTER(_u)]
         L.or_blas_version.argtypes = [C.c_char_p, C.c_int]
+        L.or_trih_solve.restype = C.c_int
+        L.or_trih_solve.argtypes = [_u, C.c_char, _u] + [_p] * 7 + [_u, _p, _u]
         self._tree_free = L.or_tree_free
         self._tree_perm = L.or_tree_perm
         self._tree_nleaves = L.or_tree_nleaves
@@ -195,6 +247,23 @@ class Oracle:
 
     def _gmres(self, h, b, x, e, m, ns):
         return self.lib.or_gmres(h, b, x, e, m, ns, None, _u(0))
+
+    def trih_solve(self, F, uplo, B):
+        """op^{-1} B for a factor layout F (dict b1, n1, b2, n2, kind, offset,
+        data as _Factors.layout), uplo 'L' or 'U'; B a vector or N x p."""
+        B = np.asarray(B, np.float64)
+        vec = B.ndim == 1
+        N = B.shape[0]
+        X = np.array(B.reshape(N, -1), order="F", copy=True)
+        a = [np.ascontiguousarray(F[f], np.uint32) for f in ("b1", "n1", "b2", "n2")]
+        kind = np.ascontiguousarray(F["kind"], np.int32)
+        off = np.ascontiguousarray(F["offset"], np.uint64)
+        data = np.ascontiguousarray(F["data"], np.float64)
+        rc = self.lib.or_trih_solve(_u(N), uplo.encode(), _u(len(kind)), *(_ptr(v) for v in a), _ptr(kind),
+                                    _ptr(off), _ptr(data), _u(X.shape[1]), _ptr(X), _u(N))
+        if rc != 0:
+            raise ValueError("malformed triangular factor")
+        return X[:, 0].copy() if vec else X
 
     def blas_version(self) -> str:
         buf = C.create_string_buffer(256)
@@ -352,6 +421,17 @@ class Reference:
         L.ref_hmat_save.argtypes = [_p, C.c_char_p, _d, _u]
         L.ref_hmat_load.restype = _p
         L.ref_hmat_load.argtypes = [_p, C.c_char_p, C.POINTER(_d), C.POINTER(_u)]
+        L.ref_hlu.restype = _p
+        L.ref_hlu.argtypes = [_p, _d, _u]
+        L.ref_factors_free.argtypes = [_p]
+        L.ref_factor_layout.restype = C.c_size_t
+        L.ref_factor_layout.argtypes = [_p, C.c_int, _p, _p]
+        L.ref_factor_data.argtypes = [_p, C.c_int, _p]
+        L.ref_factors_from_data.restype = _p
+        L.ref_factors_from_data.argtypes = [_p] * 7
+        L.ref_trisolve.argtypes = [_p, C.c_int, _u, _p, _u]
+        L.ref_gmres_hlu.restype = _u
+        L.ref_gmres_hlu.argtypes = [_p, _p, _p, _p, C.POINTER(_d), _u, C.POINTER(_u)]
         self._tree_free = L.ref_tree_free
         self._tree_perm = L.ref_tree_perm
         self._tree_nleaves = L.ref_tree_nleaves
@@ -393,6 +473,26 @@ class Reference:
         if not h:
             raise OSError(f"ref_hmat_load({path}) failed")
         return _HMat(self, h, tree.N, tree), eta.value, bmin.value
+
+    def hlu(self, H, eps, rankmax=1 << 30):
+        """The reference's HLU (H/HLU.cpp:389-395) of a copy of H: _Factors."""
+        h = self.lib.ref_hlu(H.h, _d(eps), _u(rankmax))
+        if not h:
+            raise RuntimeError("reference HLU failed")
+        return _Factors(self, h, H.tree)
+
+    def factors_from_data(self, tree, Lf, Uf):
+        """L/U mblock arrays rebuilt from factor layouts (as _Factors.layout)."""
+        keep = []
+
+        def arr(a, t):
+            a = np.ascontiguousarray(a, t)
+            keep.append(a)
+            return _ptr(a)
+        h = self.lib.ref_factors_from_data(
+            tree.h, arr(Lf["kind"], np.int32), arr(Lf["offset"], np.uint64), arr(Lf["data"], np.float64),
+            arr(Uf["kind"], np.int32), arr(Uf["offset"], np.uint64), arr(Uf["data"], np.float64))
+        return _Factors(self, h, tree)
 
     def _layout(self, H):
         lv = H.tree.leaves()

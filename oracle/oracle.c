@@ -18,6 +18,7 @@
 #endif
 
 /* ------------------------------------------------------------------------ */
+
 /* BLAS / LAPACK (Fortran ABI, MKL).  Same routines the reference wraps in
  * Include/blas.h:176-560.                                                   */
 extern double ddot_(const int*, const double*, const int*, const double*, const int*);
@@ -1217,4 +1218,132 @@ unsigned or_cg_dense(unsigned n, const double* Am, const double* b, double* x, d
   dense_op o = {n, Am};
   op_t A = {n, &o, dense_amux};
   return cg(&A, b, x, eps, nsteps);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Triangular H-solves (H/TH_solve.cpp:18-91, leaf solves H/mblock_D.cpp:
+ * 1183-1205 and Include/mblock.h:694-697).                                  */
+
+typedef struct { unsigned key, leaf; } or_key;
+
+static int cmp_key(const void* a, const void* b)
+{
+  const or_key *x = (const or_key*)a, *y = (const or_key*)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->leaf < y->leaf ? -1 : (x->leaf > y->leaf);
+}
+
+/* x[rows] -= leaf * x[cols] for one off-diagonal leaf (dense: dgemv as
+ * Include/mblock.h:98-102; low rank: k dots then axpys, H/mblock.cpp:125-138) */
+static void tri_apply(unsigned n1, unsigned n2, int kind, const double* d,
+                      const double* xc, double* xr)
+{
+  if (kind >= 0) {
+    const unsigned k = (unsigned)kind;
+    for (unsigned q = 0; q < k; ++q) {
+      const double e = -scpr(n2, d + (size_t)k * n1 + (size_t)q * n2, xc);
+      axpy(n1, e, d + (size_t)q * n1, xr);
+    }
+  } else {
+    const int mm = (int)n1, nn = (int)n2;
+    const double mone = -1.0;
+    dgemv_("N", &mm, &nn, &mone, d, &mm, xc, &I1, &DONE, xr, &I1);
+  }
+}
+
+/* PL z = b on one diagonal leaf: z_i = b[ip_i], then unit lower forward
+ * substitution on the packed columns (column j: entries j..n-1, entry j is
+ * ip_j as a value) */
+static void tri_diag_lower(unsigned n, const double* d, double* x, double* z)
+{
+  size_t o = 0;
+  for (unsigned j = 0; j < n; ++j) {
+    z[j] = x[(unsigned)d[o]];
+    o += n - j;
+  }
+  o = 0;
+  for (unsigned j = 0; j < n; ++j) {
+    const double xj = z[j];
+    for (unsigned i = j + 1; i < n; ++i) z[i] -= d[o + (i - j)] * xj;
+    o += n - j;
+  }
+  memcpy(x, z, sizeof(double) * n);
+}
+
+/* U z = b on one diagonal leaf, packed upper columns (column j: rows 0..j
+ * at offset j(j+1)/2), backward substitution */
+static void tri_diag_upper(unsigned n, const double* d, double* x)
+{
+  for (unsigned j = n; j-- > 0;) {
+    const size_t o = (size_t)j * (j + 1) / 2;
+    x[j] /= d[o + j];
+    const double xj = x[j];
+    for (unsigned i = 0; i < j; ++i) x[i] -= d[o + i] * xj;
+  }
+}
+
+int or_trih_solve(unsigned n, char uplo, unsigned nleaves, const unsigned* b1,
+                  const unsigned* n1, const unsigned* b2, const unsigned* n2,
+                  const int* kind, const size_t* offset, const double* data,
+                  unsigned p, double* B, unsigned ldB)
+{
+  blas_init();
+  const int lower = uplo == 'L';
+  or_key* dg = (or_key*)xmalloc(sizeof(or_key) * (nleaves + 1));
+  or_key* od = (or_key*)xmalloc(sizeof(or_key) * (nleaves + 1));
+  unsigned nd = 0, no = 0, maxn = 1;
+  uint64_t covered = 0;
+  for (unsigned l = 0; l < nleaves; ++l) {
+    if (kind[l] == -2) continue;
+    if (kind[l] == (lower ? -3 : -4)) {
+      if (b1[l] != b2[l] || n1[l] != n2[l]) goto bad;
+      dg[nd].key = b1[l];
+      dg[nd++].leaf = l;
+      covered += n1[l];
+      if (n1[l] > maxn) maxn = n1[l];
+    } else if (kind[l] >= -1) {
+      /* lower: columns end before the rows start; upper: the reverse */
+      if (lower ? (b2[l] + n2[l] > b1[l]) : (b1[l] + n1[l] > b2[l])) goto bad;
+      od[no].key = lower ? b2[l] + n2[l] : b2[l];
+      od[no++].leaf = l;
+    } else {
+      goto bad;
+    }
+  }
+  if (covered != n) goto bad;
+  qsort(dg, nd, sizeof(or_key), cmp_key);
+  qsort(od, no, sizeof(or_key), cmp_key);
+  double* z = (double*)xmalloc(sizeof(double) * maxn);
+  for (unsigned c = 0; c < p; ++c) {
+    double* x = B + (size_t)c * ldB;
+    if (lower) {
+      unsigned q = 0;
+      for (unsigned i = 0; i < nd; ++i) {
+        const unsigned l = dg[i].leaf, e = b1[l] + n1[l];
+        tri_diag_lower(n1[l], data + offset[l], x + b1[l], z);
+        for (; q < no && od[q].key <= e; ++q) {
+          const unsigned m = od[q].leaf;
+          tri_apply(n1[m], n2[m], kind[m], data + offset[m], x + b2[m], x + b1[m]);
+        }
+      }
+    } else {
+      unsigned q = no;
+      for (unsigned i = nd; i-- > 0;) {
+        const unsigned l = dg[i].leaf, s = b1[l];
+        tri_diag_upper(n1[l], data + offset[l], x + s);
+        for (; q > 0 && od[q - 1].key >= s; --q) {
+          const unsigned m = od[q - 1].leaf;
+          tri_apply(n1[m], n2[m], kind[m], data + offset[m], x + b2[m], x + b1[m]);
+        }
+      }
+    }
+  }
+  free(z);
+  free(dg);
+  free(od);
+  return 0;
+bad:
+  free(dg);
+  free(od);
+  return -1;
 }

@@ -115,6 +115,25 @@ unsigned or_gmres_dense(unsigned n, const double* A, const double* b,
 unsigned or_cg_dense(unsigned n, const double* A, const double* b, double* x,
                      double* eps, unsigned* nsteps);
 
+/* ---- triangular H-solves: LtHGeM_solve_ / UtHGeM_solve_ (H/TH_solve.cpp:18-91)
+ * with the leaf solves mblock::ltr_solve (H/mblock_D.cpp:1183-1205, dtpsv
+ * 'L','N','U' Include/blas.h:3749-3755) and mblock::utr_solve
+ * (Include/mblock.h:694-697, dtpsv 'U','N','N' Include/blas.h:3930-3936).
+ * B (n x p, ld ldB) is overwritten with op^{-1} B.  uplo 'L': the factor's
+ * leaves are the diagonal ones (kind -3: packed unit lower triangle, column
+ * j holding its permutation entry on the diagonal) and lower ones; 'U':
+ * diagonal (kind -4: packed upper triangle) and upper ones.  Off-diagonal
+ * leaves: kind -1 dense, k >= 0 low rank (U then V); kind -2 is skipped.
+ * Restated as a sweep over the diagonal leaves (ascending for 'L',
+ * descending for 'U'); every off-diagonal leaf is applied whole as soon as
+ * the diagonal leaves covering its columns are solved, which the
+ * reference's recursion also guarantees, in another summation order.
+ * Returns 0, or -1 for a malformed factor.                       [golden] */
+int or_trih_solve(unsigned n, char uplo, unsigned nleaves, const unsigned* b1,
+                  const unsigned* n1, const unsigned* b2, const unsigned* n2,
+                  const int* kind, const size_t* offset, const double* data,
+                  unsigned p, double* B, unsigned ldB);
+
 /* MKL identification for fixtures */
 void or_blas_version(char* buf, int len);
 

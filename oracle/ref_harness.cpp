@@ -5,7 +5,8 @@
 // TEST INFRASTRUCTURE ONLY.  This file is our own driver code: it includes the
 // reference's public headers and calls the reference's own functions
 // (cluster3d_bbx, bemcluster, blcluster::subdivide, matgenGeH_omp/apprx_unsym,
-// ACA, mblock::cpyLrM_cmpr, mltaGeHVec, mltaGeHGeM, GMRes, CG).  The MATGEN_T
+// ACA, mblock::cpyLrM_cmpr, mltaGeHVec, mltaGeHGeM, GMRes, CG, HLU and the
+// triangular H-solves).  The MATGEN_T
 // kernel class below is ours (the reference ships none).
 #include <cmath>
 #include <cstdio>
@@ -92,6 +93,33 @@ struct HMat {
   Tree* t;
   mblock<double>** A;
 };
+
+// H-LU factors on the block tree of t (H/HLU.cpp:20-66): L holds the lower
+// and diagonal leaves, U the upper and diagonal leaves, the rest are NULL
+struct Factors {
+  Tree* t;
+  mblock<double>** L;
+  mblock<double>** U;
+};
+
+// right-preconditioned operator: amux = A, precond_apply = HLU_solve
+// (Include/matrix.h:30, Include/H.h:1383-1388)
+struct PrecGeH : public GeHMatrix<double> {
+  const Factors* f;
+  PrecGeH(unsigned n, blcluster* bl, const Factors* ff) : GeHMatrix<double>(n, n, bl), f(ff) {}
+  void precond_apply(double* x) const { HLU_solve(f->t->bl, f->L, f->U, x); }
+};
+
+// leaf kind codes of the factor layout: -2 absent, -1 dense, >= 0 low rank
+// (the rank), -3 unit lower triangle with the permutation on its diagonal
+// (mblock::isLtM, packed), -4 upper triangle (mblock::isUtM, packed)
+int factor_kind(mblock<double>* m) {
+  if (!m) return -2;
+  if (m->isLrM()) return (int)m->rank();
+  if (m->isLtM()) return -3;
+  if (m->isUtM()) return -4;
+  return -1;
+}
 
 // ACA start row for bbx trees: position in the row cluster closest to its
 // centre of mass (first minimum), same rule as the oracle.
@@ -386,6 +414,133 @@ void* ref_hmat_load(void* tp, const char* path, double* eta, unsigned* bmin) {
   h->t = t;
   h->A = A;
   return h;
+}
+
+
+// ---- H-LU (H/HLU.cpp:389-395) and the triangular H-solves (H/TH_solve.cpp) ----
+
+// LU factors of a copy of h's matrix (HLU destroys its input); NULL if HLU
+// reports failure
+void* ref_hlu(void* hp, double eps, unsigned rankmax) {
+  init();
+  HMat* h = (HMat*)hp;
+  Tree* t = h->t;
+  mblock<double>** A;
+  allocmbls(t->bl, A);
+  for (unsigned i = 0; i < t->nblcks; ++i) {
+    mblock<double>* s = h->A[i];
+    const unsigned n1 = s->getn1(), n2 = s->getn2();
+    mblock<double>* m = new mblock<double>(n1, n2);
+    if (s->isLrM()) {
+      m->setrank(s->rank());
+      if (s->rank()) std::memcpy(m->getdata(), s->getdata(), sizeof(double) * s->nvals());
+    } else {
+      m->setGeM();
+      std::memcpy(m->getdata(), s->getdata(), sizeof(double) * (size_t)n1 * n2);
+    }
+    A[i] = m;
+  }
+  Factors* f = new Factors;
+  f->t = t;
+  initLtH_0(t->bl, f->L);
+  initUtH_0(t->bl, f->U);
+  Silence quiet;
+  const bool ok = HLU(t->bl, A, f->L, f->U, eps, rankmax, NULL);
+  freembls(t->bl, A);
+  if (!ok) {
+    freembls(t->bl, f->L);
+    freembls(t->bl, f->U);
+    delete f;
+    return nullptr;
+  }
+  return f;
+}
+
+void ref_factors_free(void* p) {
+  Factors* f = (Factors*)p;
+  freembls(f->t->bl, f->L);
+  freembls(f->t->bl, f->U);
+  delete f;
+}
+
+// which = 0: L, 1: U.  kind[i] per factor_kind, offset[i] into the packed
+// values; returns the number of values
+size_t ref_factor_layout(void* p, int which, int* kind, size_t* offset) {
+  Factors* f = (Factors*)p;
+  mblock<double>** M = which ? f->U : f->L;
+  size_t off = 0;
+  for (unsigned i = 0; i < f->t->nblcks; ++i) {
+    kind[i] = factor_kind(M[i]);
+    offset[i] = off;
+    if (M[i]) off += M[i]->nvals();
+  }
+  return off;
+}
+
+void ref_factor_data(void* p, int which, double* out) {
+  Factors* f = (Factors*)p;
+  mblock<double>** M = which ? f->U : f->L;
+  size_t off = 0;
+  for (unsigned i = 0; i < f->t->nblcks; ++i) {
+    if (!M[i]) continue;
+    const size_t nv = M[i]->nvals();
+    if (nv) std::memcpy(out + off, M[i]->getdata(), sizeof(double) * nv);
+    off += nv;
+  }
+}
+
+// factors rebuilt from a layout (kind/offset per leaf as ref_factor_layout)
+void* ref_factors_from_data(void* tp, const int* kind_l, const size_t* off_l, const double* dl,
+                            const int* kind_u, const size_t* off_u, const double* du) {
+  init();
+  Tree* t = (Tree*)tp;
+  Factors* f = new Factors;
+  f->t = t;
+  for (int w = 0; w < 2; ++w) {
+    const int* kind = w ? kind_u : kind_l;
+    const size_t* off = w ? off_u : off_l;
+    const double* d = w ? du : dl;
+    mblock<double>**& M = w ? f->U : f->L;
+    allocmbls(t->bl, M);
+    for (unsigned i = 0; i < t->nblcks; ++i) {
+      if (kind[i] == -2) { M[i] = NULL; continue; }
+      const unsigned n1 = t->leaves[i]->getn1(), n2 = t->leaves[i]->getn2();
+      mblock<double>* m = new mblock<double>(n1, n2);
+      if (kind[i] >= 0) m->setrank((unsigned)kind[i]);
+      else if (kind[i] == -3) m->setLtM();
+      else if (kind[i] == -4) m->setUtM();
+      else m->setGeM();
+      if (m->nvals()) std::memcpy(m->getdata(), d + off[i], sizeof(double) * m->nvals());
+      M[i] = m;
+    }
+  }
+  return f;
+}
+
+// which: 0 L X = B (LtHGeM_solve), 1 U X = B (UtHGeM_solve), 2 L U X = B
+// (HLU_solve per column); B is n x p, overwritten with X
+void ref_trisolve(void* p, int which, unsigned pp, double* B, unsigned ldB) {
+  Factors* f = (Factors*)p;
+  blcluster* bl = f->t->bl;
+  if (which == 0) LtHGeM_solve(bl, f->L, pp, B, ldB);
+  else if (which == 1) UtHGeM_solve(bl, f->U, pp, B, ldB);
+  else
+    for (unsigned j = 0; j < pp; ++j) HLU_solve(bl, f->L, f->U, B + (size_t)j * ldB);
+}
+
+// GMRes (solvers/GMRES.cpp:80-181) with HLU_solve as the right preconditioner
+unsigned ref_gmres_hlu(void* hp, void* fp, double* b, double* x, double* eps, unsigned m,
+                       unsigned* nsteps) {
+  HMat* h = (HMat*)hp;
+  PrecGeH M(h->t->N, h->t->bl, (const Factors*)fp);
+  M.blcks = h->A;
+  double e = *eps;
+  unsigned ns = *nsteps;
+  unsigned r = GMRes(M, b, x, e, m, ns);
+  M.blcks = NULL;
+  *eps = e;
+  *nsteps = ns;
+  return r;
 }
 
 }  // extern "C"

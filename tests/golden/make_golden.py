@@ -1,12 +1,14 @@
 """Write golden vectors from the COMPILED REFERENCE (run in the build container).
 
-Usage:  make -C oracle ref && python tests/golden/make_golden.py [mbls | transposed]
+Usage:  make -C oracle ref && python tests/golden/make_golden.py [mbls | transposed | hlu]
 
 The reference's own sources (read in place under /root/reference, compiled by
 oracle/Makefile into oracle/_ref/libahmedref.so) produce every expected value
 here: cluster permutation, block-tree leaves, ACA ranks, a SHA-256 of all
 stored leaf values plus each leaf's Frobenius norm, H*x, H*X, GMRes and CG results, plus ACA / addtrll on
-explicit blocks, plus SHA-256 digests of the reference's savembls files.  Only data is written (golden_*.npz); nothing from the
+explicit blocks, plus SHA-256 digests of the reference's savembls files, plus the
+reference's HLU factors of two small matrices with its triangular solves and
+HLU-preconditioned GMRes on them.  Only data is written (golden_*.npz); nothing from the
 reference travels in any other form.
 """
 from __future__ import annotations
@@ -126,8 +128,54 @@ def write_transposed(R):
     np.savez_compressed(os.path.join(OUT, "golden_transposed.npz"), **rec)
 
 
+# name, geometry, N, seed, bmin, eta, eps (ACA), HLU eps, GMRes (m, tol, nsteps)
+HLU_CASES = [
+    ("cube640", "cube", 640, 3, 16, 3.0, 1e-5, 1e-3, (30, 1e-10, 100)),
+    ("sphere700", "sphere", 700, 3, 32, 2.0, 1e-6, 1e-3, (30, 1e-10, 100)),
+]
+
+
+def write_hlu(R):
+    """The reference's HLU (H/HLU.cpp:389-395) of two small Laplace
+    H-matrices: both factors (layouts in leaf idx order: kind -2 absent, -1
+    dense, -3 packed unit lower with the permutation on its diagonal, -4
+    packed upper, k >= 0 low rank), L\\b, U\\b, (LU)\\b for one and three
+    right-hand sides (LtHGeM_solve, UtHGeM_solve, HLU_solve), and GMRes with
+    HLU_solve as right preconditioner (solvers/GMRES.cpp:80-181)."""
+    rec = {"blas": O.Oracle().blas_version()}
+    for (name, geo, N, seed, bmin, eta, eps, heps, gm) in HLU_CASES:
+        pts = points(geo, N, seed)
+        weight = AREA[geo] / N
+        T = R.tree(pts, bmin, eta * eta, O.TREE_BBX)
+        H = R.assemble(T, kernel=O.KERNEL_LAPLACE, eps=eps, weight=weight)
+        F = R.hlu(H, heps)
+        rng = np.random.default_rng(700 + seed)
+        b = rng.standard_normal(N)
+        B = rng.standard_normal((N, 3))
+        c = {"N": N, "seed": seed, "bmin": bmin, "eta2": eta * eta, "eps": eps, "hlu_eps": heps,
+             "weight": weight, "geometry": geo, "data_sha256": data_hash(H.layout()), "b": b, "B": B}
+        for which in "LU":
+            f = F.layout(which)
+            for k in ("b1", "n1", "b2", "n2", "kind", "offset", "data"):
+                c[f"{which}_{k}"] = f[k]
+            c[f"x{which}"] = F.solve(which, b)
+            c[f"X{which}"] = F.solve(which, B)
+        c["xLU"] = F.solve("LU", b)
+        m, tol, ns = gm
+        xg, rc, e, n = F.gmres(H, b, eps=tol, m=m, nsteps=ns)
+        c.update(gmres_m=m, gmres_tol=tol, gmres_nsteps_in=ns, gmres_x=xg, gmres_rc=rc, gmres_eps=e,
+                 gmres_nsteps=n)
+        for k, v in c.items():
+            rec[f"{name}_{k}"] = v
+        print("hlu", name, "L values", len(c["L_data"]), "U values", len(c["U_data"]), "gmres", rc, e, n)
+        del F, H, T
+    np.savez_compressed(os.path.join(OUT, "golden_hlu.npz"), **rec)
+
+
 def main():
     R = O.Reference()
+    if sys.argv[1:] == ["hlu"]:
+        return write_hlu(R)
     if sys.argv[1:] == ["mbls"]:
         return write_mbls(R)
     if sys.argv[1:] == ["transposed"]:
@@ -190,6 +238,7 @@ def main():
     np.savez_compressed(os.path.join(OUT, "golden_aca.npz"), **aca)
     write_mbls(R)
     write_transposed(R)
+    write_hlu(R)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,119 @@
+"""GPU triangular H-solves (LtHGeM_solve / UtHGeM_solve, H/TH_solve.cpp:18-91;
+HLU_solve, Include/H.h:1383-1388) through the C ABI.
+
+* on the reference's own HLU factors (tests/golden/golden_hlu.npz): L\\b,
+  U\\b, (LU)\\b and three right-hand sides against the reference's outputs;
+* on block factors of larger H-matrices (ahmed_amd.factors) against the
+  oracle's restatement;
+* malformed factors are rejected on the host before anything runs.
+Bar: FP64 within 1e-12 relative (max-norm)."""
+import numpy as np
+import pytest
+
+from helpers import relerr, sphere_hmatrix
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+NAMES = ["cube640", "sphere700"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from ahmed_amd import _lib
+    assert torch.cuda.is_available()
+    return _lib.Context(0)
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def hlu_case(name):
+    import os
+    from conftest import GOLDEN
+    z = np.load(os.path.join(GOLDEN, "golden_hlu.npz"), allow_pickle=False)
+    p = name + "_"
+    return {k[len(p):]: z[k] for k in z.files if k.startswith(p)}
+
+
+def factor(g, which):
+    return {k: g[f"{which}_{k}"] for k in ("b1", "n1", "b2", "n2", "kind", "offset", "data")}
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_trih_matches_reference_golden(ctx, name):
+    from ahmed_amd import _lib
+    g = hlu_case(name)
+    N = int(g["N"])
+    print(f"[gpu] trih golden {name} N={N}", flush=True)
+    F = {w: _lib.TriFactor.from_layout(ctx, N, w, factor(g, w)) for w in "LU"}
+    for w in "LU":
+        x = dev(g["b"])
+        F[w].solve(x)
+        assert relerr(x.cpu().numpy(), g[f"x{w}"]) <= 1e-12, w
+        X = dev(g["B"].T)          # (p, N): row j = right-hand side j
+        F[w].solve(X)
+        assert relerr(X.cpu().numpy().T, g[f"X{w}"]) <= 1e-12, w
+    x = dev(g["b"])
+    F["L"].solve(x)
+    F["U"].solve(x)
+    assert relerr(x.cpu().numpy(), g["xLU"]) <= 1e-12
+    info = F["L"].info()
+    assert info["n_diag"] == int((g["L_kind"] == -3).sum())
+
+
+@pytest.mark.parametrize("N,eps", [(16384, 1e-8), (32768, 1e-6)])
+def test_trih_block_factors_match_oracle(ctx, N, eps, oracle_api):
+    from ahmed_amd import _lib
+    from ahmed_amd.factors import block_factors
+    _, _, H = sphere_hmatrix(N, 64, eps)
+    Lf, Uf = block_factors(H.layout())
+    rng = np.random.default_rng(N)
+    b = rng.standard_normal(N)
+    B = rng.standard_normal((N, 2))
+    print(f"[gpu] trih block factors N={N}", flush=True)
+    for w, F in (("L", Lf), ("U", Uf)):
+        T = _lib.TriFactor.from_layout(ctx, N, w, F)
+        ref = oracle_api.trih_solve(F, w, b)
+        for rep in range(2):   # the counters and slots are reset per call
+            x = dev(b)
+            T.solve(x)
+            err = relerr(x.cpu().numpy(), ref)
+            print(f"[gpu]   {w} rep {rep}: rel.err {err:.2e}  {T.info()}")
+            assert err <= 1e-12
+        X = dev(B.T)
+        T.solve(X)
+        assert relerr(X.cpu().numpy().T, oracle_api.trih_solve(F, w, B)) <= 1e-12
+
+
+def test_trih_rejects_malformed_factors(ctx):
+    from ahmed_amd import _lib
+    g = hlu_case("cube640")
+    N = int(g["N"])
+    L = factor(g, "L")
+    leaves, data = _lib.factor_leaves(L)
+    # a diagonal leaf missing: the diagonal no longer tiles [0, N)
+    d = np.flatnonzero(leaves["flags"] == _lib.LEAF_LTM)
+    with pytest.raises(_lib.AhmError, match="tile"):
+        _lib.TriFactor.upload(ctx, N, "L", np.delete(leaves, d[3]), data)
+    # the lower factor handed over as an upper one
+    with pytest.raises(_lib.AhmError):
+        _lib.TriFactor.upload(ctx, N, "U", leaves, data)
+    # a permutation entry out of range
+    bad = data.copy()
+    bad[int(leaves["offset"][d[0]])] = 1e6
+    with pytest.raises(_lib.AhmError, match="permutation"):
+        _lib.TriFactor.upload(ctx, N, "L", leaves, bad)
+    # an off-diagonal leaf on the wrong side of the diagonal
+    off = np.flatnonzero(leaves["flags"] == 0)[0]
+    swapped = leaves.copy()
+    swapped["b1"][off], swapped["b2"][off] = leaves["b2"][off], leaves["b1"][off]
+    swapped["n1"][off], swapped["n2"][off] = leaves["n2"][off], leaves["n1"][off]
+    with pytest.raises(_lib.AhmError, match="below"):
+        _lib.TriFactor.upload(ctx, N, "L", swapped, data)
+    # p = 0 is a no-op
+    T = _lib.TriFactor.upload(ctx, N, "L", leaves, data)
+    x = dev(np.ones((0, N)))
+    T.solve(x)

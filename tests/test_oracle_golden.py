@@ -106,3 +106,68 @@ def test_aca_and_addtrll_on_explicit_blocks(oracle_api):
         assert Uo.shape == g[f"trl{c}_U"].shape
         P, Pg = Uo @ Vo.T, g[f"trl{c}_U"] @ g[f"trl{c}_V"].T
         assert np.linalg.norm(P - Pg) <= 1e-12 * np.linalg.norm(Pg)
+
+
+# ---- triangular H-solves (LtHGeM_solve / UtHGeM_solve / HLU_solve) on the
+# reference's own HLU factors (tests/golden/golden_hlu.npz)
+HLU_NAMES = ["cube640", "sphere700"]
+
+
+def hlu_case(name):
+    z = np.load(os.path.join(GOLDEN, "golden_hlu.npz"), allow_pickle=False)
+    p = name + "_"
+    return {k[len(p):]: z[k] for k in z.files if k.startswith(p)}
+
+
+def factor(g, which):
+    return {k: g[f"{which}_{k}"] for k in ("b1", "n1", "b2", "n2", "kind", "offset", "data")}
+
+
+@pytest.mark.parametrize("name", HLU_NAMES)
+def test_triangular_solves_match_reference(name, oracle_api):
+    g = hlu_case(name)
+    L, U = factor(g, "L"), factor(g, "U")
+    for which, F in (("L", L), ("U", U)):
+        x = oracle_api.trih_solve(F, which, g["b"])
+        assert np.max(np.abs(x - g[f"x{which}"])) <= 1e-13 * np.max(np.abs(g[f"x{which}"]))
+        X = oracle_api.trih_solve(F, which, g["B"])
+        assert np.max(np.abs(X - g[f"X{which}"])) <= 1e-13 * np.max(np.abs(g[f"X{which}"]))
+    xlu = oracle_api.trih_solve(U, "U", oracle_api.trih_solve(L, "L", g["b"]))
+    assert np.max(np.abs(xlu - g["xLU"])) <= 1e-13 * np.max(np.abs(g["xLU"]))
+
+
+@pytest.mark.parametrize("name", HLU_NAMES)
+def test_triangular_factors_closed_form(name, oracle_api):
+    """Independent of both solvers: the dense matrices the factor layouts
+    stand for (permutation rows of mblock::ltr_solve, H/mblock_D.cpp:
+    1183-1205) map the reference's solutions back onto b, and a dense LAPACK
+    solve of them gives the oracle's answer."""
+    from ahmed_amd.factors import dense_factor
+    g = hlu_case(name)
+    N = int(g["N"])
+    ML, MU = dense_factor(factor(g, "L"), N, "L"), dense_factor(factor(g, "U"), N, "U")
+    b = g["b"]
+    assert np.max(np.abs(ML @ g["xL"] - b)) <= 1e-13 * np.max(np.abs(b))
+    assert np.max(np.abs(MU @ g["xU"] - b)) <= 1e-12 * np.max(np.abs(b))
+    assert np.max(np.abs(ML @ (MU @ g["xLU"]) - b)) <= 1e-12 * np.max(np.abs(b))
+    xo = oracle_api.trih_solve(factor(g, "U"), "U", b)
+    assert np.max(np.abs(xo - np.linalg.solve(MU, b))) <= 1e-11 * np.max(np.abs(xo))
+
+
+@pytest.mark.parametrize("name", HLU_NAMES)
+def test_hlu_fixture_matrix_is_the_oracles(name, oracle_api):
+    """The fixture's GMRes ran on the reference's H-matrix of these points;
+    the oracle assembles the identical matrix (same SHA-256 of all values)."""
+    from ahmed_amd import geometry
+    g = hlu_case(name)
+    N, seed = int(g["N"]), int(g["seed"])
+    pts = geometry.sphere(N, seed) if str(g["geometry"]) == "sphere" else geometry.cube_surface(N, seed)
+    T = oracle_api.tree(pts, int(g["bmin"]), float(g["eta2"]))
+    H = oracle_api.assemble(T, pts, 0, 0.0, float(g["eps"]), weight=float(g["weight"]))
+    L = H.layout()
+    assert hashlib.sha256(np.ascontiguousarray(L["data"]).tobytes()).hexdigest() == str(g["data_sha256"])
+    # and the fixture's factor leaves sit on this tree's blocks
+    lv = T.leaves()
+    for k in ("b1", "n1", "b2", "n2"):
+        assert np.array_equal(lv[k], g[f"L_{k}"])
+    assert int(g["gmres_rc"]) == 0 and int(g["gmres_nsteps"]) <= 5
