@@ -104,6 +104,9 @@ SIGNATURES = {
     "ahm_trih_solve": (_i, [_p, _p, _u32, _u32]),
     "ahm_trih_solve_host": (_i, [_p, _p, _u32, _u32]),
     "ahm_trih_info": (_i, [_p, C.POINTER(_u32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "ahm_gmres_prec": (_i, [_p, _p, _p, _p, _p, C.POINTER(_d), _u32, C.POINTER(_u32), _p, _u32,
+                            C.POINTER(_u32)]),
+    "ahm_cg_prec": (_i, [_p, _p, _p, _p, _p, C.POINTER(_d), C.POINTER(_u32)]),
 }
 
 LEAF_LTM, LEAF_UTM = 1, 2
@@ -473,24 +476,37 @@ class HMatrix:
                                              p, vec, reps, C.byref(a), C.byref(b)), "matvec_profile")
         return a.value, b.value
 
-    def gmres(self, b, x, eps: float = 1e-8, m: int = 30, nsteps: int = 10000, nhist: int = 0):
+    def gmres(self, b, x, eps: float = 1e-8, m: int = 30, nsteps: int = 10000, nhist: int = 0,
+              prec=None):
         """Device GMRES(m) (solvers/GMRES.cpp semantics); b, x FP64 torch
-        tensors, x updated in place.  Returns dict(converged, eps, nsteps,
-        nmatvec, hist)."""
+        tensors, x updated in place; prec = (L, U) TriFactors (either may be
+        None) makes HLU_solve the right preconditioner.  Returns
+        dict(converged, eps, nsteps, nmatvec, hist)."""
         e, ns, mv = _d(eps), _u32(nsteps), _u32(0)
         hist = np.zeros(max(nhist, 1))
         self.ctx.bind_stream()
-        rc = load().ahm_gmres(self.h, _p(b.data_ptr()), _p(x.data_ptr()), C.byref(e), m, C.byref(ns),
-                              hist.ctypes.data_as(_p), nhist, C.byref(mv))
+        if prec is None:
+            rc = load().ahm_gmres(self.h, _p(b.data_ptr()), _p(x.data_ptr()), C.byref(e), m, C.byref(ns),
+                                  hist.ctypes.data_as(_p), nhist, C.byref(mv))
+        else:
+            L, U = prec
+            rc = load().ahm_gmres_prec(self.h, L.h if L is not None else None, U.h if U is not None else None,
+                                       _p(b.data_ptr()), _p(x.data_ptr()), C.byref(e), m, C.byref(ns),
+                                       hist.ctypes.data_as(_p), nhist, C.byref(mv))
         if rc not in (OK, 5):
             check(rc, "ahm_gmres")
         return dict(converged=rc == OK, eps=e.value, nsteps=ns.value, nmatvec=mv.value,
                     hist=hist[:nhist].copy())
 
-    def cg(self, b, x, eps: float = 1e-8, nsteps: int = 10000):
+    def cg(self, b, x, eps: float = 1e-8, nsteps: int = 10000, prec=None):
         e, ns = _d(eps), _u32(nsteps)
         self.ctx.bind_stream()
-        rc = load().ahm_cg(self.h, _p(b.data_ptr()), _p(x.data_ptr()), C.byref(e), C.byref(ns))
+        if prec is None:
+            rc = load().ahm_cg(self.h, _p(b.data_ptr()), _p(x.data_ptr()), C.byref(e), C.byref(ns))
+        else:
+            L, U = prec
+            rc = load().ahm_cg_prec(self.h, L.h if L is not None else None, U.h if U is not None else None,
+                                    _p(b.data_ptr()), _p(x.data_ptr()), C.byref(e), C.byref(ns))
         if rc not in (OK, 5):
             check(rc, "ahm_cg")
         return dict(converged=rc == OK, eps=e.value, nsteps=ns.value)

@@ -144,3 +144,15 @@ struct ahm_hmat_s {
     size_t contrib_cap = 0, tprime_cap = 0;
   } tr;
 };
+
+// triangular H-matrix factor (trih.hip): work queue, arena, counters
+struct ahm_trih_s {
+  ahm_ctx ctx = nullptr;
+  uint32_t n = 0;
+  bool lower = true;
+  uint32_t n_diag = 0, n_lr = 0;
+  uint64_t n_items = 0, nvals = 0, arena_vals = 0;
+  ahm::DeviceBuf arena, items, perm, pend0, counters, tpart;
+  uint64_t t_len = 0;
+  uint32_t grid = 0;
+};

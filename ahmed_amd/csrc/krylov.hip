@@ -186,6 +186,21 @@ struct Op {
   }
 };
 
+// precond_apply (Include/matrix.h:30) as HLU_solve (Include/H.h:1383-1388):
+// v <- U^{-1} L^{-1} v with the triangular H-solves (trih.hip); either factor
+// may be absent.  Runs on the factors' context stream, which must be the
+// operator's.
+struct Prec {
+  ahm_trih L = nullptr, U = nullptr;
+  uint32_t n = 0;
+  bool on() const { return L || U; }
+  int apply(double* v) const {
+    if (L) AHM_TRY(ahm_trih_solve(L, v, n, 1));
+    if (U) AHM_TRY(ahm_trih_solve(U, v, n, 1));
+    return AHM_OK;
+  }
+};
+
 struct Red {
   DeviceBuf part, scal;   // kRedBlocks partials (x2), scalars
   double* host = nullptr;
@@ -261,19 +276,22 @@ void app_rot(double& dx, double& dy, double cs, double sn) {
 // then hold the achieved residual and the steps taken, as the reference's
 // return value 1 does).
 int gmres_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m,
-                 uint32_t* nsteps, double* hist, uint32_t nhist, uint32_t* nmatvec) {
+                 uint32_t* nsteps, double* hist, uint32_t nhist, uint32_t* nmatvec,
+                 const Prec& M = Prec()) {
   const uint32_t n = h->n_rows;
   hipStream_t s = h->ctx->stream;
   Op A{h, n};
   AHM_TRY(A.init());
   Red R;
   AHM_TRY(R.init(m + 2));
-  DeviceBuf dV, dr, dy;
+  DeviceBuf dV, dr, dy, dxh;
   AHM_TRY(dV.alloc((size_t)n * (m + 1) * 8));
   AHM_TRY(dr.alloc((size_t)n * 8));
   AHM_TRY(dy.alloc((size_t)(m + 1) * 8));
+  if (M.on()) AHM_TRY(dxh.alloc((size_t)n * 8));
   double* V = (double*)dV.p;
   double* r = (double*)dr.p;
+  double* xh = (double*)dxh.p;
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m + 1), sn(m + 1), sv(m + 1), col(m + 2);
   uint32_t nh = 0, mv = 0;
   auto update = [&](uint32_t k) -> int {
@@ -285,8 +303,17 @@ int gmres_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m
       y[i] = t / H[i + (size_t)i * (m + 1)];
     }
     AHM_HIP(hipMemcpyAsync(dy.p, y.data(), k * 8, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(gemv_update_kernel, dim3(kEwBlocks), dim3(256), 0, s, V, n, k,
-                       (const double*)dy.p, x);
+    if (M.on()) {
+      // x += M V y (solvers/GMRES.cpp:70-73)
+      hipLaunchKernelGGL(fill_kernel, dim3(kEwBlocks), dim3(256), 0, s, xh, n, 0.0);
+      hipLaunchKernelGGL(gemv_update_kernel, dim3(kEwBlocks), dim3(256), 0, s, V, n, k,
+                         (const double*)dy.p, xh);
+      AHM_TRY(M.apply(xh));
+      hipLaunchKernelGGL(axpy_kernel, dim3(kEwBlocks), dim3(256), 0, s, 1.0, (const double*)xh, x, n);
+    } else {
+      hipLaunchKernelGGL(gemv_update_kernel, dim3(kEwBlocks), dim3(256), 0, s, V, n, k,
+                         (const double*)dy.p, x);
+    }
     AHM_HIP(hipStreamSynchronize(s));  // y lives on the host stack
     return AHM_OK;
   };
@@ -320,7 +347,14 @@ int gmres_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m
       double* vi = V + (size_t)i * n;
       double* w = V + (size_t)(i + 1) * n;
       hipLaunchKernelGGL(fill_kernel, dim3(kEwBlocks), dim3(256), 0, s, w, n, 0.0);
-      AHM_TRY(A.amux(1.0, vi, w));
+      if (M.on()) {
+        // w = A M v_i (solvers/GMRES.cpp:125-130)
+        AHM_HIP(hipMemcpyAsync(xh, vi, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+        AHM_TRY(M.apply(xh));
+        AHM_TRY(A.amux(1.0, xh, w));
+      } else {
+        AHM_TRY(A.amux(1.0, vi, w));
+      }
       ++mv;
       // MGS: k = 0..i, fused (dot from partials, axpy, next dot)
       hipLaunchKernelGGL(dot_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, w, V, n, R.p0());
@@ -370,8 +404,9 @@ int gmres_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m
   return AHM_ERR_NOT_CONVERGED;
 }
 
-// CG, solvers/CG.cpp:31-113 (no preconditioner).
-int cg_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nsteps) {
+// CG, solvers/CG.cpp:31-113; with M, r^ = M r (precond_apply) each step.
+int cg_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nsteps,
+              const Prec& M = Prec()) {
   const uint32_t n = h->n_rows;
   hipStream_t s = h->ctx->stream;
   Op A{h, n};
@@ -379,10 +414,11 @@ int cg_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nst
   Red R;
   AHM_TRY(R.init(4));
   DeviceBuf buf;
-  AHM_TRY(buf.alloc((size_t)n * 3 * 8));
+  AHM_TRY(buf.alloc((size_t)n * 4 * 8));
   double* p = (double*)buf.p;
   double* q = p + n;
   double* r = q + n;
+  double* rhat = M.on() ? r + n : r;
   double nrmb;
   AHM_TRY(norm2(s, R, b, n, &nrmb));
   if (nrmb < 1e-16) {
@@ -404,11 +440,15 @@ int cg_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nst
   double rho, rho1 = 0.0;
   const uint32_t maxsteps = *nsteps;
   for (uint32_t l = 1; l <= maxsteps; ++l) {
-    AHM_TRY(dot(s, R, r, r, n, &rho));  // rhat = r (no preconditioner)
+    if (M.on()) {
+      AHM_HIP(hipMemcpyAsync(rhat, r, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+      AHM_TRY(M.apply(rhat));
+    }
+    AHM_TRY(dot(s, R, r, rhat, n, &rho));
     if (l > 1)
-      hipLaunchKernelGGL(cg_dir_kernel, dim3(kEwBlocks), dim3(256), 0, s, rho / rho1, r, p, n);
+      hipLaunchKernelGGL(cg_dir_kernel, dim3(kEwBlocks), dim3(256), 0, s, rho / rho1, rhat, p, n);
     else
-      AHM_HIP(hipMemcpyAsync(p, r, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+      AHM_HIP(hipMemcpyAsync(p, rhat, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(fill_kernel, dim3(kEwBlocks), dim3(256), 0, s, q, n, 0.0);
     AHM_TRY(A.amux(1.0, p, q));
     double pq;
@@ -453,6 +493,49 @@ int ahm_cg(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nsteps
   const int st = cg_device(h, b, x, eps, nsteps);
   if (st == AHM_ERR_NOT_CONVERGED)
     set_error("ahm_cg: no convergence in %u steps (residual %.3e)", *nsteps, *eps);
+  return st;
+}
+
+static int prec_of(ahm_hmat h, ahm_trih L, ahm_trih U, Prec& M, const char* who) {
+  for (ahm_trih t : {L, U}) {
+    if (!t) continue;
+    uint32_t nd = 0;
+    uint64_t ni = 0, nv = 0;
+    AHM_TRY(ahm_trih_info(t, &nd, &ni, &nv));
+    if (t->ctx != h->ctx) return fail(AHM_ERR_INVALID, "%s: the factors must share the matrix's context", who);
+    if (t->n != h->n_rows) return fail(AHM_ERR_INVALID, "%s: factor of size %u for a matrix of %u rows", who, t->n,
+                                       h->n_rows);
+  }
+  M.L = L;
+  M.U = U;
+  M.n = h->n_rows;
+  return AHM_OK;
+}
+
+int ahm_gmres_prec(ahm_hmat h, ahm_trih L, ahm_trih U, const double* b, double* x, double* eps,
+                   uint32_t m, uint32_t* nsteps, double* hist, uint32_t nhist, uint32_t* nmatvec) {
+  if (!h || !b || !x || !eps || !nsteps || m == 0)
+    return fail(AHM_ERR_INVALID, "ahm_gmres_prec: bad arguments");
+  if (h->n_rows != h->n_cols) return fail(AHM_ERR_INVALID, "ahm_gmres_prec: matrix is not square");
+  Prec M;
+  AHM_TRY(prec_of(h, L, U, M, "ahm_gmres_prec"));
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  const int st = gmres_device(h, b, x, eps, m, nsteps, hist, nhist, nmatvec, M);
+  if (st == AHM_ERR_NOT_CONVERGED)
+    set_error("ahm_gmres_prec: no convergence in %u steps (residual %.3e)", *nsteps, *eps);
+  return st;
+}
+
+int ahm_cg_prec(ahm_hmat h, ahm_trih L, ahm_trih U, const double* b, double* x, double* eps,
+                uint32_t* nsteps) {
+  if (!h || !b || !x || !eps || !nsteps) return fail(AHM_ERR_INVALID, "ahm_cg_prec: bad arguments");
+  if (h->n_rows != h->n_cols) return fail(AHM_ERR_INVALID, "ahm_cg_prec: matrix is not square");
+  Prec M;
+  AHM_TRY(prec_of(h, L, U, M, "ahm_cg_prec"));
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  const int st = cg_device(h, b, x, eps, nsteps, M);
+  if (st == AHM_ERR_NOT_CONVERGED)
+    set_error("ahm_cg_prec: no convergence in %u steps (residual %.3e)", *nsteps, *eps);
   return st;
 }
 

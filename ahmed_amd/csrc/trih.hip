@@ -37,17 +37,6 @@
 
 #include "ahm_internal.h"
 
-struct ahm_trih_s {
-  ahm_ctx ctx = nullptr;
-  uint32_t n = 0;
-  bool lower = true;
-  uint32_t n_diag = 0, n_lr = 0;
-  uint64_t n_items = 0, nvals = 0, arena_vals = 0;
-  ahm::DeviceBuf arena, items, perm, pend0, counters, tpart;
-  uint64_t t_len = 0;
-  uint32_t grid = 0;
-};
-
 namespace ahm {
 
 namespace {

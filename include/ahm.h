@@ -312,6 +312,17 @@ int ahm_trih_solve_host(ahm_trih t, double* B, uint32_t ldb, uint32_t p);
 /* diagonal leaves, off-diagonal items and stored values of a factor */
 int ahm_trih_info(ahm_trih t, uint32_t* n_diag, uint64_t* n_items, uint64_t* nvals);
 
+/* GMRes / CG with precond_apply (Include/matrix.h:30) = HLU_solve
+ * (Include/H.h:1383-1388): v <- U^{-1} L^{-1} v; GMRes preconditions from
+ * the right (w = A M v_i, x += M V y: solvers/GMRES.cpp:70-73,125-130), CG
+ * uses r^ = M r (solvers/CG.cpp:69-71).  L or U may be NULL (that factor is
+ * skipped); both must live on h's context.  Arguments and returns otherwise
+ * as ahm_gmres / ahm_cg. */
+int ahm_gmres_prec(ahm_hmat h, ahm_trih L, ahm_trih U, const double* b, double* x, double* eps,
+                   uint32_t m, uint32_t* nsteps, double* hist, uint32_t nhist, uint32_t* nmatvec);
+int ahm_cg_prec(ahm_hmat h, ahm_trih L, ahm_trih U, const double* b, double* x, double* eps,
+                uint32_t* nsteps);
+
 #ifdef __cplusplus
 }
 #endif
