@@ -102,6 +102,67 @@ ahm_hmat device_matrix(blcluster* bl, mblock<T>** A, bool& changed) {
   return e.h;
 }
 
+// triangular factors: one cache entry per (tree, array, side)
+struct TriEntry {
+  ahm_trih h = nullptr;
+  unsigned long epoch = 0;
+};
+std::map<std::pair<const void*, const void*>, TriEntry> g_tri[2];
+
+ahm_trih device_factor(blcluster* bl, mblock<double>** M, char uplo) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  ahm_ctx c = context();
+  if (!c) return nullptr;
+  TriEntry& e = g_tri[uplo == 'U'][{bl, M}];
+  if (e.h && e.epoch == g_epoch) return e.h;
+  if (e.h) ahm_trih_destroy(e.h);
+  e.h = nullptr;
+  std::vector<blcluster*> lv;
+  bl->leaves(lv);
+  std::vector<ahm_leaf_desc> L;
+  size_t total = 0;
+  for (blcluster* b : lv)
+    if (M[b->getidx()]) total += M[b->getidx()]->nvals();
+  std::vector<double> data(total);
+  size_t off = 0;
+  for (blcluster* b : lv) {
+    mblock<double>* m = M[b->getidx()];
+    if (!m) continue;
+    ahm_leaf_desc d;
+    d.b1 = b->getb1() - bl->getb1();
+    d.n1 = b->getn1();
+    d.b2 = b->getb2() - bl->getb2();
+    d.n2 = b->getn2();
+    d.rank = m->isLrM() ? (int32_t)m->rank() : -1;
+    d.flags = m->isLtM() ? AHM_LEAF_LTM : m->isUtM() ? AHM_LEAF_UTM : 0u;
+    d.offset = off;
+    const size_t nv = m->nvals();
+    if (nv) std::memcpy(data.data() + off, m->getdata(), nv * sizeof(double));
+    off += nv;
+    L.push_back(d);
+  }
+  if (ahm_trih_upload(c, bl->getn1(), uplo, (uint32_t)L.size(), L.data(), data.data(), AHM_F64, &e.h) !=
+      AHM_OK) {
+    std::fprintf(stderr, "ahmed: triangular factor upload failed: %s\n", ahm_last_error());
+    e.h = nullptr;
+    return nullptr;
+  }
+  e.epoch = g_epoch;
+  return e.h;
+}
+
+void tri_solve(blcluster* bl, mblock<double>** M, char uplo, unsigned p, double* B, unsigned ldB,
+               const char* who) {
+  ahm_trih t = device_factor(bl, M, uplo);
+  if (!t) return;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (ahm_ctx_set_stream(g_ctx, nullptr) != AHM_OK) return;
+  }
+  if (ahm_trih_solve_host(t, B, ldB, p) != AHM_OK)
+    std::fprintf(stderr, "ahmed: %s failed: %s\n", who, ahm_last_error());
+}
+
 unsigned start_row(const blcluster* b, const GpuKernel& G) {
   unsigned icom;
   if (b->aca_start_row(icom)) return icom - b->getb1();
@@ -133,6 +194,14 @@ unsigned start_row(const blcluster* b, const GpuKernel& G) {
 extern "C" void ahm_surface_invalidate(void) {
   std::lock_guard<std::mutex> lock(g_mu);
   ++g_epoch;
+}
+
+void LtHGeM_solve(blcluster* blL, mblock<double>** L, unsigned p, double* B, unsigned ldB) {
+  tri_solve(blL, L, 'L', p, B, ldB, "LtHGeM_solve");
+}
+
+void UtHGeM_solve(blcluster* blU, mblock<double>** U, unsigned p, double* B, unsigned ldB) {
+  tri_solve(blU, U, 'U', p, B, ldB, "UtHGeM_solve");
 }
 
 bool mltaGeHVec(double d, blcluster* bl, mblock<double>** A, double* x, double* y) {

@@ -45,7 +45,6 @@ constexpr uint32_t kItemS = 0, kItemA = 1, kItemB = 2, kItemD = 3;
 constexpr int kTri = 64;             // rows substituted per wavefront pass
 constexpr uint32_t kMaxDiag = 2048;  // largest diagonal leaf (LDS vector)
 constexpr uint32_t kMaxW = 2048;     // columns per gemv item (LDS vector)
-constexpr uint64_t kItemValues = 48 * 1024;
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
@@ -75,18 +74,20 @@ struct Counters {
   uint32_t* pending;
 };
 
-__device__ __forceinline__ uint32_t load_acq(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint32_t load_rlx(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// spin until *p == v (eq) or *p >= v; false if the abort flag is (or gets) set
+// Spin until *p == v (eq) or *p >= v with relaxed (agent-coherent) loads;
+// an acquire load would invalidate the caches on every poll.  False if the
+// abort flag is (or gets) set.
 __device__ bool spin_until(const uint32_t* p, uint32_t v, bool eq, uint32_t* abort) {
   for (uint32_t it = 0;; ++it) {
-    const uint32_t c = load_acq(p);
+    const uint32_t c = load_rlx(p);
     if (eq ? c == v : c >= v) return true;
-    if ((it & 31) == 31 && load_acq(abort)) return false;
+    if ((it & 31) == 31 && load_rlx(abort)) return false;
     if (it > kSpinLimit) {
-      __hip_atomic_store(abort, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
     __builtin_amdgcn_s_sleep(2);
@@ -100,34 +101,58 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+// Values produced by one workgroup and read by another (the vector, the
+// low-rank slots) move through agent-coherent accesses (the sc1 loads and
+// stores of relaxed agent-scope atomics), so no cache has to be written back
+// or invalidated around a dependency: a producer waits for its own stores
+// (s_waitcnt) before the workgroup barrier that precedes its flag, and a
+// consumer reads them only after its spin on the flag has succeeded.
+__device__ __forceinline__ double ld_c(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_c(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stores_done() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 struct TriSmem {
   double z[kMaxDiag];          // S: the leaf's vector; A/B/D: the input slice
-  double tri[kTri * kTri];     // S: one triangle; A/B/D: cross-group partial sums
+  double tri[kTri * kTri];     // S: one triangle
+  double red[kPanelThreads];   // cross-group partial sums
   uint32_t item;
   uint32_t ok;
 };
 
+constexpr int kBatch = 16;     // loads in flight per thread in tile_gemv
+
 // out(r) = sum_c P[r + c*ld] * z[c] for r < m (m <= 256, W <= kMaxW), z in
-// LDS; thread t < G*m owns row t % m and columns g, g+G, ... (G = 256/m);
-// returns the row sum on threads t < m (fixed-order reduction)
+// LDS; thread t < G*m owns row t % m and columns g, g+G, ... (G = 256/m),
+// loaded kBatch at a time with clamped (always valid) addresses so that the
+// loads of a batch are in flight together; returns the row sum on threads
+// t < m (fixed-order reduction across the groups)
 __device__ __forceinline__ double tile_gemv(const double* __restrict__ P, uint32_t m, uint32_t W,
                                             uint32_t ld, const double* z, double* red) {
   const int tid = threadIdx.x;
   const int G = kPanelThreads / (int)m;
   const int row = tid % (int)m, g = tid / (int)m;
-  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  double a = 0;
   if (g < G) {
     const double* Pr = P + row;
-    int c = g;
-    for (; c + 3 * G < (int)W; c += 4 * G) {
-      a0 = fma(Pr[(uint64_t)c * ld], z[c], a0);
-      a1 = fma(Pr[(uint64_t)(c + G) * ld], z[c + G], a1);
-      a2 = fma(Pr[(uint64_t)(c + 2 * G) * ld], z[c + 2 * G], a2);
-      a3 = fma(Pr[(uint64_t)(c + 3 * G) * ld], z[c + 3 * G], a3);
+    for (int c0 = g; c0 < (int)W; c0 += kBatch * G) {
+      double v[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int c = c0 + u * G;
+        v[u] = Pr[(uint64_t)(c < (int)W ? c : c0) * ld];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int c = c0 + u * G;
+        a = fma(v[u], c < (int)W ? z[c] : 0.0, a);
+      }
     }
-    for (; c < (int)W; c += G) a0 = fma(Pr[(uint64_t)c * ld], z[c], a0);
   }
-  red[tid] = g < G ? (a0 + a1) + (a2 + a3) : 0.0;
+  red[tid] = g < G ? a : 0.0;
   __syncthreads();
   double s = 0;
   if (tid < (int)m)
@@ -135,82 +160,243 @@ __device__ __forceinline__ double tile_gemv(const double* __restrict__ P, uint32
   return s;
 }
 
-// one diagonal leaf, lower: z = P^T-gathered slice, unit lower substitution
-__device__ void solve_lower(const TriItem& it, const double* __restrict__ arena,
-                            const uint32_t* __restrict__ perm, double* acc, TriSmem& sm) {
+// A/B/D items are cut so that a thread owns at most 2 * kBatch columns of
+// its row (W <= kItemBatches * kBatch * G): their whole tile is loaded into
+// registers before the item waits for its inputs.
+constexpr int kItemBatches = 2;
+constexpr int kItemCols = kItemBatches * kBatch;   // 32 columns per thread
+
+__device__ __forceinline__ void gemv_prefetch(const double* __restrict__ P, uint32_t m, uint32_t W, uint32_t ld,
+                                              double (&v)[kItemCols]) {
+  const int tid = threadIdx.x;
+  const int G = kPanelThreads / (int)m;
+  const int row = tid % (int)m, g = tid / (int)m;
+  const int gg = g < G ? g : 0;
+#pragma unroll
+  for (int u = 0; u < kItemCols; ++u) {
+    const int c = gg + u * G;
+    v[u] = P[row + (uint64_t)(c < (int)W ? c : 0) * ld];
+  }
+}
+
+__device__ __forceinline__ double gemv_apply(const double (&v)[kItemCols], uint32_t m, uint32_t W, const double* z,
+                                             double* red) {
+  const int tid = threadIdx.x;
+  const int G = kPanelThreads / (int)m;
+  const int g = tid / (int)m;
+  double a = 0;
+  if (g < G) {
+#pragma unroll
+    for (int u = 0; u < kItemCols; ++u) {
+      const int c = g + u * G;
+      a = fma(v[u], c < (int)W ? z[c] : 0.0, a);
+    }
+  }
+  red[tid] = g < G ? a : 0.0;
+  __syncthreads();
+  double s = 0;
+  if (tid < (int)m)
+    for (int q = 0; q < G; ++q) s += red[q * (int)m + tid];
+  return s;
+}
+
+constexpr int kTriPerThread = kTri * kTri / kPanelThreads;   // 16
+
+__device__ __forceinline__ void tri_load(const double* __restrict__ blk, double* t) {
+#pragma unroll
+  for (int q = 0; q < kTriPerThread; ++q) t[q] = blk[threadIdx.x + q * kPanelThreads];
+}
+__device__ __forceinline__ void tri_store(const double* t, double* tri) {
+#pragma unroll
+  for (int q = 0; q < kTriPerThread; ++q) tri[threadIdx.x + q * kPanelThreads] = t[q];
+}
+
+// Forward substitution of wave 0's 64 rows through the 64x64 unit lower
+// triangle in LDS (zero on and above the diagonal): lane r holds z_r; step j
+// broadcasts z_j (final after step j-1) with readlane and every lane
+// subtracts its multiple.  The triangle's column entries are read from LDS
+// 16 steps ahead into registers, so a step costs only the readlane and the
+// fma (zero entries make the update a no-op above the diagonal).
+__device__ __forceinline__ double subst_lower(double zr, const double* tri) {
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int jb = 0; jb < kTri; jb += 16) {
+    double tv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) tv[u] = tri[(jb + u) * kTri + lane];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) zr = fma(-tv[u], readlane_d(zr, jb + u), zr);
+  }
+  return zr;
+}
+
+// Backward substitution through the 64x64 upper triangle (reciprocal
+// diagonal): step j scales z_j, broadcasts it and the lanes above subtract.
+__device__ __forceinline__ double subst_upper(double zr, const double* tri) {
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int jb = kTri - 16; jb >= 0; jb -= 16) {
+    double tv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) tv[u] = tri[(jb + u) * kTri + lane];
+#pragma unroll
+    for (int u = 15; u >= 0; --u) {
+      const int j = jb + u;
+      zr = lane == j ? zr * tv[u] : zr;
+      const double xj = readlane_d(zr, j);
+      zr = fma(lane < j ? -tv[u] : 0.0, xj, zr);
+    }
+  }
+  return zr;
+}
+
+// Data of a diagonal leaf that does not depend on the vector, loaded before
+// the item waits: the first triangle (into LDS), the second triangle and the
+// rectangle next to the first (registers, when it has at most 64 rows), and
+// for 'L' the row permutation.
+struct SPre {
+  uint32_t ip[kMaxDiag / kPanelThreads];
+};
+// the register prefetch of an item: an A/B/D tile, or for an S item the
+// second triangle (first kTriPerThread) and the first rectangle (last kBatch)
+using Pf = double[kItemCols];
+static_assert(kTriPerThread + kBatch == kItemCols, "S prefetch layout");
+
+// thread mapping of a rows x 64 rectangle with rows <= 64 (tile_gemv's, one
+// batch): row tid % rows, columns g, g+G, ... (G = 256 / rows >= 4, <= 16 each)
+__device__ __forceinline__ void rect_load(const double* __restrict__ R, uint32_t rows, double* v) {
+  const int tid = threadIdx.x;
+  const int G = kPanelThreads / (int)rows;
+  const int row = tid % (int)rows, g = tid / (int)rows;
+  const int cg = g < G ? g : 0;
+#pragma unroll
+  for (int u = 0; u < kBatch; ++u) {
+    const int c = cg + u * G;
+    v[u] = R[(uint64_t)(c < kTri ? c : cg) * rows + row];
+  }
+}
+
+__device__ __forceinline__ double rect_apply(const double* v, uint32_t rows, const double* z, double* red) {
+  const int tid = threadIdx.x;
+  const int G = kPanelThreads / (int)rows;
+  const int g = tid / (int)rows;
+  double a = 0;
+  if (g < G) {
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int c = g + u * G;
+      a = fma(v[u], c < kTri ? z[c] : 0.0, a);
+    }
+  }
+  red[tid] = g < G ? a : 0.0;
+  __syncthreads();
+  double s = 0;
+  if (tid < (int)rows)
+    for (int q = 0; q < G; ++q) s += red[q * (int)rows + tid];
+  return s;
+}
+
+__device__ __forceinline__ void solve_prefetch(const TriItem& it, const double* __restrict__ arena,
+                                               const uint32_t* __restrict__ perm, bool lower, TriSmem& sm,
+                                               SPre& pre, Pf& pf) {
+  const int tid = threadIdx.x;
+  const uint32_t n = it.m;
+  const uint32_t np = (n + kTri - 1) / kTri;
+  const double* blk = arena + it.data;
+  double t[kTriPerThread];
+  tri_load(blk, t);
+  // first rectangle: below the first triangle ('L') or above the last ('U')
+  const uint32_t rows0 = lower ? (n > kTri ? n - kTri : 0) : (np - 1) * kTri;
+  if (rows0 > 0 && rows0 <= kTri) rect_load(blk + kTri * kTri, rows0, pf + kTriPerThread);
+  if (np > 1) tri_load(blk + kTri * kTri + (uint64_t)rows0 * kTri, pf);
+  if (lower) {
+#pragma unroll
+    for (int q = 0; q < kMaxDiag / kPanelThreads; ++q) {
+      const uint32_t r = tid + q * kPanelThreads;
+      pre.ip[q] = r < n ? perm[it.aux + r] : 0u;
+    }
+  }
+  tri_store(t, sm.tri);
+}
+
+// one diagonal leaf, lower: z = P^T-gathered slice, unit lower substitution,
+// 64 rows at a time, then the rectangle below each triangle
+__device__ void solve_lower(const TriItem& it, const double* __restrict__ arena, double* acc,
+                            TriSmem& sm, const SPre& pre, const Pf& pf) {
   const int tid = threadIdx.x;
   const uint32_t n = it.m, s0 = it.out;
   const uint32_t np = (n + kTri - 1) / kTri;
-  for (uint32_t r = tid; r < np * kTri; r += kPanelThreads)
-    sm.z[r] = r < n ? acc[s0 + perm[it.aux + r]] : 0.0;
+#pragma unroll
+  for (int q = 0; q < kMaxDiag / kPanelThreads; ++q) {
+    const uint32_t r = tid + q * kPanelThreads;
+    if (r < np * kTri) sm.z[r] = r < n ? ld_c(acc + s0 + pre.ip[q]) : 0.0;
+  }
   const double* blk = arena + it.data;
   for (uint32_t p = 0; p < np; ++p) {
     const uint32_t c0 = p * kTri;
     const uint32_t below = n > c0 + kTri ? n - c0 - kTri : 0;
     __syncthreads();
-    for (int e = tid; e < kTri * kTri; e += kPanelThreads) sm.tri[e] = blk[e];
-    __syncthreads();
-    if (tid < kTri) {
-      double zr = sm.z[c0 + tid];
-#pragma unroll
-      for (int j = 0; j < kTri; ++j) {
-        const double xj = readlane_d(zr, j);
-        if (tid > j) zr = fma(-sm.tri[j * kTri + tid], xj, zr);
-      }
-      sm.z[c0 + tid] = zr;
-    }
+    if (tid < kTri) sm.z[c0 + tid] = subst_lower(sm.z[c0 + tid], sm.tri);
     __syncthreads();
     const double* rect = blk + kTri * kTri;
-    for (uint32_t r0 = 0; r0 < below; r0 += kPanelThreads) {
-      const uint32_t m = min(below - r0, (uint32_t)kPanelThreads);
-      const double s = tile_gemv(rect + r0, m, kTri, below, sm.z + c0, sm.tri);
+    const double* next = rect + (uint64_t)below * kTri;
+    double t[kTriPerThread];
+    if (p > 0 && p + 1 < np) tri_load(next, t);
+    if (p == 0 && below > 0 && below <= kTri) {
+      const double s = rect_apply(pf + kTriPerThread, below, sm.z + c0, sm.red);
+      if (tid < (int)below) sm.z[c0 + kTri + tid] -= s;
       __syncthreads();
-      if (tid < (int)m) sm.z[c0 + kTri + r0 + tid] -= s;
-      __syncthreads();
+    } else {
+      for (uint32_t r0 = 0; r0 < below; r0 += kPanelThreads) {
+        const uint32_t m = min(below - r0, (uint32_t)kPanelThreads);
+        const double s = tile_gemv(rect + r0, m, kTri, below, sm.z + c0, sm.red);
+        if (tid < (int)m) sm.z[c0 + kTri + r0 + tid] -= s;
+        __syncthreads();
+      }
     }
-    blk = rect + (uint64_t)below * kTri;
+    if (p + 1 < np) tri_store(p == 0 ? pf : t, sm.tri);
+    blk = next;
   }
   __syncthreads();
-  for (uint32_t r = tid; r < n; r += kPanelThreads) acc[s0 + r] = sm.z[r];
+  for (uint32_t r = tid; r < n; r += kPanelThreads) st_c(acc + s0 + r, sm.z[r]);
 }
 
-// one diagonal leaf, upper: backward substitution, reciprocal diagonal
+// one diagonal leaf, upper: backward substitution; panels are stored last
+// first, each a triangle followed by the rectangle above it
 __device__ void solve_upper(const TriItem& it, const double* __restrict__ arena, double* acc,
-                            TriSmem& sm) {
+                            TriSmem& sm, const Pf& pf) {
   const int tid = threadIdx.x;
   const uint32_t n = it.m, s0 = it.out;
   const uint32_t np = (n + kTri - 1) / kTri;
-  for (uint32_t r = tid; r < np * kTri; r += kPanelThreads) sm.z[r] = r < n ? acc[s0 + r] : 0.0;
-  // panels are stored last first: triangle, then the rectangle above it
+  for (uint32_t r = tid; r < np * kTri; r += kPanelThreads) sm.z[r] = r < n ? ld_c(acc + s0 + r) : 0.0;
   const double* blk = arena + it.data;
   for (uint32_t pp = np; pp-- > 0;) {
     const uint32_t c0 = pp * kTri;
+    const bool first = pp + 1 == np;
     __syncthreads();
-    for (int e = tid; e < kTri * kTri; e += kPanelThreads) sm.tri[e] = blk[e];
-    __syncthreads();
-    if (tid < kTri) {
-      double zr = sm.z[c0 + tid];
-#pragma unroll
-      for (int j = kTri - 1; j >= 0; --j) {
-        if (tid == j) zr *= sm.tri[j * kTri + j];
-        const double xj = readlane_d(zr, j);
-        if (tid < j) zr = fma(-sm.tri[j * kTri + tid], xj, zr);
-      }
-      sm.z[c0 + tid] = zr;
-    }
+    if (tid < kTri) sm.z[c0 + tid] = subst_upper(sm.z[c0 + tid], sm.tri);
     __syncthreads();
     const double* rect = blk + kTri * kTri;
-    for (uint32_t r0 = 0; r0 < c0; r0 += kPanelThreads) {
-      const uint32_t m = min(c0 - r0, (uint32_t)kPanelThreads);
-      const double s = tile_gemv(rect + r0, m, kTri, c0, sm.z + c0, sm.tri);
+    const double* next = rect + (uint64_t)c0 * kTri;
+    double t[kTriPerThread];
+    if (!first && pp > 0) tri_load(next, t);
+    if (first && c0 > 0 && c0 <= kTri) {
+      const double s = rect_apply(pf + kTriPerThread, c0, sm.z + c0, sm.red);
+      if (tid < (int)c0) sm.z[tid] -= s;
       __syncthreads();
-      if (tid < (int)m) sm.z[r0 + tid] -= s;
-      __syncthreads();
+    } else {
+      for (uint32_t r0 = 0; r0 < c0; r0 += kPanelThreads) {
+        const uint32_t m = min(c0 - r0, (uint32_t)kPanelThreads);
+        const double s = tile_gemv(rect + r0, m, kTri, c0, sm.z + c0, sm.red);
+        if (tid < (int)m) sm.z[r0 + tid] -= s;
+        __syncthreads();
+      }
     }
-    blk = rect + (uint64_t)c0 * kTri;
+    if (pp > 0) tri_store(first ? pf : t, sm.tri);
+    blk = next;
   }
   __syncthreads();
-  for (uint32_t r = tid; r < n; r += kPanelThreads) acc[s0 + r] = sm.z[r];
+  for (uint32_t r = tid; r < n; r += kPanelThreads) st_c(acc + s0 + r, sm.z[r]);
 }
 
 __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
@@ -221,13 +407,19 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
   for (;;) {
     if (tid == 0) {
       const uint32_t i = __hip_atomic_fetch_add(cn.head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      sm.item = (i < n_items && !load_acq(cn.abort)) ? i : kNone;
+      sm.item = (i < n_items && !load_rlx(cn.abort)) ? i : kNone;
     }
     __syncthreads();
     const uint32_t idx = sm.item;
     __syncthreads();
     if (idx == kNone) return;
     const TriItem it = items[idx];
+    SPre pre;
+    Pf pf;
+    if (it.type == kItemS)
+      solve_prefetch(it, arena, perm, lower != 0, sm, pre, pf);
+    else
+      gemv_prefetch(arena + it.data, it.m, it.W, it.ld, pf);
     if (tid == 0) {
       bool ok;
       if (it.type == kItemS) {
@@ -242,15 +434,14 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
     }
     __syncthreads();
     if (!sm.ok) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (it.type == kItemS) {
       if (lower)
-        solve_lower(it, arena, perm, acc, sm);
+        solve_lower(it, arena, acc, sm, pre, pf);
       else
-        solve_upper(it, arena, acc, sm);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        solve_upper(it, arena, acc, sm, pf);
+      stores_done();
       __syncthreads();
-      if (tid == 0) __hip_atomic_store(cn.solved + it.wait, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (tid == 0) __hip_atomic_store(cn.solved + it.wait, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       continue;
     }
     // stage the input slice
@@ -258,28 +449,28 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
       const uint32_t k = it.aux, nparts = it.parts;
       for (uint32_t l = tid; l < it.W; l += kPanelThreads) {
         double t = 0;
-        for (uint32_t c = 0; c < nparts; ++c) t += tpart[it.in + (uint64_t)c * k + l];
+        for (uint32_t c = 0; c < nparts; ++c) t += ld_c(tpart + it.in + (uint64_t)c * k + l);
         sm.z[l] = t;
       }
     } else {
-      for (uint32_t c = tid; c < it.W; c += kPanelThreads) sm.z[c] = acc[it.in + c];
+      for (uint32_t c = tid; c < it.W; c += kPanelThreads) sm.z[c] = ld_c(acc + it.in + c);
     }
     __syncthreads();
-    const double s = tile_gemv(arena + it.data, it.m, it.W, it.ld, sm.z, sm.tri);
+    const double s = gemv_apply(pf, it.m, it.W, sm.z, sm.red);
     if (tid < (int)it.m) {
       if (it.type == kItemA)
-        tpart[it.out + tid] = s;
+        st_c(tpart + it.out + tid, s);
       else
         unsafeAtomicAdd(acc + it.out + tid, -s);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    stores_done();
     __syncthreads();
     if (tid == 0) {
       if (it.type == kItemA) {
-        __hip_atomic_fetch_add(cn.tdone + it.sig0, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(cn.tdone + it.sig0, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
         for (uint32_t d = it.sig0; d <= it.sig1; ++d)
-          __hip_atomic_fetch_sub(cn.pending + d, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_sub(cn.pending + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   }
@@ -306,7 +497,7 @@ struct TriPlan {
   std::vector<double> arena;
   std::vector<uint32_t> perm;
   std::vector<TriItem> queue;
-  std::vector<uint32_t> pend0;   // per diagonal: B/D items touching its rows
+  std::vector<uint32_t> pend0;   // per diagonal: B/D tiles touching its rows
   uint32_t n_diag = 0, n_lr = 0;
   uint64_t t_len = 0, nvals = 0;
 };
@@ -454,7 +645,7 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
       const size_t base = ar.size();
       ar.insert(ar.end(), src + d.offset, src + d.offset + (size_t)d.n1 * d.n2);
       const uint32_t mrow = std::min<uint32_t>(d.n1, kPanelThreads);
-      const uint32_t wc = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(kMaxW, kItemValues / mrow));
+      const uint32_t wc = std::min<uint32_t>(kMaxW, kItemCols * (kPanelThreads / mrow));
       for (uint32_t r0 = 0; r0 < d.n1; r0 += kPanelThreads) {
         const uint32_t m = std::min<uint32_t>(kPanelThreads, d.n1 - r0);
         for (uint32_t c0 = 0; c0 < d.n2; c0 += wc) {
@@ -491,7 +682,7 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
       const uint32_t leaf_need = rows_of(d.b1, d.b1 + d.n1, e0, e1);
       // A items: rank slices of <= 256 x column chunks
       const uint32_t mk = std::min<uint32_t>(k, kPanelThreads);
-      const uint32_t wc = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(kMaxW, kItemValues / mk));
+      const uint32_t wc = std::min<uint32_t>(kMaxW, kItemCols * (kPanelThreads / mk));
       const uint32_t nchunks = (d.n2 + wc - 1) / wc;
       uint32_t na = 0;
       for (uint32_t ci = 0; ci < nchunks; ++ci) {
@@ -514,9 +705,11 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
         }
       }
       a_count.push_back(na);
-      // B items: row tiles of U times the summed t
-      for (uint32_t r0 = 0; r0 < d.n1; r0 += kPanelThreads) {
-        const uint32_t m = std::min<uint32_t>(kPanelThreads, d.n1 - r0);
+      // B items: row tiles of U times the summed t, at most 256 / ceil(k / 32)
+      // rows so that a thread owns at most 32 of the k columns
+      const uint32_t mb = kPanelThreads / ((k + kItemCols - 1) / kItemCols);
+      for (uint32_t r0 = 0; r0 < d.n1; r0 += mb) {
+        const uint32_t m = std::min<uint32_t>(mb, d.n1 - r0);
         TriItem it{};
         it.type = kItemB;
         it.data = ub + r0;
@@ -561,6 +754,11 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
     if (it.type == kItemB || it.type == kItemD)
       for (uint32_t d = it.sig0; d <= it.sig1; ++d) ++pend0[d];
 
+  // every A/B/D tile fits the per-thread register prefetch
+  for (const TriItem& it : q)
+    if (it.type != kItemS &&
+        (it.m == 0 || it.m > (uint32_t)kPanelThreads || it.W > (uint32_t)kItemCols * (kPanelThreads / it.m)))
+      return fail(AHM_ERR_INVALID, "ahm_trih_upload: internal: item of %u x %u exceeds the tile limits", it.m, it.W);
   // run the queue once on the host: every wait must hold when reached
   {
     std::vector<uint32_t> solved(D, 0), tdone(a_count.size(), 0), pend(pend0);

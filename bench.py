@@ -21,7 +21,7 @@ in plain FP64 and with FP64 iterative refinement around FP32 GMRES, the
 roofline of the dominant kernel (row-panel phase) and the reference's own
 CPU mltaGeHVec timed on this host on the identical H-matrix.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg3|cfg2] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg3|cfg2|cfg4|cfg5|trih] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -254,6 +254,115 @@ def bench_learned_slabs(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_trih(args):
+    """Triangular H-solves (LtHGeM_solve / UtHGeM_solve / HLU_solve) on the
+    FP64 sphere matrix of --config cfg2 (N = 262144, eps 1e-8): block factors
+    of the assembled matrix (ahmed_amd.factors: LU of the dense diagonal
+    leaves, the off-diagonal leaves unchanged), one persistent launch per
+    solve.  Reported beside the reference's serial HLU_solve on the identical
+    factors."""
+    import torch
+    from ahmed_amd import _lib, geometry
+    from ahmed_amd.factors import block_factors
+    N = args.n or 262144
+    pts = geometry.sphere(N, 0)
+    weight, eps, bmin, eta = 4 * np.pi / N, 1e-8, 64, 0.8
+    ctx = _lib.Context(0)
+    tree = _lib.Tree(pts, bmin, eta * eta)
+    H, _ = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_LAPLACE, 0.0, weight, eps, keep_leaves=True)
+    leaves, data = H.download()
+    t0 = time.perf_counter()
+    layout = {f: leaves[f] for f in ("b1", "n1", "b2", "n2", "rank", "offset")}
+    layout["data"] = data
+    Lf, Uf = block_factors(layout)
+    log(f"block factors: {time.perf_counter() - t0:.1f} s, L {len(Lf['data']) * 8 / 1e9:.2f} GB, "
+        f"U {len(Uf['data']) * 8 / 1e9:.2f} GB")
+    del layout
+    res = {}
+    fac = {}
+    b = torch.randn(N, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
+    for w, F in (("L", Lf), ("U", Uf)):
+        t0 = time.perf_counter()
+        T = _lib.TriFactor.from_layout(ctx, N, w, F)
+        t_up = time.perf_counter() - t0
+        fac[w] = T
+        info = T.info()
+        x = b.clone()
+        ms = timed(lambda: (x.copy_(b), T.solve(x)), torch, args.steps, args.warmup)
+        B = 8.0 * info["nvals"] + 8.0 * 2 * N
+        res[w] = {"ms": round(ms, 3), "GB/s_alg": round(B / (ms * 1e-3) / 1e9, 1),
+                  "frac_hbm_peak": round(B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "diagonal_leaves": info["n_diag"], "queue_items": info["n_items"],
+                  "nvals": info["nvals"], "upload_s": round(t_up, 2),
+                  "us_per_diagonal_leaf": round(ms * 1e3 / info["n_diag"], 2)}
+        log(f"{w} solve: {ms:.3f} ms, {res[w]['GB/s_alg']} GB/s algorithmic, "
+            f"{info['n_diag']} diagonal leaves ({res[w]['us_per_diagonal_leaf']} us each), "
+            f"{info['n_items']} items")
+    x = b.clone()
+    ms_lu = timed(lambda: (x.copy_(b), fac["L"].solve(x), fac["U"].solve(x)), torch, args.steps, args.warmup)
+    res["HLU_solve_ms"] = round(ms_lu, 3)
+    # GMRES with and without the preconditioner (b = H * 1, tol 1e-10)
+    bb = torch.zeros(N, dtype=torch.float64, device="cuda")
+    H.matvec(torch.ones(N, dtype=torch.float64, device="cuda"), bb)
+    gm = {}
+    for mode in ("plain", "hlu_solve"):
+        xs = torch.zeros(N, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = H.gmres(bb, xs, eps=1e-10, m=50, nsteps=2000, prec=None if mode == "plain" else (fac["L"], fac["U"]))
+        torch.cuda.synchronize()
+        gm[mode] = {"seconds": round(time.perf_counter() - t0, 4), "iterations": r["nsteps"],
+                    "converged": r["converged"], "residual": r["eps"]}
+        log(f"GMRES(50) {mode}: {gm[mode]}")
+    res["gmres_to_1e-10"] = gm
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        bh = b.cpu().numpy()
+        try:
+            R = O.Reference()
+            T = R.tree(pts, bmin, eta * eta, O.TREE_BBX)
+            F = R.factors_from_data(T, Lf, Uf)
+            kind = "reference"
+            run = lambda: F.solve("LU", bh)   # noqa: E731
+        except (FileNotFoundError, OSError, RuntimeError) as e:
+            log(f"reference library unavailable ({e}); timing the oracle restatement")
+            o = O.Oracle()
+            kind = "port"
+            run = lambda: o.trih_solve(Uf, "U", o.trih_solve(Lf, "L", bh))   # noqa: E731
+        ts = []
+        t_start = time.perf_counter()
+        while len(ts) < 3 and time.perf_counter() - t_start < 30:
+            t0 = time.perf_counter()
+            xc = run()
+            ts.append(time.perf_counter() - t0)
+        tc = float(np.median(ts))
+        x = b.clone()
+        fac["L"].solve(x)
+        fac["U"].solve(x)
+        diff = float(np.max(np.abs(x.cpu().numpy() - xc)) / np.max(np.abs(xc)))
+        cpu = {"value": round(tc * 1e3, 1), "unit": "ms per HLU_solve", "cores": 1, "kind": kind,
+               "sample": f"{len(ts)} serial HLU_solve calls on the identical factors, N={N}",
+               "gpu_speedup": round(tc * 1e3 / ms_lu, 1), "rel_diff_gpu_vs_cpu": diff}
+        log(f"CPU HLU_solve: {tc:.3f} s ({kind}); GPU {ms_lu:.3f} ms; rel.diff {diff:.2e}")
+    B_lu = 8.0 * (fac["L"].info()["nvals"] + fac["U"].info()["nvals"]) + 8.0 * 4 * N
+    line = {
+        "metric": "triangular H-solve (HLU_solve) ms, N=262144 sphere, 1 GPU",
+        "value": round(ms_lu, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_lu, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded sphere points; H-matrix assembled on the GPU; block factors)",
+        "config": {"workload": f"HLU_solve (L then U triangular H-solve), N={N} unit sphere, leaf size {bmin}, "
+                               f"eta {eta}, ACA eps {eps}, factors = LU of the diagonal leaves + off-diagonal leaves",
+                   "N": N},
+        "roofline": {"bound": "latency (diagonal chain) / hbm", "achieved": round(B_lu / (ms_lu * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(B_lu / (ms_lu * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "solves": res,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def profiled_traffic(config: str, N: int):
     """HBM bytes per row-panel launch from the committed FETCH_SIZE / WRITE_SIZE
     summary of the same workload (None for other workloads)."""
@@ -285,7 +394,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", choices=["cfg3", "cfg2", "cfg4", "cfg5"], default="cfg3")
+    ap.add_argument("--config", choices=["cfg3", "cfg2", "cfg4", "cfg5", "trih"], default="cfg3")
     ap.add_argument("--n", type=int, default=0, help="override N")
     ap.add_argument("--rhs", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
@@ -305,6 +414,8 @@ def main():
         return bench_batched(args)
     if args.config == "cfg5":
         return bench_learned_slabs(args)
+    if args.config == "trih":
+        return bench_trih(args)
     if args.config == "cfg2":
         N, eps, tol, geo = args.n or 262144, 1e-8, 1e-10, "unit sphere"
         pts = geometry.sphere(N, 0)

@@ -90,6 +90,27 @@ bool mltaGeHhVec(double d, blcluster* bl, mblock<double>** A, double* x, double*
 bool mltaGeHhGeM(double d, blcluster* bl, mblock<double>** A, unsigned p, double* X,
                  unsigned ldX, double* Y, unsigned ldY);
 
+// Triangular H-solves (H/TH_solve.cpp:18-91, 475-535): B (n x p, column
+// stride ldB, host memory) is overwritten with X.  blL / blU is the block
+// cluster tree the factor lives on; L holds the leaves on and below the
+// diagonal (isLtM diagonal leaves), U those on and above it (isUtM), the
+// other entries may be NULL, as initLtH_0 / initUtH_0 leave them
+// (H/initH.cpp:262-290).  The factor is uploaded on first use and cached
+// like the matrices of mltaGeHVec.
+void LtHGeM_solve(blcluster* blL, mblock<double>** L, unsigned p, double* B, unsigned ldB);
+void UtHGeM_solve(blcluster* blU, mblock<double>** U, unsigned p, double* B, unsigned ldB);
+// L x = b, U x = b and L U x = b in place (Include/H.h:1325-1388)
+inline void LtHVec_solve(blcluster* blL, mblock<double>** L, double* b) {
+  LtHGeM_solve(blL, L, 1, b, blL->getn1());
+}
+inline void UtHVec_solve(blcluster* blU, mblock<double>** U, double* b) {
+  UtHGeM_solve(blU, U, 1, b, blU->getn1());
+}
+inline void HLU_solve(blcluster* bl, mblock<double>** L, mblock<double>** U, double* b) {
+  LtHVec_solve(bl, L, b);
+  UtHVec_solve(bl, U, b);
+}
+
 // storage in bytes and largest rank (H/initH.cpp:16-78)
 unsigned long sizeH(blcluster* bl, mblock<double>** A);
 unsigned Hmax_rank(blcluster* bl, mblock<double>** A);

@@ -26,23 +26,35 @@ class mblock {
   unsigned bl_rank;
   struct st_info {
     unsigned is_LrM : 1;
+    unsigned is_LtM : 1;   // packed unit lower triangle, permutation on the diagonal
+    unsigned is_UtM : 1;   // packed upper triangle
   } info;
 
  public:
-  mblock(unsigned m, unsigned n) : data(nullptr), n1(m), n2(n), bl_rank(0) { info.is_LrM = 1; }
+  mblock(unsigned m, unsigned n) : data(nullptr), n1(m), n2(n), bl_rank(0) {
+    info.is_LrM = 1;
+    info.is_LtM = info.is_UtM = 0;
+  }
   ~mblock() { delete[] data; }
   mblock(const mblock&) = delete;
   mblock& operator=(const mblock&) = delete;
 
   // number of stored values (Include/mblock.h:286-291)
   unsigned long nvals() const {
-    return isGeM() ? (unsigned long)n1 * n2 : (unsigned long)(n1 + n2) * bl_rank;
+    if (isGeM()) return (info.is_LtM || info.is_UtM) ? (unsigned long)n1 * (n1 + 1) / 2 : (unsigned long)n1 * n2;
+    return (unsigned long)(n1 + n2) * bl_rank;
   }
   unsigned long size() const { return sizeof(T) * nvals() + sizeof(mblock<T>); }
   unsigned getn1() const { return n1; }
   unsigned getn2() const { return n2; }
   bool isLrM() const { return info.is_LrM; }
   bool isGeM() const { return !info.is_LrM; }
+  // triangular factor leaves on the diagonal of an H-LU (Include/mblock.h:
+  // 322-328): n(n+1)/2 packed column-major values; an LtM leaf is unit lower
+  // with column j's diagonal entry holding the row permutation index
+  // (mblock::decomp_LU, H/mblock_D.cpp:1078-1107), a UtM leaf is upper
+  bool isLtM() const { return !info.is_LrM && info.is_LtM; }
+  bool isUtM() const { return !info.is_LrM && info.is_UtM; }
   unsigned rank() const {
     assert(isLrM());
     return bl_rank;
@@ -53,6 +65,7 @@ class mblock {
     delete[] data;
     data = nullptr;
     info.is_LrM = 1;
+    info.is_LtM = info.is_UtM = 0;
     bl_rank = 0;
     ahm_surface_invalidate();
   }
@@ -60,13 +73,30 @@ class mblock {
     delete[] data;
     bl_rank = k;
     info.is_LrM = 1;
+    info.is_LtM = info.is_UtM = 0;
     data = k ? new T[(size_t)k * (n1 + n2)] : nullptr;
     ahm_surface_invalidate();
   }
   void setGeM() {
     delete[] data;
-    info.is_LrM = 0;
+    info.is_LrM = info.is_LtM = info.is_UtM = 0;
     data = new T[(size_t)n1 * n2];
+    ahm_surface_invalidate();
+  }
+  void setLtM() {
+    assert(n1 == n2);
+    delete[] data;
+    info.is_LrM = info.is_UtM = 0;
+    info.is_LtM = 1;
+    data = new T[(size_t)n1 * (n1 + 1) / 2];
+    ahm_surface_invalidate();
+  }
+  void setUtM() {
+    assert(n1 == n2);
+    delete[] data;
+    info.is_LrM = info.is_LtM = 0;
+    info.is_UtM = 1;
+    data = new T[(size_t)n1 * (n1 + 1) / 2];
     ahm_surface_invalidate();
   }
   // copy a low-rank matrix U V^T without recompression (mblock::cpyLrM)
@@ -88,6 +118,10 @@ class mblock {
   // reference's Hermitian / triangular / symmetric kinds stops the program
   // with a message, as the reference does for an unknown status.
   void save(std::ofstream& os) const {
+    if (isLtM() || isUtM()) {
+      std::cerr << "mblock::save: triangular leaves are not supported" << std::endl;
+      std::exit(1);
+    }
     const unsigned status = isLrM() ? 0u : 4u;
     os.write((const char*)&n1, sizeof(unsigned));
     os.write((const char*)&n2, sizeof(unsigned));
