@@ -267,7 +267,8 @@ def leaf_array(b1, n1, b2, n2, rank, offset) -> np.ndarray:
 
 class TriFactor:
     """Triangular H-matrix factor on the device (ahm_trih): uplo 'L' (unit
-    lower, LtHGeM_solve) or 'U' (upper, UtHGeM_solve)."""
+    lower, LtHGeM_solve), 'U' (upper, UtHGeM_solve), or 'l' / 'u' for solves
+    with the transpose of that factor (LtHhGeM_solve / UtHhGeM_solve)."""
 
     def __init__(self, ctx: Context, handle, n: int, uplo: str):
         self.ctx, self.h, self.n, self.uplo = ctx, handle, n, uplo

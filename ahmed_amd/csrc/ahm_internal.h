@@ -149,7 +149,7 @@ struct ahm_hmat_s {
 struct ahm_trih_s {
   ahm_ctx ctx = nullptr;
   uint32_t n = 0;
-  bool lower = true;
+  int mode = 0;   // 0 L, 1 U, 2 U^T, 3 L^T (trih_kernel's modes)
   uint32_t n_diag = 0, n_lr = 0;
   uint64_t n_items = 0, nvals = 0, arena_vals = 0;
   ahm::DeviceBuf arena, items, perm, pend0, counters, tpart;

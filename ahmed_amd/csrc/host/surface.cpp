@@ -102,18 +102,19 @@ ahm_hmat device_matrix(blcluster* bl, mblock<T>** A, bool& changed) {
   return e.h;
 }
 
-// triangular factors: one cache entry per (tree, array, side)
+// triangular factors: one cache entry per (tree, array, solve: L U l u)
 struct TriEntry {
   ahm_trih h = nullptr;
   unsigned long epoch = 0;
 };
-std::map<std::pair<const void*, const void*>, TriEntry> g_tri[2];
+std::map<std::pair<const void*, const void*>, TriEntry> g_tri[4];
 
 ahm_trih device_factor(blcluster* bl, mblock<double>** M, char uplo) {
   std::lock_guard<std::mutex> lock(g_mu);
   ahm_ctx c = context();
   if (!c) return nullptr;
-  TriEntry& e = g_tri[uplo == 'U'][{bl, M}];
+  const int slot = uplo == 'L' ? 0 : uplo == 'U' ? 1 : uplo == 'l' ? 2 : 3;
+  TriEntry& e = g_tri[slot][{bl, M}];
   if (e.h && e.epoch == g_epoch) return e.h;
   if (e.h) ahm_trih_destroy(e.h);
   e.h = nullptr;
@@ -202,6 +203,14 @@ void LtHGeM_solve(blcluster* blL, mblock<double>** L, unsigned p, double* B, uns
 
 void UtHGeM_solve(blcluster* blU, mblock<double>** U, unsigned p, double* B, unsigned ldB) {
   tri_solve(blU, U, 'U', p, B, ldB, "UtHGeM_solve");
+}
+
+void LtHhGeM_solve(blcluster* blL, mblock<double>** L, unsigned p, double* B, unsigned ldB) {
+  tri_solve(blL, L, 'l', p, B, ldB, "LtHhGeM_solve");
+}
+
+void UtHhGeM_solve(blcluster* blU, mblock<double>** U, unsigned p, double* B, unsigned ldB) {
+  tri_solve(blU, U, 'u', p, B, ldB, "UtHhGeM_solve");
 }
 
 bool mltaGeHVec(double d, blcluster* bl, mblock<double>** A, double* x, double* y) {

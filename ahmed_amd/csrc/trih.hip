@@ -211,13 +211,15 @@ __device__ __forceinline__ void tri_store(const double* t, double* tri) {
   for (int q = 0; q < kTriPerThread; ++q) tri[threadIdx.x + q * kPanelThreads] = t[q];
 }
 
-// Forward substitution of wave 0's 64 rows through the 64x64 unit lower
-// triangle in LDS (zero on and above the diagonal): lane r holds z_r; step j
-// broadcasts z_j (final after step j-1) with readlane and every lane
-// subtracts its multiple.  The triangle's column entries are read from LDS
-// 16 steps ahead into registers, so a step costs only the readlane and the
-// fma (zero entries make the update a no-op above the diagonal).
-__device__ __forceinline__ double subst_lower(double zr, const double* tri) {
+// Substitution of wave 0's 64 rows through a 64x64 triangle in LDS: lane r
+// holds z_r; step j broadcasts z_j (final after the previous steps) with
+// readlane and every lane subtracts its multiple.  The triangle's entries
+// are read from LDS 16 steps ahead into registers, so a step costs only the
+// readlane and the fma.  UNIT triangles hold zeros on the diagonal and on
+// the other side of it (the update is then a no-op there); otherwise the
+// diagonal holds the reciprocal of the pivot and lane j scales z_j first.
+template <bool UNIT>
+__device__ __forceinline__ double subst_fwd(double zr, const double* tri) {
   const int lane = threadIdx.x;
 #pragma unroll
   for (int jb = 0; jb < kTri; jb += 16) {
@@ -225,14 +227,21 @@ __device__ __forceinline__ double subst_lower(double zr, const double* tri) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) tv[u] = tri[(jb + u) * kTri + lane];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) zr = fma(-tv[u], readlane_d(zr, jb + u), zr);
+    for (int u = 0; u < 16; ++u) {
+      const int j = jb + u;
+      if (UNIT) {
+        zr = fma(-tv[u], readlane_d(zr, j), zr);
+      } else {
+        zr = lane == j ? zr * tv[u] : zr;
+        zr = fma(lane > j ? -tv[u] : 0.0, readlane_d(zr, j), zr);
+      }
+    }
   }
   return zr;
 }
 
-// Backward substitution through the 64x64 upper triangle (reciprocal
-// diagonal): step j scales z_j, broadcasts it and the lanes above subtract.
-__device__ __forceinline__ double subst_upper(double zr, const double* tri) {
+template <bool UNIT>
+__device__ __forceinline__ double subst_bwd(double zr, const double* tri) {
   const int lane = threadIdx.x;
 #pragma unroll
   for (int jb = kTri - 16; jb >= 0; jb -= 16) {
@@ -242,9 +251,12 @@ __device__ __forceinline__ double subst_upper(double zr, const double* tri) {
 #pragma unroll
     for (int u = 15; u >= 0; --u) {
       const int j = jb + u;
-      zr = lane == j ? zr * tv[u] : zr;
-      const double xj = readlane_d(zr, j);
-      zr = fma(lane < j ? -tv[u] : 0.0, xj, zr);
+      if (UNIT) {
+        zr = fma(-tv[u], readlane_d(zr, j), zr);
+      } else {
+        zr = lane == j ? zr * tv[u] : zr;
+        zr = fma(lane < j ? -tv[u] : 0.0, readlane_d(zr, j), zr);
+      }
     }
   }
   return zr;
@@ -297,19 +309,19 @@ __device__ __forceinline__ double rect_apply(const double* v, uint32_t rows, con
 }
 
 __device__ __forceinline__ void solve_prefetch(const TriItem& it, const double* __restrict__ arena,
-                                               const uint32_t* __restrict__ perm, bool lower, TriSmem& sm,
-                                               SPre& pre, Pf& pf) {
+                                               const uint32_t* __restrict__ perm, bool forward, bool unit,
+                                               TriSmem& sm, SPre& pre, Pf& pf) {
   const int tid = threadIdx.x;
   const uint32_t n = it.m;
   const uint32_t np = (n + kTri - 1) / kTri;
   const double* blk = arena + it.data;
   double t[kTriPerThread];
   tri_load(blk, t);
-  // first rectangle: below the first triangle ('L') or above the last ('U')
-  const uint32_t rows0 = lower ? (n > kTri ? n - kTri : 0) : (np - 1) * kTri;
+  // first rectangle: below the first triangle (forward) or above the last
+  const uint32_t rows0 = forward ? (n > kTri ? n - kTri : 0) : (np - 1) * kTri;
   if (rows0 > 0 && rows0 <= kTri) rect_load(blk + kTri * kTri, rows0, pf + kTriPerThread);
   if (np > 1) tri_load(blk + kTri * kTri + (uint64_t)rows0 * kTri, pf);
-  if (lower) {
+  if (unit) {
 #pragma unroll
     for (int q = 0; q < kMaxDiag / kPanelThreads; ++q) {
       const uint32_t r = tid + q * kPanelThreads;
@@ -319,24 +331,26 @@ __device__ __forceinline__ void solve_prefetch(const TriItem& it, const double* 
   tri_store(t, sm.tri);
 }
 
-// one diagonal leaf, lower: z = P^T-gathered slice, unit lower substitution,
+// one diagonal leaf, forward (L, U^T): for the unit lower factor the slice
+// is first gathered through the row permutation (P^T L, mblock::ltr_solve);
 // 64 rows at a time, then the rectangle below each triangle
-__device__ void solve_lower(const TriItem& it, const double* __restrict__ arena, double* acc,
-                            TriSmem& sm, const SPre& pre, const Pf& pf) {
+template <bool UNIT>
+__device__ void solve_fwd(const TriItem& it, const double* __restrict__ arena, double* acc, TriSmem& sm,
+                          const SPre& pre, const Pf& pf) {
   const int tid = threadIdx.x;
   const uint32_t n = it.m, s0 = it.out;
   const uint32_t np = (n + kTri - 1) / kTri;
 #pragma unroll
   for (int q = 0; q < kMaxDiag / kPanelThreads; ++q) {
     const uint32_t r = tid + q * kPanelThreads;
-    if (r < np * kTri) sm.z[r] = r < n ? ld_c(acc + s0 + pre.ip[q]) : 0.0;
+    if (r < np * kTri) sm.z[r] = r < n ? ld_c(acc + s0 + (UNIT ? pre.ip[q] : r)) : 0.0;
   }
   const double* blk = arena + it.data;
   for (uint32_t p = 0; p < np; ++p) {
     const uint32_t c0 = p * kTri;
     const uint32_t below = n > c0 + kTri ? n - c0 - kTri : 0;
     __syncthreads();
-    if (tid < kTri) sm.z[c0 + tid] = subst_lower(sm.z[c0 + tid], sm.tri);
+    if (tid < kTri) sm.z[c0 + tid] = subst_fwd<UNIT>(sm.z[c0 + tid], sm.tri);
     __syncthreads();
     const double* rect = blk + kTri * kTri;
     const double* next = rect + (uint64_t)below * kTri;
@@ -361,10 +375,13 @@ __device__ void solve_lower(const TriItem& it, const double* __restrict__ arena,
   for (uint32_t r = tid; r < n; r += kPanelThreads) st_c(acc + s0 + r, sm.z[r]);
 }
 
-// one diagonal leaf, upper: backward substitution; panels are stored last
-// first, each a triangle followed by the rectangle above it
-__device__ void solve_upper(const TriItem& it, const double* __restrict__ arena, double* acc,
-                            TriSmem& sm, const Pf& pf) {
+// one diagonal leaf, backward (U, L^T): panels are stored last first, each
+// a triangle followed by the rectangle above it; for the unit factor (L^T)
+// the result is scattered back through the row permutation
+// (mblock::ltrh_solve)
+template <bool UNIT>
+__device__ void solve_bwd(const TriItem& it, const double* __restrict__ arena, double* acc, TriSmem& sm,
+                          const SPre& pre, const Pf& pf) {
   const int tid = threadIdx.x;
   const uint32_t n = it.m, s0 = it.out;
   const uint32_t np = (n + kTri - 1) / kTri;
@@ -374,7 +391,7 @@ __device__ void solve_upper(const TriItem& it, const double* __restrict__ arena,
     const uint32_t c0 = pp * kTri;
     const bool first = pp + 1 == np;
     __syncthreads();
-    if (tid < kTri) sm.z[c0 + tid] = subst_upper(sm.z[c0 + tid], sm.tri);
+    if (tid < kTri) sm.z[c0 + tid] = subst_bwd<UNIT>(sm.z[c0 + tid], sm.tri);
     __syncthreads();
     const double* rect = blk + kTri * kTri;
     const double* next = rect + (uint64_t)c0 * kTri;
@@ -396,12 +413,20 @@ __device__ void solve_upper(const TriItem& it, const double* __restrict__ arena,
     blk = next;
   }
   __syncthreads();
-  for (uint32_t r = tid; r < n; r += kPanelThreads) st_c(acc + s0 + r, sm.z[r]);
+#pragma unroll
+  for (int q = 0; q < kMaxDiag / kPanelThreads; ++q) {
+    const uint32_t r = tid + q * kPanelThreads;
+    if (r < n) st_c(acc + s0 + (UNIT ? pre.ip[q] : r), sm.z[r]);
+  }
 }
 
 __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
     const TriItem* __restrict__ items, uint32_t n_items, const double* __restrict__ arena,
-    const uint32_t* __restrict__ perm, double* acc, double* tpart, Counters cn, int lower) {
+    const uint32_t* __restrict__ perm, double* acc, double* tpart, Counters cn, int mode) {
+  // mode: 0 L (forward, unit, gather), 1 U (backward), 2 U^T (forward),
+  // 3 L^T (backward, unit, scatter)
+  const bool forward = mode == 0 || mode == 2;
+  const bool unit = mode == 0 || mode == 3;
   __shared__ TriSmem sm;
   const int tid = threadIdx.x;
   for (;;) {
@@ -417,7 +442,7 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
     SPre pre;
     Pf pf;
     if (it.type == kItemS)
-      solve_prefetch(it, arena, perm, lower != 0, sm, pre, pf);
+      solve_prefetch(it, arena, perm, forward, unit, sm, pre, pf);
     else
       gemv_prefetch(arena + it.data, it.m, it.W, it.ld, pf);
     if (tid == 0) {
@@ -435,10 +460,14 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
     __syncthreads();
     if (!sm.ok) return;
     if (it.type == kItemS) {
-      if (lower)
-        solve_lower(it, arena, acc, sm, pre, pf);
+      if (mode == 0)
+        solve_fwd<true>(it, arena, acc, sm, pre, pf);
+      else if (mode == 1)
+        solve_bwd<false>(it, arena, acc, sm, pre, pf);
+      else if (mode == 2)
+        solve_fwd<false>(it, arena, acc, sm, pre, pf);
       else
-        solve_upper(it, arena, acc, sm, pf);
+        solve_bwd<true>(it, arena, acc, sm, pre, pf);
       stores_done();
       __syncthreads();
       if (tid == 0) __hip_atomic_store(cn.solved + it.wait, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -502,10 +531,19 @@ struct TriPlan {
   uint64_t t_len = 0, nvals = 0;
 };
 
-int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* leaves, const double* src,
+// op: 'L' / 'U' solve with the stored lower / upper factor, 'l' / 'u' with
+// its transpose.  L and U^T sweep forward, U and L^T backward; the unit
+// factor (L) carries the row permutation.  A transposed factor is turned
+// into the solve's orientation here: leaf (b1, n1, b2, n2) becomes
+// (b2, n2, b1, n1), a dense block its transpose, U V^T becomes V U^T.
+int build_plan(uint32_t n, char op, uint32_t n_leaves, const ahm_leaf_desc* leaves, const double* src,
                TriPlan& P) {
-  const char uplo = lower ? 'L' : 'U';
-  const uint32_t dflag = lower ? AHM_LEAF_LTM : AHM_LEAF_UTM;
+  const bool stored_lower = op == 'L' || op == 'l';
+  const bool trans = op == 'l' || op == 'u';
+  const bool forward = stored_lower != trans;
+  const bool unit = stored_lower;
+  const char uplo = stored_lower ? 'L' : 'U';
+  const uint32_t dflag = stored_lower ? AHM_LEAF_LTM : AHM_LEAF_UTM;
 
   // diagonal leaves, sorted by first row; they must tile [0, n)
   std::vector<Diag> dg;
@@ -536,8 +574,8 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
   std::vector<uint32_t> row_diag(n);
   for (uint32_t i = 0; i < D; ++i)
     for (uint32_t r = 0; r < dg[i].n; ++r) row_diag[dg[i].s + r] = i;
-  // solve order: step(d) = d (lower) or D-1-d (upper)
-  auto step = [&](uint32_t d) { return lower ? d : D - 1 - d; };
+  // solve order: step(d) = d (forward) or D-1-d (backward)
+  auto step = [&](uint32_t d) { return forward ? d : D - 1 - d; };
 
   std::vector<double>& ar = P.arena;
   std::vector<uint32_t>& perm = P.perm;
@@ -569,7 +607,12 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
     auto up_at = [&](uint32_t r, uint32_t c) -> double {  // r <= c
       return pk[(uint64_t)c * (c + 1) / 2 + r];
     };
-    if (lower) {
+    // entry (r, c) of the triangle in the solve's orientation
+    auto elem = [&](uint32_t r, uint32_t c) -> double {
+      if (!trans) return stored_lower ? lo_at(r, c) : up_at(r, c);
+      return stored_lower ? lo_at(c, r) : up_at(c, r);
+    };
+    if (unit) {
       it.aux = (uint32_t)perm.size();
       std::vector<char> seen(nn, 0);
       for (uint32_t j = 0; j < nn; ++j) {
@@ -581,17 +624,32 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
         seen[ip] = 1;
         perm.push_back(ip);
       }
+    }
+    // the triangle's diagonal: zero for the unit factor, else the reciprocal
+    auto diag_entry = [&](uint32_t j, double& out) -> bool {
+      if (unit) {
+        out = 0.0;
+        return true;
+      }
+      const double dj = elem(j, j);
+      out = 1.0 / dj;
+      return dj != 0.0;
+    };
+    if (forward) {
       for (uint32_t p = 0; p < np; ++p) {
         const uint32_t c0 = p * kTri;
         const size_t base = ar.size();
         ar.resize(base + kTri * kTri, 0.0);
-        for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j)
-          for (uint32_t r = j + 1; r < kTri && c0 + r < nn; ++r) ar[base + j * kTri + r] = lo_at(c0 + r, c0 + j);
+        for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j) {
+          if (!diag_entry(c0 + j, ar[base + j * kTri + j]))
+            return fail(AHM_ERR_INVALID, "ahm_trih_upload: zero on the diagonal of the leaf at row %u", d.b1);
+          for (uint32_t r = j + 1; r < kTri && c0 + r < nn; ++r) ar[base + j * kTri + r] = elem(c0 + r, c0 + j);
+        }
         const uint32_t below = nn > c0 + kTri ? nn - c0 - kTri : 0;
         const size_t rb = ar.size();
         ar.resize(rb + (size_t)below * kTri, 0.0);
         for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j)
-          for (uint32_t r = 0; r < below; ++r) ar[rb + (size_t)j * below + r] = lo_at(c0 + kTri + r, c0 + j);
+          for (uint32_t r = 0; r < below; ++r) ar[rb + (size_t)j * below + r] = elem(c0 + kTri + r, c0 + j);
       }
     } else {
       for (uint32_t pp = np; pp-- > 0;) {
@@ -599,16 +657,14 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
         const size_t base = ar.size();
         ar.resize(base + kTri * kTri, 0.0);
         for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j) {
-          for (uint32_t r = 0; r < j; ++r) ar[base + j * kTri + r] = up_at(c0 + r, c0 + j);
-          const double dj = up_at(c0 + j, c0 + j);
-          if (dj == 0.0)
+          for (uint32_t r = 0; r < j; ++r) ar[base + j * kTri + r] = elem(c0 + r, c0 + j);
+          if (!diag_entry(c0 + j, ar[base + j * kTri + j]))
             return fail(AHM_ERR_INVALID, "ahm_trih_upload: zero on the diagonal of the leaf at row %u", d.b1);
-          ar[base + j * kTri + j] = 1.0 / dj;
         }
         const size_t rb = ar.size();
         ar.resize(rb + (size_t)c0 * kTri, 0.0);
         for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j)
-          for (uint32_t r = 0; r < c0; ++r) ar[rb + (size_t)j * c0 + r] = up_at(r, c0 + j);
+          for (uint32_t r = 0; r < c0; ++r) ar[rb + (size_t)j * c0 + r] = elem(r, c0 + j);
       }
     }
     it.wait2 = kNone;
@@ -616,34 +672,50 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
   }
   for (uint32_t i = 0; i < D; ++i) {
     // previous diagonal in solve order
-    if (lower && i > 0) sitems[i].wait2 = i - 1;
-    if (!lower && i + 1 < D) sitems[i].wait2 = i + 1;
+    if (forward && i > 0) sitems[i].wait2 = i - 1;
+    if (!forward && i + 1 < D) sitems[i].wait2 = i + 1;
   }
 
   // off-diagonal leaves
   std::vector<uint32_t> a_count;  // per low-rank leaf
   uint64_t t_len = 0;
   for (uint32_t l = 0; l < n_leaves; ++l) {
-    const ahm_leaf_desc& d = leaves[l];
-    if (d.flags) continue;
-    const uint64_t r_end = (uint64_t)d.b1 + d.n1, c_end = (uint64_t)d.b2 + d.n2;
-    if (r_end > n || c_end > n) return fail(AHM_ERR_INVALID, "ahm_trih_upload: leaf %u outside [0, %u)", l, n);
-    if (d.n1 == 0 || d.n2 == 0) continue;
-    if (lower ? c_end > d.b1 : r_end > d.b2)
+    const ahm_leaf_desc& ld = leaves[l];
+    if (ld.flags) continue;
+    if ((uint64_t)ld.b1 + ld.n1 > n || (uint64_t)ld.b2 + ld.n2 > n)
+      return fail(AHM_ERR_INVALID, "ahm_trih_upload: leaf %u outside [0, %u)", l, n);
+    if (ld.n1 == 0 || ld.n2 == 0) continue;
+    const bool stored_ok = stored_lower ? (uint64_t)ld.b2 + ld.n2 <= ld.b1 : (uint64_t)ld.b1 + ld.n1 <= ld.b2;
+    if (!stored_ok)
       return fail(AHM_ERR_INVALID, "ahm_trih_upload: leaf %u is not strictly %s the diagonal", l,
-                  lower ? "below" : "above");
+                  stored_lower ? "below" : "above");
+    // the leaf in the solve's orientation
+    ahm_leaf_desc d = ld;
+    if (trans) {
+      d.b1 = ld.b2;
+      d.n1 = ld.n2;
+      d.b2 = ld.b1;
+      d.n2 = ld.n1;
+    }
     // diagonal whose solve completes a column range [c0, c1)
-    auto ready_of = [&](uint32_t c0, uint32_t c1) { return lower ? row_diag[c1 - 1] : row_diag[c0]; };
+    auto ready_of = [&](uint32_t c0, uint32_t c1) { return forward ? row_diag[c1 - 1] : row_diag[c0]; };
     // diagonals touched by rows [r0, r1), and the step needing them first
     auto rows_of = [&](uint32_t r0, uint32_t r1, uint32_t& d0, uint32_t& d1) {
       d0 = row_diag[r0];
       d1 = row_diag[r1 - 1];
-      return lower ? step(d0) : step(d1);
+      return forward ? step(d0) : step(d1);
     };
     if (d.rank < 0) {
       nvals += (uint64_t)d.n1 * d.n2;
       const size_t base = ar.size();
-      ar.insert(ar.end(), src + d.offset, src + d.offset + (size_t)d.n1 * d.n2);
+      const double* D0 = src + ld.offset;
+      if (!trans) {
+        ar.insert(ar.end(), D0, D0 + (size_t)d.n1 * d.n2);
+      } else {  // transpose: entry (r, c) of the solve's block is (c, r) of the stored one
+        ar.resize(base + (size_t)d.n1 * d.n2);
+        for (uint32_t c = 0; c < d.n2; ++c)
+          for (uint32_t r = 0; r < d.n1; ++r) ar[base + r + (size_t)c * d.n1] = D0[c + (size_t)r * ld.n1];
+      }
       const uint32_t mrow = std::min<uint32_t>(d.n1, kPanelThreads);
       const uint32_t wc = std::min<uint32_t>(kMaxW, kItemCols * (kPanelThreads / mrow));
       for (uint32_t r0 = 0; r0 < d.n1; r0 += kPanelThreads) {
@@ -669,15 +741,18 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
       if (k > kMaxW) return fail(AHM_ERR_UNSUPPORTED, "ahm_trih_upload: rank %u above %u", k, kMaxW);
       nvals += (uint64_t)k * (d.n1 + d.n2);
       const uint32_t lr = (uint32_t)a_count.size();
-      const double* U = src + d.offset;
-      const double* V = U + (size_t)k * d.n1;
+      // stored U (ld.n1 x k) then V (ld.n2 x k); transposed, V U^T
+      const double* Us = src + ld.offset;
+      const double* Vs = Us + (size_t)k * ld.n1;
+      const double* U = trans ? Vs : Us;
+      const double* V = trans ? Us : Vs;
       const size_t ub = ar.size();
       ar.insert(ar.end(), U, U + (size_t)k * d.n1);
       const size_t vb = ar.size();
       ar.resize(vb + (size_t)k * d.n2);
       for (uint32_t c = 0; c < d.n2; ++c)
         for (uint32_t q = 0; q < k; ++q) ar[vb + (size_t)c * k + q] = V[(size_t)q * d.n2 + c];
-      // the leaf's first need: its first (lower) / last (upper) row diagonal
+      // the leaf's first need: its first (forward) / last (backward) row diagonal
       uint32_t e0, e1;
       const uint32_t leaf_need = rows_of(d.b1, d.b1 + d.n1, e0, e1);
       // A items: rank slices of <= 256 x column chunks
@@ -746,7 +821,7 @@ int build_plan(uint32_t n, bool lower, uint32_t n_leaves, const ahm_leaf_desc* l
     size_t k = 0;
     for (uint32_t st = 0; st < D; ++st) {
       for (; k < n_off && need[ord[k]] == st; ++k) q.push_back(its[ord[k]]);
-      q.push_back(sitems[lower ? st : D - 1 - st]);
+      q.push_back(sitems[forward ? st : D - 1 - st]);
     }
     if (k != n_off) return fail(AHM_ERR_INVALID, "ahm_trih_upload: internal: unplaced items");
   }
@@ -793,11 +868,12 @@ int ahm_trih_upload(ahm_ctx ctx, uint32_t n, char uplo, uint32_t n_leaves, const
   using namespace ahm;
   if (!ctx || !out || (n_leaves && (!leaves || !data)))
     return fail(AHM_ERR_INVALID, "ahm_trih_upload: NULL argument");
-  if (uplo != 'L' && uplo != 'U') return fail(AHM_ERR_INVALID, "ahm_trih_upload: uplo must be 'L' or 'U'");
+  if (uplo != 'L' && uplo != 'U' && uplo != 'l' && uplo != 'u')
+    return fail(AHM_ERR_INVALID, "ahm_trih_upload: uplo must be 'L', 'U', 'l' or 'u'");
   if (data_dtype != AHM_F64) return fail(AHM_ERR_UNSUPPORTED, "ahm_trih_upload: leaf values must be AHM_F64");
   *out = nullptr;
   TriPlan P;
-  AHM_TRY(build_plan(n, uplo == 'L', n_leaves, leaves, (const double*)data, P));
+  AHM_TRY(build_plan(n, uplo, n_leaves, leaves, (const double*)data, P));
   const std::vector<double>& ar = P.arena;
   const std::vector<uint32_t>& perm = P.perm;
   const std::vector<TriItem>& q = P.queue;
@@ -807,7 +883,7 @@ int ahm_trih_upload(ahm_ctx ctx, uint32_t n, char uplo, uint32_t n_leaves, const
   std::unique_ptr<ahm_trih_s> h(new ahm_trih_s);
   h->ctx = ctx;
   h->n = n;
-  h->lower = uplo == 'L';
+  h->mode = uplo == 'L' ? 0 : uplo == 'U' ? 1 : uplo == 'u' ? 2 : 3;
   h->n_diag = D;
   h->n_lr = P.n_lr;
   h->n_items = q.size();
@@ -868,7 +944,7 @@ int ahm_trih_solve(ahm_trih t, void* B, uint32_t ldb, uint32_t p) {
                        (const uint32_t*)t->pend0.p, D);
     hipLaunchKernelGGL(trih_kernel, dim3(t->grid), dim3(kPanelThreads), 0, s, (const TriItem*)t->items.p,
                        (uint32_t)t->n_items, (const double*)t->arena.p, (const uint32_t*)t->perm.p, x,
-                       (double*)t->tpart.p, cn, t->lower ? 1 : 0);
+                       (double*)t->tpart.p, cn, t->mode);
     AHM_HIP(hipGetLastError());
   }
   uint32_t ab = 0;

@@ -293,7 +293,11 @@ int ahm_gmres_batch(ahm_hmat h, const double* b, double* x, double eps, uint32_t
  * (Include/H.h:1383-1388) is an 'L' solve followed by a 'U' solve.
  * uplo 'L': the unit lower factor (diagonal leaves AHM_LEAF_LTM, the other
  * leaves strictly below the diagonal); 'U': the upper factor (AHM_LEAF_UTM,
- * other leaves strictly above).  Off-diagonal leaves are dense (rank < 0) or
+ * other leaves strictly above).  'l' / 'u': the same factor set up for
+ * solves with its transpose, LtHhGeM_solve / UtHhGeM_solve (H/TH_solve.cpp:
+ * 41-115, real case; mblock::ltrh_solve H/mblock_D.cpp:1208-1230 applies the
+ * permutation after the unit upper solve); HCholesky_solve (Include/H.h:
+ * 1353-1358) is a 'u' solve then a 'U' solve with the same leaves.  Off-diagonal leaves are dense (rank < 0) or
  * low rank (mblock layout, U then V); the diagonal leaves must tile [0, n).
  * data: host values of data_dtype (AHM_F64); the factor is stored in FP64.
  * One persistent launch per right-hand side: the diagonal leaves are solved

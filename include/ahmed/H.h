@@ -110,6 +110,26 @@ inline void HLU_solve(blcluster* bl, mblock<double>** L, mblock<double>** U, dou
   LtHVec_solve(bl, L, b);
   UtHVec_solve(bl, U, b);
 }
+// solves with the transposes (real case of H/TH_solve.cpp:41-115): L^T X = B
+// (mblock::ltrh_solve on the diagonal: the permutation is applied on the way
+// out) and U^T X = B; each direction is uploaded and cached separately
+void LtHhGeM_solve(blcluster* blL, mblock<double>** L, unsigned p, double* B, unsigned ldB);
+void UtHhGeM_solve(blcluster* blU, mblock<double>** U, unsigned p, double* B, unsigned ldB);
+inline void LtHhVec_solve(blcluster* blL, mblock<double>** L, double* b) {
+  LtHhGeM_solve(blL, L, 1, b, blL->getn1());
+}
+inline void UtHhVec_solve(blcluster* blU, mblock<double>** U, double* b) {
+  UtHhGeM_solve(blU, U, 1, b, blU->getn2());
+}
+// U^T U x = b (Include/H.h:1353-1358) and (L U)^T x = b (Include/H.h:1390-1396)
+inline void HCholesky_solve(blcluster* bl, mblock<double>** U, double* b) {
+  UtHhVec_solve(bl, U, b);
+  UtHVec_solve(bl, U, b);
+}
+inline void HLUh_solve(blcluster* bl, mblock<double>** L, mblock<double>** U, double* b) {
+  UtHhVec_solve(bl, U, b);
+  LtHhVec_solve(bl, L, b);
+}
 
 // storage in bytes and largest rank (H/initH.cpp:16-78)
 unsigned long sizeH(blcluster* bl, mblock<double>** A);

@@ -162,12 +162,14 @@ class _Factors:
         return dict(b1=lv["b1"], n1=lv["n1"], b2=lv["b2"], n2=lv["n2"], kind=kind, offset=off, data=data)
 
     def solve(self, which, B):
-        """which 'L' (LtHGeM_solve), 'U' (UtHGeM_solve) or 'LU' (HLU_solve per
-        column): returns X with op X = B; B is a vector or N x p."""
+        """which 'L' (LtHGeM_solve), 'U' (UtHGeM_solve), 'LU' (HLU_solve per
+        column), 'Lt' (LtHhGeM_solve), 'Ut' (UtHhGeM_solve), 'UtU'
+        (HCholesky_solve with U) or 'LUt' (HLUh_solve): returns X with
+        op X = B; B is a vector or N x p."""
         B = np.asarray(B, np.float64)
         vec = B.ndim == 1
         X = np.array(B.reshape(self.N, -1), order="F", copy=True)
-        w = {"L": 0, "U": 1, "LU": 2}[which]
+        w = {"L": 0, "U": 1, "LU": 2, "Lt": 3, "Ut": 4, "UtU": 5, "LUt": 6}[which]
         self.api.lib.ref_trisolve(self.h, w, _u(X.shape[1]), _ptr(X), _u(self.N))
         return X[:, 0].copy() if vec else X
 

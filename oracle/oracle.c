@@ -1282,20 +1282,76 @@ static void tri_diag_upper(unsigned n, const double* d, double* x)
   }
 }
 
+/* x[cols] -= leaf^T x[rows] (the leaf products of mltaGeHhGeM,
+ * H/mltaGeHVec.cpp:61-104, real case) */
+static void tri_apply_t(unsigned n1, unsigned n2, int kind, const double* d,
+                        const double* xr, double* xc)
+{
+  if (kind >= 0) {
+    const unsigned k = (unsigned)kind;
+    for (unsigned q = 0; q < k; ++q) {
+      const double e = -scpr(n1, d + (size_t)q * n1, xr);
+      axpy(n2, e, d + (size_t)k * n1 + (size_t)q * n2, xc);
+    }
+  } else {
+    const int mm = (int)n1, nn = (int)n2;
+    const double mone = -1.0;
+    dgemv_("T", &mm, &nn, &mone, d, &mm, xr, &I1, &DONE, xc, &I1);
+  }
+}
+
+/* U^T z = b on one diagonal leaf (dtpsv 'U','T','N', Include/blas.h:
+ * 3962-3968): forward, column j of the packed upper triangle holds rows 0..j */
+static void tri_diag_upper_t(unsigned n, const double* d, double* x)
+{
+  for (unsigned j = 0; j < n; ++j) {
+    const size_t o = (size_t)j * (j + 1) / 2;
+    double t = x[j];
+    for (unsigned i = 0; i < j; ++i) t -= d[o + i] * x[i];
+    x[j] = t / d[o + j];
+  }
+}
+
+/* (PL)^T z = b on one diagonal leaf (mblock::ltrh_solve, H/mblock_D.cpp:
+ * 1208-1230; dtpsv 'L','T','U'): unit upper backward substitution on a copy,
+ * then scattered back through the permutation */
+static void tri_diag_lower_t(unsigned n, const double* d, double* x, double* z)
+{
+  memcpy(z, x, sizeof(double) * n);
+  size_t* oc = (size_t*)xmalloc(sizeof(size_t) * n);
+  size_t o = 0;
+  for (unsigned j = 0; j < n; ++j) {
+    oc[j] = o;
+    o += n - j;
+  }
+  for (unsigned j = n; j-- > 0;) {
+    double t = z[j];
+    for (unsigned i = j + 1; i < n; ++i) t -= d[oc[j] + (i - j)] * z[i];
+    z[j] = t;
+  }
+  for (unsigned j = 0; j < n; ++j) x[(unsigned)d[oc[j]]] = z[j];
+  free(oc);
+}
+
 int or_trih_solve(unsigned n, char uplo, unsigned nleaves, const unsigned* b1,
                   const unsigned* n1, const unsigned* b2, const unsigned* n2,
                   const int* kind, const size_t* offset, const double* data,
                   unsigned p, double* B, unsigned ldB)
 {
   blas_init();
-  const int lower = uplo == 'L';
+  /* factor stored lower ('L', 'l') or upper ('U', 'u'); lowercase solves
+   * with its transpose; the sweep runs forward for L and U^T */
+  const int stored_lower = uplo == 'L' || uplo == 'l';
+  const int trans = uplo == 'l' || uplo == 'u';
+  const int forward = stored_lower != trans;
+  if (uplo != 'L' && uplo != 'U' && uplo != 'l' && uplo != 'u') return -1;
   or_key* dg = (or_key*)xmalloc(sizeof(or_key) * (nleaves + 1));
   or_key* od = (or_key*)xmalloc(sizeof(or_key) * (nleaves + 1));
   unsigned nd = 0, no = 0, maxn = 1;
   uint64_t covered = 0;
   for (unsigned l = 0; l < nleaves; ++l) {
     if (kind[l] == -2) continue;
-    if (kind[l] == (lower ? -3 : -4)) {
+    if (kind[l] == (stored_lower ? -3 : -4)) {
       if (b1[l] != b2[l] || n1[l] != n2[l]) goto bad;
       dg[nd].key = b1[l];
       dg[nd++].leaf = l;
@@ -1303,8 +1359,12 @@ int or_trih_solve(unsigned n, char uplo, unsigned nleaves, const unsigned* b1,
       if (n1[l] > maxn) maxn = n1[l];
     } else if (kind[l] >= -1) {
       /* lower: columns end before the rows start; upper: the reverse */
-      if (lower ? (b2[l] + n2[l] > b1[l]) : (b1[l] + n1[l] > b2[l])) goto bad;
-      od[no].key = lower ? b2[l] + n2[l] : b2[l];
+      if (stored_lower ? (b2[l] + n2[l] > b1[l]) : (b1[l] + n1[l] > b2[l])) goto bad;
+      /* the position whose solve makes the leaf's input final: forward
+       * sweeps apply a leaf once its input range has ended, backward
+       * sweeps once they reach its start */
+      const unsigned in0 = trans ? b1[l] : b2[l], in1 = trans ? b1[l] + n1[l] : b2[l] + n2[l];
+      od[no].key = forward ? in1 : in0;
       od[no++].leaf = l;
     } else {
       goto bad;
@@ -1316,24 +1376,28 @@ int or_trih_solve(unsigned n, char uplo, unsigned nleaves, const unsigned* b1,
   double* z = (double*)xmalloc(sizeof(double) * maxn);
   for (unsigned c = 0; c < p; ++c) {
     double* x = B + (size_t)c * ldB;
-    if (lower) {
+    if (forward) {
       unsigned q = 0;
       for (unsigned i = 0; i < nd; ++i) {
         const unsigned l = dg[i].leaf, e = b1[l] + n1[l];
-        tri_diag_lower(n1[l], data + offset[l], x + b1[l], z);
+        if (trans) tri_diag_upper_t(n1[l], data + offset[l], x + b1[l]);
+        else tri_diag_lower(n1[l], data + offset[l], x + b1[l], z);
         for (; q < no && od[q].key <= e; ++q) {
           const unsigned m = od[q].leaf;
-          tri_apply(n1[m], n2[m], kind[m], data + offset[m], x + b2[m], x + b1[m]);
+          if (trans) tri_apply_t(n1[m], n2[m], kind[m], data + offset[m], x + b1[m], x + b2[m]);
+          else tri_apply(n1[m], n2[m], kind[m], data + offset[m], x + b2[m], x + b1[m]);
         }
       }
     } else {
       unsigned q = no;
       for (unsigned i = nd; i-- > 0;) {
         const unsigned l = dg[i].leaf, s = b1[l];
-        tri_diag_upper(n1[l], data + offset[l], x + s);
+        if (trans) tri_diag_lower_t(n1[l], data + offset[l], x + s, z);
+        else tri_diag_upper(n1[l], data + offset[l], x + s);
         for (; q > 0 && od[q - 1].key >= s; --q) {
           const unsigned m = od[q - 1].leaf;
-          tri_apply(n1[m], n2[m], kind[m], data + offset[m], x + b2[m], x + b1[m]);
+          if (trans) tri_apply_t(n1[m], n2[m], kind[m], data + offset[m], x + b1[m], x + b2[m]);
+          else tri_apply(n1[m], n2[m], kind[m], data + offset[m], x + b2[m], x + b1[m]);
         }
       }
     }

@@ -124,10 +124,14 @@ unsigned or_cg_dense(unsigned n, const double* A, const double* b, double* x,
  * j holding its permutation entry on the diagonal) and lower ones; 'U':
  * diagonal (kind -4: packed upper triangle) and upper ones.  Off-diagonal
  * leaves: kind -1 dense, k >= 0 low rank (U then V); kind -2 is skipped.
- * Restated as a sweep over the diagonal leaves (ascending for 'L',
- * descending for 'U'); every off-diagonal leaf is applied whole as soon as
- * the diagonal leaves covering its columns are solved, which the
- * reference's recursion also guarantees, in another summation order.
+ * uplo 'l' / 'u' solve with the transpose of the stored factor:
+ * LtHhGeM_solve_ / UtHhGeM_solve_ (H/TH_solve.cpp:41-115, real case), leaf
+ * solves mblock::ltrh_solve (H/mblock_D.cpp:1208-1230, dtpsv 'L','T','U'
+ * then the permutation scatter) and utrh_solve (dtpsv 'U','T','N').
+ * Restated as a sweep over the diagonal leaves (ascending for L and U^T,
+ * descending for U and L^T); every off-diagonal leaf is applied whole as
+ * soon as the diagonal leaves covering its input range are solved, which
+ * the reference's recursion also guarantees, in another summation order.
  * Returns 0, or -1 for a malformed factor.                       [golden] */
 int or_trih_solve(unsigned n, char uplo, unsigned nleaves, const unsigned* b1,
                   const unsigned* n1, const unsigned* b2, const unsigned* n2,

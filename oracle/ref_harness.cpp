@@ -518,14 +518,23 @@ void* ref_factors_from_data(void* tp, const int* kind_l, const size_t* off_l, co
 }
 
 // which: 0 L X = B (LtHGeM_solve), 1 U X = B (UtHGeM_solve), 2 L U X = B
-// (HLU_solve per column); B is n x p, overwritten with X
+// (HLU_solve per column), 3 L^T X = B (LtHhGeM_solve), 4 U^T X = B
+// (UtHhGeM_solve), 5 U^T U X = B (HCholesky_solve per column), 6 (LU)^T X = B
+// (HLUh_solve per column); B is n x p, overwritten with X
 void ref_trisolve(void* p, int which, unsigned pp, double* B, unsigned ldB) {
   Factors* f = (Factors*)p;
   blcluster* bl = f->t->bl;
   if (which == 0) LtHGeM_solve(bl, f->L, pp, B, ldB);
   else if (which == 1) UtHGeM_solve(bl, f->U, pp, B, ldB);
+  else if (which == 3) LtHhGeM_solve(bl, f->L, pp, B, ldB);
+  else if (which == 4) UtHhGeM_solve(bl, f->U, pp, B, ldB);
   else
-    for (unsigned j = 0; j < pp; ++j) HLU_solve(bl, f->L, f->U, B + (size_t)j * ldB);
+    for (unsigned j = 0; j < pp; ++j) {
+      double* b = B + (size_t)j * ldB;
+      if (which == 2) HLU_solve(bl, f->L, f->U, b);
+      else if (which == 5) HCholesky_solve(bl, f->U, b);
+      else HLUh_solve(bl, f->L, f->U, b);
+    }
 }
 
 // GMRes (solvers/GMRES.cpp:80-181) with HLU_solve as the right preconditioner

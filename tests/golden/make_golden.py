@@ -140,8 +140,9 @@ def write_hlu(R):
     H-matrices: both factors (layouts in leaf idx order: kind -2 absent, -1
     dense, -3 packed unit lower with the permutation on its diagonal, -4
     packed upper, k >= 0 low rank), L\\b, U\\b, (LU)\\b for one and three
-    right-hand sides (LtHGeM_solve, UtHGeM_solve, HLU_solve), and GMRes with
-    HLU_solve as right preconditioner (solvers/GMRES.cpp:80-181)."""
+    right-hand sides (LtHGeM_solve, UtHGeM_solve, HLU_solve), the transposed
+    solves (LtHhGeM_solve, UtHhGeM_solve, HCholesky_solve with U, HLUh_solve),
+    and GMRes with HLU_solve as right preconditioner (solvers/GMRES.cpp:80-181)."""
     rec = {"blas": O.Oracle().blas_version()}
     for (name, geo, N, seed, bmin, eta, eps, heps, gm) in HLU_CASES:
         pts = points(geo, N, seed)
@@ -160,7 +161,12 @@ def write_hlu(R):
                 c[f"{which}_{k}"] = f[k]
             c[f"x{which}"] = F.solve(which, b)
             c[f"X{which}"] = F.solve(which, B)
+            # the transposed solves LtHhGeM_solve / UtHhGeM_solve
+            c[f"x{which}t"] = F.solve(which + "t", b)
+            c[f"X{which}t"] = F.solve(which + "t", B)
         c["xLU"] = F.solve("LU", b)
+        c["xUtU"] = F.solve("UtU", b)     # HCholesky_solve with U
+        c["xLUt"] = F.solve("LUt", b)     # HLUh_solve
         m, tol, ns = gm
         xg, rc, e, n = F.gmres(H, b, eps=tol, m=m, nsteps=ns)
         c.update(gmres_m=m, gmres_tol=tol, gmres_nsteps_in=ns, gmres_x=xg, gmres_rc=rc, gmres_eps=e,

@@ -2,7 +2,9 @@
 HLU_solve, Include/H.h:1383-1388) through the C ABI.
 
 * on the reference's own HLU factors (tests/golden/golden_hlu.npz): L\\b,
-  U\\b, (LU)\\b and three right-hand sides against the reference's outputs;
+  U\\b, (LU)\\b, the transposed solves (LtHhGeM_solve, UtHhGeM_solve,
+  HCholesky_solve with U, HLUh_solve) and three right-hand sides against the
+  reference's outputs;
 * on block factors of larger H-matrices (ahmed_amd.factors) against the
   oracle's restatement;
 * malformed factors are rejected on the host before anything runs.
@@ -62,6 +64,23 @@ def test_trih_matches_reference_golden(ctx, name):
     assert relerr(x.cpu().numpy(), g["xLU"]) <= 1e-12
     info = F["L"].info()
     assert info["n_diag"] == int((g["L_kind"] == -3).sum())
+    # transposed solves: the factors set up with 'l' / 'u'
+    Ft = {w: _lib.TriFactor.from_layout(ctx, N, w.lower(), factor(g, w)) for w in "LU"}
+    for w in "LU":
+        x = dev(g["b"])
+        Ft[w].solve(x)
+        assert relerr(x.cpu().numpy(), g[f"x{w}t"]) <= 1e-12, w + "t"
+        X = dev(g["B"].T)
+        Ft[w].solve(X)
+        assert relerr(X.cpu().numpy().T, g[f"X{w}t"]) <= 1e-12, w + "t"
+    x = dev(g["b"])            # HCholesky_solve with U: U^T then U
+    Ft["U"].solve(x)
+    F["U"].solve(x)
+    assert relerr(x.cpu().numpy(), g["xUtU"]) <= 1e-12
+    x = dev(g["b"])            # HLUh_solve: U^T then L^T
+    Ft["U"].solve(x)
+    Ft["L"].solve(x)
+    assert relerr(x.cpu().numpy(), g["xLUt"]) <= 1e-12
 
 
 @pytest.mark.parametrize("N,eps", [(16384, 1e-8), (32768, 1e-6)])
@@ -74,7 +93,7 @@ def test_trih_block_factors_match_oracle(ctx, N, eps, oracle_api):
     b = rng.standard_normal(N)
     B = rng.standard_normal((N, 2))
     print(f"[gpu] trih block factors N={N}", flush=True)
-    for w, F in (("L", Lf), ("U", Uf)):
+    for w, F in (("L", Lf), ("U", Uf), ("l", Lf), ("u", Uf)):
         T = _lib.TriFactor.from_layout(ctx, N, w, F)
         ref = oracle_api.trih_solve(F, w, b)
         for rep in range(2):   # the counters and slots are reset per call
@@ -101,6 +120,10 @@ def test_trih_rejects_malformed_factors(ctx):
     # the lower factor handed over as an upper one
     with pytest.raises(_lib.AhmError):
         _lib.TriFactor.upload(ctx, N, "U", leaves, data)
+    with pytest.raises(_lib.AhmError):
+        _lib.TriFactor.upload(ctx, N, "u", leaves, data)
+    with pytest.raises(_lib.AhmError, match="uplo"):
+        _lib.TriFactor.upload(ctx, N, "X", leaves, data)
     # a permutation entry out of range
     bad = data.copy()
     bad[int(leaves["offset"][d[0]])] = 1e6

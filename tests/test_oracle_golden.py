@@ -134,6 +134,17 @@ def test_triangular_solves_match_reference(name, oracle_api):
         assert np.max(np.abs(X - g[f"X{which}"])) <= 1e-13 * np.max(np.abs(g[f"X{which}"]))
     xlu = oracle_api.trih_solve(U, "U", oracle_api.trih_solve(L, "L", g["b"]))
     assert np.max(np.abs(xlu - g["xLU"])) <= 1e-13 * np.max(np.abs(g["xLU"]))
+    # transposed solves (LtHhGeM_solve / UtHhGeM_solve), HCholesky_solve with
+    # U (U^T then U) and HLUh_solve (U^T then L^T)
+    for which, F, op in (("L", L, "l"), ("U", U, "u")):
+        x = oracle_api.trih_solve(F, op, g["b"])
+        assert np.max(np.abs(x - g[f"x{which}t"])) <= 1e-13 * np.max(np.abs(g[f"x{which}t"]))
+        X = oracle_api.trih_solve(F, op, g["B"])
+        assert np.max(np.abs(X - g[f"X{which}t"])) <= 1e-13 * np.max(np.abs(g[f"X{which}t"]))
+    xc = oracle_api.trih_solve(U, "U", oracle_api.trih_solve(U, "u", g["b"]))
+    assert np.max(np.abs(xc - g["xUtU"])) <= 1e-13 * np.max(np.abs(g["xUtU"]))
+    xh = oracle_api.trih_solve(L, "l", oracle_api.trih_solve(U, "u", g["b"]))
+    assert np.max(np.abs(xh - g["xLUt"])) <= 1e-13 * np.max(np.abs(g["xLUt"]))
 
 
 @pytest.mark.parametrize("name", HLU_NAMES)
@@ -150,6 +161,10 @@ def test_triangular_factors_closed_form(name, oracle_api):
     assert np.max(np.abs(ML @ g["xL"] - b)) <= 1e-13 * np.max(np.abs(b))
     assert np.max(np.abs(MU @ g["xU"] - b)) <= 1e-12 * np.max(np.abs(b))
     assert np.max(np.abs(ML @ (MU @ g["xLU"]) - b)) <= 1e-12 * np.max(np.abs(b))
+    assert np.max(np.abs(ML.T @ g["xLt"] - b)) <= 1e-13 * np.max(np.abs(b))
+    assert np.max(np.abs(MU.T @ g["xUt"] - b)) <= 1e-12 * np.max(np.abs(b))
+    assert np.max(np.abs(MU.T @ (MU @ g["xUtU"]) - b)) <= 1e-12 * np.max(np.abs(b))
+    assert np.max(np.abs(MU.T @ (ML.T @ g["xLUt"]) - b)) <= 1e-12 * np.max(np.abs(b))
     xo = oracle_api.trih_solve(factor(g, "U"), "U", b)
     assert np.max(np.abs(xo - np.linalg.solve(MU, b))) <= 1e-11 * np.max(np.abs(xo))
 
