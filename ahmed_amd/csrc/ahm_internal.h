@@ -154,5 +154,6 @@ struct ahm_trih_s {
   uint64_t n_items = 0, nvals = 0, arena_vals = 0;
   ahm::DeviceBuf arena, items, perm, pend0, counters, tpart;
   uint64_t t_len = 0;
-  uint32_t grid = 0;
+  uint32_t grid = 0;     // resident workgroups of trih_kernel on the device
+  uint32_t chains = 0;   // right-hand sides the counters and slots are sized for
 };

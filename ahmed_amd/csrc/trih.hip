@@ -46,6 +46,7 @@ constexpr int kTri = 64;             // rows substituted per wavefront pass
 constexpr uint32_t kMaxDiag = 2048;  // largest diagonal leaf (LDS vector)
 constexpr uint32_t kMaxW = 2048;     // columns per gemv item (LDS vector)
 constexpr uint32_t kNone = 0xffffffffu;
+constexpr uint32_t kMaxChains = 16;  // right-hand sides solved by one launch
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 struct TriItem {
@@ -64,8 +65,8 @@ struct TriItem {
   uint32_t parts;  // B: slots summed per rank entry (the leaf's column chunks)
 };
 
-// counters layout: [0] queue head, [1] abort flag, then solved[n_diag],
-// tdone[n_lr], pending[n_diag]
+// counters layout per chain: [0] queue head, [1] abort flag (chain 0's is
+// the one all chains use), then solved[n_diag], tdone[n_lr], pending[n_diag]
 struct Counters {
   uint32_t* head;
   uint32_t* abort;
@@ -420,9 +421,20 @@ __device__ void solve_bwd(const TriItem& it, const double* __restrict__ arena, d
   }
 }
 
+// One launch solves several right-hand sides as independent chains: chain
+// blockIdx.y has its own vector, slots, queue head and counters (block of
+// cstride words) and shares the read-only items, arena and the abort flag.
 __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
     const TriItem* __restrict__ items, uint32_t n_items, const double* __restrict__ arena,
-    const uint32_t* __restrict__ perm, double* acc, double* tpart, Counters cn, int mode) {
+    const uint32_t* __restrict__ perm, double* acc, uint64_t ldb, double* tpart, uint64_t tstride,
+    uint32_t* counters, uint32_t cstride, uint32_t n_diag, uint32_t n_lr, int mode) {
+  {
+    const uint32_t j = blockIdx.y;
+    acc += j * ldb;
+    tpart += j * tstride;
+  }
+  uint32_t* cb = counters + (uint64_t)blockIdx.y * cstride;
+  const Counters cn{cb, counters + 1, cb + 2, cb + 2 + n_diag, cb + 2 + n_diag + n_lr};
   // mode: 0 L (forward, unit, gather), 1 U (backward), 2 U^T (forward),
   // 3 L^T (backward, unit, scatter)
   const bool forward = mode == 0 || mode == 2;
@@ -505,12 +517,14 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
   }
 }
 
-// counters <- (0, 0, solved = 0, tdone = 0, pending = pend0)
-__global__ void trih_reset_kernel(uint32_t* counters, uint32_t n_zero, const uint32_t* pend0,
+// counters of chain blockIdx.y <- (head 0, abort 0, solved = 0, tdone = 0,
+// pending = pend0)
+__global__ void trih_reset_kernel(uint32_t* counters, uint32_t cstride, uint32_t n_zero, const uint32_t* pend0,
                                   uint32_t n_diag) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_zero) counters[i] = 0;
-  else if (i < n_zero + n_diag) counters[i] = pend0[i - n_zero];
+  uint32_t* c = counters + (uint64_t)blockIdx.y * cstride;
+  if (i < n_zero) c[i] = 0;
+  else if (i < n_zero + n_diag) c[i] = pend0[i - n_zero];
 }
 
 // ---------------------------------------------------------------- host plan
@@ -895,8 +909,7 @@ int ahm_trih_upload(ahm_ctx ctx, uint32_t n, char uplo, uint32_t n_leaves, const
   AHM_TRY(h->items.alloc(std::max<size_t>(q.size(), 1) * sizeof(TriItem)));
   AHM_TRY(h->perm.alloc(std::max<size_t>(perm.size(), 1) * sizeof(uint32_t)));
   AHM_TRY(h->pend0.alloc(std::max<size_t>(D, 1) * sizeof(uint32_t)));
-  AHM_TRY(h->counters.alloc((2 + 2 * (size_t)D + P.n_lr) * sizeof(uint32_t)));
-  AHM_TRY(h->tpart.alloc(std::max<uint64_t>(P.t_len, 1) * sizeof(double)));
+
   if (!ar.empty()) AHM_HIP(hipMemcpy(h->arena.p, ar.data(), ar.size() * sizeof(double), hipMemcpyHostToDevice));
   if (!q.empty()) AHM_HIP(hipMemcpy(h->items.p, q.data(), q.size() * sizeof(TriItem), hipMemcpyHostToDevice));
   if (!perm.empty())
@@ -905,7 +918,7 @@ int ahm_trih_upload(ahm_ctx ctx, uint32_t n, char uplo, uint32_t n_leaves, const
   int per_cu = 0, n_cu = 0;
   AHM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trih_kernel, kPanelThreads, 0));
   AHM_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  h->grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(q.size(), (uint64_t)std::max(per_cu, 1) * n_cu));
+  h->grid = (uint32_t)std::max<uint64_t>(1, (uint64_t)std::max(per_cu, 1) * n_cu);   // resident workgroups
   *out = h.release();
   return AHM_OK;
 }
@@ -933,18 +946,27 @@ int ahm_trih_solve(ahm_trih t, void* B, uint32_t ldb, uint32_t p) {
   if (p == 0 || t->n == 0) return AHM_OK;
   AHM_HIP(hipSetDevice(t->ctx->device));
   hipStream_t s = t->ctx->stream;
-  uint32_t* cnt = (uint32_t*)t->counters.p;
   const uint32_t D = t->n_diag;
   const uint32_t n_zero = 2 + D + t->n_lr;
-  Counters cn{cnt, cnt + 1, cnt + 2, cnt + 2 + D, cnt + 2 + D + t->n_lr};
-  for (uint32_t j = 0; j < p; ++j) {
-    double* x = (double*)B + (uint64_t)j * (p > 1 ? ldb : 0);
-    const uint32_t tot = n_zero + D;
-    hipLaunchKernelGGL(trih_reset_kernel, dim3((tot + 255) / 256), dim3(256), 0, s, cnt, n_zero,
+  const uint32_t cstride = n_zero + D;
+  const uint32_t chains = std::min(p, kMaxChains);
+  // per-chain counters and slots, grown to the widest call
+  if (t->chains < chains) {
+    AHM_TRY(t->counters.alloc((size_t)cstride * chains * sizeof(uint32_t)));
+    AHM_TRY(t->tpart.alloc(std::max<uint64_t>(t->t_len, 1) * chains * sizeof(double)));
+    t->chains = chains;
+  }
+  uint32_t* cnt = (uint32_t*)t->counters.p;
+  const uint64_t step = p > 1 ? ldb : 0;
+  for (uint32_t j0 = 0; j0 < p; j0 += chains) {
+    const uint32_t c = std::min(chains, p - j0);
+    double* x = (double*)B + (uint64_t)j0 * step;
+    hipLaunchKernelGGL(trih_reset_kernel, dim3((cstride + 255) / 256, c), dim3(256), 0, s, cnt, cstride, n_zero,
                        (const uint32_t*)t->pend0.p, D);
-    hipLaunchKernelGGL(trih_kernel, dim3(t->grid), dim3(kPanelThreads), 0, s, (const TriItem*)t->items.p,
-                       (uint32_t)t->n_items, (const double*)t->arena.p, (const uint32_t*)t->perm.p, x,
-                       (double*)t->tpart.p, cn, t->mode);
+    const uint32_t gx = std::max<uint32_t>(1, std::min<uint64_t>(t->n_items, t->grid / c));
+    hipLaunchKernelGGL(trih_kernel, dim3(gx, c), dim3(kPanelThreads), 0, s, (const TriItem*)t->items.p,
+                       (uint32_t)t->n_items, (const double*)t->arena.p, (const uint32_t*)t->perm.p, x, step,
+                       (double*)t->tpart.p, std::max<uint64_t>(t->t_len, 1), cnt, cstride, D, t->n_lr, t->mode);
     AHM_HIP(hipGetLastError());
   }
   uint32_t ab = 0;

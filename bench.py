@@ -298,6 +298,15 @@ def bench_trih(args):
         log(f"{w} solve: {ms:.3f} ms, {res[w]['GB/s_alg']} GB/s algorithmic, "
             f"{info['n_diag']} diagonal leaves ({res[w]['us_per_diagonal_leaf']} us each), "
             f"{info['n_items']} items")
+        # 8 right-hand sides in one launch (independent chains sharing the factor)
+        Bm = torch.randn(8, N, dtype=torch.float64, device="cuda")
+        Xm = Bm.clone()
+        ms8 = timed(lambda: (Xm.copy_(Bm), T.solve(Xm)), torch, max(3, args.steps // 2), 1)
+        B8 = 8.0 * info["nvals"] + 8.0 * 2 * N * 8
+        res[w]["rhs8"] = {"ms": round(ms8, 3), "ms_per_rhs": round(ms8 / 8, 3),
+                          "GB/s_alg": round(B8 / (ms8 * 1e-3) / 1e9, 1),
+                          "frac_hbm_peak": round(B8 / (ms8 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"{w} solve, 8 RHS in one launch: {ms8:.3f} ms ({ms8 / 8:.3f} ms per RHS)")
     x = b.clone()
     ms_lu = timed(lambda: (x.copy_(b), fac["L"].solve(x), fac["U"].solve(x)), torch, args.steps, args.warmup)
     res["HLU_solve_ms"] = round(ms_lu, 3)

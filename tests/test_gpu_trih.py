@@ -105,6 +105,12 @@ def test_trih_block_factors_match_oracle(ctx, N, eps, oracle_api):
         X = dev(B.T)
         T.solve(X)
         assert relerr(X.cpu().numpy().T, oracle_api.trih_solve(F, w, B)) <= 1e-12
+    # more right-hand sides than one launch takes (16 chains): two launches
+    B17 = rng.standard_normal((N, 17))
+    T = _lib.TriFactor.from_layout(ctx, N, "L", Lf)
+    X = dev(B17.T)
+    T.solve(X)
+    assert relerr(X.cpu().numpy().T, oracle_api.trih_solve(Lf, "L", B17)) <= 1e-12
 
 
 def test_trih_rejects_malformed_factors(ctx):
