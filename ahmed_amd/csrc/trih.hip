@@ -47,6 +47,7 @@ constexpr uint32_t kMaxDiag = 2048;  // largest diagonal leaf (LDS vector)
 constexpr uint32_t kMaxW = 2048;     // columns per gemv item (LDS vector)
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMaxChains = 16;  // right-hand sides solved by one launch
+constexpr uint32_t kPushVals = 12288;  // LDS values of a dense block an S item applies itself
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 struct TriItem {
@@ -55,7 +56,7 @@ struct TriItem {
   uint32_t m;      // rows (A/B/D), or the diagonal leaf's size (S)
   uint32_t W;      // columns (A/B/D)
   uint32_t ld;     // column stride of the panel (A/B/D)
-  uint32_t in;     // A/D: first column in acc; B: slot of chunk 0, rank 0
+  uint32_t in;     // A/D: first column in acc; B: slot of chunk 0, rank 0; S: pushed block or kNone
   uint32_t out;    // A: first slot; B/D: first row in acc; S: first row
   uint32_t wait;   // S: diagonal index; A/D: diagonal that must be solved; B: low-rank leaf
   uint32_t wait2;  // S: previous diagonal in solve order (kNone for the first); B: A-item count
@@ -120,6 +121,7 @@ struct TriSmem {
   double z[kMaxDiag];          // S: the leaf's vector; A/B/D: the input slice
   double tri[kTri * kTri];     // S: one triangle
   double red[kPanelThreads];   // cross-group partial sums
+  double push[kPushVals];      // S: the dense block it applies after its solve
   uint32_t item;
   uint32_t ok;
 };
@@ -453,10 +455,30 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
     const TriItem it = items[idx];
     SPre pre;
     Pf pf;
-    if (it.type == kItemS)
+    if (it.type == kItemS) {
       solve_prefetch(it, arena, perm, forward, unit, sm, pre, pf);
-    else
+      if (it.in != kNone) {
+        // the block it pushes into the next leaf, staged in LDS while it waits
+        const TriItem tp = items[n_items + it.in];
+        const double* src = arena + tp.data;
+        const uint32_t nv = tp.m * tp.W;
+        for (uint32_t e0 = 0; e0 < nv; e0 += 16 * kPanelThreads) {
+          double v[16];
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const uint32_t e = e0 + tid + q * kPanelThreads;
+            v[q] = src[e < nv ? e : 0];
+          }
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const uint32_t e = e0 + tid + q * kPanelThreads;
+            if (e < nv) sm.push[e] = v[q];
+          }
+        }
+      }
+    } else {
       gemv_prefetch(arena + it.data, it.m, it.W, it.ld, pf);
+    }
     if (tid == 0) {
       bool ok;
       if (it.type == kItemS) {
@@ -480,9 +502,34 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
         solve_fwd<false>(it, arena, acc, sm, pre, pf);
       else
         solve_bwd<true>(it, arena, acc, sm, pre, pf);
+      TriItem tp{};
+      if (it.in != kNone) {
+        // the dense block that reads only this leaf's solution (still in
+        // sm.z) and feeds the next leaf: applied here, saving a hand-over
+        tp = items[n_items + it.in];
+        const int m = (int)tp.m;
+        const int G = kPanelThreads / m;
+        const int row = tid % m, g = tid / m;
+        const double* zc = sm.z + (tp.in - it.out);
+        double a = 0;
+        if (g < G)
+          for (int c = g; c < (int)tp.W; c += G) a = fma(sm.push[row + c * m], zc[c], a);
+        sm.red[tid] = g < G ? a : 0.0;
+        __syncthreads();
+        if (tid < m) {
+          double sum = 0;
+          for (int q = 0; q < G; ++q) sum += sm.red[q * m + tid];
+          unsafeAtomicAdd(acc + tp.out + tid, -sum);
+        }
+      }
       stores_done();
       __syncthreads();
-      if (tid == 0) __hip_atomic_store(cn.solved + it.wait, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (tid == 0) {
+        __hip_atomic_store(cn.solved + it.wait, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (it.in != kNone)
+          for (uint32_t d = tp.sig0; d <= tp.sig1; ++d)
+            __hip_atomic_fetch_sub(cn.pending + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
       continue;
     }
     // stage the input slice
@@ -540,7 +587,8 @@ struct TriPlan {
   std::vector<double> arena;
   std::vector<uint32_t> perm;
   std::vector<TriItem> queue;
-  std::vector<uint32_t> pend0;   // per diagonal: B/D tiles touching its rows
+  std::vector<TriItem> push;     // dense blocks applied by the S item that solves their columns
+  std::vector<uint32_t> pend0;   // per diagonal: B/D tiles and pushed blocks touching its rows
   uint32_t n_diag = 0, n_lr = 0;
   uint64_t t_len = 0, nvals = 0;
 };
@@ -682,6 +730,7 @@ int build_plan(uint32_t n, char op, uint32_t n_leaves, const ahm_leaf_desc* leav
       }
     }
     it.wait2 = kNone;
+    it.in = kNone;
     sitems[i] = it;
   }
   for (uint32_t i = 0; i < D; ++i) {
@@ -729,6 +778,32 @@ int build_plan(uint32_t n, char op, uint32_t n_leaves, const ahm_leaf_desc* leav
         ar.resize(base + (size_t)d.n1 * d.n2);
         for (uint32_t c = 0; c < d.n2; ++c)
           for (uint32_t r = 0; r < d.n1; ++r) ar[base + r + (size_t)c * d.n1] = D0[c + (size_t)r * ld.n1];
+      }
+      {
+        // a block whose columns lie in one diagonal leaf and whose rows are
+        // the next step's is applied by that leaf's S item (from LDS) right
+        // after its solve; not for L^T, whose solution leaves the S item
+        // permuted
+        const uint32_t dc = row_diag[d.b2];
+        uint32_t p0, p1;
+        const uint32_t nd = rows_of(d.b1, d.b1 + d.n1, p0, p1);
+        if (!(unit && !forward) && row_diag[d.b2 + d.n2 - 1] == dc && nd == step(dc) + 1 &&
+            d.n1 <= (uint32_t)kPanelThreads && (uint64_t)d.n1 * d.n2 <= kPushVals && sitems[dc].in == kNone) {
+          TriItem tp{};
+          tp.type = kItemD;
+          tp.data = base;
+          tp.m = d.n1;
+          tp.W = d.n2;
+          tp.ld = d.n1;
+          tp.in = d.b2;
+          tp.out = d.b1;
+          tp.wait = dc;
+          tp.sig0 = p0;
+          tp.sig1 = p1;
+          sitems[dc].in = (uint32_t)P.push.size();
+          P.push.push_back(tp);
+          continue;
+        }
       }
       const uint32_t mrow = std::min<uint32_t>(d.n1, kPanelThreads);
       const uint32_t wc = std::min<uint32_t>(kMaxW, kItemCols * (kPanelThreads / mrow));
@@ -842,6 +917,8 @@ int build_plan(uint32_t n, char op, uint32_t n_leaves, const ahm_leaf_desc* leav
   for (const TriItem& it : q)
     if (it.type == kItemB || it.type == kItemD)
       for (uint32_t d = it.sig0; d <= it.sig1; ++d) ++pend0[d];
+  for (const TriItem& it : P.push)
+    for (uint32_t d = it.sig0; d <= it.sig1; ++d) ++pend0[d];
 
   // every A/B/D tile fits the per-thread register prefetch
   for (const TriItem& it : q)
@@ -860,8 +937,13 @@ int build_plan(uint32_t n, char op, uint32_t n_leaves, const ahm_leaf_desc* leav
       else
         ok = solved[it.wait] != 0;
       if (!ok) return fail(AHM_ERR_INVALID, "ahm_trih_upload: internal: queue order violates a dependency");
-      if (it.type == kItemS) solved[it.wait] = 1;
-      else if (it.type == kItemA) ++tdone[it.sig0];
+      if (it.type == kItemS) {
+        solved[it.wait] = 1;
+        if (it.in != kNone)
+          for (uint32_t d = P.push[it.in].sig0; d <= P.push[it.in].sig1; ++d) --pend[d];
+      } else if (it.type == kItemA) {
+        ++tdone[it.sig0];
+      }
       else
         for (uint32_t d = it.sig0; d <= it.sig1; ++d) --pend[d];
     }
@@ -906,12 +988,15 @@ int ahm_trih_upload(ahm_ctx ctx, uint32_t n, char uplo, uint32_t n_leaves, const
   h->t_len = P.t_len;
   AHM_HIP(hipSetDevice(ctx->device));
   AHM_TRY(h->arena.alloc(std::max<size_t>(ar.size(), 1) * sizeof(double)));
-  AHM_TRY(h->items.alloc(std::max<size_t>(q.size(), 1) * sizeof(TriItem)));
+  AHM_TRY(h->items.alloc(std::max<size_t>(q.size() + P.push.size(), 1) * sizeof(TriItem)));
   AHM_TRY(h->perm.alloc(std::max<size_t>(perm.size(), 1) * sizeof(uint32_t)));
   AHM_TRY(h->pend0.alloc(std::max<size_t>(D, 1) * sizeof(uint32_t)));
 
   if (!ar.empty()) AHM_HIP(hipMemcpy(h->arena.p, ar.data(), ar.size() * sizeof(double), hipMemcpyHostToDevice));
   if (!q.empty()) AHM_HIP(hipMemcpy(h->items.p, q.data(), q.size() * sizeof(TriItem), hipMemcpyHostToDevice));
+  if (!P.push.empty())
+    AHM_HIP(hipMemcpy((TriItem*)h->items.p + q.size(), P.push.data(), P.push.size() * sizeof(TriItem),
+                      hipMemcpyHostToDevice));
   if (!perm.empty())
     AHM_HIP(hipMemcpy(h->perm.p, perm.data(), perm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   if (D) AHM_HIP(hipMemcpy(h->pend0.p, pend0.data(), D * sizeof(uint32_t), hipMemcpyHostToDevice));
