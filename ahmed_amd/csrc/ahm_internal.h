@@ -152,7 +152,7 @@ struct ahm_trih_s {
   int mode = 0;   // 0 L, 1 U, 2 U^T, 3 L^T (trih_kernel's modes)
   uint32_t n_diag = 0, n_lr = 0;
   uint64_t n_items = 0, nvals = 0, arena_vals = 0;
-  ahm::DeviceBuf arena, items, perm, pend0, counters, tpart;
+  ahm::DeviceBuf arena, items, perm, rinv, pend0, counters, tpart;
   uint64_t t_len = 0;
   uint32_t grid = 0;     // resident workgroups of trih_kernel on the device
   uint32_t chains = 0;   // right-hand sides the counters and slots are sized for

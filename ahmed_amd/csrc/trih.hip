@@ -62,7 +62,7 @@ struct TriItem {
   uint32_t wait2;  // S: previous diagonal in solve order (kNone for the first); B: A-item count
   uint32_t sig0;   // B/D: first diagonal whose pending count drops; A: low-rank leaf
   uint32_t sig1;   // B/D: last such diagonal
-  uint32_t aux;    // B: rank k (slot stride); S: permutation offset (lower)
+  uint32_t aux;    // B: rank k (slot stride); S: permutation (unit) or reciprocal-pivot (non-unit) offset
   uint32_t parts;  // B: slots summed per rank entry (the leaf's column chunks)
 };
 
@@ -214,14 +214,12 @@ __device__ __forceinline__ void tri_store(const double* t, double* tri) {
   for (int q = 0; q < kTriPerThread; ++q) tri[threadIdx.x + q * kPanelThreads] = t[q];
 }
 
-// Substitution of wave 0's 64 rows through a 64x64 triangle in LDS: lane r
-// holds z_r; step j broadcasts z_j (final after the previous steps) with
-// readlane and every lane subtracts its multiple.  The triangle's entries
-// are read from LDS 16 steps ahead into registers, so a step costs only the
-// readlane and the fma.  UNIT triangles hold zeros on the diagonal and on
-// the other side of it (the update is then a no-op there); otherwise the
-// diagonal holds the reciprocal of the pivot and lane j scales z_j first.
-template <bool UNIT>
+// Substitution of wave 0's 64 rows through a 64x64 unit triangle in LDS
+// (zero on the diagonal and on the other side of it; a non-unit factor
+// arrives row-scaled): lane r holds z_r; step j broadcasts z_j (final after
+// the previous steps) with readlane and every lane subtracts its multiple,
+// a no-op where the entry is zero.  The entries are read from LDS 16 steps
+// ahead into registers, so a step costs only the readlane and the fma.
 __device__ __forceinline__ double subst_fwd(double zr, const double* tri) {
   const int lane = threadIdx.x;
 #pragma unroll
@@ -230,20 +228,11 @@ __device__ __forceinline__ double subst_fwd(double zr, const double* tri) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) tv[u] = tri[(jb + u) * kTri + lane];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int j = jb + u;
-      if (UNIT) {
-        zr = fma(-tv[u], readlane_d(zr, j), zr);
-      } else {
-        zr = lane == j ? zr * tv[u] : zr;
-        zr = fma(lane > j ? -tv[u] : 0.0, readlane_d(zr, j), zr);
-      }
-    }
+    for (int u = 0; u < 16; ++u) zr = fma(-tv[u], readlane_d(zr, jb + u), zr);
   }
   return zr;
 }
 
-template <bool UNIT>
 __device__ __forceinline__ double subst_bwd(double zr, const double* tri) {
   const int lane = threadIdx.x;
 #pragma unroll
@@ -252,15 +241,7 @@ __device__ __forceinline__ double subst_bwd(double zr, const double* tri) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) tv[u] = tri[(jb + u) * kTri + lane];
 #pragma unroll
-    for (int u = 15; u >= 0; --u) {
-      const int j = jb + u;
-      if (UNIT) {
-        zr = fma(-tv[u], readlane_d(zr, j), zr);
-      } else {
-        zr = lane == j ? zr * tv[u] : zr;
-        zr = fma(lane < j ? -tv[u] : 0.0, readlane_d(zr, j), zr);
-      }
-    }
+    for (int u = 15; u >= 0; --u) zr = fma(-tv[u], readlane_d(zr, jb + u), zr);
   }
   return zr;
 }
@@ -271,6 +252,7 @@ __device__ __forceinline__ double subst_bwd(double zr, const double* tri) {
 // for 'L' the row permutation.
 struct SPre {
   uint32_t ip[kMaxDiag / kPanelThreads];
+  double rv[kMaxDiag / kPanelThreads];
 };
 // the register prefetch of an item: an A/B/D tile, or for an S item the
 // second triangle (first kTriPerThread) and the first rectangle (last kBatch)
@@ -312,8 +294,8 @@ __device__ __forceinline__ double rect_apply(const double* v, uint32_t rows, con
 }
 
 __device__ __forceinline__ void solve_prefetch(const TriItem& it, const double* __restrict__ arena,
-                                               const uint32_t* __restrict__ perm, bool forward, bool unit,
-                                               TriSmem& sm, SPre& pre, Pf& pf) {
+                                               const uint32_t* __restrict__ perm, const double* __restrict__ rinv,
+                                               bool forward, bool unit, TriSmem& sm, SPre& pre, Pf& pf) {
   const int tid = threadIdx.x;
   const uint32_t n = it.m;
   const uint32_t np = (n + kTri - 1) / kTri;
@@ -324,12 +306,11 @@ __device__ __forceinline__ void solve_prefetch(const TriItem& it, const double* 
   const uint32_t rows0 = forward ? (n > kTri ? n - kTri : 0) : (np - 1) * kTri;
   if (rows0 > 0 && rows0 <= kTri) rect_load(blk + kTri * kTri, rows0, pf + kTriPerThread);
   if (np > 1) tri_load(blk + kTri * kTri + (uint64_t)rows0 * kTri, pf);
-  if (unit) {
 #pragma unroll
-    for (int q = 0; q < kMaxDiag / kPanelThreads; ++q) {
-      const uint32_t r = tid + q * kPanelThreads;
-      pre.ip[q] = r < n ? perm[it.aux + r] : 0u;
-    }
+  for (int q = 0; q < kMaxDiag / kPanelThreads; ++q) {
+    const uint32_t r = tid + q * kPanelThreads;
+    if (unit) pre.ip[q] = r < n ? perm[it.aux + r] : 0u;
+    else pre.rv[q] = r < n ? rinv[it.aux + r] : 0.0;
   }
   tri_store(t, sm.tri);
 }
@@ -346,14 +327,14 @@ __device__ void solve_fwd(const TriItem& it, const double* __restrict__ arena, d
 #pragma unroll
   for (int q = 0; q < kMaxDiag / kPanelThreads; ++q) {
     const uint32_t r = tid + q * kPanelThreads;
-    if (r < np * kTri) sm.z[r] = r < n ? ld_c(acc + s0 + (UNIT ? pre.ip[q] : r)) : 0.0;
+    if (r < np * kTri) sm.z[r] = r < n ? (UNIT ? ld_c(acc + s0 + pre.ip[q]) : ld_c(acc + s0 + r) * pre.rv[q]) : 0.0;
   }
   const double* blk = arena + it.data;
   for (uint32_t p = 0; p < np; ++p) {
     const uint32_t c0 = p * kTri;
     const uint32_t below = n > c0 + kTri ? n - c0 - kTri : 0;
     __syncthreads();
-    if (tid < kTri) sm.z[c0 + tid] = subst_fwd<UNIT>(sm.z[c0 + tid], sm.tri);
+    if (tid < kTri) sm.z[c0 + tid] = subst_fwd(sm.z[c0 + tid], sm.tri);
     __syncthreads();
     const double* rect = blk + kTri * kTri;
     const double* next = rect + (uint64_t)below * kTri;
@@ -388,13 +369,17 @@ __device__ void solve_bwd(const TriItem& it, const double* __restrict__ arena, d
   const int tid = threadIdx.x;
   const uint32_t n = it.m, s0 = it.out;
   const uint32_t np = (n + kTri - 1) / kTri;
-  for (uint32_t r = tid; r < np * kTri; r += kPanelThreads) sm.z[r] = r < n ? ld_c(acc + s0 + r) : 0.0;
+#pragma unroll
+  for (int q = 0; q < kMaxDiag / kPanelThreads; ++q) {
+    const uint32_t r = tid + q * kPanelThreads;
+    if (r < np * kTri) sm.z[r] = r < n ? (UNIT ? ld_c(acc + s0 + r) : ld_c(acc + s0 + r) * pre.rv[q]) : 0.0;
+  }
   const double* blk = arena + it.data;
   for (uint32_t pp = np; pp-- > 0;) {
     const uint32_t c0 = pp * kTri;
     const bool first = pp + 1 == np;
     __syncthreads();
-    if (tid < kTri) sm.z[c0 + tid] = subst_bwd<UNIT>(sm.z[c0 + tid], sm.tri);
+    if (tid < kTri) sm.z[c0 + tid] = subst_bwd(sm.z[c0 + tid], sm.tri);
     __syncthreads();
     const double* rect = blk + kTri * kTri;
     const double* next = rect + (uint64_t)c0 * kTri;
@@ -428,7 +413,8 @@ __device__ void solve_bwd(const TriItem& it, const double* __restrict__ arena, d
 // cstride words) and shares the read-only items, arena and the abort flag.
 __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
     const TriItem* __restrict__ items, uint32_t n_items, const double* __restrict__ arena,
-    const uint32_t* __restrict__ perm, double* acc, uint64_t ldb, double* tpart, uint64_t tstride,
+    const uint32_t* __restrict__ perm, const double* __restrict__ rinv, double* acc, uint64_t ldb, double* tpart,
+    uint64_t tstride,
     uint32_t* counters, uint32_t cstride, uint32_t n_diag, uint32_t n_lr, int mode) {
   {
     const uint32_t j = blockIdx.y;
@@ -456,7 +442,7 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
     SPre pre;
     Pf pf;
     if (it.type == kItemS) {
-      solve_prefetch(it, arena, perm, forward, unit, sm, pre, pf);
+      solve_prefetch(it, arena, perm, rinv, forward, unit, sm, pre, pf);
       if (it.in != kNone) {
         // the block it pushes into the next leaf, staged in LDS while it waits
         const TriItem tp = items[n_items + it.in];
@@ -586,6 +572,7 @@ struct Diag {
 struct TriPlan {
   std::vector<double> arena;
   std::vector<uint32_t> perm;
+  std::vector<double> rinv;      // non-unit factors: reciprocal pivots, per diagonal leaf row
   std::vector<TriItem> queue;
   std::vector<TriItem> push;     // dense blocks applied by the S item that solves their columns
   std::vector<uint32_t> pend0;   // per diagonal: B/D tiles and pushed blocks touching its rows
@@ -687,46 +674,47 @@ int build_plan(uint32_t n, char op, uint32_t n_leaves, const ahm_leaf_desc* leav
         perm.push_back(ip);
       }
     }
-    // the triangle's diagonal: zero for the unit factor, else the reciprocal
-    auto diag_entry = [&](uint32_t j, double& out) -> bool {
-      if (unit) {
-        out = 0.0;
-        return true;
+    // A non-unit triangle is solved as the unit one D^{-1} T (rows scaled by
+    // the reciprocal pivots) on D^{-1} z: every stored entry of row r
+    // (triangle and rectangles) is multiplied by 1/T_rr here and the slice
+    // by the same reciprocals once it is loaded, so one substitution routine
+    // serves both kinds and no step divides.
+    std::vector<double> rs(nn, 1.0);
+    if (!unit) {
+      it.aux = (uint32_t)P.rinv.size();
+      for (uint32_t r = 0; r < nn; ++r) {
+        const double dr = elem(r, r);
+        if (dr == 0.0)
+          return fail(AHM_ERR_INVALID, "ahm_trih_upload: zero on the diagonal of the leaf at row %u", d.b1);
+        rs[r] = 1.0 / dr;
+        P.rinv.push_back(rs[r]);
       }
-      const double dj = elem(j, j);
-      out = 1.0 / dj;
-      return dj != 0.0;
-    };
+    }
+    auto sel = [&](uint32_t r, uint32_t c) { return elem(r, c) * rs[r]; };
     if (forward) {
       for (uint32_t p = 0; p < np; ++p) {
         const uint32_t c0 = p * kTri;
         const size_t base = ar.size();
         ar.resize(base + kTri * kTri, 0.0);
-        for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j) {
-          if (!diag_entry(c0 + j, ar[base + j * kTri + j]))
-            return fail(AHM_ERR_INVALID, "ahm_trih_upload: zero on the diagonal of the leaf at row %u", d.b1);
-          for (uint32_t r = j + 1; r < kTri && c0 + r < nn; ++r) ar[base + j * kTri + r] = elem(c0 + r, c0 + j);
-        }
+        for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j)
+          for (uint32_t r = j + 1; r < kTri && c0 + r < nn; ++r) ar[base + j * kTri + r] = sel(c0 + r, c0 + j);
         const uint32_t below = nn > c0 + kTri ? nn - c0 - kTri : 0;
         const size_t rb = ar.size();
         ar.resize(rb + (size_t)below * kTri, 0.0);
         for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j)
-          for (uint32_t r = 0; r < below; ++r) ar[rb + (size_t)j * below + r] = elem(c0 + kTri + r, c0 + j);
+          for (uint32_t r = 0; r < below; ++r) ar[rb + (size_t)j * below + r] = sel(c0 + kTri + r, c0 + j);
       }
     } else {
       for (uint32_t pp = np; pp-- > 0;) {
         const uint32_t c0 = pp * kTri;
         const size_t base = ar.size();
         ar.resize(base + kTri * kTri, 0.0);
-        for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j) {
-          for (uint32_t r = 0; r < j; ++r) ar[base + j * kTri + r] = elem(c0 + r, c0 + j);
-          if (!diag_entry(c0 + j, ar[base + j * kTri + j]))
-            return fail(AHM_ERR_INVALID, "ahm_trih_upload: zero on the diagonal of the leaf at row %u", d.b1);
-        }
+        for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j)
+          for (uint32_t r = 0; r < j; ++r) ar[base + j * kTri + r] = sel(c0 + r, c0 + j);
         const size_t rb = ar.size();
         ar.resize(rb + (size_t)c0 * kTri, 0.0);
         for (uint32_t j = 0; j < kTri && c0 + j < nn; ++j)
-          for (uint32_t r = 0; r < c0; ++r) ar[rb + (size_t)j * c0 + r] = elem(r, c0 + j);
+          for (uint32_t r = 0; r < c0; ++r) ar[rb + (size_t)j * c0 + r] = sel(r, c0 + j);
       }
     }
     it.wait2 = kNone;
@@ -990,6 +978,9 @@ int ahm_trih_upload(ahm_ctx ctx, uint32_t n, char uplo, uint32_t n_leaves, const
   AHM_TRY(h->arena.alloc(std::max<size_t>(ar.size(), 1) * sizeof(double)));
   AHM_TRY(h->items.alloc(std::max<size_t>(q.size() + P.push.size(), 1) * sizeof(TriItem)));
   AHM_TRY(h->perm.alloc(std::max<size_t>(perm.size(), 1) * sizeof(uint32_t)));
+  AHM_TRY(h->rinv.alloc(std::max<size_t>(P.rinv.size(), 1) * sizeof(double)));
+  if (!P.rinv.empty())
+    AHM_HIP(hipMemcpy(h->rinv.p, P.rinv.data(), P.rinv.size() * sizeof(double), hipMemcpyHostToDevice));
   AHM_TRY(h->pend0.alloc(std::max<size_t>(D, 1) * sizeof(uint32_t)));
 
   if (!ar.empty()) AHM_HIP(hipMemcpy(h->arena.p, ar.data(), ar.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -1050,7 +1041,8 @@ int ahm_trih_solve(ahm_trih t, void* B, uint32_t ldb, uint32_t p) {
                        (const uint32_t*)t->pend0.p, D);
     const uint32_t gx = std::max<uint32_t>(1, std::min<uint64_t>(t->n_items, t->grid / c));
     hipLaunchKernelGGL(trih_kernel, dim3(gx, c), dim3(kPanelThreads), 0, s, (const TriItem*)t->items.p,
-                       (uint32_t)t->n_items, (const double*)t->arena.p, (const uint32_t*)t->perm.p, x, step,
+                       (uint32_t)t->n_items, (const double*)t->arena.p, (const uint32_t*)t->perm.p,
+                       (const double*)t->rinv.p, x, step,
                        (double*)t->tpart.p, std::max<uint64_t>(t->t_len, 1), cnt, cstride, D, t->n_lr, t->mode);
     AHM_HIP(hipGetLastError());
   }
