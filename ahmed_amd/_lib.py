@@ -39,7 +39,7 @@ class HMatStats(C.Structure):
 
 
 class Kernel(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("pad", C.c_uint32), ("kappa", C.c_double), ("weight", C.c_double)]
+    _fields_ = [("kind", C.c_int32), ("flags", C.c_uint32), ("kappa", C.c_double), ("weight", C.c_double)]
 
 
 class AssemblyStats(C.Structure):
@@ -50,6 +50,7 @@ class AssemblyStats(C.Structure):
 
 
 KERNEL_LAPLACE, KERNEL_YUKAWA, KERNEL_EXPR = 0, 1, 2
+ASSEMBLE_ACAR = 1   # Kernel.flags: ACAr instead of ACA on the admissible leaves
 
 
 class AhmError(RuntimeError):
@@ -335,11 +336,12 @@ class HMatrix:
     @classmethod
     def assemble(cls, ctx: Context, tree: "Tree", kernel: int = KERNEL_LAPLACE, kappa: float = 0.0,
                  weight: float = 1.0, eps: float = 1e-6, rankmax: int = 0xFFFFFFFF, rank_hint=None,
-                 storage: int = F64, keep_leaves: bool = False, aca_ranks: bool = False):
+                 storage: int = F64, keep_leaves: bool = False, aca_ranks: bool = False,
+                 rows_only: bool = False):
         """GPU assembly (ACA + recompression + dense leaves) on a tree; returns
         (HMatrix, stats dict); stats["aca_rank"] holds per-leaf ACA ranks when
-        aca_ranks is set."""
-        k = Kernel(kernel, 0, kappa, weight)
+        aca_ranks is set; rows_only uses ACAr (apprx_sym's variant)."""
+        k = Kernel(kernel, ASSEMBLE_ACAR if rows_only else 0, kappa, weight)
         st = AssemblyStats()
         h = _p()
         hint = None

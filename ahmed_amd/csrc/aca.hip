@@ -287,6 +287,55 @@ __device__ bool aca_col_step(const KernelSpec& K, const double* P, const AcaBloc
   return true;
 }
 
+// ACAr_step (Include/ACA.h:578-655): row pivoting only; every column is a
+// pivot candidate (S stays zero) and no success counters are kept
+__device__ bool acar_step(const KernelSpec& K, const double* P, const AcaBlock& B, uint32_t& i0,
+                          uint32_t k, AcaState& st, int* Z, const int* S, double* U, double* V,
+                          double scale, double* coef, Shared& sh) {
+  double* pu = U + (size_t)k * B.n1;
+  double* pv = V + (size_t)k * B.n2;
+  ArgMax m;
+  for (;;) {
+    m = remainder_row(K, P, B, i0, k, U, V, pv, S, coef, sh);
+    if (!(fmax(m.v, 0.0) < 1e-14 * scale)) break;
+    __syncthreads();
+    if (threadIdx.x == 0) Z[i0] = -1;
+    __syncthreads();
+    if (++st.no < B.n1) i0 = nexti_dev(B.n1, Z, i0);
+    if (!(st.no < B.n1)) break;
+  }
+  if (m.v < 0.0 || fmax(m.v, 0.0) < 1e-14 * scale) return false;
+  const uint32_t j0 = (uint32_t)m.i;
+  const double sqrtpiv = sqrt(m.v);
+  __syncthreads();
+  const double sca = sqrtpiv / pv[j0];
+  __syncthreads();
+  double nv = 0.0;
+  for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads) {
+    const double e = (pv[j] *= sca);
+    nv += e * e;
+  }
+  const double nrmv2 = block_sum(nv, sh.dv);
+  remainder_col(K, P, B, j0, k, U, V, pu, Z, coef, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) Z[i0] = -1;
+  __syncthreads();
+  ++st.no;
+  const double dsca = 1.0 / sqrtpiv;
+  double nu = 0.0;
+  ArgMax a{-1.0, 0x7fffffff};
+  for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads) {
+    const double eabs = fabs(pu[i] *= dsca);
+    nu += eabs * eabs;
+    if (Z[i] >= 0) a = amax(a, ArgMax{eabs, (int)i});
+  }
+  const double nrmu2 = block_sum(nu, sh.dv);
+  a = block_argmax(a, sh.av, sh.ai);
+  if (a.v > 0.0) i0 = (uint32_t)a.i;
+  st.nrmlsk2 = nrmu2 * nrmv2;
+  return true;
+}
+
 // sum_{l<k} (U_l.U_k)(V_l.V_k): one wavefront per term, lanes over rows
 __device__ double cross_sum(const AcaBlock& B, uint32_t k, const double* U, const double* V,
                             Shared& sh) {
@@ -331,6 +380,26 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(const AcaBlock* __rest
   double scale = fabs(kentry(K, P, B.b1 + B.n1 / 2, B.b2 + B.n2 / 2));
   int status = 0;  // 0 fail, 1 success, 2 slab capacity reached
   const uint32_t kmax = B.kmax;
+  if (K.rows_only) {
+    // ACAr (Include/ACA.h:657-720); its loop compares the rows tried with n2
+    do {
+      if (k > B.kcap) {
+        status = 2;
+        goto DONE;
+      }
+      if (!acar_step(K, P, B, i0, k, st, Z, S, U, V, scale, coef, sh)) {
+        st.nrmlsk2 = 0.0;
+        goto SUCC;
+      }
+      {
+        const double sum = cross_sum(B, k, U, V, sh);
+        nrms2 += 2.0 * sum + st.nrmlsk2;
+      }
+      ++k;
+      scale = sqrt(st.nrmlsk2 / ((double)B.n1 * B.n2));
+    } while (st.no < B.n2 && k < kmax && st.nrmlsk2 >= eps * eps * nrms2);
+    goto SUCC;
+  }
   do {
     if (k > B.kcap) {
       status = 2;
@@ -718,6 +787,7 @@ struct Grow {
 KernelSpec make_kernel(const ahm_kernel& k) {
   KernelSpec K;
   K.kind = k.kind;
+  K.rows_only = (k.flags & AHM_ASSEMBLE_ACAR) ? 1 : 0;
   K.kappa = k.kappa;
   K.c = k.kind == AHM_KERNEL_EXPR ? k.weight : k.weight * 0.079577471545947667884;
   return K;

@@ -14,6 +14,7 @@ constexpr uint32_t kAcaCapMax = 256;
 
 struct KernelSpec {
   int kind;      // AHM_KERNEL_*
+  int rows_only; // ACAr instead of ACA on the admissible leaves
   double kappa;
   double c;      // weight/(4 pi) (Laplace, Yukawa) or weight (EXPR)
 };

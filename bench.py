@@ -448,6 +448,20 @@ def main():
     st = H.stats()
     log(f"GPU assembly {t_asm:.2f} s: {st['n_lowrank']} low-rank / {st['n_dense']} dense, "
         f"nvals {st['nvals']:.4g}, ACA retries {ast['n_retries']}")
+    # the same assembly with ACAr (apprx_sym's row-pivoted ACA) on the admissible leaves
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Hr, astr = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_LAPLACE, 0.0, weight, eps, rows_only=True)
+    torch.cuda.synchronize()
+    t_acar = time.perf_counter() - t0
+    str_ = Hr.stats()
+    acar = {"gpu_assembly_s": round(t_acar, 3), "aca_ms": round(astr["aca_ms"], 1),
+            "mean_aca_rank": astr["sum_aca_rank"] / max(1, astr["n_candidates"]),
+            "mean_rank": astr["sum_rank"] / max(1, astr["n_lowrank"]), "nvals": str_["nvals"],
+            "n_lowrank": str_["n_lowrank"]}
+    del Hr
+    log(f"GPU assembly with ACAr {t_acar:.2f} s (ACA batches {astr['aca_ms']:.0f} ms vs {ast['aca_ms']:.0f} ms), "
+        f"nvals {str_['nvals']:.4g}")
 
     # ---- 1 RHS, FP64 ---------------------------------------------------------
     x = torch.randn(N, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
@@ -574,8 +588,9 @@ def main():
         "matvec_transposed": {"ms": round(mst, 4), "GB/s": round(B_alg / (mst * 1e-3) / 1e9, 2)},
         "fp32_storage": {"ms": round(ms32, 4), "GB/s": round(B32 / (ms32 * 1e-3) / 1e9, 2),
                          "rel_err_vs_fp64": err32},
-        "assembly": {"tree_s": round(t_tree, 3), "gpu_assembly_s": round(t_asm, 3),
-                     "aca_retries": ast["n_retries"], "mean_rank": ast["sum_rank"] / max(1, ast["n_lowrank"])},
+        "assembly": {"tree_s": round(t_tree, 3), "gpu_assembly_s": round(t_asm, 3), "aca_ms": round(ast["aca_ms"], 1),
+                     "aca_retries": ast["n_retries"], "mean_aca_rank": ast["sum_aca_rank"] / max(1, ast["n_candidates"]),
+                     "mean_rank": ast["sum_rank"] / max(1, ast["n_lowrank"]), "acar": acar},
         "gmres": gm,
     }
     print(json.dumps(line), flush=True)

@@ -158,10 +158,15 @@ int ahm_mbls_read(const char* path, ahm_dtype dtype, uint32_t n_leaves, ahm_leaf
 #define AHM_KERNEL_EXPR    2   /* G = exp(-kappa r)/r        */
 typedef struct {
   int32_t kind;
-  uint32_t pad;
+  uint32_t flags;    /* 0, or AHM_ASSEMBLE_ACAR */
   double kappa;
   double weight;
 } ahm_kernel;
+/* approximate the admissible leaves with ACAr, the row-pivoted variant
+ * apprx_sym uses (Include/ACA.h:578-720, Include/apprx.h:87-131), instead of
+ * ACA (Include/ACA.h:217-350); storage and recompression stay as in
+ * apprx_unsym */
+#define AHM_ASSEMBLE_ACAR 1u
 
 typedef struct {
   uint64_t n_candidates;    /* admissible leaves that ran ACA              */

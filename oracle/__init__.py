@@ -205,6 +205,10 @@ class Oracle:
         L.or_kernel_scale.argtypes = [C.c_int, _d, _d, _p, _p, _u, _u, _u, _u]
         L.or_aca_dense.restype = C.c_int
         L.or_aca_dense.argtypes = [_p, _u, _u, _d, _u, _u, _d, C.POINTER(_u), _p, _p]
+        L.or_acar_dense.restype = C.c_int
+        L.or_acar_dense.argtypes = [_p, _u, _u, _d, _u, _u, _d, C.POINTER(_u), _p, _p]
+        L.or_hmat_assemble_method.restype = _p
+        L.or_hmat_assemble_method.argtypes = [_p, _p, C.c_int, _d, _d, _d, _u, C.c_int, C.c_int]
         L.or_addtrll.argtypes = [_u, _u, _u, _p, _p, _d, _u, C.POINTER(_u), _p, _p]
         L.or_hmat_assemble.restype = _p
         L.or_hmat_assemble.argtypes = [_p, _p, C.c_int, _d, _d, _d, _u, C.c_int]
@@ -294,7 +298,8 @@ class Oracle:
         return out.reshape(n2, n1).T
 
     # -- ACA / recompression on explicit data ----------------------------------
-    def aca_dense(self, A, eps, kmax, i0=0, scale=None):
+    def aca_dense(self, A, eps, kmax, i0=0, scale=None, rows_only=False):
+        """ACA (or ACAr with rows_only) on an explicit block: (ok, U, V)."""
         A = np.asfortranarray(A, np.float64)
         n1, n2 = A.shape
         if scale is None:
@@ -302,7 +307,8 @@ class Oracle:
         U = np.zeros((kmax + 1) * n1)
         V = np.zeros((kmax + 1) * n2)
         k = _u(0)
-        ok = self.lib.or_aca_dense(_ptr(A), n1, n2, _d(eps), kmax, i0, _d(scale), C.byref(k), _ptr(U), _ptr(V))
+        f = self.lib.or_acar_dense if rows_only else self.lib.or_aca_dense
+        ok = f(_ptr(A), n1, n2, _d(eps), kmax, i0, _d(scale), C.byref(k), _ptr(U), _ptr(V))
         k = k.value
         return bool(ok), U[:k * n1].reshape(k, n1).T.copy(), V[:k * n2].reshape(k, n2).T.copy()
 
@@ -321,10 +327,11 @@ class Oracle:
 
     # -- H-matrices ----------------------------------------------------------
     def assemble(self, tree, pts, kernel=KERNEL_LAPLACE, kappa=0.0, eps=1e-6, rankmax=1 << 30, nthreads=0,
-                 weight=1.0):
+                 weight=1.0, rows_only=False):
+        """apprx_unsym on every leaf (rows_only: ACAr instead of ACA)."""
         pts = np.ascontiguousarray(pts, np.float64)
-        h = self.lib.or_hmat_assemble(tree.h, _ptr(pts), kernel, _d(kappa), _d(weight), _d(eps), _u(rankmax),
-                                      nthreads)
+        h = self.lib.or_hmat_assemble_method(tree.h, _ptr(pts), kernel, _d(kappa), _d(weight), _d(eps),
+                                             _u(rankmax), nthreads, 1 if rows_only else 0)
         return _HMat(self, h, tree.N, tree)
 
     def from_leaves(self, N, b1, n1, b2, n2, rank, offset, data):
@@ -418,6 +425,10 @@ class Reference:
         L.ref_cg.argtypes = [_p, _p, _p, C.POINTER(_d), C.POINTER(_u)]
         L.ref_aca_dense.restype = C.c_int
         L.ref_aca_dense.argtypes = [_p, _u, _u, _d, _u, _u, _d, C.POINTER(_u), _p, _p]
+        L.ref_acar_dense.restype = C.c_int
+        L.ref_acar_dense.argtypes = [_p, _u, _u, _d, _u, _u, _d, C.POINTER(_u), _p, _p]
+        L.ref_hmat_assemble_acar.restype = _p
+        L.ref_hmat_assemble_acar.argtypes = [_p, C.c_int, _d, _d, _d, _u]
         L.ref_addtrll.argtypes = [_u, _u, _u, _p, _p, _d, _u, C.POINTER(_u), _p, _p]
         L.ref_hmat_save.restype = C.c_int
         L.ref_hmat_save.argtypes = [_p, C.c_char_p, _d, _u]
@@ -451,8 +462,9 @@ class Reference:
         return _Tree(self, h, len(pts))
 
     def assemble(self, tree, pts=None, kernel=KERNEL_LAPLACE, kappa=0.0, eps=1e-6, rankmax=1 << 30, nthreads=0,
-                 weight=1.0):
-        h = self.lib.ref_hmat_assemble(tree.h, kernel, _d(kappa), _d(weight), _d(eps), _u(rankmax))
+                 weight=1.0, rows_only=False):
+        f = self.lib.ref_hmat_assemble_acar if rows_only else self.lib.ref_hmat_assemble
+        h = f(tree.h, kernel, _d(kappa), _d(weight), _d(eps), _u(rankmax))
         return _HMat(self, h, tree.N, tree)
 
     def from_data(self, tree, rank, offset, data):
@@ -506,7 +518,7 @@ class Reference:
         self.lib.ref_hmat_data(H.h, _ptr(data))
         return dict(b1=lv["b1"], n1=lv["n1"], b2=lv["b2"], n2=lv["n2"], rank=rank, offset=off, data=data)
 
-    def aca_dense(self, A, eps, kmax, i0=0, scale=None):
+    def aca_dense(self, A, eps, kmax, i0=0, scale=None, rows_only=False):
         A = np.asfortranarray(A, np.float64)
         n1, n2 = A.shape
         if scale is None:
@@ -514,7 +526,8 @@ class Reference:
         U = np.zeros((kmax + 1) * n1)
         V = np.zeros((kmax + 1) * n2)
         k = _u(0)
-        ok = self.lib.ref_aca_dense(_ptr(A), n1, n2, _d(eps), kmax, i0, _d(scale), C.byref(k), _ptr(U), _ptr(V))
+        f = self.lib.ref_acar_dense if rows_only else self.lib.ref_aca_dense
+        ok = f(_ptr(A), n1, n2, _d(eps), kmax, i0, _d(scale), C.byref(k), _ptr(U), _ptr(V))
         k = k.value
         return bool(ok), U[:k * n1].reshape(k, n1).T.copy(), V[:k * n2].reshape(k, n2).T.copy()
 

@@ -659,6 +659,89 @@ DONE:
   return result;
 }
 
+/* ACAr_step, Include/ACA.h:578-655: row pivoting only (no column
+ * counters; every column is eligible), the cross scaled by sqrt|pivot| */
+static int acar_step(const matgen* G, unsigned b1, unsigned n1, unsigned b2, unsigned n2,
+                     unsigned* i0, unsigned k, unsigned* no, int* Z, double* U, double* V,
+                     double* nrmlsk2, double scale)
+{
+  unsigned l, j0 = 0;
+  double nrmu2, nrmv2, absmax;
+  double* const pu = U + (size_t)k * n1;
+  double* const pv = V + (size_t)k * n2;
+  do {
+    G->cmpbl(G->ctx, b1 + *i0, 1, b2, n2, pv);
+    for (l = 0; l < k; ++l) axpy(n2, -U[(size_t)l * n1 + *i0], V + (size_t)l * n2, pv);
+    absmax = 0.0;
+    for (l = 0; l < n2; ++l) {
+      const double eabs = fabs(pv[l]);
+      if (eabs > absmax) { absmax = eabs; j0 = l; }
+    }
+    if (absmax < 1e-14 * scale) {
+      Z[*i0] = -1;
+      if (++(*no) < n1) *i0 = nexti(n1, Z, *i0);
+    }
+  } while (absmax < 1e-14 * scale && *no < n1);
+  if (absmax < 1e-14 * scale) return 0;
+
+  const double sqrtpiv = sqrt(absmax);
+  nrmv2 = 0.0;
+  const double sca = sqrtpiv / pv[j0];
+  for (l = 0; l < n2; ++l) {
+    const double e = (pv[l] *= sca);
+    nrmv2 += e * e;
+  }
+  G->cmpbl(G->ctx, b1, n1, b2 + j0, 1, pu);
+  for (l = 0; l < k; ++l) axpy(n1, -V[(size_t)l * n2 + j0], U + (size_t)l * n1, pu);
+  Z[*i0] = -1;
+  ++(*no);
+  const double dsca = 1.0 / sqrtpiv;
+  absmax = nrmu2 = 0.0;
+  for (l = 0; l < n1; ++l) {
+    const double eabs = fabs(pu[l] *= dsca);
+    nrmu2 += eabs * eabs;
+    if (Z[l] >= 0 && eabs > absmax) { *i0 = l; absmax = eabs; }
+  }
+  *nrmlsk2 = nrmu2 * nrmv2;
+  return 1;
+}
+
+/* ACAr, Include/ACA.h:657-720.  The loop test compares the number of rows
+ * tried with n2, as the reference does. */
+static int acar(const matgen* G, unsigned b1, unsigned n1, unsigned b2, unsigned n2,
+                double eps, unsigned kmax, unsigned i0, unsigned* kout, double* U, double* V)
+{
+  unsigned l, no = 0, k = 0;
+  double nrmlsk2 = 0.0, nrms2 = 0.0, sum;
+  double scale = G->scale(G->ctx, b1, n1, b2, n2);
+  int* Z = (int*)xmalloc(sizeof(int) * n1);
+  int result;
+  for (l = 0; l < n1; ++l) Z[l] = 0;
+  do {
+    if (!acar_step(G, b1, n1, b2, n2, &i0, k, &no, Z, U, V, &nrmlsk2, scale)) {
+      nrmlsk2 = 0.0;
+      break;
+    }
+    sum = 0.0;
+    for (l = 0; l < k; ++l)
+      sum += scpr(n1, U + (size_t)l * n1, U + (size_t)k * n1) *
+             scpr(n2, V + (size_t)l * n2, V + (size_t)k * n2);
+    nrms2 += 2.0 * sum + nrmlsk2;
+    ++k;
+    scale = sqrt(nrmlsk2 / ((double)n1 * n2));
+  } while (no < n2 && k < kmax && nrmlsk2 >= eps * eps * nrms2);
+  {
+    const double epsa = sqrt(nrmlsk2 / nrms2);
+    result = !(epsa >= eps);   /* NaN (0/0) compares false: success */
+  }
+  free(Z);
+  *kout = k;
+  return result;
+}
+
+int or_acar_dense(const double* A, unsigned n1, unsigned n2, double eps, unsigned kmax,
+                  unsigned i0, double scale, unsigned* k, double* U, double* V);
+
 /* scale callback returning a fixed value (used by or_aca_dense) */
 static __thread double tl_scale;
 static double fixed_scale(void* c, unsigned a, unsigned b, unsigned d, unsigned e)
@@ -675,6 +758,16 @@ int or_aca_dense(const double* A, unsigned n1, unsigned n2, double eps, unsigned
   matgen G = {&c, dgen_cmpbl, fixed_scale};
   tl_scale = scale;
   return aca(&G, 0, n1, 0, n2, eps, kmax, i0, k, U, V);
+}
+
+int or_acar_dense(const double* A, unsigned n1, unsigned n2, double eps, unsigned kmax,
+                  unsigned i0, double scale, unsigned* k, double* U, double* V)
+{
+  blas_init();
+  dgen_ctx c = {A, n1};
+  matgen G = {&c, dgen_cmpbl, fixed_scale};
+  tl_scale = scale;
+  return acar(&G, 0, n1, 0, n2, eps, kmax, i0, k, U, V);
 }
 
 /* ======================================================================== */
@@ -790,9 +883,10 @@ static or_hmat* hmat_alloc(unsigned N, unsigned nl)
 static size_t leaf_nvals(unsigned n1, unsigned n2, int rank)
 { return rank < 0 ? (size_t)n1 * n2 : (size_t)rank * (n1 + n2); }
 
-/* apprx_unsym, Include/apprx.h:136-172 */
+/* apprx_unsym, Include/apprx.h:136-172; method 1 uses ACAr as apprx_sym
+ * does (Include/apprx.h:87-131) */
 static double* apprx_unsym(const matgen* G, unsigned b1, unsigned n1, unsigned b2, unsigned n2,
-                           int adm, unsigned i0, double eps, unsigned rankmax, int* rank)
+                           int adm, unsigned i0, double eps, unsigned rankmax, int method, int* rank)
 {
   /* 64-bit product: the reference's 32-bit n1*n2 wraps for n1*n2 >= 2^32 */
   unsigned maxk = (unsigned)((unsigned long long)n1 * n2 / (n1 + n2) + 4);
@@ -801,7 +895,8 @@ static double* apprx_unsym(const matgen* G, unsigned b1, unsigned n1, unsigned b
     unsigned k;
     double* U = (double*)xmalloc(sizeof(double) * (size_t)(maxk + 1) * n1);
     double* V = (double*)xmalloc(sizeof(double) * (size_t)(maxk + 1) * n2);
-    int succ = aca(G, b1, n1, b2, n2, eps, maxk, i0, &k, U, V);
+    int succ = method == 1 ? acar(G, b1, n1, b2, n2, eps, maxk, i0, &k, U, V)
+                           : aca(G, b1, n1, b2, n2, eps, maxk, i0, &k, U, V);
     double* out = NULL;
     if (succ) *rank = (int)addtrll(n1, n2, k, U, n1, V, n2, eps, k, &out);
     free(U);
@@ -817,6 +912,13 @@ static double* apprx_unsym(const matgen* G, unsigned b1, unsigned n1, unsigned b
 or_hmat* or_hmat_assemble(const or_tree* t, const double* pts, int kernel, double kappa,
                           double weight, double eps, unsigned rankmax, int nthreads)
 {
+  return or_hmat_assemble_method(t, pts, kernel, kappa, weight, eps, rankmax, nthreads, 0);
+}
+
+or_hmat* or_hmat_assemble_method(const or_tree* t, const double* pts, int kernel, double kappa,
+                                 double weight, double eps, unsigned rankmax, int nthreads,
+                                 int method)
+{
   blas_init();
   (void)pts;
   or_hmat* h = hmat_alloc(t->N, t->nl);
@@ -830,7 +932,7 @@ or_hmat* or_hmat_assemble(const or_tree* t, const double* pts, int kernel, doubl
   for (int l = 0; l < (int)t->nl; ++l) {
     const unsigned i0 = t->cl[t->rcl[l]].icom - t->b1[l];
     parts[l] = apprx_unsym(&G, t->b1[l], t->n1[l], t->b2[l], t->n2[l], t->adm[l], i0, eps,
-                           rankmax, &h->rank[l]);
+                           rankmax, method, &h->rank[l]);
   }
   size_t off = 0;
   for (unsigned l = 0; l < t->nl; ++l) {

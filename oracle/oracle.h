@@ -69,6 +69,12 @@ int or_aca_dense(const double* A, unsigned n1, unsigned n2, double eps,
                  unsigned kmax, unsigned i0, double scale, unsigned* k,
                  double* U, double* V);
 
+/* ---- ACAr on an explicit dense block (Include/ACA.h:578-720): the
+ * simplified row-pivoted ACA of apprx_sym; same buffers.      [golden+closed] */
+int or_acar_dense(const double* A, unsigned n1, unsigned n2, double eps,
+                  unsigned kmax, unsigned i0, double scale, unsigned* k,
+                  double* U, double* V);
+
 /* ---- recompression addtrll via cpyLrM_cmpr (H/mblock_D.cpp:357-465)
  * Input U (n1 x k), V (n2 x k); output rank *kt and Uo (n1 x kt), Vo (n2 x kt)
  * (buffers of n1*k, n2*k).                                    [golden+closed] */
@@ -81,6 +87,11 @@ void or_addtrll(unsigned n1, unsigned n2, unsigned k, const double* U,
 or_hmat* or_hmat_assemble(const or_tree* t, const double* pts, int kernel,
                           double kappa, double weight, double eps,
                           unsigned rankmax, int nthreads);
+/* method 0: ACA (apprx_unsym), 1: ACAr per admissible leaf (the
+ * approximation apprx_sym uses, Include/apprx.h:87-131)          [golden] */
+or_hmat* or_hmat_assemble_method(const or_tree* t, const double* pts, int kernel,
+                                 double kappa, double weight, double eps,
+                                 unsigned rankmax, int nthreads, int method);
 /* H-matrix from explicit leaves (mblock layout: dense n1*n2 col-major;
  * low-rank U (n1 x k) then V (n2 x k)); rank[i] < 0 marks a dense leaf. */
 or_hmat* or_hmat_from_leaves(unsigned N, unsigned nleaves, const unsigned* b1,

@@ -145,7 +145,7 @@ unsigned bbx_icom(const Tree* t, unsigned b1, unsigned n1) {
 // Include/apprx.h:136-172 restated for a plain blcluster with an explicit
 // start row (apprx_unsym itself needs a bemblcluster).
 void apprx_bbx(KernelGen& G, mblock<double>*& mbl, blcluster* bl, unsigned i0, double eps,
-               unsigned rankmax) {
+               unsigned rankmax, int method = 0) {
   unsigned n1 = bl->getn1(), n2 = bl->getn2(), b1 = bl->getb1(), b2 = bl->getb2();
   unsigned maxk = n1 * n2 / (n1 + n2) + 4;
   if (maxk > rankmax) maxk = rankmax;
@@ -154,7 +154,9 @@ void apprx_bbx(KernelGen& G, mblock<double>*& mbl, blcluster* bl, unsigned i0, d
   if (bl->isadm() && maxk) {
     unsigned k;
     double *U, *V;
-    succ = ::ACA(G, b1, n1, b2, n2, eps, maxk, i0, k, U, V);
+    // method 1: ACAr, the approximation apprx_sym uses (Include/apprx.h:109)
+    succ = method == 1 ? ::ACAr(G, b1, n1, b2, n2, eps, maxk, i0, k, U, V)
+                       : ::ACA(G, b1, n1, b2, n2, eps, maxk, i0, k, U, V);
     if (succ) mbl->cpyLrM_cmpr(k, U, n1, V, n2, eps, k);
     delete[] V;
     delete[] U;
@@ -259,6 +261,21 @@ void* ref_hmat_assemble(void* tp, int kernel, double kappa, double weight, doubl
     for (unsigned i = 0; i < t->nblcks; ++i)
       apprx_bbx(G, h->A[i], t->leaves[i], t->i0[i], eps, rankmax);
   }
+  return h;
+}
+
+// every leaf with ACAr on the admissible ones (both tree kinds, start row as
+// in ref_hmat_assemble), unsymmetric storage
+void* ref_hmat_assemble_acar(void* tp, int kernel, double kappa, double weight, double eps,
+                             unsigned rankmax) {
+  init();
+  Tree* t = (Tree*)tp;
+  HMat* h = new HMat;
+  h->t = t;
+  allocmbls(t->bl, h->A);
+  KernelGen G = {kernel, kappa, weight, t->pts.data(), t->op.data()};
+  Silence quiet;
+  for (unsigned i = 0; i < t->nblcks; ++i) apprx_bbx(G, h->A[i], t->leaves[i], t->i0[i], eps, rankmax, 1);
   return h;
 }
 
@@ -368,6 +385,20 @@ int ref_aca_dense(const double* A, unsigned n1, unsigned n2, double eps, unsigne
   DenseGen G = {A, n1, scale};
   double *Up, *Vp;
   bool ok = ::ACA(G, 0, n1, 0, n2, eps, kmax, i0, *k, Up, Vp);
+  std::memcpy(U, Up, sizeof(double) * (size_t)(kmax + 1) * n1);
+  std::memcpy(V, Vp, sizeof(double) * (size_t)(kmax + 1) * n2);
+  delete[] Up;
+  delete[] Vp;
+  return ok ? 1 : 0;
+}
+
+// ACAr on an explicit dense block (Include/ACA.h:657-720)
+int ref_acar_dense(const double* A, unsigned n1, unsigned n2, double eps, unsigned kmax,
+                   unsigned i0, double scale, unsigned* k, double* U, double* V) {
+  init();
+  DenseGen G = {A, n1, scale};
+  double *Up, *Vp;
+  bool ok = ::ACAr(G, 0, n1, 0, n2, eps, kmax, i0, *k, Up, Vp);
   std::memcpy(U, Up, sizeof(double) * (size_t)(kmax + 1) * n1);
   std::memcpy(V, Vp, sizeof(double) * (size_t)(kmax + 1) * n2);
   delete[] Up;

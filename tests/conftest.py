@@ -26,7 +26,8 @@ def oracle_api():
 def golden_names():
     return sorted(f[len("golden_"):-4] for f in os.listdir(GOLDEN)
                   if f.startswith("golden_") and f.endswith(".npz")
-                  and f not in ("golden_aca.npz", "golden_mbls.npz", "golden_transposed.npz", "golden_hlu.npz"))
+                  and f not in ("golden_aca.npz", "golden_mbls.npz", "golden_transposed.npz", "golden_hlu.npz",
+                                "golden_acar.npz"))
 
 
 def load_golden(name):

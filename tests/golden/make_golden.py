@@ -1,6 +1,6 @@
 """Write golden vectors from the COMPILED REFERENCE (run in the build container).
 
-Usage:  make -C oracle ref && python tests/golden/make_golden.py [mbls | transposed | hlu]
+Usage:  make -C oracle ref && python tests/golden/make_golden.py [mbls | transposed | hlu | acar]
 
 The reference's own sources (read in place under /root/reference, compiled by
 oracle/Makefile into oracle/_ref/libahmedref.so) produce every expected value
@@ -178,8 +178,30 @@ def write_hlu(R):
     np.savez_compressed(os.path.join(OUT, "golden_hlu.npz"), **rec)
 
 
+def write_acar(R):
+    """Every golden matrix assembled with ACAr on its admissible leaves (the
+    approximation apprx_sym uses, Include/apprx.h:87-131; storage and
+    recompression as apprx_unsym): golden_acar.npz holds <case>_rank and
+    <case>_data_sha256 (SHA-256 of all leaf values)."""
+    rec = {}
+    for (name, geo, N, seed, bmin, eta, kind, kern, kappa, w, eps, _, _) in CASES:
+        pts = points(geo, N, seed)
+        T = R.tree(pts, bmin, eta * eta, kind)
+        H = R.assemble(T, kernel=kern, kappa=kappa, eps=eps, weight=weight_of(w, geo, N), rows_only=True)
+        L = H.layout()
+        rec[f"{name}_rank"] = L["rank"]
+        rec[f"{name}_data_sha256"] = data_hash(L)
+        rec[f"{name}_x"] = np.random.default_rng(900 + seed).uniform(-1, 1, N)
+        rec[f"{name}_y"] = H.matvec(rec[f"{name}_x"])
+        print("acar", name, "lowrank", int((L["rank"] >= 0).sum()), "sum rank", int(L["rank"].clip(0).sum()))
+        del H, T
+    np.savez_compressed(os.path.join(OUT, "golden_acar.npz"), **rec)
+
+
 def main():
     R = O.Reference()
+    if sys.argv[1:] == ["acar"]:
+        return write_acar(R)
     if sys.argv[1:] == ["hlu"]:
         return write_hlu(R)
     if sys.argv[1:] == ["mbls"]:
@@ -239,12 +261,18 @@ def main():
         Uo, Vo = R.addtrll(U, V, eps)
         aca[f"trl{c}_U"] = Uo
         aca[f"trl{c}_V"] = Vo
-        print("aca", c, ok, U.shape[1], "->", Uo.shape[1])
+        # ACAr, the row-pivoted variant of apprx_sym (Include/ACA.h:657-720)
+        okr, Ur, Vr = R.aca_dense(A, eps, kmax, i0, rows_only=True)
+        aca[f"acar{c}_ok"] = okr
+        aca[f"acar{c}_U"] = Ur
+        aca[f"acar{c}_V"] = Vr
+        print("aca", c, ok, U.shape[1], "->", Uo.shape[1], "acar", okr, Ur.shape[1])
     aca["blas"] = ver
     np.savez_compressed(os.path.join(OUT, "golden_aca.npz"), **aca)
     write_mbls(R)
     write_transposed(R)
     write_hlu(R)
+    write_acar(R)
 
 
 if __name__ == "__main__":

@@ -78,6 +78,47 @@ def test_gpu_assembly_vs_reference(ctx, name):
     assert relerr(y.cpu().numpy(), g["y"]) <= 50 * eps
 
 
+@pytest.mark.parametrize("name", golden_names())
+def test_gpu_acar_assembly_vs_reference(ctx, name):
+    """AHM_ASSEMBLE_ACAR: ACAr (Include/ACA.h:578-720) on the admissible
+    leaves, against the reference's ACAr assembly (golden_acar.npz), by the
+    same bars as ACA."""
+    import os
+    from ahmed_amd import _lib
+    from conftest import GOLDEN
+    g = load_golden(name)
+    z = np.load(os.path.join(GOLDEN, "golden_acar.npz"), allow_pickle=False)
+    pts = golden_points(g)
+    print(f"[gpu] ACAr assembly {name}", flush=True)
+    T = _lib.Tree(pts, int(g["bmin"]), float(g["eta2"]), int(g["kind"]))
+    D, st = _lib.HMatrix.assemble(ctx, T, int(g["kernel"]), float(g["kappa"]), float(g["weight"]),
+                                  float(g["eps"]), keep_leaves=True, rows_only=True)
+    leaves, data = D.download()
+    ranks = leaves["rank"]
+    gr = z[f"{name}_rank"]
+    same = np.mean(ranks == gr)
+    print(f"[gpu]   ranks equal to the reference's on {same:.4f} of the leaves")
+    assert same >= 0.97, same
+    assert np.max(np.abs(ranks.astype(int) - gr.astype(int))) <= 2
+    op, _ = T.perm()
+    o = oracle_api()
+    eps = float(g["eps"])
+    for d, blk in leaf_blocks(leaves, data):
+        if d["rank"] < 0:
+            continue
+        b1, n1, b2, n2 = (int(d[f]) for f in ("b1", "n1", "b2", "n2"))
+        K = o.kernel_block(int(g["kernel"]), float(g["kappa"]), pts, op, b1, n1, b2, n2,
+                           weight=float(g["weight"]))
+        assert np.linalg.norm(blk - K) <= 20 * eps * np.linalg.norm(K)
+    Hc = o.from_leaves(T.N, leaves["b1"], leaves["n1"], leaves["b2"], leaves["n2"], ranks,
+                       leaves["offset"], data)
+    x = z[f"{name}_x"]
+    y = torch.zeros(T.N, dtype=torch.float64, device="cuda")
+    D.matvec(torch.as_tensor(x, device="cuda"), y)
+    assert relerr(y.cpu().numpy(), Hc.matvec(x)) <= 1e-12
+    assert relerr(y.cpu().numpy(), z[f"{name}_y"]) <= 50 * eps
+
+
 def test_rank_hint_changes_nothing(ctx):
     """Under-sized rank slabs make ACA rerun with a larger slab: same ranks."""
     from ahmed_amd import _lib, geometry

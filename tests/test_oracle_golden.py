@@ -108,6 +108,39 @@ def test_aca_and_addtrll_on_explicit_blocks(oracle_api):
         assert np.linalg.norm(P - Pg) <= 1e-12 * np.linalg.norm(Pg)
 
 
+def test_acar_on_explicit_blocks(oracle_api):
+    """ACAr (Include/ACA.h:657-720) on the same explicit blocks: identical to
+    the reference's crosses; where it succeeds, within 20 eps of the block."""
+    g = load_golden("aca")
+    for c in range(4):
+        A = g[f"aca{c}_A"]
+        eps, kmax, i0, scale = g[f"aca{c}_par"]
+        ok, U, V = oracle_api.aca_dense(A, eps, int(kmax), int(i0), scale, rows_only=True)
+        assert ok == bool(g[f"acar{c}_ok"])
+        assert U.shape == g[f"acar{c}_U"].shape
+        np.testing.assert_allclose(U, g[f"acar{c}_U"], rtol=0, atol=1e-13 * np.abs(A).max())
+        np.testing.assert_allclose(V, g[f"acar{c}_V"], rtol=0, atol=1e-13 * np.abs(A).max())
+        if ok:
+            assert np.linalg.norm(A - U @ V.T) <= 20 * eps * np.linalg.norm(A)
+
+
+def test_acar_assembly_matches_reference(case, oracle_api):
+    """Every golden matrix assembled with ACAr on its admissible leaves:
+    identical ranks and bit-identical leaf values (SHA-256) to the
+    reference's (tests/golden/golden_acar.npz)."""
+    g, T, _ = case
+    z = np.load(os.path.join(GOLDEN, "golden_acar.npz"), allow_pickle=False)
+    name = g["name"]
+    pts = golden_points(g)
+    H = oracle_api.assemble(T, pts, int(g["kernel"]), float(g["kappa"]), float(g["eps"]),
+                            weight=float(g["weight"]), rows_only=True)
+    L = H.layout()
+    assert np.array_equal(L["rank"], z[f"{name}_rank"])
+    assert hashlib.sha256(np.ascontiguousarray(L["data"]).tobytes()).hexdigest() == str(z[f"{name}_data_sha256"])
+    y = H.matvec(z[f"{name}_x"])
+    assert np.max(np.abs(y - z[f"{name}_y"])) <= 1e-13 * np.max(np.abs(z[f"{name}_y"]))
+
+
 # ---- triangular H-solves (LtHGeM_solve / UtHGeM_solve / HLU_solve) on the
 # reference's own HLU factors (tests/golden/golden_hlu.npz)
 HLU_NAMES = ["cube640", "sphere700"]
