@@ -54,6 +54,11 @@ struct ReduceTask {
   uint32_t nchunks;
 };
 
+// Zeroed values after the last panel of an arena: the MFMA products load
+// whole 16-row tiles of 16 columns, so a panel (at most kPanelThreads rows)
+// may be read up to 15 columns and 15 values past its end.
+constexpr uint64_t kArenaSlack = 16 * (uint64_t)kPanelThreads + 64;
+
 // zsrc entry: bit 31 set -> index into t, else index into x
 constexpr uint32_t kFromT = 0x80000000u;
 

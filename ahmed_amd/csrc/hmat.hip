@@ -230,20 +230,29 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemvp_kernel(
 }
 
 // ------------------------------------------------------------------------
-// p right-hand sides: the same panels as a matrix product on the MFMA units.
+// p >= 17 right-hand sides: the same panels as a matrix product on the MFMA
+// units.
 //
 // out(m x p) (+)= P (m x W) * Z (W x p), one task per workgroup and 32
-// right-hand sides per blockIdx.y.  The output is tiled 16 x 16; wavefront w
-// owns the row tiles w, w+4, w+8, w+12 and both 16-column tiles of its RHS
-// group, so every panel value is loaded once per 32 right-hand sides.  Z is
+// right-hand sides per blockIdx.y.  The output is tiled 16 x 16: nrt row
+// tiles times two column tiles (right-hand sides 0-15, 16-31 of the group).
+// The 2*nrt units are spread evenly over the four wavefronts: row tile
+// w + 4q goes to wavefront w whole (both column tiles, one panel load for
+// two MFMAs); when nrt is not a multiple of 4 the last row tiles are split by
+// column tile so that no wavefront holds more than ceil(2*nrt/4) units.  Z is
 // gathered into LDS in chunks of kMmChunk columns from interleaved copies of
-// x and t (the right-hand sides of one source row contiguous); the panel
-// loads of four k-steps are issued before their MFMAs.  Per k-step of 4:
+// x and t (the 32 right-hand sides of one source row contiguous).  Per k-step
+// of 4:
 //   A (16 x 4 of P):  lane l holds P[row0 + l%16][k0 + l/16]
 //   B (4 x 16 of Z):  lane l holds Z[k0 + l/16][col0 + l%16]
 //   C (16 x 16):      f64: lane l, value r -> (l/16 + 4r, l%16)
 //                     f32: lane l, value r -> (4(l/16) + r, l%16)
 // (lane layouts of v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 on gfx950).
+// A lane's panel element of a k-step sits at a fixed offset l/16*m + l%16
+// from a base that is uniform over the wavefront, so the loads take a scalar
+// base and a 32-bit lane offset.  A tile's rows beyond m are loaded too (the
+// next column's values, or the zeroed slack after the last panel): they only
+// reach output rows beyond m, which are never stored.
 constexpr int kMmChunk = 128;
 constexpr int kMmRhs = 32;
 
@@ -266,123 +275,229 @@ template <> struct MmAcc<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane / 16) + r; }
 };
 
-// Z chunk into LDS: zs[c][j] = source row of column c (value j of 2^LPW),
-// zero for c >= cw.  Each thread loads its source-list entries, then its values.
-template <typename T, int LPW>
+// Z chunk into LDS: zs[c][j] = value j of the source row of column c, zero
+// for c >= cw.  In two passes of 8 columns per thread: each thread loads its
+// source-list entries, then its values, so that the two rounds of loads are
+// each in flight together; columns beyond cw read column 0's (valid) source.
+template <typename T>
 __device__ __forceinline__ void mm_gather(T (&zs)[kMmChunk][kMmRhs + 1], const uint32_t* __restrict__ zsrc,
                                           int cw, const T* __restrict__ xg, const T* __restrict__ tg) {
-  constexpr int kPw = 1 << LPW;
-  constexpr int kG = (kMmChunk << LPW) / kPanelThreads;
-  uint32_t src[kG];
+  constexpr int kCs = kPanelThreads / kMmRhs;           // columns per step
+  constexpr int kG = kMmChunk / kCs / 2;                 // steps per pass
+  const int j = threadIdx.x & (kMmRhs - 1);
+  const int cb = threadIdx.x / kMmRhs;
 #pragma unroll
-  for (int i = 0; i < kG; ++i) {
-    const int c = (int)(threadIdx.x + i * kPanelThreads) >> LPW;
-    src[i] = zsrc[c < cw ? c : 0];
-  }
-  T val[kG];
+  for (int h = 0; h < 2; ++h) {
+    uint32_t src[kG];
 #pragma unroll
-  for (int i = 0; i < kG; ++i) {
-    const int j = (int)(threadIdx.x + i * kPanelThreads) & (kPw - 1);
-    const uint32_t sv = src[i];
-    const T* row = (sv & kFromT) ? tg + (uint64_t)(sv & ~kFromT) * kPw : xg + (uint64_t)sv * kPw;
-    val[i] = row[j];
+    for (int i = 0; i < kG; ++i) {
+      const int c = cb + (h * kG + i) * kCs;
+      src[i] = zsrc[c < cw ? c : 0];
+    }
+    T val[kG];
+#pragma unroll
+    for (int i = 0; i < kG; ++i) {
+      const uint32_t sv = src[i];
+      const T* row = (sv & kFromT) ? tg + (uint64_t)(sv & ~kFromT) * kMmRhs : xg + (uint64_t)sv * kMmRhs;
+      val[i] = row[j];
+    }
+#pragma unroll
+    for (int i = 0; i < kG; ++i) {
+      const int c = cb + (h * kG + i) * kCs;
+      zs[c][j] = c < cw ? val[i] : (T)0;
+    }
+  }
+}
+
+// Panel loads of round r (16 columns, four k-steps) for the wavefront's NW
+// whole row tiles (w, w+4, ...) and, with HALF, row tile hrt.  lob: the
+// lane's byte offset within a k-step.  The last round of a task may run past
+// its W columns (into the next task's values or the arena's slack); those
+// columns are zeroed in mm_mma.
+template <typename T, int NW, bool HALF, int NA>
+__device__ __forceinline__ void mm_load(T (&A)[4][NA], const char* __restrict__ P, uint32_t r, uint32_t m,
+                                        uint32_t lob, int w, int hrt) {
+  const char* Pr = P + (uint64_t)r * 16u * m * sizeof(T);
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const char* Ps = Pr + (uint64_t)(4u * st * m) * sizeof(T) + lob;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) A[st][q] = *(const T*)(Ps + (uint64_t)(16u * (w + 4 * q)) * sizeof(T));
+    if (HALF) A[st][NA - 1] = *(const T*)(Ps + (uint64_t)(16u * hrt) * sizeof(T));
+  }
+}
+
+// MFMAs of round r on the loaded panel values and the Z rows of the round in
+// LDS; whole tiles against both column tiles (TWO), the half tile against
+// column tile hct.  LAST: columns at or beyond W use zero.
+template <typename T, int NW, bool HALF, bool TWO, bool LAST, int NA>
+__device__ __forceinline__ void mm_mma(const T (&A)[4][NA], uint32_t r, uint32_t W, int li, int lk, int hct,
+                                       const T (&zs)[kMmChunk][kMmRhs + 1],
+                                       typename MmAcc<T>::type (&acc)[4][2],
+                                       typename MmAcc<T>::type& acch) {
+  const int kb = (int)(r & (kMmChunk / 16 - 1)) * 16;
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const int k = kb + 4 * st + lk;   // zero-filled beyond the chunk's width
+    const T b0 = zs[k][li];
+    const T b1 = zs[k][16 + li];
+    const bool in = !LAST || r * 16u + 4u * st + lk < W;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      const T a = in ? A[st][q] : (T)0;
+      acc[q][0] = MmAcc<T>::mma(a, b0, acc[q][0]);
+      if (TWO) acc[q][1] = MmAcc<T>::mma(a, b1, acc[q][1]);
+    }
+    if (HALF) acch = MmAcc<T>::mma(in ? A[st][NA - 1] : (T)0, hct ? b1 : b0, acch);
+  }
+}
+
+template <typename T, bool PHASE_B>
+__device__ __forceinline__ void mm_store(const GemvTask& tk, int rt, int ct, const typename MmAcc<T>::type& c,
+                                         int lane, uint32_t rhs0, uint32_t p, T* __restrict__ tg,
+                                         T* __restrict__ y, uint32_t ldy, T* __restrict__ part,
+                                         uint32_t part_len, T alpha) {
+  const int jj = ct * 16 + (lane & 15);
+  const uint32_t j = rhs0 + jj;
+  if (j >= p) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t row = rt * 16 + MmAcc<T>::row(lane, r);
+    if (row >= tk.m) continue;
+    const T v = c[r];
+    if (tk.partial)
+      part[(uint64_t)j * part_len + tk.out + row] = v;
+    else if (PHASE_B)
+      y[(uint64_t)j * ldy + tk.out + row] += alpha * v;
+    else
+      tg[(uint64_t)(tk.out + row) * kMmRhs + jj] = v;
+  }
+}
+
+// The whole task for one wavefront shape (NW whole row tiles, HALF, TWO).
+// The panel loads run one round (16 columns) ahead of the MFMAs, across the
+// Z chunk boundaries too; every wavefront of the workgroup walks the same
+// rounds, so the workgroup barriers of the Z gathers pair up across shapes.
+template <typename T, bool PHASE_B, int NW, bool HALF, bool TWO>
+__device__ __forceinline__ void mm_task(const GemvTask& tk, const uint32_t* __restrict__ zsrc,
+                                        const T* __restrict__ arena, const T* __restrict__ xg, T* __restrict__ tg,
+                                        T (&zs)[kMmChunk][kMmRhs + 1], int w, int hrt, int hct, uint32_t rhs0,
+                                        uint32_t p, T* __restrict__ y, uint32_t ldy, T* __restrict__ part,
+                                        uint32_t part_len, T alpha) {
+  typedef typename MmAcc<T>::type acc_t;
+  constexpr int NA = NW + (HALF ? 1 : 0) > 0 ? NW + (HALF ? 1 : 0) : 1;
+  constexpr bool WORK = NW > 0 || HALF;
+  const int lane = threadIdx.x & 63;
+  const uint32_t m = tk.m, W = tk.W;
+  acc_t acc[4][2], acch = acc_t{0, 0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+  const int li = lane & 15, lk = lane >> 4;
+  const uint32_t lob = ((uint32_t)lk * m + (uint32_t)li) * (uint32_t)sizeof(T);
+  const char* P = (const char*)(arena + tk.data);
+  const uint32_t R = (W + 15) / 16;
+  auto gather = [&](uint32_t r) {
+    if (r % (kMmChunk / 16)) return;
+    const uint32_t c0 = r * 16;
+    __syncthreads();
+    mm_gather<T>(zs, zsrc + tk.zoff + c0, (int)min((uint32_t)kMmChunk, W - c0), xg, tg);
+    __syncthreads();
+  };
+  // up to two whole row tiles: the panel loads of round r + 1 are issued
+  // before the MFMAs of round r; wider shapes load and multiply round by
+  // round to stay within the register budget of three wavefronts per SIMD
+  constexpr bool PIPE = NW <= 2;
+  T A0[4][NA], A1[4][PIPE ? NA : 1];
+  if (WORK && PIPE) mm_load<T, NW, HALF, NA>(A0, P, 0, m, lob, w, hrt);
+  uint32_t r = 0;
+  for (; r + 1 < R; ++r) {
+    gather(r);
+    if (!WORK) continue;
+    if (PIPE) {
+      mm_load<T, NW, HALF, PIPE ? NA : 1>(A1, P, r + 1, m, lob, w, hrt);
+    } else {
+      mm_load<T, NW, HALF, NA>(A0, P, r, m, lob, w, hrt);
+    }
+    mm_mma<T, NW, HALF, TWO, false, NA>(A0, r, W, li, lk, hct, zs, acc, acch);
+    if (PIPE) {
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int q = 0; q < NA; ++q) A0[st][q] = A1[st][PIPE ? q : 0];
+    }
+  }
+  gather(r);   // the last round (W >= 1, so R >= 1)
+  if (WORK) {
+    if (!PIPE) mm_load<T, NW, HALF, NA>(A0, P, r, m, lob, w, hrt);
+    mm_mma<T, NW, HALF, TWO, true, NA>(A0, r, W, li, lk, hct, zs, acc, acch);
   }
 #pragma unroll
-  for (int i = 0; i < kG; ++i) {
-    const int e = threadIdx.x + i * kPanelThreads;
-    const int c = e >> LPW;
-    zs[c][e & (kPw - 1)] = c < cw ? val[i] : (T)0;
+  for (int q = 0; q < NW; ++q) {
+    mm_store<T, PHASE_B>(tk, w + 4 * q, 0, acc[q][0], lane, rhs0, p, tg, y, ldy, part, part_len, alpha);
+    if (TWO) mm_store<T, PHASE_B>(tk, w + 4 * q, 1, acc[q][1], lane, rhs0, p, tg, y, ldy, part, part_len, alpha);
   }
+  if (HALF) mm_store<T, PHASE_B>(tk, hrt, hct, acch, lane, rhs0, p, tg, y, ldy, part, part_len, alpha);
 }
 
 template <typename T, bool PHASE_B>
 __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
     const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
     const T* __restrict__ arena, const T* __restrict__ xi, uint32_t n_x, T* __restrict__ ti,
-    uint32_t t_len, int pw, T* __restrict__ y, uint32_t ldy, T* __restrict__ part,
+    uint32_t t_len, T* __restrict__ y, uint32_t ldy, T* __restrict__ part,
     uint32_t part_len, T alpha, uint32_t p) {
-  typedef typename MmAcc<T>::type acc_t;
   __shared__ T zs[kMmChunk][kMmRhs + 1];
   const GemvTask tk = tasks[blockIdx.x];
   const uint32_t g = blockIdx.y, rhs0 = g * kMmRhs;
-  const T* xg = xi + (uint64_t)g * n_x * pw;
-  T* tg = ti + (uint64_t)g * t_len * pw;
-  const int m = (int)tk.m;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nrt = (m + 15) / 16;
-  const bool two = pw > 16 && p - rhs0 > 16;  // second 16-column tile needed
-  acc_t acc[4][2];
-  for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
-  const T* P = arena + tk.data;
-  const int li = lane & 15, lk = lane >> 4;
-  for (uint32_t c0 = 0; c0 < tk.W; c0 += kMmChunk) {
-    const int cw = (int)min((uint32_t)kMmChunk, tk.W - c0);
-    __syncthreads();
-    // gather: pw consecutive threads read the contiguous run of one source
-    // row.  All source-list entries of the thread first, then all values, so
-    // the two rounds of loads are each in flight together; columns beyond cw
-    // read column 0's (valid) source and store zero.
-    if (pw > 16)
-      mm_gather<T, 5>(zs, zsrc + tk.zoff + c0, cw, xg, tg);
-    else if (pw > 8)
-      mm_gather<T, 4>(zs, zsrc + tk.zoff + c0, cw, xg, tg);
-    else
-      mm_gather<T, 3>(zs, zsrc + tk.zoff + c0, cw, xg, tg);
-    __syncthreads();
-    // 4 k-steps (16 columns) per round: all panel loads first, then MFMAs
-    for (int k0 = 0; k0 < cw; k0 += 16) {
-      T a[4][4];
-#pragma unroll
-      for (int st = 0; st < 4; ++st) {
-        const int k = k0 + 4 * st + lk;
-        const T* Pk = P + (uint64_t)(c0 + k) * m;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          // unconditional load from a valid address (row 0 of column c0),
-          // zero outside the task
-          const int row = (w + 4 * q) * 16 + li;
-          const bool in = row < m && k < cw;
-          const T v = (in ? Pk : P + (uint64_t)c0 * m)[in ? row : 0];
-          a[st][q] = in ? v : (T)0;
-        }
-      }
-#pragma unroll
-      for (int st = 0; st < 4; ++st) {
-        const int k = k0 + 4 * st + lk;   // < kMmChunk; zero-filled beyond cw
-        const T b0 = li < pw ? zs[k][li] : (T)0;   // pw = 8: columns 8-15 unused
-        const T b1 = zs[k][16 + li];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (w + 4 * q < nrt) {
-            acc[q][0] = MmAcc<T>::mma(a[st][q], b0, acc[q][0]);
-            if (two) acc[q][1] = MmAcc<T>::mma(a[st][q], b1, acc[q][1]);
-          }
-      }
+  const T* xg = xi + (uint64_t)g * n_x * kMmRhs;
+  T* tg = ti + (uint64_t)g * t_len * kMmRhs;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int nrt = (int)(tk.m + 15) / 16;
+  const bool two = p - rhs0 > 16;   // second column tile needed
+  // this wavefront's units: nwhole row tiles w + 4q with both column tiles,
+  // and at most one row tile hrt with column tile hct alone
+  int nwhole, hrt = -1, hct = 0;
+  if (two) {
+    const int full = nrt >> 2, rem = nrt & 3;
+    nwhole = full + (rem == 3 && w < 2 ? 1 : 0);
+    if (rem == 1 && w < 2) {
+      hrt = 4 * full;
+      hct = w;
+    } else if (rem == 2) {
+      hrt = 4 * full + (w >> 1);
+      hct = w & 1;
+    } else if (rem == 3 && w >= 2) {
+      hrt = 4 * full + 2;
+      hct = w - 2;
     }
+  } else {
+    nwhole = (nrt - w + 3) >> 2;   // row tiles w, w+4, ... below nrt
   }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int rt = w + 4 * q;
-    if (rt >= nrt) continue;
-#pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (ct == 1 && !two) continue;
-        const int row = rt * 16 + MmAcc<T>::row(lane, r);
-        const int jj = ct * 16 + li;
-        const uint32_t j = rhs0 + jj;
-        if (row >= m || j >= p) continue;
-        const T v = acc[q][ct][r];
-        if (tk.partial)
-          part[(uint64_t)j * part_len + tk.out + row] = v;
-        else if (PHASE_B)
-          y[(uint64_t)j * ldy + tk.out + row] += alpha * v;
-        else
-          tg[(uint64_t)(tk.out + row) * pw + jj] = v;
-      }
+#define AHM_MM_TASK(NW, HALF, TWO)                                                                    \
+  mm_task<T, PHASE_B, NW, HALF, TWO>(tk, zsrc, arena, xg, tg, zs, w, hrt, hct, rhs0, p, y, ldy, part, \
+                                     part_len, alpha)
+  const int shape = nwhole * 4 + (hrt >= 0 ? 2 : 0) + (two ? 1 : 0);
+  switch (shape) {
+    case 0: case 1: AHM_MM_TASK(0, false, false); break;
+    case 2: AHM_MM_TASK(0, true, false); break;
+    case 3: AHM_MM_TASK(0, true, true); break;
+    case 4: AHM_MM_TASK(1, false, false); break;
+    case 5: AHM_MM_TASK(1, false, true); break;
+    case 6: AHM_MM_TASK(1, true, false); break;
+    case 7: AHM_MM_TASK(1, true, true); break;
+    case 8: AHM_MM_TASK(2, false, false); break;
+    case 9: AHM_MM_TASK(2, false, true); break;
+    case 10: AHM_MM_TASK(2, true, false); break;
+    case 11: AHM_MM_TASK(2, true, true); break;
+    case 12: AHM_MM_TASK(3, false, false); break;
+    case 13: AHM_MM_TASK(3, false, true); break;
+    case 14: AHM_MM_TASK(3, true, false); break;
+    case 15: AHM_MM_TASK(3, true, true); break;
+    case 16: AHM_MM_TASK(4, false, false); break;
+    default: AHM_MM_TASK(4, false, true); break;
   }
+#undef AHM_MM_TASK
 }
 
 // X (n x p, right-hand sides ldx apart) -> Xi: groups of 32 right-hand sides,
@@ -731,7 +846,7 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
       if (h->n_a)
         hipLaunchKernelGGL((panel_mm_kernel<TV, false>), dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,
                            (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
-                           (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, pw, Y, ldy, part,
+                           (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, Y, ldy, part,
                            h->part_len, alpha, p);
       if (h->n_ra)
         hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, p), dim3(kPanelThreads), 0, s,
@@ -741,7 +856,7 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
       if (h->n_b)
         hipLaunchKernelGGL((panel_mm_kernel<TV, true>), dim3(h->n_b, ng), dim3(kPanelThreads), 0, s,
                            (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
-                           (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, pw, Y, ldy, part,
+                           (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, Y, ldy, part,
                            h->part_len, alpha, p);
     }
   }
@@ -818,7 +933,11 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
   int st = AHM_OK;
   DeviceBuf jobs;
   do {
-    if ((st = h->arena.alloc(P.nvals * ss))) break;
+    if ((st = h->arena.alloc((P.nvals + kArenaSlack) * ss))) break;
+    if (hipMemsetAsync((char*)h->arena.p + P.nvals * ss, 0, kArenaSlack * ss, s) != hipSuccess) {
+      st = fail(AHM_ERR_HIP, "arena slack");
+      break;
+    }
     timer.mark("arena allocation");
     if ((st = upload_vec(jobs, P.jobs, s))) break;
     if ((st = upload_vec(h->tasks_a, P.a, s))) break;
@@ -945,7 +1064,11 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
   h->part_len = src->part_len;
   int st = AHM_OK;
   do {
-    if ((st = h->arena.alloc(h->nvals * 4))) break;
+    if ((st = h->arena.alloc((h->nvals + kArenaSlack) * 4))) break;
+    if (hipMemsetAsync((char*)h->arena.p + h->nvals * 4, 0, kArenaSlack * 4, s) != hipSuccess) {
+      st = fail(AHM_ERR_HIP, "arena slack");
+      break;
+    }
     if ((st = upload_vec(h->tasks_a, h->h_tasks_a, s))) break;
     if ((st = upload_vec(h->tasks_b, h->h_tasks_b, s))) break;
     if ((st = upload_vec(h->zsrc, h->h_zsrc, s))) break;
