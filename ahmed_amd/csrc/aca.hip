@@ -114,6 +114,43 @@ __device__ void block_argmin_int(int& key, int& idx, int* shk, int* shi) {
     }
 }
 
+// Long loops over the rows of a large leaf are bound by memory latency when
+// each step waits for its own loads; these helpers issue eight steps' loads
+// before using them and keep the order of the sums.
+
+// v - sum_{l<k} c[l] * M[l*ld + i], terms subtracted in l order
+__device__ __forceinline__ double sub_terms(double v, const double* c, const double* M, size_t ld, uint32_t i,
+                                            uint32_t k) {
+  uint32_t l = 0;
+  for (; l + 8 <= k; l += 8) {
+    double m[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = M[(size_t)(l + u) * ld + i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v = fma(-c[l + u], m[u], v);
+  }
+  for (; l < k; ++l) v = fma(-c[l], M[(size_t)l * ld + i], v);
+  return v;
+}
+
+// acc + sum a[i] b[i] over i = i0, i0 + step, ... < n, in that order
+__device__ __forceinline__ double strided_dot(const double* a, const double* b, uint32_t i0, uint32_t n,
+                                              uint32_t step, double acc) {
+  uint32_t i = i0;
+  for (; i + 7 * step < n; i += 8 * step) {
+    double x[8], y[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      x[u] = a[i + u * step];
+      y[u] = b[i + u * step];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = fma(x[u], y[u], acc);
+  }
+  for (; i < n; i += step) acc = fma(a[i], b[i], acc);
+  return acc;
+}
+
 struct Shared {
   double dv[kWaves];
   double av[kWaves];
@@ -147,8 +184,7 @@ __device__ ArgMax remainder_row(const KernelSpec& K, const double* P, const AcaB
   __syncthreads();
   ArgMax a{-1.0, 0x7fffffff};
   for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads) {
-    double v = kentry(K, P, B.b1 + i0, B.b2 + j);
-    for (uint32_t l = 0; l < k; ++l) v = fma(-ucoef[l], V[(size_t)l * B.n2 + j], v);
+    const double v = sub_terms(kentry(K, P, B.b1 + i0, B.b2 + j), ucoef, V, B.n2, j, k);
     pv[j] = v;
     if (S[j] >= 0) a = amax(a, ArgMax{fabs(v), (int)j});
   }
@@ -164,8 +200,7 @@ __device__ ArgMax remainder_col(const KernelSpec& K, const double* P, const AcaB
   __syncthreads();
   ArgMax a{-1.0, 0x7fffffff};
   for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads) {
-    double v = kentry(K, P, B.b1 + i, B.b2 + j0);
-    for (uint32_t l = 0; l < k; ++l) v = fma(-vcoef[l], U[(size_t)l * B.n1 + i], v);
+    const double v = sub_terms(kentry(K, P, B.b1 + i, B.b2 + j0), vcoef, U, B.n1, i, k);
     pu[i] = v;
     if (Z[i] >= 0) a = amax(a, ArgMax{fabs(v), (int)i});
   }
@@ -344,9 +379,8 @@ __device__ double cross_sum(const AcaBlock& B, uint32_t k, const double* U, cons
   const double* vk = V + (size_t)k * B.n2;
   double acc = 0.0;  // per-wave sum of its terms, in l order
   for (uint32_t l = w; l < k; l += kWaves) {
-    double du = 0.0, dv = 0.0;
-    for (uint32_t i = lane; i < B.n1; i += 64) du = fma(U[(size_t)l * B.n1 + i], uk[i], du);
-    for (uint32_t j = lane; j < B.n2; j += 64) dv = fma(V[(size_t)l * B.n2 + j], vk[j], dv);
+    double du = strided_dot(U + (size_t)l * B.n1, uk, lane, B.n1, 64, 0.0);
+    double dv = strided_dot(V + (size_t)l * B.n2, vk, lane, B.n2, 64, 0.0);
     du = wave_sum(du);
     dv = wave_sum(dv);
     acc += du * dv;
@@ -492,8 +526,7 @@ __device__ void house_qr(double* A, uint32_t n, uint32_t k, double* tau, Shared&
   const uint32_t r = min(n, k);
   for (uint32_t j = 0; j < r; ++j) {
     double* a = A + (size_t)j * n;
-    double s = 0.0;
-    for (uint32_t i = j + 1 + threadIdx.x; i < n; i += kAcaThreads) s = fma(a[i], a[i], s);
+    const double s = strided_dot(a, a, j + 1 + threadIdx.x, n, kAcaThreads, 0.0);
     const double xn2 = block_sum(s, sh.dv);
     __syncthreads();
     const double alpha = a[j];
@@ -515,17 +548,15 @@ __device__ void house_qr(double* A, uint32_t n, uint32_t k, double* tau, Shared&
     // w_c = v^T A[:, c] for c > j, one wavefront per column
     for (uint32_t c = j + 1 + w; c < k; c += kWaves) {
       const double* ac = A + (size_t)c * n;
-      double d = 0.0;
-      for (uint32_t i = j + 1 + lane; i < n; i += 64) d = fma(a[i], ac[i], d);
+      double d = strided_dot(a, ac, j + 1 + lane, n, 64, 0.0);
       d = wave_sum(d);
       if (lane == 0) wbuf[c] = d + ac[j];
     }
     __syncthreads();
-    const uint32_t rows = n - j, cols = k - j - 1;
-    for (uint32_t e = threadIdx.x; e < rows * cols; e += kAcaThreads) {
-      const uint32_t i = j + e % rows, c = j + 1 + e / rows;
-      const double v = (i == j) ? 1.0 : a[i];
-      A[(size_t)c * n + i] -= t * wbuf[c] * v;
+    for (uint32_t c = j + 1; c < k; ++c) {
+      const double tw = t * wbuf[c];
+      double* ac = A + (size_t)c * n;
+      for (uint32_t i = j + threadIdx.x; i < n; i += kAcaThreads) ac[i] -= tw * (i == j ? 1.0 : a[i]);
     }
     __syncthreads();
   }
@@ -542,17 +573,16 @@ __device__ void apply_q(const double* A, uint32_t n, uint32_t r, const double* t
     __syncthreads();
     for (uint32_t c = w; c < m; c += kWaves) {
       const double* xc = X + (size_t)c * n;
-      double d = 0.0;
-      for (uint32_t i = j + 1 + lane; i < n; i += 64) d = fma(a[i], xc[i], d);
+      double d = strided_dot(a, xc, (uint32_t)j + 1 + lane, n, 64, 0.0);
       d = wave_sum(d);
       if (lane == 0) wbuf[c] = d + xc[j];
     }
     __syncthreads();
-    const uint32_t rows = n - j;
-    for (uint32_t e = threadIdx.x; e < rows * m; e += kAcaThreads) {
-      const uint32_t i = j + e % rows, c = e / rows;
-      const double v = ((uint32_t)j == i) ? 1.0 : a[i];
-      X[(size_t)c * n + i] -= t * wbuf[c] * v;
+    for (uint32_t c = 0; c < m; ++c) {
+      const double tw = t * wbuf[c];
+      double* xc = X + (size_t)c * n;
+      for (uint32_t i = (uint32_t)j + threadIdx.x; i < n; i += kAcaThreads)
+        xc[i] -= tw * ((uint32_t)j == i ? 1.0 : a[i]);
     }
   }
   __syncthreads();
@@ -653,7 +683,18 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock*
   for (uint32_t e = threadIdx.x; e < mmin * nmin; e += kAcaThreads) {
     const uint32_t i = e % mmin, j = e / mmin;
     double d = 0.0;
-    for (uint32_t l = max(i, j); l < k; ++l) d += U[(size_t)l * B.n1 + i] * V[(size_t)l * B.n2 + j];
+    uint32_t l = max(i, j);
+    for (; l + 8 <= k; l += 8) {
+      double x[8], y[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        x[u] = U[(size_t)(l + u) * B.n1 + i];
+        y[u] = V[(size_t)(l + u) * B.n2 + j];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) d += x[u] * y[u];
+    }
+    for (; l < k; ++l) d += U[(size_t)l * B.n1 + i] * V[(size_t)l * B.n2 + j];
     R[e] = d;
     fs += d * d;
   }
@@ -715,14 +756,12 @@ __global__ __launch_bounds__(kAcaThreads) void expand_kernel(const AcaBlock* __r
   const double* tau2 = tau1 + (B.kcap + 1);
   double* Uo = out + out_off[blockIdx.x];
   double* Vo = Uo + (size_t)B.n1 * kt;
-  for (uint32_t e = threadIdx.x; e < B.n1 * kt; e += kAcaThreads) {
-    const uint32_t i = e % B.n1, c = e / B.n1;
-    Uo[e] = i < mmin ? Ws[(size_t)c * mmin + i] : 0.0;
-  }
-  for (uint32_t e = threadIdx.x; e < B.n2 * kt; e += kAcaThreads) {
-    const uint32_t i = e % B.n2, c = e / B.n2;
-    Vo[e] = i < nmin ? Zs[(size_t)c * nmin + i] : 0.0;
-  }
+  for (uint32_t c = 0; c < kt; ++c)
+    for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads)
+      Uo[(size_t)c * B.n1 + i] = i < mmin ? Ws[(size_t)c * mmin + i] : 0.0;
+  for (uint32_t c = 0; c < kt; ++c)
+    for (uint32_t i = threadIdx.x; i < B.n2; i += kAcaThreads)
+      Vo[(size_t)c * B.n2 + i] = i < nmin ? Zs[(size_t)c * nmin + i] : 0.0;
   __syncthreads();
   apply_q(U, B.n1, mmin, tau1, Uo, kt, wbuf);
   apply_q(V, B.n2, nmin, tau2, Vo, kt, wbuf);
@@ -733,12 +772,16 @@ __global__ void dense_fill_kernel(const DenseJob* __restrict__ jobs, const doubl
                                   const KernelSpec* __restrict__ Ks, double* __restrict__ out) {
   const DenseJob J = jobs[blockIdx.x];
   const KernelSpec K = Ks[J.kidx];
-  const uint64_t n = (uint64_t)J.n1 * J.n2;
   double* D = out + J.off;
-  for (uint64_t e = threadIdx.x; e < n; e += blockDim.x) {
-    const uint32_t i = (uint32_t)(e % J.n1), j = (uint32_t)(e / J.n1);
-    D[e] = kentry(K, P, J.b1 + i, J.b2 + j);
-  }
+  // column by column (no per-element division); a workgroup covers
+  // blockDim.x / n1 columns at a time when a column is shorter than it
+  const uint32_t per = J.n1 >= blockDim.x ? 1u : blockDim.x / J.n1;
+  const uint32_t sub = threadIdx.x / (J.n1 >= blockDim.x ? blockDim.x : J.n1);
+  const uint32_t i0 = threadIdx.x - sub * (J.n1 >= blockDim.x ? blockDim.x : J.n1);
+  const uint32_t step = J.n1 >= blockDim.x ? blockDim.x : J.n1;
+  if (sub >= per) return;
+  for (uint32_t j = sub; j < J.n2; j += per)
+    for (uint32_t i = i0; i < J.n1; i += step) D[(uint64_t)j * J.n1 + i] = kentry(K, P, J.b1 + i, J.b2 + j);
 }
 
 }  // namespace ahm
@@ -846,7 +889,15 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
   std::vector<size_t> todo(cand.size());
   std::iota(todo.begin(), todo.end(), 0);
   std::stable_sort(todo.begin(), todo.end(), [&](size_t a, size_t b) { return cap[a] > cap[b]; });
-  const uint64_t budget = (uint64_t)24 << 30;  // bytes of ACA scratch per batch
+  // bytes of ACA scratch per batch: a quarter of the free device memory,
+  // between 4 and 96 GiB.  Every batch lasts about as long as its largest
+  // leaves take, so fewer, larger batches finish sooner.
+  uint64_t budget = (uint64_t)24 << 30;
+  {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b)
+      budget = std::min<uint64_t>(std::max<uint64_t>(free_b / 4, (uint64_t)4 << 30), (uint64_t)96 << 30);
+  }
   std::vector<Part*> parts;
   std::vector<uint32_t> part_of(nl, 0);
   std::vector<uint64_t> off_in_part(nl, 0);
