@@ -151,6 +151,27 @@ __device__ __forceinline__ double strided_dot(const double* a, const double* b, 
   return acc;
 }
 
+// x[i] -= s * (i == j ? 1 : a[i]) for i = i0, i0 + step, ... < n (a Householder
+// update of one column; a is not written)
+__device__ __forceinline__ void house_axpy(double* x, const double* a, double s, uint32_t j, uint32_t i0,
+                                           uint32_t n, uint32_t step) {
+  uint32_t i = i0;
+  for (; i + 7 * step < n; i += 8 * step) {
+    double xv[8], av[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      xv[u] = x[i + u * step];
+      av[u] = a[i + u * step];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t ii = i + u * step;
+      x[ii] = xv[u] - s * (ii == j ? 1.0 : av[u]);
+    }
+  }
+  for (; i < n; i += step) x[i] -= s * (i == j ? 1.0 : a[i]);
+}
+
 struct Shared {
   double dv[kWaves];
   double av[kWaves];
@@ -555,8 +576,7 @@ __device__ void house_qr(double* A, uint32_t n, uint32_t k, double* tau, Shared&
     __syncthreads();
     for (uint32_t c = j + 1; c < k; ++c) {
       const double tw = t * wbuf[c];
-      double* ac = A + (size_t)c * n;
-      for (uint32_t i = j + threadIdx.x; i < n; i += kAcaThreads) ac[i] -= tw * (i == j ? 1.0 : a[i]);
+      house_axpy(A + (size_t)c * n, a, tw, j, j + threadIdx.x, n, kAcaThreads);
     }
     __syncthreads();
   }
@@ -580,9 +600,7 @@ __device__ void apply_q(const double* A, uint32_t n, uint32_t r, const double* t
     __syncthreads();
     for (uint32_t c = 0; c < m; ++c) {
       const double tw = t * wbuf[c];
-      double* xc = X + (size_t)c * n;
-      for (uint32_t i = (uint32_t)j + threadIdx.x; i < n; i += kAcaThreads)
-        xc[i] -= tw * ((uint32_t)j == i ? 1.0 : a[i]);
+      house_axpy(X + (size_t)c * n, a, tw, (uint32_t)j, (uint32_t)j + threadIdx.x, n, kAcaThreads);
     }
   }
   __syncthreads();
@@ -653,28 +671,12 @@ __device__ void jacobi_svd(double* R, uint32_t mm, uint32_t nn, double* Zm, Shar
 
 // QR of U and V, core R = R1 R2^T, Jacobi SVD, truncation.  Leaves the
 // sorted, truncated core factors in the scratch: W = R Z (columns scaled by
-// the singular values, mmin x kt, ld mmin) and Z (nmin x kt, ld nmin).
-__global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock* __restrict__ blocks,
-                                                                 AcaResult* __restrict__ res,
-                                                                 double eps, double* __restrict__ slab) {
-  __shared__ Shared sh;
-  __shared__ double wbuf[kAcaCapMax + 1];
-  __shared__ double sig[kAcaCapMax + 1];
-  __shared__ int order[kAcaCapMax + 1];
-  const AcaBlock B = blocks[blockIdx.x];
-  const AcaResult r0 = res[blockIdx.x];
-  if (r0.status != 1) return;
-  const uint32_t k = r0.k;
-  if (k == 0) return;
-  double* U = slab + B.uoff;
-  double* V = slab + B.voff;
-  const size_t sq = (size_t)(B.kcap + 1) * (B.kcap + 1);
-  double* R = slab + B.roff;
-  double* Zm = R + sq;
-  double* Ws = Zm + sq;
-  double* Zs = Ws + sq;
-  double* tau1 = Zs + sq;
-  double* tau2 = tau1 + (B.kcap + 1);
+// the singular values, mmin x kt, ld mmin) and Z (nmin x kt, ld nmin).  U, V,
+// tau1, tau2, R and Zm are the leaf's working arrays, in LDS or in the scratch.
+__device__ __forceinline__ void rc_body(const AcaBlock& B, AcaResult* __restrict__ res, uint32_t k, double eps,
+                                        double* U, double* V, double* R, double* Zm, double* Ws, double* Zs,
+                                        double* tau1, double* tau2, Shared& sh, double* wbuf, double* sig,
+                                        int* order) {
   house_qr(U, B.n1, k, tau1, sh, wbuf);
   house_qr(V, B.n2, k, tau2, sh, wbuf);
   const uint32_t mmin = min(B.n1, k), nmin = min(B.n2, k), amin = min(mmin, nmin);
@@ -735,6 +737,58 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock*
   if (threadIdx.x == 0) res[blockIdx.x].kt = kt;
 }
 
+
+// Doubles of dynamic LDS per workgroup of the recompression and expansion:
+// a leaf whose working arrays fit runs on LDS copies, without the round trips
+// to memory that every workgroup barrier would otherwise wait for.
+constexpr uint32_t kRcLds = 4352;
+
+__global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock* __restrict__ blocks,
+                                                                 AcaResult* __restrict__ res,
+                                                                 double eps, double* __restrict__ slab) {
+  __shared__ Shared sh;
+  __shared__ double wbuf[kAcaCapMax + 1];
+  __shared__ double sig[kAcaCapMax + 1];
+  __shared__ int order[kAcaCapMax + 1];
+  extern __shared__ double rc_lds[];
+  const AcaBlock B = blocks[blockIdx.x];
+  const AcaResult r0 = res[blockIdx.x];
+  if (r0.status != 1) return;
+  const uint32_t k = r0.k;
+  if (k == 0) return;
+  double* U = slab + B.uoff;
+  double* V = slab + B.voff;
+  const size_t sq = (size_t)(B.kcap + 1) * (B.kcap + 1);
+  double* R = slab + B.roff;
+  double* Zm = R + sq;
+  double* Ws = Zm + sq;
+  double* Zs = Ws + sq;
+  double* tau1 = Zs + sq;
+  double* tau2 = tau1 + (B.kcap + 1);
+  const uint32_t mmin = min(B.n1, k), nmin = min(B.n2, k);
+  const size_t nu = (size_t)B.n1 * k, nv = (size_t)B.n2 * k;
+  if (nu + nv + 2 * (size_t)k + (size_t)mmin * nmin + (size_t)nmin * nmin <= kRcLds) {
+    double* Ul = rc_lds;
+    double* Vl = Ul + nu;
+    double* t1 = Vl + nv;
+    double* t2 = t1 + k;
+    double* Rl = t2 + k;
+    double* Zl = Rl + (size_t)mmin * nmin;
+    for (uint32_t e = threadIdx.x; e < nu; e += kAcaThreads) Ul[e] = U[e];
+    for (uint32_t e = threadIdx.x; e < nv; e += kAcaThreads) Vl[e] = V[e];
+    __syncthreads();
+    rc_body(B, res, k, eps, Ul, Vl, Rl, Zl, Ws, Zs, t1, t2, sh, wbuf, sig, order);
+    __syncthreads();
+    // the reflectors and their factors are what the expansion reads
+    for (uint32_t e = threadIdx.x; e < nu; e += kAcaThreads) U[e] = Ul[e];
+    for (uint32_t e = threadIdx.x; e < nv; e += kAcaThreads) V[e] = Vl[e];
+    for (uint32_t e = threadIdx.x; e < mmin; e += kAcaThreads) tau1[e] = t1[e];
+    for (uint32_t e = threadIdx.x; e < nmin; e += kAcaThreads) tau2[e] = t2[e];
+  } else {
+    rc_body(B, res, k, eps, U, V, R, Zm, Ws, Zs, tau1, tau2, sh, wbuf, sig, order);
+  }
+}
+
 // final U = Q1 [W; 0] (n1 x kt), V = Q2 [Z; 0] (n2 x kt) at the leaf's offset
 __global__ __launch_bounds__(kAcaThreads) void expand_kernel(const AcaBlock* __restrict__ blocks,
                                                              const AcaResult* __restrict__ res,
@@ -756,15 +810,30 @@ __global__ __launch_bounds__(kAcaThreads) void expand_kernel(const AcaBlock* __r
   const double* tau2 = tau1 + (B.kcap + 1);
   double* Uo = out + out_off[blockIdx.x];
   double* Vo = Uo + (size_t)B.n1 * kt;
-  for (uint32_t c = 0; c < kt; ++c)
-    for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads)
-      Uo[(size_t)c * B.n1 + i] = i < mmin ? Ws[(size_t)c * mmin + i] : 0.0;
-  for (uint32_t c = 0; c < kt; ++c)
-    for (uint32_t i = threadIdx.x; i < B.n2; i += kAcaThreads)
-      Vo[(size_t)c * B.n2 + i] = i < nmin ? Zs[(size_t)c * nmin + i] : 0.0;
-  __syncthreads();
-  apply_q(U, B.n1, mmin, tau1, Uo, kt, wbuf);
-  apply_q(V, B.n2, nmin, tau2, Vo, kt, wbuf);
+  extern __shared__ double rc_lds[];
+  // one factor: Q [core; 0] with Q's reflectors in A (n x r) and the core in C
+  // (r x kt, ld r), into O (n x kt); on LDS copies when they fit
+  auto factor = [&](const double* A, uint32_t n, uint32_t r, const double* tau, const double* C, double* O) {
+    const size_t na = (size_t)n * r, no = (size_t)n * kt;
+    double* X = O;
+    const double* Ar = A;
+    if (na + no <= kRcLds) {
+      double* Al = rc_lds;
+      for (uint32_t e = threadIdx.x; e < na; e += kAcaThreads) Al[e] = A[e];
+      Ar = Al;
+      X = Al + na;
+    }
+    for (uint32_t c = 0; c < kt; ++c)
+      for (uint32_t i = threadIdx.x; i < n; i += kAcaThreads) X[(size_t)c * n + i] = i < r ? C[(size_t)c * r + i] : 0.0;
+    __syncthreads();
+    apply_q(Ar, n, r, tau, X, kt, wbuf);   // ends with a barrier
+    if (X != O) {
+      for (uint32_t e = threadIdx.x; e < no; e += kAcaThreads) O[e] = X[e];
+      __syncthreads();
+    }
+  };
+  factor(U, B.n1, mmin, tau1, Ws, Uo);
+  factor(V, B.n2, nmin, tau2, Zs, Vo);
 }
 
 // dense leaves (and failed ACA): cmpbl of the whole block, column-major
@@ -962,7 +1031,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       hipLaunchKernelGGL(aca_kernel, dim3(nb), dim3(kAcaThreads), 0, s, (const AcaBlock*)dblk.p,
                          (AcaResult*)dres.p, (const double*)dP.p, (const KernelSpec*)dK.p, eps,
                          (double*)dslab.p, (int*)discr.p);
-      hipLaunchKernelGGL(recompress_kernel, dim3(nb), dim3(kAcaThreads), 0, s,
+      hipLaunchKernelGGL(recompress_kernel, dim3(nb), dim3(kAcaThreads), kRcLds * sizeof(double), s,
                          (const AcaBlock*)dblk.p, (AcaResult*)dres.p, eps, (double*)dslab.p);
       std::vector<AcaResult> rr(nb);
       if (hipGetLastError() != hipSuccess ||
@@ -1010,7 +1079,8 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
           cleanup();
           return st;
         }
-        hipLaunchKernelGGL(expand_kernel, dim3(nb), dim3(kAcaThreads), 0, s, (const AcaBlock*)dblk.p,
+        hipLaunchKernelGGL(expand_kernel, dim3(nb), dim3(kAcaThreads), kRcLds * sizeof(double), s,
+                           (const AcaBlock*)dblk.p,
                            (const AcaResult*)dres.p, (const uint64_t*)doff.p,
                            (const double*)dslab.p, (double*)part->buf.p);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
