@@ -442,12 +442,28 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     H, ast = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_LAPLACE, 0.0, weight, eps,
-                                   keep_leaves=not args.no_cpu)
+                                   keep_leaves=not args.no_cpu, aca_ranks=True)
     torch.cuda.synchronize()
     t_asm = time.perf_counter() - t0
     st = H.stats()
     log(f"GPU assembly {t_asm:.2f} s: {st['n_lowrank']} low-rank / {st['n_dense']} dense, "
         f"nvals {st['nvals']:.4g}, ACA retries {ast['n_retries']}")
+    # assembly work (VALU-bound): kernel entries evaluated by ACA (one row and
+    # one column per step) and by the dense leaves, and the ACA updates
+    # (2 k^2 (n1+n2) flops per admissible leaf of k steps)
+    lv = tree.leaves()
+    adm = lv["adm"].astype(bool)
+    nsum = (lv["n1"].astype(np.float64) + lv["n2"].astype(np.float64))[adm]
+    kaca = ast["aca_rank"].astype(np.float64)[adm]
+    asm_work = {"kernel_evals_aca": float((kaca * nsum).sum()), "kernel_evals_dense": float(st["nvals_dense"]),
+                "update_flops": float((2.0 * kaca * kaca * nsum).sum())}
+    aca_gflops = asm_work["update_flops"] / (ast["aca_ms"] * 1e-3) / 1e9
+    asm_work.update({"aca_phase_update_GFLOP/s": round(aca_gflops, 2),
+                     "aca_phase_evals_per_s": round(asm_work["kernel_evals_aca"] / (ast["aca_ms"] * 1e-3), 1),
+                     "frac_fp64_valu_peak": round(aca_gflops / 78600.0, 5)})
+    log(f"assembly work: {asm_work['kernel_evals_aca']:.3g} ACA + {asm_work['kernel_evals_dense']:.3g} dense kernel "
+        f"entries, {asm_work['update_flops']:.3g} update flops; ACA phase {aca_gflops:.1f} GFLOP/s "
+        f"({100 * aca_gflops / 78600.0:.2f} % of the FP64 vector peak)")
     # the same assembly with ACAr (apprx_sym's row-pivoted ACA) on the admissible leaves
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -590,7 +606,7 @@ def main():
                          "rel_err_vs_fp64": err32},
         "assembly": {"tree_s": round(t_tree, 3), "gpu_assembly_s": round(t_asm, 3), "aca_ms": round(ast["aca_ms"], 1),
                      "aca_retries": ast["n_retries"], "mean_aca_rank": ast["sum_aca_rank"] / max(1, ast["n_candidates"]),
-                     "mean_rank": ast["sum_rank"] / max(1, ast["n_lowrank"]), "acar": acar},
+                     "mean_rank": ast["sum_rank"] / max(1, ast["n_lowrank"]), "work": asm_work, "acar": acar},
         "gmres": gm,
     }
     print(json.dumps(line), flush=True)
