@@ -181,3 +181,65 @@ def test_autotuned_chunking_keeps_results(ctx):
         assert relerr(y.cpu().numpy(), y0.cpu().numpy()) <= 1e-14
         assert relerr(Y.cpu().numpy(), Y0.cpu().numpy()) <= 1e-14
     assert relerr(y0.cpu().numpy(), H.matvec(x.cpu().numpy())) <= 1e-12
+
+
+def block_matrix(sizes, seed):
+    """Leaves of a block matrix on row/column segments of the given sizes:
+    dense diagonal blocks, low-rank off-diagonal blocks of ranks 1..6, with
+    the dense N x N matrix they represent (column-major leaf values as in
+    mblock: dense n1 x n2, low-rank U n1 x k then V n2 x k)."""
+    from ahmed_amd import _lib
+    rng = np.random.default_rng(seed)
+    b = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    N = int(b[-1])
+    L = {f: [] for f in ("b1", "n1", "b2", "n2", "rank", "offset")}
+    chunks, off = [], 0
+    A = np.zeros((N, N))
+    for i, mi in enumerate(sizes):
+        for j, mj in enumerate(sizes):
+            if i == j:
+                k, vals = -1, rng.standard_normal(mi * mj)
+                A[b[i]:b[i + 1], b[j]:b[j + 1]] = vals.reshape(mj, mi).T
+            else:
+                k = int(1 + (i * 7 + j * 3) % 6)
+                vals = rng.standard_normal(k * (mi + mj))
+                U = vals[:mi * k].reshape(k, mi).T
+                V = vals[mi * k:].reshape(k, mj).T
+                A[b[i]:b[i + 1], b[j]:b[j + 1]] = U @ V.T
+            for f, v in zip(("b1", "n1", "b2", "n2", "rank", "offset"), (b[i], mi, b[j], mj, k, off)):
+                L[f].append(v)
+            chunks.append(vals)
+            off += vals.size
+    leaves = _lib.leaf_array(L["b1"], L["n1"], L["b2"], L["n2"], L["rank"], L["offset"])
+    return N, leaves, np.concatenate(chunks), A
+
+
+# Row panels of every height class of the MFMA product's wavefront shapes:
+# 1-4 row tiles (one whole tile or less per wavefront), 5-8 (16-column
+# rounds, split tiles), 9-16 (several whole tiles per wavefront), and a
+# segment of 300 rows cut into 256 + 44.  The V^T panels take the summed
+# ranks of a column segment (up to about 90 rows).  p = 17 and 32: one group
+# of 32 right-hand sides (two column tiles); 33 and 48: a second group with
+# one column tile; 64: two full groups.
+MFMA_SIZES = [1, 5, 16, 17, 31, 33, 48, 63, 65, 80, 96, 100, 112, 127, 128, 129, 144, 176, 191,
+              200, 224, 240, 255, 256, 300]
+
+
+@pytest.mark.parametrize("p", [17, 32, 33, 48, 64])
+def test_mfma_product_every_panel_height(ctx, p):
+    from ahmed_amd import _lib
+    N, leaves, data, A = block_matrix(MFMA_SIZES, 11)
+    print(f"[gpu] MFMA panel heights N={N} p={p}", flush=True)
+    D = _lib.HMatrix.upload(ctx, N, N, leaves, data)
+    rng = np.random.default_rng(p)
+    X = rng.standard_normal((N, p))
+    Y0 = rng.standard_normal((N, p))
+    Y = dev(Y0.T.copy())
+    D.matvec(dev(X.T.copy()), Y, -1.25)
+    ref = Y0 - 1.25 * (A @ X)
+    assert relerr(Y.cpu().numpy().T, ref) <= 1e-12
+    if p == 32:
+        D32 = D.convert()
+        Y32 = dev(np.zeros((p, N)), torch.float32)
+        D32.matvec(dev(X.T.copy(), torch.float32), Y32, 1.0)
+        assert relerr(Y32.cpu().numpy().T.astype(np.float64), A @ X) <= 5e-6
