@@ -609,11 +609,22 @@ __device__ void apply_q(const double* A, uint32_t n, uint32_t r, const double* t
 // One-sided Jacobi SVD of R (mm x nn, ld mm): R <- R Z with orthogonal Z
 // (nn x nn, ld nn) until the columns of R are mutually orthogonal.  Column
 // pairs are visited in round-robin tournament order, one wavefront per pair.
+// A pair counts as orthogonal when |p.q| <= mm * eps * |p| |q| (the rounding
+// error of the computed p.q itself) or when either column is below
+// eps * |R|_F; without the second test the columns a rank-deficient core
+// drives to zero stay rounding noise that is rotated in every sweep (a third
+// of the leaves of a 1M-point matrix swept until the sweep limit).
 __device__ void jacobi_svd(double* R, uint32_t mm, uint32_t nn, double* Zm, Shared& sh) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double tol = (double)max(mm, 2u) * 2.220446049250313e-16;
   for (uint32_t e = threadIdx.x; e < nn * nn; e += kAcaThreads) Zm[e] = (e % nn == e / nn) ? 1.0 : 0.0;
   __syncthreads();
   if (nn < 2) return;
+  // columns below eps * |R|_F are rounding noise (the rotations keep |R|_F):
+  // they are left alone, or they would be rotated on noise in every sweep
+  double f = 0.0;
+  for (uint32_t e = threadIdx.x; e < mm * nn; e += kAcaThreads) f = fma(R[e], R[e], f);
+  const double negl = 4.930380657631324e-32 * block_sum(f, sh.dv);   // eps^2 |R|_F^2
   const uint32_t np = (nn + 1) & ~1u;  // even number of players (one bye if odd)
   __shared__ int rot;
   for (int sweep = 0; sweep < 40; ++sweep) {
@@ -643,7 +654,7 @@ __device__ void jacobi_svd(double* R, uint32_t mm, uint32_t nn, double* Zm, Shar
         al = wave_sum(al);
         be = wave_sum(be);
         ga = wave_sum(ga);
-        if (fabs(ga) <= 1e-15 * sqrt(al * be) || ga == 0.0) continue;
+        if (fabs(ga) <= tol * sqrt(al * be) || ga == 0.0 || fmin(al, be) <= negl) continue;
         const double zeta = (be - al) / (2.0 * ga);
         const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(fma(zeta, zeta, 1.0)));
         const double c = 1.0 / sqrt(fma(t, t, 1.0)), s = c * t;
