@@ -37,14 +37,22 @@ __global__ __launch_bounds__(256) void relayout_kernel(const CopyJob* __restrict
                                                        const TS* __restrict__ src,
                                                        TD* __restrict__ dst) {
   const CopyJob j = jobs[blockIdx.x];
-  const uint64_t n = (uint64_t)j.rows * j.cols;
-  for (uint64_t e = threadIdx.x; e < n; e += blockDim.x) {
-    const uint32_t r = (uint32_t)(e % j.rows), c = (uint32_t)(e / j.rows);
-    const TS v = src[j.src + r + (uint64_t)c * j.src_ld];
-    if (j.transpose)
-      dst[j.dst + c + (uint64_t)r * j.dst_ld] = (TD)v;
-    else
-      dst[j.dst + r + (uint64_t)c * j.dst_ld] = (TD)v;
+  if (j.rows == 0) return;
+  // column by column, no per-element division: a column shorter than the
+  // workgroup shares it with the next blockDim.x / rows - 1 columns
+  const uint32_t step = j.rows >= blockDim.x ? blockDim.x : j.rows;
+  const uint32_t per = blockDim.x / step;
+  const uint32_t sub = threadIdx.x / step, r0 = threadIdx.x - sub * step;
+  if (sub >= per) return;
+  for (uint32_t c = sub; c < j.cols; c += per) {
+    const TS* s = src + j.src + (uint64_t)c * j.src_ld;
+    if (j.transpose) {
+      TD* d = dst + j.dst + c;
+      for (uint32_t r = r0; r < j.rows; r += step) d[(uint64_t)r * j.dst_ld] = (TD)s[r];
+    } else {
+      TD* d = dst + j.dst + (uint64_t)c * j.dst_ld;
+      for (uint32_t r = r0; r < j.rows; r += step) d[r] = (TD)s[r];
+    }
   }
 }
 
