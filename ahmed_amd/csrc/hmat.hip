@@ -145,13 +145,13 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
 }
 
 // ------------------------------------------------------------------------
-// 2 <= p <= 16 right-hand sides: the single-RHS kernel's thread mapping and
-// streaming with NP = 8 or 16 accumulators per thread.  At these widths the
+// 2 <= p <= 8 right-hand sides: the single-RHS kernel's thread mapping and
+// streaming with NP = 8 accumulators per thread.  At these widths the
 // products are bound by the panel stream, not by arithmetic, and the flat
-// per-row loads of the single-RHS kernel stream at the full rate (the MFMA
-// fragments' 16-row pieces do not); Z comes from the NP-wide interleaved
-// copies of x and t (one NP*8-byte row per column), staged in LDS and read
-// as broadcasts.
+// per-row loads of the single-RHS kernel stream at the full rate; Z comes
+// from the NP-wide interleaved copies of x and t (one NP*8-byte row per
+// column), staged in LDS and read as broadcasts.  From 9 right-hand sides on
+// the MFMA kernel below is faster (one 16-column tile up to 16).
 constexpr int kZChunkP = 256;   // columns of Z staged per pass
 
 template <typename TS, typename TV, bool PHASE_B, int NP>
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemvp_kernel(
 }
 
 // ------------------------------------------------------------------------
-// p >= 17 right-hand sides: the same panels as a matrix product on the MFMA
+// p >= 9 right-hand sides: the same panels as a matrix product on the MFMA
 // units.
 //
 // out(m x p) (+)= P (m x W) * Z (W x p), one task per workgroup and 32
@@ -768,8 +768,8 @@ static int upload_vec(DeviceBuf& b, const std::vector<T>& v, hipStream_t s) {
 }
 
 // multi-RHS products keep x and t interleaved in groups of 32 right-hand
-// sides (16 when p <= 16, 8 when p <= 8): rhs_width values per row and group
-static uint32_t rhs_width(uint32_t p) { return p <= 8 ? 8u : p <= 16 ? 16u : (uint32_t)kMmRhs; }
+// sides (8 when p <= 8): rhs_width values per row and group
+static uint32_t rhs_width(uint32_t p) { return p <= 8 ? 8u : (uint32_t)kMmRhs; }
 static uint32_t rhs_slots(uint32_t p) {
   return p == 1 ? 1u : ((p + kMmRhs - 1) / kMmRhs) * rhs_width(p);
 }
@@ -819,7 +819,7 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
                          (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
                          (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
   } else {
-    // p >= 2 on interleaved copies of x and t: p <= 16 with the streaming
+    // p >= 2 on interleaved copies of x and t: p <= 8 with the streaming
     // kernel, wider with MFMA panel products (32 right-hand sides per
     // workgroup)
     const uint32_t ng = (p + kMmRhs - 1) / kMmRhs;
@@ -827,7 +827,7 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
     TV* xi = (TV*)h->xbuf.p;
     hipLaunchKernelGGL((interleave_kernel<TV>), dim3((h->n_cols + 63) / 64, ng), dim3(256), 0, s, X,
                        ldx, h->n_cols, p, pw, xi);
-    if (pw <= 16) {
+    if (pw == 8) {
       auto run = [&](auto np) {
         constexpr int NP = decltype(np)::value;
         if (h->n_a)
@@ -846,10 +846,7 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
                              (const TS*)h->arena.p, (const TV*)xi, t, h->t_len, Y, ldy, part,
                              h->part_len, alpha, p);
       };
-      if (pw == 8)
-        run(std::integral_constant<int, 8>());
-      else
-        run(std::integral_constant<int, 16>());
+      run(std::integral_constant<int, 8>());
     } else {
       if (h->n_a)
         hipLaunchKernelGGL((panel_mm_kernel<TV, false>), dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,

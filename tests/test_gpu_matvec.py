@@ -138,7 +138,8 @@ def test_rank_zero_and_all_dense_edge_cases(ctx):
         _lib.HMatrix.upload(ctx, N, N, bad, data)
 
 
-# 3, 8: 8-wide streaming kernel; 12, 16: 16-wide; 17, 40: MFMA panel products
+# 3, 8: 8-wide streaming kernel; 12, 16: MFMA panel products with one
+# 16-column tile; 17, 40: MFMA with two tiles / a second group
 @pytest.mark.parametrize("p", [3, 8, 12, 16, 17, 40])
 def test_multi_rhs_odd_counts_fp64_and_fp32(ctx, p):
     _, T, H = sphere_hmatrix(16384, 64, 1e-8)
