@@ -1010,6 +1010,18 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     parts.clear();
   };
   Grow gblk, gres, gslab, gscr, goff;
+  // the slab scratch is sized once, for the largest batch there can be (the
+  // budget, or all first attempts if they take less): growing it batch by
+  // batch re-allocates tens of GiB, which is slow right after large frees
+  {
+    uint64_t all = 0;
+    for (size_t c = 0; c < cand.size(); ++c) {
+      const uint32_t l = cand[c];
+      const uint64_t kc = cap[c], sq = (kc + 1) * (kc + 1);
+      all += (kc + 1) * ((uint64_t)n1[l] + n2[l]) + 4 * sq + 2 * (kc + 1);
+    }
+    if (all && gslab.need(std::min<uint64_t>(all * 8, budget))) return fail(AHM_ERR_HIP, "ACA scratch");
+  }
   timer.mark("set-up (points, candidates)");
   const auto t_aca0 = std::chrono::steady_clock::now();
   // one pass over a growing list: a leaf whose slab was too small goes back
