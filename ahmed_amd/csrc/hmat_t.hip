@@ -119,7 +119,21 @@ __global__ __launch_bounds__(256) void csr_sum_kernel(const uint32_t* __restrict
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= n) return;
   TV s = 0;
-  for (uint32_t j = off[i]; j < off[i + 1]; ++j) s += slots[pos[j]];
+  // eight positions, then their eight slots, in flight together; the sum
+  // keeps the slot order
+  uint32_t j = off[i];
+  const uint32_t e = off[i + 1];
+  for (; j + 8 <= e; j += 8) {
+    uint32_t q[8];
+    TV v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) q[u] = pos[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = slots[q[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; j < e; ++j) s += slots[pos[j]];
   if (ASSIGN)
     out[i] = s;
   else
