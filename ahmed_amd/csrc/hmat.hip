@@ -546,7 +546,15 @@ __global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
   const uint64_t stride = pw ? (uint64_t)pw : 1;
   for (uint32_t i = threadIdx.x; i < r.m; i += kPanelThreads) {
     TV s = 0;
-    for (uint32_t c = 0; c < r.nchunks; ++c) s += pr[(uint64_t)c * r.m + i];
+    uint32_t c = 0;
+    for (; c + 8 <= r.nchunks; c += 8) {   // eight loads in flight, summed in order
+      TV v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = pr[(uint64_t)(c + u) * r.m + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; c < r.nchunks; ++c) s += pr[(uint64_t)c * r.m + i];
     TV* o = y + base + (r.row + i) * stride;
     if (ASSIGN)
       *o = s;
