@@ -121,7 +121,6 @@ struct TriSmem {
   double z[kMaxDiag];          // S: the leaf's vector; A/B/D: the input slice
   double tri[kTri * kTri];     // S: one triangle
   double red[kPanelThreads];   // cross-group partial sums
-  double push[kPushVals];      // S: the dense block it applies after its solve
   uint32_t item;
   uint32_t ok;
 };
@@ -411,6 +410,12 @@ __device__ void solve_bwd(const TriItem& it, const double* __restrict__ arena, d
 // One launch solves several right-hand sides as independent chains: chain
 // blockIdx.y has its own vector, slots, queue head and counters (block of
 // cstride words) and shares the read-only items, arena and the abort flag.
+//
+// PUSH_LDS: an S item stages the dense block it pushes in LDS while it waits
+// (one workgroup per CU: the fastest single chain); without, it reads that
+// block from the arena after its solve (two workgroups per CU, for launches
+// of several chains, whose workers are shared out).
+template <bool PUSH_LDS>
 __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
     const TriItem* __restrict__ items, uint32_t n_items, const double* __restrict__ arena,
     const uint32_t* __restrict__ perm, const double* __restrict__ rinv, double* acc, uint64_t ldb, double* tpart,
@@ -428,6 +433,7 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
   const bool forward = mode == 0 || mode == 2;
   const bool unit = mode == 0 || mode == 3;
   __shared__ TriSmem sm;
+  __shared__ double push_lds[PUSH_LDS ? kPushVals : 1];   // S: the dense block it pushes
   const int tid = threadIdx.x;
   for (;;) {
     if (tid == 0) {
@@ -443,7 +449,7 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
     Pf pf;
     if (it.type == kItemS) {
       solve_prefetch(it, arena, perm, rinv, forward, unit, sm, pre, pf);
-      if (it.in != kNone) {
+      if (PUSH_LDS && it.in != kNone) {
         // the block it pushes into the next leaf, staged in LDS while it waits
         const TriItem tp = items[n_items + it.in];
         const double* src = arena + tp.data;
@@ -458,7 +464,7 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
 #pragma unroll
           for (int q = 0; q < 16; ++q) {
             const uint32_t e = e0 + tid + q * kPanelThreads;
-            if (e < nv) sm.push[e] = v[q];
+            if (e < nv) push_lds[e] = v[q];
           }
         }
       }
@@ -497,9 +503,10 @@ __global__ __launch_bounds__(kPanelThreads) void trih_kernel(
         const int G = kPanelThreads / m;
         const int row = tid % m, g = tid / m;
         const double* zc = sm.z + (tp.in - it.out);
+        const double* pb = PUSH_LDS ? push_lds : arena + tp.data;
         double a = 0;
         if (g < G)
-          for (int c = g; c < (int)tp.W; c += G) a = fma(sm.push[row + c * m], zc[c], a);
+          for (int c = g; c < (int)tp.W; c += G) a = fma(pb[row + c * m], zc[c], a);
         sm.red[tid] = g < G ? a : 0.0;
         __syncthreads();
         if (tid < m) {
@@ -992,9 +999,11 @@ int ahm_trih_upload(ahm_ctx ctx, uint32_t n, char uplo, uint32_t n_leaves, const
     AHM_HIP(hipMemcpy(h->perm.p, perm.data(), perm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   if (D) AHM_HIP(hipMemcpy(h->pend0.p, pend0.data(), D * sizeof(uint32_t), hipMemcpyHostToDevice));
   int per_cu = 0, n_cu = 0;
-  AHM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trih_kernel, kPanelThreads, 0));
+  AHM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trih_kernel<true>, kPanelThreads, 0));
   AHM_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
   h->grid = (uint32_t)std::max<uint64_t>(1, (uint64_t)std::max(per_cu, 1) * n_cu);   // resident workgroups
+  AHM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trih_kernel<false>, kPanelThreads, 0));
+  h->grid_multi = (uint32_t)std::max<uint64_t>(1, (uint64_t)std::max(per_cu, 1) * n_cu);
   *out = h.release();
   return AHM_OK;
 }
@@ -1039,11 +1048,20 @@ int ahm_trih_solve(ahm_trih t, void* B, uint32_t ldb, uint32_t p) {
     double* x = (double*)B + (uint64_t)j0 * step;
     hipLaunchKernelGGL(trih_reset_kernel, dim3((cstride + 255) / 256, c), dim3(256), 0, s, cnt, cstride, n_zero,
                        (const uint32_t*)t->pend0.p, D);
-    const uint32_t gx = std::max<uint32_t>(1, std::min<uint64_t>(t->n_items, t->grid / c));
-    hipLaunchKernelGGL(trih_kernel, dim3(gx, c), dim3(kPanelThreads), 0, s, (const TriItem*)t->items.p,
-                       (uint32_t)t->n_items, (const double*)t->arena.p, (const uint32_t*)t->perm.p,
-                       (const double*)t->rinv.p, x, step,
-                       (double*)t->tpart.p, std::max<uint64_t>(t->t_len, 1), cnt, cstride, D, t->n_lr, t->mode);
+    // one chain: LDS-staged pushes, one workgroup per CU; several chains:
+    // two per CU without the staging, shared out among the chains
+    const uint32_t gx =
+        std::max<uint32_t>(1, std::min<uint64_t>(t->n_items, (c > 1 ? t->grid_multi : t->grid) / c));
+    if (c > 1)
+      hipLaunchKernelGGL(trih_kernel<false>, dim3(gx, c), dim3(kPanelThreads), 0, s, (const TriItem*)t->items.p,
+                         (uint32_t)t->n_items, (const double*)t->arena.p, (const uint32_t*)t->perm.p,
+                         (const double*)t->rinv.p, x, step, (double*)t->tpart.p, std::max<uint64_t>(t->t_len, 1),
+                         cnt, cstride, D, t->n_lr, t->mode);
+    else
+      hipLaunchKernelGGL(trih_kernel<true>, dim3(gx, c), dim3(kPanelThreads), 0, s, (const TriItem*)t->items.p,
+                         (uint32_t)t->n_items, (const double*)t->arena.p, (const uint32_t*)t->perm.p,
+                         (const double*)t->rinv.p, x, step, (double*)t->tpart.p, std::max<uint64_t>(t->t_len, 1),
+                         cnt, cstride, D, t->n_lr, t->mode);
     AHM_HIP(hipGetLastError());
   }
   uint32_t ab = 0;
