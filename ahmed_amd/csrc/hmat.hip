@@ -32,12 +32,45 @@ namespace ahm {
 // ------------------------------------------------------------------------
 // kernels
 
+// LDS tile of the transposed copy: rb rows x cols columns (cols <= 256, the
+// panel height) with one value of padding per column
+constexpr uint32_t kRelayoutTile = 4096;
+
 template <typename TD, typename TS>
 __global__ __launch_bounds__(256) void relayout_kernel(const CopyJob* __restrict__ jobs,
                                                        const TS* __restrict__ src,
                                                        TD* __restrict__ dst) {
   const CopyJob j = jobs[blockIdx.x];
   if (j.rows == 0) return;
+  if (j.transpose) {
+    // V (rows x cols, column-major) -> V^T rows of the panel: a tile of rb
+    // rows is read column by column and written row by row, so both sides
+    // move contiguous runs (rb values per column, cols values per row)
+    __shared__ TD tile[kRelayoutTile + 256];
+    const uint32_t cols = j.cols;
+    uint32_t lg = 0;  // rb = 2^lg, the largest power of two with rb * cols <= kRelayoutTile
+    while ((2u << lg) * cols <= kRelayoutTile) ++lg;
+    const uint32_t rb = 1u << lg, ld = rb + 1;
+    const uint32_t lgl = lg < 8 ? lg : 8;  // load lanes per column: min(rb, 256)
+    const uint32_t lr = threadIdx.x & ((1u << lgl) - 1), lc = threadIdx.x >> lgl;
+    const uint32_t lstep = 256u >> lgl;
+    const uint32_t step = cols, per = 256 / step;  // store: per rows of cols lanes
+    const uint32_t sub = threadIdx.x / step, c0 = threadIdx.x - sub * step;
+    for (uint32_t rbase = 0; rbase < j.rows; rbase += rb) {
+      const uint32_t rn = j.rows - rbase < rb ? j.rows - rbase : rb;
+      for (uint32_t c = lc; c < cols; c += lstep) {
+        const TS* s = src + j.src + (uint64_t)c * j.src_ld + rbase;
+        for (uint32_t r = lr; r < rn; r += 1u << lgl) tile[c * ld + r] = (TD)s[r];
+      }
+      __syncthreads();
+      if (sub < per) {
+        TD* d = dst + j.dst + (uint64_t)rbase * j.dst_ld + c0;
+        for (uint32_t r = sub; r < rn; r += per) d[(uint64_t)r * j.dst_ld] = tile[c0 * ld + r];
+      }
+      __syncthreads();
+    }
+    return;
+  }
   // column by column, no per-element division: a column shorter than the
   // workgroup shares it with the next blockDim.x / rows - 1 columns
   const uint32_t step = j.rows >= blockDim.x ? blockDim.x : j.rows;
@@ -46,13 +79,8 @@ __global__ __launch_bounds__(256) void relayout_kernel(const CopyJob* __restrict
   if (sub >= per) return;
   for (uint32_t c = sub; c < j.cols; c += per) {
     const TS* s = src + j.src + (uint64_t)c * j.src_ld;
-    if (j.transpose) {
-      TD* d = dst + j.dst + c;
-      for (uint32_t r = r0; r < j.rows; r += step) d[(uint64_t)r * j.dst_ld] = (TD)s[r];
-    } else {
-      TD* d = dst + j.dst + (uint64_t)c * j.dst_ld;
-      for (uint32_t r = r0; r < j.rows; r += step) d[r] = (TD)s[r];
-    }
+    TD* d = dst + j.dst + (uint64_t)c * j.dst_ld;
+    for (uint32_t r = r0; r < j.rows; r += step) d[r] = (TD)s[r];
   }
 }
 
