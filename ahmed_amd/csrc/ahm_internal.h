@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ahm.h"
@@ -58,6 +59,30 @@ struct ReduceTask {
 // whole 16-row tiles of 16 columns, so a panel (at most kPanelThreads rows)
 // may be read up to 15 columns and 15 values past its end.
 constexpr uint64_t kArenaSlack = 16 * (uint64_t)kPanelThreads + 64;
+
+// std::allocator whose value-less construct leaves trivial elements
+// uninitialised: resize() of a large host array that is filled right after
+// skips a serial zeroing pass over it.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <typename U>
+  void construct(U* p) noexcept {
+    ::new ((void*)p) U;
+  }
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+template <typename T>
+using HostVec = std::vector<T, NoInitAlloc<T>>;
 
 // zsrc entry: bit 31 set -> index into t, else index into x
 constexpr uint32_t kFromT = 0x80000000u;
@@ -137,7 +162,7 @@ struct ahm_hmat_s {
   std::vector<ahm_leaf_desc> leaves;
   // host copies of the plan (used by ahm_hmat_convert)
   std::vector<ahm::GemvTask> h_tasks_a, h_tasks_b;
-  std::vector<uint32_t> h_zsrc;
+  ahm::HostVec<uint32_t> h_zsrc;
   // transposed product (built on first use, hmat_t.hip): column-dot tasks of
   // both phases and, per destination (t index / column), the panel columns
   // that contribute to it (CSR, fixed order)

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <type_traits>
 
 #include "ahm_internal.h"
@@ -599,8 +600,8 @@ struct Plan {
   std::vector<GemvTask> a, b, a_panels, panels;
   std::vector<ReduceTask> r, ra;
   uint32_t part_len = 0;
-  std::vector<uint32_t> zsrc;
-  std::vector<CopyJob> jobs;
+  HostVec<uint32_t> zsrc;
+  HostVec<CopyJob> jobs;
   uint64_t nvals = 0;
   uint32_t t_len = 0, max_rank = 0;
   uint64_t n_dense = 0, n_lowrank = 0;
@@ -641,6 +642,42 @@ static void chunk_panels(const std::vector<GemvTask>& panels, uint64_t chunk_val
   });
 }
 
+// Host threads of the plan construction (at most 16).
+static unsigned plan_threads() {
+  const unsigned hw = std::thread::hardware_concurrency();
+  return std::max(1u, std::min(hw ? hw : 1u, 16u));
+}
+
+// Calls f(lo, hi) on host threads for contiguous ranges [lo, hi) of the n
+// items whose weight prefix sums are wsum (wsum[0] = 0, wsum[n] = total), cut
+// so that each range carries about the same weight.  f writes only to its own
+// items' pre-sized output ranges.
+template <typename F>
+static void parallel_by_weight(const std::vector<uint64_t>& wsum, const F& f) {
+  const size_t n = wsum.size() - 1;
+  const unsigned T = plan_threads();
+  if (T == 1 || n < 4 * (size_t)T) {
+    f((size_t)0, n);
+    return;
+  }
+  std::vector<size_t> cut(T + 1, n);
+  cut[0] = 0;
+  for (unsigned k = 1; k < T; ++k)
+    cut[k] = (size_t)(std::lower_bound(wsum.begin(), wsum.end(), wsum[n] / T * k) - wsum.begin());
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for (unsigned k = 0; k < T; ++k)
+    if (cut[k] < cut[k + 1]) th.emplace_back(f, cut[k], cut[k + 1]);
+  for (auto& t : th) t.join();
+}
+
+// The plan of a leaf list: phase A V^T panels (low-rank leaves grouped by
+// column cluster, <= 256 rows per panel) and phase B row panels (one per
+// row segment between leaf row boundaries, <= 256 rows each, [D | U]
+// columns of every leaf covering the segment, in leaf order), their column
+// source lists and the copy jobs that lay the leaves out.  Sizes are counted
+// first; the arrays are then filled in parallel, group by group and segment
+// by segment, each at its counted offset.
 static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_leaf_desc* L,
                       Plan& P) {
   const uint32_t TM = kPanelThreads;
@@ -654,125 +691,204 @@ static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_l
     if (d.rank >= 0 && (uint64_t)d.rank > std::min(d.n1, d.n2) + 64u)
       return fail(AHM_ERR_INVALID, "leaf %u rank %d exceeds its dimensions", l, d.rank);
   }
-  // t offsets: low-rank leaves grouped by column cluster, in leaf order
+  // low-rank leaves ordered by (b2, n2), leaf order among equals: a counting
+  // sort on b2 (stable), then an insertion sort on n2 inside each b2 bucket
   std::vector<uint32_t> lr;
-  for (uint32_t l = 0; l < nl; ++l)
-    if (L[l].rank > 0) lr.push_back(l);
-  std::stable_sort(lr.begin(), lr.end(), [&](uint32_t a, uint32_t b) {
-    return L[a].b2 != L[b].b2 ? L[a].b2 < L[b].b2 : L[a].n2 < L[b].n2;
-  });
-  std::vector<uint32_t> toff(nl, 0);
-  uint64_t arena = 0, t_len = 0;
-  size_t i = 0;
-  while (i < lr.size()) {
+  {
+    std::vector<uint32_t> start((size_t)n_cols + 1, 0);
+    size_t nlr = 0;
+    for (uint32_t l = 0; l < nl; ++l)
+      if (L[l].rank > 0) {
+        start[L[l].b2 + 1]++;
+        ++nlr;
+      }
+    for (uint32_t c = 0; c < n_cols; ++c) start[c + 1] += start[c];
+    lr.resize(nlr);
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+    for (uint32_t l = 0; l < nl; ++l)
+      if (L[l].rank > 0) lr[fill[L[l].b2]++] = l;
+    for (uint32_t c = 0; c < n_cols; ++c)
+      for (uint32_t q = start[c] + 1; q < start[c + 1]; ++q) {  // insertion sort, stable
+        const uint32_t v = lr[q];
+        uint32_t w = q;
+        for (; w > start[c] && L[lr[w - 1]].n2 > L[v].n2; --w) lr[w] = lr[w - 1];
+        lr[w] = v;
+      }
+  }
+  // phase A groups: runs of equal (b2, n2); K = sum of their ranks
+  struct GroupA {
+    size_t i, j;           // lr[i, j)
+    uint64_t K, arena, zoff, job, task, t0;
+  };
+  std::vector<GroupA> ga;
+  std::vector<uint64_t> wa(1, 0);
+  uint64_t arena = 0, t_len = 0, nz = 0, nj = 0, nt = 0;
+  for (size_t i = 0; i < lr.size();) {
     size_t j = i;
-    uint64_t K = 0;
-    while (j < lr.size() && L[lr[j]].b2 == L[lr[i]].b2 && L[lr[j]].n2 == L[lr[i]].n2) {
-      toff[lr[j]] = (uint32_t)(t_len + K);
-      K += (uint64_t)L[lr[j]].rank;
+    const ahm_leaf_desc& d0 = L[lr[i]];
+    uint64_t K = 0, jobs = 0;
+    while (j < lr.size() && L[lr[j]].b2 == d0.b2 && L[lr[j]].n2 == d0.n2) {
+      const uint64_t k = (uint64_t)L[lr[j]].rank;
+      jobs += (K + k - 1) / TM - K / TM + 1;  // panels the leaf's rows fall into
+      K += k;
       ++j;
     }
-    const uint32_t b2 = L[lr[i]].b2, n2 = L[lr[i]].n2;
-    // V^T panel of K rows, tiled into <= TM rows
-    for (uint64_t r0 = 0; r0 < K; r0 += TM) {
-      const uint32_t m = (uint32_t)std::min<uint64_t>(TM, K - r0);
-      GemvTask tk{};
-      tk.data = arena;
-      tk.m = m;
-      tk.W = n2;
-      tk.zoff = P.zsrc.size();
-      tk.out = (uint32_t)(t_len + r0);
-      for (uint32_t c = 0; c < n2; ++c) P.zsrc.push_back(b2 + c);
-      // copy the V columns of the leaves that fall into rows [r0, r0+m)
-      uint64_t rowbase = 0;
-      for (size_t q = i; q < j; ++q) {
-        const ahm_leaf_desc& d = L[lr[q]];
-        const uint64_t k = (uint64_t)d.rank;
-        const uint64_t lo = std::max(rowbase, r0), hi = std::min(rowbase + k, r0 + m);
-        if (lo < hi) {
-          CopyJob cj{};
-          cj.src = d.offset + (uint64_t)d.n1 * k + (lo - rowbase) * d.n2;
-          cj.src_ld = d.n2;
-          cj.rows = d.n2;
-          cj.cols = (uint32_t)(hi - lo);
-          cj.dst = arena + (lo - r0);
-          cj.dst_ld = m;
-          cj.transpose = 1;
-          P.jobs.push_back(cj);
-        }
-        rowbase += k;
-      }
-      P.a_panels.push_back(tk);
-      arena += (uint64_t)m * n2;
-    }
+    const uint64_t ntile = (K + TM - 1) / TM;
+    ga.push_back(GroupA{i, j, K, arena, nz, nj, nt, t_len});
+    arena += K * d0.n2;
+    nz += ntile * d0.n2;
+    nj += jobs;
+    nt += ntile;
     t_len += K;
+    wa.push_back(wa.back() + ntile * d0.n2 + 8 * jobs);
     i = j;
   }
   if (t_len >= kFromT) return fail(AHM_ERR_UNSUPPORTED, "sum of ranks %llu too large", (unsigned long long)t_len);
+  std::vector<uint32_t> toff(nl, 0);
+  P.a_panels.resize(nt);
   // row segments: cut at every leaf row boundary
+  std::vector<uint32_t> seg_at((size_t)n_rows + 1, 0);  // 1 at cuts, then the cut's index
+  seg_at[0] = seg_at[n_rows] = 1;
+  for (uint32_t l = 0; l < nl; ++l) seg_at[L[l].b1] = seg_at[L[l].b1 + L[l].n1] = 1;
   std::vector<uint32_t> cuts;
-  cuts.reserve(2 * (size_t)nl + 2);
-  cuts.push_back(0);
-  cuts.push_back(n_rows);
-  for (uint32_t l = 0; l < nl; ++l) {
-    cuts.push_back(L[l].b1);
-    cuts.push_back(L[l].b1 + L[l].n1);
-  }
-  std::sort(cuts.begin(), cuts.end());
-  cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+  for (uint32_t r = 0; r <= n_rows; ++r)
+    if (seg_at[r]) {
+      seg_at[r] = (uint32_t)cuts.size();
+      cuts.push_back(r);
+    }
   const size_t nseg = cuts.size() - 1;
   // leaves covering each segment (leaf order = reference traversal order)
   std::vector<uint32_t> cnt(nseg + 1, 0);
-  std::vector<std::pair<uint32_t, uint32_t>> span(nl);
   for (uint32_t l = 0; l < nl; ++l) {
-    if (L[l].rank == 0) { span[l] = {0, 0}; continue; }
-    const uint32_t s0 = (uint32_t)(std::lower_bound(cuts.begin(), cuts.end(), L[l].b1) - cuts.begin());
-    const uint32_t s1 =
-        (uint32_t)(std::lower_bound(cuts.begin(), cuts.end(), L[l].b1 + L[l].n1) - cuts.begin());
-    span[l] = {s0, s1};
-    for (uint32_t s = s0; s < s1; ++s) cnt[s + 1]++;
+    if (L[l].rank == 0) continue;
+    const uint32_t s0 = seg_at[L[l].b1], s1 = seg_at[L[l].b1 + L[l].n1];
+    for (uint32_t q = s0; q < s1; ++q) cnt[q + 1]++;
   }
-  for (size_t s = 0; s < nseg; ++s) cnt[s + 1] += cnt[s];
+  for (size_t q = 0; q < nseg; ++q) cnt[q + 1] += cnt[q];
   std::vector<uint32_t> cover(cnt[nseg]);
   {
     std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
-    for (uint32_t l = 0; l < nl; ++l)
-      for (uint32_t s = span[l].first; s < span[l].second; ++s) cover[fill[s]++] = l;
-  }
-  for (size_t s = 0; s < nseg; ++s) {
-    const uint32_t r0s = cuts[s], r1s = cuts[s + 1];
-    for (uint32_t r0 = r0s; r0 < r1s; r0 += TM) {
-      const uint32_t m = std::min(TM, r1s - r0);
-      GemvTask tk{};
-      tk.data = arena;
-      tk.m = m;
-      tk.zoff = P.zsrc.size();
-      tk.out = r0;
-      uint32_t W = 0;
-      for (uint32_t q = cnt[s]; q < cnt[s + 1]; ++q) {
-        const ahm_leaf_desc& d = L[cover[q]];
-        const uint32_t cols = d.rank < 0 ? d.n2 : (uint32_t)d.rank;
-        CopyJob cj{};
-        cj.src = d.offset + (r0 - d.b1);
-        cj.src_ld = d.n1;
-        cj.rows = m;
-        cj.cols = cols;
-        cj.dst = arena + (uint64_t)W * m;
-        cj.dst_ld = m;
-        cj.transpose = 0;
-        P.jobs.push_back(cj);
-        if (d.rank < 0)
-          for (uint32_t c = 0; c < cols; ++c) P.zsrc.push_back(d.b2 + c);
-        else
-          for (uint32_t c = 0; c < cols; ++c) P.zsrc.push_back(kFromT | (toff[cover[q]] + c));
-        W += cols;
-      }
-      tk.W = W;
-      tk.partial = 0;
-      if (W) P.panels.push_back(tk);
-      arena += (uint64_t)m * W;
+    for (uint32_t l = 0; l < nl; ++l) {
+      if (L[l].rank == 0) continue;
+      const uint32_t s0 = seg_at[L[l].b1], s1 = seg_at[L[l].b1 + L[l].n1];
+      for (uint32_t q = s0; q < s1; ++q) cover[fill[q]++] = l;
     }
   }
+  // phase B segment sizes: W columns, one panel per 256 rows
+  struct SegB {
+    uint64_t arena, zoff, job, task;
+    uint32_t W;
+  };
+  std::vector<SegB> sb(nseg);
+  std::vector<uint64_t> wb(nseg + 1, 0);
+  uint64_t ntb = 0;
+  for (size_t q = 0; q < nseg; ++q) {
+    uint64_t W = 0;
+    for (uint32_t c = cnt[q]; c < cnt[q + 1]; ++c) {
+      const ahm_leaf_desc& d = L[cover[c]];
+      W += d.rank < 0 ? d.n2 : (uint32_t)d.rank;
+    }
+    if (W > 0xFFFFFFFFull) return fail(AHM_ERR_UNSUPPORTED, "row segment %zu has %llu columns", q, (unsigned long long)W);
+    const uint64_t rows = cuts[q + 1] - cuts[q], ntile = W ? (rows + TM - 1) / TM : 0;
+    const uint64_t jobs = ntile * (cnt[q + 1] - cnt[q]);
+    sb[q] = SegB{arena, nz, nj, ntb, (uint32_t)W};
+    arena += rows * W;
+    nz += ntile * W;
+    nj += jobs;
+    ntb += ntile;
+    wb[q + 1] = wb[q] + ntile * W + 8 * jobs;
+  }
+  P.zsrc.resize(nz);
+  P.jobs.resize(nj);
+  P.panels.resize(ntb);
+  uint32_t* const Z = P.zsrc.data();
+  CopyJob* const J = P.jobs.data();
+  // phase A fill: V^T panel of K rows, tiled into <= TM rows
+  parallel_by_weight(wa, [&](size_t g0, size_t g1) {
+    for (size_t g = g0; g < g1; ++g) {
+      const GroupA& G = ga[g];
+      const uint32_t b2 = L[lr[G.i]].b2, n2 = L[lr[G.i]].n2;
+      uint64_t K = 0;
+      for (size_t q = G.i; q < G.j; ++q) {
+        toff[lr[q]] = (uint32_t)(G.t0 + K);
+        K += (uint64_t)L[lr[q]].rank;
+      }
+      uint64_t ar = G.arena, zo = G.zoff, jo = G.job, ta = G.task;
+      for (uint64_t r0 = 0; r0 < K; r0 += TM) {
+        const uint32_t m = (uint32_t)std::min<uint64_t>(TM, K - r0);
+        GemvTask tk{};
+        tk.data = ar;
+        tk.m = m;
+        tk.W = n2;
+        tk.zoff = zo;
+        tk.out = (uint32_t)(G.t0 + r0);
+        for (uint32_t c = 0; c < n2; ++c) Z[zo + c] = b2 + c;
+        zo += n2;
+        // copy the V columns of the leaves that fall into rows [r0, r0+m)
+        uint64_t rowbase = 0;
+        for (size_t q = G.i; q < G.j; ++q) {
+          const ahm_leaf_desc& d = L[lr[q]];
+          const uint64_t k = (uint64_t)d.rank;
+          const uint64_t lo = std::max(rowbase, r0), hi = std::min(rowbase + k, r0 + m);
+          if (lo < hi) {
+            CopyJob cj{};
+            cj.src = d.offset + (uint64_t)d.n1 * k + (lo - rowbase) * d.n2;
+            cj.src_ld = d.n2;
+            cj.rows = d.n2;
+            cj.cols = (uint32_t)(hi - lo);
+            cj.dst = ar + (lo - r0);
+            cj.dst_ld = m;
+            cj.transpose = 1;
+            J[jo++] = cj;
+          }
+          rowbase += k;
+        }
+        P.a_panels[ta++] = tk;
+        ar += (uint64_t)m * n2;
+      }
+    }
+  });
+  // phase B fill
+  parallel_by_weight(wb, [&](size_t q0, size_t q1) {
+    for (size_t q = q0; q < q1; ++q) {
+      const SegB& S = sb[q];
+      if (!S.W) continue;
+      const uint32_t r0s = cuts[q], r1s = cuts[q + 1];
+      uint64_t ar = S.arena, zo = S.zoff, jo = S.job, ta = S.task;
+      for (uint32_t r0 = r0s; r0 < r1s; r0 += TM) {
+        const uint32_t m = std::min(TM, r1s - r0);
+        GemvTask tk{};
+        tk.data = ar;
+        tk.m = m;
+        tk.zoff = zo;
+        tk.out = r0;
+        uint32_t W = 0;
+        for (uint32_t c = cnt[q]; c < cnt[q + 1]; ++c) {
+          const ahm_leaf_desc& d = L[cover[c]];
+          const uint32_t cols = d.rank < 0 ? d.n2 : (uint32_t)d.rank;
+          CopyJob cj{};
+          cj.src = d.offset + (r0 - d.b1);
+          cj.src_ld = d.n1;
+          cj.rows = m;
+          cj.cols = cols;
+          cj.dst = ar + (uint64_t)W * m;
+          cj.dst_ld = m;
+          cj.transpose = 0;
+          J[jo++] = cj;
+          if (d.rank < 0)
+            for (uint32_t k = 0; k < cols; ++k) Z[zo + k] = d.b2 + k;
+          else
+            for (uint32_t k = 0; k < cols; ++k) Z[zo + k] = kFromT | (toff[cover[c]] + k);
+          zo += cols;
+          W += cols;
+        }
+        tk.W = W;
+        tk.partial = 0;
+        P.panels[ta++] = tk;
+        ar += (uint64_t)m * W;
+      }
+    }
+  });
   // wide panels of both phases cut into column chunks, biggest tasks first
   // (the hardware dispatches workgroups in order); the two phases use the
   // partial-sum slots in turn
@@ -796,8 +912,8 @@ static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_l
   return AHM_OK;
 }
 
-template <typename T>
-static int upload_vec(DeviceBuf& b, const std::vector<T>& v, hipStream_t s) {
+template <typename T, typename A>
+static int upload_vec(DeviceBuf& b, const std::vector<T, A>& v, hipStream_t s) {
   AHM_TRY(b.alloc(v.size() * sizeof(T)));
   if (!v.empty()) AHM_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
   return AHM_OK;
