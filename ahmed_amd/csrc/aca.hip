@@ -630,6 +630,211 @@ __device__ void apply_q(const double* A, uint32_t n, uint32_t r, const double* t
   __syncthreads();
 }
 
+// Q [C; 0] into O (n x kt, ld n) for Q = H_0 ... H_{r-1}, the reflectors of
+// A (n x r, ld n; unit diagonal implicit, vector below it) and tau, and the
+// core C (r x kt, ld r), in compact WY form: Q = I - Y T Y^T with
+// T^{-1} = diag(1/tau) + the strictly upper part of Y^T Y (the UT transform
+// of Joffrain, Low, Quintana-Orti, van de Geijn and Van Zee, 2006).  Then
+// Q [C; 0] = [C; 0] - Y M with T^{-1} M = Y_top^T C (Y_top: rows 0..r-1),
+// solved by back substitution; tau_j = 0 (H_j = I) gives a zero row j of M.
+// Dots of Y^T Y all at once, a triangular solve per column and a row-parallel
+// product take four barriers instead of two per reflector in apply_q.
+// lds holds n*r + r*r + 2*r*kt + r values.
+__device__ void wy_apply(const double* A, uint32_t n, uint32_t r, const double* tau,
+                         const double* C, uint32_t kt, double* O, double* lds) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* Y = lds;                      // n x r
+  double* G = Y + (size_t)n * r;        // r x r, strictly upper part
+  double* Cl = G + (size_t)r * r;       // r x kt
+  double* S = Cl + (size_t)r * kt;      // r x kt: Y_top^T C, then M
+  double* tl = S + (size_t)r * kt;      // r: tau
+  for (uint32_t e = threadIdx.x; e < n * r; e += kAcaThreads) Y[e] = A[e];
+  for (uint32_t e = threadIdx.x; e < r; e += kAcaThreads) tl[e] = tau[e];
+  for (uint32_t e = threadIdx.x; e < r * kt; e += kAcaThreads) Cl[e] = C[e];
+  __syncthreads();
+  // G[i + j r] = y_i . y_j (i < j) = A[j, i] + sum_{m > j} A[m, i] A[m, j]
+  const uint32_t npair = r * (r - 1) / 2;
+  auto pair_of = [](uint32_t p, uint32_t& i, uint32_t& j) {  // p = j (j - 1) / 2 + i
+    j = (uint32_t)((1.0 + sqrt(1.0 + 8.0 * (double)p)) * 0.5);
+    while (j * (j - 1) / 2 > p) --j;
+    while ((j + 1) * j / 2 <= p) ++j;
+    i = p - j * (j - 1) / 2;
+  };
+  if (n <= 256) {  // one thread per pair
+    for (uint32_t p = threadIdx.x; p < npair; p += kAcaThreads) {
+      uint32_t i, j;
+      pair_of(p, i, j);
+      const double* yi = Y + (size_t)i * n;
+      const double* yj = Y + (size_t)j * n;
+      double d = yi[j];
+      for (uint32_t m = j + 1; m < n; ++m) d = fma(yi[m], yj[m], d);
+      G[i + (size_t)j * r] = d;
+    }
+  } else {  // one wavefront per pair
+    for (uint32_t p = w; p < npair; p += kWaves) {
+      uint32_t i, j;
+      pair_of(p, i, j);
+      const double* yi = Y + (size_t)i * n;
+      const double* yj = Y + (size_t)j * n;
+      double d = 0.0;
+      for (uint32_t m = j + 1 + lane; m < n; m += 64) d = fma(yi[m], yj[m], d);
+      d = wave_sum(d);
+      if (lane == 0) G[i + (size_t)j * r] = yi[j] + d;
+    }
+  }
+  // S[i + c r] = C[i, c] + sum_{i < m < r} A[m, i] C[m, c]  (r <= 65 here)
+  {
+    const uint32_t per = kAcaThreads / r, sub = threadIdx.x / r, i = threadIdx.x - sub * r;
+    if (sub < per)
+      for (uint32_t c = sub; c < kt; c += per) {
+        const double* cc = Cl + (size_t)c * r;
+        const double* yi = Y + (size_t)i * n;
+        double v = cc[i];
+        for (uint32_t m = i + 1; m < r; ++m) v = fma(yi[m], cc[m], v);
+        S[i + (size_t)c * r] = v;
+      }
+  }
+  __syncthreads();
+  // M[i, c] = tau_i (S[i, c] - sum_{l > i} G[i, l] M[l, c]), in place over S
+  for (uint32_t c = threadIdx.x; c < kt; c += kAcaThreads) {
+    double* sc = S + (size_t)c * r;
+    for (int i = (int)r - 1; i >= 0; --i) {
+      double v = sc[i];
+      for (uint32_t l = (uint32_t)i + 1; l < r; ++l) v = fma(-G[i + (size_t)l * r], sc[l], v);
+      sc[i] = tl[i] * v;
+    }
+  }
+  __syncthreads();
+  // O[m, c] = [m < r] C[m, c] - sum_{i < min(m, r)} A[m, i] M[i, c] - [m < r] M[m, c]
+  const uint32_t step = n >= kAcaThreads ? kAcaThreads : n, per = kAcaThreads / step;
+  const uint32_t sub = threadIdx.x / step, m0 = threadIdx.x - sub * step;
+  if (sub < per)
+    for (uint32_t c = sub; c < kt; c += per) {
+      const double* mc = S + (size_t)c * r;
+      for (uint32_t m = m0; m < n; m += step) {
+        const uint32_t ie = m < r ? m : r;
+        double v = 0.0;
+        for (uint32_t i = 0; i < ie; ++i) v = fma(Y[m + (size_t)i * n], mc[i], v);
+        O[m + (size_t)c * n] = m < r ? (Cl[m + (size_t)c * r] - mc[m]) - v : -v;
+      }
+    }
+  __syncthreads();
+}
+
+typedef double aca_f64x4 __attribute__((ext_vector_type(4)));
+
+// wy_apply for reflectors too long for LDS: A stays in memory; the core, M,
+// tau and the Gram matrix live in LDS (r*r + 2*r*kt + r values, r <= 16*NB).
+// Y^T Y runs on v_mfma_f64_16x16x4f64, one wavefront per group of 16 rows
+// (groups w, w+4, ...): lane l loads rows 4(l/16) .. 4(l/16)+3 of column
+// 16b + l%16 of each column block b and feeds row 4(l/16)+s of the group to
+// k-step s (A operand: lane l holds (row l%16, k l/16); B: (k l/16, column
+// l%16); result value q of lane l: (l/16 + 4q, l%16)).  The four
+// wavefronts' tiles are summed in LDS in wavefront order.  The product
+// O = [C; 0] - Y M runs one row per thread, eight columns of M at a time.
+template <int NB>
+__device__ void wy_apply_big(const double* A, uint32_t n, uint32_t r, const double* tau,
+                             const double* C, uint32_t kt, double* O, double* lds) {
+  constexpr int NT = NB * (NB + 1) / 2;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* G = lds;                      // r x r: first the top r x r of A, then y_i . y_j
+  double* S = G + (size_t)r * r;        // r x kt: Y_top^T C, then M
+  double* Cl = S + (size_t)r * kt;      // r x kt
+  double* tl = Cl + (size_t)r * kt;     // r
+  for (uint32_t c = 0; c < r; ++c)
+    for (uint32_t m = threadIdx.x; m < r; m += kAcaThreads) G[m + (size_t)c * r] = A[m + (size_t)c * n];
+  for (uint32_t e = threadIdx.x; e < r * kt; e += kAcaThreads) Cl[e] = C[e];
+  for (uint32_t e = threadIdx.x; e < r; e += kAcaThreads) tl[e] = tau[e];
+  __syncthreads();
+  {  // S[i + c r] = C[i, c] + sum_{i < m < r} A[m, i] C[m, c]
+    const uint32_t per = kAcaThreads / r, sub = threadIdx.x / r, i = threadIdx.x - sub * r;
+    if (sub < per)
+      for (uint32_t c = sub; c < kt; c += per) {
+        const double* cc = Cl + (size_t)c * r;
+        const double* ai = G + (size_t)i * r;
+        double v = cc[i];
+        for (uint32_t m = i + 1; m < r; ++m) v = fma(ai[m], cc[m], v);
+        S[i + (size_t)c * r] = v;
+      }
+  }
+  aca_f64x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = aca_f64x4{0.0, 0.0, 0.0, 0.0};
+  const uint32_t li = lane & 15, kk = lane >> 4, ng = (n + 15) / 16;
+  for (uint32_t g = w; g < ng; g += kWaves) {
+    double y[NB][4];
+    const uint32_t mb = g * 16 + 4 * kk;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const uint32_t c = b * 16 + li;
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const uint32_t m = mb + st;
+        double v = 0.0;
+        if (c < r && m < n) v = m > c ? A[m + (size_t)c * n] : (m == c ? 1.0 : 0.0);
+        y[b][st] = v;
+      }
+    }
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      int t = 0;
+#pragma unroll
+      for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+        for (int bj = bi; bj < NB; ++bj, ++t)
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[bi][st], y[bj][st], acc[t], 0, 0, 0);
+    }
+  }
+  __syncthreads();  // S done and the top of A no longer read: G is free
+  for (int q = 0; q < kWaves; ++q) {
+    if (w == q) {
+      int t = 0;
+#pragma unroll
+      for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+        for (int bj = bi; bj < NB; ++bj, ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const uint32_t i = bi * 16 + kk + 4 * v, j = bj * 16 + li;
+            if (i < j && j < r) G[i + (size_t)j * r] = (q == 0 ? 0.0 : G[i + (size_t)j * r]) + acc[t][v];
+          }
+    }
+    __syncthreads();
+  }
+  // M[i, c] = tau_i (S[i, c] - sum_{l > i} G[i, l] M[l, c]), in place over S
+  for (uint32_t c = threadIdx.x; c < kt; c += kAcaThreads) {
+    double* sc = S + (size_t)c * r;
+    for (int i = (int)r - 1; i >= 0; --i) {
+      double v = sc[i];
+      for (uint32_t l = (uint32_t)i + 1; l < r; ++l) v = fma(-G[i + (size_t)l * r], sc[l], v);
+      sc[i] = tl[i] * v;
+    }
+  }
+  __syncthreads();
+  // O[m, c] = [m < r] (C[m, c] - M[m, c]) - sum_{i < min(m, r)} A[m, i] M[i, c]
+  for (uint32_t m = threadIdx.x; m < n; m += kAcaThreads) {
+    const uint32_t ie = m < r ? m : r;
+    for (uint32_t c0 = 0; c0 < kt; c0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = 0.0;
+      for (uint32_t i = 0; i < ie; ++i) {
+        const double a = A[m + (size_t)i * n];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (c0 + u < kt) v[u] = fma(a, S[i + (size_t)(c0 + u) * r], v[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (c0 + u < kt) {
+          const uint32_t c = c0 + u;
+          O[m + (size_t)c * n] = m < r ? (Cl[m + (size_t)c * r] - S[m + (size_t)c * r]) - v[u] : -v[u];
+        }
+    }
+  }
+  __syncthreads();
+}
+
 // One-sided Jacobi SVD of R (mm x nn, ld mm): R <- R Z with orthogonal Z
 // (nn x nn, ld nn) until the columns of R are mutually orthogonal.  Column
 // pairs are visited in round-robin tournament order, one wavefront per pair.
@@ -825,7 +1030,8 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock*
 }
 
 // final U = Q1 [W; 0] (n1 x kt), V = Q2 [Z; 0] (n2 x kt) at the leaf's offset
-__global__ __launch_bounds__(kAcaThreads) void expand_kernel(const AcaBlock* __restrict__ blocks,
+template <bool BIG>
+__global__ __launch_bounds__(kAcaThreads, 4) void expand_kernel(const AcaBlock* __restrict__ blocks,
                                                              const AcaResult* __restrict__ res,
                                                              const uint64_t* __restrict__ out_off,
                                                              const double* __restrict__ slab_c,
@@ -850,6 +1056,18 @@ __global__ __launch_bounds__(kAcaThreads) void expand_kernel(const AcaBlock* __r
   // (r x kt, ld r), into O (n x kt); on LDS copies when they fit
   auto factor = [&](const double* A, uint32_t n, uint32_t r, const double* tau, const double* C, double* O) {
     const size_t na = (size_t)n * r, no = (size_t)n * kt;
+    const size_t nsm = (size_t)r * r + 2 * (size_t)r * kt + r;
+    const bool big = na + nsm > kRcLds && nsm <= kRcLds && r <= 48;
+    if (big != BIG) return;  // the other instance expands this factor
+    if constexpr (BIG) {
+      if (r <= 16) wy_apply_big<1>(A, n, r, tau, C, kt, O, rc_lds);
+      else if (r <= 32) wy_apply_big<2>(A, n, r, tau, C, kt, O, rc_lds);
+      else wy_apply_big<3>(A, n, r, tau, C, kt, O, rc_lds);
+      return;
+    } else if (na + nsm <= kRcLds) {
+      wy_apply(A, n, r, tau, C, kt, O, rc_lds);
+      return;
+    }
     double* X = O;
     const double* Ar = A;
     if (na + no <= kRcLds) {
@@ -1126,7 +1344,13 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
           cleanup();
           return st;
         }
-        hipLaunchKernelGGL(expand_kernel, dim3(nb), dim3(kAcaThreads), kRcLds * sizeof(double), s,
+        // factors whose reflectors outgrow LDS but whose small matrices fit
+        // run in a second instance (MFMA Gram, more registers)
+        hipLaunchKernelGGL(expand_kernel<false>, dim3(nb), dim3(kAcaThreads), kRcLds * sizeof(double), s,
+                           (const AcaBlock*)dblk.p,
+                           (const AcaResult*)dres.p, (const uint64_t*)doff.p,
+                           (const double*)dslab.p, (double*)part->buf.p);
+        hipLaunchKernelGGL(expand_kernel<true>, dim3(nb), dim3(kAcaThreads), kRcLds * sizeof(double), s,
                            (const AcaBlock*)dblk.p,
                            (const AcaResult*)dres.p, (const uint64_t*)doff.p,
                            (const double*)dslab.p, (double*)part->buf.p);

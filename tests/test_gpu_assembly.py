@@ -146,3 +146,56 @@ def test_rankmax_one_forces_dense(ctx):
     # ACA with maxk = 1 fails on these blocks -> every leaf stored dense
     assert st["n_candidates"] == int(lv["adm"].sum())
     assert (leaves["rank"] < 0).mean() > 0.9
+
+
+def expansion_path(n, r, kt, lds=4352):
+    """Which expansion a factor of the recompressed leaf takes on the GPU:
+    'lds' (reflectors in LDS), 'long' (reflectors in memory, MFMA Gram) or
+    'seq' (reflectors applied one by one)."""
+    small = r * r + 2 * r * kt + r
+    if n * r + small <= lds:
+        return "lds"
+    if small <= lds and r <= 48:
+        return "long"
+    return "seq"
+
+
+def test_expansion_paths_accuracy(ctx):
+    """Leaves whose factors expand through each path (compact WY with the
+    reflectors in LDS, compact WY with an MFMA Gram matrix for reflectors in
+    memory, one-by-one reflectors) approximate the exact kernel block to
+    20*eps in Frobenius norm (Include/apprx.h:47)."""
+    from ahmed_amd import _lib, geometry
+    N, eps = 16000, 1e-6
+    pts = geometry.cube_surface(N, 3)
+    T = _lib.Tree(pts, 64, 0.64)
+    print(f"[gpu] expansion paths N={N}", flush=True)
+    D, st = _lib.HMatrix.assemble(ctx, T, _lib.KERNEL_LAPLACE, 0.0, 1.0 / N, eps, keep_leaves=True,
+                                  aca_ranks=True)
+    leaves, data = D.download()
+    kaca = st["aca_rank"]
+    op, _ = T.perm()
+    o = oracle_api()
+    by_path = {"lds": [], "long": [], "seq": []}
+    for q, d in enumerate(leaves):
+        kt = int(d["rank"])
+        if kt <= 0:
+            continue
+        n1, n2, k = int(d["n1"]), int(d["n2"]), int(kaca[q])
+        paths = {expansion_path(n1, min(n1, k), kt), expansion_path(n2, min(n2, k), kt)}
+        for p in paths:
+            by_path[p].append(q)
+    print({p: len(v) for p, v in by_path.items()}, flush=True)
+    assert len(by_path["lds"]) > 0 and len(by_path["long"]) > 0
+    rng = np.random.default_rng(0)
+    for p, idx in by_path.items():
+        # the largest leaves of the path and a random sample of the rest
+        idx = sorted(idx, key=lambda q: -int(leaves[q]["n1"]) * int(leaves[q]["n2"]))
+        pick = idx[:6] + list(rng.choice(idx, size=min(10, len(idx)), replace=False)) if idx else []
+        for q in pick:
+            d = leaves[q]
+            b1, n1, b2, n2 = (int(d[f]) for f in ("b1", "n1", "b2", "n2"))
+            K = o.kernel_block(_lib.KERNEL_LAPLACE, 0.0, pts, op, b1, n1, b2, n2, weight=1.0 / N)
+            _, blk = next(leaf_blocks(leaves[q:q + 1], data))
+            err = np.linalg.norm(blk - K) / np.linalg.norm(K)
+            assert err <= 20 * eps, (p, q, n1, n2, int(d["rank"]), err)
