@@ -65,6 +65,7 @@ SIGNATURES = {
     "ahm_last_error": (C.c_char_p, []),
     "ahm_ctx_create": (_i, [_i, C.POINTER(_p)]),
     "ahm_ctx_destroy": (_i, [_p]),
+    "ahm_ctx_release_cached": (_i, [_p]),
     "ahm_ctx_set_stream": (_i, [_p, _p]),
     "ahm_ctx_sync": (_i, [_p]),
     "ahm_tree_build": (_i, [_p, _u32, _u32, _d, _i, C.POINTER(_p)]),
@@ -180,6 +181,10 @@ class Context:
 
     def sync(self):
         check(load().ahm_ctx_sync(self.h), "ahm_ctx_sync")
+
+    def release_cached(self):
+        """Free the cached device blocks (>= 1 GiB) of this device."""
+        check(load().ahm_ctx_release_cached(self.h), "ahm_ctx_release_cached")
 
     def close(self):
         if self.h:

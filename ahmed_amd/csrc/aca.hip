@@ -1217,8 +1217,12 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
   uint64_t budget = (uint64_t)24 << 30;
   {
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b)
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b) {
+      free_b += cache_bytes(dev);   // blocks the allocation cache holds are free to reuse
       budget = std::min<uint64_t>(std::max<uint64_t>(free_b / 4, (uint64_t)4 << 30), (uint64_t)96 << 30);
+    }
   }
   std::vector<Part*> parts;
   std::vector<uint32_t> part_of(nl, 0);

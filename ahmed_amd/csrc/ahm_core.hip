@@ -3,6 +3,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
+#include <vector>
 
 #include "ahm_internal.h"
 
@@ -29,22 +31,108 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+namespace {
+constexpr size_t kCacheMin = (size_t)1 << 30;     // blocks at least this large are cached
+constexpr size_t kCacheMax = (size_t)128 << 30;   // per device
+struct CachedBlock {
+  void* p;
+  size_t cap;
+  int dev;
+};
+std::mutex g_cache_mu;
+std::vector<CachedBlock> g_cache;
+
+size_t cache_bytes_locked(int dev) {
+  size_t s = 0;
+  for (const CachedBlock& c : g_cache)
+    if (c.dev == dev) s += c.cap;
+  return s;
+}
+}  // namespace
+
+size_t cache_bytes(int device) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  return cache_bytes_locked(device);
+}
+
+void cache_flush(int device) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (size_t i = 0; i < g_cache.size();) {
+    if (g_cache[i].dev == device) {
+      (void)hipSetDevice(device);
+      (void)hipFree(g_cache[i].p);
+      g_cache.erase(g_cache.begin() + (ptrdiff_t)i);
+    } else {
+      ++i;
+    }
+  }
+  (void)hipSetDevice(cur);
+}
+
 int DeviceBuf::alloc(size_t b) {
   release();
   if (b == 0) return AHM_OK;
+  (void)hipGetDevice(&dev);
+  if (b >= kCacheMin) {
+    // best fit among the cached blocks of at most twice the size: a small
+    // request must not take a block that a later large one would reuse
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    size_t best = g_cache.size();
+    for (size_t i = 0; i < g_cache.size(); ++i) {
+      const CachedBlock& c = g_cache[i];
+      if (c.dev == dev && c.cap >= b && c.cap / 2 <= b && (best == g_cache.size() || c.cap < g_cache[best].cap))
+        best = i;
+    }
+    if (best < g_cache.size()) {
+      p = g_cache[best].p;
+      cap = g_cache[best].cap;
+      bytes = b;
+      g_cache.erase(g_cache.begin() + (ptrdiff_t)best);
+      return AHM_OK;
+    }
+  }
   hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess && cache_bytes(dev) > 0) {
+    (void)hipGetLastError();
+    cache_flush(dev);
+    e = hipMalloc(&p, b);
+  }
   if (e != hipSuccess) {
     p = nullptr;
     return fail(AHM_ERR_NOMEM, "hipMalloc(%zu bytes): %s", b, hipGetErrorString(e));
   }
-  bytes = b;
+  bytes = cap = b;
   return AHM_OK;
 }
 
 void DeviceBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p) {
+    if (cap >= kCacheMin) {
+      // the same wait as hipFree's: work still using the block finishes first
+      int cur = 0;
+      (void)hipGetDevice(&cur);
+      (void)hipSetDevice(dev);
+      (void)hipDeviceSynchronize();
+      std::lock_guard<std::mutex> lk(g_cache_mu);
+      g_cache.push_back(CachedBlock{p, cap, dev});
+      // over the limit: free the oldest blocks of the device
+      while (cache_bytes_locked(dev) > kCacheMax) {
+        for (size_t i = 0; i < g_cache.size(); ++i)
+          if (g_cache[i].dev == dev) {
+            (void)hipFree(g_cache[i].p);
+            g_cache.erase(g_cache.begin() + (ptrdiff_t)i);
+            break;
+          }
+      }
+      (void)hipSetDevice(cur);
+    } else {
+      (void)hipFree(p);
+    }
+  }
   p = nullptr;
-  bytes = 0;
+  bytes = cap = 0;
 }
 
 static double now_ms() {
@@ -102,7 +190,14 @@ int ahm_ctx_destroy(ahm_ctx c) {
     (void)hipStreamSynchronize(c->own);
     (void)hipStreamDestroy(c->own);
   }
+  ahm::cache_flush(c->device);
   delete c;
+  return AHM_OK;
+}
+
+int ahm_ctx_release_cached(ahm_ctx c) {
+  if (!c) return ahm::fail(AHM_ERR_INVALID, "ahm_ctx_release_cached: NULL context");
+  ahm::cache_flush(c->device);
   return AHM_OK;
 }
 

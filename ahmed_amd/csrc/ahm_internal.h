@@ -96,13 +96,25 @@ struct CopyJob {
   uint32_t transpose, pad;
 };
 
+// Device allocation.  Blocks of at least 1 GiB are not freed on release but
+// kept in a process-wide cache (per device, at most 128 GiB) and handed to
+// the next large request of at least half their size (best fit): a 50 GB
+// hipMalloc right after large frees took 2.3 s on MI355X, against under
+// 1 ms on memory not used before, so a second assembly in one process would
+// pay that for each large buffer.  A failed hipMalloc empties the device's
+// cache and retries; cache_flush(device) (context destruction,
+// ahm_ctx_release_cached) returns the blocks.
 struct DeviceBuf {
   void* p = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;   // requested size
+  size_t cap = 0;     // size of the block (>= bytes when it came from the cache)
+  int dev = 0;
   int alloc(size_t b);
   void release();
   ~DeviceBuf() { release(); }
 };
+size_t cache_bytes(int device);   // bytes held by the cache on the device
+void cache_flush(int device);
 
 size_t dtype_size(ahm_dtype t);
 

@@ -1138,8 +1138,10 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
     if (keep_leaves) {
       h->leafdata.p = raw.p;
       h->leafdata.bytes = raw.bytes;
+      h->leafdata.cap = raw.cap;
+      h->leafdata.dev = raw.dev;
       raw.p = nullptr;
-      raw.bytes = 0;
+      raw.bytes = raw.cap = 0;
     }
   } while (0);
   if (st) {

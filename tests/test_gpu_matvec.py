@@ -244,3 +244,31 @@ def test_mfma_product_every_panel_height(ctx, p):
         Y32 = dev(np.zeros((p, N)), torch.float32)
         D32.matvec(dev(X.T.copy(), torch.float32), Y32, 1.0)
         assert relerr(Y32.cpu().numpy().T.astype(np.float64), A @ X) <= 5e-6
+
+
+def test_large_blocks_reused_from_the_allocation_cache(ctx):
+    """Buffers of >= 1 GiB go to the allocation cache when released and are
+    handed to the next large request (best fit): matrices uploaded one after
+    the other into reused blocks multiply correctly, the second one smaller
+    than the first, and releasing the cache leaves the context usable."""
+    from ahmed_amd import _lib
+    rng = np.random.default_rng(7)
+    for N in (12000, 11600):   # 1.15 GB and 1.08 GB of dense values
+        h = N // 2
+        print(f"[gpu] allocation cache N={N}", flush=True)
+        A = rng.standard_normal((N, N))
+        leaves = _lib.leaf_array([0, h], [h, N - h], [0, 0], [N, N], [-1, -1], [0, h * N])
+        data = np.concatenate([A[:h].T.ravel(), A[h:].T.ravel()])
+        H = _lib.HMatrix.upload(ctx, N, N, leaves, data)
+        x = rng.standard_normal(N)
+        y = torch.zeros(N, dtype=torch.float64, device="cuda")
+        H.matvec(dev(x), y)
+        assert relerr(y.cpu().numpy(), A @ x) <= 1e-13
+        del H
+    ctx.release_cached()
+    H = _lib.HMatrix.upload(ctx, 64, 64, _lib.leaf_array([0], [64], [0], [64], [-1], [0]),
+                            np.arange(64 * 64, dtype=np.float64))
+    y = torch.zeros(64, dtype=torch.float64, device="cuda")
+    H.matvec(dev(np.ones(64)), y)
+    ref = np.arange(64 * 64, dtype=np.float64).reshape(64, 64).T @ np.ones(64)
+    assert relerr(y.cpu().numpy(), ref) <= 1e-15
