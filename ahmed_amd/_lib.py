@@ -66,6 +66,7 @@ SIGNATURES = {
     "ahm_ctx_create": (_i, [_i, C.POINTER(_p)]),
     "ahm_ctx_destroy": (_i, [_p]),
     "ahm_ctx_release_cached": (_i, [_p]),
+    "ahm_ctx_cached_bytes": (_i, [_p, C.POINTER(C.c_uint64)]),
     "ahm_ctx_set_stream": (_i, [_p, _p]),
     "ahm_ctx_sync": (_i, [_p]),
     "ahm_tree_build": (_i, [_p, _u32, _u32, _d, _i, C.POINTER(_p)]),
@@ -185,6 +186,12 @@ class Context:
     def release_cached(self):
         """Free the cached device blocks (>= 1 GiB) of this device."""
         check(load().ahm_ctx_release_cached(self.h), "ahm_ctx_release_cached")
+
+    def cached_bytes(self) -> int:
+        """Bytes of released device blocks the allocation cache holds."""
+        v = C.c_uint64(0)
+        check(load().ahm_ctx_cached_bytes(self.h, C.byref(v)), "ahm_ctx_cached_bytes")
+        return int(v.value)
 
     def close(self):
         if self.h:
@@ -336,7 +343,9 @@ class HMatrix:
         check(load().ahm_hmat_upload(ctx.h, n_rows, n_cols, len(leaves), leaves.ctypes.data_as(_p),
                                      data.ctypes.data_as(_p), ddt, storage, int(keep_leaves),
                                      C.byref(h)), "ahm_hmat_upload")
-        return cls(ctx, h, n_rows, n_cols, storage)
+        H = cls(ctx, h, n_rows, n_cols, storage)
+        H.leaf_dtype = ddt
+        return H
 
     @classmethod
     def assemble(cls, ctx: Context, tree: "Tree", kernel: int = KERNEL_LAPLACE, kappa: float = 0.0,
@@ -431,15 +440,21 @@ class HMatrix:
         leaves["b1"], leaves["b2"] = lv["b1"], lv["b2"]
         return cls.upload(ctx, tree.N, tree.N, leaves, data, storage, keep_leaves), tree
 
-    def download(self):
-        st = self.stats()
-        leaves = np.zeros(st["n_leaves"], LEAF_DTYPE)
+    def leaves(self) -> np.ndarray:
+        """Leaf descriptors (idx order; rank -1 = dense), without the values."""
+        leaves = np.zeros(self.stats()["n_leaves"], LEAF_DTYPE)
         check(load().ahm_hmat_download(self.h, leaves.ctypes.data_as(_p), None), "download")
-        nv = 0
-        for d in leaves:
-            nv = max(nv, int(d["offset"]) + (int(d["n1"]) * int(d["n2"]) if d["rank"] < 0
-                                             else int(d["rank"]) * (int(d["n1"]) + int(d["n2"]))))
-        data = np.zeros(nv, np.float64)
+        return leaves
+
+    def download(self):
+        leaves = self.leaves()
+        n1, n2 = leaves["n1"].astype(np.int64), leaves["n2"].astype(np.int64)
+        k = leaves["rank"].astype(np.int64)
+        nv = np.where(k < 0, n1 * n2, k * (n1 + n2)) + leaves["offset"].astype(np.int64)
+        # kept values are in the element type they were uploaded in (FP64 for
+        # matrices assembled on the GPU)
+        dt = np.float32 if getattr(self, "leaf_dtype", F64) == F32 else np.float64
+        data = np.empty(int(nv.max()) if len(nv) else 0, dt)
         check(load().ahm_hmat_download(self.h, None, data.ctypes.data_as(_p)), "download")
         return leaves, data
 

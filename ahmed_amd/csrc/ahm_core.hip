@@ -201,6 +201,12 @@ int ahm_ctx_release_cached(ahm_ctx c) {
   return AHM_OK;
 }
 
+int ahm_ctx_cached_bytes(ahm_ctx c, uint64_t* bytes) {
+  if (!c || !bytes) return ahm::fail(AHM_ERR_INVALID, "ahm_ctx_cached_bytes: NULL argument");
+  *bytes = ahm::cache_bytes(c->device);
+  return AHM_OK;
+}
+
 int ahm_ctx_set_stream(ahm_ctx c, void* s) {
   if (!c) return ahm::fail(AHM_ERR_INVALID, "ahm_ctx_set_stream: NULL context");
   c->stream = (hipStream_t)s;  // NULL: HIP's default (null) stream

@@ -926,14 +926,24 @@ static uint32_t rhs_slots(uint32_t p) {
   return p == 1 ? 1u : ((p + kMmRhs - 1) / kMmRhs) * rhs_width(p);
 }
 
+static size_t ws_t_bytes(ahm_hmat h, uint32_t p, size_t esize) {
+  return (size_t)h->t_len * rhs_slots(p) * esize;
+}
+static size_t ws_part_bytes(ahm_hmat h, uint32_t p, size_t esize) {
+  return (size_t)h->part_len * p * esize;
+}
+static size_t ws_x_bytes(ahm_hmat h, uint32_t p, size_t esize) {
+  return p == 1 ? 0 : (size_t)h->n_cols * rhs_slots(p) * esize;
+}
+
 static int ensure_tbuf(ahm_hmat h, uint32_t p, size_t esize) {
-  const size_t need = (size_t)h->t_len * rhs_slots(p) * esize;
+  const size_t need = ws_t_bytes(h, p, esize);
   if (need > h->tbuf_cap) {
     AHM_HIP(hipStreamSynchronize(h->ctx->stream));
     AHM_TRY(h->tbuf.alloc(need));
     h->tbuf_cap = need;
   }
-  const size_t xneed = p == 1 ? 0 : (size_t)h->n_cols * rhs_slots(p) * esize;
+  const size_t xneed = ws_x_bytes(h, p, esize);
   if (xneed > h->xbuf_cap) {
     AHM_HIP(hipStreamSynchronize(h->ctx->stream));
     AHM_TRY(h->xbuf.alloc(xneed));
@@ -943,7 +953,7 @@ static int ensure_tbuf(ahm_hmat h, uint32_t p, size_t esize) {
 }
 
 static int ensure_pbuf(ahm_hmat h, uint32_t p, size_t esize) {
-  const size_t need = (size_t)h->part_len * p * esize;
+  const size_t need = ws_part_bytes(h, p, esize);
   if (need <= h->pbuf_cap) return AHM_OK;
   AHM_HIP(hipStreamSynchronize(h->ctx->stream));
   AHM_TRY(h->pbuf.alloc(need));
@@ -951,12 +961,23 @@ static int ensure_pbuf(ahm_hmat h, uint32_t p, size_t esize) {
   return AHM_OK;
 }
 
+// Device workspace of one product: V^T X (t_len values per right-hand-side
+// slot), the partial sums of chunked panels (part_len per right-hand side)
+// and, for p >= 2, the interleaved copy of X (n_cols per slot).
+struct Workspace {
+  void* t;
+  void* part;
+  void* xi;
+};
+
+static Workspace own_workspace(ahm_hmat h) { return Workspace{h->tbuf.p, h->pbuf.p, h->xbuf.p}; }
+
 template <typename TS, typename TV>
-static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y, uint32_t ldy,
-                         uint32_t p, hipEvent_t ev_mid = nullptr) {
+static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X, uint32_t ldx, TV* Y,
+                         uint32_t ldy, uint32_t p, hipEvent_t ev_mid = nullptr) {
   hipStream_t s = h->ctx->stream;
-  TV* t = (TV*)h->tbuf.p;
-  TV* part = (TV*)h->pbuf.p;
+  TV* t = (TV*)ws.t;
+  TV* part = (TV*)ws.part;
   if (p == 1) {
     if (h->n_a)
       hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, false>), dim3(h->n_a, 1), dim3(kPanelThreads), 0, s,
@@ -976,7 +997,7 @@ static int launch_matvec(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y,
     // workgroup)
     const uint32_t ng = (p + kMmRhs - 1) / kMmRhs;
     const int pw = (int)rhs_width(p);
-    TV* xi = (TV*)h->xbuf.p;
+    TV* xi = (TV*)ws.xi;
     hipLaunchKernelGGL((interleave_kernel<TV>), dim3((h->n_cols + 63) / 64, ng), dim3(256), 0, s, X,
                        ldx, h->n_cols, p, pw, xi);
     if (pw == 8) {
@@ -1323,12 +1344,14 @@ int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void*
   if (h->storage == AHM_F64 && vec == AHM_F64) {
     AHM_TRY(ensure_tbuf(h, p, 8));
     AHM_TRY(ensure_pbuf(h, p, 8));
-    return launch_matvec<double, double>(h, alpha, (const double*)X, ldx, (double*)Y, ldy, p);
+    return launch_matvec<double, double>(h, own_workspace(h), alpha, (const double*)X, ldx, (double*)Y,
+                                         ldy, p);
   }
   if (h->storage == AHM_F32 && vec == AHM_F32) {
     AHM_TRY(ensure_tbuf(h, p, 4));
     AHM_TRY(ensure_pbuf(h, p, 4));
-    return launch_matvec<float, float>(h, (float)alpha, (const float*)X, ldx, (float*)Y, ldy, p);
+    return launch_matvec<float, float>(h, own_workspace(h), (float)alpha, (const float*)X, ldx, (float*)Y,
+                                       ldy, p);
   }
   return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec: storage %d with vectors %d", h->storage, vec);
 }
@@ -1355,9 +1378,11 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
   for (uint32_t r = 0; r < reps && st == AHM_OK; ++r) {
     if (hipEventRecord(e0, h->ctx->stream) != hipSuccess) { st = fail(AHM_ERR_HIP, "event"); break; }
     if (vec == AHM_F64)
-      st = launch_matvec<double, double>(h, alpha, (const double*)X, ldx, (double*)Y, ldy, p, e1);
+      st = launch_matvec<double, double>(h, own_workspace(h), alpha, (const double*)X, ldx, (double*)Y,
+                                         ldy, p, e1);
     else
-      st = launch_matvec<float, float>(h, (float)alpha, (const float*)X, ldx, (float*)Y, ldy, p, e1);
+      st = launch_matvec<float, float>(h, own_workspace(h), (float)alpha, (const float*)X, ldx, (float*)Y,
+                                       ldy, p, e1);
     if (st) break;
     float a = 0.f, b = 0.f;
     if (hipEventRecord(e2, h->ctx->stream) != hipSuccess || hipEventSynchronize(e2) != hipSuccess ||
@@ -1403,10 +1428,10 @@ int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint
     AHM_TRY(ensure_pbuf(h, p, es));
     auto run = [&]() {
       return vec == AHM_F64
-                 ? launch_matvec<double, double>(h, 1.0, (const double*)dx.p, h->n_cols, (double*)dy.p,
-                                                 h->n_rows, p)
-                 : launch_matvec<float, float>(h, 1.0f, (const float*)dx.p, h->n_cols, (float*)dy.p,
-                                               h->n_rows, p);
+                 ? launch_matvec<double, double>(h, own_workspace(h), 1.0, (const double*)dx.p, h->n_cols,
+                                                 (double*)dy.p, h->n_rows, p)
+                 : launch_matvec<float, float>(h, own_workspace(h), 1.0f, (const float*)dx.p, h->n_cols,
+                                               (float*)dy.p, h->n_rows, p);
     };
     if ((st = run())) break;  // warm-up
     AHM_HIP(hipEventRecord(e0, s));

@@ -197,6 +197,30 @@ extern "C" void ahm_surface_invalidate(void) {
   ++g_epoch;
 }
 
+// Drops every cached device copy made from the leaf array A (any tree): the
+// array is about to be freed, and a later array at the same address must not
+// find its handles.  Called by freembls.
+extern "C" void ahm_surface_forget(const void* A) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  for (auto it = g_cache.begin(); it != g_cache.end();) {
+    if (it->first.second == A) {
+      if (it->second.h) ahm_hmat_destroy(it->second.h);
+      it = g_cache.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  for (auto& m : g_tri)
+    for (auto it = m.begin(); it != m.end();) {
+      if (it->first.second == A) {
+        if (it->second.h) ahm_trih_destroy(it->second.h);
+        it = m.erase(it);
+      } else {
+        ++it;
+      }
+    }
+}
+
 void LtHGeM_solve(blcluster* blL, mblock<double>** L, unsigned p, double* B, unsigned ldB) {
   tri_solve(blL, L, 'L', p, B, ldB, "LtHGeM_solve");
 }

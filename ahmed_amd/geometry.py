@@ -36,3 +36,26 @@ def cube_surface(n: int, seed: int = 0) -> np.ndarray:
 def ellipsoid(n: int, axes, seed: int = 0) -> np.ndarray:
     """N points on the ellipsoid with semi-axes ``axes`` (sphere points scaled)."""
     return sphere(n, seed) * np.asarray(axes, np.float64)[None, :]
+
+
+def ellipsoid_area(axes) -> float:
+    """Surface area of an ellipsoid (Knud Thomsen's approximation, p = 1.6075)."""
+    a, b, c = (float(v) for v in axes)
+    p = 1.6075
+    return 4 * np.pi * (((a * b) ** p + (a * c) ** p + (b * c) ** p) / 3) ** (1 / p)
+
+
+def uneven_batch(n_mat: int = 64, seed: int = 2024, n_min: int = 4096, n_max: int = 65536):
+    """The batch of independent problems: per system N log-uniform in
+    [n_min, n_max], an ellipsoid with semi-axes (1, a, b), a, b ~ U(1, 4)
+    (points seeded 1000 + s), and kappa ~ U(0, 5).  Returns a list of
+    (points, kappa, quadrature weight = area / N)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_mat):
+        N = int(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        a, b = rng.uniform(1, 4, 2)
+        kappa = float(rng.uniform(0, 5))
+        axes = (1.0, a, b)
+        out.append((ellipsoid(N, axes, seed=1000 + s), kappa, ellipsoid_area(axes) / N))
+    return out

@@ -61,13 +61,42 @@ def timed(fn, torch, steps: int, warmup: int):
     return e0.elapsed_time(e1) / steps  # ms per step
 
 
-def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, budget_s=20.0, gmres=None):
-    """The reference's compiled mltaGeHVec (oracle/_ref) on this host's cores,
-    on the identical leaves; falls back to the oracle restatement.  gmres =
-    (b, tol, m): also time one serial GMRes solve of H x = b (x0 = 0)."""
+def host_cpu_info() -> dict:
+    """CPUs of this host: online count, the ones this process may run on, the
+    share the baseline uses, and the model name from /proc/cpuinfo."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    # one GPU's share of the host: the box runs at most 16 worker threads per
+    # GPU (OMP_NUM_THREADS is set to that there)
+    share = min(affinity, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
+    return {"online": os.cpu_count(), "affinity": affinity, "used": share, "model": model}
+
+
+def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, y_gpu_of, budget_s=20.0, gmres=None, assembly=None):
+    """The reference's compiled code (oracle/_ref) on this host's cores, on the
+    identical leaves; falls back to the oracle restatement.
+
+    y_gpu_of(x) -> the GPU product of host vector x (compared with the
+    reference's product of the same x).  gmres = (b, tol, m): also time one
+    GMRes solve of H x = b (x0 = 0), serial and with the leaf-parallel
+    operator on all used cores.  assembly = (kernel, kappa, weight, eps,
+    stride, gpu_ranks): time the reference's OpenMP leaf loop
+    (matgenGeH_omp) on every stride-th leaf on all used cores and compare the
+    final ranks of those leaves with the GPU's."""
     import oracle as O
+    cpu = host_cpu_info()
+    nthr = cpu["used"]
     x = np.random.default_rng(7).standard_normal(len(pts))
     out = {}
+    R = None
     try:
         R = O.Reference()
         T = R.tree(pts, bmin, eta2, O.TREE_BBX)
@@ -78,6 +107,7 @@ def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, budget_s=20.0, gmres=None
         kind = "reference"
     except (FileNotFoundError, OSError, RuntimeError) as e:
         log(f"reference library unavailable ({e}); timing the oracle restatement")
+        R = None
         o = O.Oracle()
         H = o.from_leaves(len(pts), leaves["b1"], leaves["n1"], leaves["b2"], leaves["n2"], leaves["rank"],
                           leaves["offset"], data)
@@ -90,9 +120,31 @@ def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, budget_s=20.0, gmres=None
         y = H.matvec(x)
         ts.append(time.perf_counter() - t0)
     t = float(np.median(ts))
+    err = float(np.max(np.abs(y_gpu_of(x) - y)) / np.max(np.abs(y)))
+    log(f"GPU H-matvec vs the {kind}'s on the same x: rel.err {err:.2e}")
     out.update(value=round(B_alg / t / 1e9, 3), unit="GB/s", cores=1, kind=kind,
                sample=f"{len(ts)} serial mltaGeHVec calls (median {t:.3f} s each) on the identical "
-                      f"H-matrix, N={len(pts)}")
+                      f"H-matrix, N={len(pts)}",
+               rel_err_vs_gpu=err, host_cpus=cpu)
+    # all used cores: the reference's leaf products, leaf-parallel with a
+    # private y per thread (its own mltaGeHVec_omp shares y between threads)
+    try:
+        if R is not None:
+            par = lambda: R.matvec_omp(H, x, nthreads=nthr)   # noqa: E731
+        else:
+            par = lambda: o.matvec_omp(H, x, nthreads=nthr)   # noqa: E731
+        par()
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 3 and time.perf_counter() - t0 < budget_s:
+            yo = par()
+            reps += 1
+        to = (time.perf_counter() - t0) / reps
+        out["all_cores"] = {"value": round(B_alg / to / 1e9, 3), "unit": "GB/s", "cores": nthr, "kind": kind,
+                            "ms": round(to * 1e3, 2),
+                            "rel_diff_vs_serial": float(np.max(np.abs(yo - y)) / np.max(np.abs(y)))}
+    except Exception as e:  # report, never hide
+        out["all_cores"] = {"error": str(e)}
     if gmres is not None:
         b, tol, m = gmres
         t0 = time.perf_counter()
@@ -101,24 +153,35 @@ def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, budget_s=20.0, gmres=None
         out["gmres"] = {"seconds": round(tg, 3), "rc": rc, "residual": res, "iterations": steps, "cores": 1,
                         "kind": kind, "sample": f"one serial GMRes({m}) solve to {tol:g} on the identical H-matrix"}
         log(f"CPU GMRes({m}): {tg:.2f} s, {steps} iterations, residual {res:.2e}")
-    # all-core leaf-parallel restatement (per-thread y), for reference
-    try:
-        o = O.Oracle()
-        Ho = o.from_leaves(len(pts), leaves["b1"], leaves["n1"], leaves["b2"], leaves["n2"], leaves["rank"],
-                           leaves["offset"], data)
-        ncore = len(os.sched_getaffinity(0))
-        nthr = min(ncore, 16)
-        o.matvec_omp(Ho, x, nthreads=nthr)
-        t0 = time.perf_counter()
-        reps = 0
-        while reps < 3 and time.perf_counter() - t0 < budget_s:
-            yo = o.matvec_omp(Ho, x, nthreads=nthr)
-            reps += 1
-        to = (time.perf_counter() - t0) / reps
-        out["all_cores"] = {"value": round(B_alg / to / 1e9, 3), "unit": "GB/s", "cores": nthr,
-                            "kind": "port", "rel_diff_vs_serial": float(np.max(np.abs(yo - y)) / np.max(np.abs(y)))}
-    except Exception as e:  # report, never hide
-        out["all_cores"] = {"error": str(e)}
+        if R is not None:
+            t0 = time.perf_counter()
+            xs2, rc2, res2, steps2 = R.gmres_omp(H, b, eps=tol, m=m, nsteps=2000, nthreads=nthr)
+            tg2 = time.perf_counter() - t0
+            out["gmres_all_cores"] = {"seconds": round(tg2, 3), "rc": rc2, "iterations": steps2, "cores": nthr,
+                                      "kind": kind, "sample": f"the reference's GMRes({m}) to {tol:g} with the "
+                                                              "leaf-parallel operator"}
+            log(f"CPU GMRes({m}) on {nthr} cores: {tg2:.2f} s, {steps2} iterations")
+    del H
+    if assembly is not None and R is not None:
+        kern, kappa, weight, eps, stride, gpu_ranks = assembly
+        HR, sec, sampled = R.assemble_omp(T, kern, kappa, eps, weight=weight, stride=stride, nthreads=nthr)
+        kr = HR.layout()["rank"]
+        del HR
+        n1 = leaves["n1"].astype(np.float64)
+        n2 = leaves["n2"].astype(np.float64)
+        share = float((n1 * n2)[sampled].sum() / (n1 * n2).sum())
+        adm = kr >= 0
+        same = float(np.mean(kr[sampled] == gpu_ranks[sampled]))
+        out["assembly"] = {"seconds_sample": round(sec, 3), "cores": nthr, "kind": kind, "stride": stride,
+                           "leaves_sampled": int(sampled.sum()), "block_area_share": round(share, 4),
+                           "extrapolated_s": round(sec / share, 2),
+                           "final_ranks_equal_to_gpu": round(same, 4),
+                           "sample": f"matgenGeH_omp's OpenMP leaf loop (apprx + ACA + cpyLrM_cmpr) on every "
+                                     f"{stride}-th leaf of the leaf list, {nthr} threads; extrapolated by the "
+                                     f"sampled share of the block area"}
+        log(f"CPU assembly sample: {sec:.2f} s for {int(sampled.sum())} leaves ({share:.1%} of the block area, "
+            f"{nthr} threads) -> {sec / share:.1f} s extrapolated; ranks equal to the GPU's on {same:.2%}")
+        del adm
     return out
 
 
@@ -129,19 +192,11 @@ def bench_batched(args):
     phase for every product, batched GMRES(50) with hipGraph-captured steps."""
     import torch
     from ahmed_amd import _lib, geometry
-    rng = np.random.default_rng(2024)
-    n_mat = 64
-    trees, kernels, sizes = [], [], []
-    for s in range(n_mat):
-        N = int(np.exp(rng.uniform(np.log(4096), np.log(65536))))
-        a, b = rng.uniform(1, 4, 2)
-        kappa = rng.uniform(0, 5)
-        pts = geometry.ellipsoid(N, (1.0, a, b), seed=1000 + s)
-        p = 1.6075
-        area = 4 * np.pi * ((a ** p + b ** p + (a * b) ** p) / 3) ** (1 / p)
-        trees.append(_lib.Tree(pts, 64, 0.64))
-        kernels.append((_lib.KERNEL_YUKAWA, kappa, area / N))
-        sizes.append(N)
+    probs = geometry.uneven_batch()
+    n_mat = len(probs)
+    trees = [_lib.Tree(pts, 64, 0.64) for pts, _, _ in probs]
+    kernels = [(_lib.KERNEL_YUKAWA, kappa, w) for _, kappa, w in probs]
+    sizes = [len(pts) for pts, _, _ in probs]
     ctx = _lib.Context(0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -476,6 +531,9 @@ def main():
             "mean_rank": astr["sum_rank"] / max(1, astr["n_lowrank"]), "nvals": str_["nvals"],
             "n_lowrank": str_["n_lowrank"]}
     del Hr
+    # the assemblies' scratch and the ACAr matrix sit in the allocation cache:
+    # hand that memory back before the product and solver phases
+    ctx.release_cached()
     log(f"GPU assembly with ACAr {t_acar:.2f} s (ACA batches {astr['aca_ms']:.0f} ms vs {ast['aca_ms']:.0f} ms), "
         f"nvals {str_['nvals']:.4g}")
 
@@ -546,6 +604,13 @@ def main():
         log(f"GMRES(50) {mode}: {dt:.3f} s, residual {true:.2e}, {info}")
     gm["tolerance"] = tol
     gm["speedup_mixed_vs_fp64"] = round(gm["fp64"]["seconds"] / gm["mixed"]["seconds"], 3)
+    # what the products alone allow: the FP64 solve's products against the
+    # refinement's FP64 residual products plus its FP32 inner products
+    n64 = gm["fp64"]["matvecs"]
+    nres = gm["mixed"]["refinements"]
+    nin = gm["mixed"]["inner_iterations"]
+    gm["speedup_bound_from_products"] = round(n64 * ms / (nres * ms + nin * ms32), 3)
+    gm["target_speedup"] = 3.0
 
     sr = stream_read_gbs(torch)
     log(f"streaming read (torch sum over 4 GiB): {sr:.1f} GB/s")
@@ -563,10 +628,19 @@ def main():
     if not args.no_cpu:
         leaves, data = H.download()
         log(f"CPU baseline on {len(leaves)} leaves ({data.nbytes / 1e9:.1f} GB)")
-        cpu = cpu_baseline(pts, leaves, data, bmin, eta * eta, B_alg, gmres=(b.cpu().numpy(), tol, 50))
+
+        def y_gpu_of(xh):
+            yd = torch.zeros(N, dtype=torch.float64, device="cuda")
+            H.matvec(torch.as_tensor(xh, device="cuda"), yd)
+            return yd.cpu().numpy()
+        stride = 1 if N <= 262144 else 8
+        cpu = cpu_baseline(pts, leaves, data, bmin, eta * eta, B_alg, y_gpu_of, gmres=(b.cpu().numpy(), tol, 50),
+                           assembly=(_lib.KERNEL_LAPLACE, 0.0, weight, eps, stride, leaves["rank"]))
         if "gmres" in cpu:
             gm["speedup_mixed_vs_cpu_serial"] = round(cpu["gmres"]["seconds"] / gm["mixed"]["seconds"], 1)
             gm["speedup_fp64_vs_cpu_serial"] = round(cpu["gmres"]["seconds"] / gm["fp64"]["seconds"], 1)
+        if "gmres_all_cores" in cpu:
+            gm["speedup_mixed_vs_cpu_all_cores"] = round(cpu["gmres_all_cores"]["seconds"] / gm["mixed"]["seconds"], 1)
         log(f"CPU: {cpu}")
         del data
 
@@ -597,6 +671,10 @@ def main():
                      "bytes_per_launch": bytes_b, "ms_per_launch": round(ms_b, 4),
                      "phase_a": {"ms": round(ms_a, 4), "GB/s": round(bytes_a / (ms_a * 1e-3) / 1e9, 2)}},
         "cpu_baseline": cpu,
+        "parity": None if cpu is None else {
+            "rel_err_gpu_vs_reference": cpu["rel_err_vs_gpu"], "bar": 1e-12,
+            "ok": bool(cpu["rel_err_vs_gpu"] <= 1e-12), "against": cpu["kind"],
+            "what": "GPU FP64 H-matvec vs the reference's mltaGeHVec on the identical H-matrix and x"},
         "matvec_multi_rhs": {"rhs": p, "ms": round(msp, 4),
                              "TF/s": round(F_alg / (msp * 1e-3) / 1e12, 3),
                              "mfma_frac_fp64": round(F_alg / (msp * 1e-3) / 1e12 / FP64_PEAK_TFS, 4),

@@ -98,6 +98,8 @@ int ahm_ctx_sync(ahm_ctx ctx);
  * device is destroyed, and by this call (e.g. before handing the memory to
  * another allocator).  No counterpart in the reference (host memory). */
 int ahm_ctx_release_cached(ahm_ctx ctx);
+/* bytes the allocation cache holds on the context's device */
+int ahm_ctx_cached_bytes(ahm_ctx ctx, uint64_t* bytes);
 
 /* -------- trees (host): the C++ surface's cluster tree + block cluster tree
  * pts: N x 3 (original order).  Replaces the caller-side sequence
@@ -128,8 +130,9 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out);
 int ahm_hmat_destroy(ahm_hmat h);
 int ahm_hmat_stats_get(ahm_hmat h, ahm_hmat_stats* st);
 /* leaves (n_leaves descs, offsets into data) and values in mblock layout, in
- * the matrix's storage precision; needs keep_leaves.  data may be NULL to
- * query descriptors only. */
+ * the element type they were uploaded in (data_dtype; FP64 for matrices
+ * assembled on the GPU); needs keep_leaves.  data may be NULL to query
+ * descriptors only (any matrix). */
 int ahm_hmat_download(ahm_hmat h, ahm_leaf_desc* leaves, void* data);
 
 /* -------- on-disk H-matrix: the reference's savembls / loadmbls format -----

@@ -23,6 +23,7 @@ void allocmbls(blcluster* bl, mblock<T>**& A) {
 template <class T>
 void freembls(unsigned n, mblock<T>**& A) {
   if (A) {
+    ahm_surface_forget(A);
     for (unsigned i = 0; i < n; ++i) delete A[i];
     delete[] A;
     A = nullptr;

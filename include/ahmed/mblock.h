@@ -17,6 +17,8 @@
 #include <iostream>
 
 extern "C" void ahm_surface_invalidate(void);
+// releases the device copies the surface keeps of the leaf array A
+extern "C" void ahm_surface_forget(const void* A);
 
 template <class T>
 class mblock {

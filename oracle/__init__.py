@@ -421,12 +421,18 @@ class Reference:
         L.ref_hmat_matmat.argtypes = [_p, _d, _u, _p, _u, _p, _u]
         L.ref_gmres.restype = _u
         L.ref_gmres.argtypes = [_p, _p, _p, C.POINTER(_d), _u, C.POINTER(_u)]
+        L.ref_hmat_matvec_omp.restype = C.c_int
+        L.ref_hmat_matvec_omp.argtypes = [_p, _d, _p, _p, C.c_int]
+        L.ref_gmres_omp.restype = _u
+        L.ref_gmres_omp.argtypes = [_p, _p, _p, C.POINTER(_d), _u, C.POINTER(_u), C.c_int]
         L.ref_cg.restype = _u
         L.ref_cg.argtypes = [_p, _p, _p, C.POINTER(_d), C.POINTER(_u)]
         L.ref_aca_dense.restype = C.c_int
         L.ref_aca_dense.argtypes = [_p, _u, _u, _d, _u, _u, _d, C.POINTER(_u), _p, _p]
         L.ref_acar_dense.restype = C.c_int
         L.ref_acar_dense.argtypes = [_p, _u, _u, _d, _u, _u, _d, C.POINTER(_u), _p, _p]
+        L.ref_hmat_assemble_omp.restype = _p
+        L.ref_hmat_assemble_omp.argtypes = [_p, C.c_int, _d, _d, _d, _u, _u, C.c_int, C.POINTER(_d), _p]
         L.ref_hmat_assemble_acar.restype = _p
         L.ref_hmat_assemble_acar.argtypes = [_p, C.c_int, _d, _d, _d, _u]
         L.ref_addtrll.argtypes = [_u, _u, _u, _p, _p, _d, _u, C.POINTER(_u), _p, _p]
@@ -466,6 +472,35 @@ class Reference:
         f = self.lib.ref_hmat_assemble_acar if rows_only else self.lib.ref_hmat_assemble
         h = f(tree.h, kernel, _d(kappa), _d(weight), _d(eps), _u(rankmax))
         return _HMat(self, h, tree.N, tree)
+
+    def assemble_omp(self, tree, kernel=KERNEL_LAPLACE, kappa=0.0, eps=1e-6, weight=1.0, stride=1, nthreads=0,
+                     rankmax=1 << 30):
+        """matgenGeH_omp's OpenMP leaf loop (Include/matgen_omp.h:59-75) over
+        every stride-th leaf of the leaf list: (H, seconds of the loop, sampled
+        mask by leaf idx).  Leaves outside the sample are rank 0."""
+        nthreads = nthreads or len(os.sched_getaffinity(0))
+        sec = _d(0.0)
+        mask = np.zeros(self.lib.ref_tree_nleaves(tree.h), np.uint8)
+        h = self.lib.ref_hmat_assemble_omp(tree.h, kernel, _d(kappa), _d(weight), _d(eps), _u(rankmax),
+                                           _u(stride), nthreads, C.byref(sec), _ptr(mask))
+        return _HMat(self, h, tree.N, tree), sec.value, mask.astype(bool)
+
+    def matvec_omp(self, H, x, d=1.0, nthreads=0):
+        """y = d A x with the reference's leaf products on nthreads cores
+        (private y per thread, summed)."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(H.N)
+        self.lib.ref_hmat_matvec_omp(H.h, _d(d), _ptr(x), _ptr(y), nthreads or len(os.sched_getaffinity(0)))
+        return y
+
+    def gmres_omp(self, H, b, x0=None, eps=1e-8, m=30, nsteps=10000, nthreads=0):
+        """The reference's GMRes with the leaf-parallel operator on nthreads cores."""
+        b = np.ascontiguousarray(b, np.float64).copy()
+        x = np.zeros(H.N) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+        e, ns = _d(eps), _u(nsteps)
+        rc = self.lib.ref_gmres_omp(H.h, _ptr(b), _ptr(x), C.byref(e), _u(m), C.byref(ns),
+                                    nthreads or len(os.sched_getaffinity(0)))
+        return x, int(rc), e.value, ns.value
 
     def from_data(self, tree, rank, offset, data):
         """The reference's mblock array filled with given leaf values (idx order)."""

@@ -16,6 +16,8 @@
 #include <sstream>
 #include <vector>
 
+#include <omp.h>
+
 #include "H.h"
 #include "apprx.h"
 #include "bemblcluster.h"
@@ -167,6 +169,54 @@ void apprx_bbx(KernelGen& G, mblock<double>*& mbl, blcluster* bl, unsigned i0, d
   }
 }
 
+// Leaf-parallel operator on all host cores for the CPU baseline: the leaves
+// in idx order are cut into nthreads contiguous intervals of equal stored-
+// value count; each thread applies the reference's own leaf product
+// (mblock::mltaVec, H/mblock.cpp:321-323) into a private y, and the private
+// vectors are summed.  The reference's mltaGeHVec_omp writes one shared y
+// from every thread (H/mltaGeHVec_omp.cpp:23-36), which is a race.
+struct OmpGeH : public Matrix<double> {
+  const Tree* t;
+  mblock<double>** A;
+  int nthreads;
+  std::vector<unsigned> cut;
+  mutable std::vector<double> ybuf;
+  OmpGeH(const Tree* tt, mblock<double>** AA, int nt)
+      : Matrix<double>(tt->N, tt->N), t(tt), A(AA), nthreads(nt > 0 ? nt : 1) {
+    size_t total = 0;
+    for (unsigned l = 0; l < t->nblcks; ++l) total += A[l]->nvals();
+    cut.assign(nthreads + 1, t->nblcks);
+    cut[0] = 0;
+    size_t acc = 0;
+    unsigned l = 0;
+    for (int k = 1; k < nthreads; ++k) {
+      const size_t target = total / nthreads * (size_t)k;
+      while (l < t->nblcks && acc < target) acc += A[l++]->nvals();
+      cut[k] = l;
+    }
+    ybuf.resize((size_t)nthreads * t->N);
+  }
+  void amux(double d, double* x, double* y) const {
+    const unsigned N = t->N;
+#pragma omp parallel num_threads(nthreads)
+    {
+      const int k = omp_get_thread_num();
+      double* yk = ybuf.data() + (size_t)k * N;
+      std::memset(yk, 0, sizeof(double) * N);
+      for (unsigned l = cut[k]; l < cut[k + 1]; ++l) {
+        const blcluster* b = t->leaves[l];
+        A[l]->mltaVec(d, x + b->getb2(), yk + b->getb1());
+      }
+    }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long i = 0; i < (long)N; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < nthreads; ++k) s += ybuf[(size_t)k * N + i];
+      y[i] += s;
+    }
+  }
+};
+
 void init() {
   static bool done = false;
   if (!done) { MKL_CBWR_Set(2); done = true; }
@@ -261,6 +311,52 @@ void* ref_hmat_assemble(void* tp, int kernel, double kappa, double weight, doubl
     for (unsigned i = 0; i < t->nblcks; ++i)
       apprx_bbx(G, h->A[i], t->leaves[i], t->i0[i], eps, rankmax);
   }
+  return h;
+}
+
+// matgenGeH_omp's leaf loop (Include/matgen_omp.h:59-75: an OpenMP dynamic
+// schedule over the gen_BlSequence leaf list, one apprx call per leaf, the
+// MATGEN_T object shared by all threads) run on every stride-th entry of that
+// list with nthreads threads.  bbx trees take apprx_bbx (the start row their
+// clusters lack); PCA trees apprx_unsym itself.  Leaves outside the sample
+// stay rank 0 and are marked 0 in sampled[] (by idx).  *seconds: wall time of
+// the loop.  stride 1 is the whole assembly.
+void* ref_hmat_assemble_omp(void* tp, int kernel, double kappa, double weight, double eps,
+                            unsigned rankmax, unsigned stride, int nthreads, double* seconds,
+                            unsigned char* sampled) {
+  init();
+  Tree* t = (Tree*)tp;
+  HMat* h = new HMat;
+  h->t = t;
+  allocmbls(t->bl, h->A);
+  KernelGen G = {kernel, kappa, weight, t->pts.data(), t->op.data()};
+  blcluster** list;
+  gen_BlSequence(t->bl, list);
+  if (stride == 0) stride = 1;
+  const int n = (int)t->nblcks;
+  for (int i = 0; i < n; ++i) sampled[i] = 0;
+  for (int i = 0; i < n; i += (int)stride) sampled[list[i]->getidx()] = 1;
+  Silence quiet;
+  const double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+  for (int i = 0; i < n; ++i) {
+    blcluster* b = list[i];
+    const unsigned idx = b->getidx();
+    if (i % (int)stride) continue;
+    if (t->kind == 1)
+      apprx_unsym(G, h->A[idx], (bemblcluster<Point, Point>*)b, eps, rankmax);
+    else
+      apprx_bbx(G, h->A[idx], b, t->i0[idx], eps, rankmax);
+  }
+  *seconds = omp_get_wtime() - t0;
+  for (int i = 0; i < n; ++i) {
+    const unsigned idx = list[i]->getidx();
+    if (!h->A[idx]) {
+      h->A[idx] = new mblock<double>(list[i]->getn1(), list[i]->getn2());
+      h->A[idx]->setrank(0);
+    }
+  }
+  delete[] list;
   return h;
 }
 
@@ -360,6 +456,28 @@ unsigned ref_gmres(void* p, double* b, double* x, double* eps, unsigned m, unsig
   unsigned ns = *nsteps;
   unsigned r = GMRes(M, b, x, e, m, ns);
   M.blcks = NULL;  // the H-matrix stays owned by h
+  *eps = e;
+  *nsteps = ns;
+  return r;
+}
+
+// y += d A x on nthreads host cores (OmpGeH)
+int ref_hmat_matvec_omp(void* p, double d, double* x, double* y, int nthreads) {
+  HMat* h = (HMat*)p;
+  OmpGeH M(h->t, h->A, nthreads);
+  M.amux(d, x, y);
+  return 1;
+}
+
+// the reference's GMRes (solvers/GMRES.cpp:80-181) on the leaf-parallel
+// operator: products on nthreads cores, the solver's vector work serial
+unsigned ref_gmres_omp(void* p, double* b, double* x, double* eps, unsigned m, unsigned* nsteps,
+                       int nthreads) {
+  HMat* h = (HMat*)p;
+  OmpGeH M(h->t, h->A, nthreads);
+  double e = *eps;
+  unsigned ns = *nsteps;
+  unsigned r = GMRes(M, b, x, e, m, ns);
   *eps = e;
   *nsteps = ns;
   return r;

@@ -85,27 +85,32 @@ static int run(bool pca, unsigned N) {
   }
   // GMRes and CG through the operator interface
   std::fprintf(stderr, "[drop_in] matvec checked\n");
-  GeHMatrix<double> M(N, N, bl);
-  M.blcks = A;
-  std::vector<double> ones(N, 1.0), b(N, 0.0), sol(N, 0.0);
-  M.amux(1.0, ones.data(), b.data());
-  double eps = 1e-10;
-  unsigned steps = 500;
-  unsigned rc = GMRes(M, b.data(), sol.data(), eps, 50, steps);
-  double xerr = 0.0;
-  for (unsigned i = 0; i < N; ++i) xerr = std::fmax(xerr, std::fabs(sol[i] - 1.0));
-  if (rc != 0 || xerr > 1e-8) {
-    std::printf("GMRes rc=%u eps=%.3e steps=%u err=%.3e\n", rc, eps, steps, xerr);
-    return 1;
+  // GeHMatrix frees its leaves through the tree (freembls walks bl), so it
+  // must be destroyed before the tree: the reference's order rule
+  // (Include/matrix.h:351).  The inner scope ends M's life first.
+  {
+    GeHMatrix<double> M(N, N, bl);
+    M.blcks = A;
+    std::vector<double> ones(N, 1.0), b(N, 0.0), sol(N, 0.0);
+    M.amux(1.0, ones.data(), b.data());
+    double eps = 1e-10;
+    unsigned steps = 500;
+    unsigned rc = GMRes(M, b.data(), sol.data(), eps, 50, steps);
+    double xerr = 0.0;
+    for (unsigned i = 0; i < N; ++i) xerr = std::fmax(xerr, std::fabs(sol[i] - 1.0));
+    if (rc != 0 || xerr > 1e-8) {
+      std::printf("GMRes rc=%u eps=%.3e steps=%u err=%.3e\n", rc, eps, steps, xerr);
+      return 1;
+    }
+    std::vector<double> sol2(N, 0.0);
+    double eps2 = 1e-10;
+    unsigned steps2 = 500;
+    unsigned rc2 = CG(M, b.data(), sol2.data(), eps2, steps2);
+    std::printf("drop_in %s N=%u leaves=%u size=%.1f MB max_rank=%u matvec_err=%.2e "
+                "GMRes(%u steps, %.1e) CG(rc=%u, %u steps, %.1e)\n",
+                pca ? "bemcluster" : "cluster3d_bbx", N, nblcks, sizeH(bl, A) / 1e6, Hmax_rank(bl, A),
+                err / ref, steps, eps, rc2, steps2, eps2);
   }
-  std::vector<double> sol2(N, 0.0);
-  double eps2 = 1e-10;
-  unsigned steps2 = 500;
-  unsigned rc2 = CG(M, b.data(), sol2.data(), eps2, steps2);
-  std::printf("drop_in %s N=%u leaves=%u size=%.1f MB max_rank=%u matvec_err=%.2e "
-              "GMRes(%u steps, %.1e) CG(rc=%u, %u steps, %.1e)\n",
-              pca ? "bemcluster" : "cluster3d_bbx", N, nblcks, sizeH(bl, A) / 1e6, Hmax_rank(bl, A),
-              err / ref, steps, eps, rc2, steps2, eps2);
   delete bl;
   delete root;
   return 0;

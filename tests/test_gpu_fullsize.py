@@ -1,0 +1,276 @@
+"""Parity at the full workload size of every configuration: the H-matrix
+assembled on the GPU is downloaded and handed, leaf for leaf, to the
+reference's own CPU code (compiled from its sources into oracle/_ref; the
+oracle restatement where that library is absent), and the GPU products and
+solvers are compared with the reference's mltaGeHVec / mltaGeHGeM / GMRes
+(H/mltaGeHVec.cpp:567-577, solvers/GMRES.cpp:80-181) on that identical matrix.
+
+Bars: FP64 products within 1e-12 relative (max-norm); GMRES converges on both
+sides with step counts within one and solutions within 1e-8 relative (both
+stop on the implicit residual, whose rounding differs between the two
+summation orders); FP32 storage within 2e-6 (1 RHS) and 5e-6 (32 RHS) of the
+FP64 product, each error printed; the refined mixed-precision solution has a
+true FP64 residual below its tolerance, measured with the reference's own
+product.  ACA ranks of the GPU assembly equal the reference's on at least 97 %
+of a leaf sample and differ by at most 2 elsewhere (the bar of the golden
+assembly tests).
+
+Each case prints its name and sizes before it touches the GPU.  Host memory:
+the N = 1M cases hold the 50 GB of leaf values twice (download and the
+reference's mblock copies)."""
+import gc
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import relerr
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def log(msg):
+    print(f"[gpu fullsize {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from ahmed_amd import _lib
+    assert torch.cuda.is_available()
+    c = _lib.Context(0)
+    yield c
+    c.release_cached()
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+def reference_twin(pts, bmin, eta2, leaves, data):
+    """(api, H, kind): the reference's mblock array holding the given leaves
+    (idx order of the tree of pts), or the oracle's if the compiled reference
+    is absent."""
+    try:
+        R = O.Reference()
+    except FileNotFoundError:
+        R = None
+    if R is not None:
+        T = R.tree(pts, bmin, eta2, O.TREE_BBX)
+        lv = T.leaves()
+        for f in ("b1", "n1", "b2", "n2"):
+            assert np.array_equal(lv[f], leaves[f]), f"reference tree differs in {f}"
+        return R, R.from_data(T, leaves["rank"], leaves["offset"], data), "reference"
+    o = O.Oracle()
+    return o, o.from_leaves(len(pts), leaves["b1"], leaves["n1"], leaves["b2"], leaves["n2"],
+                            leaves["rank"], leaves["offset"], data), "oracle"
+
+
+def gpu_products_vs_cpu(H, C, N, seed, p=32):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(N)
+    y = torch.zeros(N, dtype=torch.float64, device="cuda")
+    H.matvec(dev(x), y)
+    t0 = time.perf_counter()
+    ycpu = C.matvec(x)
+    t1 = time.perf_counter() - t0
+    e1 = relerr(y.cpu().numpy(), ycpu)
+    X = rng.standard_normal((N, p))
+    Y = torch.zeros((p, N), dtype=torch.float64, device="cuda")
+    H.matvec(dev(X.T.copy()), Y)
+    t0 = time.perf_counter()
+    Ycpu = C.matmat(X)
+    tp = time.perf_counter() - t0
+    ep = relerr(Y.cpu().numpy().T, Ycpu)
+    log(f"FP64 1 RHS rel.err {e1:.2e} (CPU {t1:.1f} s); {p} RHS rel.err {ep:.2e} (CPU {tp:.1f} s)")
+    return e1, ep, (x, ycpu, X, Ycpu)
+
+
+@pytest.mark.timeout(900)
+def test_sphere_262144_fp64_products_and_gmres_vs_reference(ctx):
+    """N = 262144 on the unit sphere, leaf 64, eta 0.8, ACA eps 1e-8: 1 and 32
+    right-hand sides, then GMRES(50) to 1e-10 with b = H * 1."""
+    from ahmed_amd import _lib, geometry
+    N, eps = 262144, 1e-8
+    log(f"sphere N={N} eps={eps}: GPU assembly")
+    pts = geometry.sphere(N, 0)
+    tree = _lib.Tree(pts, 64, 0.64)
+    H, _ = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_LAPLACE, 0.0, 4 * np.pi / N, eps, keep_leaves=True)
+    leaves, data = H.download()
+    log(f"{len(leaves)} leaves, {data.nbytes / 1e9:.1f} GB downloaded")
+    api, C, kind = reference_twin(pts, 64, 0.64, leaves, data)
+    del data
+    gc.collect()
+    log(f"CPU side: {kind}")
+    e1, e32, _ = gpu_products_vs_cpu(H, C, N, 21)
+    assert e1 <= 1e-12 and e32 <= 1e-12
+    b = C.matvec(np.ones(N))
+    xg = torch.zeros(N, dtype=torch.float64, device="cuda")
+    r = H.gmres(dev(b), xg, eps=1e-10, m=50, nsteps=2000)
+    t0 = time.perf_counter()
+    xc, rc, res, steps = C.gmres(b, eps=1e-10, m=50, nsteps=2000)
+    log(f"GMRES(50) to 1e-10: GPU {r['nsteps']} steps ({r['eps']:.3e}), {kind} {steps} steps ({res:.3e}, "
+        f"{time.perf_counter() - t0:.1f} s); solutions differ by {relerr(xg.cpu().numpy(), xc):.2e}")
+    assert r["converged"] and rc == 0
+    assert abs(int(r["nsteps"]) - int(steps)) <= 1
+    assert relerr(xg.cpu().numpy(), xc) <= 1e-8
+
+
+@pytest.mark.timeout(1200)
+def test_cube_1M_fp64_fp32_and_refinement_vs_reference(ctx):
+    """N = 1048576 on the unit-cube surface, leaf 64, eta 0.8, ACA eps 1e-5
+    (Laplace, weight 6/N): FP64 products with 1 and 32 right-hand sides vs the
+    reference; FP32 storage with its error vs the FP64 product stated; FP64
+    refinement around FP32 GMRES(50) to 1e-8 checked with the reference's
+    product; FP64 GMRES(50) against the reference's GMRes."""
+    from ahmed_amd import _lib, geometry
+    N, eps = 1048576, 1e-5
+    log(f"cube N={N} eps={eps}: GPU assembly")
+    pts = geometry.cube_surface(N, 0)
+    tree = _lib.Tree(pts, 64, 0.64)
+    H, _ = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_LAPLACE, 0.0, 6.0 / N, eps, keep_leaves=True)
+    leaves, data = H.download()
+    log(f"{len(leaves)} leaves, {data.nbytes / 1e9:.1f} GB downloaded")
+    api, C, kind = reference_twin(pts, 64, 0.64, leaves, data)
+    del data
+    gc.collect()
+    log(f"CPU side: {kind}")
+    e1, e32, (x, ycpu, X, Ycpu) = gpu_products_vs_cpu(H, C, N, 22)
+    assert e1 <= 1e-12 and e32 <= 1e-12
+    # FP32 storage (copyH, H/initH.cpp:152-168) with FP32 accumulation
+    H32 = H.convert()
+    y32 = torch.zeros(N, dtype=torch.float32, device="cuda")
+    H32.matvec(dev(x, torch.float32), y32)
+    f1 = relerr(y32.cpu().numpy().astype(np.float64), ycpu)
+    Y32 = torch.zeros((32, N), dtype=torch.float32, device="cuda")
+    H32.matvec(dev(X.T.copy(), torch.float32), Y32)
+    f32 = relerr(Y32.cpu().numpy().T.astype(np.float64), Ycpu)
+    log(f"FP32 storage vs the reference's FP64 product: 1 RHS {f1:.2e}, 32 RHS {f32:.2e}")
+    assert f1 <= 2e-6 and f32 <= 5e-6
+    del Y32, X, Ycpu
+    # b = H * 1 from the reference's product; FP64 GMRES on both sides
+    b = C.matvec(np.ones(N))
+    bn = float(np.linalg.norm(b))
+    xg = torch.zeros(N, dtype=torch.float64, device="cuda")
+    r = H.gmres(dev(b), xg, eps=1e-8, m=50, nsteps=2000)
+    t0 = time.perf_counter()
+    xc, rc, res, steps = C.gmres(b, eps=1e-8, m=50, nsteps=2000)
+    log(f"FP64 GMRES(50) to 1e-8: GPU {r['nsteps']} steps, {kind} {steps} steps "
+        f"({time.perf_counter() - t0:.1f} s); solutions differ by {relerr(xg.cpu().numpy(), xc):.2e}")
+    assert r["converged"] and rc == 0
+    assert abs(int(r["nsteps"]) - int(steps)) <= 1
+    assert relerr(xg.cpu().numpy(), xc) <= 1e-8
+    # mixed precision: its true residual, by the reference's product
+    xm = torch.zeros(N, dtype=torch.float64, device="cuda")
+    rm = H.gmres_ir(H32, dev(b), xm, eps=1e-8, m=50, inner_eps=1e-5, inner_steps=2000)
+    xmh = xm.cpu().numpy()
+    true_res = float(np.linalg.norm(C.matvec(xmh, d=-1.0, y=b)) / bn)
+    log(f"FP64 refinement around FP32 GMRES: {rm['outer']} refinements, {rm['inner']} inner steps, "
+        f"reported {rm['eps']:.2e}, true residual (reference product) {true_res:.2e}")
+    assert rm["converged"] and true_res <= 1e-8
+
+
+@pytest.mark.timeout(1200)
+def test_uneven_batch_64_matrices_vs_reference(ctx):
+    """64 independent H-matrices, N log-uniform in [4096, 65536] on seeded
+    ellipsoids, exp(-k r)/(4 pi r) with k ~ U(0, 5), ACA eps 1e-6, leaf 64:
+    one batched GPU assembly; every system's block of the batched 1- and
+    8-RHS products against the reference on its own tree; lockstep GMRES(50)
+    to 1e-8 against the reference's GMRes on every eighth system."""
+    from ahmed_amd import _lib, geometry
+    probs = geometry.uneven_batch()
+    log(f"uneven batch: {len(probs)} systems, N total {sum(len(p) for p, _, _ in probs)}")
+    trees = [_lib.Tree(p, 64, 0.64) for p, _, _ in probs]
+    kernels = [(_lib.KERNEL_YUKAWA, k, w) for _, k, w in probs]
+    HB, st = _lib.HMatrix.assemble_batch(ctx, trees, kernels, eps=1e-6, keep_leaves=True)
+    off = HB.segments()
+    assert len(off) == len(probs) + 1
+    lb, db = HB.download()
+    log(f"batched assembly: {len(lb)} leaves, {db.nbytes / 1e9:.1f} GB")
+    Nt = HB.n_rows
+    rng = np.random.default_rng(24)
+    x = rng.standard_normal(Nt)
+    X = rng.standard_normal((Nt, 8))
+    y = torch.zeros(Nt, dtype=torch.float64, device="cuda")
+    HB.matvec(dev(x), y)
+    Y = torch.zeros((8, Nt), dtype=torch.float64, device="cuda")
+    HB.matvec(dev(X.T.copy()), Y)
+    yh, Yh = y.cpu().numpy(), Y.cpu().numpy().T
+    ones = torch.ones(Nt, dtype=torch.float64, device="cuda")
+    bb = torch.zeros(Nt, dtype=torch.float64, device="cuda")
+    HB.matvec(ones, bb)
+    xg = torch.zeros(Nt, dtype=torch.float64, device="cuda")
+    rg = HB.gmres_batch(bb, xg, eps=1e-8, m=50, nsteps=2000, use_graph=True)
+    assert rg["converged"].all()
+    worst1 = worst8 = worst_x = 0.0
+    for s, (pts, kappa, w) in enumerate(probs):
+        a, e = int(off[s]), int(off[s + 1])
+        sel = (lb["b1"] >= a) & (lb["b1"] < e)
+        ls = lb[sel].copy()
+        ls["b1"] -= a
+        ls["b2"] -= a
+        _, C, kind = reference_twin(pts, 64, 0.64, ls, db)
+        worst1 = max(worst1, relerr(yh[a:e], C.matvec(x[a:e])))
+        worst8 = max(worst8, relerr(Yh[a:e], C.matmat(np.asfortranarray(X[a:e]))))
+        if s % 8 == 0:
+            xc, rc, res, steps = C.gmres(bb[a:e].cpu().numpy(), eps=1e-8, m=50, nsteps=2000)
+            dx = relerr(xg[a:e].cpu().numpy(), xc)
+            log(f"system {s} (N={e - a}, kappa={kappa:.2f}): GMRES steps GPU {rg['nsteps'][s]}, {kind} {steps}, "
+                f"solutions differ by {dx:.2e}")
+            assert rc == 0 and abs(int(rg["nsteps"][s]) - int(steps)) <= 1
+            worst_x = max(worst_x, dx)
+        del C
+    log(f"all {len(probs)} systems vs {kind}: 1 RHS {worst1:.2e}, 8 RHS {worst8:.2e}, GMRES x {worst_x:.2e}")
+    assert worst1 <= 1e-12 and worst8 <= 1e-12 and worst_x <= 1e-8
+
+
+@pytest.mark.timeout(900)
+def test_cube_1M_learned_slabs_same_matrix_and_ranks_vs_reference(ctx):
+    """N = 1048576 cube surface, exp(-2 r)/r, ACA eps 1e-6: the assembly with
+    MLP-sized rank slabs equals the one with fixed slabs (ACA ranks, stored
+    values count, bit-identical products with 1 and 8 right-hand sides), and
+    the final ranks of a sample of every 64th leaf of the reference's leaf
+    list match the reference's own assembly of those leaves."""
+    from ahmed_amd import _lib, geometry
+    from ahmed_amd.rank_mlp import leaf_features, load_params, predict_hints
+    N, eps, kappa = 1048576, 1e-6, 2.0
+    log(f"cube N={N} exp(-2r)/r eps={eps}: learned vs fixed slabs")
+    pts = geometry.cube_surface(N, 0)
+    tree = _lib.Tree(pts, 64, 0.64)
+    feats, adm, maxk = leaf_features(pts, tree, eps, kappa)
+    hint_adm, _ = predict_hints(ctx, feats, load_params(), maxk=maxk)
+    hint = np.full(tree.n_leaves, 48, np.uint32)
+    hint[adm] = hint_adm
+    H0, s0 = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_EXPR, kappa, 6.0 / N, eps, aca_ranks=True)
+    H1, s1 = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_EXPR, kappa, 6.0 / N, eps, rank_hint=hint,
+                                   aca_ranks=True)
+    under = float(np.mean(hint_adm < s0["aca_rank"][adm]))
+    log(f"under-predicted {under:.2%}, reruns {s1['n_retries']}, slabs {s1['slab_bytes'] / 1e9:.1f} GB "
+        f"vs {s0['slab_bytes'] / 1e9:.1f} GB")
+    assert np.array_equal(s0["aca_rank"], s1["aca_rank"])
+    l0, l1 = H0.leaves(), H1.leaves()
+    assert np.array_equal(l0["rank"], l1["rank"])
+    assert H0.stats()["nvals"] == H1.stats()["nvals"]
+    rng = np.random.default_rng(25)
+    for p in (1, 8):
+        X = dev(rng.standard_normal(N) if p == 1 else rng.standard_normal((p, N)))
+        Y0, Y1 = torch.zeros_like(X), torch.zeros_like(X)
+        H0.matvec(X, Y0)
+        H1.matvec(X, Y1)
+        assert torch.equal(Y0, Y1), f"products with {p} RHS differ"
+    del H1
+    try:
+        R = O.Reference()
+    except FileNotFoundError:
+        pytest.skip("compiled reference absent: rank sample not checked")
+    T = R.tree(pts, 64, 0.64, O.TREE_BBX)
+    HR, sec, sampled = R.assemble_omp(T, O.KERNEL_EXPR, kappa, eps, weight=6.0 / N, stride=64)
+    kr = HR.layout()["rank"]
+    sel = sampled & adm
+    same = float(np.mean(kr[sel] == l0["rank"][sel]))
+    worst = int(np.max(np.abs(kr[sel].astype(np.int64) - l0["rank"][sel].astype(np.int64))))
+    dense_same = bool(np.array_equal(kr[sampled & ~adm], l0["rank"][sampled & ~adm]))
+    log(f"reference assembly of {int(sampled.sum())} sampled leaves ({sec:.1f} s): final ranks equal on "
+        f"{same:.2%} of {int(sel.sum())} admissible ones, largest difference {worst}")
+    assert same >= 0.97 and worst <= 2 and dense_same

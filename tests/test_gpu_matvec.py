@@ -246,26 +246,75 @@ def test_mfma_product_every_panel_height(ctx, p):
         assert relerr(Y32.cpu().numpy().T.astype(np.float64), A @ X) <= 5e-6
 
 
+def cache_test_matrix(N, rng, poison):
+    """Leaves [0, h) x [0, N) and [h, N) x [0, h) dense, [h, N) x [h, N) of rank
+    6 (so products use the V^T x workspace as well as the arena); poison:
+    values NaN / +Inf / -Inf instead of random numbers."""
+    from ahmed_amd import _lib
+    h, k = N // 2, 6
+    sizes = [h * N, (N - h) * h, k * 2 * (N - h)]
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    leaves = _lib.leaf_array([0, h, h], [h, N - h, N - h], [0, 0, h], [N, h, N - h], [-1, -1, k], off)
+    if poison:
+        data = np.tile(np.array([np.nan, np.inf, -np.inf]), sum(sizes) // 3 + 1)[:sum(sizes)]
+        return leaves, data, None
+    data = rng.standard_normal(sum(sizes))
+    A = np.empty((N, N))
+    A[:h] = data[:h * N].reshape(N, h).T
+    A[h:, :h] = data[off[1]:off[2]].reshape(h, N - h).T
+    U = data[off[2]:off[2] + k * (N - h)].reshape(k, N - h).T
+    V = data[off[2] + k * (N - h):].reshape(k, N - h).T
+    A[h:, h:] = U @ V.T
+    return leaves, data, A
+
+
 def test_large_blocks_reused_from_the_allocation_cache(ctx):
     """Buffers of >= 1 GiB go to the allocation cache when released and are
-    handed to the next large request (best fit): matrices uploaded one after
-    the other into reused blocks multiply correctly, the second one smaller
-    than the first, and releasing the cache leaves the context usable."""
+    handed to the next large request (best fit).  A matrix of NaN / Inf
+    values is multiplied (1 and 32 right-hand sides, so every workspace holds
+    them too) and released: the cache grows by its arena.  The next, smaller
+    matrix takes that block (the cache shrinks) and its products are finite
+    and correct, so no path relies on fresh device memory being zero.
+    Releasing the cache empties it and leaves the context usable."""
     from ahmed_amd import _lib
     rng = np.random.default_rng(7)
-    for N in (12000, 11600):   # 1.15 GB and 1.08 GB of dense values
-        h = N // 2
-        print(f"[gpu] allocation cache N={N}", flush=True)
-        A = rng.standard_normal((N, N))
-        leaves = _lib.leaf_array([0, h], [h, N - h], [0, 0], [N, N], [-1, -1], [0, h * N])
-        data = np.concatenate([A[:h].T.ravel(), A[h:].T.ravel()])
-        H = _lib.HMatrix.upload(ctx, N, N, leaves, data)
-        x = rng.standard_normal(N)
-        y = torch.zeros(N, dtype=torch.float64, device="cuda")
-        H.matvec(dev(x), y)
-        assert relerr(y.cpu().numpy(), A @ x) <= 1e-13
-        del H
     ctx.release_cached()
+    assert ctx.cached_bytes() == 0
+    print("[gpu] allocation cache: poisoned matrix N=16000", flush=True)
+    leaves, data, _ = cache_test_matrix(16000, rng, poison=True)
+    H = _lib.HMatrix.upload(ctx, 16000, 16000, leaves, data)
+    for p in (1, 32):
+        X = torch.ones((p, 16000), dtype=torch.float64, device="cuda").squeeze(0)
+        Y = torch.zeros_like(X)
+        H.matvec(X, Y)
+    torch.cuda.synchronize()
+    before = ctx.cached_bytes()
+    del H
+    grown = ctx.cached_bytes()
+    print(f"[gpu] cached bytes {before} -> {grown} after release", flush=True)
+    assert grown >= before + 8 * len(data)
+    del data
+    N = 15500
+    leaves, data, A = cache_test_matrix(N, rng, poison=False)
+    H = _lib.HMatrix.upload(ctx, N, N, leaves, data)
+    after = ctx.cached_bytes()
+    print(f"[gpu] cached bytes {grown} -> {after} after the second upload", flush=True)
+    assert after < grown
+    x = rng.standard_normal(N)
+    y = torch.zeros(N, dtype=torch.float64, device="cuda")
+    H.matvec(dev(x), y)
+    yh = y.cpu().numpy()
+    assert np.all(np.isfinite(yh))
+    assert relerr(yh, A @ x) <= 1e-13
+    X = rng.standard_normal((N, 32))
+    Y = torch.zeros((32, N), dtype=torch.float64, device="cuda")
+    H.matvec(dev(X.T.copy()), Y)
+    Yh = Y.cpu().numpy().T
+    assert np.all(np.isfinite(Yh))
+    assert relerr(Yh, A @ X) <= 1e-12
+    del H
+    ctx.release_cached()
+    assert ctx.cached_bytes() == 0
     H = _lib.HMatrix.upload(ctx, 64, 64, _lib.leaf_array([0], [64], [0], [64], [-1], [0]),
                             np.arange(64 * 64, dtype=np.float64))
     y = torch.zeros(64, dtype=torch.float64, device="cuda")
