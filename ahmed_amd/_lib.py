@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -174,6 +175,9 @@ class Context:
         h = _p()
         check(lib.ahm_ctx_create(device, C.byref(h)), "ahm_ctx_create")
         self.h = h
+        # handles made on this context: closed before the context itself,
+        # since every handle refers to it
+        self._children = weakref.WeakSet()
 
     def bind_stream(self):
         torch = _torch()
@@ -195,6 +199,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for c in list(getattr(self, "_children", ())):
+                c.close()
             load().ahm_ctx_destroy(self.h)
             self.h = None
 
@@ -285,6 +291,7 @@ class TriFactor:
 
     def __init__(self, ctx: Context, handle, n: int, uplo: str):
         self.ctx, self.h, self.n, self.uplo = ctx, handle, n, uplo
+        ctx._children.add(self)
 
     @classmethod
     def upload(cls, ctx: Context, n: int, uplo: str, leaves: np.ndarray, data: np.ndarray) -> "TriFactor":
@@ -301,11 +308,17 @@ class TriFactor:
         leaves, data = factor_leaves(layout)
         return cls.upload(ctx, n, uplo, leaves, data)
 
+    def close(self):
+        # a handle whose context is already destroyed (finalisation order at
+        # interpreter exit) is dropped without touching the freed context
+        if self.h:
+            if self.ctx.h:
+                load().ahm_trih_destroy(self.h)
+            self.h = None
+
     def __del__(self):
         try:
-            if self.h:
-                load().ahm_trih_destroy(self.h)
-                self.h = None
+            self.close()
         except Exception:
             pass
 
@@ -331,6 +344,7 @@ class HMatrix:
 
     def __init__(self, ctx: Context, handle, n_rows: int, n_cols: int, storage: int):
         self.ctx, self.h, self.n_rows, self.n_cols, self.storage = ctx, handle, n_rows, n_cols, storage
+        ctx._children.add(self)
 
     @classmethod
     def upload(cls, ctx: Context, n_rows, n_cols, leaves: np.ndarray, data: np.ndarray,
@@ -565,8 +579,10 @@ class HMatrix:
         return y2
 
     def close(self):
+        # as TriFactor.close: never destroy through a freed context
         if self.h:
-            load().ahm_hmat_destroy(self.h)
+            if self.ctx.h:
+                load().ahm_hmat_destroy(self.h)
             self.h = None
 
     def __del__(self):

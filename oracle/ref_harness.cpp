@@ -91,9 +91,13 @@ struct Tree {
   std::vector<unsigned> i0;
 };
 
+// nl: leaf count, kept here so that freeing never reads the tree (the
+// Python wrappers may free a tree before a matrix built on it, e.g. when the
+// interpreter finalises both together)
 struct HMat {
   Tree* t;
   mblock<double>** A;
+  unsigned nl;
 };
 
 // H-LU factors on the block tree of t (H/HLU.cpp:20-66): L holds the lower
@@ -102,6 +106,7 @@ struct Factors {
   Tree* t;
   mblock<double>** L;
   mblock<double>** U;
+  unsigned nl;
 };
 
 // right-preconditioned operator: amux = A, precond_apply = HLU_solve
@@ -302,6 +307,7 @@ void* ref_hmat_assemble(void* tp, int kernel, double kappa, double weight, doubl
   Tree* t = (Tree*)tp;
   HMat* h = new HMat;
   h->t = t;
+  h->nl = t->nblcks;
   allocmbls(t->bl, h->A);
   KernelGen G = {kernel, kappa, weight, t->pts.data(), t->op.data()};
   Silence quiet;
@@ -328,6 +334,7 @@ void* ref_hmat_assemble_omp(void* tp, int kernel, double kappa, double weight, d
   Tree* t = (Tree*)tp;
   HMat* h = new HMat;
   h->t = t;
+  h->nl = t->nblcks;
   allocmbls(t->bl, h->A);
   KernelGen G = {kernel, kappa, weight, t->pts.data(), t->op.data()};
   blcluster** list;
@@ -368,6 +375,7 @@ void* ref_hmat_assemble_acar(void* tp, int kernel, double kappa, double weight, 
   Tree* t = (Tree*)tp;
   HMat* h = new HMat;
   h->t = t;
+  h->nl = t->nblcks;
   allocmbls(t->bl, h->A);
   KernelGen G = {kernel, kappa, weight, t->pts.data(), t->op.data()};
   Silence quiet;
@@ -383,6 +391,7 @@ void* ref_hmat_from_data(void* tp, const int* rank, const size_t* offset, const 
   Tree* t = (Tree*)tp;
   HMat* h = new HMat;
   h->t = t;
+  h->nl = t->nblcks;
   allocmbls(t->bl, h->A);
   for (unsigned i = 0; i < t->nblcks; ++i) {
     blcluster* b = t->leaves[i];
@@ -403,7 +412,7 @@ void* ref_hmat_from_data(void* tp, const int* rank, const size_t* offset, const 
 
 void ref_hmat_free(void* p) {
   HMat* h = (HMat*)p;
-  freembls(h->t->bl, h->A);
+  freembls(h->nl, h->A);
   delete h;
 }
 
@@ -561,6 +570,7 @@ void* ref_hmat_load(void* tp, const char* path, double* eta, unsigned* bmin) {
       return nullptr;
   HMat* h = new HMat;
   h->t = t;
+  h->nl = t->nblcks;
   h->A = A;
   return h;
 }
@@ -591,6 +601,7 @@ void* ref_hlu(void* hp, double eps, unsigned rankmax) {
   }
   Factors* f = new Factors;
   f->t = t;
+  f->nl = t->nblcks;
   initLtH_0(t->bl, f->L);
   initUtH_0(t->bl, f->U);
   Silence quiet;
@@ -607,8 +618,8 @@ void* ref_hlu(void* hp, double eps, unsigned rankmax) {
 
 void ref_factors_free(void* p) {
   Factors* f = (Factors*)p;
-  freembls(f->t->bl, f->L);
-  freembls(f->t->bl, f->U);
+  freembls(f->nl, f->L);
+  freembls(f->nl, f->U);
   delete f;
 }
 
@@ -645,6 +656,7 @@ void* ref_factors_from_data(void* tp, const int* kind_l, const size_t* off_l, co
   Tree* t = (Tree*)tp;
   Factors* f = new Factors;
   f->t = t;
+  f->nl = t->nblcks;
   for (int w = 0; w < 2; ++w) {
     const int* kind = w ? kind_u : kind_l;
     const size_t* off = w ? off_u : off_l;
