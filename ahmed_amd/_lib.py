@@ -97,6 +97,7 @@ SIGNATURES = {
     "ahm_gmres_batch": (_i, [_p, _p, _p, _d, _u32, _u32, _i, _p, _p, _p]),
     "ahm_rank_predict": (_i, [_p, _p, _u32, _p, C.c_float, _u32, _p, _p, _p]),
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
+    "ahm_hmat_matvec_noalloc": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_autotune": (_i, [_p, _u32, _i, _u32, C.POINTER(C.c_uint64), C.POINTER(_d)]),
     "ahm_hmat_matvec_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
     "ahm_hmat_matvec_t": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
@@ -483,6 +484,20 @@ class HMatrix:
         self.ctx.bind_stream()
         check(load().ahm_hmat_matvec(self.h, alpha, _p(x.data_ptr()), ldx, _p(y.data_ptr()), ldy, p,
                                      vec), "ahm_hmat_matvec")
+        return y
+
+    def matvec_noalloc(self, x, y, alpha: float = 1.0):
+        """matvec through ahm_hmat_matvec_noalloc: the workspace is the
+        library's static per-device array, nothing is allocated."""
+        torch = _torch()
+        vec = F64 if x.dtype == torch.float64 else F32
+        if x.dim() == 1:
+            p, ldx, ldy = 1, x.shape[0], y.shape[0]
+        else:
+            p, ldx, ldy = x.shape[0], x.stride(0), y.stride(0)
+        self.ctx.bind_stream()
+        check(load().ahm_hmat_matvec_noalloc(self.h, alpha, _p(x.data_ptr()), ldx, _p(y.data_ptr()), ldy, p,
+                                             vec), "ahm_hmat_matvec_noalloc")
         return y
 
     def matvec_t(self, x, y, alpha: float = 1.0):

@@ -129,6 +129,10 @@ struct PhaseTimer {
   double t_last;
 };
 
+// y += alpha H x on a caller-provided workspace (hmat.hip); no allocation
+int matvec_on_workspace(ahm_hmat h, void* base, size_t bytes, double alpha, const void* X, uint32_t ldx,
+                        void* Y, uint32_t ldy, uint32_t p, ahm_dtype vec, const char* who);
+
 int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_leaf_desc>&& leaves,
                struct DeviceBuf& raw, ahm_dtype data_dtype, ahm_dtype storage, int keep_leaves,
                ahm_hmat* out);

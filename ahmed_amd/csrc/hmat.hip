@@ -1045,6 +1045,34 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
   return AHM_OK;
 }
 
+// One product on a workspace the caller provides (bytes at base, 256-byte
+// aligned): t, then the partial sums, then the interleaved X, each rounded up
+// to 256 bytes.  Allocates nothing; refuses workspaces that are too small.
+int matvec_on_workspace(ahm_hmat h, void* base, size_t bytes, double alpha, const void* X, uint32_t ldx,
+                        void* Y, uint32_t ldy, uint32_t p, ahm_dtype vec, const char* who) {
+  if (!h || !X || !Y) return fail(AHM_ERR_INVALID, "%s: NULL argument", who);
+  if (p == 0) return AHM_OK;
+  if (p > 65535) return fail(AHM_ERR_INVALID, "%s: p=%u > 65535", who, p);
+  if (p > 1 && (ldx < h->n_cols || ldy < h->n_rows))
+    return fail(AHM_ERR_INVALID, "%s: ldx=%u ldy=%u for %u x %u", who, ldx, ldy, h->n_rows, h->n_cols);
+  if (!((h->storage == AHM_F64 && vec == AHM_F64) || (h->storage == AHM_F32 && vec == AHM_F32)))
+    return fail(AHM_ERR_UNSUPPORTED, "%s: storage %d with vectors %d", who, h->storage, vec);
+  const size_t es = dtype_size(vec);
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t tb = up(ws_t_bytes(h, p, es)), pb = up(ws_part_bytes(h, p, es)), xb = up(ws_x_bytes(h, p, es));
+  if (tb + pb + xb > bytes)
+    return fail(AHM_ERR_INVALID,
+                "%s: this product needs %zu bytes of workspace (V^T x %zu, partial sums %zu, "
+                "interleaved x %zu), more than the %zu available",
+                who, tb + pb + xb, tb, pb, xb, bytes);
+  char* b = (char*)base;
+  const Workspace ws{b, b + tb, b + tb + pb};
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  if (vec == AHM_F64)
+    return launch_matvec<double, double>(h, ws, alpha, (const double*)X, ldx, (double*)Y, ldy, p);
+  return launch_matvec<float, float>(h, ws, (float)alpha, (const float*)X, ldx, (float*)Y, ldy, p);
+}
+
 }  // namespace ahm
 
 namespace ahm {

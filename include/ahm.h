@@ -238,6 +238,19 @@ int ahm_rank_predict(ahm_ctx ctx, const float* feats, uint32_t n, const float* p
  * is asked for, which is refused).  p = 1 is the single-vector path. */
 int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y,
                     uint32_t ldy, uint32_t p, ahm_dtype vec);
+/* The same product for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Its workspace (V^T x, the
+ * partial sums of chunked panels, the interleaved right-hand sides) is one
+ * static 256 MiB device array per GPU, held from the first call on that GPU
+ * until the process ends; the call allocates nothing.  Products that need
+ * more workspace than that (t_len + partial slots, times the right-hand-side
+ * slots) are refused with AHM_ERR_INVALID; ahm_hmat_matvec serves them.  All
+ * calls on one GPU share the array: calls that overlap in time on one GPU
+ * (other threads, other streams) race and must be serialised by the caller.
+ * Results are bit-identical to ahm_hmat_matvec's. */
+int ahm_hmat_matvec_noalloc(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y,
+                            uint32_t ldy, uint32_t p, ahm_dtype vec);
 /* Runs the product `reps` times with HIP events around each of the two
  * launches (V^T x, then the row panels) and returns their mean durations in
  * milliseconds (instrumentation; same arguments as ahm_hmat_matvec). */
