@@ -174,6 +174,13 @@ struct ahm_hmat_s {
   size_t tbuf_cap = 0;        // bytes
   ahm::DeviceBuf xbuf;        // interleaved copy of X for multi-RHS products
   size_t xbuf_cap = 0;        // bytes
+  // AHM_BF16 storage (FP32 values, V^T X of multi-RHS products on BF16
+  // MFMA): BF16 copy of the phase-A tasks in k-groups of 8 columns, the
+  // offset of each task's copy (parallel to tasks_a), the BF16 interleaved X
+  ahm::DeviceBuf bfarena, bfoff;
+  uint64_t bf_vals = 0;
+  ahm::DeviceBuf xbbuf;
+  size_t xbbuf_cap = 0;       // bytes
   ahm::DeviceBuf leafdata;    // optional mblock-layout copy (keep_leaves)
   std::vector<ahm_leaf_desc> leaves;
   // host copies of the plan (used by ahm_hmat_convert)

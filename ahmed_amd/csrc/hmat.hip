@@ -593,6 +593,132 @@ __global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
 }
 
 // ------------------------------------------------------------------------
+// V^T X of multi-RHS products on BF16 MFMA (AHM_BF16 storage): V and X are
+// rounded to BF16, the products accumulate in FP32 on
+// v_mfma_f32_16x16x32_bf16.  Lane layout (gfx950): lane l holds
+//   A (16 x 32): A[l%16][8*(l/16) .. 8*(l/16)+7]
+//   B (32 x 16): B[8*(l/16) .. 8*(l/16)+7][l%16]
+//   C (16 x 16): value r -> (4*(l/16) + r, l%16).
+// The BF16 copy of a phase-A task (m x W values, column-major in the FP32
+// arena) holds ceil(W/8) k-groups, each m rows of 8 consecutive columns, so
+// a lane's A operand is one 16-byte load: element (r, c) at
+// ((c/8)*m + r)*8 + c%8, zero for c >= W.
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+// FP32 -> BF16, round to nearest even (finite values)
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// one workgroup per phase-A task: its BF16 copy at bfoff[task]
+__global__ __launch_bounds__(256) void bf16_panels_kernel(const GemvTask* __restrict__ tasks,
+                                                          const uint64_t* __restrict__ bfoff,
+                                                          const float* __restrict__ arena,
+                                                          uint16_t* __restrict__ out) {
+  const GemvTask tk = tasks[blockIdx.x];
+  const uint64_t base = bfoff[blockIdx.x];
+  const uint32_t m = tk.m, W = tk.W;
+  const uint64_t n = (uint64_t)m * ((W + 7) & ~7u);
+  for (uint64_t e = threadIdx.x; e < n; e += 256) {
+    const uint32_t i = (uint32_t)(e & 7), q = (uint32_t)(e >> 3);
+    const uint32_t r = q % m, c = (q / m) * 8 + i;
+    out[base + e] = c < W ? bf16_rne(arena[tk.data + (uint64_t)c * m + r]) : (uint16_t)0;
+  }
+}
+
+__global__ void bf16_copy_kernel(const float* __restrict__ src, uint16_t* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = bf16_rne(src[i]);
+}
+
+// Z columns staged per pass (one per thread)
+constexpr int kBfChunk = kPanelThreads;
+
+// t(m x 32) = V^T X for one phase-A task and 32 right-hand sides
+// (blockIdx.y): Z gathered into LDS transposed (zs[rhs][column], BF16 rows
+// of the interleaved X), wavefront w owns row tiles w, w+4, w+8, w+12 with
+// both 16-column tiles.  Output as panel_mm_kernel's phase A: t interleaved
+// (32 values per row), or the chunk's partial sums.
+__global__ __launch_bounds__(kPanelThreads) void panel_bf16_kernel(
+    const GemvTask* __restrict__ tasks, const uint64_t* __restrict__ bfoff, const uint32_t* __restrict__ zsrc,
+    const uint16_t* __restrict__ bfa, const uint16_t* __restrict__ xb, uint32_t n_x, float* __restrict__ ti,
+    uint32_t t_len, float* __restrict__ part, uint32_t part_len, uint32_t p) {
+  __shared__ __attribute__((aligned(16))) uint16_t zs[kMmRhs][kBfChunk + 8];
+  const GemvTask tk = tasks[blockIdx.x];
+  const uint64_t base = bfoff[blockIdx.x];
+  const uint32_t g = blockIdx.y, rhs0 = g * kMmRhs;
+  const uint16_t* xg = xb + (uint64_t)g * n_x * kMmRhs;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint32_t m = tk.m, W = tk.W, ngr = (W + 7) / 8;
+  const int nrt = (int)(m + 15) / 16;
+  const bool two = p - rhs0 > 16;
+  const int li = lane & 15, lk = lane >> 4;
+  mm_f32x4 acc[4][2];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q][0] = acc[q][1] = mm_f32x4{0, 0, 0, 0};
+  for (uint32_t c0 = 0; c0 < W; c0 += kBfChunk) {
+    const uint32_t cw = min((uint32_t)kBfChunk, W - c0);
+    __syncthreads();
+    {
+      const uint32_t c = threadIdx.x;
+      uint4 v[4] = {};
+      if (c < cw) {
+        const uint4* row = (const uint4*)(xg + (uint64_t)zsrc[tk.zoff + c0 + c] * kMmRhs);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = row[q];
+      }
+      const uint16_t* vv = (const uint16_t*)v;
+#pragma unroll
+      for (int j = 0; j < kMmRhs; ++j) zs[j][c] = vv[j];
+    }
+    __syncthreads();
+    const uint32_t nks = (cw + 31) / 32;
+    for (uint32_t ks = 0; ks < nks; ++ks) {
+      const uint32_t kc = ks * 32 + (uint32_t)lk * 8;   // chunk-local first column of the lane's group
+      const uint32_t kg = (c0 + kc) / 8;                // the task's k-group (c0 is a multiple of 8)
+      const bf16x8_t b0 = __builtin_bit_cast(bf16x8_t, *(const uint4*)&zs[li][kc]);
+      const bf16x8_t b1 = __builtin_bit_cast(bf16x8_t, *(const uint4*)&zs[16 + li][kc]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rt = w + 4 * q;
+        if (rt >= nrt) break;
+        uint4 av = {0, 0, 0, 0};
+        if (kg < ngr) av = *(const uint4*)(bfa + base + ((uint64_t)kg * m + (uint32_t)(rt * 16 + li)) * 8);
+        const bf16x8_t a = __builtin_bit_cast(bf16x8_t, av);
+        acc[q][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[q][0], 0, 0, 0);
+        if (two) acc[q][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[q][1], 0, 0, 0);
+      }
+    }
+  }
+  float* tg = ti + (uint64_t)g * t_len * kMmRhs;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int rt = w + 4 * q;
+    if (rt >= nrt) break;
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      if (ct == 1 && !two) break;
+      const int jj = ct * 16 + li;
+      const uint32_t j = rhs0 + jj;
+      if (j >= p) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t row = (uint32_t)(rt * 16 + 4 * lk + r);
+        if (row >= m) continue;
+        const float v = acc[q][ct][r];
+        if (tk.partial)
+          part[(uint64_t)j * part_len + tk.out + row] = v;
+        else
+          tg[(uint64_t)(tk.out + row) * kMmRhs + jj] = v;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
 // plan: host-side construction of the tasks and copy jobs
 
 struct Plan {
@@ -943,6 +1069,12 @@ static int ensure_tbuf(ahm_hmat h, uint32_t p, size_t esize) {
     AHM_TRY(h->tbuf.alloc(need));
     h->tbuf_cap = need;
   }
+  const size_t xbneed = h->storage == AHM_BF16 && p >= 9 ? ws_x_bytes(h, p, esize) / 2 : 0;
+  if (xbneed > h->xbbuf_cap) {
+    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+    AHM_TRY(h->xbbuf.alloc(xbneed));
+    h->xbbuf_cap = xbneed;
+  }
   const size_t xneed = ws_x_bytes(h, p, esize);
   if (xneed > h->xbuf_cap) {
     AHM_HIP(hipStreamSynchronize(h->ctx->stream));
@@ -968,9 +1100,10 @@ struct Workspace {
   void* t;
   void* part;
   void* xi;
+  void* xb;   // BF16 copy of xi (AHM_BF16 storage, p >= 9)
 };
 
-static Workspace own_workspace(ahm_hmat h) { return Workspace{h->tbuf.p, h->pbuf.p, h->xbuf.p}; }
+static Workspace own_workspace(ahm_hmat h) { return Workspace{h->tbuf.p, h->pbuf.p, h->xbuf.p, h->xbbuf.p}; }
 
 template <typename TS, typename TV>
 static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X, uint32_t ldx, TV* Y,
@@ -1021,11 +1154,23 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
       };
       run(std::integral_constant<int, 8>());
     } else {
-      if (h->n_a)
+      bool bf16_a = false;
+      if constexpr (std::is_same<TV, float>::value) bf16_a = h->storage == AHM_BF16;
+      if (h->n_a && bf16_a) {
+        // V^T X on BF16 MFMA: X rounded to BF16, FP32 accumulation
+        hipLaunchKernelGGL(bf16_copy_kernel, dim3(2048), dim3(256), 0, s, (const float*)xi, (uint16_t*)ws.xb,
+                           (uint64_t)h->n_cols * kMmRhs * ng);
+        hipLaunchKernelGGL(panel_bf16_kernel, dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,
+                           (const GemvTask*)h->tasks_a.p, (const uint64_t*)h->bfoff.p,
+                           (const uint32_t*)h->zsrc.p, (const uint16_t*)h->bfarena.p,
+                           (const uint16_t*)ws.xb, h->n_cols, (float*)t, h->t_len, (float*)part,
+                           h->part_len, p);
+      } else if (h->n_a) {
         hipLaunchKernelGGL((panel_mm_kernel<TV, false>), dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,
                            (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
                            (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, Y, ldy, part,
                            h->part_len, alpha, p);
+      }
       if (h->n_ra)
         hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, p), dim3(kPanelThreads), 0, s,
                            (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha,
@@ -1055,6 +1200,7 @@ int matvec_on_workspace(ahm_hmat h, void* base, size_t bytes, double alpha, cons
   if (p > 65535) return fail(AHM_ERR_INVALID, "%s: p=%u > 65535", who, p);
   if (p > 1 && (ldx < h->n_cols || ldy < h->n_rows))
     return fail(AHM_ERR_INVALID, "%s: ldx=%u ldy=%u for %u x %u", who, ldx, ldy, h->n_rows, h->n_cols);
+  // the static workspace carries no BF16 copy of X: F64 and F32 storage only
   if (!((h->storage == AHM_F64 && vec == AHM_F64) || (h->storage == AHM_F32 && vec == AHM_F32)))
     return fail(AHM_ERR_UNSUPPORTED, "%s: storage %d with vectors %d", who, h->storage, vec);
   const size_t es = dtype_size(vec);
@@ -1066,7 +1212,7 @@ int matvec_on_workspace(ahm_hmat h, void* base, size_t bytes, double alpha, cons
                 "interleaved x %zu), more than the %zu available",
                 who, tb + pb + xb, tb, pb, xb, bytes);
   char* b = (char*)base;
-  const Workspace ws{b, b + tb, b + tb + pb};
+  const Workspace ws{b, b + tb, b + tb + pb, nullptr};
   AHM_HIP(hipSetDevice(h->ctx->device));
   if (vec == AHM_F64)
     return launch_matvec<double, double>(h, ws, alpha, (const double*)X, ldx, (double*)Y, ldy, p);
@@ -1079,6 +1225,40 @@ namespace ahm {
 
 // re-cut the panels of both phases into chunks of about cv values (the arena
 // is unchanged: chunks are column ranges of the same panels)
+// BF16 copy of the phase-A tasks for AHM_BF16 storage (after every change
+// of the tasks): each task's m x W values in ceil(W/8) k-groups of m rows of 8
+// columns, at an offset that is a multiple of 8 values; zeroed slack after the
+// last copy, since the last row tile of a task reads up to 15 rows past m.
+static int build_bf16(ahm_hmat h) {
+  const std::vector<GemvTask>& A = h->h_tasks_a;
+  std::vector<uint64_t> off(A.size());
+  uint64_t tot = 0;
+  for (size_t i = 0; i < A.size(); ++i) {
+    const GemvTask& tk = A[i];
+    if (tk.m > (uint32_t)kPanelThreads)
+      return fail(AHM_ERR_UNSUPPORTED, "BF16 V^T X: phase-A task of %u rows", tk.m);
+    for (uint32_t c = 0; c < tk.W; ++c)
+      if (h->h_zsrc[tk.zoff + c] >= h->n_cols)
+        return fail(AHM_ERR_INVALID, "BF16 V^T X: phase-A column source out of range");
+    off[i] = tot;
+    tot += (uint64_t)tk.m * ((tk.W + 7) & ~7u);
+  }
+  constexpr uint64_t kSlack = 256;
+  hipStream_t s = h->ctx->stream;
+  AHM_HIP(hipStreamSynchronize(s));
+  AHM_TRY(h->bfarena.alloc((tot + kSlack) * sizeof(uint16_t)));
+  AHM_HIP(hipMemsetAsync((uint16_t*)h->bfarena.p + tot, 0, kSlack * sizeof(uint16_t), s));
+  AHM_TRY(upload_vec(h->bfoff, off, s));
+  if (!A.empty())
+    hipLaunchKernelGGL(bf16_panels_kernel, dim3((uint32_t)A.size()), dim3(256), 0, s,
+                       (const GemvTask*)h->tasks_a.p, (const uint64_t*)h->bfoff.p, (const float*)h->arena.p,
+                       (uint16_t*)h->bfarena.p);
+  AHM_HIP(hipGetLastError());
+  AHM_HIP(hipStreamSynchronize(s));
+  h->bf_vals = tot;
+  return AHM_OK;
+}
+
 int set_chunking(ahm_hmat h, uint64_t cv) {
   std::vector<GemvTask> a, b;
   std::vector<ReduceTask> ra, r;
@@ -1104,6 +1284,7 @@ int set_chunking(ahm_hmat h, uint64_t cv) {
   h->h_tasks_b = std::move(b);
   h->h_tasks_r = std::move(r);
   h->chunk_values = cv;
+  if (h->storage == AHM_BF16) AHM_TRY(build_bf16(h));
   return AHM_OK;
 }
 
@@ -1241,8 +1422,8 @@ int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_le
 
 int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
   if (!src || !out) return fail(AHM_ERR_INVALID, "ahm_hmat_convert: NULL argument");
-  if (src->storage != AHM_F64 || storage != AHM_F32)
-    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_convert: only F64 -> F32");
+  if (src->storage != AHM_F64 || (storage != AHM_F32 && storage != AHM_BF16))
+    return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_convert: only F64 -> F32 or BF16");
   AHM_HIP(hipSetDevice(src->ctx->device));
   hipStream_t s = src->ctx->stream;
   ahm_hmat h = new ahm_hmat_s;
@@ -1294,6 +1475,7 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
       st = fail(AHM_ERR_HIP, "ahm_hmat_convert: cast kernel failed");
       break;
     }
+    if (storage == AHM_BF16 && (st = build_bf16(h))) break;
   } while (0);
   if (st) {
     delete h;
@@ -1322,7 +1504,7 @@ int ahm_hmat_stats_get(ahm_hmat h, ahm_hmat_stats* st) {
   st->nvals_u = h->nv_u;
   st->nvals_v = h->nv_v;
   st->t_len = h->t_len;
-  st->bytes_per_value = dtype_size(h->storage);
+  st->bytes_per_value = h->storage == AHM_BF16 ? 4 : dtype_size(h->storage);
   st->n_tasks_a = h->n_a;
   st->n_tasks_b = h->n_b;
   st->max_rank = h->max_rank;
@@ -1375,7 +1557,7 @@ int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void*
     return launch_matvec<double, double>(h, own_workspace(h), alpha, (const double*)X, ldx, (double*)Y,
                                          ldy, p);
   }
-  if (h->storage == AHM_F32 && vec == AHM_F32) {
+  if ((h->storage == AHM_F32 || h->storage == AHM_BF16) && vec == AHM_F32) {
     AHM_TRY(ensure_tbuf(h, p, 4));
     AHM_TRY(ensure_pbuf(h, p, 4));
     return launch_matvec<float, float>(h, own_workspace(h), (float)alpha, (const float*)X, ldx, (float*)Y,
@@ -1389,7 +1571,7 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
                             double* ms_a, double* ms_b) {
   if (!h || !X || !Y || !ms_a || !ms_b || reps == 0)
     return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_profile: bad arguments");
-  if (!((h->storage == AHM_F64 && vec == AHM_F64) || (h->storage == AHM_F32 && vec == AHM_F32)))
+  if (!((h->storage == AHM_F64 && vec == AHM_F64) || ((h->storage == AHM_F32 || h->storage == AHM_BF16) && vec == AHM_F32)))
     return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec_profile: storage %d with vectors %d",
                 h->storage, vec);
   if (p == 0 || p > 65535 || (p > 1 && (ldx < h->n_cols || ldy < h->n_rows)))
@@ -1433,7 +1615,7 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
 int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint64_t* chunk_values,
                       double* best_ms) {
   if (!h || p == 0 || reps == 0) return fail(AHM_ERR_INVALID, "ahm_hmat_autotune: bad arguments");
-  if (!((h->storage == AHM_F64 && vec == AHM_F64) || (h->storage == AHM_F32 && vec == AHM_F32)))
+  if (!((h->storage == AHM_F64 && vec == AHM_F64) || ((h->storage == AHM_F32 || h->storage == AHM_BF16) && vec == AHM_F32)))
     return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_autotune: storage %d with vectors %d", h->storage, vec);
   AHM_HIP(hipSetDevice(h->ctx->device));
   hipStream_t s = h->ctx->stream;

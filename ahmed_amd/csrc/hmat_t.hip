@@ -252,7 +252,7 @@ int ahm_hmat_matvec_t(ahm_hmat h, double alpha, const void* X, uint32_t ldx, voi
     return fail(AHM_ERR_INVALID, "ahm_hmat_matvec_t: ldx=%u ldy=%u for %u x %u", ldx, ldy, h->n_rows,
                 h->n_cols);
   const bool f64 = h->storage == AHM_F64 && vec == AHM_F64;
-  const bool f32 = h->storage == AHM_F32 && vec == AHM_F32;
+  const bool f32 = (h->storage == AHM_F32 || h->storage == AHM_BF16) && vec == AHM_F32;
   if (!f64 && !f32)
     return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec_t: storage %d with vectors %d", h->storage, vec);
   AHM_HIP(hipSetDevice(h->ctx->device));

@@ -169,7 +169,7 @@ struct Op {
   uint32_t n;
   DeviceBuf x32, y32;
   int init() {
-    if (h->storage == AHM_F32) {
+    if (h->storage != AHM_F64) {   // FP32 values (AHM_F32, AHM_BF16)
       AHM_TRY(x32.alloc((size_t)n * 4));
       AHM_TRY(y32.alloc((size_t)n * 4));
     }

@@ -578,6 +578,27 @@ def main():
     err32 = float((y32.double() - yref).abs().max() / yref.abs().max())
     log(f"FP32 H-matvec 1 RHS: {ms32:.3f} ms, {B32 / (ms32 * 1e-3) / 1e9:.1f} GB/s, rel.err vs FP64 {err32:.2e}")
 
+    # ---- p RHS with FP32 storage, and with V^T X on BF16 MFMA ----------------
+    Y64 = torch.zeros(p, N, dtype=torch.float64, device="cuda")
+    H.matvec(X, Y64)
+    X32p = X.float()
+    Y32p = torch.zeros(p, N, dtype=torch.float32, device="cuda")
+    ms32p = timed(lambda: H32.matvec(X32p, Y32p), torch, max(3, args.steps // 4), 1)
+    Y32p.zero_()
+    H32.matvec(X32p, Y32p)
+    err32p = float((Y32p.double() - Y64).abs().max() / Y64.abs().max())
+    Hb = H.convert(_lib.BF16)
+    ms_bp = timed(lambda: Hb.matvec(X32p, Y32p), torch, max(3, args.steps // 4), 1)
+    Y32p.zero_()
+    Hb.matvec(X32p, Y32p)
+    errbp = float((Y32p.double() - Y64).abs().max() / Y64.abs().max())
+    ms_bf_a, ms_bf_b = Hb.profile(X32p, Y32p, reps=3)
+    B_bf = 2.0 * st["nvals_v"] + 4.0 * (st["nvals_dense"] + st["nvals_u"]) + 4.0 * p * (N + 2 * N)
+    log(f"{p} RHS: FP32 storage {ms32p:.3f} ms ({F_alg / (ms32p * 1e-3) / 1e12:.2f} TF/s, rel.err {err32p:.2e}); "
+        f"BF16 V^T X {ms_bp:.3f} ms ({F_alg / (ms_bp * 1e-3) / 1e12:.2f} TF/s, rel.err {errbp:.2e}; "
+        f"phase A {ms_bf_a:.3f} ms, phase B {ms_bf_b:.3f} ms)")
+    del Hb, Y64, X32p, Y32p
+
     # ---- GMRES(50) time-to-tolerance: FP64 vs FP64 refinement around FP32 ----
     b = torch.zeros(N, dtype=torch.float64, device="cuda")
     H.matvec(torch.ones(N, dtype=torch.float64, device="cuda"), b)   # b = H * 1
@@ -681,7 +702,15 @@ def main():
                              "GB/s_alg": round(Bp / (msp * 1e-3) / 1e9, 2)},
         "matvec_transposed": {"ms": round(mst, 4), "GB/s": round(B_alg / (mst * 1e-3) / 1e9, 2)},
         "fp32_storage": {"ms": round(ms32, 4), "GB/s": round(B32 / (ms32 * 1e-3) / 1e9, 2),
-                         "rel_err_vs_fp64": err32},
+                         "rel_err_vs_fp64": err32,
+                         "rhs32": {"ms": round(ms32p, 4), "TF/s": round(F_alg / (ms32p * 1e-3) / 1e12, 3),
+                                   "frac_fp32_peak": round(F_alg / (ms32p * 1e-3) / 1e12 / FP32_PEAK_TFS, 4),
+                                   "rel_err_vs_fp64": err32p}},
+        "bf16_vtx_rhs32": {"what": "U, dense in FP32; V^T X on BF16 MFMA (V, X rounded to BF16), FP32 accumulation",
+                           "ms": round(ms_bp, 4), "TF/s": round(F_alg / (ms_bp * 1e-3) / 1e12, 3),
+                           "GB/s_alg": round(B_bf / (ms_bp * 1e-3) / 1e9, 1),
+                           "phase_a_ms": round(ms_bf_a, 4), "phase_b_ms": round(ms_bf_b, 4),
+                           "rel_err_vs_fp64": errbp},
         "assembly": {"tree_s": round(t_tree, 3), "gpu_assembly_s": round(t_asm, 3), "aca_ms": round(ast["aca_ms"], 1),
                      "aca_retries": ast["n_retries"], "mean_aca_rank": ast["sum_aca_rank"] / max(1, ast["n_candidates"]),
                      "mean_rank": ast["sum_rank"] / max(1, ast["n_lowrank"]), "work": asm_work, "acar": acar},

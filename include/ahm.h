@@ -125,7 +125,13 @@ int ahm_tree_leaves(ahm_tree t, uint32_t* b1, uint32_t* n1, uint32_t* b2, uint32
 int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_leaves,
                     const ahm_leaf_desc* leaves, const void* data, ahm_dtype data_dtype,
                     ahm_dtype storage, int keep_leaves, ahm_hmat* out);
-/* Re-layout an existing matrix in another storage precision (FP64 -> FP32). */
+/* Re-layout an existing matrix in another storage precision: FP64 -> FP32
+ * (copyH, H/initH.cpp:152-168), or FP64 -> AHM_BF16: the FP32 matrix plus a
+ * BF16 copy of every V factor, so that products with 9 or more right-hand
+ * sides form V^T X on BF16 MFMA (V and X rounded to BF16 by round-to-nearest-
+ * even, FP32 accumulation) and U (V^T X) and the dense blocks in FP32.  An
+ * AHM_BF16 matrix takes AHM_F32 vectors; with fewer right-hand sides it
+ * multiplies exactly as its FP32 copy. */
 int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out);
 int ahm_hmat_destroy(ahm_hmat h);
 int ahm_hmat_stats_get(ahm_hmat h, ahm_hmat_stats* st);

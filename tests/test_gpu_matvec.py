@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from conftest import golden_names
-from helpers import golden_hmatrix, leaves_of, relerr, sphere_hmatrix
+from helpers import block_matrix, golden_hmatrix, leaves_of, relerr, sphere_hmatrix
 
 pytestmark = pytest.mark.gpu
 
@@ -182,37 +182,6 @@ def test_autotuned_chunking_keeps_results(ctx):
         assert relerr(y.cpu().numpy(), y0.cpu().numpy()) <= 1e-14
         assert relerr(Y.cpu().numpy(), Y0.cpu().numpy()) <= 1e-14
     assert relerr(y0.cpu().numpy(), H.matvec(x.cpu().numpy())) <= 1e-12
-
-
-def block_matrix(sizes, seed):
-    """Leaves of a block matrix on row/column segments of the given sizes:
-    dense diagonal blocks, low-rank off-diagonal blocks of ranks 1..6, with
-    the dense N x N matrix they represent (column-major leaf values as in
-    mblock: dense n1 x n2, low-rank U n1 x k then V n2 x k)."""
-    from ahmed_amd import _lib
-    rng = np.random.default_rng(seed)
-    b = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-    N = int(b[-1])
-    L = {f: [] for f in ("b1", "n1", "b2", "n2", "rank", "offset")}
-    chunks, off = [], 0
-    A = np.zeros((N, N))
-    for i, mi in enumerate(sizes):
-        for j, mj in enumerate(sizes):
-            if i == j:
-                k, vals = -1, rng.standard_normal(mi * mj)
-                A[b[i]:b[i + 1], b[j]:b[j + 1]] = vals.reshape(mj, mi).T
-            else:
-                k = int(1 + (i * 7 + j * 3) % 6)
-                vals = rng.standard_normal(k * (mi + mj))
-                U = vals[:mi * k].reshape(k, mi).T
-                V = vals[mi * k:].reshape(k, mj).T
-                A[b[i]:b[i + 1], b[j]:b[j + 1]] = U @ V.T
-            for f, v in zip(("b1", "n1", "b2", "n2", "rank", "offset"), (b[i], mi, b[j], mj, k, off)):
-                L[f].append(v)
-            chunks.append(vals)
-            off += vals.size
-    leaves = _lib.leaf_array(L["b1"], L["n1"], L["b2"], L["n2"], L["rank"], L["offset"])
-    return N, leaves, np.concatenate(chunks), A
 
 
 # Row panels of every height class of the MFMA product's wavefront shapes:
