@@ -1221,7 +1221,10 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     (void)hipGetDevice(&dev);
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b) {
       free_b += cache_bytes(dev);   // blocks the allocation cache holds are free to reuse
-      budget = std::min<uint64_t>(std::max<uint64_t>(free_b / 4, (uint64_t)4 << 30), (uint64_t)96 << 30);
+      // in steps of 16 GiB, so that the next assembly asks for the same size
+      // and the allocation cache can hand it this one's block
+      const uint64_t q = (uint64_t)16 << 30;
+      budget = std::min<uint64_t>(std::max<uint64_t>(free_b / 4 / q * q, (uint64_t)4 << 30), (uint64_t)96 << 30);
     }
   }
   std::vector<Part*> parts;
@@ -1342,7 +1345,11 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       part->len = tot;
       if (tot) {
         DeviceBuf& doff = goff.buf;
-        st = part->buf.alloc(tot * 8);
+        // sizes of 1 GiB and more in steps of 4 GiB: blocks of one assembly's
+        // batches fit the next assembly's (the allocation cache reuses a block
+        // for requests of at least half its size)
+        const uint64_t g4 = (uint64_t)4 << 30;
+        st = part->buf.alloc(tot * 8 >= ((uint64_t)1 << 30) ? (tot * 8 + g4 - 1) / g4 * g4 : tot * 8);
         if (!st) st = goff.put(off, s);
         if (st) {
           cleanup();
@@ -1367,6 +1374,13 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       // reruns join the next batch
       todo.insert(todo.begin() + (ptrdiff_t)pos, rerun.begin(), rerun.end());
     }
+  }
+  // the batches' small scratch goes back now; the slab scratch is kept for
+  // the leaf buffer below (a fresh allocation of tens of GB costs seconds on
+  // MI355X right after large frees)
+  for (Grow* g : {&gblk, &gres, &gscr, &goff}) {
+    g->buf.release();
+    g->cap = 0;
   }
   timer.mark("ACA batches (kernels, result copies, slab decisions)");
   const double aca_ms =
@@ -1397,7 +1411,22 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     }
   }
   timer.mark("leaf list");
-  int st = data.alloc(total * 8);
+  int st = AHM_OK;
+  if (gslab.buf.p && gslab.buf.cap >= total * 8 && total) {
+    // the slab scratch (no longer needed) becomes the leaf buffer
+    data.release();
+    data.p = gslab.buf.p;
+    data.bytes = total * 8;
+    data.cap = gslab.buf.cap;
+    data.dev = gslab.buf.dev;
+    gslab.buf.p = nullptr;
+    gslab.buf.bytes = gslab.buf.cap = 0;
+    gslab.cap = 0;
+  } else {
+    gslab.buf.release();
+    gslab.cap = 0;
+    st = data.alloc(total * 8);
+  }
   timer.mark("leaf buffer allocation");
   if (st) {
     cleanup();

@@ -33,7 +33,22 @@ int fail(int code, const char* fmt, ...) {
 
 namespace {
 constexpr size_t kCacheMin = (size_t)1 << 30;     // blocks at least this large are cached
-constexpr size_t kCacheMax = (size_t)128 << 30;   // per device
+// per device: three quarters of its memory, so that the large buffers of one
+// assembly (slab scratch / leaf buffer, arena, result parts) all stay cached
+// for the next one
+size_t cache_max(int dev) {
+  static size_t cap[64] = {};
+  if (dev < 0 || dev >= 64) return (size_t)128 << 30;
+  if (!cap[dev]) {
+    size_t fr = 0, tot = 0;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(dev);
+    cap[dev] = hipMemGetInfo(&fr, &tot) == hipSuccess && tot ? tot / 4 * 3 : (size_t)128 << 30;
+    (void)hipSetDevice(cur);
+  }
+  return cap[dev];
+}
 struct CachedBlock {
   void* p;
   size_t cap;
@@ -93,7 +108,13 @@ int DeviceBuf::alloc(size_t b) {
       return AHM_OK;
     }
   }
+  const bool timing = b >= kCacheMin && std::getenv("AHM_TIMING") != nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
   hipError_t e = hipMalloc(&p, b);
+  if (timing)
+    std::fprintf(stderr, "[ahm timing] hipMalloc %.2f GB: %.1f ms (cache holds %.2f GB)\n", b / 1e9,
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(),
+                 cache_bytes(dev) / 1e9);
   if (e != hipSuccess && cache_bytes(dev) > 0) {
     (void)hipGetLastError();
     cache_flush(dev);
@@ -118,7 +139,8 @@ void DeviceBuf::release() {
       std::lock_guard<std::mutex> lk(g_cache_mu);
       g_cache.push_back(CachedBlock{p, cap, dev});
       // over the limit: free the oldest blocks of the device
-      while (cache_bytes_locked(dev) > kCacheMax) {
+      const size_t cmax = cache_max(dev);
+      while (cache_bytes_locked(dev) > cmax) {
         for (size_t i = 0; i < g_cache.size(); ++i)
           if (g_cache[i].dev == dev) {
             (void)hipFree(g_cache[i].p);

@@ -97,7 +97,7 @@ struct CopyJob {
 };
 
 // Device allocation.  Blocks of at least 1 GiB are not freed on release but
-// kept in a process-wide cache (per device, at most 128 GiB) and handed to
+// kept in a process-wide cache (per device, at most 3/4 of its memory) and handed to
 // the next large request of at least half their size (best fit): a 50 GB
 // hipMalloc right after large frees took 2.3 s on MI355X, against under
 // 1 ms on memory not used before, so a second assembly in one process would

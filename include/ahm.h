@@ -92,8 +92,9 @@ int ahm_ctx_destroy(ahm_ctx ctx);
 int ahm_ctx_set_stream(ahm_ctx ctx, void* hip_stream);
 int ahm_ctx_sync(ahm_ctx ctx);
 /* device blocks of >= 1 GiB are not freed when released but cached (up to
- * 128 GiB per device) for the next large allocation: a 50 GB hipMalloc
- * right after large frees took 2.3 s, against under 1 ms on unused memory.
+ * three quarters of the device memory) for the next large allocation: a
+ * 50 GB hipMalloc right after large frees took 2.3 s, against under 1 ms on
+ * unused memory.
  * The cache is emptied when an allocation fails, when a context of the
  * device is destroyed, and by this call (e.g. before handing the memory to
  * another allocator).  No counterpart in the reference (host memory). */
