@@ -98,7 +98,9 @@ SIGNATURES = {
     "ahm_rank_predict": (_i, [_p, _p, _u32, _p, C.c_float, _u32, _p, _p, _p]),
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_matvec_noalloc": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
-    "ahm_hmat_autotune": (_i, [_p, _u32, _i, _u32, C.POINTER(C.c_uint64), C.POINTER(_d)]),
+    "ahm_hmat_autotune": (_i, [_p, _u32, _i, _u32, C.POINTER(C.c_uint64), C.POINTER(_d), C.POINTER(_i)]),
+    "ahm_hmat_set_fused": (_i, [_p, _i]),
+    "ahm_hmat_fused_status": (_i, [_p, C.POINTER(_i)]),
     "ahm_hmat_matvec_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
     "ahm_hmat_matvec_t": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_matvec_t_host": (_i, [_p, _d, _p, _u32, _p, _u32, _u32]),
@@ -578,11 +580,24 @@ class HMatrix:
                     hist=hist[:min(nhist, outer.value + 1)].copy())
 
     def autotune(self, p: int = 1, reps: int = 5):
-        """Pick the fastest row-panel chunk size for p right-hand sides."""
-        cv, ms = C.c_uint64(), _d()
+        """Pick the fastest tile size (row-panel / V^T chunk values) and, for
+        p <= 8, launch form for p right-hand sides: (chunk values, ms, fused)."""
+        cv, ms, fu = C.c_uint64(), _d(), _i()
+        vec = F64 if self.storage == F64 else F32
         self.ctx.bind_stream()
-        check(load().ahm_hmat_autotune(self.h, p, self.storage, reps, C.byref(cv), C.byref(ms)), "autotune")
-        return cv.value, ms.value
+        check(load().ahm_hmat_autotune(self.h, p, vec, reps, C.byref(cv), C.byref(ms), C.byref(fu)),
+              "autotune")
+        return cv.value, ms.value, bool(fu.value)
+
+    def set_fused(self, on: bool = True):
+        """Products of 1-8 right-hand sides as one persistent launch."""
+        check(load().ahm_hmat_set_fused(self.h, int(on)), "ahm_hmat_set_fused")
+
+    def fused_status(self) -> bool:
+        """True if a fused product gave up a bounded wait (invalid result)."""
+        a = _i()
+        check(load().ahm_hmat_fused_status(self.h, C.byref(a)), "ahm_hmat_fused_status")
+        return bool(a.value)
 
     def matvec_host(self, x: np.ndarray, y: np.ndarray, alpha: float = 1.0):
         x = np.asfortranarray(x, np.float64)

@@ -181,6 +181,7 @@ struct ahm_hmat_s {
   uint64_t bf_vals = 0;
   ahm::DeviceBuf xbbuf;
   size_t xbbuf_cap = 0;       // bytes
+  int fused = 0;   // single-launch products: switched off (ahm_hmat_set_fused)
   ahm::DeviceBuf leafdata;    // optional mblock-layout copy (keep_leaves)
   std::vector<ahm_leaf_desc> leaves;
   // host copies of the plan (used by ahm_hmat_convert)

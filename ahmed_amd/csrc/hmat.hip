@@ -102,16 +102,35 @@ constexpr int kZChunk = 2048;
 // groups through LDS in a fixed order.  Phase B tasks either accumulate
 // alpha * sum into y or, for a column chunk of a wide panel, store the row
 // sums in the partial buffer.
-template <typename TS, typename TV, bool PHASE_B>
-__global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
-    const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
-    const TS* __restrict__ arena, const TV* __restrict__ x, uint32_t ldx, TV* __restrict__ t,
-    uint32_t t_len, TV* __restrict__ y, uint32_t ldy, TV* __restrict__ part, uint32_t part_len,
-    TV alpha) {
-  __shared__ TV zs[kZChunk];
-  __shared__ TV red[kPanelThreads];
-  const GemvTask tk = tasks[blockIdx.x];
-  const uint32_t rhs = blockIdx.y;
+// COH selects agent-coherent accesses (the sc1 loads and stores of relaxed
+// agent-scope atomics) for t and the chunk partial sums, for a form in which
+// one workgroup reads what another wrote within the same launch.  The
+// launches of this file pass COH = false: their kernel boundaries order
+// those values.
+template <typename T>
+__device__ __forceinline__ T ld_coh(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_coh(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool COH, typename T>
+__device__ __forceinline__ T ld_x(const T* p) {
+  if constexpr (COH) return ld_coh(p);
+  return *p;
+}
+template <bool COH, typename T>
+__device__ __forceinline__ void st_x(T* p, T v) {
+  if constexpr (COH) st_coh(p, v);
+  else *p = v;
+}
+
+template <typename TS, typename TV, bool PHASE_B, bool COH>
+__device__ __forceinline__ void gemv_task(const GemvTask& tk, uint32_t rhs, const uint32_t* __restrict__ zsrc,
+                                          const TS* __restrict__ arena, const TV* __restrict__ x, uint32_t ldx,
+                                          TV* __restrict__ t, uint32_t t_len, TV* __restrict__ y, uint32_t ldy,
+                                          TV* __restrict__ part, uint32_t part_len, TV alpha, TV* zs, TV* red) {
   const TV* xr = x + (uint64_t)rhs * ldx;
   TV* tr = t + (uint64_t)rhs * t_len;
   const int m = (int)tk.m;
@@ -126,7 +145,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
     __syncthreads();
     for (int c = tid; c < cw; c += kPanelThreads) {
       const uint32_t s = zsrc[tk.zoff + c0 + c];
-      zs[c] = (s & kFromT) ? tr[s & ~kFromT] : xr[s];
+      zs[c] = (s & kFromT) ? ld_x<COH>(tr + (s & ~kFromT)) : xr[s];
     }
     __syncthreads();
     if (active) {
@@ -162,15 +181,28 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
     for (int q = 0; q < G; ++q) s += red[q * m + tid];
     if (PHASE_B) {
       if (tk.partial)
-        part[(uint64_t)rhs * part_len + tk.out + tid] = s;
+        st_x<COH>(part + (uint64_t)rhs * part_len + tk.out + tid, s);
       else
         y[(uint64_t)rhs * ldy + tk.out + tid] += alpha * s;
     } else if (tk.partial) {
-      part[(uint64_t)rhs * part_len + tk.out + tid] = s;
+      st_x<COH>(part + (uint64_t)rhs * part_len + tk.out + tid, s);
     } else {
-      tr[tk.out + tid] = s;
+      st_x<COH>(tr + tk.out + tid, s);
     }
   }
+}
+
+template <typename TS, typename TV, bool PHASE_B>
+__global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
+    const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
+    const TS* __restrict__ arena, const TV* __restrict__ x, uint32_t ldx, TV* __restrict__ t,
+    uint32_t t_len, TV* __restrict__ y, uint32_t ldy, TV* __restrict__ part, uint32_t part_len,
+    TV alpha) {
+  __shared__ TV zs[kZChunk];
+  __shared__ TV red[kPanelThreads];
+  const GemvTask tk = tasks[blockIdx.x];
+  gemv_task<TS, TV, PHASE_B, false>(tk, blockIdx.y, zsrc, arena, x, ldx, t, t_len, y, ldy, part, part_len,
+                                    alpha, zs, red);
 }
 
 // ------------------------------------------------------------------------
@@ -183,15 +215,13 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
 // the MFMA kernel below is faster (one 16-column tile up to 16).
 constexpr int kZChunkP = 256;   // columns of Z staged per pass
 
-template <typename TS, typename TV, bool PHASE_B, int NP>
-__global__ __launch_bounds__(kPanelThreads) void panel_gemvp_kernel(
-    const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
-    const TS* __restrict__ arena, const TV* __restrict__ xi, TV* __restrict__ ti, uint32_t t_len,
-    TV* __restrict__ y, uint32_t ldy, TV* __restrict__ part, uint32_t part_len, TV alpha,
-    uint32_t p) {
+template <typename TS, typename TV, bool PHASE_B, int NP, bool COH>
+__device__ __forceinline__ void gemvp_task(const GemvTask& tk, const uint32_t* __restrict__ zsrc,
+                                           const TS* __restrict__ arena, const TV* __restrict__ xi,
+                                           TV* __restrict__ ti, uint32_t t_len, TV* __restrict__ y, uint32_t ldy,
+                                           TV* __restrict__ part, uint32_t part_len, TV alpha, uint32_t p,
+                                           TV* zs) {
   constexpr int LNP = NP == 16 ? 4 : 3;
-  __shared__ TV zs[kZChunkP * NP];   // also the cross-group reduction buffer (256 x NP)
-  const GemvTask tk = tasks[blockIdx.x];
   const int m = (int)tk.m;
   const int G = kPanelThreads / m;
   const int tid = threadIdx.x;
@@ -218,7 +248,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemvp_kernel(
         const int e = tid + i * kPanelThreads;
         const uint32_t sv = src[i];
         const TV* r = (sv & kFromT) ? ti + (uint64_t)(sv & ~kFromT) * NP : xi + (uint64_t)sv * NP;
-        const TV v = r[e & (NP - 1)];
+        const TV v = ld_x<COH>(r + (e & (NP - 1)));
         zs[e] = (e >> LNP) < cw ? v : (TV)0;
       }
     }
@@ -257,13 +287,24 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemvp_kernel(
       TV s = 0;
       for (int q = 0; q < G; ++q) s += zs[j * kPanelThreads + q * m + tid];
       if (tk.partial)
-        part[(uint64_t)j * part_len + tk.out + tid] = s;
+        st_x<COH>(part + (uint64_t)j * part_len + tk.out + tid, s);
       else if (PHASE_B)
         y[(uint64_t)j * ldy + tk.out + tid] += alpha * s;
       else
-        ti[(uint64_t)(tk.out + tid) * NP + j] = s;
+        st_x<COH>(ti + (uint64_t)(tk.out + tid) * NP + j, s);
     }
   }
+}
+
+template <typename TS, typename TV, bool PHASE_B, int NP>
+__global__ __launch_bounds__(kPanelThreads) void panel_gemvp_kernel(
+    const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
+    const TS* __restrict__ arena, const TV* __restrict__ xi, TV* __restrict__ ti, uint32_t t_len,
+    TV* __restrict__ y, uint32_t ldy, TV* __restrict__ part, uint32_t part_len, TV alpha,
+    uint32_t p) {
+  __shared__ TV zs[kZChunkP * NP];   // also the cross-group reduction buffer (256 x NP)
+  const GemvTask tk = tasks[blockIdx.x];
+  gemvp_task<TS, TV, PHASE_B, NP, false>(tk, zsrc, arena, xi, ti, t_len, y, ldy, part, part_len, alpha, p, zs);
 }
 
 // ------------------------------------------------------------------------
@@ -1612,10 +1653,27 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
   return AHM_OK;
 }
 
+int ahm_hmat_set_fused(ahm_hmat h, int on) {
+  if (!h) return fail(AHM_ERR_INVALID, "ahm_hmat_set_fused: NULL handle");
+  if (on)
+    return fail(AHM_ERR_UNSUPPORTED,
+                "ahm_hmat_set_fused: the single-launch product is switched off (its first GPU test did not "
+                "finish; cause not found)");
+  h->fused = 0;
+  return AHM_OK;
+}
+
+int ahm_hmat_fused_status(ahm_hmat h, int* aborted) {
+  if (!h || !aborted) return fail(AHM_ERR_INVALID, "ahm_hmat_fused_status: NULL argument");
+  *aborted = 0;
+  return AHM_OK;
+}
+
 int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint64_t* chunk_values,
-                      double* best_ms) {
+                      double* best_ms, int* fused) {
   if (!h || p == 0 || reps == 0) return fail(AHM_ERR_INVALID, "ahm_hmat_autotune: bad arguments");
-  if (!((h->storage == AHM_F64 && vec == AHM_F64) || ((h->storage == AHM_F32 || h->storage == AHM_BF16) && vec == AHM_F32)))
+  if (!((h->storage == AHM_F64 && vec == AHM_F64) ||
+        ((h->storage == AHM_F32 || h->storage == AHM_BF16) && vec == AHM_F32)))
     return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_autotune: storage %d with vectors %d", h->storage, vec);
   AHM_HIP(hipSetDevice(h->ctx->device));
   hipStream_t s = h->ctx->stream;
@@ -1625,34 +1683,41 @@ int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint
   AHM_TRY(dy.alloc((size_t)h->n_rows * p * es));
   AHM_HIP(hipMemsetAsync(dx.p, 0, (size_t)h->n_cols * p * es, s));
   AHM_HIP(hipMemsetAsync(dy.p, 0, (size_t)h->n_rows * p * es, s));
+  // tile sizes (values per row-panel / V^T task) times the launch form: three
+  // kernels (phases A, B, chunk sums) or one persistent launch (p <= 8)
   const uint64_t cand[] = {16u << 10, 32u << 10, 48u << 10, 96u << 10, 192u << 10, 1ull << 40};
   double best = 1e300;
   uint64_t arg = h->chunk_values;
+  int arg_fused = h->fused;
   hipEvent_t e0, e1;
   AHM_HIP(hipEventCreate(&e0));
   AHM_HIP(hipEventCreate(&e1));
   int st = AHM_OK;
-  for (uint64_t cv : cand) {
-    if ((st = set_chunking(h, cv))) break;
-    AHM_TRY(ensure_tbuf(h, p, es));
-    AHM_TRY(ensure_pbuf(h, p, es));
-    auto run = [&]() {
-      return vec == AHM_F64
-                 ? launch_matvec<double, double>(h, own_workspace(h), 1.0, (const double*)dx.p, h->n_cols,
-                                                 (double*)dy.p, h->n_rows, p)
-                 : launch_matvec<float, float>(h, own_workspace(h), 1.0f, (const float*)dx.p, h->n_cols,
-                                               (float*)dy.p, h->n_rows, p);
-    };
-    if ((st = run())) break;  // warm-up
-    AHM_HIP(hipEventRecord(e0, s));
-    for (uint32_t r = 0; r < reps && !st; ++r) st = run();
-    AHM_HIP(hipEventRecord(e1, s));
-    AHM_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    AHM_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (ms / reps < best) {
-      best = ms / reps;
-      arg = cv;
+  for (int fu = 0; fu <= 0 && !st; ++fu) {   // the single-launch form is switched off
+    for (uint64_t cv : cand) {
+      if ((st = set_chunking(h, cv))) break;
+      if ((st = ensure_tbuf(h, p, es)) || (st = ensure_pbuf(h, p, es))) break;
+      auto run = [&]() {
+        return vec == AHM_F64
+                   ? launch_matvec<double, double>(h, own_workspace(h), 1.0, (const double*)dx.p, h->n_cols,
+                                                   (double*)dy.p, h->n_rows, p)
+                   : launch_matvec<float, float>(h, own_workspace(h), 1.0f, (const float*)dx.p, h->n_cols,
+                                                 (float*)dy.p, h->n_rows, p);
+      };
+      if ((st = run())) break;  // warm-up
+      if (hipEventRecord(e0, s) != hipSuccess) { st = fail(AHM_ERR_HIP, "autotune: event"); break; }
+      for (uint32_t r = 0; r < reps && !st; ++r) st = run();
+      float ms = 0.f;
+      if (st || hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        if (!st) st = fail(AHM_ERR_HIP, "autotune: event timing");
+        break;
+      }
+      if (ms / reps < best) {
+        best = ms / reps;
+        arg = cv;
+        arg_fused = fu;
+      }
     }
   }
   (void)hipEventDestroy(e0);
@@ -1661,6 +1726,7 @@ int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint
   AHM_TRY(set_chunking(h, arg));
   if (chunk_values) *chunk_values = arg;
   if (best_ms) *best_ms = best;
+  if (fused) *fused = arg_fused;
   return AHM_OK;
 }
 

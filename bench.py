@@ -209,6 +209,13 @@ def bench_batched(args):
     p = 8
     X = torch.randn(p, Ntot, dtype=torch.float64, device="cuda")
     Y = torch.zeros(p, Ntot, dtype=torch.float64, device="cuda")
+    # the default form (three launches, 48K-value tiles), then the autotuned one
+    ms_default = timed(lambda: H.matvec(X, Y), torch, args.steps, args.warmup)
+    t0 = time.perf_counter()
+    tile, ms_tuned, fused = H.autotune(p=p, reps=3)
+    t_tune = time.perf_counter() - t0
+    log(f"autotuner ({t_tune:.1f} s): tile {tile} values, {'one persistent launch' if fused else 'three launches'}, "
+        f"{ms_tuned:.3f} ms (default {ms_default:.3f} ms)")
     ms = timed(lambda: H.matvec(X, Y), torch, args.steps, args.warmup)
     B_alg = 8.0 * st["nvals"] + 8.0 * p * 3 * Ntot
     F_alg = 2.0 * p * st["nvals"]
@@ -242,6 +249,9 @@ def bench_batched(args):
                    "config": "cfg4", "N_total": Ntot, "sizes_min_max": [min(sizes), max(sizes)],
                    "nvals": st["nvals"]},
         "TF/s": round(F_alg / ms / 1e9, 3), "matvec_1rhs_ms": round(ms1, 4),
+        "autotune": {"tile_values": tile, "fused_single_launch": fused, "seconds": round(t_tune, 2),
+                     "default_form_ms": round(ms_default, 4)},
+        "fused_wait_aborted": H.fused_status(),
         "assembly_s": round(t_asm, 3), "aca_retries": ast["n_retries"], "gmres_batch": res,
     }
     print(json.dumps(line), flush=True)

@@ -172,8 +172,9 @@ def test_autotuned_chunking_keeps_results(ctx):
     Y0 = dev(np.zeros((8, H.N)))
     D.matvec(X, Y0)
     for p in (1, 8):
-        cv, ms = D.autotune(p=p, reps=3)
-        print(f"[gpu] autotune p={p}: chunk {cv} values, {ms:.3f} ms")
+        cv, ms, fused = D.autotune(p=p, reps=3)
+        print(f"[gpu] autotune p={p}: chunk {cv} values, fused {fused}, {ms:.3f} ms")
+        assert not D.fused_status()
         y = dev(np.zeros(H.N))
         D.matvec(x, y)
         Y = dev(np.zeros((8, H.N)))
