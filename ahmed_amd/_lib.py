@@ -95,6 +95,7 @@ SIGNATURES = {
     "ahm_assemble_batch": (_i, [_p, _u32, _p, _p, _d, _u32, _i, _i, C.POINTER(AssemblyStats), C.POINTER(_p)]),
     "ahm_hmat_segments": (_i, [_p, C.POINTER(_u32), _p]),
     "ahm_gmres_batch": (_i, [_p, _p, _p, _d, _u32, _u32, _i, _p, _p, _p]),
+    "ahm_hmat_set_graphs": (_i, [_p, _i]),
     "ahm_rank_predict": (_i, [_p, _p, _u32, _p, C.c_float, _u32, _p, _p, _p]),
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_matvec_noalloc": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
@@ -551,6 +552,11 @@ class HMatrix:
             check(rc, "ahm_gmres")
         return dict(converged=rc == OK, eps=e.value, nsteps=ns.value, nmatvec=mv.value,
                     hist=hist[:nhist].copy())
+
+    def set_graphs(self, on: bool = True):
+        """GMRES steps of this handle replayed from captured hipGraphs (default)
+        or launched one by one."""
+        check(load().ahm_hmat_set_graphs(self.h, int(on)), "ahm_hmat_set_graphs")
 
     def cg(self, b, x, eps: float = 1e-8, nsteps: int = 10000, prec=None):
         e, ns = _d(eps), _u32(nsteps)

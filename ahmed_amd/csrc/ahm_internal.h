@@ -87,6 +87,19 @@ using HostVec = std::vector<T, NoInitAlloc<T>>;
 // zsrc entry: bit 31 set -> index into t, else index into x
 constexpr uint32_t kFromT = 0x80000000u;
 
+// Device-side switch of the product kernels used inside solvers: a
+// workgroup whose task belongs to a frozen system (frozen[seg[task]] != 0,
+// set on the device by the solver's convergence test) returns at once, so
+// steps queued ahead of the host's convergence check cost launches only.
+// frozen == nullptr: no gate; seg == nullptr: every task is system 0's.
+struct Gate {
+  const uint32_t* frozen;
+  const uint32_t* seg;
+};
+__device__ __forceinline__ bool gated(const Gate g) {
+  return g.frozen && g.frozen[g.seg ? g.seg[blockIdx.x] : 0];
+}
+
 // Copy of a rows x cols block from the mblock-layout buffer into the arena.
 // transpose = 0: dst[r + c*dst_ld] = src[r + c*src_ld]
 // transpose = 1: dst[c + r*dst_ld] = src[r + c*src_ld]   (V -> V^T tile)
@@ -132,6 +145,58 @@ struct PhaseTimer {
 // y += alpha H x on a caller-provided workspace (hmat.hip); no allocation
 int matvec_on_workspace(ahm_hmat h, void* base, size_t bytes, double alpha, const void* X, uint32_t ldx,
                         void* Y, uint32_t ldy, uint32_t p, ahm_dtype vec, const char* who);
+
+// Single-RHS products of the solvers (hmat.hip).  matvec_prepare sizes the
+// handle's workspace for them (call it before a graph capture: the gated
+// product allocates nothing and refuses to run on a workspace too small);
+// seg[0..3] give the system of each phase-A, phase-B, phase-A reduction and
+// phase-B reduction task (nullptr: all system 0).  matvec_signature lists
+// every value a p = 1 launch bakes in (task lists, counts, arena and
+// workspace addresses): a captured product stays valid while it is unchanged.
+int matvec_prepare(ahm_hmat h, ahm_dtype vec);
+int matvec_gated(ahm_hmat h, double alpha, const void* x, void* y, ahm_dtype vec, const uint32_t* frozen,
+                 const uint32_t* const seg[4]);
+void matvec_signature(ahm_hmat h, std::vector<uint64_t>& sig);
+
+// solver state kept with a handle between calls (gmres.hip)
+struct GmresCache;
+void gmres_cache_free(ahm_hmat h);
+
+// precond_apply (Include/matrix.h:30) as HLU_solve (Include/H.h:1383-1388):
+// v <- U^{-1} L^{-1} v with the triangular H-solves (trih.hip); either factor
+// may be absent.  Runs on the factors' context stream, which must be the
+// operator's.
+struct Prec {
+  ahm_trih L = nullptr, U = nullptr;
+  uint32_t n = 0;
+  bool on() const { return L || U; }
+  int apply(double* v) const {
+    if (L) AHM_TRY(ahm_trih_solve(L, v, n, 1));
+    if (U) AHM_TRY(ahm_trih_solve(U, v, n, 1));
+    return AHM_OK;
+  }
+};
+
+// Restarted MGS-GMRES(m) (solvers/GMRES.cpp:80-181) on S independent
+// systems (gmres.hip); see the file for the device state and the step
+// protocol.  S = 1 unless batch (the handle's diagonal blocks).
+struct GmresArgs {
+  const double* b = nullptr;
+  double* x = nullptr;
+  double eps = 0.0;
+  uint32_t m = 0, maxsteps = 0;
+  int use_graph = 1;
+  bool batch = false;
+  const Prec* M = nullptr;
+  double* hist = nullptr;     // single system: implicit residual per step
+  uint32_t nhist = 0;
+  // outputs, S entries each (S = 1 unless batch)
+  double* eps_out = nullptr;
+  uint32_t* steps_out = nullptr;
+  int* conv_out = nullptr;
+  uint32_t* nmatvec = nullptr;   // single system: H-matvecs done
+};
+int gmres_run(ahm_hmat h, const GmresArgs& a);
 
 int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_leaf_desc>&& leaves,
                struct DeviceBuf& raw, ahm_dtype data_dtype, ahm_dtype storage, int keep_leaves,
@@ -197,6 +262,8 @@ struct ahm_hmat_s {
     ahm::DeviceBuf contrib, tprime;   // one value per panel column; t' = U^T x
     size_t contrib_cap = 0, tprime_cap = 0;
   } tr;
+  ahm::GmresCache* gm = nullptr;   // GMRES buffers and captured steps (gmres.hip)
+  int solver_graphs = 1;           // GMRES steps replayed as hipGraphs (ahm_hmat_set_graphs)
 };
 
 // triangular H-matrix factor (trih.hip): work queue, arena, counters

@@ -197,9 +197,10 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemv_kernel(
     const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
     const TS* __restrict__ arena, const TV* __restrict__ x, uint32_t ldx, TV* __restrict__ t,
     uint32_t t_len, TV* __restrict__ y, uint32_t ldy, TV* __restrict__ part, uint32_t part_len,
-    TV alpha) {
+    TV alpha, Gate g) {
   __shared__ TV zs[kZChunk];
   __shared__ TV red[kPanelThreads];
+  if (gated(g)) return;
   const GemvTask tk = tasks[blockIdx.x];
   gemv_task<TS, TV, PHASE_B, false>(tk, blockIdx.y, zsrc, arena, x, ldx, t, t_len, y, ldy, part, part_len,
                                     alpha, zs, red);
@@ -608,7 +609,8 @@ __global__ __launch_bounds__(256) void interleave_kernel(const T* __restrict__ X
 template <typename TV, bool ASSIGN>
 __global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
     const ReduceTask* __restrict__ tasks, const TV* __restrict__ part, uint32_t part_len,
-    TV* __restrict__ y, uint32_t ldy, TV alpha, int pw) {
+    TV* __restrict__ y, uint32_t ldy, TV alpha, int pw, Gate g) {
+  if (gated(g)) return;
   const ReduceTask r = tasks[blockIdx.x];
   const uint32_t rhs = blockIdx.y;
   const TV* pr = part + (uint64_t)rhs * part_len + r.part;
@@ -1142,6 +1144,9 @@ struct Workspace {
   void* part;
   void* xi;
   void* xb;   // BF16 copy of xi (AHM_BF16 storage, p >= 9)
+  // single-RHS products of a solver: gates of the phase-A, phase-B,
+  // phase-A reduction and phase-B reduction tasks (none by default)
+  Gate gate[4];
 };
 
 static Workspace own_workspace(ahm_hmat h) { return Workspace{h->tbuf.p, h->pbuf.p, h->xbuf.p, h->xbbuf.p}; }
@@ -1156,15 +1161,18 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
     if (h->n_a)
       hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, false>), dim3(h->n_a, 1), dim3(kPanelThreads), 0, s,
                          (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
-                         (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
+                         (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha,
+                         ws.gate[0]);
     if (h->n_ra)
       hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, 1), dim3(kPanelThreads), 0, s,
-                         (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha, 0);
+                         (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha, 0,
+                         ws.gate[2]);
     if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
     if (h->n_b)
       hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, true>), dim3(h->n_b, 1), dim3(kPanelThreads), 0, s,
                          (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
-                         (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha);
+                         (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha,
+                         ws.gate[1]);
   } else {
     // p >= 2 on interleaved copies of x and t: p <= 8 with the streaming
     // kernel, wider with MFMA panel products (32 right-hand sides per
@@ -1185,7 +1193,7 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
         if (h->n_ra)
           hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, p), dim3(kPanelThreads), 0, s,
                              (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t,
-                             h->t_len, alpha, pw);
+                             h->t_len, alpha, pw, Gate{});
         if (ev_mid) (void)hipEventRecord(ev_mid, s);
         if (h->n_b)
           hipLaunchKernelGGL((panel_gemvp_kernel<TS, TV, true, NP>), dim3(h->n_b), dim3(kPanelThreads), 0,
@@ -1215,7 +1223,7 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
       if (h->n_ra)
         hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, p), dim3(kPanelThreads), 0, s,
                            (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha,
-                           pw);
+                           pw, Gate{});
       if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
       if (h->n_b)
         hipLaunchKernelGGL((panel_mm_kernel<TV, true>), dim3(h->n_b, ng), dim3(kPanelThreads), 0, s,
@@ -1226,7 +1234,8 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
   }
   if (h->n_r)
     hipLaunchKernelGGL((chunk_reduce_kernel<TV, false>), dim3(h->n_r, p), dim3(kPanelThreads), 0, s,
-                       (const ReduceTask*)h->tasks_r.p, (const TV*)part, h->part_len, Y, ldy, alpha, 0);
+                       (const ReduceTask*)h->tasks_r.p, (const TV*)part, h->part_len, Y, ldy, alpha, 0,
+                       p == 1 ? ws.gate[3] : Gate{});
   AHM_HIP(hipGetLastError());
   return AHM_OK;
 }
@@ -1258,6 +1267,37 @@ int matvec_on_workspace(ahm_hmat h, void* base, size_t bytes, double alpha, cons
   if (vec == AHM_F64)
     return launch_matvec<double, double>(h, ws, alpha, (const double*)X, ldx, (double*)Y, ldy, p);
   return launch_matvec<float, float>(h, ws, (float)alpha, (const float*)X, ldx, (float*)Y, ldy, p);
+}
+
+int matvec_prepare(ahm_hmat h, ahm_dtype vec) {
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  AHM_TRY(ensure_tbuf(h, 1, dtype_size(vec)));
+  AHM_TRY(ensure_pbuf(h, 1, dtype_size(vec)));
+  return AHM_OK;
+}
+
+int matvec_gated(ahm_hmat h, double alpha, const void* x, void* y, ahm_dtype vec, const uint32_t* frozen,
+                 const uint32_t* const seg[4]) {
+  const size_t es = dtype_size(vec);
+  if (ws_t_bytes(h, 1, es) > h->tbuf_cap || ws_part_bytes(h, 1, es) > h->pbuf_cap)
+    return fail(AHM_ERR_INVALID, "gated product: workspace not prepared");
+  Workspace ws = own_workspace(h);
+  for (int q = 0; q < 4; ++q) ws.gate[q] = Gate{frozen, seg ? seg[q] : nullptr};
+  if (h->storage == AHM_F64 && vec == AHM_F64)
+    return launch_matvec<double, double>(h, ws, alpha, (const double*)x, h->n_cols, (double*)y, h->n_rows, 1);
+  if (h->storage != AHM_F64 && vec == AHM_F32)
+    return launch_matvec<float, float>(h, ws, (float)alpha, (const float*)x, h->n_cols, (float*)y, h->n_rows,
+                                       1);
+  return fail(AHM_ERR_UNSUPPORTED, "gated product: storage %d with vectors %d", h->storage, vec);
+}
+
+void matvec_signature(ahm_hmat h, std::vector<uint64_t>& sig) {
+  sig = {(uint64_t)(uintptr_t)h->tasks_a.p, (uint64_t)(uintptr_t)h->tasks_b.p,
+         (uint64_t)(uintptr_t)h->tasks_ra.p, (uint64_t)(uintptr_t)h->tasks_r.p,
+         (uint64_t)(uintptr_t)h->zsrc.p,    (uint64_t)(uintptr_t)h->arena.p,
+         (uint64_t)(uintptr_t)h->tbuf.p,    (uint64_t)(uintptr_t)h->pbuf.p,
+         h->n_a, h->n_b, h->n_ra, h->n_r, h->t_len, h->part_len, h->n_rows, h->n_cols,
+         (uint64_t)h->storage, h->chunk_values};
 }
 
 }  // namespace ahm
@@ -1530,6 +1570,7 @@ int ahm_hmat_destroy(ahm_hmat h) {
   if (!h) return AHM_OK;
   (void)hipSetDevice(h->ctx->device);
   (void)hipStreamSynchronize(h->ctx->stream);
+  gmres_cache_free(h);
   delete h;
   return AHM_OK;
 }

@@ -1,22 +1,19 @@
-// krylov.hip -- GMRES(m), CG and FP64 iterative refinement on the GPU.
+// krylov.hip -- the GMRES entry points, CG and FP64 iterative refinement on
+// the GPU.
 //
-// GMRes follows solvers/GMRES.cpp:80-181 step for step: restarted,
-// right-preconditioned (no preconditioner here), modified Gram-Schmidt,
-// Givens rotations, convergence on the implicit residual |s_{i+1}|/|b|, x
-// updated at convergence or restart, true residual recomputed per restart.
-// CG follows solvers/CG.cpp:31-113.
+// GMRes (solvers/GMRES.cpp:80-181) runs in gmres.hip, with the Hessenberg
+// matrix, the Givens rotations and the convergence test on the device; this
+// file maps the reference's in/out arguments onto it.  CG follows
+// solvers/CG.cpp:31-113.
 //
 // Vectors, the orthogonalisation and the rotations stay FP64 on the device
 // for every matrix precision; with an FP32 matrix only the H-matvec runs in
 // FP32 (the Krylov vector is rounded to FP32 on the way in, the product
-// widened to FP64 on the way out).  The (m+1) x m Hessenberg column and the
-// Givens rotations live on the host (O(m) work per step; one device->host
-// copy per Arnoldi step carries the new column and the residual estimate).
+// widened to FP64 on the way out).
 //
 // Reductions are deterministic: a fixed grid of kRedBlocks workgroups, each
 // with a fixed element set, writes one partial per workgroup; consumers sum
-// the partials in a fixed order.  The MGS step k fuses "h_k = <w, v_k>
-// (from partials), w -= h_k v_k, partials of <w, v_{k+1}>" in one launch.
+// the partials in a fixed order.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -76,48 +73,11 @@ __global__ __launch_bounds__(kRedThreads) void asum_kernel(const double* __restr
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// h = sum(part_in); out[0] = h (block 0); w -= h v; part_out = <w, next> (or <w,w>)
-__global__ __launch_bounds__(kRedThreads) void mgs_kernel(double* __restrict__ w,
-                                                          const double* __restrict__ v,
-                                                          const double* __restrict__ next,
-                                                          uint32_t n,
-                                                          const double* __restrict__ part_in,
-                                                          double* __restrict__ hout,
-                                                          double* __restrict__ part_out) {
-  const double h = sum_partials(part_in);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
-  double s = 0.0;
-  for (uint32_t i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += kRedBlocks * kRedThreads) {
-    const double wi = fma(-h, v[i], w[i]);
-    w[i] = wi;
-    s = fma(wi, next ? next[i] : wi, s);
-  }
-  s = group_sum(s);
-  if (threadIdx.x == 0) part_out[blockIdx.x] = s;
-}
-
-// h = sqrt(sum(part)); out = h; w *= 1/h  (Arnoldi normalisation)
-__global__ __launch_bounds__(kRedThreads) void normalize_kernel(double* __restrict__ w, uint32_t n,
-                                                                const double* __restrict__ part,
-                                                                double* __restrict__ hout) {
-  const double h = sqrt(sum_partials(part));
-  if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
-  const double inv = 1.0 / h;
-  for (uint32_t i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += kRedBlocks * kRedThreads)
-    w[i] *= inv;
-}
-
 // result = sum(part) into out (single workgroup)
 __global__ __launch_bounds__(kRedThreads) void finish_kernel(const double* __restrict__ part,
                                                              double* __restrict__ out) {
   const double h = sum_partials(part);
   if (threadIdx.x == 0) *out = h;
-}
-
-__global__ void scale_copy_kernel(const double* __restrict__ src, double* __restrict__ dst,
-                                  uint32_t n, double a) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    dst[i] = a * src[i];
 }
 
 __global__ void fill_kernel(double* __restrict__ d, uint32_t n, double v) {
@@ -133,16 +93,6 @@ __global__ void to_f32_kernel(const double* __restrict__ s, float* __restrict__ 
 __global__ void add_f32_kernel(const float* __restrict__ s, double* __restrict__ d, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     d[i] += (double)s[i];
-}
-
-// x += V y, V n x k column-major (ld n), y in device memory (GMRES update)
-__global__ void gemv_update_kernel(const double* __restrict__ V, uint32_t n, uint32_t k,
-                                   const double* __restrict__ y, double* __restrict__ x) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (uint32_t l = 0; l < k; ++l) s = fma(V[(uint64_t)l * n + i], y[l], s);
-    x[i] += s;
-  }
 }
 
 // y = a*x + y
@@ -182,21 +132,6 @@ struct Op {
     AHM_HIP(hipMemsetAsync(y32.p, 0, (size_t)n * 4, s));
     AHM_TRY(ahm_hmat_matvec(h, alpha, x32.p, n, y32.p, n, 1, AHM_F32));
     hipLaunchKernelGGL(add_f32_kernel, dim3(kEwBlocks), dim3(256), 0, s, (const float*)y32.p, y, n);
-    return AHM_OK;
-  }
-};
-
-// precond_apply (Include/matrix.h:30) as HLU_solve (Include/H.h:1383-1388):
-// v <- U^{-1} L^{-1} v with the triangular H-solves (trih.hip); either factor
-// may be absent.  Runs on the factors' context stream, which must be the
-// operator's.
-struct Prec {
-  ahm_trih L = nullptr, U = nullptr;
-  uint32_t n = 0;
-  bool on() const { return L || U; }
-  int apply(double* v) const {
-    if (L) AHM_TRY(ahm_trih_solve(L, v, n, 1));
-    if (U) AHM_TRY(ahm_trih_solve(U, v, n, 1));
     return AHM_OK;
   }
 };
@@ -248,161 +183,7 @@ int dot(hipStream_t s, Red& R, const double* a, const double* b, uint32_t n, dou
   return AHM_OK;
 }
 
-// genPlRot / applPlRot, solvers/GMRES.cpp:33-55
-void gen_rot(double dx, double dy, double& cs, double& sn) {
-  if (dy == 0.0) {
-    cs = 1.0;
-    sn = 0.0;
-  } else if (std::fabs(dy) > std::fabs(dx)) {
-    const double t = dx / dy;
-    sn = 1.0 / std::sqrt(1.0 + t * t);
-    cs = t * sn;
-  } else {
-    const double t = dy / dx;
-    cs = 1.0 / std::sqrt(1.0 + t * t);
-    sn = t * cs;
-  }
-}
-void app_rot(double& dx, double& dy, double cs, double sn) {
-  const double t = cs * dx + sn * dy;
-  dy = cs * dy - sn * dx;
-  dx = t;
-}
-
 }  // namespace
-
-// Restarted MGS-GMRES(m) on the device.  b, x: FP64 device vectors (x in/out).
-// Returns AHM_OK on convergence, AHM_ERR_NOT_CONVERGED otherwise (eps, nsteps
-// then hold the achieved residual and the steps taken, as the reference's
-// return value 1 does).
-int gmres_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m,
-                 uint32_t* nsteps, double* hist, uint32_t nhist, uint32_t* nmatvec,
-                 const Prec& M = Prec()) {
-  const uint32_t n = h->n_rows;
-  hipStream_t s = h->ctx->stream;
-  Op A{h, n};
-  AHM_TRY(A.init());
-  Red R;
-  AHM_TRY(R.init(m + 2));
-  DeviceBuf dV, dr, dy, dxh;
-  AHM_TRY(dV.alloc((size_t)n * (m + 1) * 8));
-  AHM_TRY(dr.alloc((size_t)n * 8));
-  AHM_TRY(dy.alloc((size_t)(m + 1) * 8));
-  if (M.on()) AHM_TRY(dxh.alloc((size_t)n * 8));
-  double* V = (double*)dV.p;
-  double* r = (double*)dr.p;
-  double* xh = (double*)dxh.p;
-  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m + 1), sn(m + 1), sv(m + 1), col(m + 2);
-  uint32_t nh = 0, mv = 0;
-  auto update = [&](uint32_t k) -> int {
-    // solve the k x k upper triangle H y = s (dtrtrs 'U','N','N'), x += V y
-    std::vector<double> y(sv.begin(), sv.begin() + k);
-    for (int i = (int)k - 1; i >= 0; --i) {
-      double t = y[i];
-      for (uint32_t l = i + 1; l < k; ++l) t -= H[i + (size_t)l * (m + 1)] * y[l];
-      y[i] = t / H[i + (size_t)i * (m + 1)];
-    }
-    AHM_HIP(hipMemcpyAsync(dy.p, y.data(), k * 8, hipMemcpyHostToDevice, s));
-    if (M.on()) {
-      // x += M V y (solvers/GMRES.cpp:70-73)
-      hipLaunchKernelGGL(fill_kernel, dim3(kEwBlocks), dim3(256), 0, s, xh, n, 0.0);
-      hipLaunchKernelGGL(gemv_update_kernel, dim3(kEwBlocks), dim3(256), 0, s, V, n, k,
-                         (const double*)dy.p, xh);
-      AHM_TRY(M.apply(xh));
-      hipLaunchKernelGGL(axpy_kernel, dim3(kEwBlocks), dim3(256), 0, s, 1.0, (const double*)xh, x, n);
-    } else {
-      hipLaunchKernelGGL(gemv_update_kernel, dim3(kEwBlocks), dim3(256), 0, s, V, n, k,
-                         (const double*)dy.p, x);
-    }
-    AHM_HIP(hipStreamSynchronize(s));  // y lives on the host stack
-    return AHM_OK;
-  };
-  double normb;
-  AHM_TRY(norm2(s, R, b, n, &normb));
-  if (normb == 0.0) {
-    hipLaunchKernelGGL(fill_kernel, dim3(kEwBlocks), dim3(256), 0, s, x, n, 0.0);
-    *eps = 0.0;
-    *nsteps = 0;
-    AHM_HIP(hipStreamSynchronize(s));
-    return AHM_OK;
-  }
-  // r = b - A x
-  AHM_TRY(initial_residual(s, R, A, b, x, r, n, mv));
-  double beta, resid;
-  AHM_TRY(norm2(s, R, r, n, &beta));
-  if ((resid = beta / normb) <= *eps) {
-    *eps = resid;
-    *nsteps = 0;
-    if (nmatvec) *nmatvec = mv;
-    return AHM_OK;
-  }
-  uint32_t j = 1;
-  const uint32_t maxsteps = *nsteps;
-  while (j <= maxsteps) {
-    hipLaunchKernelGGL(scale_copy_kernel, dim3(kEwBlocks), dim3(256), 0, s, r, V, n, 1.0 / beta);
-    sv[0] = beta;
-    for (uint32_t q = 1; q <= m; ++q) sv[q] = 0.0;
-    uint32_t i;
-    for (i = 0; i < m && j <= maxsteps; i++, j++) {
-      double* vi = V + (size_t)i * n;
-      double* w = V + (size_t)(i + 1) * n;
-      hipLaunchKernelGGL(fill_kernel, dim3(kEwBlocks), dim3(256), 0, s, w, n, 0.0);
-      if (M.on()) {
-        // w = A M v_i (solvers/GMRES.cpp:125-130)
-        AHM_HIP(hipMemcpyAsync(xh, vi, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-        AHM_TRY(M.apply(xh));
-        AHM_TRY(A.amux(1.0, xh, w));
-      } else {
-        AHM_TRY(A.amux(1.0, vi, w));
-      }
-      ++mv;
-      // MGS: k = 0..i, fused (dot from partials, axpy, next dot)
-      hipLaunchKernelGGL(dot_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, w, V, n, R.p0());
-      double* pin = R.p0();
-      double* pout = R.p1();
-      for (uint32_t k = 0; k <= i; ++k) {
-        const double* next = (k < i) ? V + (size_t)(k + 1) * n : nullptr;
-        hipLaunchKernelGGL(mgs_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, w,
-                           V + (size_t)k * n, next, n, pin, R.s() + k, pout);
-        std::swap(pin, pout);
-      }
-      hipLaunchKernelGGL(normalize_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, w, n, pin,
-                         R.s() + i + 1);
-      AHM_HIP(hipMemcpyAsync(col.data(), R.s(), (i + 2) * 8, hipMemcpyDeviceToHost, s));
-      AHM_HIP(hipStreamSynchronize(s));
-      double* Hc = H.data() + (size_t)i * (m + 1);
-      for (uint32_t k = 0; k <= i + 1; ++k) Hc[k] = col[k];
-      for (uint32_t k = 0; k < i; ++k) app_rot(Hc[k], Hc[k + 1], cs[k], sn[k]);
-      gen_rot(Hc[i], Hc[i + 1], cs[i], sn[i]);
-      app_rot(Hc[i], Hc[i + 1], cs[i], sn[i]);
-      app_rot(sv[i], sv[i + 1], cs[i], sn[i]);
-      resid = std::fabs(sv[i + 1] / normb);
-      if (hist && nh < nhist) hist[nh++] = resid;
-      if (resid < *eps) {
-        AHM_TRY(update(i + 1));
-        *eps = resid;
-        *nsteps = j;
-        if (nmatvec) *nmatvec = mv;
-        return AHM_OK;
-      }
-    }
-    // the reference passes m here; i < m only when nsteps ran out mid-cycle
-    AHM_TRY(update(i));
-    AHM_HIP(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-    AHM_TRY(A.amux(-1.0, x, r));
-    ++mv;
-    AHM_TRY(norm2(s, R, r, n, &beta));
-    if ((resid = beta / normb) < *eps) {
-      *eps = resid;
-      *nsteps = j;
-      if (nmatvec) *nmatvec = mv;
-      return AHM_OK;
-    }
-  }
-  *eps = resid;  // nsteps keeps its input value, as in the reference
-  if (nmatvec) *nmatvec = mv;
-  return AHM_ERR_NOT_CONVERGED;
-}
 
 // CG, solvers/CG.cpp:31-113; with M, r^ = M r (precond_apply) each step.
 int cg_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nsteps,
@@ -468,11 +249,45 @@ int cg_device(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nst
   return AHM_ERR_NOT_CONVERGED;
 }
 
+// One system through gmres_run (gmres.hip) with the reference's in/out
+// arguments: eps, nsteps in; achieved residual and steps out.
+int gmres_single(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m, uint32_t* nsteps,
+                 double* hist, uint32_t nhist, uint32_t* nmatvec, const Prec* M = nullptr) {
+  GmresArgs a;
+  a.b = b;
+  a.x = x;
+  a.eps = *eps;
+  a.m = m;
+  a.maxsteps = *nsteps;
+  a.use_graph = h->solver_graphs;
+  a.M = M;
+  a.hist = hist;
+  a.nhist = hist ? nhist : 0;
+  double e = 0.0;
+  uint32_t steps = 0;
+  int conv = 0;
+  a.eps_out = &e;
+  a.steps_out = &steps;
+  a.conv_out = &conv;
+  a.nmatvec = nmatvec;
+  const int st = gmres_run(h, a);
+  if (st != AHM_OK && st != AHM_ERR_NOT_CONVERGED) return st;
+  *eps = e;
+  *nsteps = steps;
+  return st;
+}
+
 }  // namespace ahm
 
 using namespace ahm;
 
 extern "C" {
+
+int ahm_hmat_set_graphs(ahm_hmat h, int on) {
+  if (!h) return fail(AHM_ERR_INVALID, "ahm_hmat_set_graphs: NULL handle");
+  h->solver_graphs = on ? 1 : 0;
+  return AHM_OK;
+}
 
 int ahm_gmres(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m, uint32_t* nsteps,
               double* hist, uint32_t nhist, uint32_t* nmatvec) {
@@ -480,7 +295,7 @@ int ahm_gmres(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m, u
     return fail(AHM_ERR_INVALID, "ahm_gmres: bad arguments");
   if (h->n_rows != h->n_cols) return fail(AHM_ERR_INVALID, "ahm_gmres: matrix is not square");
   AHM_HIP(hipSetDevice(h->ctx->device));
-  const int st = gmres_device(h, b, x, eps, m, nsteps, hist, nhist, nmatvec);
+  const int st = gmres_single(h, b, x, eps, m, nsteps, hist, nhist, nmatvec);
   if (st == AHM_ERR_NOT_CONVERGED)
     set_error("ahm_gmres: no convergence in %u steps (residual %.3e)", *nsteps, *eps);
   return st;
@@ -520,7 +335,7 @@ int ahm_gmres_prec(ahm_hmat h, ahm_trih L, ahm_trih U, const double* b, double* 
   Prec M;
   AHM_TRY(prec_of(h, L, U, M, "ahm_gmres_prec"));
   AHM_HIP(hipSetDevice(h->ctx->device));
-  const int st = gmres_device(h, b, x, eps, m, nsteps, hist, nhist, nmatvec, M);
+  const int st = gmres_single(h, b, x, eps, m, nsteps, hist, nhist, nmatvec, &M);
   if (st == AHM_ERR_NOT_CONVERGED)
     set_error("ahm_gmres_prec: no convergence in %u steps (residual %.3e)", *nsteps, *eps);
   return st;
@@ -593,7 +408,7 @@ int ahm_gmres_ir(ahm_hmat hi, ahm_hmat lo, const double* b, double* x, double* e
     hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, s, d, n, 0.0);
     double ie = std::max(inner_eps, 0.5 * (*eps) / resid);
     uint32_t ns = inner_steps, mvs = 0;
-    const int gs = gmres_device(lo, r, d, &ie, m, &ns, nullptr, 0, &mvs);
+    const int gs = gmres_single(lo, r, d, &ie, m, &ns, nullptr, 0, &mvs);
     if (gs != AHM_OK && gs != AHM_ERR_NOT_CONVERGED) return gs;
     inner_total += ns;
     hipLaunchKernelGGL(axpy_kernel, dim3(1024), dim3(256), 0, s, 1.0, (const double*)d, x, n);

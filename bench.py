@@ -185,6 +185,39 @@ def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, y_gpu_of, budget_s=20.0, 
     return out
 
 
+def small_system_gmres(ctx, torch):
+    """GMRES(30) to 1e-8 on the reference's own example (unit sphere N=8192,
+    leaf 32, eta 0.8, ACA eps 1e-6, Laplace with quadrature weight 4 pi/N,
+    b = A*1), where a step's launches, not its product, set the step time:
+    captured steps replayed as hipGraphs against one-by-one launches.  Each
+    mode: one untimed solve, then the median of 5 solves."""
+    from ahmed_amd import _lib, geometry
+    N = 8192
+    tree = _lib.Tree(geometry.sphere(N, 1), 32, 0.8)
+    H, _ = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_LAPLACE, 0.0, 4 * np.pi / N, 1e-6)
+    b = torch.zeros(N, dtype=torch.float64, device="cuda")
+    H.matvec(torch.ones(N, dtype=torch.float64, device="cuda"), b)
+    out = {"workload": "unit sphere N=8192, leaf 32, eta 0.8, ACA eps 1e-6, Laplace 4 pi/N, GMRES(30) to 1e-8"}
+    for graph in (True, False):
+        H.set_graphs(graph)
+        dts = []
+        for rep in range(6):
+            xs = torch.zeros(N, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = H.gmres(b, xs, eps=1e-8, m=30, nsteps=300)
+            torch.cuda.synchronize()
+            dts.append(time.perf_counter() - t0)
+        t = float(np.median(dts[1:]))
+        out["graph" if graph else "eager"] = {"seconds": round(t, 5), "iterations": r["nsteps"],
+                                              "converged": r["converged"],
+                                              "us_per_step": round(t / max(r["nsteps"], 1) * 1e6, 1)}
+        log(f"GMRES(30) N=8192 {'graph' if graph else 'eager'}: {t * 1e3:.2f} ms, {r['nsteps']} steps")
+    out["graph_speedup_vs_eager"] = round(out["eager"]["seconds"] / out["graph"]["seconds"], 3)
+    del H
+    return out
+
+
 def bench_batched(args):
     """64 independent H-matrices, N log-uniform in [4096, 65536] on seeded
     ellipsoids with axis ratios U(1,4), kernel exp(-k r)/(4 pi r) with
@@ -226,19 +259,29 @@ def bench_batched(args):
         f"1 RHS {ms1:.3f} ms ({(8.0 * st['nvals'] + 24.0 * Ntot) / ms1 / 1e6:.1f} GB/s)")
     b = torch.zeros(Ntot, dtype=torch.float64, device="cuda")
     H.matvec(torch.ones(Ntot, dtype=torch.float64, device="cuda"), b)
+    # each mode twice: the first graph call captures the Arnoldi steps (kept
+    # with the handle), the second replays them
     res = {}
     for graph in (True, False):
-        xs = torch.zeros(Ntot, dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r = H.gmres_batch(b, xs, eps=1e-8, m=50, nsteps=2000, use_graph=graph)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        res["graph" if graph else "eager"] = dict(seconds=round(dt, 4), all_converged=bool(r["converged"].all()),
+        dts = []
+        for rep in range(2):
+            xs = torch.zeros(Ntot, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = H.gmres_batch(b, xs, eps=1e-8, m=50, nsteps=2000, use_graph=graph)
+            torch.cuda.synchronize()
+            dts.append(time.perf_counter() - t0)
+        res["graph" if graph else "eager"] = dict(seconds=round(dts[1], 4), first_call_s=round(dts[0], 4),
+                                                  all_converged=bool(r["converged"].all()),
                                                   max_steps=int(r["nsteps"].max()),
                                                   mean_steps=float(r["nsteps"].mean()),
                                                   max_err_vs_ones=float((xs - 1).abs().max()))
-        log(f"batched GMRES(50) {'graph' if graph else 'eager'}: {dt:.3f} s, steps max {r['nsteps'].max()}")
+        log(f"batched GMRES(50) {'graph' if graph else 'eager'}: {dts[1]:.4f} s "
+            f"(first call {dts[0]:.4f} s), steps max {r['nsteps'].max()} mean {r['nsteps'].mean():.2f}")
+    res["graph_speedup_vs_eager"] = round(res["eager"]["seconds"] / res["graph"]["seconds"], 3)
+    # converged systems are frozen on the device: their product tasks stop,
+    # so the products cost about mean/max of a lockstep solve's
+    res["lockstep_products_avoided"] = round(1.0 - res["graph"]["mean_steps"] / res["graph"]["max_steps"], 3)
     line = {
         "metric": "batched H-matvec HBM GB/s (64 independent H-matrices, 8 RHS)",
         "value": round(B_alg / ms / 1e6, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -472,6 +515,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override N")
     ap.add_argument("--rhs", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--slow-alpha", type=float, default=100.0,
+                    help="also solve with the off-diagonal weight times this (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a plain 2 GiB read+write kernel (scale for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -612,19 +657,27 @@ def main():
     # ---- GMRES(50) time-to-tolerance: FP64 vs FP64 refinement around FP32 ----
     b = torch.zeros(N, dtype=torch.float64, device="cuda")
     H.matvec(torch.ones(N, dtype=torch.float64, device="cuda"), b)   # b = H * 1
+    # each solve twice: the first call captures the Arnoldi steps into
+    # hipGraphs kept with the handle, the timed second call replays them;
+    # "fp64_eager" launches the same kernels one by one
     gm = {}
-    for mode in ("fp64", "mixed"):
-        xs = torch.zeros(N, dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        if mode == "fp64":
-            r = H.gmres(b, xs, eps=tol, m=50, nsteps=2000)
-            info = {"iterations": r["nsteps"], "matvecs": r["nmatvec"]}
-        else:
-            r = H.gmres_ir(H32, b, xs, eps=tol, m=50, inner_eps=1e-5, inner_steps=2000)
-            info = {"refinements": r["outer"], "inner_iterations": r["inner"]}
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
+    for mode in ("fp64", "fp64_eager", "mixed"):
+        H.set_graphs(mode != "fp64_eager")
+        dts = []
+        for rep in range(2):
+            xs = torch.zeros(N, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if mode != "mixed":
+                r = H.gmres(b, xs, eps=tol, m=50, nsteps=2000)
+                info = {"iterations": r["nsteps"], "matvecs": r["nmatvec"]}
+            else:
+                r = H.gmres_ir(H32, b, xs, eps=tol, m=50, inner_eps=1e-5, inner_steps=2000)
+                info = {"refinements": r["outer"], "inner_iterations": r["inner"]}
+            torch.cuda.synchronize()
+            dts.append(time.perf_counter() - t0)
+        dt = dts[1]
+        info["first_call_s"] = round(dts[0], 4)
         res = torch.zeros(N, dtype=torch.float64, device="cuda")
         res.copy_(b)
         H.matvec(xs, res, -1.0)
@@ -633,8 +686,48 @@ def main():
         gm[mode] = dict(seconds=round(dt, 4), converged=r["converged"], reported_residual=r["eps"],
                         true_residual_fp64=true, max_err_vs_ones=err, **info)
         log(f"GMRES(50) {mode}: {dt:.3f} s, residual {true:.2e}, {info}")
+    H.set_graphs(True)
     gm["tolerance"] = tol
     gm["speedup_mixed_vs_fp64"] = round(gm["fp64"]["seconds"] / gm["mixed"]["seconds"], 3)
+    gm["graph_speedup_vs_eager"] = round(gm["fp64_eager"]["seconds"] / gm["fp64"]["seconds"], 3)
+    # the same geometry with a stronger off-diagonal part (weight alpha * 6/N):
+    # a system that needs tens of GMRES iterations, where the refinement's
+    # FP32 inner products can outweigh its FP64 residual products
+    if args.slow_alpha > 0:
+        del H32
+        ctx.release_cached()
+        Hs, _ = _lib.HMatrix.assemble(ctx, tree, _lib.KERNEL_LAPLACE, 0.0, args.slow_alpha * weight, eps)
+        Hs32 = Hs.convert()
+        bs = torch.zeros(N, dtype=torch.float64, device="cuda")
+        Hs.matvec(torch.ones(N, dtype=torch.float64, device="cuda"), bs)
+        slow = {"weight": f"{args.slow_alpha:g} x the weight above", "tolerance": tol}
+        for mode in ("fp64", "mixed"):
+            dts = []
+            for rep in range(2):
+                xs = torch.zeros(N, dtype=torch.float64, device="cuda")
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                if mode == "fp64":
+                    r = Hs.gmres(bs, xs, eps=tol, m=50, nsteps=5000)
+                    info = {"iterations": r["nsteps"], "matvecs": r["nmatvec"]}
+                else:
+                    r = Hs.gmres_ir(Hs32, bs, xs, eps=tol, m=50, inner_eps=1e-5, inner_steps=5000)
+                    info = {"refinements": r["outer"], "inner_iterations": r["inner"]}
+                torch.cuda.synchronize()
+                dts.append(time.perf_counter() - t0)
+            dt = dts[1]
+            info["first_call_s"] = round(dts[0], 4)
+            res = bs.clone()
+            Hs.matvec(xs, res, -1.0)
+            slow[mode] = dict(seconds=round(dt, 4), converged=r["converged"],
+                              true_residual_fp64=float(res.norm() / bs.norm()), **info)
+            log(f"GMRES(50) {mode}, weight x{args.slow_alpha:g}: {dt:.3f} s, {info}")
+        slow["speedup_mixed_vs_fp64"] = round(slow["fp64"]["seconds"] / slow["mixed"]["seconds"], 3)
+        slow["speedup_bound_from_products"] = round(
+            slow["fp64"]["matvecs"] * ms / (slow["mixed"]["refinements"] * ms + slow["mixed"]["inner_iterations"] * ms32), 3)
+        gm["slower_system"] = slow
+        del Hs, Hs32, bs
+    gm["small_system"] = small_system_gmres(ctx, torch)
     # what the products alone allow: the FP64 solve's products against the
     # refinement's FP64 residual products plus its FP32 inner products
     n64 = gm["fp64"]["matvecs"]

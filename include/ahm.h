@@ -308,9 +308,17 @@ int ahm_hmat_matvec_t_host(ahm_hmat h, double alpha, const double* X, uint32_t l
  * Krylov vectors, orthogonalisation and rotations stay FP64.
  * hist (host, may be NULL) receives up to nhist residual estimates;
  * nmatvec (may be NULL) the number of H-matvecs. */
-/* GMRes, solvers/GMRES.cpp:80-181 (MGS + Givens, restart length m) */
+/* GMRes, solvers/GMRES.cpp:80-181 (MGS + Givens, restart length m).  The
+ * Hessenberg matrix, the rotations and the convergence test run on the
+ * device; each Arnoldi step is replayed from a hipGraph captured on its
+ * second use (ahm_hmat_set_graphs).  The Krylov basis (n x (m+1) FP64) and the
+ * graphs stay with the handle between calls. */
 int ahm_gmres(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m,
               uint32_t* nsteps, double* hist, uint32_t nhist, uint32_t* nmatvec);
+/* on != 0 (the default): the GMRES entry points of this handle replay
+ * captured Arnoldi steps; 0: they launch the same kernels one by one.
+ * No counterpart in the reference. */
+int ahm_hmat_set_graphs(ahm_hmat h, int on);
 /* CG, solvers/CG.cpp:31-113 */
 int ahm_cg(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nsteps);
 /* FP64 iterative refinement: r = b - A64 x (FP64 H-matvec), inner GMRES(m)
@@ -324,12 +332,14 @@ int ahm_gmres_ir(ahm_hmat hi, ahm_hmat lo, const double* b, double* x, double* e
 
 /* GMRes(m) on every independent system of a batched matrix
  * (ahm_assemble_batch) at once: per-system Krylov spaces, rotations and
- * convergence tests (solvers/GMRES.cpp:80-181 each), Arnoldi steps in
- * lockstep so one H-matvec launch per phase serves all systems; with
- * use_graph != 0 each Arnoldi step is captured once into a hipGraph and
- * replayed.  b, x: FP64 device vectors over all systems.  Per system
- * (may be NULL): achieved residual, steps (the reference's nsteps output),
- * converged flag.  AHM_OK if every system converged. */
+ * convergence tests (solvers/GMRES.cpp:80-181 each) on the device, Arnoldi
+ * steps in lockstep so one H-matvec launch per phase serves all systems; a
+ * converged system is frozen on the device and its product tasks skipped.
+ * With use_graph != 0 each Arnoldi step is captured into a hipGraph on its
+ * second use (kept with the handle) and replayed.  b, x: FP64 device
+ * vectors over all systems.  Per system (may be NULL): achieved residual, steps (the
+ * reference's nsteps output), converged flag.  AHM_OK if every system
+ * converged. */
 int ahm_gmres_batch(ahm_hmat h, const double* b, double* x, double eps, uint32_t m,
                     uint32_t nsteps, int use_graph, double* eps_out, uint32_t* steps_out,
                     int* converged);

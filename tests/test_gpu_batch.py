@@ -5,9 +5,10 @@ at once (lockstep Arnoldi steps, optionally hipGraph-captured).
 
 Checks: every diagonal block equals the single-matrix assembly bit for bit;
 the batched product equals the per-matrix products bit for bit; batched
-GMRES solves every system with the step count of its single solve (+-1) and
-its solution within 1e-8; graph-replayed and eager batched GMRES agree bit
-for bit (every replay is checked through the final result and steps)."""
+GMRES gives every system the step count and, bit for bit, the solution of
+its single solve (same reduction chunks, products and rotations; converged
+systems frozen on the device); graph-replayed and eager batched GMRES agree
+bit for bit (every replay is checked through the final result and steps)."""
 import numpy as np
 import pytest
 
@@ -86,7 +87,8 @@ def test_batched_assembly_matvec_and_gmres(ctx):
     for s, Hs in enumerate(singles):
         xs = torch.zeros(int(off[s + 1] - off[s]), dtype=torch.float64, device="cuda")
         r1 = Hs.gmres(b[off[s]:off[s + 1]].contiguous(), xs, eps=1e-10, m=30, nsteps=500)
+        # same chunks, products and rotations as the system alone: bit for bit
         assert r1["converged"]
-        assert abs(int(r1["nsteps"]) - int(rg["nsteps"][s])) <= 1
-        assert float((xs - xg[off[s]:off[s + 1]]).abs().max()) <= 1e-8
+        assert int(r1["nsteps"]) == int(rg["nsteps"][s])
+        assert torch.equal(xs, xg[off[s]:off[s + 1]])
     print(f"[gpu] batch of {len(probs)}: steps {rg['nsteps'].tolist()}")

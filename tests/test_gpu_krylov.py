@@ -76,6 +76,45 @@ def test_gmres_edge_cases(ctx):
     assert abs(true - r["eps"]) <= 1e-6 * true
 
 
+def test_gmres_graph_replay_matches_eager(ctx):
+    """Arnoldi steps replayed from captured hipGraphs give the one-by-one
+    launches' iterates bit for bit, on the capturing call, on later calls,
+    and after the product's workspace moves (graphs captured again)."""
+    _, T, H = sphere_hmatrix(32768, 64, 1e-8)
+    D = upload(ctx, H)
+    b = dev(H.matvec(np.random.default_rng(5).standard_normal(H.N)))
+
+    def solve(graph):
+        D.set_graphs(graph)
+        x = dev(np.zeros(H.N))
+        r = D.gmres(b, x, eps=1e-10, m=8, nsteps=500, nhist=200)
+        return r, x
+
+    re, xe = solve(False)
+    print(f"[gpu] GMRES(8) eager: {re['nsteps']} steps, {re['nmatvec']} products", flush=True)
+    assert re["converged"] and re["nsteps"] > 8   # restarted
+    # products: one per step and one per restart ((nsteps - 1) // m restarts)
+    assert re["nmatvec"] == re["nsteps"] + (re["nsteps"] - 1) // 8
+    # the implicit residuals decrease (Givens on the device) and end below eps
+    h = re["hist"][:re["nsteps"]]
+    assert np.all(np.diff(h) <= 1e-15) and h[-1] < 1e-10
+    for tag in ("capture", "replay"):
+        rg, xg = solve(True)
+        print(f"[gpu] GMRES(8) graph ({tag}): {rg['nsteps']} steps", flush=True)
+        assert rg["converged"] and rg["nsteps"] == re["nsteps"] and rg["eps"] == re["eps"]
+        assert rg["nmatvec"] == re["nmatvec"]
+        assert np.array_equal(rg["hist"], re["hist"])
+        assert torch.equal(xg, xe)
+    # a 32-RHS product grows the shared workspace: the steps are captured again
+    X = torch.randn(32, H.N, dtype=torch.float64, device="cuda")
+    Y = torch.zeros(32, H.N, dtype=torch.float64, device="cuda")
+    D.matvec(X, Y)
+    rg, xg = solve(True)
+    assert rg["nsteps"] == re["nsteps"] and torch.equal(xg, xe)
+    true = np.linalg.norm(b.cpu().numpy() - H.matvec(xe.cpu().numpy())) / np.linalg.norm(b.cpu().numpy())
+    assert true <= 1.01e-10
+
+
 def test_fp32_gmres_and_refinement_reach_fp64_accuracy(ctx):
     _, T, H = sphere_hmatrix(32768, 64, 1e-8)
     D = upload(ctx, H)
