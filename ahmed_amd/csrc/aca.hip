@@ -31,6 +31,14 @@ namespace ahm {
 
 constexpr int kAcaThreads = 256;
 constexpr int kWaves = kAcaThreads / 64;
+// The ACA kernel runs leaves of up to kSmallLeaf rows + columns one
+// wavefront each (no workgroup barriers to wait on, four times as many leaves
+// per CU), larger leaves with kAcaThreads threads; its helpers take the
+// workgroup size from blockDim.
+constexpr uint32_t kSmallLeaf = 1024;
+constexpr int kSmallThreads = 64;
+__device__ __forceinline__ uint32_t nthr() { return blockDim.x; }
+__device__ __forceinline__ int nwaves() { return (int)(blockDim.x >> 6); }
 
 __device__ __forceinline__ double kentry(const KernelSpec& K, const double* __restrict__ P,
                                          uint32_t i, uint32_t j) {
@@ -82,7 +90,7 @@ __device__ double block_sum(double v, double* sh) {
   if ((threadIdx.x & 63) == 0) sh[w] = v;
   __syncthreads();
   double s = sh[0];
-  for (int q = 1; q < kWaves; ++q) s += sh[q];
+  for (int q = 1; q < nwaves(); ++q) s += sh[q];
   return s;
 }
 
@@ -109,7 +117,7 @@ __device__ ArgMax block_argmax(ArgMax a, double* shv, int* shi) {
   }
   __syncthreads();
   ArgMax r{shv[0], shi[0]};
-  for (int q = 1; q < kWaves; ++q) r = amax(r, ArgMax{shv[q], shi[q]});
+  for (int q = 1; q < nwaves(); ++q) r = amax(r, ArgMax{shv[q], shi[q]});
   return r;
 }
 
@@ -131,7 +139,7 @@ __device__ void block_argmin_int(int& key, int& idx, int* shk, int* shi) {
   __syncthreads();
   key = shk[0];
   idx = shi[0];
-  for (int q = 1; q < kWaves; ++q)
+  for (int q = 1; q < nwaves(); ++q)
     if (shk[q] < key || (shk[q] == key && shi[q] < idx)) {
       key = shk[q];
       idx = shi[q];
@@ -196,12 +204,24 @@ __device__ __forceinline__ void house_axpy(double* x, const double* a, double s,
   for (; i < n; i += step) x[i] -= s * (i == j ? 1.0 : a[i]);
 }
 
+// ACA steps with at most kFuseK earlier rank columns form the dot products
+// of the new vectors with the earlier ones (the cross term of the Frobenius
+// norm update) inside the remainder passes, from the values those passes
+// load anyway, so the factors are streamed once per step instead of twice.
+constexpr uint32_t kFuseK = 16;
+
 struct Shared {
   double dv[kWaves];
   double av[kWaves];
   int ai[kWaves];
   int ki[kWaves];
   int kk[kWaves];
+};
+
+// LDS of the fused dots (ACA kernel only)
+struct DotShared {
+  double dw[kWaves][kFuseK];   // per-wave partial dots
+  double dA[kFuseK], dB[kFuseK], dC[kFuseK];
 };
 
 // ------------------------------------------------------------------------
@@ -225,10 +245,10 @@ __device__ ArgMax remainder_row(const KernelSpec& K, const double* P, const AcaB
                                 uint32_t i0, uint32_t k, const double* U, const double* V,
                                 double* pv, const int* S, double* ucoef, Shared& sh) {
   __syncthreads();
-  for (uint32_t l = threadIdx.x; l < k; l += kAcaThreads) ucoef[l] = U[(size_t)l * B.n1 + i0];
+  for (uint32_t l = threadIdx.x; l < k; l += nthr()) ucoef[l] = U[(size_t)l * B.n1 + i0];
   __syncthreads();
   ArgMax a{-1.0, 0x7fffffff};
-  for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads) {
+  for (uint32_t j = threadIdx.x; j < B.n2; j += nthr()) {
     const double v = sub_terms(kentry(K, P, B.b1 + i0, B.b2 + j), ucoef, V, B.n2, j, k);
     pv[j] = v;
     if (S[j] >= 0) a = amax(a, ArgMax{fabs(v), (int)j});
@@ -241,10 +261,10 @@ __device__ ArgMax remainder_col(const KernelSpec& K, const double* P, const AcaB
                                 uint32_t j0, uint32_t k, const double* U, const double* V,
                                 double* pu, const int* Z, double* vcoef, Shared& sh) {
   __syncthreads();
-  for (uint32_t l = threadIdx.x; l < k; l += kAcaThreads) vcoef[l] = V[(size_t)l * B.n2 + j0];
+  for (uint32_t l = threadIdx.x; l < k; l += nthr()) vcoef[l] = V[(size_t)l * B.n2 + j0];
   __syncthreads();
   ArgMax a{-1.0, 0x7fffffff};
-  for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads) {
+  for (uint32_t i = threadIdx.x; i < B.n1; i += nthr()) {
     const double v = sub_terms(kentry(K, P, B.b1 + i, B.b2 + j0), vcoef, U, B.n1, i, k);
     pu[i] = v;
     if (Z[i] >= 0) a = amax(a, ArgMax{fabs(v), (int)i});
@@ -252,16 +272,124 @@ __device__ ArgMax remainder_col(const KernelSpec& K, const double* P, const AcaB
   return block_argmax(a, sh.av, sh.ai);
 }
 
+// Per-thread dot accumulators of the fused remainder passes, in dynamic LDS
+// (as registers they would halve the kernel's occupancy): element
+// acc[l * blockDim + thread], kFuseK x blockDim values.
+typedef double* DotAcc;
+
+// acc[l][thread] += M_l[i] * v for l < k <= kFuseK, eight loads in flight
+__device__ __forceinline__ void dots_acc(DotAcc acc, double v, const double* M, size_t ld, uint32_t i,
+                                         uint32_t k) {
+  const uint32_t t = threadIdx.x, nt = nthr();
+  uint32_t l = 0;
+  for (; l + 8 <= k; l += 8) {
+    double m[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = M[(size_t)(l + u) * ld + i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[(l + u) * nt + t] = fma(m[u], v, acc[(l + u) * nt + t]);
+  }
+  for (; l < k; ++l) acc[l * nt + t] = fma(M[(size_t)l * ld + i], v, acc[l * nt + t]);
+}
+
+__device__ __forceinline__ void dots_zero(DotAcc acc, uint32_t k) {
+  for (uint32_t l = 0; l < k; ++l) acc[l * nthr() + threadIdx.x] = 0.0;
+}
+
+// out[l] = sum of acc[l][.] over the workgroup (l < k): lanes by wave_sum,
+// then waves in order; ends with a barrier
+__device__ void block_dots(DotAcc acc, uint32_t k, double* out, DotShared& sh) {
+  const int w = threadIdx.x >> 6;
+  for (uint32_t l = 0; l < k; ++l) {
+    const double s = wave_sum(acc[l * nthr() + threadIdx.x]);
+    if ((threadIdx.x & 63) == 0) sh.dw[w][l] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < k) {
+    double s = sh.dw[0][threadIdx.x];
+    for (int q = 1; q < nwaves(); ++q) s += sh.dw[q][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// remainder_row for k <= kFuseK, with the dots of the unscaled row pv with
+// the earlier V columns: dA[l] over the live columns (S >= 0, the ones the
+// pivot scaling reaches), dB[l] over the retired ones.  pv and the arg-max
+// are those of remainder_row bit for bit (same terms in the same order).
+// The dots re-read the column's V entries (cache hits) after pv is stored.
+__device__ ArgMax remainder_row_dots(const KernelSpec& K, const double* P, const AcaBlock& B,
+                                     uint32_t i0, uint32_t k, const double* U, const double* V,
+                                     double* pv, const int* S, double* ucoef, Shared& sh, DotAcc acc,
+                                     DotShared& ds) {
+  __syncthreads();
+  for (uint32_t l = threadIdx.x; l < k; l += nthr()) ucoef[l] = U[(size_t)l * B.n1 + i0];
+  __syncthreads();
+  ArgMax a{-1.0, 0x7fffffff};
+  dots_zero(acc, k);
+  bool retired = false;
+  for (uint32_t j = threadIdx.x; j < B.n2; j += nthr()) {
+    const double v = sub_terms(kentry(K, P, B.b1 + i0, B.b2 + j), ucoef, V, B.n2, j, k);
+    pv[j] = v;
+    if (S[j] >= 0) {
+      a = amax(a, ArgMax{fabs(v), (int)j});
+      dots_acc(acc, v, V, B.n2, j, k);
+    } else {
+      retired = true;
+    }
+  }
+  block_dots(acc, k, ds.dA, ds);
+  // the retired columns (at most a few per leaf) in a second, sparse pass
+  dots_zero(acc, k);
+  if (retired)
+    for (uint32_t j = threadIdx.x; j < B.n2; j += nthr())
+      if (S[j] < 0) dots_acc(acc, pv[j], V, B.n2, j, k);
+  block_dots(acc, k, ds.dB, ds);
+  return block_argmax(a, sh.av, sh.ai);
+}
+
+// remainder_col for k <= kFuseK, with dC[l] = U_l . pu (unscaled pu)
+__device__ ArgMax remainder_col_dots(const KernelSpec& K, const double* P, const AcaBlock& B,
+                                     uint32_t j0, uint32_t k, const double* U, const double* V,
+                                     double* pu, const int* Z, double* vcoef, Shared& sh, DotAcc acc,
+                                     DotShared& ds) {
+  __syncthreads();
+  for (uint32_t l = threadIdx.x; l < k; l += nthr()) vcoef[l] = V[(size_t)l * B.n2 + j0];
+  __syncthreads();
+  ArgMax a{-1.0, 0x7fffffff};
+  dots_zero(acc, k);
+  for (uint32_t i = threadIdx.x; i < B.n1; i += nthr()) {
+    const double v = sub_terms(kentry(K, P, B.b1 + i, B.b2 + j0), vcoef, U, B.n1, i, k);
+    pu[i] = v;
+    if (Z[i] >= 0) a = amax(a, ArgMax{fabs(v), (int)i});
+    dots_acc(acc, v, U, B.n1, i, k);
+  }
+  block_dots(acc, k, ds.dC, ds);
+  return block_argmax(a, sh.av, sh.ai);
+}
+
+// sum_{l<k} (U_l.U_k)(V_l.V_k) (Include/ACA.h:257-260) from the fused dots:
+// U_k = dsca * pu everywhere, V_k = sca * pv on the live columns and pv on
+// the retired ones
+__device__ __forceinline__ double fused_cross(const DotShared& sh, uint32_t k, double sca, double dsca) {
+  double sum = 0.0;
+  for (uint32_t l = 0; l < k; ++l) sum += (dsca * sh.dC[l]) * fma(sca, sh.dA[l], sh.dB[l]);
+  return sum;
+}
+
 // ACA_row_step (Include/ACA.h:46-130).  Returns false if no pivot was found.
+// With k <= kFuseK the cross term of the norm update comes back in cross.
 __device__ bool aca_row_step(const KernelSpec& K, const double* P, const AcaBlock& B,
                              uint32_t& klast, uint32_t& i0, uint32_t k, AcaState& st, int* Z,
                              int* S, double* U, double* V, double scale, double* coef,
-                             Shared& sh) {
+                             Shared& sh, DotAcc dacc, DotShared& ds, double& cross) {
   double* pu = U + (size_t)k * B.n1;
   double* pv = V + (size_t)k * B.n2;
+  const bool fuse = k <= kFuseK;
   ArgMax m;
   for (;;) {
-    m = remainder_row(K, P, B, i0, k, U, V, pv, S, coef, sh);
+    m = fuse ? remainder_row_dots(K, P, B, i0, k, U, V, pv, S, coef, sh, dacc, ds)
+             : remainder_row(K, P, B, i0, k, U, V, pv, S, coef, sh);
     if (!(fmax(m.v, 0.0) < 1e-14 * scale)) break;
     __syncthreads();
     if (threadIdx.x == 0) Z[i0] = -1;
@@ -278,7 +406,7 @@ __device__ bool aca_row_step(const KernelSpec& K, const double* P, const AcaBloc
   const double sca = sqrtpiv / pv[j0];
   __syncthreads();
   double nv = 0.0;
-  for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads)
+  for (uint32_t j = threadIdx.x; j < B.n2; j += nthr())
     if (S[j] >= 0) {
       const double e = (pv[j] *= sca);
       nv += e * e;
@@ -286,7 +414,10 @@ __device__ bool aca_row_step(const KernelSpec& K, const double* P, const AcaBloc
     }
   const double nrmv2 = block_sum(nv, sh.dv);
   // column j0 of the remainder; pivot row/column retired afterwards
-  remainder_col(K, P, B, j0, k, U, V, pu, Z, coef, sh);
+  if (fuse)
+    remainder_col_dots(K, P, B, j0, k, U, V, pu, Z, coef, sh, dacc, ds);
+  else
+    remainder_col(K, P, B, j0, k, U, V, pu, Z, coef, sh);
   __syncthreads();
   klast = (uint32_t)Z[i0];
   __syncthreads();
@@ -299,7 +430,7 @@ __device__ bool aca_row_step(const KernelSpec& K, const double* P, const AcaBloc
   const double dsca = 1.0 / sqrtpiv;
   double nu = 0.0;
   ArgMax a{-1.0, 0x7fffffff};
-  for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads) {
+  for (uint32_t i = threadIdx.x; i < B.n1; i += nthr()) {
     const double eabs = fabs(pu[i] *= dsca);
     if (Z[i] >= 0 && eabs > thr) ++Z[i];
     nu += eabs * eabs;
@@ -309,6 +440,7 @@ __device__ bool aca_row_step(const KernelSpec& K, const double* P, const AcaBloc
   a = block_argmax(a, sh.av, sh.ai);
   if (a.v > 0.0) i0 = (uint32_t)a.i;
   st.nrmlsk2 = nrmu2 * nrmv2;
+  if (fuse) cross = fused_cross(ds, k, sca, dsca);
   return true;
 }
 
@@ -336,7 +468,7 @@ __device__ bool aca_col_step(const KernelSpec& K, const double* P, const AcaBloc
   const double sca = sqrtpiv / pu[i0];
   __syncthreads();
   double nu = 0.0;
-  for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads)
+  for (uint32_t i = threadIdx.x; i < B.n1; i += nthr())
     if (Z[i] >= 0) {
       const double e = (pu[i] *= sca);
       nu += e * e;
@@ -354,7 +486,7 @@ __device__ bool aca_col_step(const KernelSpec& K, const double* P, const AcaBloc
   const double dsca = 1.0 / sqrtpiv;
   double nv = 0.0;
   ArgMax a{-1.0, 0x7fffffff};
-  for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads) {
+  for (uint32_t j = threadIdx.x; j < B.n2; j += nthr()) {
     const double eabs = fabs(pv[j] *= dsca);
     if (S[j] >= 0 && eabs > thr) ++S[j];
     nv += eabs * eabs;
@@ -371,12 +503,14 @@ __device__ bool aca_col_step(const KernelSpec& K, const double* P, const AcaBloc
 // pivot candidate (S stays zero) and no success counters are kept
 __device__ bool acar_step(const KernelSpec& K, const double* P, const AcaBlock& B, uint32_t& i0,
                           uint32_t k, AcaState& st, int* Z, const int* S, double* U, double* V,
-                          double scale, double* coef, Shared& sh) {
+                          double scale, double* coef, Shared& sh, DotAcc dacc, DotShared& ds, double& cross) {
   double* pu = U + (size_t)k * B.n1;
   double* pv = V + (size_t)k * B.n2;
+  const bool fuse = k <= kFuseK;
   ArgMax m;
   for (;;) {
-    m = remainder_row(K, P, B, i0, k, U, V, pv, S, coef, sh);
+    m = fuse ? remainder_row_dots(K, P, B, i0, k, U, V, pv, S, coef, sh, dacc, ds)
+             : remainder_row(K, P, B, i0, k, U, V, pv, S, coef, sh);
     if (!(fmax(m.v, 0.0) < 1e-14 * scale)) break;
     __syncthreads();
     if (threadIdx.x == 0) Z[i0] = -1;
@@ -391,12 +525,15 @@ __device__ bool acar_step(const KernelSpec& K, const double* P, const AcaBlock& 
   const double sca = sqrtpiv / pv[j0];
   __syncthreads();
   double nv = 0.0;
-  for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads) {
+  for (uint32_t j = threadIdx.x; j < B.n2; j += nthr()) {
     const double e = (pv[j] *= sca);
     nv += e * e;
   }
   const double nrmv2 = block_sum(nv, sh.dv);
-  remainder_col(K, P, B, j0, k, U, V, pu, Z, coef, sh);
+  if (fuse)
+    remainder_col_dots(K, P, B, j0, k, U, V, pu, Z, coef, sh, dacc, ds);
+  else
+    remainder_col(K, P, B, j0, k, U, V, pu, Z, coef, sh);
   __syncthreads();
   if (threadIdx.x == 0) Z[i0] = -1;
   __syncthreads();
@@ -404,7 +541,7 @@ __device__ bool acar_step(const KernelSpec& K, const double* P, const AcaBlock& 
   const double dsca = 1.0 / sqrtpiv;
   double nu = 0.0;
   ArgMax a{-1.0, 0x7fffffff};
-  for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads) {
+  for (uint32_t i = threadIdx.x; i < B.n1; i += nthr()) {
     const double eabs = fabs(pu[i] *= dsca);
     nu += eabs * eabs;
     if (Z[i] >= 0) a = amax(a, ArgMax{eabs, (int)i});
@@ -413,17 +550,19 @@ __device__ bool acar_step(const KernelSpec& K, const double* P, const AcaBlock& 
   a = block_argmax(a, sh.av, sh.ai);
   if (a.v > 0.0) i0 = (uint32_t)a.i;
   st.nrmlsk2 = nrmu2 * nrmv2;
+  if (fuse) cross = fused_cross(ds, k, sca, dsca);
   return true;
 }
 
 // sum_{l<k} (U_l.U_k)(V_l.V_k): one wavefront per term, lanes over rows
+// (steps with more than kFuseK earlier columns)
 __device__ double cross_sum(const AcaBlock& B, uint32_t k, const double* U, const double* V,
                             Shared& sh) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double* uk = U + (size_t)k * B.n1;
   const double* vk = V + (size_t)k * B.n2;
   double acc = 0.0;  // per-wave sum of its terms, in l order
-  for (uint32_t l = w; l < k; l += kWaves) {
+  for (uint32_t l = w; l < k; l += nwaves()) {
     double du = strided_dot(U + (size_t)l * B.n1, uk, lane, B.n1, 64, 0.0);
     double dv = strided_dot(V + (size_t)l * B.n2, vk, lane, B.n2, 64, 0.0);
     du = wave_sum(du);
@@ -434,7 +573,7 @@ __device__ double cross_sum(const AcaBlock& B, uint32_t k, const double* U, cons
   return block_sum(acc, sh.dv);
 }
 
-__global__ __launch_bounds__(kAcaThreads) void aca_kernel(const AcaBlock* __restrict__ blocks,
+__global__ __launch_bounds__(kAcaThreads, 4) void aca_kernel(const AcaBlock* __restrict__ blocks,
                                                           AcaResult* __restrict__ res,
                                                           const double* __restrict__ P,
                                                           const KernelSpec* __restrict__ Ks,
@@ -443,14 +582,16 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(const AcaBlock* __rest
                                                           int* __restrict__ iscr) {
   __shared__ Shared sh;
   __shared__ double coef[kAcaCapMax + 1];
+  __shared__ DotShared ds;
+  extern __shared__ double dacc[];   // kFuseK x blockDim dot accumulators
   const AcaBlock B = blocks[blockIdx.x];
   const KernelSpec K = Ks[B.kidx];
   double* U = slab + B.uoff;
   double* V = slab + B.voff;
   int* Z = iscr + B.zoff;
   int* S = Z + B.n1;
-  for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads) Z[i] = 0;
-  for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads) S[j] = 0;
+  for (uint32_t i = threadIdx.x; i < B.n1; i += nthr()) Z[i] = 0;
+  for (uint32_t j = threadIdx.x; j < B.n2; j += nthr()) S[j] = 0;
   __syncthreads();
   AcaState st{0, 0.0};
   int kmin = -2, keps = -1;
@@ -466,12 +607,13 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(const AcaBlock* __rest
         status = 2;
         goto DONE;
       }
-      if (!acar_step(K, P, B, i0, k, st, Z, S, U, V, scale, coef, sh)) {
+      double cross = 0.0;
+      if (!acar_step(K, P, B, i0, k, st, Z, S, U, V, scale, coef, sh, dacc, ds, cross)) {
         st.nrmlsk2 = 0.0;
         goto SUCC;
       }
       {
-        const double sum = cross_sum(B, k, U, V, sh);
+        const double sum = k <= kFuseK ? cross : cross_sum(B, k, U, V, sh);
         nrms2 += 2.0 * sum + st.nrmlsk2;
       }
       ++k;
@@ -485,12 +627,13 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(const AcaBlock* __rest
       goto DONE;
     }
     inext = i0;
-    if (!aca_row_step(K, P, B, klast, inext, k, st, Z, S, U, V, scale, coef, sh)) {
+    double cross = 0.0;
+    if (!aca_row_step(K, P, B, klast, inext, k, st, Z, S, U, V, scale, coef, sh, dacc, ds, cross)) {
       st.nrmlsk2 = 0.0;
       goto SUCC;
     }
     {
-      const double sum = cross_sum(B, k, U, V, sh);
+      const double sum = k <= kFuseK ? cross : cross_sum(B, k, U, V, sh);
       nrms2 += 2.0 * sum + st.nrmlsk2;
     }
     if (st.nrmlsk2 < eps * eps * nrms2 && keps == -1) keps = (int)(klast + 1u);
@@ -498,7 +641,7 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(const AcaBlock* __rest
     scale = sqrt(st.nrmlsk2 / ((double)B.n1 * B.n2));
     if (keps > 0) {
       int key = 0x7fffffff, idx = 0x7fffffff;
-      for (uint32_t i = threadIdx.x; i < B.n1; i += kAcaThreads)
+      for (uint32_t i = threadIdx.x; i < B.n1; i += nthr())
         if (Z[i] >= 0 && (Z[i] < key || (Z[i] == key && (int)i < idx))) {
           key = Z[i];
           idx = (int)i;
@@ -520,7 +663,7 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(const AcaBlock* __rest
   do {
     {
       int key = 0x7fffffff, idx = 0x7fffffff;
-      for (uint32_t j = threadIdx.x; j < B.n2; j += kAcaThreads)
+      for (uint32_t j = threadIdx.x; j < B.n2; j += nthr())
         if (S[j] >= 0 && (S[j] < key || (S[j] == key && (int)j < idx))) {
           key = S[j];
           idx = (int)j;
@@ -557,6 +700,7 @@ DONE:
     res[blockIdx.x].status = status;
     res[blockIdx.x].k = k;
     res[blockIdx.x].kt = 0;
+    res[blockIdx.x].pad = 0;   // recompression method (set by recompress_kernel)
   }
 }
 
@@ -913,13 +1057,13 @@ __device__ void jacobi_svd(double* R, uint32_t mm, uint32_t nn, double* Zm, Shar
 // sorted, truncated core factors in the scratch: W = R Z (columns scaled by
 // the singular values, mmin x kt, ld mmin) and Z (nmin x kt, ld nmin).  U, V,
 // tau1, tau2, R and Zm are the leaf's working arrays, in LDS or in the scratch.
-__device__ __forceinline__ void rc_body(const AcaBlock& B, AcaResult* __restrict__ res, uint32_t k, double eps,
-                                        double* U, double* V, double* R, double* Zm, double* Ws, double* Zs,
-                                        double* tau1, double* tau2, Shared& sh, double* wbuf, double* sig,
-                                        int* order) {
-  house_qr(U, B.n1, k, tau1, sh, wbuf);
-  house_qr(V, B.n2, k, tau2, sh, wbuf);
-  const uint32_t mmin = min(B.n1, k), nmin = min(B.n2, k), amin = min(mmin, nmin);
+// The core of the recompression from the triangular factors R1 (rows
+// 0..mmin-1 of the upper triangle of U, ld ld1) and R2 (of V, ld ld2).
+__device__ __forceinline__ void rc_core(const AcaBlock& B, AcaResult* __restrict__ res, uint32_t k, double eps,
+                                        const double* U, size_t ld1, uint32_t mmin, const double* V, size_t ld2,
+                                        uint32_t nmin, double* R, double* Zm, double* Ws, double* Zs, Shared& sh,
+                                        double* sig, int* order) {
+  const uint32_t amin = min(mmin, nmin);
   // R = R1 R2^T (utrmmh, Include/blas.h:2767-2781)
   double fs = 0.0;
   for (uint32_t e = threadIdx.x; e < mmin * nmin; e += kAcaThreads) {
@@ -930,13 +1074,13 @@ __device__ __forceinline__ void rc_body(const AcaBlock& B, AcaResult* __restrict
       double x[8], y[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        x[u] = U[(size_t)(l + u) * B.n1 + i];
-        y[u] = V[(size_t)(l + u) * B.n2 + j];
+        x[u] = U[(size_t)(l + u) * ld1 + i];
+        y[u] = V[(size_t)(l + u) * ld2 + j];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) d += x[u] * y[u];
     }
-    for (; l < k; ++l) d += U[(size_t)l * B.n1 + i] * V[(size_t)l * B.n2 + j];
+    for (; l < k; ++l) d += U[(size_t)l * ld1 + i] * V[(size_t)l * ld2 + j];
     R[e] = d;
     fs += d * d;
   }
@@ -977,11 +1121,217 @@ __device__ __forceinline__ void rc_body(const AcaBlock& B, AcaResult* __restrict
   if (threadIdx.x == 0) res[blockIdx.x].kt = kt;
 }
 
+// QR of U and V (Householder, in place), then the core: leaves the sorted,
+// truncated core factors in the scratch: W = R Z (columns scaled by the
+// singular values, mmin x kt, ld mmin) and Z (nmin x kt, ld nmin).
+__device__ __forceinline__ void rc_body(const AcaBlock& B, AcaResult* __restrict__ res, uint32_t k, double eps,
+                                        double* U, double* V, double* R, double* Zm, double* Ws, double* Zs,
+                                        double* tau1, double* tau2, Shared& sh, double* wbuf, double* sig,
+                                        int* order) {
+  house_qr(U, B.n1, k, tau1, sh, wbuf);
+  house_qr(V, B.n2, k, tau2, sh, wbuf);
+  rc_core(B, res, k, eps, U, B.n1, min(B.n1, k), V, B.n2, min(B.n2, k), R, Zm, Ws, Zs, sh, sig, order);
+}
+
+// ------------------------------------------------------------------------
+// Factors of at most kCq columns: Cholesky QR twice (CholeskyQR2; Fukaya,
+// Nakatsukasa, Yanagisawa and Yamamoto, 2014).  F (n x k) -> F1 = F R1^{-1}
+// with R1^T R1 = F^T F, then R2^T R2 = F1^T F1; Q = F1 R2^{-1} and
+// R = R2 R1.  The factor is read three times and written once, against once
+// per reflector for the Householder QR; each Gram matrix is one MFMA pass.
+// A factor too close to rank deficiency for the Cholesky factor to be
+// trusted (a pivot below 1e-12 of its column's squared norm) keeps the
+// Householder QR.  The expansion then forms U' = F1 (R2^{-1} W) directly.
+constexpr uint32_t kCq = 16;
+
+// G (16 x 16, ld 16) = F^T F for F (n x k, ld n), k <= 16, on
+// v_mfma_f64_16x16x4f64: wavefront w takes 16-row groups w, w+4, ...;
+// lane l loads rows 4(l/16) .. 4(l/16)+3 of column l%16 of the group and
+// feeds row 4(l/16)+s to k-step s (the operand layout of wy_apply_big);
+// the wavefronts' tiles are summed in LDS in wavefront order.  part holds
+// kWaves x 256 values.
+__device__ void cq_gram(const double* F, uint32_t n, uint32_t k, double* G, double* part) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t li = lane & 15, kk = lane >> 4, ng = (n + 15) / 16;
+  aca_f64x4 acc = aca_f64x4{0.0, 0.0, 0.0, 0.0};
+  for (uint32_t g = w; g < ng; g += kWaves) {
+    const uint32_t mb = g * 16 + 4 * kk;
+    double y[4];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const uint32_t m = mb + st;
+      y[st] = (li < k && m < n) ? F[(size_t)li * n + m] : 0.0;
+    }
+#pragma unroll
+    for (int st = 0; st < 4; ++st) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(y[st], y[st], acc, 0, 0, 0);
+  }
+  // value v of lane l: G[kk + 4v][li] of this wavefront's rows
+#pragma unroll
+  for (int v = 0; v < 4; ++v) part[w * 256 + (kk + 4 * v) + li * 16] = acc[v];
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < 256; e += kAcaThreads) {
+    double t = part[e];
+    for (int q = 1; q < kWaves; ++q) t += part[q * 256 + e];
+    G[e] = t;
+  }
+  __syncthreads();
+}
+
+// R (k x k upper, ld 16, zero below) with R^T R = G; false when a pivot is
+// not above 1e-12 of its column's diagonal entry of G.  One thread.
+__device__ bool cq_chol(const double* G, uint32_t k, double* R) {
+  for (uint32_t j = 0; j < k; ++j) {
+    for (uint32_t i = j + 1; i < k; ++i) R[i + 16 * j] = 0.0;
+    double d = G[j + 16 * j];
+    for (uint32_t p = 0; p < j; ++p) d -= R[p + 16 * j] * R[p + 16 * j];
+    if (!(d > 1e-12 * G[j + 16 * j])) return false;
+    const double r = sqrt(d);
+    R[j + 16 * j] = r;
+    for (uint32_t i = j + 1; i < k; ++i) {
+      double t = G[j + 16 * i];
+      for (uint32_t p = 0; p < j; ++p) t -= R[p + 16 * j] * R[p + 16 * i];
+      R[j + 16 * i] = t / r;
+    }
+  }
+  return true;
+}
+
+// Ri = R^{-1} (k x k upper, ld 16).  One thread.
+__device__ void cq_inv(const double* R, uint32_t k, double* Ri) {
+  for (uint32_t j = 0; j < k; ++j) {
+    for (uint32_t i = j + 1; i < k; ++i) Ri[i + 16 * j] = 0.0;
+    Ri[j + 16 * j] = 1.0 / R[j + 16 * j];
+    for (int i = (int)j - 1; i >= 0; --i) {
+      double t = 0.0;
+      for (uint32_t p = (uint32_t)i + 1; p <= j; ++p) t = fma(R[i + 16 * p], Ri[p + 16 * j], t);
+      Ri[i + 16 * j] = -t / R[i + 16 * i];
+    }
+  }
+}
+
+// C = A B for upper triangular A, B (k x k, ld 16).  One thread.
+__device__ void cq_mul_upper(const double* A, const double* Bm, uint32_t k, double* Cm) {
+  for (uint32_t j = 0; j < k; ++j)
+    for (uint32_t i = 0; i < k; ++i) {
+      double t = 0.0;
+      for (uint32_t p = i; p <= j; ++p) t = fma(A[i + 16 * p], Bm[p + 16 * j], t);
+      Cm[i + 16 * j] = i <= j ? t : 0.0;
+    }
+}
+
+// F <- F T in place for upper triangular T (k x k, ld 16 in LDS), one row
+// per thread (row c of the result needs entries 0..c of the row: columns
+// are rewritten from the last)
+__device__ void cq_apply(double* F, uint32_t n, uint32_t k, const double* T) {
+  for (uint32_t i = threadIdx.x; i < n; i += kAcaThreads) {
+    double f[kCq];
+#pragma unroll
+    for (uint32_t l = 0; l < kCq; ++l) f[l] = l < k ? F[(size_t)l * n + i] : 0.0;
+#pragma unroll
+    for (int c = (int)kCq - 1; c >= 0; --c) {
+      if ((uint32_t)c >= k) continue;
+      double t = 0.0;
+#pragma unroll
+      for (int l = 0; l <= c; ++l) t = fma(f[l], T[l + 16 * c], t);
+      f[c] = t;
+    }
+#pragma unroll
+    for (uint32_t l = 0; l < kCq; ++l)
+      if (l < k) F[(size_t)l * n + i] = f[l];
+  }
+  __syncthreads();
+}
+
+// LDS of the CholeskyQR2 recompression (doubles): Gram partials, then the
+// 16 x 16 matrices below
+constexpr uint32_t kCqLds = kWaves * 256 + 12 * 256;
+
+// CholeskyQR2 of U and V and the core; false (U, V as on entry) when either
+// factor keeps the Householder QR.  On success U, V hold F1 and Ws, Zs the
+// expansion's multipliers R2^{-1} W and R2^{-1} Z (k x kt, ld k).
+__device__ bool cq_recompress(const AcaBlock& B, AcaResult* __restrict__ res, uint32_t k, double eps, double* U,
+                              double* V, double* Ws, double* Zs, Shared& sh, double* sig, int* order,
+                              double* lds) {
+  double* part = lds;
+  double* G = part + kWaves * 256;
+  double* R1u = G + 256;
+  double* R1v = R1u + 256;
+  double* Iu = R1v + 256;     // inverses (first R1^{-1}, then R2^{-1})
+  double* Iv = Iu + 256;
+  double* R2u = Iv + 256;
+  double* R2v = R2u + 256;
+  double* Ru = R2v + 256;
+  double* Rv = Ru + 256;
+  double* Rc = Rv + 256;      // core, then its rotations
+  double* Zc = Rc + 256;
+  __shared__ int ok;
+  // first pass: both Cholesky factors before either factor is changed
+  cq_gram(U, B.n1, k, G, part);
+  if (threadIdx.x == 0) {
+    ok = cq_chol(G, k, R1u);
+    if (ok) cq_inv(R1u, k, Iu);
+  }
+  __syncthreads();
+  if (!ok) return false;
+  cq_gram(V, B.n2, k, G, part);
+  if (threadIdx.x == 0) {
+    ok = cq_chol(G, k, R1v);
+    if (ok) cq_inv(R1v, k, Iv);
+  }
+  __syncthreads();
+  if (!ok) return false;
+  cq_apply(U, B.n1, k, Iu);
+  cq_apply(V, B.n2, k, Iv);
+  // second pass
+  cq_gram(U, B.n1, k, G, part);
+  if (threadIdx.x == 0) ok = cq_chol(G, k, R2u);
+  __syncthreads();
+  if (ok) {
+    cq_gram(V, B.n2, k, G, part);
+    if (threadIdx.x == 0) ok = cq_chol(G, k, R2v);
+    __syncthreads();
+  }
+  if (!ok) {   // back to the factors as they came (to rounding)
+    cq_apply(U, B.n1, k, R1u);
+    cq_apply(V, B.n2, k, R1v);
+    return false;
+  }
+  if (threadIdx.x == 0) {
+    cq_mul_upper(R2u, R1u, k, Ru);
+    cq_mul_upper(R2v, R1v, k, Rv);
+    cq_inv(R2u, k, Iu);
+    cq_inv(R2v, k, Iv);
+  }
+  __syncthreads();
+  rc_core(B, res, k, eps, Ru, 16, k, Rv, 16, k, Rc, Zc, Ws, Zs, sh, sig, order);
+  __syncthreads();
+  const uint32_t kt = res[blockIdx.x].kt;
+  // multipliers M = R2^{-1} W (k x kt, ld k), through LDS
+  for (uint32_t e = threadIdx.x; e < k * kt; e += kAcaThreads) {
+    const uint32_t i = e % k, c = e / k;
+    double tu = 0.0, tv = 0.0;
+    for (uint32_t l = i; l < k; ++l) {
+      tu = fma(Iu[i + 16 * l], Ws[l + (size_t)k * c], tu);
+      tv = fma(Iv[i + 16 * l], Zs[l + (size_t)k * c], tv);
+    }
+    part[e] = tu;
+    part[256 + e] = tv;
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < k * kt; e += kAcaThreads) {
+    Ws[e] = part[e];
+    Zs[e] = part[256 + e];
+  }
+  if (threadIdx.x == 0) res[blockIdx.x].pad = 1;
+  return true;
+}
+
 
 // Doubles of dynamic LDS per workgroup of the recompression and expansion:
 // a leaf whose working arrays fit runs on LDS copies, without the round trips
 // to memory that every workgroup barrier would otherwise wait for.
 constexpr uint32_t kRcLds = 4352;
+static_assert(kCqLds <= kRcLds, "CholeskyQR2 working set exceeds the recompression LDS");
 
 __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock* __restrict__ blocks,
                                                                  AcaResult* __restrict__ res,
@@ -1007,7 +1357,14 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock*
   double* tau2 = tau1 + (B.kcap + 1);
   const uint32_t mmin = min(B.n1, k), nmin = min(B.n2, k);
   const size_t nu = (size_t)B.n1 * k, nv = (size_t)B.n2 * k;
-  if (nu + nv + 2 * (size_t)k + (size_t)mmin * nmin + (size_t)nmin * nmin <= kRcLds) {
+  const bool fits = nu + nv + 2 * (size_t)k + (size_t)mmin * nmin + (size_t)nmin * nmin <= kRcLds;
+  // factors that fit in LDS keep the Householder QR on LDS copies (no
+  // memory round trips); longer ones with few columns take CholeskyQR2
+  if (!fits && k <= kCq && B.n1 >= 2 * k && B.n2 >= 2 * k &&
+      cq_recompress(B, res, k, eps, U, V, Ws, Zs, sh, sig, order, rc_lds))
+    return;
+  __syncthreads();
+  if (fits) {
     double* Ul = rc_lds;
     double* Vl = Ul + nu;
     double* t1 = Vl + nv;
@@ -1041,6 +1398,40 @@ __global__ __launch_bounds__(kAcaThreads, 4) void expand_kernel(const AcaBlock* 
   const AcaResult r0 = res[blockIdx.x];
   if (r0.status != 1 || r0.kt == 0) return;
   const uint32_t k = r0.k, kt = r0.kt;
+  if (r0.pad == 1) {
+    // CholeskyQR2 leaves: U' = F1_u M_u, V' = F1_v M_v (multipliers k x kt)
+    if constexpr (!BIG) {
+      const size_t sq = (size_t)(B.kcap + 1) * (B.kcap + 1);
+      const double* Mu = slab_c + B.roff + 2 * sq;
+      const double* Mv = Mu + sq;
+      double* Uo = out + out_off[blockIdx.x];
+      double* Vo = Uo + (size_t)B.n1 * kt;
+      extern __shared__ double rc_lds[];
+      double* Ml = rc_lds;
+      for (uint32_t e = threadIdx.x; e < k * kt; e += kAcaThreads) {
+        Ml[e] = Mu[e];
+        Ml[256 + e] = Mv[e];
+      }
+      __syncthreads();
+      auto gemm = [&](const double* F, uint32_t n, const double* M, double* O) {
+        for (uint32_t i = threadIdx.x; i < n; i += kAcaThreads) {
+          double f[kCq];
+#pragma unroll
+          for (uint32_t l = 0; l < kCq; ++l) f[l] = l < k ? F[(size_t)l * n + i] : 0.0;
+          for (uint32_t c = 0; c < kt; ++c) {
+            double t = 0.0;
+#pragma unroll
+            for (uint32_t l = 0; l < kCq; ++l)
+              if (l < k) t = fma(f[l], M[l + (size_t)k * c], t);
+            O[(size_t)c * n + i] = t;
+          }
+        }
+      };
+      gemm(slab_c + B.uoff, B.n1, Ml, Uo);
+      gemm(slab_c + B.voff, B.n2, Ml + 256, Vo);
+    }
+    return;
+  }
   const uint32_t mmin = min(B.n1, k), nmin = min(B.n2, k);
   const double* U = slab_c + B.uoff;
   const double* V = slab_c + B.voff;
@@ -1206,11 +1597,8 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     cap[c] = std::min(h, lim);
   }
   uint64_t retries = 0, slab_total = 0, slab_peak = 0;
-  // largest slabs first: the leaves most likely to need a rerun go early, so
-  // their reruns share the following batches with many small first attempts
   std::vector<size_t> todo(cand.size());
   std::iota(todo.begin(), todo.end(), 0);
-  std::stable_sort(todo.begin(), todo.end(), [&](size_t a, size_t b) { return cap[a] > cap[b]; });
   // bytes of ACA scratch per batch: a quarter of the free device memory,
   // between 4 and 96 GiB.  Every batch lasts about as long as its largest
   // leaves take, so fewer, larger batches finish sooner.
@@ -1235,6 +1623,19 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     parts.clear();
   };
   Grow gblk, gres, gslab, gscr, goff;
+  // second stream of the small-leaf ACA launches and its two ordering events
+  struct Aux {
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Aux() {
+      if (s) (void)hipStreamSynchronize(s);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } aux;
+  AHM_HIP(hipEventCreateWithFlags(&aux.e0, hipEventDisableTiming));
+  AHM_HIP(hipEventCreateWithFlags(&aux.e1, hipEventDisableTiming));
   // the slab scratch is sized once, for the largest batch there can be (the
   // budget, or all first attempts if they take less): growing it batch by
   // batch re-allocates tens of GiB, which is slow right after large frees
@@ -1246,6 +1647,10 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       all += (kc + 1) * ((uint64_t)n1[l] + n2[l]) + 4 * sq + 2 * (kc + 1);
     }
     if (all && gslab.need(std::min<uint64_t>(all * 8, budget))) return fail(AHM_ERR_HIP, "ACA scratch");
+    // largest slabs first: the leaves most likely to need a rerun go early,
+    // so their reruns share the following batches with many small first
+    // attempts
+    std::stable_sort(todo.begin(), todo.end(), [&](size_t a, size_t b) { return cap[a] > cap[b]; });
   }
   timer.mark("set-up (points, candidates)");
   const auto t_aca0 = std::chrono::steady_clock::now();
@@ -1259,12 +1664,30 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       std::vector<AcaBlock> blk;
       uint64_t slab = 0, iscr = 0;
       size_t end = pos;
-      while (end < todo.size()) {
-        const size_t c = todo[end];
+      auto slab_need = [&](size_t c) {
         const uint32_t l = cand[c];
         const uint64_t kc = cap[c], sq = (kc + 1) * (kc + 1);
-        const uint64_t need = (kc + 1) * ((uint64_t)n1[l] + n2[l]) + 4 * sq + 2 * (kc + 1);
-        if (end > pos && (slab + need) * 8 > budget) break;
+        return (kc + 1) * ((uint64_t)n1[l] + n2[l]) + 4 * sq + 2 * (kc + 1);
+      };
+      for (uint64_t acc = 0; end < todo.size(); ++end) {
+        const uint64_t need = slab_need(todo[end]);
+        if (end > pos && (acc + need) * 8 > budget) break;
+        acc += need;
+      }
+      // the batch's small leaves (one wavefront each) after its large ones:
+      // two launches of the ACA kernel over the two ranges
+      auto small = [&](size_t c) {
+        const uint32_t l = cand[c];
+        return (uint64_t)n1[l] + n2[l] <= kSmallLeaf;
+      };
+      const size_t nbig = (size_t)(std::stable_partition(todo.begin() + pos, todo.begin() + end,
+                                                         [&](size_t c) { return !small(c); }) -
+                                   (todo.begin() + pos));
+      for (size_t e = pos; e < end; ++e) {
+        const size_t c = todo[e];
+        const uint32_t l = cand[c];
+        const uint64_t kc = cap[c];
+        const uint64_t need = slab_need(c);
         AcaBlock B{};
         B.b1 = b1[l];
         B.n1 = n1[l];
@@ -1281,7 +1704,6 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
         slab += need;
         iscr += (uint64_t)n1[l] + n2[l];
         blk.push_back(B);
-        ++end;
       }
       slab_total += slab * 8;
       slab_peak = std::max<uint64_t>(slab_peak, slab * 8);
@@ -1300,9 +1722,31 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
         return st;
       }
       const uint32_t nb = (uint32_t)blk.size();
-      hipLaunchKernelGGL(aca_kernel, dim3(nb), dim3(kAcaThreads), 0, s, (const AcaBlock*)dblk.p,
-                         (AcaResult*)dres.p, (const double*)dP.p, (const KernelSpec*)dK.p, eps,
-                         (double*)dslab.p, (int*)discr.p);
+      // the small leaves run on a second stream beside the large ones, so
+      // that they fill the CUs the large leaves' tail leaves idle
+      const bool two = nbig && nb > nbig;
+      if (two) {
+        if (!aux.s && hipStreamCreateWithFlags(&aux.s, hipStreamNonBlocking) != hipSuccess) aux.s = nullptr;
+        if (aux.s && (hipEventRecord(aux.e0, s) != hipSuccess || hipStreamWaitEvent(aux.s, aux.e0, 0) != hipSuccess)) {
+          cleanup();
+          return fail(AHM_ERR_HIP, "ACA stream ordering");
+        }
+      }
+      hipStream_t s_small = two && aux.s ? aux.s : s;
+      if (nbig)
+        hipLaunchKernelGGL(aca_kernel, dim3((uint32_t)nbig), dim3(kAcaThreads), kFuseK * kAcaThreads * sizeof(double),
+                           s, (const AcaBlock*)dblk.p, (AcaResult*)dres.p, (const double*)dP.p,
+                           (const KernelSpec*)dK.p, eps, (double*)dslab.p, (int*)discr.p);
+      if (nb > nbig)
+        hipLaunchKernelGGL(aca_kernel, dim3(nb - (uint32_t)nbig), dim3(kSmallThreads),
+                           kFuseK * kSmallThreads * sizeof(double), s_small, (const AcaBlock*)dblk.p + nbig,
+                           (AcaResult*)dres.p + nbig, (const double*)dP.p, (const KernelSpec*)dK.p, eps,
+                           (double*)dslab.p, (int*)discr.p);
+      if (s_small != s &&
+          (hipEventRecord(aux.e1, s_small) != hipSuccess || hipStreamWaitEvent(s, aux.e1, 0) != hipSuccess)) {
+        cleanup();
+        return fail(AHM_ERR_HIP, "ACA stream ordering");
+      }
       hipLaunchKernelGGL(recompress_kernel, dim3(nb), dim3(kAcaThreads), kRcLds * sizeof(double), s,
                          (const AcaBlock*)dblk.p, (AcaResult*)dres.p, eps, (double*)dslab.p);
       std::vector<AcaResult> rr(nb);
