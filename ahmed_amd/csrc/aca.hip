@@ -37,6 +37,9 @@ constexpr int kWaves = kAcaThreads / 64;
 // workgroup size from blockDim.
 constexpr uint32_t kSmallLeaf = 1024;
 constexpr int kSmallThreads = 64;
+// leaves beyond kLargeLeaf rows + columns run with kLargeThreads threads
+constexpr uint32_t kLargeLeaf = 8192;
+constexpr int kLargeThreads = 512;
 __device__ __forceinline__ uint32_t nthr() { return blockDim.x; }
 __device__ __forceinline__ int nwaves() { return (int)(blockDim.x >> 6); }
 
@@ -210,17 +213,20 @@ __device__ __forceinline__ void house_axpy(double* x, const double* a, double s,
 // load anyway, so the factors are streamed once per step instead of twice.
 constexpr uint32_t kFuseK = 16;
 
+// per-wavefront partials of the workgroup reductions (up to kLargeThreads)
+constexpr int kMaxWaves = kLargeThreads / 64;
+
 struct Shared {
-  double dv[kWaves];
-  double av[kWaves];
-  int ai[kWaves];
-  int ki[kWaves];
-  int kk[kWaves];
+  double dv[kMaxWaves];
+  double av[kMaxWaves];
+  int ai[kMaxWaves];
+  int ki[kMaxWaves];
+  int kk[kMaxWaves];
 };
 
 // LDS of the fused dots (ACA kernel only)
 struct DotShared {
-  double dw[kWaves][kFuseK];   // per-wave partial dots
+  double dw[kMaxWaves][kFuseK];   // per-wave partial dots
   double dA[kFuseK], dB[kFuseK], dC[kFuseK];
 };
 
@@ -328,14 +334,30 @@ __device__ ArgMax remainder_row_dots(const KernelSpec& K, const double* P, const
   ArgMax a{-1.0, 0x7fffffff};
   dots_zero(acc, k);
   bool retired = false;
-  for (uint32_t j = threadIdx.x; j < B.n2; j += nthr()) {
+  // two columns per iteration: both columns' loads are issued before either
+  // is stored, so twice as many are in flight
+  const uint32_t nt = nthr();
+  for (uint32_t j = threadIdx.x; j < B.n2; j += 2 * nt) {
+    const uint32_t j2 = j + nt;
+    const bool two = j2 < B.n2;
+    const int s1 = S[j], s2 = two ? S[j2] : -1;
     const double v = sub_terms(kentry(K, P, B.b1 + i0, B.b2 + j), ucoef, V, B.n2, j, k);
+    const double v2 = two ? sub_terms(kentry(K, P, B.b1 + i0, B.b2 + j2), ucoef, V, B.n2, j2, k) : 0.0;
     pv[j] = v;
-    if (S[j] >= 0) {
+    if (two) pv[j2] = v2;
+    if (s1 >= 0) {
       a = amax(a, ArgMax{fabs(v), (int)j});
       dots_acc(acc, v, V, B.n2, j, k);
     } else {
       retired = true;
+    }
+    if (two) {
+      if (s2 >= 0) {
+        a = amax(a, ArgMax{fabs(v2), (int)j2});
+        dots_acc(acc, v2, V, B.n2, j2, k);
+      } else {
+        retired = true;
+      }
     }
   }
   block_dots(acc, k, ds.dA, ds);
@@ -358,11 +380,21 @@ __device__ ArgMax remainder_col_dots(const KernelSpec& K, const double* P, const
   __syncthreads();
   ArgMax a{-1.0, 0x7fffffff};
   dots_zero(acc, k);
-  for (uint32_t i = threadIdx.x; i < B.n1; i += nthr()) {
+  const uint32_t nt = nthr();
+  for (uint32_t i = threadIdx.x; i < B.n1; i += 2 * nt) {   // two rows per iteration
+    const uint32_t i2 = i + nt;
+    const bool two = i2 < B.n1;
+    const int z1 = Z[i], z2 = two ? Z[i2] : -1;
     const double v = sub_terms(kentry(K, P, B.b1 + i, B.b2 + j0), vcoef, U, B.n1, i, k);
+    const double v2 = two ? sub_terms(kentry(K, P, B.b1 + i2, B.b2 + j0), vcoef, U, B.n1, i2, k) : 0.0;
     pu[i] = v;
-    if (Z[i] >= 0) a = amax(a, ArgMax{fabs(v), (int)i});
+    if (two) pu[i2] = v2;
+    if (z1 >= 0) a = amax(a, ArgMax{fabs(v), (int)i});
     dots_acc(acc, v, U, B.n1, i, k);
+    if (two) {
+      if (z2 >= 0) a = amax(a, ArgMax{fabs(v2), (int)i2});
+      dots_acc(acc, v2, U, B.n1, i2, k);
+    }
   }
   block_dots(acc, k, ds.dC, ds);
   return block_argmax(a, sh.av, sh.ai);
@@ -573,7 +605,8 @@ __device__ double cross_sum(const AcaBlock& B, uint32_t k, const double* U, cons
   return block_sum(acc, sh.dv);
 }
 
-__global__ __launch_bounds__(kAcaThreads, 4) void aca_kernel(const AcaBlock* __restrict__ blocks,
+template <int MAXT>
+__global__ __launch_bounds__(MAXT, 4) void aca_kernel(const AcaBlock* __restrict__ blocks,
                                                           AcaResult* __restrict__ res,
                                                           const double* __restrict__ P,
                                                           const KernelSpec* __restrict__ Ks,
@@ -987,7 +1020,86 @@ __device__ void wy_apply_big(const double* A, uint32_t n, uint32_t r, const doub
 // eps * |R|_F; without the second test the columns a rank-deficient core
 // drives to zero stay rounding noise that is rotated in every sweep (a third
 // of the leaves of a 1M-point matrix swept until the sweep limit).
+// The same iteration for mm, nn <= 16, on the first wavefront alone: four
+// pairs at a time, sixteen lanes a pair (lane i owns row i of the pair's
+// columns and of their rotations), sums over the sixteen lanes by a fixed
+// xor butterfly; no workgroup barriers inside the sweeps.
+__device__ void jacobi_svd16(double* R, uint32_t mm, uint32_t nn, double* Zm, Shared& sh) {
+  const double tol = (double)max(mm, 2u) * 2.220446049250313e-16;
+  for (uint32_t e = threadIdx.x; e < nn * nn; e += kAcaThreads) Zm[e] = (e % nn == e / nn) ? 1.0 : 0.0;
+  __syncthreads();
+  if (nn < 2) return;
+  double f = 0.0;
+  for (uint32_t e = threadIdx.x; e < mm * nn; e += kAcaThreads) f = fma(R[e], R[e], f);
+  const double negl = 4.930380657631324e-32 * block_sum(f, sh.dv);   // eps^2 |R|_F^2
+  if (threadIdx.x < 64) {
+    const uint32_t g = threadIdx.x >> 4, i = threadIdx.x & 15;
+    const uint32_t np = (nn + 1) & ~1u;
+    auto sum16 = [](double v) {
+      v += __shfl_xor(v, 8, 16);
+      v += __shfl_xor(v, 4, 16);
+      v += __shfl_xor(v, 2, 16);
+      v += __shfl_xor(v, 1, 16);
+      return v;
+    };
+    for (int sweep = 0; sweep < 40; ++sweep) {
+      bool rotated = false;
+      for (uint32_t round = 0; round < np - 1; ++round) {
+        for (uint32_t base = 0; base < np / 2; base += 4) {
+          const uint32_t pi = base + g;
+          auto player = [&](uint32_t s) -> uint32_t {
+            return s == 0 ? 0 : 1 + (s - 1 + round) % (np - 1);
+          };
+          uint32_t p = 0, q = 0;
+          bool live = pi < np / 2;
+          if (live) {
+            p = player(pi);
+            q = player(np - 1 - pi);
+            live = p < nn && q < nn;
+            if (p > q) {
+              const uint32_t tt = p;
+              p = q;
+              q = tt;
+            }
+          }
+          double* ap = R + (size_t)p * mm;
+          double* aq = R + (size_t)q * mm;
+          const double x = live && i < mm ? ap[i] : 0.0;
+          const double y = live && i < mm ? aq[i] : 0.0;
+          const double al = sum16(x * x), be = sum16(y * y), ga = sum16(x * y);
+          if (!live || fabs(ga) <= tol * sqrt(al * be) || ga == 0.0 || fmin(al, be) <= negl) continue;
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(fma(zeta, zeta, 1.0)));
+          const double c = 1.0 / sqrt(fma(t, t, 1.0)), s = c * t;
+          if (i < mm) {
+            ap[i] = c * x - s * y;
+            aq[i] = s * x + c * y;
+          }
+          if (i < nn) {
+            double* zp = Zm + (size_t)p * nn;
+            double* zq = Zm + (size_t)q * nn;
+            const double zx = zp[i], zy = zq[i];
+            zp[i] = c * zx - s * zy;
+            zq[i] = s * zx + c * zy;
+          }
+          rotated = true;
+        }
+        // the next round reads rows that other lanes of this wavefront wrote
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      if (!__any(rotated)) break;
+    }
+  }
+  __syncthreads();
+}
+
 __device__ void jacobi_svd(double* R, uint32_t mm, uint32_t nn, double* Zm, Shared& sh) {
+  if (mm <= 16 && nn <= 16) {
+    jacobi_svd16(R, mm, nn, Zm, sh);
+    return;
+  }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double tol = (double)max(mm, 2u) * 2.220446049250313e-16;
   for (uint32_t e = threadIdx.x; e < nn * nn; e += kAcaThreads) Zm[e] = (e % nn == e / nn) ? 1.0 : 0.0;
@@ -1674,15 +1786,15 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
         if (end > pos && (acc + need) * 8 > budget) break;
         acc += need;
       }
-      // the batch's small leaves (one wavefront each) after its large ones:
-      // two launches of the ACA kernel over the two ranges
-      auto small = [&](size_t c) {
-        const uint32_t l = cand[c];
-        return (uint64_t)n1[l] + n2[l] <= kSmallLeaf;
-      };
-      const size_t nbig = (size_t)(std::stable_partition(todo.begin() + pos, todo.begin() + end,
-                                                         [&](size_t c) { return !small(c); }) -
-                                   (todo.begin() + pos));
+      // the batch's leaves by size class (large, middle, small), one launch
+      // of the ACA kernel per class
+      auto size_of = [&](size_t c) { return (uint64_t)n1[cand[c]] + n2[cand[c]]; };
+      const size_t nlarge = (size_t)(std::stable_partition(todo.begin() + pos, todo.begin() + end,
+                                                           [&](size_t c) { return size_of(c) > kLargeLeaf; }) -
+                                     (todo.begin() + pos));
+      const size_t nbig = nlarge + (size_t)(std::stable_partition(todo.begin() + pos + nlarge, todo.begin() + end,
+                                                                  [&](size_t c) { return size_of(c) > kSmallLeaf; }) -
+                                            (todo.begin() + pos + nlarge));
       for (size_t e = pos; e < end; ++e) {
         const size_t c = todo[e];
         const uint32_t l = cand[c];
@@ -1722,9 +1834,10 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
         return st;
       }
       const uint32_t nb = (uint32_t)blk.size();
-      // the small leaves run on a second stream beside the large ones, so
-      // that they fill the CUs the large leaves' tail leaves idle
-      const bool two = nbig && nb > nbig;
+      // the large leaves run on the context's stream, the middle and small
+      // ones after each other on a second stream beside them, so that they
+      // fill the CUs the large leaves' tail leaves idle
+      const bool two = nlarge && nb > nlarge;
       if (two) {
         if (!aux.s && hipStreamCreateWithFlags(&aux.s, hipStreamNonBlocking) != hipSuccess) aux.s = nullptr;
         if (aux.s && (hipEventRecord(aux.e0, s) != hipSuccess || hipStreamWaitEvent(aux.s, aux.e0, 0) != hipSuccess)) {
@@ -1733,15 +1846,21 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
         }
       }
       hipStream_t s_small = two && aux.s ? aux.s : s;
-      if (nbig)
-        hipLaunchKernelGGL(aca_kernel, dim3((uint32_t)nbig), dim3(kAcaThreads), kFuseK * kAcaThreads * sizeof(double),
-                           s, (const AcaBlock*)dblk.p, (AcaResult*)dres.p, (const double*)dP.p,
-                           (const KernelSpec*)dK.p, eps, (double*)dslab.p, (int*)discr.p);
-      if (nb > nbig)
-        hipLaunchKernelGGL(aca_kernel, dim3(nb - (uint32_t)nbig), dim3(kSmallThreads),
-                           kFuseK * kSmallThreads * sizeof(double), s_small, (const AcaBlock*)dblk.p + nbig,
-                           (AcaResult*)dres.p + nbig, (const double*)dP.p, (const KernelSpec*)dK.p, eps,
-                           (double*)dslab.p, (int*)discr.p);
+      auto launch = [&](size_t b0, size_t b1, int nt, hipStream_t st) {
+        if (b1 <= b0) return;
+        const size_t lds = kFuseK * (size_t)nt * sizeof(double);
+        if (nt > kAcaThreads)
+          hipLaunchKernelGGL(aca_kernel<kLargeThreads>, dim3((uint32_t)(b1 - b0)), dim3(nt), lds, st,
+                             (const AcaBlock*)dblk.p + b0, (AcaResult*)dres.p + b0, (const double*)dP.p,
+                             (const KernelSpec*)dK.p, eps, (double*)dslab.p, (int*)discr.p);
+        else
+          hipLaunchKernelGGL(aca_kernel<kAcaThreads>, dim3((uint32_t)(b1 - b0)), dim3(nt), lds, st,
+                             (const AcaBlock*)dblk.p + b0, (AcaResult*)dres.p + b0, (const double*)dP.p,
+                             (const KernelSpec*)dK.p, eps, (double*)dslab.p, (int*)discr.p);
+      };
+      launch(0, nlarge, kLargeThreads, s);
+      launch(nlarge, nbig, kAcaThreads, s_small);
+      launch(nbig, nb, kSmallThreads, s_small);
       if (s_small != s &&
           (hipEventRecord(aux.e1, s_small) != hipSuccess || hipStreamWaitEvent(s, aux.e1, 0) != hipSuccess)) {
         cleanup();
