@@ -748,7 +748,7 @@ __device__ void house_qr(double* A, uint32_t n, uint32_t k, double* tau, Shared&
   const uint32_t r = min(n, k);
   for (uint32_t j = 0; j < r; ++j) {
     double* a = A + (size_t)j * n;
-    const double s = strided_dot(a, a, j + 1 + threadIdx.x, n, kAcaThreads, 0.0);
+    const double s = strided_dot(a, a, j + 1 + threadIdx.x, n, nthr(), 0.0);
     const double xn2 = block_sum(s, sh.dv);
     __syncthreads();
     const double alpha = a[j];
@@ -760,7 +760,7 @@ __device__ void house_qr(double* A, uint32_t n, uint32_t k, double* tau, Shared&
     }
     __syncthreads();
     if (xn2 > 0.0)
-      for (uint32_t i = j + 1 + threadIdx.x; i < n; i += kAcaThreads) a[i] *= scal;
+      for (uint32_t i = j + 1 + threadIdx.x; i < n; i += nthr()) a[i] *= scal;
     if (threadIdx.x == 0) {
       a[j] = beta;
       tau[j] = t;
@@ -768,7 +768,7 @@ __device__ void house_qr(double* A, uint32_t n, uint32_t k, double* tau, Shared&
     __syncthreads();
     if (t == 0.0) continue;
     // w_c = v^T A[:, c] for c > j, one wavefront per column
-    for (uint32_t c = j + 1 + w; c < k; c += kWaves) {
+    for (uint32_t c = j + 1 + w; c < k; c += nwaves()) {
       const double* ac = A + (size_t)c * n;
       double d = strided_dot(a, ac, j + 1 + lane, n, 64, 0.0);
       d = wave_sum(d);
@@ -777,7 +777,7 @@ __device__ void house_qr(double* A, uint32_t n, uint32_t k, double* tau, Shared&
     __syncthreads();
     for (uint32_t c = j + 1; c < k; ++c) {
       const double tw = t * wbuf[c];
-      house_axpy(A + (size_t)c * n, a, tw, j, j + threadIdx.x, n, kAcaThreads);
+      house_axpy(A + (size_t)c * n, a, tw, j, j + threadIdx.x, n, nthr());
     }
     __syncthreads();
   }
@@ -1026,11 +1026,11 @@ __device__ void wy_apply_big(const double* A, uint32_t n, uint32_t r, const doub
 // xor butterfly; no workgroup barriers inside the sweeps.
 __device__ void jacobi_svd16(double* R, uint32_t mm, uint32_t nn, double* Zm, Shared& sh) {
   const double tol = (double)max(mm, 2u) * 2.220446049250313e-16;
-  for (uint32_t e = threadIdx.x; e < nn * nn; e += kAcaThreads) Zm[e] = (e % nn == e / nn) ? 1.0 : 0.0;
+  for (uint32_t e = threadIdx.x; e < nn * nn; e += nthr()) Zm[e] = (e % nn == e / nn) ? 1.0 : 0.0;
   __syncthreads();
   if (nn < 2) return;
   double f = 0.0;
-  for (uint32_t e = threadIdx.x; e < mm * nn; e += kAcaThreads) f = fma(R[e], R[e], f);
+  for (uint32_t e = threadIdx.x; e < mm * nn; e += nthr()) f = fma(R[e], R[e], f);
   const double negl = 4.930380657631324e-32 * block_sum(f, sh.dv);   // eps^2 |R|_F^2
   if (threadIdx.x < 64) {
     const uint32_t g = threadIdx.x >> 4, i = threadIdx.x & 15;
@@ -1102,13 +1102,13 @@ __device__ void jacobi_svd(double* R, uint32_t mm, uint32_t nn, double* Zm, Shar
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double tol = (double)max(mm, 2u) * 2.220446049250313e-16;
-  for (uint32_t e = threadIdx.x; e < nn * nn; e += kAcaThreads) Zm[e] = (e % nn == e / nn) ? 1.0 : 0.0;
+  for (uint32_t e = threadIdx.x; e < nn * nn; e += nthr()) Zm[e] = (e % nn == e / nn) ? 1.0 : 0.0;
   __syncthreads();
   if (nn < 2) return;
   // columns below eps * |R|_F are rounding noise (the rotations keep |R|_F):
   // they are left alone, or they would be rotated on noise in every sweep
   double f = 0.0;
-  for (uint32_t e = threadIdx.x; e < mm * nn; e += kAcaThreads) f = fma(R[e], R[e], f);
+  for (uint32_t e = threadIdx.x; e < mm * nn; e += nthr()) f = fma(R[e], R[e], f);
   const double negl = 4.930380657631324e-32 * block_sum(f, sh.dv);   // eps^2 |R|_F^2
   const uint32_t np = (nn + 1) & ~1u;  // even number of players (one bye if odd)
   __shared__ int rot;
@@ -1116,7 +1116,7 @@ __device__ void jacobi_svd(double* R, uint32_t mm, uint32_t nn, double* Zm, Shar
     if (threadIdx.x == 0) rot = 0;
     __syncthreads();
     for (uint32_t round = 0; round < np - 1; ++round) {
-      for (uint32_t pi = w; pi < np / 2; pi += kWaves) {
+      for (uint32_t pi = w; pi < np / 2; pi += nwaves()) {
         // tournament pairing: player 0 fixed, others rotate
         auto player = [&](uint32_t s) -> uint32_t {
           return s == 0 ? 0 : 1 + (s - 1 + round) % (np - 1);
@@ -1178,7 +1178,7 @@ __device__ __forceinline__ void rc_core(const AcaBlock& B, AcaResult* __restrict
   const uint32_t amin = min(mmin, nmin);
   // R = R1 R2^T (utrmmh, Include/blas.h:2767-2781)
   double fs = 0.0;
-  for (uint32_t e = threadIdx.x; e < mmin * nmin; e += kAcaThreads) {
+  for (uint32_t e = threadIdx.x; e < mmin * nmin; e += nthr()) {
     const uint32_t i = e % mmin, j = e / mmin;
     double d = 0.0;
     uint32_t l = max(i, j);
@@ -1198,19 +1198,19 @@ __device__ __forceinline__ void rc_core(const AcaBlock& B, AcaResult* __restrict
   }
   const double fro = sqrt(block_sum(fs, sh.dv));
   const double thresh = eps * fro / (double)sqrtf((float)mmin * nmin);
-  for (uint32_t e = threadIdx.x; e < mmin * nmin; e += kAcaThreads)
+  for (uint32_t e = threadIdx.x; e < mmin * nmin; e += nthr())
     if (fabs(R[e]) < thresh) R[e] = 0.0;
   __syncthreads();
   jacobi_svd(R, mmin, nmin, Zm, sh);
   // singular values = column norms; order descending (ties by index)
-  for (uint32_t c = threadIdx.x >> 6; c < nmin; c += kWaves) {
+  for (uint32_t c = threadIdx.x >> 6; c < nmin; c += nwaves()) {
     double s = 0.0;
     for (uint32_t i = threadIdx.x & 63; i < mmin; i += 64) s = fma(R[(size_t)c * mmin + i], R[(size_t)c * mmin + i], s);
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) sig[c] = sqrt(s);
   }
   __syncthreads();
-  for (uint32_t c = threadIdx.x; c < nmin; c += kAcaThreads) {
+  for (uint32_t c = threadIdx.x; c < nmin; c += nthr()) {
     uint32_t rank = 0;
     for (uint32_t d = 0; d < nmin; ++d)
       if (sig[d] > sig[c] || (sig[d] == sig[c] && d < c)) ++rank;
@@ -1222,11 +1222,11 @@ __device__ __forceinline__ void rc_core(const AcaBlock& B, AcaResult* __restrict
   const double s0 = sig[order[0]];
   while (kt > 0 && (sig[order[kt - 1]] <= eps * s0 || sig[order[kt - 1]] < 1e-64)) --kt;
   // sorted, truncated copies of the core factors
-  for (uint32_t e = threadIdx.x; e < mmin * kt; e += kAcaThreads) {
+  for (uint32_t e = threadIdx.x; e < mmin * kt; e += nthr()) {
     const uint32_t i = e % mmin, c = e / mmin;
     Ws[e] = R[(size_t)order[c] * mmin + i];
   }
-  for (uint32_t e = threadIdx.x; e < nmin * kt; e += kAcaThreads) {
+  for (uint32_t e = threadIdx.x; e < nmin * kt; e += nthr()) {
     const uint32_t i = e % nmin, c = e / nmin;
     Zs[e] = Zm[(size_t)order[c] * nmin + i];
   }
@@ -1266,7 +1266,7 @@ __device__ void cq_gram(const double* F, uint32_t n, uint32_t k, double* G, doub
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t li = lane & 15, kk = lane >> 4, ng = (n + 15) / 16;
   aca_f64x4 acc = aca_f64x4{0.0, 0.0, 0.0, 0.0};
-  for (uint32_t g = w; g < ng; g += kWaves) {
+  for (uint32_t g = w; g < ng; g += nwaves()) {
     const uint32_t mb = g * 16 + 4 * kk;
     double y[4];
 #pragma unroll
@@ -1281,9 +1281,9 @@ __device__ void cq_gram(const double* F, uint32_t n, uint32_t k, double* G, doub
 #pragma unroll
   for (int v = 0; v < 4; ++v) part[w * 256 + (kk + 4 * v) + li * 16] = acc[v];
   __syncthreads();
-  for (uint32_t e = threadIdx.x; e < 256; e += kAcaThreads) {
+  for (uint32_t e = threadIdx.x; e < 256; e += nthr()) {
     double t = part[e];
-    for (int q = 1; q < kWaves; ++q) t += part[q * 256 + e];
+    for (int q = 1; q < nwaves(); ++q) t += part[q * 256 + e];
     G[e] = t;
   }
   __syncthreads();
@@ -1335,7 +1335,7 @@ __device__ void cq_mul_upper(const double* A, const double* Bm, uint32_t k, doub
 // per thread (row c of the result needs entries 0..c of the row: columns
 // are rewritten from the last)
 __device__ void cq_apply(double* F, uint32_t n, uint32_t k, const double* T) {
-  for (uint32_t i = threadIdx.x; i < n; i += kAcaThreads) {
+  for (uint32_t i = threadIdx.x; i < n; i += nthr()) {
     double f[kCq];
 #pragma unroll
     for (uint32_t l = 0; l < kCq; ++l) f[l] = l < k ? F[(size_t)l * n + i] : 0.0;
@@ -1354,9 +1354,10 @@ __device__ void cq_apply(double* F, uint32_t n, uint32_t k, const double* T) {
   __syncthreads();
 }
 
-// LDS of the CholeskyQR2 recompression (doubles): Gram partials, then the
-// 16 x 16 matrices below
-constexpr uint32_t kCqLds = kWaves * 256 + 12 * 256;
+// LDS of the CholeskyQR2 recompression (doubles): Gram partials of the
+// workgroup's wavefronts, then eight 16 x 16 matrices
+constexpr uint32_t kCqLds = kWaves * 256 + 8 * 256;
+constexpr uint32_t kCqLdsSmall = 256 + 8 * 256;   // one wavefront
 
 // CholeskyQR2 of U and V and the core; false (U, V as on entry) when either
 // factor keeps the Householder QR.  On success U, V hold F1 and Ws, Zs the
@@ -1364,18 +1365,20 @@ constexpr uint32_t kCqLds = kWaves * 256 + 12 * 256;
 __device__ bool cq_recompress(const AcaBlock& B, AcaResult* __restrict__ res, uint32_t k, double eps, double* U,
                               double* V, double* Ws, double* Zs, Shared& sh, double* sig, int* order,
                               double* lds) {
+  // eight 16 x 16 matrices after the Gram partials, reused as they free up
   double* part = lds;
-  double* G = part + kWaves * 256;
-  double* R1u = G + 256;
-  double* R1v = R1u + 256;
-  double* Iu = R1v + 256;     // inverses (first R1^{-1}, then R2^{-1})
-  double* Iv = Iu + 256;
-  double* R2u = Iv + 256;
-  double* R2v = R2u + 256;
-  double* Ru = R2v + 256;
-  double* Rv = Ru + 256;
-  double* Rc = Rv + 256;      // core, then its rotations
-  double* Zc = Rc + 256;
+  double* M = part + (size_t)nwaves() * 256;
+  double* G = M;             // Gram matrix, then R = R2 R1 of U
+  double* R1u = M + 256;     // then the core
+  double* R1v = M + 512;     // then the core's rotations
+  double* Iu = M + 768;      // R1^{-1}, then R2^{-1}
+  double* Iv = M + 1024;
+  double* R2u = M + 1280;    // then scratch of the multipliers
+  double* R2v = M + 1536;
+  double* Rv = M + 1792;
+  double* Ru = G;
+  double* Rc = R1u;
+  double* Zc = R1v;
   __shared__ int ok;
   // first pass: both Cholesky factors before either factor is changed
   cq_gram(U, B.n1, k, G, part);
@@ -1409,7 +1412,7 @@ __device__ bool cq_recompress(const AcaBlock& B, AcaResult* __restrict__ res, ui
     return false;
   }
   if (threadIdx.x == 0) {
-    cq_mul_upper(R2u, R1u, k, Ru);
+    cq_mul_upper(R2u, R1u, k, Ru);   // G is free: the last Gram is factored
     cq_mul_upper(R2v, R1v, k, Rv);
     cq_inv(R2u, k, Iu);
     cq_inv(R2v, k, Iv);
@@ -1419,20 +1422,20 @@ __device__ bool cq_recompress(const AcaBlock& B, AcaResult* __restrict__ res, ui
   __syncthreads();
   const uint32_t kt = res[blockIdx.x].kt;
   // multipliers M = R2^{-1} W (k x kt, ld k), through LDS
-  for (uint32_t e = threadIdx.x; e < k * kt; e += kAcaThreads) {
+  for (uint32_t e = threadIdx.x; e < k * kt; e += nthr()) {
     const uint32_t i = e % k, c = e / k;
     double tu = 0.0, tv = 0.0;
     for (uint32_t l = i; l < k; ++l) {
       tu = fma(Iu[i + 16 * l], Ws[l + (size_t)k * c], tu);
       tv = fma(Iv[i + 16 * l], Zs[l + (size_t)k * c], tv);
     }
-    part[e] = tu;
-    part[256 + e] = tv;
+    R2u[e] = tu;
+    R2v[e] = tv;
   }
   __syncthreads();
-  for (uint32_t e = threadIdx.x; e < k * kt; e += kAcaThreads) {
-    Ws[e] = part[e];
-    Zs[e] = part[256 + e];
+  for (uint32_t e = threadIdx.x; e < k * kt; e += nthr()) {
+    Ws[e] = R2u[e];
+    Zs[e] = R2v[e];
   }
   if (threadIdx.x == 0) res[blockIdx.x].pad = 1;
   return true;
@@ -1445,9 +1448,12 @@ __device__ bool cq_recompress(const AcaBlock& B, AcaResult* __restrict__ res, ui
 constexpr uint32_t kRcLds = 4352;
 static_assert(kCqLds <= kRcLds, "CholeskyQR2 working set exceeds the recompression LDS");
 
+// small != 0: one-wavefront workgroups with kCqLdsSmall doubles of LDS
+// (CholeskyQR2, else the Householder QR in memory); otherwise kRcLds
 __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock* __restrict__ blocks,
                                                                  AcaResult* __restrict__ res,
-                                                                 double eps, double* __restrict__ slab) {
+                                                                 double eps, double* __restrict__ slab,
+                                                                 int small) {
   __shared__ Shared sh;
   __shared__ double wbuf[kAcaCapMax + 1];
   __shared__ double sig[kAcaCapMax + 1];
@@ -1469,7 +1475,7 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock*
   double* tau2 = tau1 + (B.kcap + 1);
   const uint32_t mmin = min(B.n1, k), nmin = min(B.n2, k);
   const size_t nu = (size_t)B.n1 * k, nv = (size_t)B.n2 * k;
-  const bool fits = nu + nv + 2 * (size_t)k + (size_t)mmin * nmin + (size_t)nmin * nmin <= kRcLds;
+  const bool fits = !small && nu + nv + 2 * (size_t)k + (size_t)mmin * nmin + (size_t)nmin * nmin <= kRcLds;
   // factors that fit in LDS keep the Householder QR on LDS copies (no
   // memory round trips); longer ones with few columns take CholeskyQR2
   if (!fits && k <= kCq && B.n1 >= 2 * k && B.n2 >= 2 * k &&
@@ -1483,16 +1489,16 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(const AcaBlock*
     double* t2 = t1 + k;
     double* Rl = t2 + k;
     double* Zl = Rl + (size_t)mmin * nmin;
-    for (uint32_t e = threadIdx.x; e < nu; e += kAcaThreads) Ul[e] = U[e];
-    for (uint32_t e = threadIdx.x; e < nv; e += kAcaThreads) Vl[e] = V[e];
+    for (uint32_t e = threadIdx.x; e < nu; e += nthr()) Ul[e] = U[e];
+    for (uint32_t e = threadIdx.x; e < nv; e += nthr()) Vl[e] = V[e];
     __syncthreads();
     rc_body(B, res, k, eps, Ul, Vl, Rl, Zl, Ws, Zs, t1, t2, sh, wbuf, sig, order);
     __syncthreads();
     // the reflectors and their factors are what the expansion reads
-    for (uint32_t e = threadIdx.x; e < nu; e += kAcaThreads) U[e] = Ul[e];
-    for (uint32_t e = threadIdx.x; e < nv; e += kAcaThreads) V[e] = Vl[e];
-    for (uint32_t e = threadIdx.x; e < mmin; e += kAcaThreads) tau1[e] = t1[e];
-    for (uint32_t e = threadIdx.x; e < nmin; e += kAcaThreads) tau2[e] = t2[e];
+    for (uint32_t e = threadIdx.x; e < nu; e += nthr()) U[e] = Ul[e];
+    for (uint32_t e = threadIdx.x; e < nv; e += nthr()) V[e] = Vl[e];
+    for (uint32_t e = threadIdx.x; e < mmin; e += nthr()) tau1[e] = t1[e];
+    for (uint32_t e = threadIdx.x; e < nmin; e += nthr()) tau2[e] = t2[e];
   } else {
     rc_body(B, res, k, eps, U, V, R, Zm, Ws, Zs, tau1, tau2, sh, wbuf, sig, order);
   }
@@ -1735,19 +1741,21 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     parts.clear();
   };
   Grow gblk, gres, gslab, gscr, goff;
-  // second stream of the small-leaf ACA launches and its two ordering events
+  // second stream of the middle and small leaves and its ordering events
   struct Aux {
     hipStream_t s = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     ~Aux() {
       if (s) (void)hipStreamSynchronize(s);
       if (e0) (void)hipEventDestroy(e0);
       if (e1) (void)hipEventDestroy(e1);
+      if (e2) (void)hipEventDestroy(e2);
       if (s) (void)hipStreamDestroy(s);
     }
   } aux;
   AHM_HIP(hipEventCreateWithFlags(&aux.e0, hipEventDisableTiming));
   AHM_HIP(hipEventCreateWithFlags(&aux.e1, hipEventDisableTiming));
+  AHM_HIP(hipEventCreateWithFlags(&aux.e2, hipEventDisableTiming));
   // the slab scratch is sized once, for the largest batch there can be (the
   // budget, or all first attempts if they take less): growing it batch by
   // batch re-allocates tens of GiB, which is slow right after large frees
@@ -1860,14 +1868,25 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
       };
       launch(0, nlarge, kLargeThreads, s);
       launch(nlarge, nbig, kAcaThreads, s_small);
-      launch(nbig, nb, kSmallThreads, s_small);
+      // the recompression of the large and middle leaves waits for both
       if (s_small != s &&
           (hipEventRecord(aux.e1, s_small) != hipSuccess || hipStreamWaitEvent(s, aux.e1, 0) != hipSuccess)) {
         cleanup();
         return fail(AHM_ERR_HIP, "ACA stream ordering");
       }
-      hipLaunchKernelGGL(recompress_kernel, dim3(nb), dim3(kAcaThreads), kRcLds * sizeof(double), s,
-                         (const AcaBlock*)dblk.p, (AcaResult*)dres.p, eps, (double*)dslab.p);
+      launch(nbig, nb, kSmallThreads, s_small);
+      if (nbig)
+        hipLaunchKernelGGL(recompress_kernel, dim3((uint32_t)nbig), dim3(kAcaThreads), kRcLds * sizeof(double), s,
+                           (const AcaBlock*)dblk.p, (AcaResult*)dres.p, eps, (double*)dslab.p, 0);
+      if (nb > nbig)
+        hipLaunchKernelGGL(recompress_kernel, dim3(nb - (uint32_t)nbig), dim3(kSmallThreads),
+                           kCqLdsSmall * sizeof(double), s_small, (const AcaBlock*)dblk.p + nbig,
+                           (AcaResult*)dres.p + nbig, eps, (double*)dslab.p, 1);
+      if (s_small != s &&
+          (hipEventRecord(aux.e2, s_small) != hipSuccess || hipStreamWaitEvent(s, aux.e2, 0) != hipSuccess)) {
+        cleanup();
+        return fail(AHM_ERR_HIP, "ACA stream ordering");
+      }
       std::vector<AcaResult> rr(nb);
       if (hipGetLastError() != hipSuccess ||
           hipMemcpyAsync(rr.data(), dres.p, nb * sizeof(AcaResult), hipMemcpyDeviceToHost, s) !=
