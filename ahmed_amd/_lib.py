@@ -596,11 +596,13 @@ class HMatrix:
         return cv.value, ms.value, bool(fu.value)
 
     def set_fused(self, on: bool = True):
-        """Products of 1-8 right-hand sides as one persistent launch."""
+        """Products of 1-8 right-hand sides in one launch (each row panel
+        forms the V^T x of its own low-rank leaves)."""
         check(load().ahm_hmat_set_fused(self.h, int(on)), "ahm_hmat_set_fused")
 
     def fused_status(self) -> bool:
-        """True if a fused product gave up a bounded wait (invalid result)."""
+        """Always False: the single launch has no waits to give up (kept for
+        the C-ABI's status query)."""
         a = _i()
         check(load().ahm_hmat_fused_status(self.h, C.byref(a)), "ahm_hmat_fused_status")
         return bool(a.value)

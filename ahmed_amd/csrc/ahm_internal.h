@@ -48,6 +48,13 @@ struct GemvTask {
 // kChunkValues values; their row sums meet in a fixed-order reduction.
 constexpr uint64_t kChunkValues = 48 * 1024;
 
+// One low-rank leaf's U columns in a row panel (single-launch products):
+// panel columns col .. col+k-1 take V^T x from rows row0 .. row0+k-1 of the
+// V^T panel apanel.
+struct FusedGroup {
+  uint32_t col, k, apanel, row0;
+};
+
 struct ReduceTask {
   uint32_t row;      // first output row (t index in phase A, y row in phase B)
   uint32_t m;        // rows
@@ -246,7 +253,12 @@ struct ahm_hmat_s {
   uint64_t bf_vals = 0;
   ahm::DeviceBuf xbbuf;
   size_t xbbuf_cap = 0;       // bytes
-  int fused = 0;   // single-launch products: switched off (ahm_hmat_set_fused)
+  // single-launch products of 1..8 right-hand sides (ahm_hmat_set_fused):
+  // row panels in launch order, V^T panels, U-column groups and their
+  // offsets per row panel
+  int fused = 0, fz_built = 0;
+  ahm::DeviceBuf fz_panels, fz_apanels, fz_groups, fz_goff;
+  uint32_t fz_n = 0;
   ahm::DeviceBuf leafdata;    // optional mblock-layout copy (keep_leaves)
   std::vector<ahm_leaf_desc> leaves;
   // host copies of the plan (used by ahm_hmat_convert)

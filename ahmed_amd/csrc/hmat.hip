@@ -309,6 +309,151 @@ __global__ __launch_bounds__(kPanelThreads) void panel_gemvp_kernel(
 }
 
 // ------------------------------------------------------------------------
+// 1 <= p <= 8 right-hand sides in ONE launch (ahm_hmat_set_fused).  One
+// workgroup per row panel [D | U]; the V^T x values its U columns need are
+// formed by the workgroup itself from the V^T panels (each low-rank leaf's
+// t is used by its own row panel only), so no workgroup waits for another
+// and no chunk sums are needed.  X and Y are read and written in place
+// (p columns ldx / ldy apart), without the interleaved copies.  The price:
+// x of a column cluster is read once per low-rank leaf instead of once per
+// V^T panel, and the V^T values are read k at a time down a column of their
+// panel.
+//
+// Per chunk of kZChunkP panel columns the workgroup stages Z (8 values per
+// column, zero beyond p) in LDS: x columns directly, and each piece of a
+// FusedGroup in the chunk by one wavefront: lane (r, cg) (r < kk rows of the
+// piece, cg < 64/kk column groups) sums the V^T row r against x over the
+// columns cg, cg + 64/kk, ... of the V^T panel, and the column groups are
+// summed through LDS in order.  Then the panel stream runs as in the
+// p <= 8 kernel above.
+template <typename TS, typename TV>
+__global__ __launch_bounds__(kPanelThreads) void fused_kernel(
+    const GemvTask* __restrict__ panels, const GemvTask* __restrict__ apanels,
+    const FusedGroup* __restrict__ groups, const uint32_t* __restrict__ goff, const uint32_t* __restrict__ zsrc,
+    const TS* __restrict__ arena, const TV* __restrict__ X, uint32_t ldx, TV* __restrict__ Y, uint32_t ldy,
+    TV alpha, uint32_t p) {
+  constexpr int NP = 8;
+  __shared__ TV zs[kZChunkP * NP];            // also the cross-group reduction buffer
+  __shared__ TV wred[kPanelThreads / 64][64 * NP];
+  const GemvTask tk = panels[blockIdx.x];
+  const int m = (int)tk.m;
+  const int G = kPanelThreads / m;
+  const int tid = threadIdx.x;
+  const int row = tid % m, g = tid / m;
+  const bool active = g < G;
+  const int w = tid >> 6, lane = tid & 63;
+  const TS* P = arena + tk.data + row;
+  const uint32_t g0 = goff[blockIdx.x], g1 = goff[blockIdx.x + 1];
+  TV a[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) a[j] = 0;
+  for (uint32_t c0 = 0; c0 < tk.W; c0 += kZChunkP) {
+    const uint32_t cw = min((uint32_t)kZChunkP, tk.W - c0);
+    __syncthreads();
+    // x columns (the t columns are written by the pieces below)
+    for (uint32_t e = tid; e < cw * NP; e += kPanelThreads) {
+      const uint32_t c = e / NP, j = e % NP;
+      const uint32_t src = zsrc[tk.zoff + c0 + c];
+      if (!(src & kFromT)) zs[e] = j < p ? X[(uint64_t)j * ldx + src] : (TV)0;
+    }
+    // pieces of the groups in this chunk, round robin over the wavefronts
+    uint32_t q = 0;
+    for (uint32_t gi = g0; gi < g1; ++gi) {
+      const FusedGroup fg = groups[gi];
+      if (fg.col >= c0 + cw || fg.col + fg.k <= c0) continue;
+      if ((int)(q++ % (kPanelThreads / 64)) != w) continue;
+      const uint32_t cs = max(fg.col, c0), ce = min(fg.col + fg.k, c0 + cw);
+      const GemvTask A = apanels[fg.apanel];
+      const TS* Pa = arena + A.data + fg.row0 + (cs - fg.col);
+      for (uint32_t rb = 0; rb < ce - cs; rb += 64) {
+        const uint32_t nr = min(64u, ce - cs - rb);
+        const uint32_t r = lane % nr, cg = lane / nr, ncg = 64 / nr;
+        TV acc[NP];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) acc[j] = 0;
+        if (cg < ncg) {
+          // the V^T panel's columns are the x entries xb, xb+1, ... (checked
+          // when the plan is built); four columns' loads in flight
+          const TV* Xc = X + zsrc[A.zoff];
+          const TS* Pr = Pa + rb + r;
+          uint32_t cc = cg;
+          for (; cc + 3 * ncg < A.W; cc += 4 * ncg) {
+            TV v[4], xv[4][NP];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              v[u] = (TV)Pr[(uint64_t)(cc + u * ncg) * A.m];
+#pragma unroll
+              for (int j = 0; j < NP; ++j) xv[u][j] = (uint32_t)j < p ? Xc[(uint64_t)j * ldx + cc + u * ncg] : (TV)0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+              for (int j = 0; j < NP; ++j) acc[j] = fma(v[u], xv[u][j], acc[j]);
+          }
+          for (; cc < A.W; cc += ncg) {
+            const TV v = (TV)Pr[(uint64_t)cc * A.m];
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+              if ((uint32_t)j < p) acc[j] = fma(v, Xc[(uint64_t)j * ldx + cc], acc[j]);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < NP; ++j) wred[w][lane * NP + j] = cg < ncg ? acc[j] : (TV)0;
+        // the column groups are summed by other lanes of this wavefront
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < nr)
+#pragma unroll
+          for (int j = 0; j < NP; ++j) {
+            TV t = 0;
+            for (uint32_t qq = 0; qq < ncg; ++qq) t += wred[w][(lane + qq * nr) * NP + j];
+            zs[(cs - c0 + rb + lane) * NP + j] = t;
+          }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const TS* Pc = P + (uint64_t)c0 * m;
+      const uint32_t step = (uint32_t)(G * m);
+      uint32_t off = (uint32_t)g * m;
+      int c = g;
+      for (; c + 3 * G < (int)cw; c += 4 * G, off += 4 * step) {
+        const TV p0 = (TV)Pc[off];
+        const TV p1 = (TV)Pc[off + step];
+        const TV p2 = (TV)Pc[off + 2 * step];
+        const TV p3 = (TV)Pc[off + 3 * step];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+          a[j] = fma(p0, zs[NP * c + j], a[j]);
+          a[j] = fma(p1, zs[NP * (c + G) + j], a[j]);
+          a[j] = fma(p2, zs[NP * (c + 2 * G) + j], a[j]);
+          a[j] = fma(p3, zs[NP * (c + 3 * G) + j], a[j]);
+        }
+      }
+      for (; c < (int)cw; c += G, off += step) {
+        const TV p0 = (TV)Pc[off];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) a[j] = fma(p0, zs[NP * c + j], a[j]);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NP; ++j) zs[j * kPanelThreads + tid] = active ? a[j] : (TV)0;
+  __syncthreads();
+  if (tid < m)
+    for (uint32_t j = 0; j < p; ++j) {
+      TV sum = 0;
+      for (int q2 = 0; q2 < G; ++q2) sum += zs[j * kPanelThreads + q2 * m + tid];
+      Y[(uint64_t)j * ldy + tk.out + tid] += alpha * sum;
+    }
+}
+
+// ------------------------------------------------------------------------
 // p >= 9 right-hand sides: the same panels as a matrix product on the MFMA
 // units.
 //
@@ -1157,6 +1302,16 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
   hipStream_t s = h->ctx->stream;
   TV* t = (TV*)ws.t;
   TV* part = (TV*)ws.part;
+  if (h->fused && p <= 8 && !ws.gate[0].frozen) {
+    if (h->fz_n)
+      hipLaunchKernelGGL((fused_kernel<TS, TV>), dim3(h->fz_n), dim3(kPanelThreads), 0, s,
+                         (const GemvTask*)h->fz_panels.p, (const GemvTask*)h->fz_apanels.p,
+                         (const FusedGroup*)h->fz_groups.p, (const uint32_t*)h->fz_goff.p,
+                         (const uint32_t*)h->zsrc.p, (const TS*)h->arena.p, X, ldx, Y, ldy, alpha, p);
+    if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
+    AHM_HIP(hipGetLastError());
+    return AHM_OK;
+  }
   if (p == 1) {
     if (h->n_a)
       hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, false>), dim3(h->n_a, 1), dim3(kPanelThreads), 0, s,
@@ -1337,6 +1492,78 @@ static int build_bf16(ahm_hmat h) {
   AHM_HIP(hipGetLastError());
   AHM_HIP(hipStreamSynchronize(s));
   h->bf_vals = tot;
+  return AHM_OK;
+}
+
+// Plan of the single-launch product: the unchunked row panels (largest
+// work first), the unchunked V^T panels, and per row panel its groups of U
+// columns that take consecutive rows of one V^T panel (one group per
+// low-rank leaf).
+static int build_fused(ahm_hmat h) {
+  const std::vector<GemvTask>& B = h->h_panels;
+  const std::vector<GemvTask>& A = h->h_panels_a;
+  const uint32_t none = 0xFFFFFFFFu;
+  std::vector<uint32_t> ta(h->t_len, none), tr(h->t_len, 0);
+  for (size_t a = 0; a < A.size(); ++a) {
+    const GemvTask& tk = A[a];
+    if (tk.partial || (uint64_t)tk.out + tk.m > h->t_len)
+      return fail(AHM_ERR_INVALID, "single launch: V^T panel %zu out of range", a);
+    // the kernel takes the panel's x entries as a contiguous range
+    const uint32_t x0 = tk.W ? h->h_zsrc[tk.zoff] : 0;
+    if ((uint64_t)x0 + tk.W > h->n_cols) return fail(AHM_ERR_INVALID, "single launch: V^T panel %zu columns", a);
+    for (uint32_t c = 0; c < tk.W; ++c)
+      if (h->h_zsrc[tk.zoff + c] != x0 + c)
+        return fail(AHM_ERR_UNSUPPORTED, "single launch: V^T panel %zu columns are not one range", a);
+    for (uint32_t r = 0; r < tk.m; ++r) {
+      ta[tk.out + r] = (uint32_t)a;
+      tr[tk.out + r] = r;
+    }
+  }
+  std::vector<uint64_t> cost(B.size(), 0);
+  std::vector<std::vector<FusedGroup>> per(B.size());
+  for (size_t b = 0; b < B.size(); ++b) {
+    const GemvTask& tk = B[b];
+    if (tk.partial || tk.m == 0 || tk.m > (uint32_t)kPanelThreads || (uint64_t)tk.out + tk.m > h->n_rows)
+      return fail(AHM_ERR_INVALID, "single launch: row panel %zu", b);
+    cost[b] = (uint64_t)tk.m * tk.W;
+    for (uint32_t c = 0; c < tk.W; ++c) {
+      const uint32_t src = h->h_zsrc[tk.zoff + c];
+      if (!(src & kFromT)) continue;
+      const uint32_t t = src & ~kFromT;
+      if (t >= h->t_len || ta[t] == none) return fail(AHM_ERR_INVALID, "single launch: t source %u", t);
+      std::vector<FusedGroup>& gv = per[b];
+      if (!gv.empty()) {
+        FusedGroup& l = gv.back();
+        if (l.apanel == ta[t] && l.row0 + l.k == tr[t] && l.col + l.k == c) {
+          ++l.k;
+          continue;
+        }
+      }
+      gv.push_back(FusedGroup{c, 1, ta[t], tr[t]});
+    }
+    for (const FusedGroup& fg : per[b]) cost[b] += (uint64_t)fg.k * A[fg.apanel].W;
+  }
+  std::vector<uint32_t> order(B.size());
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
+  std::vector<GemvTask> panels(B.size());
+  std::vector<FusedGroup> groups;
+  std::vector<uint32_t> goff(B.size() + 1, 0);
+  for (size_t q = 0; q < order.size(); ++q) {
+    panels[q] = B[order[q]];
+    goff[q] = (uint32_t)groups.size();
+    groups.insert(groups.end(), per[order[q]].begin(), per[order[q]].end());
+  }
+  goff[B.size()] = (uint32_t)groups.size();
+  hipStream_t s = h->ctx->stream;
+  AHM_HIP(hipStreamSynchronize(s));
+  AHM_TRY(upload_vec(h->fz_panels, panels, s));
+  AHM_TRY(upload_vec(h->fz_apanels, A, s));
+  AHM_TRY(upload_vec(h->fz_groups, groups, s));
+  AHM_TRY(upload_vec(h->fz_goff, goff, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  h->fz_n = (uint32_t)panels.size();
+  h->fz_built = 1;
   return AHM_OK;
 }
 
@@ -1696,17 +1923,17 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
 
 int ahm_hmat_set_fused(ahm_hmat h, int on) {
   if (!h) return fail(AHM_ERR_INVALID, "ahm_hmat_set_fused: NULL handle");
-  if (on)
-    return fail(AHM_ERR_UNSUPPORTED,
-                "ahm_hmat_set_fused: the single-launch product is switched off (its first GPU test did not "
-                "finish; cause not found)");
-  h->fused = 0;
+  if (on && !h->fz_built) {
+    AHM_HIP(hipSetDevice(h->ctx->device));
+    AHM_TRY(build_fused(h));
+  }
+  h->fused = on ? 1 : 0;
   return AHM_OK;
 }
 
 int ahm_hmat_fused_status(ahm_hmat h, int* aborted) {
   if (!h || !aborted) return fail(AHM_ERR_INVALID, "ahm_hmat_fused_status: NULL argument");
-  *aborted = 0;
+  *aborted = 0;   // the single launch has no waits to give up
   return AHM_OK;
 }
 
@@ -1734,8 +1961,12 @@ int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint
   AHM_HIP(hipEventCreate(&e0));
   AHM_HIP(hipEventCreate(&e1));
   int st = AHM_OK;
-  for (int fu = 0; fu <= 0 && !st; ++fu) {   // the single-launch form is switched off
+  const int saved_fused = h->fused;
+  for (int fu = 0; fu <= (p <= 8 ? 1 : 0) && !st; ++fu) {
+    if (fu && !h->fz_built && (st = build_fused(h))) break;
+    h->fused = fu;
     for (uint64_t cv : cand) {
+      if (fu && cv != cand[0]) break;   // the single launch runs unchunked panels
       if ((st = set_chunking(h, cv))) break;
       if ((st = ensure_tbuf(h, p, es)) || (st = ensure_pbuf(h, p, es))) break;
       auto run = [&]() {
@@ -1763,6 +1994,7 @@ int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  h->fused = st ? saved_fused : arg_fused;
   if (st) return st;
   AHM_TRY(set_chunking(h, arg));
   if (chunk_values) *chunk_values = arg;

@@ -221,8 +221,9 @@ def small_system_gmres(ctx, torch):
 def bench_batched(args):
     """64 independent H-matrices, N log-uniform in [4096, 65536] on seeded
     ellipsoids with axis ratios U(1,4), kernel exp(-k r)/(4 pi r) with
-    k ~ U(0,5), ACA eps 1e-6, 8 RHS each: one batched assembly, one launch per
-    phase for every product, batched GMRES(50) with hipGraph-captured steps."""
+    k ~ U(0,5), ACA eps 1e-6, 8 RHS each: one batched assembly, every
+    product in the form the autotuner picks (three launches or one), batched
+    GMRES(50) with hipGraph-captured steps."""
     import torch
     from ahmed_amd import _lib, geometry
     probs = geometry.uneven_batch()
@@ -247,8 +248,16 @@ def bench_batched(args):
     t0 = time.perf_counter()
     tile, ms_tuned, fused = H.autotune(p=p, reps=3)
     t_tune = time.perf_counter() - t0
-    log(f"autotuner ({t_tune:.1f} s): tile {tile} values, {'one persistent launch' if fused else 'three launches'}, "
+    log(f"autotuner ({t_tune:.1f} s): tile {tile} values, {'one launch' if fused else 'three launches'}, "
         f"{ms_tuned:.3f} ms (default {ms_default:.3f} ms)")
+    # both forms at the tuned tile size, then the tuned form for the line
+    forms = {}
+    for f in (False, True):
+        H.set_fused(f)
+        forms["single_launch_ms" if f else "three_launches_ms"] = round(
+            timed(lambda: H.matvec(X, Y), torch, args.steps, args.warmup), 4)
+    H.set_fused(fused)
+    log(f"8 RHS: three launches {forms['three_launches_ms']:.3f} ms, one launch {forms['single_launch_ms']:.3f} ms")
     ms = timed(lambda: H.matvec(X, Y), torch, args.steps, args.warmup)
     B_alg = 8.0 * st["nvals"] + 8.0 * p * 3 * Ntot
     F_alg = 2.0 * p * st["nvals"]
@@ -293,7 +302,7 @@ def bench_batched(args):
                    "nvals": st["nvals"]},
         "TF/s": round(F_alg / ms / 1e9, 3), "matvec_1rhs_ms": round(ms1, 4),
         "autotune": {"tile_values": tile, "fused_single_launch": fused, "seconds": round(t_tune, 2),
-                     "default_form_ms": round(ms_default, 4)},
+                     "default_form_ms": round(ms_default, 4), **forms},
         "fused_wait_aborted": H.fused_status(),
         "assembly_s": round(t_asm, 3), "aca_retries": ast["n_retries"], "gmres_batch": res,
     }

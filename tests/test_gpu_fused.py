@@ -16,10 +16,11 @@ import pytest
 from helpers import block_matrix, leaves_of, relerr, sphere_hmatrix
 
 pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="the single-launch product did not finish in its first GPU run "
-                                      "(torch.cuda.synchronize blocked > 120 s in "
-                                      "test_fused_equals_three_launches_sphere[1]); the cause was not found, so "
-                                      "the form is switched off (ahm_hmat_set_fused refuses it)")]
+              pytest.mark.skip(reason="tests of the removed work-queue form of the single launch, which did "
+                                      "not finish in its first GPU run (torch.cuda.synchronize blocked > 120 s in "
+                                      "test_fused_equals_three_launches_sphere[1]; cause not found); its "
+                                      "successor, without waits between workgroups, is tested in "
+                                      "test_gpu_single_launch.py")]
 torch = pytest.importorskip("torch")
 
 
