@@ -96,6 +96,7 @@ SIGNATURES = {
     "ahm_hmat_segments": (_i, [_p, C.POINTER(_u32), _p]),
     "ahm_gmres_batch": (_i, [_p, _p, _p, _d, _u32, _u32, _i, _p, _p, _p]),
     "ahm_hmat_set_graphs": (_i, [_p, _i]),
+    "ahm_hmat_set_arnoldi": (_i, [_p, _i]),
     "ahm_rank_predict": (_i, [_p, _p, _u32, _p, C.c_float, _u32, _p, _p, _p]),
     "ahm_hmat_matvec": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
     "ahm_hmat_matvec_noalloc": (_i, [_p, _d, _p, _u32, _p, _u32, _u32, _i]),
@@ -414,7 +415,7 @@ class HMatrix:
         check(load().ahm_hmat_segments(self.h, C.byref(ns), off.ctypes.data_as(_p)), "segments")
         return off
 
-    def gmres_batch(self, b, x, eps: float = 1e-8, m: int = 50, nsteps: int = 10000, use_graph: bool = True):
+    def gmres_batch(self, b, x, eps: float = 1e-8, m: int = 50, nsteps: int = 10000, use_graph: bool = False):
         S = len(self.segments()) - 1
         e = np.zeros(S)
         st = np.zeros(S, np.uint32)
@@ -554,9 +555,15 @@ class HMatrix:
                     hist=hist[:nhist].copy())
 
     def set_graphs(self, on: bool = True):
-        """GMRES steps of this handle replayed from captured hipGraphs (default)
-        or launched one by one."""
+        """GMRES steps of this handle replayed from captured hipGraphs, or
+        launched one by one (the default)."""
         check(load().ahm_hmat_set_graphs(self.h, int(on)), "ahm_hmat_set_graphs")
+
+    def set_arnoldi(self, mode: int = 0):
+        """Form of the GMRES orthogonalisation and rotations: 1 chunked
+        launches, 2 one workgroup per system (systems up to 16384 rows),
+        0 automatic.  Both forms give bit-identical iterates."""
+        check(load().ahm_hmat_set_arnoldi(self.h, int(mode)), "ahm_hmat_set_arnoldi")
 
     def cg(self, b, x, eps: float = 1e-8, nsteps: int = 10000, prec=None):
         e, ns = _d(eps), _u32(nsteps)

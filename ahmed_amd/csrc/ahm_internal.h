@@ -275,7 +275,8 @@ struct ahm_hmat_s {
     size_t contrib_cap = 0, tprime_cap = 0;
   } tr;
   ahm::GmresCache* gm = nullptr;   // GMRES buffers and captured steps (gmres.hip)
-  int solver_graphs = 1;           // GMRES steps replayed as hipGraphs (ahm_hmat_set_graphs)
+  int solver_graphs = 0;           // GMRES steps replayed as hipGraphs (ahm_hmat_set_graphs)
+  int arnoldi = 0;                 // GMRES orthogonalisation form (ahm_hmat_set_arnoldi)
 };
 
 // triangular H-matrix factor (trih.hip): work queue, arena, counters

@@ -64,8 +64,40 @@ struct State {
   uint32_t *frozen, *conv, *taken, *kcyc, *kupd, *rsteps, *restarts, *zero_b, *nh, *nhist, *flag;
 };
 
+// Wavefront sum by the xor butterfly v += v[lane ^ o], o = 32, 16, .., 1,
+// valid in every lane, with VALU exchanges instead of LDS permutes: xor 32
+// and 16 by the lane-half swaps, 8 by a row rotation, and 4 and 2 by row
+// rotations too, which read a lane equal to lane ^ o once the earlier steps
+// have made lanes l, l ^ 8 (and l ^ 4) equal; 1 by a quad permutation.
+// Every lane adds the same two values as the butterfly's lane, so the sum
+// is bit for bit the butterfly's.
+template <int CTRL>
+__device__ __forceinline__ double gm_dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double gm_xor32(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  const bool up = threadIdx.x & 32;
+  return __hiloint2double((int)(up ? b[0] : b[1]), (int)(up ? a[0] : a[1]));
+}
+__device__ __forceinline__ double gm_xor16(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  const bool up = threadIdx.x & 16;
+  return __hiloint2double((int)(up ? b[0] : b[1]), (int)(up ? a[0] : a[1]));
+}
 __device__ __forceinline__ double gm_wsum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v += gm_xor32(v);
+  v += gm_xor16(v);
+  v += gm_dpp<0x128>(v);   // row_ror:8
+  v += gm_dpp<0x124>(v);   // row_ror:4
+  v += gm_dpp<0x122>(v);   // row_ror:2
+  v += gm_dpp<0xB1>(v);    // quad_perm [1,0,3,2]
   return v;
 }
 
@@ -263,23 +295,21 @@ __device__ __forceinline__ void gm_app_rot(double& dx, double& dy, double cs, do
   dx = t;
 }
 
-// Rotations of step i (solvers/GMRES.cpp:140-157), one workgroup per
-// system: the new Hessenberg column and the rotations are staged in LDS,
-// one lane applies the i old rotations, generates the new one, rotates s
-// and tests |s_{i+1}| / |b| < eps.  A converged system is frozen with
-// rsteps = its step count (the reference's j).
-__global__ __launch_bounds__(64) void gm_givens_kernel(State st, uint32_t m, uint32_t i) {
-  extern __shared__ double gl[];   // column (m+2), cs (m+1), sn (m+1)
-  const uint32_t s = blockIdx.x;
-  if (st.frozen[s]) return;
+// Rotations of step i (solvers/GMRES.cpp:140-157) for system s, run by one
+// workgroup: the new Hessenberg column and the rotations are staged in LDS
+// (gl: column m+2, cs m+1, sn m+1), one lane applies the i old rotations,
+// generates the new one, rotates s and tests |s_{i+1}| / |b| < eps.  A
+// converged system is frozen with rsteps = its step count (the reference's
+// j).  The history is kept when there is one system (one workgroup).
+__device__ void gm_givens_sys(const State& st, uint32_t m, uint32_t i, uint32_t s, double* gl) {
   double* Hc = st.H + (size_t)s * (m + 1) * m + (size_t)i * (m + 1);
   double* cs = st.cs + (size_t)s * (m + 1);
   double* sn = st.sn + (size_t)s * (m + 1);
   double* col = gl;
   double* lc = gl + (m + 2);
   double* ls = lc + (m + 1);
-  for (uint32_t k = threadIdx.x; k <= i + 1; k += 64) col[k] = Hc[k];
-  for (uint32_t k = threadIdx.x; k < i; k += 64) {
+  for (uint32_t k = threadIdx.x; k <= i + 1; k += blockDim.x) col[k] = Hc[k];
+  for (uint32_t k = threadIdx.x; k < i; k += blockDim.x) {
     lc[k] = cs[k];
     ls[k] = sn[k];
   }
@@ -318,7 +348,175 @@ __global__ __launch_bounds__(64) void gm_givens_kernel(State st, uint32_t m, uin
     }
   }
   __syncthreads();
-  for (uint32_t k = threadIdx.x; k <= i + 1; k += 64) Hc[k] = col[k];
+  for (uint32_t k = threadIdx.x; k <= i + 1; k += blockDim.x) Hc[k] = col[k];
+}
+
+__global__ __launch_bounds__(64) void gm_givens_kernel(State st, uint32_t m, uint32_t i) {
+  extern __shared__ double gl[];
+  if (st.frozen[blockIdx.x]) return;
+  gm_givens_sys(st, m, i, blockIdx.x, gl);
+}
+
+// Arnoldi step i after the product, for systems of at most kArChunks
+// chunks: one workgroup per system runs what the chunked launches do -- the
+// partials of <w, v_0>, the MGS passes k = 0..i, the normalisation -- and
+// then the rotations, in one launch instead of i + 4.
+//
+// Thread T works as thread T % kGmThreads of the chunked kernels'
+// workgroup, on chunks T / kGmThreads + kArGroups j: the same rows in the
+// same order and the same wavefront sums.  Every thread then adds the
+// chunk partials itself in the order gm_group_sum and gm_seg_sum add them
+// (the partial of chunk c in lane c of wavefront 0 and zeros elsewhere,
+// the xor butterfly, the four wavefront sums), so the iterates are bit for
+// bit those of the chunked form.  w and the next basis vector stay in
+// registers; v_{k+2} is loaded while pass k's partials are summed, and one
+// barrier per pass orders the partials (two LDS buffers in turn).
+constexpr int kArGroups = 4;
+constexpr int kArThreads = kArGroups * kGmThreads;
+constexpr uint32_t kArChunks = 16;
+constexpr int kWaves = kGmThreads / 64;
+// auto choice: one workgroup per system for systems of up to this many chunks
+constexpr uint32_t kArAutoChunks = 16;
+
+// h from the pass's wavefront sums ws[c][q] of chunks c < nc (nc <= 64):
+// every wavefront forms what wavefront 0 of gm_seg_sum holds (chunk c's
+// partial in lane c, zeros elsewhere) and runs the same butterfly; the
+// other three wavefront sums of gm_group_sum are zeros
+__device__ __forceinline__ double ar_sum(const double (*ws)[kWaves], uint32_t nc) {
+  const uint32_t l = threadIdx.x & 63;
+  double v = 0.0;
+  if (l < nc) {
+    double p = 0.0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) p += ws[l][q];
+    v += p;
+  }
+  v = gm_wsum(v);
+  double h = 0.0;
+  h += v;
+#pragma unroll
+  for (int q = 1; q < kWaves; ++q) h += 0.0;
+  return h;
+}
+
+template <int CPT>
+__global__ __launch_bounds__(kArThreads) void gm_arnoldi_kernel(double* __restrict__ V, size_t n,
+                                                                const SegChunk* __restrict__ ch,
+                                                                const uint32_t* __restrict__ cb, State st,
+                                                                uint32_t m, uint32_t i) {
+  constexpr int R = kChunk / kGmThreads;   // rows per thread and chunk
+  constexpr int E = CPT * R;               // rows per thread
+  constexpr bool kPrefetch = CPT <= 2;     // three vectors of E rows fit the registers
+  extern __shared__ double gl[];
+  __shared__ double ws[2][kArChunks][kWaves];
+  const uint32_t s = blockIdx.x;
+  if (st.frozen[s]) return;
+  const uint32_t c0 = cb[s], nc = cb[s + 1] - c0;
+  const uint32_t g = threadIdx.x / kGmThreads, t = threadIdx.x % kGmThreads;
+  const int wave = (int)(t >> 6);
+  const bool lead = (t & 63) == 0;
+  // chunk j of this thread: rows beg[j] + kGmThreads q < end[j] (none past nc)
+  uint32_t beg[CPT], end[CPT];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const uint32_t c = g + kArGroups * j;
+    beg[j] = end[j] = 0;
+    if (c < nc) {
+      const SegChunk C = ch[c0 + c];
+      beg[j] = C.begin + t;
+      end[j] = C.end;
+    }
+  }
+#define AR_ROW(j, q) (beg[j] + (uint32_t)(kGmThreads * (q)))
+#define AR_OK(j, q) (AR_ROW(j, q) < end[j])
+  auto load = [&](const double* __restrict__ src, double* dst) {
+#pragma unroll
+    for (int j = 0; j < CPT; ++j)
+#pragma unroll
+      for (int q = 0; q < R; ++q) dst[j * R + q] = AR_OK(j, q) ? src[AR_ROW(j, q)] : 0.0;
+  };
+  // each chunk's fma chain over its rows, wavefront sums into ws[buf]
+  auto partials = [&](const double* a, const double* b, int buf) {
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const uint32_t c = g + kArGroups * j;
+      if (c >= nc) continue;   // uniform in the wavefront
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        if (AR_OK(j, q)) acc = fma(a[j * R + q], b[j * R + q], acc);
+      acc = gm_wsum(acc);
+      if (lead) ws[buf][c][wave] = acc;
+    }
+  };
+  double* __restrict__ w = V + (size_t)(i + 1) * n;
+  double* Hc = st.H + (size_t)s * (m + 1) * m + (size_t)i * (m + 1);
+  double wv[E], va[E];
+  load(w, wv);
+  load(V, va);
+  partials(wv, va, 0);
+  int buf = 0;
+  if constexpr (kPrefetch) {
+    // three vectors in registers, in turn: pass k updates by v_k, forms the
+    // partials with v_{k+1}, and loads v_{k+3} into v_k's registers, two
+    // passes before its first use
+    double rb[E], rc[E];
+    if (i >= 1) load(V + n, rb);
+    if (i >= 2) load(V + 2 * n, rc);
+    auto pass = [&](double* cur, double* nxt, uint32_t k) {
+      __syncthreads();
+      const double h = ar_sum(ws[buf], nc);
+      if (threadIdx.x == 0) Hc[k] = h;
+#pragma unroll
+      for (int x = 0; x < E; ++x) wv[x] = fma(-h, cur[x], wv[x]);   // rows past end: unused
+      buf ^= 1;
+      if (k < i) {
+        partials(wv, nxt, buf);
+        if (k + 3 <= i) load(V + (size_t)(k + 3) * n, cur);
+        return true;
+      }
+      partials(wv, wv, buf);
+      return false;
+    };
+    for (uint32_t k = 0;; k += 3) {
+      if (!pass(va, rb, k)) break;
+      if (!pass(rb, rc, k + 1)) break;
+      if (!pass(rc, va, k + 2)) break;
+    }
+  } else {
+    // two vectors in registers: v_{k+1} is loaded after the update by v_k
+    for (uint32_t k = 0; k <= i; ++k) {
+      __syncthreads();
+      const double h = ar_sum(ws[buf], nc);
+      if (threadIdx.x == 0) Hc[k] = h;
+#pragma unroll
+      for (int x = 0; x < E; ++x) wv[x] = fma(-h, va[x], wv[x]);
+      buf ^= 1;
+      if (k < i) {
+        load(V + (size_t)(k + 1) * n, va);
+        partials(wv, va, buf);
+      } else {
+        partials(wv, wv, buf);
+      }
+    }
+  }
+  __syncthreads();
+  const double h = sqrt(ar_sum(ws[buf], nc));
+  if (threadIdx.x == 0) Hc[i + 1] = h;
+  if (h != 0.0) {
+    const double inv = 1.0 / h;
+#pragma unroll
+    for (int x = 0; x < E; ++x) wv[x] *= inv;
+  }
+#pragma unroll
+  for (int j = 0; j < CPT; ++j)
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+      if (AR_OK(j, q)) w[AR_ROW(j, q)] = wv[j * R + q];
+#undef AR_ROW
+#undef AR_OK
+  __syncthreads();   // thread 0's column entries visible to the workgroup
+  gm_givens_sys(st, m, i, s, gl);
 }
 
 // update (solvers/GMRES.cpp:58-77), first half: y = H^{-1} s on the k x k
@@ -434,6 +632,8 @@ __global__ __launch_bounds__(kGmThreads) void gm_test_kernel(const double* __res
 struct GmresCache {
   int dev = 0;
   uint32_t n = 0, m = 0, S = 0, nch = 0, nhist_cap = 0;
+  uint32_t max_nc = 0;          // chunks of the largest system
+  int ar_one = -1;              // form of the captured steps: 1 one workgroup per system, 0 chunked
   ahm_dtype vec = AHM_F64;
   bool batch = false, prec = false;
   std::vector<uint64_t> sig;
@@ -570,6 +770,7 @@ int get_cache(ahm_hmat h, const GmresArgs& a, GmresCache*& out) {
     }
     cb[S] = (uint32_t)chunks.size();
     c->nch = cb[S];
+    for (uint32_t s = 0; s < S; ++s) c->max_nc = std::max(c->max_nc, cb[s + 1] - cb[s]);
     AHM_HIP(hipStreamCreateWithFlags(&c->gs, hipStreamNonBlocking));
     AHM_TRY(c->dch.alloc(chunks.size() * sizeof(SegChunk)));
     AHM_TRY(c->dcb.alloc(cb.size() * 4));
@@ -642,12 +843,23 @@ int get_cache(ahm_hmat h, const GmresArgs& a, GmresCache*& out) {
   return AHM_OK;
 }
 
+// The Arnoldi form of a solve: mode 1 the chunked launches, 2 one workgroup
+// per system where every system has at most kArChunks chunks, 0 (auto) the
+// latter where it is also the faster
+bool arnoldi_one(int mode, uint32_t S, uint32_t max_nc) {
+  if (max_nc > kArChunks || mode == 1) return false;
+  if (mode == 2) return true;
+  (void)S;
+  return max_nc <= kArAutoChunks;
+}
+
 struct Run {
   ahm_hmat h;
   GmresCache& c;
   const GmresArgs& a;
   hipStream_t gs;
   uint32_t n, m, S, nch;
+  bool one;   // the orthogonalisation and rotations in gm_arnoldi_kernel
   const SegChunk* ch() const { return (const SegChunk*)c.dch.p; }
   const uint32_t* cb() const { return (const uint32_t*)c.dcb.p; }
   double* V() const { return (double*)c.dV.p; }
@@ -683,6 +895,21 @@ struct Run {
       src = xh;
     }
     AHM_TRY(op(1.0, src, w));
+    if (one) {
+      const size_t lds = (size_t)(3 * m + 4) * 8;
+      if (c.max_nc <= kArGroups)
+        hipLaunchKernelGGL(gm_arnoldi_kernel<1>, dim3(S), dim3(kArThreads), lds, gs, V(), (size_t)n, ch(), cb(),
+                           c.st, m, i);
+      else if (c.max_nc <= 2 * kArGroups)
+        hipLaunchKernelGGL(gm_arnoldi_kernel<2>, dim3(S), dim3(kArThreads), lds, gs, V(), (size_t)n, ch(), cb(),
+                           c.st, m, i);
+      else
+        hipLaunchKernelGGL(gm_arnoldi_kernel<4>, dim3(S), dim3(kArThreads), lds, gs, V(), (size_t)n, ch(), cb(),
+                           c.st, m, i);
+      AHM_HIP(hipGetLastError());
+      AHM_HIP(hipMemcpyAsync(c.hflags + (size_t)i * S, c.st.frozen, (size_t)S * 4, hipMemcpyDeviceToHost, gs));
+      return AHM_OK;
+    }
     hipLaunchKernelGGL(gm_dot_kernel, dim3(nch), dim3(kGmThreads), 0, gs, (const double*)w, (const double*)V(),
                        ch(), frozen(), p0());
     double* pin = p0();
@@ -769,7 +996,12 @@ int gmres_run(ahm_hmat h, const GmresArgs& a) {
     c.drop_graphs();
     c.sig = sig;
   }
-  Run R{h, c, a, c.gs, c.n, c.m, c.S, c.nch};
+  const int one = arnoldi_one(h->arnoldi, c.S, c.max_nc) ? 1 : 0;
+  if (one != c.ar_one) {
+    c.drop_graphs();
+    c.ar_one = one;
+  }
+  Run R{h, c, a, c.gs, c.n, c.m, c.S, c.nch, one != 0};
   const uint32_t S = c.S, m = c.m, n = c.n;
   hipStream_t user = h->ctx->stream;
   hipEvent_t ev0;

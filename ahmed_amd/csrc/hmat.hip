@@ -1953,7 +1953,7 @@ int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint
   AHM_HIP(hipMemsetAsync(dy.p, 0, (size_t)h->n_rows * p * es, s));
   // tile sizes (values per row-panel / V^T task) times the launch form: three
   // kernels (phases A, B, chunk sums) or one persistent launch (p <= 8)
-  const uint64_t cand[] = {16u << 10, 32u << 10, 48u << 10, 96u << 10, 192u << 10, 1ull << 40};
+  const uint64_t cand[] = {4u << 10, 8u << 10, 16u << 10, 32u << 10, 48u << 10, 96u << 10, 192u << 10, 1ull << 40};
   double best = 1e300;
   uint64_t arg = h->chunk_values;
   int arg_fused = h->fused;

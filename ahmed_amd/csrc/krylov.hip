@@ -289,6 +289,13 @@ int ahm_hmat_set_graphs(ahm_hmat h, int on) {
   return AHM_OK;
 }
 
+int ahm_hmat_set_arnoldi(ahm_hmat h, int mode) {
+  if (!h) return fail(AHM_ERR_INVALID, "ahm_hmat_set_arnoldi: NULL handle");
+  if (mode < 0 || mode > 2) return fail(AHM_ERR_INVALID, "ahm_hmat_set_arnoldi: mode %d", mode);
+  h->arnoldi = mode;
+  return AHM_OK;
+}
+
 int ahm_gmres(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m, uint32_t* nsteps,
               double* hist, uint32_t nhist, uint32_t* nmatvec) {
   if (!h || !b || !x || !eps || !nsteps || m == 0)

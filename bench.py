@@ -188,9 +188,11 @@ def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, y_gpu_of, budget_s=20.0, 
 def small_system_gmres(ctx, torch):
     """GMRES(30) to 1e-8 on the reference's own example (unit sphere N=8192,
     leaf 32, eta 0.8, ACA eps 1e-6, Laplace with quadrature weight 4 pi/N,
-    b = A*1), where a step's launches, not its product, set the step time:
-    captured steps replayed as hipGraphs against one-by-one launches.  Each
-    mode: one untimed solve, then the median of 5 solves."""
+    b = A*1), where a step's launches, not its product, set the step time.
+    Modes: one-by-one launches (the default) and captured steps replayed as
+    hipGraphs, each with the orthogonalisation and rotations in one
+    workgroup per system (the automatic choice at this size) and in chunked
+    launches.  Each mode: one untimed solve, then the median of 5 solves."""
     from ahmed_amd import _lib, geometry
     N = 8192
     tree = _lib.Tree(geometry.sphere(N, 1), 32, 0.8)
@@ -198,8 +200,11 @@ def small_system_gmres(ctx, torch):
     b = torch.zeros(N, dtype=torch.float64, device="cuda")
     H.matvec(torch.ones(N, dtype=torch.float64, device="cuda"), b)
     out = {"workload": "unit sphere N=8192, leaf 32, eta 0.8, ACA eps 1e-6, Laplace 4 pi/N, GMRES(30) to 1e-8"}
-    for graph in (True, False):
+    xs_ref = None
+    for name, graph, mode in (("eager", False, 0), ("graph", True, 0), ("eager_chunked", False, 1),
+                              ("graph_chunked", True, 1)):
         H.set_graphs(graph)
+        H.set_arnoldi(mode)
         dts = []
         for rep in range(6):
             xs = torch.zeros(N, dtype=torch.float64, device="cuda")
@@ -208,12 +213,17 @@ def small_system_gmres(ctx, torch):
             r = H.gmres(b, xs, eps=1e-8, m=30, nsteps=300)
             torch.cuda.synchronize()
             dts.append(time.perf_counter() - t0)
+        if xs_ref is None:
+            xs_ref = xs
         t = float(np.median(dts[1:]))
-        out["graph" if graph else "eager"] = {"seconds": round(t, 5), "iterations": r["nsteps"],
-                                              "converged": r["converged"],
-                                              "us_per_step": round(t / max(r["nsteps"], 1) * 1e6, 1)}
-        log(f"GMRES(30) N=8192 {'graph' if graph else 'eager'}: {t * 1e3:.2f} ms, {r['nsteps']} steps")
+        out[name] = {"seconds": round(t, 5), "iterations": r["nsteps"], "converged": r["converged"],
+                     "us_per_step": round(t / max(r["nsteps"], 1) * 1e6, 1),
+                     "x_equal_to_first_mode": bool(torch.equal(xs, xs_ref))}
+        log(f"GMRES(30) N=8192 {name}: {t * 1e3:.2f} ms, {r['nsteps']} steps")
+    H.set_arnoldi(0)
+    H.set_graphs(False)
     out["graph_speedup_vs_eager"] = round(out["eager"]["seconds"] / out["graph"]["seconds"], 3)
+    out["one_workgroup_speedup_vs_chunked"] = round(out["eager_chunked"]["seconds"] / out["eager"]["seconds"], 3)
     del H
     return out
 
@@ -271,7 +281,7 @@ def bench_batched(args):
     # each mode twice: the first graph call captures the Arnoldi steps (kept
     # with the handle), the second replays them
     res = {}
-    for graph in (True, False):
+    for name, graph in (("eager", False), ("graph", True)):
         dts = []
         for rep in range(2):
             xs = torch.zeros(Ntot, dtype=torch.float64, device="cuda")
@@ -280,17 +290,15 @@ def bench_batched(args):
             r = H.gmres_batch(b, xs, eps=1e-8, m=50, nsteps=2000, use_graph=graph)
             torch.cuda.synchronize()
             dts.append(time.perf_counter() - t0)
-        res["graph" if graph else "eager"] = dict(seconds=round(dts[1], 4), first_call_s=round(dts[0], 4),
-                                                  all_converged=bool(r["converged"].all()),
-                                                  max_steps=int(r["nsteps"].max()),
-                                                  mean_steps=float(r["nsteps"].mean()),
-                                                  max_err_vs_ones=float((xs - 1).abs().max()))
-        log(f"batched GMRES(50) {'graph' if graph else 'eager'}: {dts[1]:.4f} s "
+        res[name] = dict(seconds=round(dts[1], 4), first_call_s=round(dts[0], 4),
+                         all_converged=bool(r["converged"].all()), max_steps=int(r["nsteps"].max()),
+                         mean_steps=float(r["nsteps"].mean()), max_err_vs_ones=float((xs - 1).abs().max()))
+        log(f"batched GMRES(50) {name}: {dts[1]:.4f} s "
             f"(first call {dts[0]:.4f} s), steps max {r['nsteps'].max()} mean {r['nsteps'].mean():.2f}")
     res["graph_speedup_vs_eager"] = round(res["eager"]["seconds"] / res["graph"]["seconds"], 3)
     # converged systems are frozen on the device: their product tasks stop,
     # so the products cost about mean/max of a lockstep solve's
-    res["lockstep_products_avoided"] = round(1.0 - res["graph"]["mean_steps"] / res["graph"]["max_steps"], 3)
+    res["lockstep_products_avoided"] = round(1.0 - res["eager"]["mean_steps"] / res["eager"]["max_steps"], 3)
     line = {
         "metric": "batched H-matvec HBM GB/s (64 independent H-matrices, 8 RHS)",
         "value": round(B_alg / ms / 1e6, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -666,12 +674,12 @@ def main():
     # ---- GMRES(50) time-to-tolerance: FP64 vs FP64 refinement around FP32 ----
     b = torch.zeros(N, dtype=torch.float64, device="cuda")
     H.matvec(torch.ones(N, dtype=torch.float64, device="cuda"), b)   # b = H * 1
-    # each solve twice: the first call captures the Arnoldi steps into
-    # hipGraphs kept with the handle, the timed second call replays them;
-    # "fp64_eager" launches the same kernels one by one
+    # each solve twice, the second timed; "fp64_graph": the first call
+    # captures the Arnoldi steps into hipGraphs kept with the handle, the
+    # timed second call replays them (the others launch the kernels one by one)
     gm = {}
-    for mode in ("fp64", "fp64_eager", "mixed"):
-        H.set_graphs(mode != "fp64_eager")
+    for mode in ("fp64", "fp64_graph", "mixed"):
+        H.set_graphs(mode == "fp64_graph")
         dts = []
         for rep in range(2):
             xs = torch.zeros(N, dtype=torch.float64, device="cuda")
@@ -695,10 +703,10 @@ def main():
         gm[mode] = dict(seconds=round(dt, 4), converged=r["converged"], reported_residual=r["eps"],
                         true_residual_fp64=true, max_err_vs_ones=err, **info)
         log(f"GMRES(50) {mode}: {dt:.3f} s, residual {true:.2e}, {info}")
-    H.set_graphs(True)
+    H.set_graphs(False)
     gm["tolerance"] = tol
     gm["speedup_mixed_vs_fp64"] = round(gm["fp64"]["seconds"] / gm["mixed"]["seconds"], 3)
-    gm["graph_speedup_vs_eager"] = round(gm["fp64_eager"]["seconds"] / gm["fp64"]["seconds"], 3)
+    gm["graph_speedup_vs_eager"] = round(gm["fp64"]["seconds"] / gm["fp64_graph"]["seconds"], 3)
     # the same geometry with a stronger off-diagonal part (weight alpha * 6/N):
     # a system that needs tens of GMRES iterations, where the refinement's
     # FP32 inner products can outweigh its FP64 residual products

@@ -265,7 +265,7 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
                             uint32_t ldy, uint32_t p, ahm_dtype vec, uint32_t reps,
                             double* ms_phase_a, double* ms_phase_b);
 /* Autotuner: times `reps` products with p right-hand sides for each tile
- * size (column-chunk size of the row panels and V^T panels: 16K .. 192K
+ * size (column-chunk size of the row panels and V^T panels: 4K .. 192K
  * values, or none) and, for p <= 8, the single launch (ahm_hmat_set_fused),
  * and keeps the fastest.  fused (may be NULL) receives the launch form (0:
  * three kernels, 1: single launch).  Products agree to the rounding of the
@@ -311,16 +311,26 @@ int ahm_hmat_matvec_t_host(ahm_hmat h, double alpha, const double* X, uint32_t l
  * nmatvec (may be NULL) the number of H-matvecs. */
 /* GMRes, solvers/GMRES.cpp:80-181 (MGS + Givens, restart length m).  The
  * Hessenberg matrix, the rotations and the convergence test run on the
- * device; each Arnoldi step is replayed from a hipGraph captured on its
- * second use (ahm_hmat_set_graphs).  The Krylov basis (n x (m+1) FP64) and the
- * graphs stay with the handle between calls. */
+ * device (ahm_hmat_set_arnoldi); the host queues the Arnoldi steps ahead and
+ * reads the convergence flags two steps behind.  The Krylov basis
+ * (n x (m+1) FP64) stays with the handle between calls. */
 int ahm_gmres(ahm_hmat h, const double* b, double* x, double* eps, uint32_t m,
               uint32_t* nsteps, double* hist, uint32_t nhist, uint32_t* nmatvec);
-/* on != 0 (the default): the GMRES entry points of this handle replay
- * captured Arnoldi steps; 0: they launch the same kernels one by one.
- * No counterpart in the reference. */
+/* on != 0: the GMRES entry points of this handle capture each Arnoldi step
+ * into a hipGraph on its second use and replay it from then on (the graphs
+ * stay with the handle); 0 (the default): they launch the same kernels one
+ * by one, which measured as fast or faster on MI355X.  Both give
+ * bit-identical iterates.  No counterpart in the reference. */
 int ahm_hmat_set_graphs(ahm_hmat h, int on);
-/* CG, solvers/CG.cpp:31-113 */
+/* Form of the GMRES orthogonalisation and rotations after each product:
+ * 1: chunked launches over all systems (dot, one launch per MGS pass,
+ * normalisation, rotations); 2: one workgroup per system does all of it in
+ * one launch, for systems of at most 16384 rows (larger ones keep the
+ * chunked launches); 0 (the default): the second form where the systems
+ * are small or many.  Both give bit-identical iterates.  No counterpart in
+ * the reference. */
+int ahm_hmat_set_arnoldi(ahm_hmat h, int mode);
+/* CG,solvers/CG.cpp:31-113 */
 int ahm_cg(ahm_hmat h, const double* b, double* x, double* eps, uint32_t* nsteps);
 /* FP64 iterative refinement: r = b - A64 x (FP64 H-matvec), inner GMRES(m)
  * on `lo` (typically the FP32 copy of the same matrix) to inner_eps relative,
