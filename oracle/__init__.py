@@ -22,6 +22,8 @@ REFLIB = os.path.join(HERE, "_ref", "libahmedref.so")
 REFSRC = "/root/reference"
 
 TREE_BBX, TREE_PCA = 0, 1
+TREE_SYM = 0x100   # or'ed into a tree kind: symmetric block tree (subdivide_sym)
+HEM = -5           # rank code of a packed Hermitian diagonal leaf (mblock::isHeM)
 KERNEL_LAPLACE, KERNEL_YUKAWA, KERNEL_EXPR = 0, 1, 2
 
 _u = C.c_uint
@@ -41,8 +43,8 @@ def _ptr(a):
 
 
 class _Tree:
-    def __init__(self, api, handle, N):
-        self.api, self.h, self.N = api, handle, N
+    def __init__(self, api, handle, N, sym=False):
+        self.api, self.h, self.N, self.sym = api, handle, N, sym
 
     def __del__(self):
         try:
@@ -67,8 +69,8 @@ class _Tree:
 class _HMat:
     """H-matrix handle: leaves in idx order, mblock data layout."""
 
-    def __init__(self, api, handle, N, tree=None):
-        self.api, self.h, self.N, self.tree = api, handle, N, tree
+    def __init__(self, api, handle, N, tree=None, sym=False):
+        self.api, self.h, self.N, self.tree, self.sym = api, handle, N, tree, sym
 
     def __del__(self):
         try:
@@ -82,21 +84,24 @@ class _HMat:
     def matvec(self, x, d=1.0, y=None):
         x = np.ascontiguousarray(x, np.float64)
         y = np.zeros(self.N) if y is None else np.ascontiguousarray(y, np.float64).copy()
-        self.api._matvec(self.h, _d(d), _ptr(x), _ptr(y))
+        f = self.api._matvec_sym if self.sym else self.api._matvec
+        f(self.h, _d(d), _ptr(x), _ptr(y))
         return y
 
     def matvec_t(self, x, d=1.0, y=None):
         """y += d A^T x (mltaGeHhVec); x has the row length, y the column length."""
         x = np.ascontiguousarray(x, np.float64)
         y = np.zeros(self.N) if y is None else np.ascontiguousarray(y, np.float64).copy()
-        self.api._matvec_t(self.h, _d(d), _ptr(x), _ptr(y))
+        f = self.api._matvec_sym if self.sym else self.api._matvec_t   # symmetric: A^T = A
+        f(self.h, _d(d), _ptr(x), _ptr(y))
         return y
 
     def matmat(self, X, d=1.0, Y=None):
         X = np.asfortranarray(X, np.float64)
         p = X.shape[1]
         Y = np.zeros((self.N, p), order="F") if Y is None else np.array(Y, np.float64, order="F", copy=True)
-        self.api._matmat(self.h, _d(d), _u(p), _ptr(X), _u(self.N), _ptr(Y), _u(self.N))
+        f = self.api._matmat_sym if self.sym else self.api._matmat
+        f(self.h, _d(d), _u(p), _ptr(X), _u(self.N), _ptr(Y), _u(self.N))
         return Y
 
     def gmres(self, b, x0=None, eps=1e-8, m=30, nsteps=10000):
@@ -104,7 +109,8 @@ class _HMat:
         x = np.zeros(self.N) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
         e = _d(eps)
         ns = _u(nsteps)
-        rc = self.api._gmres(self.h, _ptr(b), _ptr(x), C.byref(e), _u(m), C.byref(ns))
+        f = self.api._gmres_sym if self.sym else self.api._gmres
+        rc = f(self.h, _ptr(b), _ptr(x), C.byref(e), _u(m), C.byref(ns))
         return x, int(rc), e.value, ns.value
 
     def cg(self, b, x0=None, eps=1e-8, nsteps=10000):
@@ -112,7 +118,8 @@ class _HMat:
         x = np.zeros(self.N) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
         e = _d(eps)
         ns = _u(nsteps)
-        rc = self.api._cg(self.h, _ptr(b), _ptr(x), C.byref(e), C.byref(ns))
+        f = self.api._cg_sym if self.sym else self.api._cg
+        rc = f(self.h, _ptr(b), _ptr(x), C.byref(e), C.byref(ns))
         return x, int(rc), e.value, ns.value
 
     def to_dense(self):
@@ -122,12 +129,20 @@ class _HMat:
         data = L["data"]
         for l in range(len(L["b1"])):
             b1, n1, b2, n2, k, o = (int(L[f][l]) for f in ("b1", "n1", "b2", "n2", "rank", "offset"))
-            if k < 0:
+            if k == HEM:
+                B = np.zeros((n1, n1))
+                iu = np.triu_indices(n1)
+                # packed upper triangle, column by column: (i, j) at i + j (j + 1) / 2
+                B[iu] = data[o + iu[0] + iu[1] * (iu[1] + 1) // 2]
+                A[b1:b1 + n1, b2:b2 + n2] = B + np.triu(B, 1).T
+            elif k < 0:
                 A[b1:b1 + n1, b2:b2 + n2] = data[o:o + n1 * n2].reshape(n2, n1).T
             elif k > 0:
                 U = data[o:o + n1 * k].reshape(k, n1).T
                 V = data[o + n1 * k:o + (n1 + n2) * k].reshape(k, n2).T
                 A[b1:b1 + n1, b2:b2 + n2] = U @ V.T
+            if self.sym and b1 != b2 and k != 0:
+                A[b2:b2 + n2, b1:b1 + n1] = A[b1:b1 + n1, b2:b2 + n2].T
         return A
 
 
@@ -280,7 +295,7 @@ class Oracle:
     def tree(self, pts, bmin, eta2, kind=TREE_BBX):
         pts = np.ascontiguousarray(pts, np.float64)
         h = self.lib.or_tree_build(_ptr(pts), _u(len(pts)), _u(bmin), _d(eta2), kind)
-        return _Tree(self, h, len(pts))
+        return _Tree(self, h, len(pts), sym=bool(kind & TREE_SYM))
 
     def clusters(self, tree):
         nc = self.lib.or_tree_nclusters(tree.h)
@@ -451,6 +466,20 @@ class Reference:
         L.ref_trisolve.argtypes = [_p, C.c_int, _u, _p, _u]
         L.ref_gmres_hlu.restype = _u
         L.ref_gmres_hlu.argtypes = [_p, _p, _p, _p, C.POINTER(_d), _u, C.POINTER(_u)]
+        L.ref_hmat_assemble_sym.restype = _p
+        L.ref_hmat_assemble_sym.argtypes = [_p, C.c_int, _d, _d, _d, _u]
+        L.ref_hmat_matvec_sym.restype = C.c_int
+        L.ref_hmat_matvec_sym.argtypes = [_p, _d, _p, _p]
+        L.ref_cg_sym.restype = _u
+        L.ref_cg_sym.argtypes = [_p, _p, _p, C.POINTER(_d), C.POINTER(_u)]
+        L.ref_gmres_sym.restype = _u
+        L.ref_gmres_sym.argtypes = [_p, _p, _p, C.POINTER(_d), _u, C.POINTER(_u)]
+        L.ref_hmat_matmat_sym.restype = C.c_int
+        L.ref_hmat_matmat_sym.argtypes = [_p, _d, _u, _p, _u, _p, _u]
+        self._matmat_sym = L.ref_hmat_matmat_sym
+        self._matvec_sym = L.ref_hmat_matvec_sym
+        self._cg_sym = L.ref_cg_sym
+        self._gmres_sym = L.ref_gmres_sym
         self._tree_free = L.ref_tree_free
         self._tree_perm = L.ref_tree_perm
         self._tree_nleaves = L.ref_tree_nleaves
@@ -465,10 +494,15 @@ class Reference:
     def tree(self, pts, bmin, eta2, kind=TREE_BBX):
         pts = np.ascontiguousarray(pts, np.float64)
         h = self.lib.ref_tree_build(_ptr(pts), _u(len(pts)), _u(bmin), _d(eta2), kind)
-        return _Tree(self, h, len(pts))
+        return _Tree(self, h, len(pts), sym=bool(kind & TREE_SYM))
 
     def assemble(self, tree, pts=None, kernel=KERNEL_LAPLACE, kappa=0.0, eps=1e-6, rankmax=1 << 30, nthreads=0,
                  weight=1.0, rows_only=False):
+        """matgenGeH_omp's leaf rule (rows_only: ACAr); on a symmetric tree
+        (kind | TREE_SYM) the symmetric assembly matgenHeH_omp / apprx_sym."""
+        if tree.sym:
+            h = self.lib.ref_hmat_assemble_sym(tree.h, kernel, _d(kappa), _d(weight), _d(eps), _u(rankmax))
+            return _HMat(self, h, tree.N, tree, sym=True)
         f = self.lib.ref_hmat_assemble_acar if rows_only else self.lib.ref_hmat_assemble
         h = f(tree.h, kernel, _d(kappa), _d(weight), _d(eps), _u(rankmax))
         return _HMat(self, h, tree.N, tree)
@@ -508,7 +542,7 @@ class Reference:
         offset = np.ascontiguousarray(offset, np.uint64)
         data = np.ascontiguousarray(data, np.float64)
         h = self.lib.ref_hmat_from_data(tree.h, _ptr(rank), _ptr(offset), _ptr(data))
-        return _HMat(self, h, tree.N, tree)
+        return _HMat(self, h, tree.N, tree, sym=tree.sym)
 
     def save(self, H, path, eta, bmin):
         """The reference's savembls (Include/H.h:90-103) of H to path."""

@@ -28,6 +28,8 @@ extern void   dscal_(const int*, const double*, double*, const int*);
 extern void   dcopy_(const int*, const double*, const int*, double*, const int*);
 extern void   dgemv_(const char*, const int*, const int*, const double*, const double*,
                      const int*, const double*, const int*, const double*, double*, const int*);
+extern void   dspmv_(const char*, const int*, const double*, const double*, const double*,
+                     const int*, const double*, double*, const int*);
 extern void   dgemm_(const char*, const char*, const int*, const int*, const int*,
                      const double*, const double*, const int*, const double*, const int*,
                      const double*, double*, const int*);
@@ -94,6 +96,7 @@ typedef struct {
 
 struct or_tree {
   int kind;
+  int sym;          /* symmetric block tree: leaves on and above the diagonal */
   unsigned N;
   double* pts;
   unsigned *op, *po;
@@ -102,6 +105,12 @@ struct or_tree {
   unsigned nl, capl;
   unsigned *b1, *n1, *b2, *n2, *rcl;
   unsigned char* adm;
+  double eta2;
+  /* symmetric trees: the leaf ranges in the order mltaHeHVec_ visits them
+     (H/mltaGeHVec.cpp:420-446): op 0 = y += A x over leaves [l0, l1),
+     op 1 = y += A^T x over [l0, l1) */
+  unsigned nsched, capsched;
+  unsigned* sched;   /* triples (op, l0, l1) */
 };
 
 static const double* P(const or_tree* t, unsigned orig) { return t->pts + 3 * (size_t)orig; }
@@ -356,10 +365,48 @@ static void subdivide(or_tree* t, unsigned c1, unsigned c2, double eta2)
   } else add_leaf(t, c1, c2, 0);
 }
 
+/* blcluster::subdivide_sym_, cluster/blcluster.cpp:90-113: diagonal blocks
+   are never admissible; below the diagonal nothing, diagonal sons recurse
+   symmetrically, the sons above the diagonal subdivide as above, in
+   row-major son order */
+static void push_sched(or_tree* t, unsigned op, unsigned l0, unsigned l1)
+{
+  if (t->nsched == t->capsched) {
+    t->capsched = t->capsched ? 2 * t->capsched : 1024;
+    t->sched = (unsigned*)realloc(t->sched, sizeof(unsigned) * 3 * t->capsched);
+    if (!t->sched) abort();
+  }
+  unsigned* e = t->sched + 3 * (size_t)t->nsched++;
+  e[0] = op; e[1] = l0; e[2] = l1;
+}
+
+static void subdivide_sym(or_tree* t, unsigned c)
+{
+  const or_cl* a = &t->cl[c];
+  if (a->son[0] >= 0) {
+    const int s[2] = {a->son[0], a->son[1]};
+    for (int i = 0; i < 2; ++i) {
+      subdivide_sym(t, (unsigned)s[i]);
+      for (int j = i + 1; j < 2; ++j) {
+        const unsigned l0 = t->nl;
+        subdivide(t, (unsigned)s[i], (unsigned)s[j], t->eta2);
+        push_sched(t, 0, l0, t->nl);
+        push_sched(t, 1, l0, t->nl);
+      }
+    }
+  } else {
+    push_sched(t, 0, t->nl, t->nl + 1);
+    add_leaf(t, c, c, 0);
+  }
+}
+
 or_tree* or_tree_build(const double* pts, unsigned N, unsigned bmin, double eta2, int kind)
 {
   if (N == 0) return NULL;
   or_tree* t = (or_tree*)calloc(1, sizeof(or_tree));
+  t->sym = (kind & OR_TREE_SYM) != 0;
+  kind &= 0xff;
+  t->eta2 = eta2;
   t->kind = kind;
   t->N = N;
   t->pts = (double*)xmalloc(sizeof(double) * 3 * (size_t)N);
@@ -376,7 +423,8 @@ or_tree* or_tree_build(const double* pts, unsigned N, unsigned bmin, double eta2
     pca_init(t, r);
     pca_split(t, r, bmin);
   }
-  subdivide(t, r, r, eta2);
+  if (t->sym) subdivide_sym(t, r);
+  else subdivide(t, r, r, eta2);
   return t;
 }
 
@@ -385,6 +433,7 @@ void or_tree_free(or_tree* t)
   if (!t) return;
   free(t->pts); free(t->op); free(t->po); free(t->cl);
   free(t->b1); free(t->n1); free(t->b2); free(t->n2); free(t->rcl); free(t->adm);
+  free(t->sched);
   free(t);
 }
 
@@ -861,10 +910,13 @@ void or_addtrll(unsigned n1, unsigned n2, unsigned k, const double* U, const dou
 struct or_hmat {
   unsigned N, nl;
   unsigned *b1, *n1, *b2, *n2;
-  int* rank;           /* -1 dense */
+  int* rank;           /* -1 dense, OR_HEM packed Hermitian diagonal leaf */
   size_t* off;
   size_t nvals;
   double* data;
+  int sym;             /* symmetric: leaves on and above the diagonal only */
+  unsigned nsched;     /* mltaHeHVec_ visiting order (or_tree.sched), or 0 */
+  unsigned* sched;
 };
 
 static or_hmat* hmat_alloc(unsigned N, unsigned nl)
@@ -881,7 +933,10 @@ static or_hmat* hmat_alloc(unsigned N, unsigned nl)
 }
 
 static size_t leaf_nvals(unsigned n1, unsigned n2, int rank)
-{ return rank < 0 ? (size_t)n1 * n2 : (size_t)rank * (n1 + n2); }
+{
+  if (rank == OR_HEM) return (size_t)n1 * (n1 + 1) / 2;
+  return rank < 0 ? (size_t)n1 * n2 : (size_t)rank * (n1 + n2);
+}
 
 /* apprx_unsym, Include/apprx.h:136-172; method 1 uses ACAr as apprx_sym
  * does (Include/apprx.h:87-131) */
@@ -931,8 +986,25 @@ or_hmat* or_hmat_assemble_method(const or_tree* t, const double* pts, int kernel
 #endif
   for (int l = 0; l < (int)t->nl; ++l) {
     const unsigned i0 = t->cl[t->rcl[l]].icom - t->b1[l];
+    /* apprx_sym (Include/apprx.h:87-131): ACAr, and the diagonal blocks
+       that stay dense are stored packed Hermitian (setHeM + cmpblsym) */
     parts[l] = apprx_unsym(&G, t->b1[l], t->n1[l], t->b2[l], t->n2[l], t->adm[l], i0, eps,
-                           rankmax, method, &h->rank[l]);
+                           rankmax, t->sym ? 1 : method, &h->rank[l]);
+    if (t->sym && h->rank[l] < 0 && t->b1[l] == t->b2[l]) {
+      const unsigned n = t->n1[l];
+      double* Pk = (double*)xmalloc(sizeof(double) * ((size_t)n * (n + 1) / 2 + 1));
+      for (unsigned j = 0; j < n; ++j)
+        for (unsigned i = 0; i <= j; ++i) Pk[i + (size_t)j * (j + 1) / 2] = parts[l][i + (size_t)j * n];
+      free(parts[l]);
+      parts[l] = Pk;
+      h->rank[l] = OR_HEM;
+    }
+  }
+  if (t->sym) {
+    h->sym = 1;
+    h->nsched = t->nsched;
+    h->sched = (unsigned*)xmalloc(sizeof(unsigned) * 3 * (size_t)(t->nsched + 1));
+    memcpy(h->sched, t->sched, sizeof(unsigned) * 3 * (size_t)t->nsched);
   }
   size_t off = 0;
   for (unsigned l = 0; l < t->nl; ++l) {
@@ -964,6 +1036,7 @@ or_hmat* or_hmat_from_leaves(unsigned N, unsigned nl, const unsigned* b1, const 
     h->rank[l] = rank[l];
     h->off[l] = off;
     off += leaf_nvals(n1[l], n2[l], rank[l]);
+    if (rank[l] == OR_HEM) h->sym = 1;
   }
   h->off[nl] = off;
   h->nvals = off;
@@ -980,6 +1053,7 @@ void or_hmat_free(or_hmat* h)
   if (!h) return;
   free(h->b1); free(h->n1); free(h->b2); free(h->n2); free(h->rank); free(h->off);
   free(h->data);
+  free(h->sched);
   free(h);
 }
 
@@ -1016,8 +1090,40 @@ static int leaf_vec(const or_hmat* h, unsigned l, double d, const double* x, dou
     }
     return 1;
   }
+  if (h->rank[l] == OR_HEM) {
+    /* mltaHeMVec: blas::hemva = dspmv 'U' (Include/mblock.h:137-141,
+       Include/blas.h:1311-1315) */
+    int nn = (int)n1;
+    dspmv_("U", &nn, &d, data, xp, &I1, &DONE, yp, &I1);
+    return 1;
+  }
   gemva(n1, n2, d, data, xp, yp);
   return 1;
+}
+
+static void leaf_vec_t(const or_hmat* h, unsigned l, double d, const double* x, double* y);
+
+/* y += d A x for a symmetric H-matrix, mltaHeHVec_ (H/mltaGeHVec.cpp:
+   420-446): every stored leaf forward, and the leaves off the diagonal also
+   transposed.  With the tree's schedule the leaf ranges come in the
+   recursion's order (forward over a son block's leaves, then transposed over
+   the same leaves); without it (matrices built from leaf lists) all leaves
+   forward, then the off-diagonal ones transposed. */
+static void matvec_sym(const or_hmat* h, double d, const double* x, double* y)
+{
+  if (h->nsched) {
+    for (unsigned s = 0; s < h->nsched; ++s) {
+      const unsigned* e = h->sched + 3 * (size_t)s;
+      for (unsigned l = e[1]; l < e[2]; ++l) {
+        if (e[0] == 0) leaf_vec(h, l, d, x, y);
+        else leaf_vec_t(h, l, d, x, y);
+      }
+    }
+    return;
+  }
+  for (unsigned l = 0; l < h->nl; ++l) leaf_vec(h, l, d, x, y);
+  for (unsigned l = 0; l < h->nl; ++l)
+    if (h->b1[l] != h->b2[l]) leaf_vec_t(h, l, d, x, y);
 }
 
 /* mltaGeHVec_ (H/mltaGeHVec.cpp:16-35): the DFS recursion visits the leaves
@@ -1025,6 +1131,10 @@ static int leaf_vec(const or_hmat* h, unsigned l, double d, const double* x, dou
 int or_hmat_matvec(const or_hmat* h, double d, const double* x, double* y)
 {
   blas_init();
+  if (h->sym) {
+    matvec_sym(h, d, x, y);
+    return 1;
+  }
   int changed = 0;
   for (unsigned l = 0; l < h->nl; ++l) changed |= leaf_vec(h, l, d, x, y);
   return changed;
@@ -1034,26 +1144,38 @@ int or_hmat_matvec(const or_hmat* h, double d, const double* x, double* y)
  * leaves in idx order; low rank: e = d <U_l, x> then y += e V_l
  * (mltaLrMhVec_, H/mblock.cpp:159-170); dense: dgemv 'T' (mltaGeMhVec,
  * Include/mblock.h:112-115).  x is indexed by rows, y by columns. */
+static void leaf_vec_t(const or_hmat* h, unsigned l, double d, const double* x, double* y)
+{
+  const unsigned n1 = h->n1[l], n2 = h->n2[l];
+  const double* data = h->data + h->off[l];
+  const double* xp = x + h->b1[l];
+  double* yp = y + h->b2[l];
+  if (h->rank[l] >= 0) {
+    const unsigned k = (unsigned)h->rank[l];
+    for (unsigned q = 0; q < k; ++q) {
+      const double e = d * scpr(n1, data + (size_t)q * n1, xp);
+      axpy(n2, e, data + (size_t)k * n1 + (size_t)q * n2, yp);
+    }
+  } else if (h->rank[l] == OR_HEM) {
+    int nn = (int)n1;   /* symmetric block: A^T = A */
+    dspmv_("U", &nn, &d, data, xp, &I1, &DONE, yp, &I1);
+  } else {
+    int mm = (int)n1, nn = (int)n2;
+    dgemv_("T", &mm, &nn, &d, data, &mm, xp, &I1, &DONE, yp, &I1);
+  }
+}
+
 int or_hmat_matvec_t(const or_hmat* h, double d, const double* x, double* y)
 {
   blas_init();
+  if (h->sym) {   /* a symmetric matrix is its own transpose */
+    matvec_sym(h, d, x, y);
+    return 1;
+  }
   int changed = 0;
   for (unsigned l = 0; l < h->nl; ++l) {
-    const unsigned n1 = h->n1[l], n2 = h->n2[l];
-    const double* data = h->data + h->off[l];
-    const double* xp = x + h->b1[l];
-    double* yp = y + h->b2[l];
-    if (h->rank[l] >= 0) {
-      const unsigned k = (unsigned)h->rank[l];
-      if (!k) continue;
-      for (unsigned q = 0; q < k; ++q) {
-        const double e = d * scpr(n1, data + (size_t)q * n1, xp);
-        axpy(n2, e, data + (size_t)k * n1 + (size_t)q * n2, yp);
-      }
-    } else {
-      int mm = (int)n1, nn = (int)n2;
-      dgemv_("T", &mm, &nn, &d, data, &mm, xp, &I1, &DONE, yp, &I1);
-    }
+    if (h->rank[l] == 0) continue;
+    leaf_vec_t(h, l, d, x, y);
     changed = 1;
   }
   return changed;
@@ -1065,6 +1187,10 @@ int or_hmat_matmat(const or_hmat* h, double d, unsigned p, const double* X, unsi
                    double* Y, unsigned ldY)
 {
   blas_init();
+  if (h->sym) {   /* mltaHeHGeM_ (H/mltaGeHVec.cpp:448-472), column by column */
+    for (unsigned c = 0; c < p; ++c) matvec_sym(h, d, X + (size_t)c * ldX, Y + (size_t)c * ldY);
+    return 1;
+  }
   int changed = 0;
   const int pp = (int)p, lx = (int)ldX, ly = (int)ldY;
   for (unsigned l = 0; l < h->nl; ++l) {
@@ -1090,6 +1216,10 @@ int or_hmat_matmat(const or_hmat* h, double d, unsigned p, const double* X, unsi
 int or_hmat_matvec_omp(const or_hmat* h, double d, const double* x, double* y, int nthreads)
 {
   blas_init();
+  if (h->sym) {
+    matvec_sym(h, d, x, y);
+    return 1;
+  }
 #ifdef _OPENMP
   if (nthreads <= 0) nthreads = omp_get_max_threads();
 #else

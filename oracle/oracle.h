@@ -28,6 +28,8 @@ extern "C" {
 /* cluster tree kinds */
 #define OR_TREE_BBX 0   /* cluster3d_bbx + centre-of-mass start row (golden) */
 #define OR_TREE_PCA 1   /* bemcluster (cluster3d_pca + icom quirk)   (golden) */
+#define OR_TREE_SYM 0x100 /* or-ed into the kind: symmetric block tree (subdivide_sym) */
+#define OR_HEM (-5)     /* rank code of a packed Hermitian diagonal leaf (mblock::isHeM) */
 
 /* kernel generators (MATGEN_T plugins; the reference ships none) */
 #define OR_KERNEL_LAPLACE 0  /* 1/(4 pi r), diagonal 1.0                    */

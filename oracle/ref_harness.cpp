@@ -66,6 +66,12 @@ struct KernelGen {
   double scale(unsigned b1, unsigned n1, unsigned b2, unsigned n2) const {
     return std::fabs(val(b1 + n1 / 2, b2 + n2 / 2));
   }
+  // upper triangle of a diagonal block, packed column by column (the cmpblsym
+  // contract, Include/matgen_omp.h:30-32)
+  void cmpblsym(unsigned b1, unsigned n1, double* d) const {
+    for (unsigned j = 0; j < n1; ++j)
+      for (unsigned i = 0; i <= j; ++i) d[i + (size_t)j * (j + 1) / 2] = val(b1 + i, b1 + j);
+  }
 };
 
 struct DenseGen {
@@ -81,6 +87,7 @@ struct DenseGen {
 
 struct Tree {
   int kind;
+  bool sym;  // symmetric block tree (blcluster::subdivide_sym)
   unsigned N;
   std::vector<Point> pts;
   std::vector<unsigned> op, po;
@@ -174,6 +181,35 @@ void apprx_bbx(KernelGen& G, mblock<double>*& mbl, blcluster* bl, unsigned i0, d
   }
 }
 
+// Include/apprx.h:87-131 (apprx_sym) restated for a plain blcluster with an
+// explicit start row: ACAr on admissible blocks, diagonal blocks packed
+// Hermitian (mblock::setHeM + cmpblsym), other blocks dense.
+void apprx_sym_bbx(KernelGen& G, mblock<double>*& mbl, blcluster* bl, unsigned i0, double eps,
+                   unsigned rankmax) {
+  unsigned n1 = bl->getn1(), n2 = bl->getn2(), b1 = bl->getb1(), b2 = bl->getb2();
+  unsigned maxk = n1 * n2 / (n1 + n2) + 4;
+  if (maxk > rankmax) maxk = rankmax;
+  mbl = new mblock<double>(n1, n2);
+  bool succ = false;
+  if (bl->isadm() && maxk) {
+    unsigned k;
+    double *U, *V;
+    succ = ::ACAr(G, b1, n1, b2, n2, eps, maxk, i0, k, U, V);
+    if (succ) mbl->cpyLrM_cmpr(k, U, n1, V, n2, eps, k);
+    delete[] V;
+    delete[] U;
+  }
+  if (!succ) {
+    if (bl->isdbl()) {
+      mbl->setHeM();
+      G.cmpblsym(b1, n1, mbl->getdata());
+    } else {
+      mbl->setGeM();
+      G.cmpbl(b1, n1, b2, n2, mbl->getdata());
+    }
+  }
+}
+
 // Leaf-parallel operator on all host cores for the CPU baseline: the leaves
 // in idx order are cut into nthreads contiguous intervals of equal stored-
 // value count; each thread applies the reference's own leaf product
@@ -238,9 +274,13 @@ struct Silence {
 
 extern "C" {
 
+// kind: 0 bbx, 1 bemcluster (PCA); | 0x100: symmetric block tree
+// (blcluster::subdivide_sym, cluster/blcluster.cpp:90-113)
 void* ref_tree_build(const double* pts, unsigned N, unsigned bmin, double eta2, int kind) {
   init();
   Tree* t = new Tree;
+  t->sym = (kind & 0x100) != 0;
+  kind &= 0xff;
   t->kind = kind;
   t->N = N;
   t->pts.resize(N);
@@ -260,7 +300,10 @@ void* ref_tree_build(const double* pts, unsigned N, unsigned bmin, double eta2, 
     t->root = r;
     t->bl = new bemblcluster<Point, Point>(r, r);
   }
-  t->bl->subdivide(t->root, t->root, eta2, t->nblcks);
+  if (t->sym)
+    t->bl->subdivide_sym(t->root, eta2, t->nblcks);
+  else
+    t->bl->subdivide(t->root, t->root, eta2, t->nblcks);
   blcluster** list;
   gen_BlSequence(t->bl, list);
   t->leaves.assign(t->nblcks, nullptr);
@@ -383,6 +426,26 @@ void* ref_hmat_assemble_acar(void* tp, int kernel, double kappa, double weight, 
   return h;
 }
 
+// the symmetric assembly: matgenHeH_omp (Include/matgen_omp.h:77-100,
+// apprx_sym per leaf) on bemcluster trees; on bbx trees the same leaf rule
+// with the bbx start row (apprx_sym_bbx)
+void* ref_hmat_assemble_sym(void* tp, int kernel, double kappa, double weight, double eps,
+                            unsigned rankmax) {
+  init();
+  Tree* t = (Tree*)tp;
+  HMat* h = new HMat;
+  h->t = t;
+  h->nl = t->nblcks;
+  allocmbls(t->bl, h->A);
+  KernelGen G = {kernel, kappa, weight, t->pts.data(), t->op.data()};
+  Silence quiet;
+  if (t->kind == 1)
+    matgenHeH_omp(G, t->nblcks, (bemblcluster<Point, Point>*)t->bl, eps, rankmax, h->A);
+  else
+    for (unsigned i = 0; i < t->nblcks; ++i) apprx_sym_bbx(G, h->A[i], t->leaves[i], t->i0[i], eps, rankmax);
+  return h;
+}
+
 // an mblock array holding given leaf values (mblock layout, leaves in idx
 // order of this tree): lets the reference's own mltaGeHVec run on a matrix
 // assembled elsewhere
@@ -397,7 +460,10 @@ void* ref_hmat_from_data(void* tp, const int* rank, const size_t* offset, const 
     blcluster* b = t->leaves[i];
     const unsigned n1 = b->getn1(), n2 = b->getn2();
     mblock<double>* m = new mblock<double>(n1, n2);
-    if (rank[i] < 0) {
+    if (rank[i] == -5) {
+      m->setHeM();
+      std::memcpy(m->getdata(), data + offset[i], sizeof(double) * (size_t)n1 * (n1 + 1) / 2);
+    } else if (rank[i] < 0) {
       m->setGeM();
       std::memcpy(m->getdata(), data + offset[i], sizeof(double) * (size_t)n1 * n2);
     } else {
@@ -416,13 +482,14 @@ void ref_hmat_free(void* p) {
   delete h;
 }
 
-// rank[i] = -1 for dense leaves; returns total stored values
+// rank[i] = -1 for dense leaves, -5 for packed Hermitian (mblock::isHeM)
+// diagonal leaves; returns total stored values
 size_t ref_hmat_layout(void* p, int* rank, size_t* offset) {
   HMat* h = (HMat*)p;
   size_t off = 0;
   for (unsigned i = 0; i < h->t->nblcks; ++i) {
     mblock<double>* m = h->A[i];
-    rank[i] = m->isLrM() ? (int)m->rank() : -1;
+    rank[i] = m->isLrM() ? (int)m->rank() : m->isHeM() ? -5 : -1;
     offset[i] = off;
     off += m->nvals();
   }
@@ -449,6 +516,49 @@ int ref_hmat_matvec(void* p, double d, double* x, double* y) {
 int ref_hmat_matvec_t(void* p, double d, double* x, double* y) {
   HMat* h = (HMat*)p;
   return mltaGeHhVec(d, h->t->bl, h->A, x, y) ? 1 : 0;
+}
+
+// y += d A x for a symmetric H-matrix (mltaHeHVec, H/mltaGeHVec.cpp:420-446)
+int ref_hmat_matvec_sym(void* p, double d, double* x, double* y) {
+  HMat* h = (HMat*)p;
+  mltaHeHVec(d, h->t->bl, h->A, x, y);
+  return 1;
+}
+
+// Y += d A X for a symmetric H-matrix (mltaHeHGeM, H/mltaGeHVec.cpp:448-472)
+int ref_hmat_matmat_sym(void* p, double d, unsigned pp, double* X, unsigned ldX, double* Y,
+                        unsigned ldY) {
+  HMat* h = (HMat*)p;
+  mltaHeHGeM(d, h->t->bl, h->A, pp, X, ldX, Y, ldY);
+  return 1;
+}
+
+// the reference's CG (solvers/CG.cpp:31-113) and GMRes on HeHMatrix
+// (Include/matrix.h:307-323), the operator of its own BEM example
+unsigned ref_cg_sym(void* p, double* b, double* x, double* eps, unsigned* nsteps) {
+  HMat* h = (HMat*)p;
+  HeHMatrix<double> M(h->t->N, h->t->bl);
+  M.blcks = h->A;
+  double e = *eps;
+  unsigned ns = *nsteps;
+  unsigned r = CG(M, b, x, e, ns);
+  M.blcks = NULL;
+  *eps = e;
+  *nsteps = ns;
+  return r;
+}
+
+unsigned ref_gmres_sym(void* p, double* b, double* x, double* eps, unsigned m, unsigned* nsteps) {
+  HMat* h = (HMat*)p;
+  HeHMatrix<double> M(h->t->N, h->t->bl);
+  M.blcks = h->A;
+  double e = *eps;
+  unsigned ns = *nsteps;
+  unsigned r = GMRes(M, b, x, e, m, ns);
+  M.blcks = NULL;
+  *eps = e;
+  *nsteps = ns;
+  return r;
 }
 
 int ref_hmat_matmat(void* p, double d, unsigned pp, double* X, unsigned ldX, double* Y,

@@ -1,6 +1,6 @@
 """Write golden vectors from the COMPILED REFERENCE (run in the build container).
 
-Usage:  make -C oracle ref && python tests/golden/make_golden.py [mbls | transposed | hlu | acar]
+Usage:  make -C oracle ref && python tests/golden/make_golden.py [mbls | transposed | hlu | acar | heh]
 
 The reference's own sources (read in place under /root/reference, compiled by
 oracle/Makefile into oracle/_ref/libahmedref.so) produce every expected value
@@ -8,7 +8,8 @@ here: cluster permutation, block-tree leaves, ACA ranks, a SHA-256 of all
 stored leaf values plus each leaf's Frobenius norm, H*x, H*X, GMRes and CG results, plus ACA / addtrll on
 explicit blocks, plus SHA-256 digests of the reference's savembls files, plus the
 reference's HLU factors of two small matrices with its triangular solves and
-HLU-preconditioned GMRes on them.  Only data is written (golden_*.npz); nothing from the
+HLU-preconditioned GMRes on them, plus the symmetric path of its BEM example
+(subdivide_sym, apprx_sym, mltaHeHVec, CG and GMRes on HeHMatrix).  Only data is written (golden_*.npz); nothing from the
 reference travels in any other form.
 """
 from __future__ import annotations
@@ -198,8 +199,65 @@ def write_acar(R):
     np.savez_compressed(os.path.join(OUT, "golden_acar.npz"), **rec)
 
 
+# name, geometry, N, seed, bmin, eta, tree kind, kernel, kappa, weight, eps, cg (tol, nsteps), gmres (m, tol, nsteps)
+HEH_CASES = [
+    ("sphere4096", "sphere", 4096, 5, 32, 0.8, O.TREE_BBX, O.KERNEL_LAPLACE, 0.0, "area", 1e-6,
+     (1e-10, 400), (30, 1e-10, 400)),
+    ("pca_yukawa1500", "sphere", 1500, 7, 16, 0.8, O.TREE_PCA, O.KERNEL_YUKAWA, 2.0, "area", 1e-8,
+     (1e-10, 400), (50, 1e-10, 400)),
+    ("cube_expr2048", "cube", 2048, 3, 16, 0.8, O.TREE_BBX, O.KERNEL_EXPR, 2.0, "area", 1e-5,
+     (1e-8, 400), None),
+]
+
+
+def write_heh(R):
+    """The symmetric path of the reference's BEM example (Examples/PrecondBEM.cpp:
+    360-375, 450): subdivide_sym (cluster/blcluster.cpp:90-113), the symmetric
+    assembly apprx_sym (Include/apprx.h:87-131: ACAr, packed Hermitian diagonal
+    leaves; matgenHeH_omp for bemcluster trees), mltaHeHVec / mltaHeHGeM
+    (H/mltaGeHVec.cpp:420-472) and CG / GMRes on HeHMatrix
+    (Include/matrix.h:307-323).  golden_heh.npz holds <case>_<field>; rank
+    code -5 marks the packed Hermitian diagonal leaves (n(n+1)/2 values)."""
+    rec = {"blas": O.Oracle().blas_version()}
+    for (name, geo, N, seed, bmin, eta, kind, kern, kappa, w, eps, cgp, gm) in HEH_CASES:
+        pts = points(geo, N, seed)
+        weight = weight_of(w, geo, N)
+        T = R.tree(pts, bmin, eta * eta, kind | O.TREE_SYM)
+        op, po = T.perm()
+        lv = T.leaves()
+        H = R.assemble(T, kernel=kern, kappa=kappa, eps=eps, weight=weight)
+        L = H.layout()
+        rng = np.random.default_rng(1100 + seed)
+        x = rng.uniform(-1, 1, N)
+        y0 = rng.standard_normal(N)
+        X = rng.standard_normal((N, 3))
+        c = dict(N=N, seed=seed, bmin=bmin, eta2=eta * eta, kind=kind, kernel=kern, kappa=kappa, weight=weight,
+                 eps=eps, geometry=geo, op_perm=op, po_perm=po,
+                 b1=lv["b1"], n1=lv["n1"], b2=lv["b2"], n2=lv["n2"], adm=lv["adm"], i0=lv["i0"],
+                 rank=L["rank"], data_sha256=data_hash(L), nvals=len(L["data"]),
+                 x=x, y0=y0, d=-0.6, y=H.matvec(x, d=-0.6, y=y0), X=X, Y=H.matmat(X, d=0.5))
+        b = H.matvec(np.ones(N))
+        c["b"] = b
+        tol, ns = cgp
+        xc, rc, e, n = H.cg(b, eps=tol, nsteps=ns)
+        c.update(cg_tol=tol, cg_nsteps_in=ns, cg_x=xc, cg_rc=rc, cg_eps=e, cg_nsteps=n)
+        if gm is not None:
+            m, tol, ns = gm
+            xg, rc2, e2, n2 = H.gmres(b, eps=tol, m=m, nsteps=ns)
+            c.update(gmres_m=m, gmres_tol=tol, gmres_nsteps_in=ns, gmres_x=xg, gmres_rc=rc2, gmres_eps=e2,
+                     gmres_nsteps=n2)
+        for k, v in c.items():
+            rec[f"{name}_{k}"] = v
+        print("heh", name, "leaves", len(lv["b1"]), "diag", int((L["rank"] == O.HEM).sum()),
+              "lowrank", int((L["rank"] >= 0).sum()), "nvals", len(L["data"]), "cg", rc, e, n)
+        del H, T
+    np.savez_compressed(os.path.join(OUT, "golden_heh.npz"), **rec)
+
+
 def main():
     R = O.Reference()
+    if sys.argv[1:] == ["heh"]:
+        return write_heh(R)
     if sys.argv[1:] == ["acar"]:
         return write_acar(R)
     if sys.argv[1:] == ["hlu"]:
@@ -273,6 +331,7 @@ def main():
     write_transposed(R)
     write_hlu(R)
     write_acar(R)
+    write_heh(R)
 
 
 if __name__ == "__main__":
