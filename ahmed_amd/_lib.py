@@ -36,7 +36,7 @@ class HMatStats(C.Structure):
                 ("nvals", C.c_uint64), ("nvals_dense", C.c_uint64), ("nvals_u", C.c_uint64),
                 ("nvals_v", C.c_uint64), ("t_len", C.c_uint64), ("bytes_per_value", C.c_uint64),
                 ("n_tasks_a", C.c_uint64), ("n_tasks_b", C.c_uint64), ("max_rank", C.c_uint32),
-                ("n_rows", C.c_uint32), ("n_cols", C.c_uint32), ("pad", C.c_uint32)]
+                ("n_rows", C.c_uint32), ("n_cols", C.c_uint32), ("symmetric", C.c_uint32)]
 
 
 class Kernel(C.Structure):
@@ -52,6 +52,7 @@ class AssemblyStats(C.Structure):
 
 KERNEL_LAPLACE, KERNEL_YUKAWA, KERNEL_EXPR = 0, 1, 2
 ASSEMBLE_ACAR = 1   # Kernel.flags: ACAr instead of ACA on the admissible leaves
+ASSEMBLE_SYM = 2    # Kernel.flags: symmetric storage (set by assemble on AHM_TREE_SYM trees)
 
 
 class AhmError(RuntimeError):
@@ -118,7 +119,8 @@ SIGNATURES = {
     "ahm_cg_prec": (_i, [_p, _p, _p, _p, _p, C.POINTER(_d), C.POINTER(_u32)]),
 }
 
-LEAF_LTM, LEAF_UTM = 1, 2
+LEAF_LTM, LEAF_UTM, LEAF_HEM = 1, 2, 4
+HEM = -5   # rank code of packed Hermitian diagonal leaves in layouts (oracle convention)
 
 
 def factor_leaves(layout) -> tuple:
@@ -217,6 +219,7 @@ class Context:
 
 
 TREE_BBX, TREE_PCA = 0, 1
+TREE_SYM = 0x100   # or'ed into a kind: symmetric block tree (subdivide_sym)
 
 
 def mbls_save(path, eta: float, bmin: int, leaves: np.ndarray, data: np.ndarray) -> None:
@@ -248,6 +251,7 @@ class Tree:
     def __init__(self, pts, bmin: int, eta2: float, kind: int = TREE_BBX):
         pts = np.ascontiguousarray(pts, np.float64)
         self.N = len(pts)
+        self.sym = bool(kind & TREE_SYM)
         h = _p()
         check(load().ahm_tree_build(pts.ctypes.data_as(_p), self.N, bmin, eta2, kind, C.byref(h)),
               "ahm_tree_build")
@@ -283,9 +287,13 @@ class Tree:
 
 
 def leaf_array(b1, n1, b2, n2, rank, offset) -> np.ndarray:
+    """Leaf descriptors from a layout; rank code HEM (-5) marks packed
+    Hermitian diagonal leaves (rank -1 with flag LEAF_HEM)."""
     a = np.zeros(len(rank), LEAF_DTYPE)
+    rank = np.asarray(rank)
     a["b1"], a["n1"], a["b2"], a["n2"] = b1, n1, b2, n2
-    a["rank"], a["offset"] = rank, offset
+    a["rank"], a["offset"] = np.where(rank == HEM, -1, rank), offset
+    a["flags"] = np.where(rank == HEM, LEAF_HEM, 0)
     return a
 
 
@@ -436,7 +444,7 @@ class HMatrix:
     def stats(self) -> dict:
         st = HMatStats()
         check(load().ahm_hmat_stats_get(self.h, C.byref(st)), "ahm_hmat_stats_get")
-        return {f: getattr(st, f) for f, _ in HMatStats._fields_ if f != "pad"}
+        return {f: getattr(st, f) for f, _ in HMatStats._fields_}
 
     def save(self, path, eta: float, bmin: int) -> None:
         """savembls of this matrix (needs keep_leaves): the reference's format."""
@@ -469,7 +477,9 @@ class HMatrix:
         leaves = self.leaves()
         n1, n2 = leaves["n1"].astype(np.int64), leaves["n2"].astype(np.int64)
         k = leaves["rank"].astype(np.int64)
-        nv = np.where(k < 0, n1 * n2, k * (n1 + n2)) + leaves["offset"].astype(np.int64)
+        hem = (leaves["flags"] & LEAF_HEM) != 0
+        nv = np.where(k >= 0, k * (n1 + n2), np.where(hem, n1 * (n1 + 1) // 2, n1 * n2)) + \
+            leaves["offset"].astype(np.int64)
         # kept values are in the element type they were uploaded in (FP64 for
         # matrices assembled on the GPU)
         dt = np.float32 if getattr(self, "leaf_dtype", F64) == F32 else np.float64

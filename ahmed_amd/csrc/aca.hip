@@ -1598,12 +1598,21 @@ __global__ __launch_bounds__(kAcaThreads, 4) void expand_kernel(const AcaBlock* 
   factor(V, B.n2, nmin, tau2, Zs, Vo);
 }
 
-// dense leaves (and failed ACA): cmpbl of the whole block, column-major
+// dense leaves (and failed ACA): cmpbl of the whole block, column-major;
+// packed diagonal blocks of symmetric matrices: cmpblsym, the upper triangle
+// column by column, (i, j) at i + j(j+1)/2 (Include/matgen_omp.h:30-32)
 __global__ void dense_fill_kernel(const DenseJob* __restrict__ jobs, const double* __restrict__ P,
                                   const KernelSpec* __restrict__ Ks, double* __restrict__ out) {
   const DenseJob J = jobs[blockIdx.x];
   const KernelSpec K = Ks[J.kidx];
   double* D = out + J.off;
+  if (J.packed) {
+    for (uint32_t j = 0; j < J.n2; ++j) {
+      double* c = D + (uint64_t)j * (j + 1) / 2;
+      for (uint32_t i = threadIdx.x; i <= j; i += blockDim.x) c[i] = kentry(K, P, J.b1 + i, J.b2 + j);
+    }
+    return;
+  }
   // column by column (no per-element division); a workgroup covers
   // blockDim.x / n1 columns at a time when a column is shorter than it
   const uint32_t per = J.n1 >= blockDim.x ? 1u : blockDim.x / J.n1;
@@ -1661,7 +1670,10 @@ struct Grow {
 KernelSpec make_kernel(const ahm_kernel& k) {
   KernelSpec K;
   K.kind = k.kind;
-  K.rows_only = (k.flags & AHM_ASSEMBLE_ACAR) ? 1 : 0;
+  // apprx_sym (Include/apprx.h:87-131) approximates with ACAr
+  K.sym = (k.flags & AHM_ASSEMBLE_SYM) ? 1 : 0;
+  K.rows_only = (k.flags & (AHM_ASSEMBLE_ACAR | AHM_ASSEMBLE_SYM)) ? 1 : 0;
+  K.pad = 0;
   K.kappa = k.kappa;
   K.c = k.kind == AHM_KERNEL_EXPR ? k.weight : k.weight * 0.079577471545947667884;
   return K;
@@ -1984,11 +1996,18 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     d.n2 = n2[l];
     d.rank = rank[l];
     d.flags = 0;
+    const uint32_t kx = leaf_kernel ? leaf_kernel[l] : 0u;
     if (rank[l] > 0) d.offset = part_base[part_of[l]] + off_in_part[l];
     else if (rank[l] == 0) d.offset = 0;
-    else {
+    else if (Ks[kx].sym && b1[l] == b2[l] && n1[l] == n2[l]) {
+      // diagonal leaf of a symmetric matrix: mblock::setHeM + cmpblsym
       d.offset = total;
-      dj.push_back(DenseJob{b1[l], n1[l], b2[l], n2[l], total, leaf_kernel ? leaf_kernel[l] : 0u, 0u});
+      d.flags = AHM_LEAF_HEM;
+      dj.push_back(DenseJob{b1[l], n1[l], b2[l], n2[l], total, kx, 1u});
+      total += (uint64_t)n1[l] * (n1[l] + 1) / 2;
+    } else {
+      d.offset = total;
+      dj.push_back(DenseJob{b1[l], n1[l], b2[l], n2[l], total, kx, 0u});
       total += (uint64_t)n1[l] * n2[l];
     }
   }

@@ -17,6 +17,8 @@ struct KernelSpec {
   int rows_only; // ACAr instead of ACA on the admissible leaves
   double kappa;
   double c;      // weight/(4 pi) (Laplace, Yukawa) or weight (EXPR)
+  int sym;       // symmetric storage: diagonal dense leaves packed (AHM_LEAF_HEM)
+  int pad;
 };
 
 struct AcaBlock {
@@ -39,7 +41,8 @@ struct AcaResult {
 struct DenseJob {
   uint32_t b1, n1, b2, n2;
   uint64_t off;
-  uint32_t kidx, pad;
+  uint32_t kidx;
+  uint32_t packed;   // 1: diagonal block, upper triangle packed column by column
 };
 
 KernelSpec make_kernel(const ahm_kernel& k);

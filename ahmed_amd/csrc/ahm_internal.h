@@ -107,9 +107,20 @@ __device__ __forceinline__ bool gated(const Gate g) {
   return g.frozen && g.frozen[g.seg ? g.seg[blockIdx.x] : 0];
 }
 
+// stored values of a leaf in mblock layout: n1*n2 dense, n(n+1)/2 packed
+// Hermitian (AHM_LEAF_HEM), k*(n1+n2) low rank
+inline uint64_t leaf_values(const ahm_leaf_desc& d) {
+  if (d.rank >= 0) return (uint64_t)d.rank * ((uint64_t)d.n1 + d.n2);
+  if (d.flags & AHM_LEAF_HEM) return (uint64_t)d.n1 * (d.n1 + 1) / 2;
+  return (uint64_t)d.n1 * d.n2;
+}
+
 // Copy of a rows x cols block from the mblock-layout buffer into the arena.
 // transpose = 0: dst[r + c*dst_ld] = src[r + c*src_ld]
 // transpose = 1: dst[c + r*dst_ld] = src[r + c*src_ld]   (V -> V^T tile)
+// transpose = 2: rows r0 = pad .. pad+rows-1 of a symmetric n x n block
+//                packed upper (n = src_ld): dst[r + c*dst_ld] = A(pad+r, c),
+//                A(i, j) = src[min + max(max+1)/2] with min/max of (i, j)
 struct CopyJob {
   uint64_t src, dst;
   uint32_t rows, cols, src_ld, dst_ld;
@@ -164,6 +175,15 @@ int matvec_prepare(ahm_hmat h, ahm_dtype vec);
 int matvec_gated(ahm_hmat h, double alpha, const void* x, void* y, ahm_dtype vec, const uint32_t* frozen,
                  const uint32_t* const seg[4]);
 void matvec_signature(ahm_hmat h, std::vector<uint64_t>& sig);
+
+// The transposed part of symmetric products (hmat_t.hip): sym_prepare builds
+// its column index and sizes its buffers for vectors of type vec (allocates;
+// call before a graph capture); sym_transposed adds alpha S^T X for the
+// leaves off the diagonal, column by column, and allocates nothing.  Both do
+// nothing for unsymmetric matrices.
+int sym_prepare(ahm_hmat h, ahm_dtype vec);
+int sym_transposed(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y, uint32_t ldy, uint32_t p,
+                   ahm_dtype vec, Gate g);
 
 // solver state kept with a handle between calls (gmres.hip)
 struct GmresCache;
@@ -259,6 +279,7 @@ struct ahm_hmat_s {
   int fused = 0, fz_built = 0;
   ahm::DeviceBuf fz_panels, fz_apanels, fz_groups, fz_goff;
   uint32_t fz_n = 0;
+  int sym = 0;                // symmetric storage (AHM_LEAF_HEM diagonal leaves)
   ahm::DeviceBuf leafdata;    // optional mblock-layout copy (keep_leaves)
   std::vector<ahm_leaf_desc> leaves;
   // host copies of the plan (used by ahm_hmat_convert)

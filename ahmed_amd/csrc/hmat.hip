@@ -43,6 +43,20 @@ __global__ __launch_bounds__(256) void relayout_kernel(const CopyJob* __restrict
                                                        TD* __restrict__ dst) {
   const CopyJob j = jobs[blockIdx.x];
   if (j.rows == 0) return;
+  if (j.transpose == 2) {
+    // rows pad .. pad+rows-1 of a packed symmetric block: A(i, k) with
+    // i <= k at i + k(k+1)/2, else at k + i(i+1)/2
+    const TS* s = src + j.src;
+    for (uint32_t c = 0; c < j.cols; ++c) {
+      const uint64_t cc = (uint64_t)c * (c + 1) / 2;
+      TD* d = dst + j.dst + (uint64_t)c * j.dst_ld;
+      for (uint32_t r = threadIdx.x; r < j.rows; r += blockDim.x) {
+        const uint64_t i = (uint64_t)j.pad + r;
+        d[r] = (TD)s[i <= c ? i + cc : c + i * (i + 1) / 2];
+      }
+    }
+    return;
+  }
   if (j.transpose) {
     // V (rows x cols, column-major) -> V^T rows of the panel: a tile of rb
     // rows is read column by column and written row by row, so both sides
@@ -1004,6 +1018,10 @@ static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_l
                   d.b2, d.n2, n_rows, n_cols);
     if (d.rank >= 0 && (uint64_t)d.rank > std::min(d.n1, d.n2) + 64u)
       return fail(AHM_ERR_INVALID, "leaf %u rank %d exceeds its dimensions", l, d.rank);
+    if ((d.flags & AHM_LEAF_HEM) && (d.rank >= 0 || d.b1 != d.b2 || d.n1 != d.n2))
+      return fail(AHM_ERR_INVALID, "leaf %u: a packed Hermitian leaf must be a dense diagonal block", l);
+    if (d.flags & ~AHM_LEAF_HEM)
+      return fail(AHM_ERR_INVALID, "leaf %u: flags %u (triangular leaves belong to ahm_trih_upload)", l, d.flags);
   }
   // low-rank leaves ordered by (b2, n2), leaf order among equals: a counting
   // sort on b2 (stable), then an insertion sort on n2 inside each b2 bucket
@@ -1188,6 +1206,11 @@ static int build_plan(uint32_t n_rows, uint32_t n_cols, uint32_t nl, const ahm_l
           cj.dst = ar + (uint64_t)W * m;
           cj.dst_ld = m;
           cj.transpose = 0;
+          if (d.flags & AHM_LEAF_HEM) {   // expanded from the packed upper triangle
+            cj.src = d.offset;
+            cj.transpose = 2;
+            cj.pad = r0 - d.b1;
+          }
           J[jo++] = cj;
           if (d.rank < 0)
             for (uint32_t k = 0; k < cols; ++k) Z[zo + k] = d.b2 + k;
@@ -1408,6 +1431,7 @@ int matvec_on_workspace(ahm_hmat h, void* base, size_t bytes, double alpha, cons
   // the static workspace carries no BF16 copy of X: F64 and F32 storage only
   if (!((h->storage == AHM_F64 && vec == AHM_F64) || (h->storage == AHM_F32 && vec == AHM_F32)))
     return fail(AHM_ERR_UNSUPPORTED, "%s: storage %d with vectors %d", who, h->storage, vec);
+  if (h->sym) return fail(AHM_ERR_UNSUPPORTED, "%s: symmetric matrices need the transposed part's buffers", who);
   const size_t es = dtype_size(vec);
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t tb = up(ws_t_bytes(h, p, es)), pb = up(ws_part_bytes(h, p, es)), xb = up(ws_x_bytes(h, p, es));
@@ -1428,7 +1452,7 @@ int matvec_prepare(ahm_hmat h, ahm_dtype vec) {
   AHM_HIP(hipSetDevice(h->ctx->device));
   AHM_TRY(ensure_tbuf(h, 1, dtype_size(vec)));
   AHM_TRY(ensure_pbuf(h, 1, dtype_size(vec)));
-  return AHM_OK;
+  return sym_prepare(h, vec);
 }
 
 int matvec_gated(ahm_hmat h, double alpha, const void* x, void* y, ahm_dtype vec, const uint32_t* frozen,
@@ -1438,12 +1462,16 @@ int matvec_gated(ahm_hmat h, double alpha, const void* x, void* y, ahm_dtype vec
     return fail(AHM_ERR_INVALID, "gated product: workspace not prepared");
   Workspace ws = own_workspace(h);
   for (int q = 0; q < 4; ++q) ws.gate[q] = Gate{frozen, seg ? seg[q] : nullptr};
+  if (h->sym && seg) return fail(AHM_ERR_UNSUPPORTED, "gated product: symmetric matrices are single systems");
   if (h->storage == AHM_F64 && vec == AHM_F64)
-    return launch_matvec<double, double>(h, ws, alpha, (const double*)x, h->n_cols, (double*)y, h->n_rows, 1);
-  if (h->storage != AHM_F64 && vec == AHM_F32)
-    return launch_matvec<float, float>(h, ws, (float)alpha, (const float*)x, h->n_cols, (float*)y, h->n_rows,
-                                       1);
-  return fail(AHM_ERR_UNSUPPORTED, "gated product: storage %d with vectors %d", h->storage, vec);
+    AHM_TRY((launch_matvec<double, double>(h, ws, alpha, (const double*)x, h->n_cols, (double*)y, h->n_rows,
+                                           1)));
+  else if (h->storage != AHM_F64 && vec == AHM_F32)
+    AHM_TRY((launch_matvec<float, float>(h, ws, (float)alpha, (const float*)x, h->n_cols, (float*)y,
+                                         h->n_rows, 1)));
+  else
+    return fail(AHM_ERR_UNSUPPORTED, "gated product: storage %d with vectors %d", h->storage, vec);
+  return sym_transposed(h, alpha, x, h->n_rows, y, h->n_cols, 1, vec, Gate{frozen, nullptr});
 }
 
 void matvec_signature(ahm_hmat h, std::vector<uint64_t>& sig) {
@@ -1452,7 +1480,10 @@ void matvec_signature(ahm_hmat h, std::vector<uint64_t>& sig) {
          (uint64_t)(uintptr_t)h->zsrc.p,    (uint64_t)(uintptr_t)h->arena.p,
          (uint64_t)(uintptr_t)h->tbuf.p,    (uint64_t)(uintptr_t)h->pbuf.p,
          h->n_a, h->n_b, h->n_ra, h->n_r, h->t_len, h->part_len, h->n_rows, h->n_cols,
-         (uint64_t)h->storage, h->chunk_values};
+         (uint64_t)h->storage, h->chunk_values,
+         // the transposed part of symmetric products
+         (uint64_t)h->sym, (uint64_t)(uintptr_t)h->tr.contrib.p, (uint64_t)(uintptr_t)h->tr.tprime.p,
+         (uint64_t)(uintptr_t)h->tr.tasks_a.p, (uint64_t)(uintptr_t)h->tr.tasks_b.p};
 }
 
 }  // namespace ahm
@@ -1604,7 +1635,7 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
   AHM_TRY(build_plan(n_rows, n_cols, (uint32_t)leaves.size(), leaves.data(), P));
   timer.mark("plan (panels, chunks, copy jobs)");
   for (const ahm_leaf_desc& d : leaves) {
-    const uint64_t nv = d.rank < 0 ? (uint64_t)d.n1 * d.n2 : (uint64_t)d.rank * (d.n1 + d.n2);
+    const uint64_t nv = leaf_values(d);
     if ((d.offset + nv) * dtype_size(data_dtype) > raw.bytes && nv)
       return fail(AHM_ERR_INVALID, "leaf values beyond the data buffer");
   }
@@ -1622,6 +1653,8 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
   h->nv_dense = P.nv_dense;
   h->nv_u = P.nv_u;
   h->nv_v = P.nv_v;
+  for (const ahm_leaf_desc& d : leaves)
+    if (d.flags & AHM_LEAF_HEM) h->sym = 1;
   h->leaves = std::move(leaves);
   hipStream_t s = ctx->stream;
   const size_t ss = dtype_size(storage);
@@ -1715,7 +1748,7 @@ int ahm_hmat_upload(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, uint32_t n_le
   uint64_t total = 0;
   for (uint32_t l = 0; l < n_leaves; ++l) {
     const ahm_leaf_desc& d = leaves[l];
-    const uint64_t nv = d.rank < 0 ? (uint64_t)d.n1 * d.n2 : (uint64_t)d.rank * (d.n1 + d.n2);
+    const uint64_t nv = leaf_values(d);
     total = std::max(total, d.offset + nv);
   }
   if (total && !data) return fail(AHM_ERR_INVALID, "ahm_hmat_upload: data is NULL");
@@ -1748,6 +1781,7 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
   h->nv_dense = src->nv_dense;
   h->nv_u = src->nv_u;
   h->nv_v = src->nv_v;
+  h->sym = src->sym;
   h->leaves = src->leaves;
   h->h_tasks_a = src->h_tasks_a;
   h->h_tasks_b = src->h_tasks_b;
@@ -1819,6 +1853,7 @@ int ahm_hmat_stats_get(ahm_hmat h, ahm_hmat_stats* st) {
   st->max_rank = h->max_rank;
   st->n_rows = h->n_rows;
   st->n_cols = h->n_cols;
+  st->symmetric = (uint32_t)h->sym;
   return AHM_OK;
 }
 
@@ -1863,14 +1898,18 @@ int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void*
   if (h->storage == AHM_F64 && vec == AHM_F64) {
     AHM_TRY(ensure_tbuf(h, p, 8));
     AHM_TRY(ensure_pbuf(h, p, 8));
-    return launch_matvec<double, double>(h, own_workspace(h), alpha, (const double*)X, ldx, (double*)Y,
-                                         ldy, p);
+    AHM_TRY(sym_prepare(h, vec));
+    AHM_TRY((launch_matvec<double, double>(h, own_workspace(h), alpha, (const double*)X, ldx, (double*)Y,
+                                           ldy, p)));
+    return sym_transposed(h, alpha, X, ldx, Y, ldy, p, vec, Gate{});
   }
   if ((h->storage == AHM_F32 || h->storage == AHM_BF16) && vec == AHM_F32) {
     AHM_TRY(ensure_tbuf(h, p, 4));
     AHM_TRY(ensure_pbuf(h, p, 4));
-    return launch_matvec<float, float>(h, own_workspace(h), (float)alpha, (const float*)X, ldx, (float*)Y,
-                                       ldy, p);
+    AHM_TRY(sym_prepare(h, vec));
+    AHM_TRY((launch_matvec<float, float>(h, own_workspace(h), (float)alpha, (const float*)X, ldx, (float*)Y,
+                                         ldy, p)));
+    return sym_transposed(h, alpha, X, ldx, Y, ldy, p, vec, Gate{});
   }
   return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec: storage %d with vectors %d", h->storage, vec);
 }

@@ -15,6 +15,12 @@
 // Each output is summed by one thread in a fixed order: deterministic, no
 // atomics.  The panels are read once; the slots cost 8 bytes per panel
 // column (one per 65-331 values at the benchmark sizes).
+//
+// Symmetric matrices (AHM_LEAF_HEM diagonal leaves; mltaHeHVec,
+// H/mltaGeHVec.cpp:420-446) store the leaves on and above the block diagonal
+// only: their product is the forward product plus these four launches with
+// the slots of the diagonal leaves left out of step 4 (those blocks are
+// stored whole in the arena, so the forward product already applied them).
 #include <algorithm>
 #include <vector>
 
@@ -92,8 +98,9 @@ template <typename TS, typename TV>
 __global__ __launch_bounds__(256) void coldot_kernel(const ColTask* __restrict__ tasks,
                                                      const TS* __restrict__ arena,
                                                      const TV* __restrict__ v,
-                                                     TV* __restrict__ slots) {
+                                                     TV* __restrict__ slots, Gate g) {
   __shared__ TV out[1024];
+  if (gated(g)) return;
   const ColTask tk = tasks[blockIdx.x];
   const uint32_t m = tk.m;
   if (m > 128)
@@ -115,7 +122,8 @@ template <typename TV, bool ASSIGN>
 __global__ __launch_bounds__(256) void csr_sum_kernel(const uint32_t* __restrict__ off,
                                                       const uint32_t* __restrict__ pos,
                                                       const TV* __restrict__ slots, uint32_t n,
-                                                      TV* __restrict__ out, TV alpha) {
+                                                      TV* __restrict__ out, TV alpha, Gate g) {
+  if (gated(g)) return;
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= n) return;
   TV s = 0;
@@ -184,13 +192,34 @@ int build_transposed(ahm_hmat h) {
   std::vector<ColTask> ta, tb;
   col_tasks(h->h_panels_a, ta);
   col_tasks(h->h_panels, tb);
+  // symmetric storage: the rows of each diagonal leaf; a row panel lies in
+  // one of them, and its dense columns inside that range are the diagonal
+  // leaf's own (blocks of a block tree overlap the diagonal only there)
+  std::vector<uint8_t> skip;
+  if (h->sym) {
+    std::vector<uint32_t> dlo(h->n_rows, 0), dhi(h->n_rows, 0);
+    for (const ahm_leaf_desc& d : h->leaves)
+      if (d.flags & AHM_LEAF_HEM)
+        for (uint32_t r = d.b1; r < d.b1 + d.n1; ++r) {
+          dlo[r] = d.b1;
+          dhi[r] = d.b1 + d.n1;
+        }
+    skip.assign(h->h_zsrc.size(), 0);
+    for (const GemvTask& p : h->h_panels) {
+      const uint32_t lo = dlo[p.out], hi = dhi[p.out];
+      for (uint32_t c = 0; c < p.W; ++c) {
+        const uint32_t src = h->h_zsrc[p.zoff + c];
+        if (!(src & kFromT) && src >= lo && src < hi) skip[p.zoff + c] = 1;
+      }
+    }
+  }
   std::vector<uint32_t> dt, st, dy, sy;
   for (uint32_t q = 0; q < (uint32_t)h->h_zsrc.size(); ++q) {
     const uint32_t s = h->h_zsrc[q];
     if (s & kFromT) {
       dt.push_back(s & ~kFromT);
       st.push_back(q);
-    } else {
+    } else if (skip.empty() || !skip[q]) {
       dy.push_back(s);
       sy.push_back(q);
     }
@@ -213,7 +242,7 @@ int build_transposed(ahm_hmat h) {
 }
 
 template <typename TS, typename TV>
-int launch_t(ahm_hmat h, TV alpha, const TV* x, TV* y) {
+int launch_t(ahm_hmat h, TV alpha, const TV* x, TV* y, Gate g = Gate{}) {
   auto& T = h->tr;
   hipStream_t s = h->ctx->stream;
   TV* slots = (TV*)T.contrib.p;
@@ -221,23 +250,66 @@ int launch_t(ahm_hmat h, TV alpha, const TV* x, TV* y) {
   const TS* arena = (const TS*)h->arena.p;
   if (T.n_b)
     hipLaunchKernelGGL((coldot_kernel<TS, TV>), dim3(T.n_b), dim3(256), 0, s,
-                       (const ColTask*)T.tasks_b.p, arena, x, slots);
+                       (const ColTask*)T.tasks_b.p, arena, x, slots, g);
   if (h->t_len)
     hipLaunchKernelGGL((csr_sum_kernel<TV, true>), dim3((h->t_len + 255) / 256), dim3(256), 0, s,
                        (const uint32_t*)T.off_t.p, (const uint32_t*)T.pos_t.p, (const TV*)slots,
-                       h->t_len, tp, (TV)1);
+                       h->t_len, tp, (TV)1, g);
   if (T.n_a)
     hipLaunchKernelGGL((coldot_kernel<TS, TV>), dim3(T.n_a), dim3(256), 0, s,
-                       (const ColTask*)T.tasks_a.p, arena, (const TV*)tp, slots);
+                       (const ColTask*)T.tasks_a.p, arena, (const TV*)tp, slots, g);
   if (h->n_cols)
     hipLaunchKernelGGL((csr_sum_kernel<TV, false>), dim3((h->n_cols + 255) / 256), dim3(256), 0, s,
                        (const uint32_t*)T.off_y.p, (const uint32_t*)T.pos_y.p, (const TV*)slots,
-                       h->n_cols, y, alpha);
+                       h->n_cols, y, alpha, g);
   AHM_HIP(hipGetLastError());
   return AHM_OK;
 }
 
+// slots and t' sized for vectors of es bytes (allocates; before any capture)
+int ensure_t_buffers(ahm_hmat h, size_t es) {
+  auto& T = h->tr;
+  if (h->n_zsrc * es > T.contrib_cap) {
+    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+    AHM_TRY(T.contrib.alloc(std::max<size_t>(1, h->n_zsrc) * es));
+    T.contrib_cap = h->n_zsrc * es;
+  }
+  if ((size_t)h->t_len * es > T.tprime_cap) {
+    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+    AHM_TRY(T.tprime.alloc(std::max<size_t>(1, h->t_len) * es));
+    T.tprime_cap = (size_t)h->t_len * es;
+  }
+  return AHM_OK;
+}
+
 }  // namespace
+
+int sym_prepare(ahm_hmat h, ahm_dtype vec) {
+  if (!h->sym) return AHM_OK;
+  AHM_HIP(hipSetDevice(h->ctx->device));
+  AHM_TRY(build_transposed(h));
+  return ensure_t_buffers(h, dtype_size(vec));
+}
+
+// the transposed part of a symmetric product: y += alpha S^T x over the
+// leaves off the diagonal, column by column of X / Y; allocates nothing
+// after sym_prepare
+int sym_transposed(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y, uint32_t ldy, uint32_t p,
+                   ahm_dtype vec, Gate g) {
+  if (!h->sym) return AHM_OK;
+  const size_t es = dtype_size(vec);
+  if (!h->tr.built || h->n_zsrc * es > h->tr.contrib_cap || (size_t)h->t_len * es > h->tr.tprime_cap)
+    return fail(AHM_ERR_INVALID, "symmetric product: transposed part not prepared");
+  for (uint32_t j = 0; j < p; ++j) {
+    if (vec == AHM_F64)
+      AHM_TRY((launch_t<double, double>(h, alpha, (const double*)X + (size_t)j * ldx,
+                                        (double*)Y + (size_t)j * ldy, g)));
+    else
+      AHM_TRY((launch_t<float, float>(h, (float)alpha, (const float*)X + (size_t)j * ldx,
+                                      (float*)Y + (size_t)j * ldy, g)));
+  }
+  return AHM_OK;
+}
 
 }  // namespace ahm
 
@@ -255,20 +327,12 @@ int ahm_hmat_matvec_t(ahm_hmat h, double alpha, const void* X, uint32_t ldx, voi
   const bool f32 = (h->storage == AHM_F32 || h->storage == AHM_BF16) && vec == AHM_F32;
   if (!f64 && !f32)
     return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec_t: storage %d with vectors %d", h->storage, vec);
+  // a symmetric matrix is its own transpose
+  if (h->sym) return ahm_hmat_matvec(h, alpha, X, ldx, Y, ldy, p, vec);
   AHM_HIP(hipSetDevice(h->ctx->device));
   AHM_TRY(build_transposed(h));
-  auto& T = h->tr;
   const size_t es = f64 ? 8 : 4;
-  if (h->n_zsrc * es > T.contrib_cap) {
-    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
-    AHM_TRY(T.contrib.alloc(std::max<size_t>(1, h->n_zsrc) * es));
-    T.contrib_cap = h->n_zsrc * es;
-  }
-  if ((size_t)h->t_len * es > T.tprime_cap) {
-    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
-    AHM_TRY(T.tprime.alloc(std::max<size_t>(1, h->t_len) * es));
-    T.tprime_cap = (size_t)h->t_len * es;
-  }
+  AHM_TRY(ensure_t_buffers(h, es));
   for (uint32_t j = 0; j < p; ++j) {
     if (f64)
       AHM_TRY((launch_t<double, double>(h, alpha, (const double*)X + (size_t)j * ldx,

@@ -65,7 +65,7 @@ void flatten(blcluster* bl, mblock<T>** A, std::vector<ahm_leaf_desc>& L, std::v
     d.b2 = b->getb2() - bl->getb2();
     d.n2 = b->getn2();
     d.rank = m->isLrM() ? (int32_t)m->rank() : -1;
-    d.flags = 0;
+    d.flags = m->isHeM() ? AHM_LEAF_HEM : 0u;
     d.offset = off;
     const size_t nv = m->nvals();
     if (nv) std::memcpy(data.data() + off, m->getdata(), nv * sizeof(T));
@@ -260,6 +260,25 @@ bool mltaGeHGeM(double d, blcluster* bl, mblock<double>** A, unsigned p, double*
   return changed;
 }
 
+// the device matrix of a symmetric leaf array is symmetric (its packed
+// diagonal leaves mark it), so the plain product applies both triangles
+void mltaHeHVec(double d, blcluster* bl, mblock<double>** A, double* x, double* y) {
+  bool changed = false;
+  ahm_hmat h = device_matrix(bl, A, changed);
+  if (!h) return;
+  if (ahm_hmat_matvec_host(h, d, x, bl->getn2(), y, bl->getn1(), 1) != AHM_OK)
+    std::fprintf(stderr, "ahmed: mltaHeHVec failed: %s\n", ahm_last_error());
+}
+
+void mltaHeHGeM(double d, blcluster* bl, mblock<double>** A, unsigned p, double* X, unsigned ldX,
+                double* Y, unsigned ldY) {
+  bool changed = false;
+  ahm_hmat h = device_matrix(bl, A, changed);
+  if (!h) return;
+  if (ahm_hmat_matvec_host(h, d, X, ldX, Y, ldY, p) != AHM_OK)
+    std::fprintf(stderr, "ahmed: mltaHeHGeM failed: %s\n", ahm_last_error());
+}
+
 bool mltaGeHhVec(double d, blcluster* bl, mblock<double>** A, double* x, double* y) {
   bool changed = false;
   ahm_hmat h = device_matrix(bl, A, changed);
@@ -333,6 +352,8 @@ void copyH(blcluster* bl, mblock<double>** A, mblock<float>** B) {
     B[i] = new mblock<float>(b->getn1(), b->getn2());
     if (a->isLrM())
       B[i]->setrank(a->rank());
+    else if (a->isHeM())
+      B[i]->setHeM();
     else
       B[i]->setGeM();
     const unsigned long nv = a->nvals();
@@ -340,11 +361,13 @@ void copyH(blcluster* bl, mblock<double>** A, mblock<float>** B) {
   }
 }
 
-void matgenGeH_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double eps, unsigned rankmax,
-                   mblock<double>** A) {
+namespace {
+
+void matgen_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double eps, unsigned rankmax,
+                mblock<double>** A, uint32_t flags, const char* who) {
   std::vector<blcluster*> lv;
   bl->leaves(lv);
-  if (lv.size() != nblcks) std::fprintf(stderr, "ahmed: matgenGeH_gpu: %zu leaves, nblcks=%u\n", lv.size(), nblcks);
+  if (lv.size() != nblcks) std::fprintf(stderr, "ahmed: %s: %zu leaves, nblcks=%u\n", who, lv.size(), nblcks);
   const size_t nl = lv.size();
   std::vector<uint32_t> b1(nl), n1(nl), b2(nl), n2(nl), i0(nl);
   std::vector<uint8_t> adm(nl);
@@ -361,6 +384,7 @@ void matgenGeH_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double ep
   k.kind = G.kind;
   k.kappa = G.kappa;
   k.weight = G.weight;
+  k.flags = flags;
   ahm_hmat h = nullptr;
   {
     std::lock_guard<std::mutex> lock(g_mu);
@@ -368,17 +392,14 @@ void matgenGeH_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double ep
     if (ahm_assemble(g_ctx, G.pts, G.op_perm, G.N, (uint32_t)nl, b1.data(), n1.data(), b2.data(),
                      n2.data(), adm.data(), i0.data(), &k, eps, rankmax, nullptr, AHM_F64, 1,
                      nullptr, nullptr, &h) != AHM_OK) {
-      std::fprintf(stderr, "ahmed: matgenGeH_gpu failed: %s\n", ahm_last_error());
+      std::fprintf(stderr, "ahmed: %s failed: %s\n", who, ahm_last_error());
       return;
     }
   }
   std::vector<ahm_leaf_desc> L(nl);
   ahm_hmat_download(h, L.data(), nullptr);
   size_t total = 0;
-  for (const ahm_leaf_desc& d : L) {
-    const size_t nv = d.rank < 0 ? (size_t)d.n1 * d.n2 : (size_t)d.rank * (d.n1 + d.n2);
-    total = std::max(total, (size_t)d.offset + nv);
-  }
+  for (const ahm_leaf_desc& d : L) total = std::max(total, (size_t)(d.offset + ahm::leaf_values(d)));
   std::vector<double> data(total);
   ahm_hmat_download(h, nullptr, data.data());
   for (size_t l = 0; l < nl; ++l) {
@@ -386,10 +407,12 @@ void matgenGeH_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double ep
     mblock<double>*& m = A[lv[l]->getidx()];
     delete m;
     m = new mblock<double>(d.n1, d.n2);
-    if (d.rank < 0)
-      m->setGeM();
-    else
+    if (d.rank >= 0)
       m->setrank((unsigned)d.rank);
+    else if (d.flags & AHM_LEAF_HEM)
+      m->setHeM();
+    else
+      m->setGeM();
     if (m->nvals()) std::memcpy(m->getdata(), data.data() + d.offset, m->nvals() * sizeof(double));
   }
   // the device copy of exactly these leaves serves the following products
@@ -401,19 +424,38 @@ void matgenGeH_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double ep
   e.changed = true;
 }
 
+}  // namespace
+
+void matgenGeH_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double eps, unsigned rankmax,
+                   mblock<double>** A) {
+  matgen_gpu(G, nblcks, bl, eps, rankmax, A, 0u, "matgenGeH_gpu");
+}
+
+void matgenHeH_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double eps, unsigned rankmax,
+                   mblock<double>** A) {
+  matgen_gpu(G, nblcks, bl, eps, rankmax, A, AHM_ASSEMBLE_SYM, "matgenHeH_gpu");
+}
+
 namespace {
 
 template <class F>
 unsigned krylov(const Matrix<double>& A, double* b, double* x, F solve) {
-  const GeHMatrix<double>* G = dynamic_cast<const GeHMatrix<double>*>(&A);
-  if (!G) {
-    std::fprintf(stderr, "ahmed: the GPU solvers need a GeHMatrix<double> operator\n");
+  blcluster* tree = nullptr;
+  mblock<double>** blcks = nullptr;
+  if (const GeHMatrix<double>* G = dynamic_cast<const GeHMatrix<double>*>(&A)) {
+    tree = G->blclTree;
+    blcks = G->blcks;
+  } else if (const HeHMatrix<double>* S = dynamic_cast<const HeHMatrix<double>*>(&A)) {
+    tree = S->blclTree;
+    blcks = S->blcks;
+  } else {
+    std::fprintf(stderr, "ahmed: the GPU solvers need a GeHMatrix<double> or HeHMatrix<double> operator\n");
     return 2;
   }
   bool changed;
-  ahm_hmat h = device_matrix(G->blclTree, G->blcks, changed);
+  ahm_hmat h = device_matrix(tree, blcks, changed);
   if (!h) return 2;
-  const size_t n = G->blclTree->getn1();
+  const size_t n = tree->getn1();
   double *db = nullptr, *dx = nullptr;
   if (hipMalloc(&db, n * 8) != hipSuccess || hipMalloc(&dx, n * 8) != hipSuccess) {
     std::fprintf(stderr, "ahmed: out of device memory\n");

@@ -50,6 +50,7 @@ unsigned bbx_start_row(const std::vector<Point>& pts, const std::vector<unsigned
 
 struct ahm_tree_s {
   int kind = 0;
+  bool sym = false;   // symmetric block tree (subdivide_sym)
   uint32_t N = 0;
   std::vector<Point> pts;
   std::vector<unsigned> op, po;
@@ -69,10 +70,13 @@ extern "C" {
 int ahm_tree_build(const double* pts, uint32_t N, uint32_t bmin, double eta2, int kind,
                    ahm_tree* out) {
   if (!pts || !out || N == 0) return ahm::fail(AHM_ERR_INVALID, "ahm_tree_build: bad arguments");
+  const bool sym = (kind & AHM_TREE_SYM) != 0;
+  kind &= ~AHM_TREE_SYM;
   if (kind != AHM_TREE_BBX && kind != AHM_TREE_PCA)
     return ahm::fail(AHM_ERR_INVALID, "ahm_tree_build: unknown tree kind %d", kind);
   ahm_tree t = new ahm_tree_s;
   t->kind = kind;
+  t->sym = sym;
   t->N = N;
   t->pts.resize(N);
   for (uint32_t i = 0; i < N; ++i)
@@ -92,7 +96,10 @@ int ahm_tree_build(const double* pts, uint32_t N, uint32_t bmin, double eta2, in
     t->root = r;
     t->bl = new bemblcluster<Point, Point>(r, r);
   }
-  t->bl->subdivide(t->root, t->root, eta2, nbl);
+  if (sym)
+    t->bl->subdivide_sym(t->root, eta2, nbl);
+  else
+    t->bl->subdivide(t->root, t->root, eta2, nbl);
   t->bl->leaves(t->leaves);
   t->ncl = 1 + t->root->getncl();
   t->i0.resize(t->leaves.size());
@@ -155,8 +162,11 @@ extern "C" int ahm_assemble_tree(ahm_ctx ctx, ahm_tree t, const ahm_kernel* kern
   std::vector<double> pts(3 * (size_t)t->N);
   for (uint32_t i = 0; i < t->N; ++i)
     for (int j = 0; j < 3; ++j) pts[3 * (size_t)i + j] = t->pts[i].x[j];
+  if (!kernel) return ahm::fail(AHM_ERR_INVALID, "ahm_assemble_tree: NULL kernel");
+  ahm_kernel k = *kernel;
+  if (t->sym) k.flags |= AHM_ASSEMBLE_SYM;   // apprx_sym on a symmetric tree
   return ahm_assemble(ctx, pts.data(), t->op.data(), t->N, (uint32_t)nl, b1.data(), n1.data(),
-                      b2.data(), n2.data(), adm.data(), t->i0.data(), kernel, eps, rankmax,
+                      b2.data(), n2.data(), adm.data(), t->i0.data(), &k, eps, rankmax,
                       rank_hint, storage, keep_leaves, stats, aca_rank, out);
 }
 
@@ -172,6 +182,8 @@ extern "C" int ahm_assemble_batch(ahm_ctx ctx, uint32_t n_mat, const ahm_tree* t
   std::vector<uint32_t> off(n_mat + 1, 0);
   for (uint32_t s = 0; s < n_mat; ++s) {
     if (!trees[s]) return ahm::fail(AHM_ERR_INVALID, "ahm_assemble_batch: tree %u is NULL", s);
+    if (trees[s]->sym)
+      return ahm::fail(AHM_ERR_UNSUPPORTED, "ahm_assemble_batch: tree %u is symmetric", s);
     if ((uint64_t)off[s] + trees[s]->N >= 0x7fffffffu)
       return ahm::fail(AHM_ERR_UNSUPPORTED, "ahm_assemble_batch: too many points");
     off[s + 1] = off[s] + trees[s]->N;

@@ -47,6 +47,10 @@ typedef struct ahm_trih_s* ahm_trih;
 /* cluster tree kinds */
 #define AHM_TREE_BBX 0   /* cluster3d_bbx (Include/cluster_bbx.h:236-302)      */
 #define AHM_TREE_PCA 1   /* bemcluster  (Include/bemcluster.h:17-59)          */
+/* or'ed into the kind: symmetric block cluster tree (blcluster::subdivide_sym,
+ * cluster/blcluster.cpp:90-113): only the blocks on and above the diagonal
+ * are generated, diagonal blocks are never admissible */
+#define AHM_TREE_SYM 0x100
 
 /* One leaf of the block cluster tree with its storage.  rank < 0: dense leaf
  * (mblock::isGeM), rank >= 0: low-rank leaf (mblock::isLrM, rank 0 = empty).
@@ -67,6 +71,14 @@ typedef struct {
  * AHM_LEAF_UTM  upper triangle (mblock::isUtM), column j holds rows 0..j. */
 #define AHM_LEAF_LTM 1u
 #define AHM_LEAF_UTM 2u
+/* Dense diagonal leaf of a symmetric H-matrix (mblock::isHeM,
+ * Include/mblock.h:137-141): b1 == b2, n1 == n2 = n, rank < 0, the upper
+ * triangle packed column by column, n(n+1)/2 values ((i, j), i <= j, at
+ * i + j(j+1)/2).  A matrix holding such leaves is symmetric: its leaves are
+ * those of a symmetric block tree (on and above the diagonal), and every
+ * product applies each stored leaf and the transpose of each leaf off the
+ * diagonal (mltaHeHVec, H/mltaGeHVec.cpp:420-446). */
+#define AHM_LEAF_HEM 4u
 
 typedef struct {
   uint64_t n_leaves, n_dense, n_lowrank;
@@ -79,7 +91,7 @@ typedef struct {
   uint64_t n_tasks_a, n_tasks_b;
   uint32_t max_rank;
   uint32_t n_rows, n_cols;
-  uint32_t pad;
+  uint32_t symmetric;        /* 1: symmetric storage (AHM_LEAF_HEM leaves)    */
 } ahm_hmat_stats;
 
 const char* ahm_last_error(void);
@@ -184,6 +196,12 @@ typedef struct {
  * ACA (Include/ACA.h:217-350); storage and recompression stay as in
  * apprx_unsym */
 #define AHM_ASSEMBLE_ACAR 1u
+/* symmetric storage, the reference's matgenHeH_omp / apprx_sym
+ * (Include/matgen_omp.h:77-100, Include/apprx.h:87-131): the leaves are those
+ * of a symmetric block tree; admissible leaves take ACAr, dense leaves on the
+ * diagonal (b1 == b2) are stored packed (AHM_LEAF_HEM) and the matrix is
+ * symmetric.  ahm_assemble_tree sets it for trees built with AHM_TREE_SYM. */
+#define AHM_ASSEMBLE_SYM 2u
 
 typedef struct {
   uint64_t n_candidates;    /* admissible leaves that ran ACA              */

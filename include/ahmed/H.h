@@ -84,6 +84,15 @@ bool mltaGeHVec(float d, blcluster* bl, mblock<float>** A, float* x, float* y);
 bool mltaGeHGeM(double d, blcluster* bl, mblock<double>** A, unsigned p, double* X,
                 unsigned ldX, double* Y, unsigned ldY);
 
+// y += d A x and Y += d A X for a symmetric H-matrix (mltaHeHVec /
+// mltaHeHGeM, H/mltaGeHVec.cpp:420-472): bl is a symmetric block tree
+// (subdivide_sym), A holds its leaves on and above the diagonal with packed
+// Hermitian diagonal leaves (isHeM); every leaf is applied, the ones off the
+// diagonal also transposed.  x, y offset to bl->getb1() (== getb2()).
+void mltaHeHVec(double d, blcluster* bl, mblock<double>** A, double* x, double* y);
+void mltaHeHGeM(double d, blcluster* bl, mblock<double>** A, unsigned p, double* X, unsigned ldX,
+                double* Y, unsigned ldY);
+
 // y += d A^T x and Y += d A^T X (the real case of mltaGeHhVec / mltaGeHhGeM,
 // H/mltaGeHVec.cpp:61-104): x, X offset to row bl->getb1(), y, Y to column
 // bl->getb2().  Returns true if some leaf contributed.

@@ -4,7 +4,8 @@
 // A block (b1, n1, b2, n2) is a leaf when its cluster pair is admissible or
 // when either cluster is a leaf; otherwise it gets ns1 x ns2 sons stored
 // row-major.  Leaves are numbered (idx) in depth-first, row-major son order,
-// which is also the order the H-matrix products visit them.
+// which is also the order the H-matrix products visit them.  The symmetric
+// tree (subdivide_sym) leaves the sons below the diagonal NULL.
 #ifndef AHMED_BLCLUSTER_H
 #define AHMED_BLCLUSTER_H
 
@@ -121,6 +122,40 @@ class blcluster {
                          unsigned maxdepth = 0) {
     nblcks = 0;
     subdivide_(cl1, cl2, eta2, nblcks, 0, maxdepth);
+  }
+
+  // symmetric block cluster tree of (cl, cl) (cluster/blcluster.cpp:90-113):
+  // a diagonal block is never admissible; its sons below the diagonal stay
+  // NULL, the diagonal sons recurse symmetrically and the sons above the
+  // diagonal are subdivided as in the unsymmetric tree, in row-major order
+  virtual void subdivide_sym(cluster* cl, double eta2, unsigned& nblcks, unsigned maxdepth = 0) {
+    nblcks = 0;
+    subdivide_sym_(cl, eta2, nblcks, 0, maxdepth);
+  }
+
+ protected:
+  void subdivide_sym_(cluster* cl, double eta2, unsigned& nblcks, unsigned lvl, unsigned maxdepth) {
+    setadm(false);
+    if ((!maxdepth || lvl < maxdepth) && cl->isnleaf()) {
+      const unsigned ns = cl->getns();
+      blcluster** s = new blcluster*[ns * ns]();
+      ns1 = ns2 = ns;
+      sons = s;
+      for (unsigned i = 0; i < ns; ++i) {
+        cluster* ci = cl->getson(i);
+        blcluster* d = clone(ci, ci);
+        s[i * ns + i] = d;
+        d->subdivide_sym_(ci, eta2, nblcks, lvl + 1, maxdepth);
+        for (unsigned j = i + 1; j < ns; ++j) {
+          blcluster* u = clone(ci, cl->getson(j));
+          s[i * ns + j] = u;
+          u->subdivide_(ci, cl->getson(j), eta2, nblcks, lvl + 1, maxdepth);
+        }
+      }
+    } else {
+      setsons(0, 0, nullptr);
+      idx = nblcks++;
+    }
   }
 };
 

@@ -26,4 +26,11 @@ struct GpuKernel {
 void matgenGeH_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double eps,
                    unsigned rankmax, mblock<double>** A);
 
+// The symmetric assembly (matgenHeH_omp, Include/matgen_omp.h:95-100, with
+// apprx_sym per leaf, Include/apprx.h:87-131) on a symmetric block tree
+// (subdivide_sym): ACAr on the admissible leaves, the diagonal leaves dense
+// packed Hermitian (isHeM), the other dense leaves general.
+void matgenHeH_gpu(const GpuKernel& G, unsigned nblcks, blcluster* bl, double eps,
+                   unsigned rankmax, mblock<double>** A);
+
 #endif

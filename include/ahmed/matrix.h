@@ -1,5 +1,6 @@
 // matrix.h -- operator interface of the Krylov solvers (drop-in for the
-// reference's Matrix<T> / GeHMatrix<T>, Include/matrix.h:20-32,352-370).
+// reference's Matrix<T> / GeHMatrix<T> / HeHMatrix<T>, Include/matrix.h:
+// 20-32,307-323,352-370).
 #ifndef AHMED_MATRIX_H
 #define AHMED_MATRIX_H
 
@@ -26,6 +27,20 @@ struct GeHMatrix : public Matrix<T> {
     if (blclTree) freembls(blclTree, blcks);
   }
   void amux(T d, T* x, T* y) const override { mltaGeHVec(d, blclTree, blcks, x, y); }
+};
+
+// Symmetric H-matrix on a symmetric block tree (subdivide_sym), leaves on and
+// above the diagonal (matgenHeH_gpu); the tree must outlive it, the leaves
+// are freed with it.
+template <class T>
+struct HeHMatrix : public Matrix<T> {
+  mblock<T>** blcks;
+  blcluster* blclTree;
+  HeHMatrix(unsigned n, blcluster* tree = nullptr) : Matrix<T>(n, n), blcks(nullptr), blclTree(tree) {}
+  ~HeHMatrix() {
+    if (blclTree) freembls(blclTree, blcks);
+  }
+  void amux(T d, T* x, T* y) const override { mltaHeHVec(d, blclTree, blcks, x, y); }
 };
 
 #endif

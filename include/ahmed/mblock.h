@@ -30,12 +30,13 @@ class mblock {
     unsigned is_LrM : 1;
     unsigned is_LtM : 1;   // packed unit lower triangle, permutation on the diagonal
     unsigned is_UtM : 1;   // packed upper triangle
+    unsigned is_HeM : 1;   // symmetric, upper triangle packed (diagonal leaves of HeH matrices)
   } info;
 
  public:
   mblock(unsigned m, unsigned n) : data(nullptr), n1(m), n2(n), bl_rank(0) {
     info.is_LrM = 1;
-    info.is_LtM = info.is_UtM = 0;
+    info.is_LtM = info.is_UtM = info.is_HeM = 0;
   }
   ~mblock() { delete[] data; }
   mblock(const mblock&) = delete;
@@ -43,7 +44,9 @@ class mblock {
 
   // number of stored values (Include/mblock.h:286-291)
   unsigned long nvals() const {
-    if (isGeM()) return (info.is_LtM || info.is_UtM) ? (unsigned long)n1 * (n1 + 1) / 2 : (unsigned long)n1 * n2;
+    if (isGeM())
+      return (info.is_LtM || info.is_UtM || info.is_HeM) ? (unsigned long)n1 * (n1 + 1) / 2
+                                                         : (unsigned long)n1 * n2;
     return (unsigned long)(n1 + n2) * bl_rank;
   }
   unsigned long size() const { return sizeof(T) * nvals() + sizeof(mblock<T>); }
@@ -57,6 +60,10 @@ class mblock {
   // (mblock::decomp_LU, H/mblock_D.cpp:1078-1107), a UtM leaf is upper
   bool isLtM() const { return !info.is_LrM && info.is_LtM; }
   bool isUtM() const { return !info.is_LrM && info.is_UtM; }
+  // Hermitian (real: symmetric) dense leaf on the diagonal of a symmetric
+  // H-matrix (Include/mblock.h:137-141,364-370): the upper triangle packed
+  // column by column, (i, j) with i <= j at i + j(j+1)/2
+  bool isHeM() const { return !info.is_LrM && info.is_HeM; }
   unsigned rank() const {
     assert(isLrM());
     return bl_rank;
@@ -67,7 +74,7 @@ class mblock {
     delete[] data;
     data = nullptr;
     info.is_LrM = 1;
-    info.is_LtM = info.is_UtM = 0;
+    info.is_LtM = info.is_UtM = info.is_HeM = 0;
     bl_rank = 0;
     ahm_surface_invalidate();
   }
@@ -75,20 +82,20 @@ class mblock {
     delete[] data;
     bl_rank = k;
     info.is_LrM = 1;
-    info.is_LtM = info.is_UtM = 0;
+    info.is_LtM = info.is_UtM = info.is_HeM = 0;
     data = k ? new T[(size_t)k * (n1 + n2)] : nullptr;
     ahm_surface_invalidate();
   }
   void setGeM() {
     delete[] data;
-    info.is_LrM = info.is_LtM = info.is_UtM = 0;
+    info.is_LrM = info.is_LtM = info.is_UtM = info.is_HeM = 0;
     data = new T[(size_t)n1 * n2];
     ahm_surface_invalidate();
   }
   void setLtM() {
     assert(n1 == n2);
     delete[] data;
-    info.is_LrM = info.is_UtM = 0;
+    info.is_LrM = info.is_UtM = info.is_HeM = 0;
     info.is_LtM = 1;
     data = new T[(size_t)n1 * (n1 + 1) / 2];
     ahm_surface_invalidate();
@@ -96,8 +103,16 @@ class mblock {
   void setUtM() {
     assert(n1 == n2);
     delete[] data;
-    info.is_LrM = info.is_LtM = 0;
+    info.is_LrM = info.is_LtM = info.is_HeM = 0;
     info.is_UtM = 1;
+    data = new T[(size_t)n1 * (n1 + 1) / 2];
+    ahm_surface_invalidate();
+  }
+  void setHeM() {
+    assert(n1 == n2);
+    delete[] data;
+    info.is_LrM = info.is_LtM = info.is_UtM = 0;
+    info.is_HeM = 1;
     data = new T[(size_t)n1 * (n1 + 1) / 2];
     ahm_surface_invalidate();
   }
@@ -120,8 +135,8 @@ class mblock {
   // reference's Hermitian / triangular / symmetric kinds stops the program
   // with a message, as the reference does for an unknown status.
   void save(std::ofstream& os) const {
-    if (isLtM() || isUtM()) {
-      std::cerr << "mblock::save: triangular leaves are not supported" << std::endl;
+    if (isLtM() || isUtM() || isHeM()) {
+      std::cerr << "mblock::save: packed (triangular / Hermitian) leaves are not supported" << std::endl;
       std::exit(1);
     }
     const unsigned status = isLrM() ? 0u : 4u;

@@ -1,9 +1,10 @@
 // solvers.h -- Krylov solvers (drop-in for the reference's GMRes / CG,
-// Include/solvers.h:14-34).  For a GeHMatrix<double> the whole iteration runs
-// on the GPU (ahm_gmres / ahm_cg); eps and nsteps are in/out exactly as in
-// the reference (solvers/GMRES.cpp:80-181, solvers/CG.cpp:31-113).  Returns 0
-// on convergence, 1 otherwise, 2 if the operator is not a GeHMatrix<double>
-// (a user-defined host operator cannot be applied on the device).
+// Include/solvers.h:14-34).  For a GeHMatrix<double> or HeHMatrix<double> the
+// whole iteration runs on the GPU (ahm_gmres / ahm_cg); eps and nsteps are
+// in/out exactly as in the reference (solvers/GMRES.cpp:80-181,
+// solvers/CG.cpp:31-113).  Returns 0 on convergence, 1 otherwise, 2 if the
+// operator is neither (a user-defined host operator cannot be applied on the
+// device).
 #ifndef AHMED_SOLVERS_H
 #define AHMED_SOLVERS_H
 

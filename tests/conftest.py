@@ -27,7 +27,7 @@ def golden_names():
     return sorted(f[len("golden_"):-4] for f in os.listdir(GOLDEN)
                   if f.startswith("golden_") and f.endswith(".npz")
                   and f not in ("golden_aca.npz", "golden_mbls.npz", "golden_transposed.npz", "golden_hlu.npz",
-                                "golden_acar.npz"))
+                                "golden_acar.npz", "golden_heh.npz"))
 
 
 def load_golden(name):

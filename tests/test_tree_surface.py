@@ -45,3 +45,24 @@ def test_single_point_and_tiny_trees(oracle_api):
         # a one-point diagonal block is admissible through the reference's weak
         # admissibility (its box touches itself exactly)
         assert lv["adm"][0] == ref["adm"][0] == (1 if N == 1 else 0)
+
+
+def test_symmetric_tree_matches_reference_golden():
+    """subdivide_sym through the C ABI (kind | AHM_TREE_SYM) against the
+    reference's symmetric trees (golden_heh.npz; cluster/blcluster.cpp:90-113)."""
+    import os
+    from ahmed_amd import _lib
+    from conftest import GOLDEN
+    G = np.load(os.path.join(GOLDEN, "golden_heh.npz"), allow_pickle=False)
+    names = sorted({k[:-2] for k in G.files if k.endswith("_N")})
+    assert names
+    for name in names:
+        geo, N, seed = str(G[f"{name}_geometry"]), int(G[f"{name}_N"]), int(G[f"{name}_seed"])
+        pts = geometry.sphere(N, seed) if geo == "sphere" else geometry.cube_surface(N, seed)
+        T = _lib.Tree(pts, int(G[f"{name}_bmin"]), float(G[f"{name}_eta2"]), int(G[f"{name}_kind"]) | _lib.TREE_SYM)
+        assert T.sym
+        op, po = T.perm()
+        assert np.array_equal(op, G[f"{name}_op_perm"]) and np.array_equal(po, G[f"{name}_po_perm"])
+        lv = T.leaves()
+        for f in ("b1", "n1", "b2", "n2", "adm", "i0"):
+            assert np.array_equal(lv[f], G[f"{name}_{f}"]), (name, f)
