@@ -184,6 +184,14 @@ void matvec_signature(ahm_hmat h, std::vector<uint64_t>& sig);
 int sym_prepare(ahm_hmat h, ahm_dtype vec);
 int sym_transposed(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y, uint32_t ldy, uint32_t p,
                    ahm_dtype vec, Gate g);
+// Single right-hand side, symmetric: the phase-B launch that reads each row
+// panel value once for both directions (y rows += alpha [D | U] z, or the
+// chunk's partial sums, and the column dots of the panel with x_rows into the
+// slots), then, after the phase-B chunk reduction, sym_rest: t' = U^T x,
+// V t', and y += alpha (slot sums) (hmat_t.hip).  After sym_prepare.
+int sym_phase_b(ahm_hmat h, double alpha, const void* x, const void* t, void* y, void* part, ahm_dtype vec,
+                Gate g);
+int sym_rest(ahm_hmat h, double alpha, void* y, ahm_dtype vec, Gate g);
 
 // solver state kept with a handle between calls (gmres.hip)
 struct GmresCache;
@@ -292,6 +300,12 @@ struct ahm_hmat_s {
     bool built = false;
     ahm::DeviceBuf tasks_a, tasks_b, off_t, pos_t, off_y, pos_y;
     uint32_t n_a = 0, n_b = 0;
+    // symmetric products: the phase-B tasks (same chunks and partial-sum
+    // slots as tasks_b of the handle) with the first row of their panel,
+    // for the single pass that forms row sums and column dots together
+    bool sym_built = false;
+    ahm::DeviceBuf sym_tasks;
+    uint32_t n_sym = 0;
     ahm::DeviceBuf contrib, tprime;   // one value per panel column; t' = U^T x
     size_t contrib_cap = 0, tprime_cap = 0;
   } tr;

@@ -1325,7 +1325,11 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
   hipStream_t s = h->ctx->stream;
   TV* t = (TV*)ws.t;
   TV* part = (TV*)ws.part;
-  if (h->fused && p <= 8 && !ws.gate[0].frozen) {
+  const ahm_dtype vec = std::is_same<TV, double>::value ? AHM_F64 : AHM_F32;
+  // symmetric storage, one right-hand side: the transposed part rides on the
+  // row-panel pass (after sym_prepare); wider products add it afterwards
+  const bool sym1 = p == 1 && h->sym && h->tr.sym_built;
+  if (h->fused && p <= 8 && !ws.gate[0].frozen && !h->sym) {
     if (h->fz_n)
       hipLaunchKernelGGL((fused_kernel<TS, TV>), dim3(h->fz_n), dim3(kPanelThreads), 0, s,
                          (const GemvTask*)h->fz_panels.p, (const GemvTask*)h->fz_apanels.p,
@@ -1346,7 +1350,9 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
                          (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha, 0,
                          ws.gate[2]);
     if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
-    if (h->n_b)
+    if (sym1)   // both directions of the row panels in one pass
+      AHM_TRY(sym_phase_b(h, (double)alpha, X, t, Y, part, vec, ws.gate[1]));
+    else if (h->n_b)
       hipLaunchKernelGGL((panel_gemv_kernel<TS, TV, true>), dim3(h->n_b, 1), dim3(kPanelThreads), 0, s,
                          (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
                          (const TS*)h->arena.p, X, ldx, t, h->t_len, Y, ldy, part, h->part_len, alpha,
@@ -1415,6 +1421,7 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
                        (const ReduceTask*)h->tasks_r.p, (const TV*)part, h->part_len, Y, ldy, alpha, 0,
                        p == 1 ? ws.gate[3] : Gate{});
   AHM_HIP(hipGetLastError());
+  if (sym1) AHM_TRY(sym_rest(h, (double)alpha, Y, vec, ws.gate[3]));
   return AHM_OK;
 }
 
@@ -1471,7 +1478,8 @@ int matvec_gated(ahm_hmat h, double alpha, const void* x, void* y, ahm_dtype vec
                                          h->n_rows, 1)));
   else
     return fail(AHM_ERR_UNSUPPORTED, "gated product: storage %d with vectors %d", h->storage, vec);
-  return sym_transposed(h, alpha, x, h->n_rows, y, h->n_cols, 1, vec, Gate{frozen, nullptr});
+  if (h->sym && !h->tr.sym_built) return fail(AHM_ERR_INVALID, "gated product: symmetric part not prepared");
+  return AHM_OK;
 }
 
 void matvec_signature(ahm_hmat h, std::vector<uint64_t>& sig) {
@@ -1483,7 +1491,8 @@ void matvec_signature(ahm_hmat h, std::vector<uint64_t>& sig) {
          (uint64_t)h->storage, h->chunk_values,
          // the transposed part of symmetric products
          (uint64_t)h->sym, (uint64_t)(uintptr_t)h->tr.contrib.p, (uint64_t)(uintptr_t)h->tr.tprime.p,
-         (uint64_t)(uintptr_t)h->tr.tasks_a.p, (uint64_t)(uintptr_t)h->tr.tasks_b.p};
+         (uint64_t)(uintptr_t)h->tr.tasks_a.p, (uint64_t)(uintptr_t)h->tr.tasks_b.p,
+         (uint64_t)(uintptr_t)h->tr.sym_tasks.p, (uint64_t)h->tr.n_sym};
 }
 
 }  // namespace ahm
@@ -1623,6 +1632,7 @@ int set_chunking(ahm_hmat h, uint64_t cv) {
   h->h_tasks_b = std::move(b);
   h->h_tasks_r = std::move(r);
   h->chunk_values = cv;
+  h->tr.sym_built = false;   // the symmetric pass follows the phase-B tasks
   if (h->storage == AHM_BF16) AHM_TRY(build_bf16(h));
   return AHM_OK;
 }
@@ -1901,7 +1911,7 @@ int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void*
     AHM_TRY(sym_prepare(h, vec));
     AHM_TRY((launch_matvec<double, double>(h, own_workspace(h), alpha, (const double*)X, ldx, (double*)Y,
                                            ldy, p)));
-    return sym_transposed(h, alpha, X, ldx, Y, ldy, p, vec, Gate{});
+    return p > 1 ? sym_transposed(h, alpha, X, ldx, Y, ldy, p, vec, Gate{}) : AHM_OK;
   }
   if ((h->storage == AHM_F32 || h->storage == AHM_BF16) && vec == AHM_F32) {
     AHM_TRY(ensure_tbuf(h, p, 4));
@@ -1909,7 +1919,7 @@ int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void*
     AHM_TRY(sym_prepare(h, vec));
     AHM_TRY((launch_matvec<float, float>(h, own_workspace(h), (float)alpha, (const float*)X, ldx, (float*)Y,
                                          ldy, p)));
-    return sym_transposed(h, alpha, X, ldx, Y, ldy, p, vec, Gate{});
+    return p > 1 ? sym_transposed(h, alpha, X, ldx, Y, ldy, p, vec, Gate{}) : AHM_OK;
   }
   return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_matvec: storage %d with vectors %d", h->storage, vec);
 }
@@ -1962,6 +1972,7 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
 
 int ahm_hmat_set_fused(ahm_hmat h, int on) {
   if (!h) return fail(AHM_ERR_INVALID, "ahm_hmat_set_fused: NULL handle");
+  if (on && h->sym) return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_set_fused: symmetric matrices");
   if (on && !h->fz_built) {
     AHM_HIP(hipSetDevice(h->ctx->device));
     AHM_TRY(build_fused(h));
@@ -2001,7 +2012,7 @@ int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint
   AHM_HIP(hipEventCreate(&e1));
   int st = AHM_OK;
   const int saved_fused = h->fused;
-  for (int fu = 0; fu <= (p <= 8 ? 1 : 0) && !st; ++fu) {
+  for (int fu = 0; fu <= (p <= 8 && !h->sym ? 1 : 0) && !st; ++fu) {
     if (fu && !h->fz_built && (st = build_fused(h))) break;
     h->fused = fu;
     for (uint64_t cv : cand) {

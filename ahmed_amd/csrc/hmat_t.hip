@@ -117,6 +117,140 @@ __global__ __launch_bounds__(256) void coldot_kernel(const ColTask* __restrict__
     coldot_body<TS, TV, 1, 1>(tk, arena, v, slots, out);
 }
 
+// Symmetric matrices, one right-hand side: a phase-B task (row panel chunk
+// of m <= 256 rows, W columns) read once for both directions.  The lanes of
+// a group of GS hold rows gl, gl + GS, ... (as coldot_body); a group takes
+// four columns per pass; per value one fma into the column dot with x_rows
+// (slots, summed over the group by shuffles) and one into the lane's row
+// sums with z (the column's source value, staged in LDS per pass).  The row
+// sums of the kGroups groups meet in LDS in group order.
+struct SymTask {
+  uint64_t data;   // arena offset of the chunk
+  uint64_t zoff;   // source-list position of its first column (= its first slot)
+  uint32_t m, W;
+  uint32_t row0;   // first row of the chunk's panel (rows of x for the column dots)
+  uint32_t out;    // y row (partial == 0) or partial-sum offset (partial == 1)
+  uint32_t partial, pad;
+};
+
+template <typename TS, typename TV, int GS, int NQ>
+__device__ __forceinline__ void sym_body(const SymTask& tk, const uint32_t* __restrict__ zsrc,
+                                         const TS* __restrict__ arena, const TV* __restrict__ x,
+                                         const TV* __restrict__ t, TV* __restrict__ y, TV* __restrict__ part,
+                                         TV* __restrict__ slots, TV alpha, TV* zs, TV* red) {
+  constexpr int kPerWave = 64 / GS;
+  constexpr int kGroups = 4 * kPerWave;
+  constexpr uint32_t kZ = 1024;   // columns of z staged at a time
+  const int m = (int)tk.m;
+  const int lane = threadIdx.x & 63;
+  const int gl = lane & (GS - 1);
+  const int grp = (int)(threadIdx.x >> 6) * kPerWave + lane / GS;
+  const int nq = (m + GS - 1) / GS;
+  TV xr[NQ], acc[NQ];
+  int roff[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int r = gl + GS * q;
+    const bool in = r < m;
+    roff[q] = in ? r : 0;
+    xr[q] = in ? x[tk.row0 + r] : (TV)0;
+    acc[q] = 0;
+  }
+  for (uint32_t c0 = 0; c0 < tk.W; c0 += kZ) {
+    const uint32_t cw = min(kZ, tk.W - c0);
+    const TS* P = arena + tk.data + (uint64_t)c0 * m;
+    __syncthreads();   // the previous chunk's z is consumed
+    {
+      // source entries first, then their values, four of each in flight
+      uint32_t src[kZ / 256];
+#pragma unroll
+      for (int k = 0; k < (int)(kZ / 256); ++k) {
+        const uint32_t i = threadIdx.x + 256u * k;
+        src[k] = zsrc[tk.zoff + c0 + (i < cw ? i : 0)];
+      }
+#pragma unroll
+      for (int k = 0; k < (int)(kZ / 256); ++k) {
+        const uint32_t i = threadIdx.x + 256u * k;
+        const TV v = (src[k] & kFromT) ? t[src[k] & ~kFromT] : x[src[k]];
+        zs[i] = i < cw ? v : (TV)0;
+      }
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < cw; base += 4u * kGroups) {
+      TV sd[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t c = base + (uint32_t)(u * kGroups + grp);
+        const bool in = c < cw;
+        const TS* Pc = P + (uint64_t)(in ? c : base) * m;
+        const TV zc = zs[in ? c : 0];
+        TV s = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          if (q < nq) {
+            const TV v = (TV)Pc[roff[q]];
+            s = fma(v, xr[q], s);
+            acc[q] = fma(v, in ? zc : (TV)0, acc[q]);
+          }
+        sd[u] = s;
+      }
+#pragma unroll
+      for (int o = GS / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sd[u] += __shfl_xor(sd[u], o, 64);
+      if (gl == 0)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t c = base + (uint32_t)(u * kGroups + grp);
+          if (c < cw) slots[tk.zoff + c0 + c] = sd[u];
+        }
+    }
+  }
+  // row sums: group g's partial sums of row r at red[g * m + r], summed in group order
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int r = gl + GS * q;
+    if (q < nq && r < m) red[grp * m + r] = acc[q];
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < m; r += 256) {
+    TV sum = 0;
+    for (int g2 = 0; g2 < kGroups; ++g2) sum += red[g2 * m + r];
+    if (tk.partial)
+      part[tk.out + r] = sum;
+    else
+      y[tk.out + r] += alpha * sum;
+  }
+}
+
+template <typename TS, typename TV>
+__global__ __launch_bounds__(256) void sym_panel_kernel(const SymTask* __restrict__ tasks,
+                                                        const uint32_t* __restrict__ zsrc,
+                                                        const TS* __restrict__ arena, const TV* __restrict__ x,
+                                                        const TV* __restrict__ t, TV* __restrict__ y,
+                                                        TV* __restrict__ part, TV* __restrict__ slots, TV alpha,
+                                                        Gate g) {
+  // 1024 staged z values; kGroups * m <= 2048 row partials
+  __shared__ TV zs[1024];
+  __shared__ TV red[2048];
+  if (gated(g)) return;
+  const SymTask tk = tasks[blockIdx.x];
+  const uint32_t m = tk.m;
+  if (m > 128)
+    sym_body<TS, TV, 32, 8>(tk, zsrc, arena, x, t, y, part, slots, alpha, zs, red);
+  else if (m > 64)
+    sym_body<TS, TV, 16, 8>(tk, zsrc, arena, x, t, y, part, slots, alpha, zs, red);
+  else if (m > 4)
+    sym_body<TS, TV, 8, 8>(tk, zsrc, arena, x, t, y, part, slots, alpha, zs, red);
+  else if (m > 2)
+    sym_body<TS, TV, 4, 1>(tk, zsrc, arena, x, t, y, part, slots, alpha, zs, red);
+  else if (m > 1)
+    sym_body<TS, TV, 2, 1>(tk, zsrc, arena, x, t, y, part, slots, alpha, zs, red);
+  else
+    sym_body<TS, TV, 1, 1>(tk, zsrc, arena, x, t, y, part, slots, alpha, zs, red);
+}
+
 // out[i] (=|+= alpha *) sum_{j in [off[i], off[i+1])} slots[pos[j]]
 template <typename TV, bool ASSIGN>
 __global__ __launch_bounds__(256) void csr_sum_kernel(const uint32_t* __restrict__ off,
@@ -284,11 +418,98 @@ int ensure_t_buffers(ahm_hmat h, size_t es) {
 
 }  // namespace
 
+// the handle's phase-B tasks as SymTasks: a chunk's panel is the row panel
+// whose arena range holds the chunk's values
+int build_sym_tasks(ahm_hmat h) {
+  auto& T = h->tr;
+  if (T.sym_built) return AHM_OK;
+  std::vector<std::pair<uint64_t, uint32_t>> start;   // (arena offset, first row) of every panel
+  start.reserve(h->h_panels.size());
+  for (const GemvTask& p : h->h_panels) start.emplace_back(p.data, p.out);
+  std::sort(start.begin(), start.end());
+  std::vector<SymTask> st(h->h_tasks_b.size());
+  for (size_t i = 0; i < st.size(); ++i) {
+    const GemvTask& b = h->h_tasks_b[i];
+    if (b.m > 256) return fail(AHM_ERR_UNSUPPORTED, "symmetric product: row panel of %u rows", b.m);
+    auto it = std::upper_bound(start.begin(), start.end(), std::make_pair(b.data, 0xFFFFFFFFu));
+    if (it == start.begin()) return fail(AHM_ERR_INVALID, "symmetric product: chunk outside the panels");
+    --it;
+    SymTask& s = st[i];
+    s.data = b.data;
+    s.zoff = b.zoff;
+    s.m = b.m;
+    s.W = b.W;
+    s.row0 = it->second;
+    s.out = b.out;
+    s.partial = b.partial;
+    s.pad = 0;
+    if (!b.partial && b.out != s.row0) return fail(AHM_ERR_INVALID, "symmetric product: panel rows disagree");
+  }
+  AHM_TRY(upload(T.sym_tasks, st, h->ctx->stream));
+  AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+  T.n_sym = (uint32_t)st.size();
+  T.sym_built = true;
+  return AHM_OK;
+}
+
 int sym_prepare(ahm_hmat h, ahm_dtype vec) {
   if (!h->sym) return AHM_OK;
   AHM_HIP(hipSetDevice(h->ctx->device));
   AHM_TRY(build_transposed(h));
+  AHM_TRY(build_sym_tasks(h));
   return ensure_t_buffers(h, dtype_size(vec));
+}
+
+int sym_phase_b(ahm_hmat h, double alpha, const void* x, const void* t, void* y, void* part, ahm_dtype vec,
+                Gate g) {
+  auto& T = h->tr;
+  const size_t es = dtype_size(vec);
+  if (!T.built || !T.sym_built || h->n_zsrc * es > T.contrib_cap)
+    return fail(AHM_ERR_INVALID, "symmetric product: not prepared");
+  if (!T.n_sym) return AHM_OK;
+  hipStream_t s = h->ctx->stream;
+  if (vec == AHM_F64)
+    hipLaunchKernelGGL((sym_panel_kernel<double, double>), dim3(T.n_sym), dim3(256), 0, s,
+                       (const SymTask*)T.sym_tasks.p, (const uint32_t*)h->zsrc.p, (const double*)h->arena.p,
+                       (const double*)x, (const double*)t, (double*)y, (double*)part, (double*)T.contrib.p,
+                       alpha, g);
+  else
+    hipLaunchKernelGGL((sym_panel_kernel<float, float>), dim3(T.n_sym), dim3(256), 0, s,
+                       (const SymTask*)T.sym_tasks.p, (const uint32_t*)h->zsrc.p, (const float*)h->arena.p,
+                       (const float*)x, (const float*)t, (float*)y, (float*)part, (float*)T.contrib.p,
+                       (float)alpha, g);
+  AHM_HIP(hipGetLastError());
+  return AHM_OK;
+}
+
+// the transposed part after the slots of the phase-B columns are filled:
+// t' = sum of U-column dots, the V^T panels' column dots with t', and
+// y += alpha * (dense-column and V-column slot sums)
+int sym_rest(ahm_hmat h, double alpha, void* y, ahm_dtype vec, Gate g) {
+  auto& T = h->tr;
+  hipStream_t s = h->ctx->stream;
+  auto run = [&](auto tag) {
+    using TV = decltype(tag);
+    TV* slots = (TV*)T.contrib.p;
+    TV* tp = (TV*)T.tprime.p;
+    if (h->t_len)
+      hipLaunchKernelGGL((csr_sum_kernel<TV, true>), dim3((h->t_len + 255) / 256), dim3(256), 0, s,
+                         (const uint32_t*)T.off_t.p, (const uint32_t*)T.pos_t.p, (const TV*)slots, h->t_len, tp,
+                         (TV)1, g);
+    if (T.n_a)
+      hipLaunchKernelGGL((coldot_kernel<TV, TV>), dim3(T.n_a), dim3(256), 0, s, (const ColTask*)T.tasks_a.p,
+                         (const TV*)h->arena.p, (const TV*)tp, slots, g);
+    if (h->n_cols)
+      hipLaunchKernelGGL((csr_sum_kernel<TV, false>), dim3((h->n_cols + 255) / 256), dim3(256), 0, s,
+                         (const uint32_t*)T.off_y.p, (const uint32_t*)T.pos_y.p, (const TV*)slots, h->n_cols,
+                         (TV*)y, (TV)alpha, g);
+  };
+  if (vec == AHM_F64)
+    run(double{});
+  else
+    run(float{});
+  AHM_HIP(hipGetLastError());
+  return AHM_OK;
 }
 
 // the transposed part of a symmetric product: y += alpha S^T x over the

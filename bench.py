@@ -185,6 +185,58 @@ def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, y_gpu_of, budget_s=20.0, 
     return out
 
 
+def symmetric_path(ctx, torch, pts, bmin, eta, weight, eps, tol, H, x, ms_geh, steps):
+    """The symmetric storage of the same matrix (the reference's BEM example:
+    subdivide_sym, apprx_sym with ACAr and packed Hermitian diagonal leaves,
+    mltaHeHVec, CG on HeHMatrix): assembly time, stored values against the
+    unsymmetric matrix, the product (forward part plus the transposed part of
+    the off-diagonal leaves) and its difference from the unsymmetric product,
+    CG to the solver tolerance."""
+    from ahmed_amd import _lib
+    N = len(pts)
+    t0 = time.perf_counter()
+    Ts = _lib.Tree(pts, bmin, eta * eta, _lib.TREE_BBX | _lib.TREE_SYM)
+    t_tree = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S, ast = _lib.HMatrix.assemble(ctx, Ts, _lib.KERNEL_LAPLACE, 0.0, weight, eps)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    ss = S.stats()
+    y = torch.zeros(N, dtype=torch.float64, device="cuda")
+    ms = timed(lambda: S.matvec(x, y), torch, steps, 2)
+    y.zero_()
+    S.matvec(x, y)
+    yg = torch.zeros(N, dtype=torch.float64, device="cuda")
+    H.matvec(x, yg)
+    diff = float((y - yg).abs().max() / yg.abs().max())
+    b = torch.zeros(N, dtype=torch.float64, device="cuda")
+    S.matvec(torch.ones(N, dtype=torch.float64, device="cuda"), b)
+    xs = torch.zeros(N, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = S.cg(b, xs, eps=tol, nsteps=5000)
+    torch.cuda.synchronize()
+    t_cg = time.perf_counter() - t0
+    # stored values: the arena holds the diagonal leaves whole (n^2 values)
+    lv = Ts.leaves()
+    dg = lv["b1"] == lv["b2"]
+    n = lv["n1"][dg].astype(np.float64)
+    stored = ss["nvals"] - float((n * n - n * (n + 1) / 2).sum())
+    out = {"workload": f"the same points, symmetric block tree (subdivide_sym), ACAr, eps {eps}",
+           "tree_s": round(t_tree, 3), "gpu_assembly_s": round(t_asm, 3), "leaves": ss["n_leaves"],
+           "stored_values": stored, "stored_vs_unsymmetric": round(stored / H.stats()["nvals"], 4),
+           "matvec_ms": round(ms, 4), "matvec_ms_unsymmetric": round(ms_geh, 4),
+           "rel_diff_vs_unsymmetric_product": diff,
+           "cg": {"seconds": round(t_cg, 4), "converged": r["converged"], "iterations": r["nsteps"],
+                  "residual": r["eps"], "max_err_vs_ones": float((xs - 1).abs().max())}}
+    log(f"symmetric storage: assembly {t_asm:.2f} s, {stored:.4g} values ({out['stored_vs_unsymmetric']:.3f} of "
+        f"the unsymmetric), product {ms:.3f} ms (unsymmetric {ms_geh:.3f} ms), rel.diff {diff:.2e}; "
+        f"CG {t_cg:.3f} s, {r['nsteps']} steps")
+    del S
+    return out
+
+
 def small_system_gmres(ctx, torch):
     """GMRES(30) to 1e-8 on the reference's own example (unit sphere N=8192,
     leaf 32, eta 0.8, ACA eps 1e-6, Laplace with quadrature weight 4 pi/N,
@@ -745,6 +797,8 @@ def main():
         gm["slower_system"] = slow
         del Hs, Hs32, bs
     gm["small_system"] = small_system_gmres(ctx, torch)
+    sym = symmetric_path(ctx, torch, pts, bmin, eta, weight, eps, tol, H, x, ms, args.steps)
+    ctx.release_cached()
     # what the products alone allow: the FP64 solve's products against the
     # refinement's FP64 residual products plus its FP32 inner products
     n64 = gm["fp64"]["matvecs"]
@@ -835,6 +889,7 @@ def main():
                      "aca_retries": ast["n_retries"], "mean_aca_rank": ast["sum_aca_rank"] / max(1, ast["n_candidates"]),
                      "mean_rank": ast["sum_rank"] / max(1, ast["n_lowrank"]), "work": asm_work, "acar": acar},
         "gmres": gm,
+        "symmetric": sym,
     }
     print(json.dumps(line), flush=True)
 
