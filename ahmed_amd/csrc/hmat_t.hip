@@ -39,17 +39,16 @@ struct ColTask {
 };
 
 // slots[zoff + c] = sum_r P[r][c] v[base + r].  Lanes form groups of GS
-// (32 for m > 128, 16 for m > 8, else the power of two >= m); a group takes
-// four columns at a time and its lanes hold rows gl, gl + GS, ... (at most 8
-// passes, m <= 256); the group sums
+// (32 for m > 128, 16 for m > 64, 8 for m > 4, else the power of two >= m);
+// a group takes four columns at a time and its lanes hold rows gl, gl + GS,
+// ... (at most 8 passes, m <= 256); the group sums
 // are fixed butterflies of shuffles.  Out-of-range rows and columns read a
 // clamped, valid address (row 0 / the group's first column) and are weighted
 // by zero or not stored, so every load is unconditional and a group's loads
 // are in flight together.
 template <typename TS, typename TV, int GS, int NQ>
 __device__ __forceinline__ void coldot_body(const ColTask& tk, const TS* __restrict__ arena,
-                                            const TV* __restrict__ v, TV* __restrict__ slots,
-                                            TV* __restrict__ out) {
+                                            const TV* __restrict__ v, TV* __restrict__ slots) {
   constexpr int kPerWave = 64 / GS;
   constexpr int kGroups = 4 * kPerWave;
   const int m = (int)tk.m;
@@ -67,8 +66,8 @@ __device__ __forceinline__ void coldot_body(const ColTask& tk, const TS* __restr
     vr[q] = in ? v[tk.base + r] : (TV)0;
   }
   const TS* P = arena + tk.data;
-  // the workgroup takes 4 * kGroups consecutive columns per pass; their sums
-  // meet in LDS (out, 4 * kGroups <= 1024 values) and leave in coalesced stores
+  // the workgroup takes 4 * kGroups consecutive columns per pass; each
+  // group's first lane stores its four sums (no workgroup barrier)
   for (uint32_t base = 0; base < tk.W; base += 4u * kGroups) {
     TV s[4];
 #pragma unroll
@@ -86,11 +85,10 @@ __device__ __forceinline__ void coldot_body(const ColTask& tk, const TS* __restr
       for (int u = 0; u < 4; ++u) s[u] += __shfl_xor(s[u], o, 64);
     if (gl == 0)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) out[u * kGroups + grp] = s[u];
-    __syncthreads();
-    const uint32_t n = min(4u * kGroups, tk.W - base);
-    for (uint32_t t = threadIdx.x; t < n; t += 256) slots[tk.zoff + base + t] = out[t];
-    __syncthreads();
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t c = base + (uint32_t)(u * kGroups + grp);
+        if (c < tk.W) slots[tk.zoff + c] = s[u];
+      }
   }
 }
 
@@ -99,22 +97,21 @@ __global__ __launch_bounds__(256) void coldot_kernel(const ColTask* __restrict__
                                                      const TS* __restrict__ arena,
                                                      const TV* __restrict__ v,
                                                      TV* __restrict__ slots, Gate g) {
-  __shared__ TV out[1024];
   if (gated(g)) return;
   const ColTask tk = tasks[blockIdx.x];
   const uint32_t m = tk.m;
   if (m > 128)
-    coldot_body<TS, TV, 32, 8>(tk, arena, v, slots, out);
-  else if (m > 8)
-    coldot_body<TS, TV, 16, 8>(tk, arena, v, slots, out);
+    coldot_body<TS, TV, 32, 8>(tk, arena, v, slots);
+  else if (m > 64)
+    coldot_body<TS, TV, 16, 8>(tk, arena, v, slots);
   else if (m > 4)
-    coldot_body<TS, TV, 8, 1>(tk, arena, v, slots, out);
+    coldot_body<TS, TV, 8, 8>(tk, arena, v, slots);
   else if (m > 2)
-    coldot_body<TS, TV, 4, 1>(tk, arena, v, slots, out);
+    coldot_body<TS, TV, 4, 1>(tk, arena, v, slots);
   else if (m > 1)
-    coldot_body<TS, TV, 2, 1>(tk, arena, v, slots, out);
+    coldot_body<TS, TV, 2, 1>(tk, arena, v, slots);
   else
-    coldot_body<TS, TV, 1, 1>(tk, arena, v, slots, out);
+    coldot_body<TS, TV, 1, 1>(tk, arena, v, slots);
 }
 
 // Symmetric matrices, one right-hand side: a phase-B task (row panel chunk
