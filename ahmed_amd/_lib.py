@@ -610,11 +610,13 @@ class HMatrix:
         self.ctx.bind_stream()
         check(load().ahm_hmat_autotune(self.h, p, vec, reps, C.byref(cv), C.byref(ms), C.byref(fu)),
               "autotune")
-        return cv.value, ms.value, bool(fu.value)
+        return cv.value, ms.value, int(fu.value)
 
-    def set_fused(self, on: bool = True):
-        """Products of 1-8 right-hand sides in one launch (each row panel
-        forms the V^T x of its own low-rank leaves)."""
+    def set_fused(self, on: int = 1):
+        """Launch form of products of 1-8 right-hand sides: 0 three kernels,
+        1 (True) one launch whose row panels form the V^T x of their own
+        low-rank leaves, 2 one persistent cooperative launch of the
+        three-kernel tasks with grid barriers between the phases."""
         check(load().ahm_hmat_set_fused(self.h, int(on)), "ahm_hmat_set_fused")
 
     def fused_status(self) -> bool:

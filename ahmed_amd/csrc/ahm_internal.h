@@ -284,7 +284,8 @@ struct ahm_hmat_s {
   // single-launch products of 1..8 right-hand sides (ahm_hmat_set_fused):
   // row panels in launch order, V^T panels, U-column groups and their
   // offsets per row panel
-  int fused = 0, fz_built = 0;
+  int fused = 0, fz_built = 0;   // launch form: 0 three kernels, 1 row-panel single launch, 2 persistent
+  int pgrid[4] = {0, 0, 0, 0};    // resident workgroups of the persistent kernel (f64/f32 x 1/several RHS)
   ahm::DeviceBuf fz_panels, fz_apanels, fz_groups, fz_goff;
   uint32_t fz_n = 0;
   int sym = 0;                // symmetric storage (AHM_LEAF_HEM diagonal leaves)

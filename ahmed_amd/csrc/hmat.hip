@@ -25,6 +25,8 @@
 #include <thread>
 #include <type_traits>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "ahm_internal.h"
 #include "ahm_aca.h"
 
@@ -741,12 +743,11 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
 // X (n x p, right-hand sides ldx apart) -> Xi: groups of 32 right-hand sides,
 // row i of group g holding its pw values contiguously at (g*n + i)*pw
 // (zero beyond p).  64 rows x 32 right-hand sides per block through LDS.
-template <typename T>
-__global__ __launch_bounds__(256) void interleave_kernel(const T* __restrict__ X, uint32_t ldx,
-                                                         uint32_t n, uint32_t p, int pw,
-                                                         T* __restrict__ Xi) {
-  __shared__ T tile[32][65];
-  const uint32_t i0 = blockIdx.x * 64u, g = blockIdx.y;
+template <typename T, bool COH = false>
+__device__ __forceinline__ void interleave_tile(const T* __restrict__ X, uint32_t ldx, uint32_t n, uint32_t p,
+                                                int pw, T* __restrict__ Xi, uint32_t bx, uint32_t g,
+                                                T (&tile)[32][65]) {
+  const uint32_t i0 = bx * 64u;
   for (int e = threadIdx.x; e < 32 * 64; e += 256) {
     const int jj = e >> 6, ii = e & 63;
     const uint32_t rhs = g * 32u + jj, i = i0 + ii;
@@ -757,21 +758,26 @@ __global__ __launch_bounds__(256) void interleave_kernel(const T* __restrict__ X
   for (int e = threadIdx.x; e < (64 << lpw); e += 256) {
     const int ii = e >> lpw, jj = e & (pw - 1);
     const uint32_t i = i0 + ii;
-    if (i < n) Xi[((uint64_t)g * n + i) * pw + jj] = tile[jj][ii];
+    if (i < n) st_x<COH>(Xi + ((uint64_t)g * n + i) * pw + jj, tile[jj][ii]);
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void interleave_kernel(const T* __restrict__ X, uint32_t ldx,
+                                                         uint32_t n, uint32_t p, int pw,
+                                                         T* __restrict__ Xi) {
+  __shared__ T tile[32][65];
+  interleave_tile<T>(X, ldx, n, p, pw, Xi, blockIdx.x, blockIdx.y, tile);
 }
 
 // Sum over chunks (chunk order fixed) for chunked panels: phase B adds
 // alpha * sum to y rows, phase A (ASSIGN) stores the sum into t.
 // pw > 0: y is interleaved (element (row, rhs) at ((rhs/32)*ldy + row)*pw +
 // rhs%32, the layout of t in multi-RHS products), else y[rhs*ldy + row].
-template <typename TV, bool ASSIGN>
-__global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
-    const ReduceTask* __restrict__ tasks, const TV* __restrict__ part, uint32_t part_len,
-    TV* __restrict__ y, uint32_t ldy, TV alpha, int pw, Gate g) {
-  if (gated(g)) return;
-  const ReduceTask r = tasks[blockIdx.x];
-  const uint32_t rhs = blockIdx.y;
+template <typename TV, bool ASSIGN, bool COH = false>
+__device__ __forceinline__ void reduce_task(const ReduceTask& r, uint32_t rhs, const TV* __restrict__ part,
+                                            uint32_t part_len, TV* __restrict__ y, uint32_t ldy, TV alpha,
+                                            int pw) {
   const TV* pr = part + (uint64_t)rhs * part_len + r.part;
   const uint64_t base = pw ? (uint64_t)(rhs >> 5) * ldy * pw + (rhs & 31) : (uint64_t)rhs * ldy;
   const uint64_t stride = pw ? (uint64_t)pw : 1;
@@ -781,17 +787,101 @@ __global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
     for (; c + 8 <= r.nchunks; c += 8) {   // eight loads in flight, summed in order
       TV v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = pr[(uint64_t)(c + u) * r.m + i];
+      for (int u = 0; u < 8; ++u) v[u] = ld_x<COH>(pr + (uint64_t)(c + u) * r.m + i);
 #pragma unroll
       for (int u = 0; u < 8; ++u) s += v[u];
     }
-    for (; c < r.nchunks; ++c) s += pr[(uint64_t)c * r.m + i];
+    for (; c < r.nchunks; ++c) s += ld_x<COH>(pr + (uint64_t)c * r.m + i);
     TV* o = y + base + (r.row + i) * stride;
     if (ASSIGN)
-      *o = s;
+      st_x<COH>(o, s);
     else
       *o += alpha * s;
   }
+}
+
+template <typename TV, bool ASSIGN>
+__global__ __launch_bounds__(kPanelThreads) void chunk_reduce_kernel(
+    const ReduceTask* __restrict__ tasks, const TV* __restrict__ part, uint32_t part_len,
+    TV* __restrict__ y, uint32_t ldy, TV alpha, int pw, Gate g) {
+  if (gated(g)) return;
+  reduce_task<TV, ASSIGN>(tasks[blockIdx.x], blockIdx.y, part, part_len, y, ldy, alpha, pw);
+}
+
+// ------------------------------------------------------------------------
+// One persistent launch per product of 1 <= p <= 8 right-hand sides
+// (ahm_hmat_set_fused(h, 2)): the grid holds as many workgroups as the
+// device keeps resident (a cooperative launch, which the runtime refuses
+// otherwise) and walks the same task lists as the separate launches, phase
+// after phase, with a grid barrier between them: [interleave X,] V^T x, its
+// chunk sums, the row panels, their chunk sums.  Tasks go round-robin over
+// the workgroups in list order (largest first).  Values one workgroup writes
+// and another reads (t, partial sums, interleaved X) are stored and loaded
+// with agent-coherent accesses across the grid barriers.  The arithmetic and
+// its order are those of the separate launches, so the results are the same
+// bit for bit.  A gated
+// product (single system, frozen) returns at once in every workgroup.
+template <typename TS, typename TV>
+struct PersistArgs {
+  const GemvTask* tasks_a;
+  const GemvTask* tasks_b;
+  const ReduceTask* tasks_ra;
+  const ReduceTask* tasks_r;
+  const uint32_t* zsrc;
+  const TS* arena;
+  const TV* X;
+  TV* Y;
+  TV* t;
+  TV* part;
+  TV* xi;
+  uint32_t ldx, ldy, p, n_cols, t_len, part_len;
+  uint32_t n_a, n_b, n_ra, n_r;
+  TV alpha;
+  const uint32_t* frozen;
+};
+
+// the grid barrier between phases
+__device__ __forceinline__ void phase_barrier(cooperative_groups::grid_group& grid) {
+  grid.sync();
+}
+
+template <typename TS, typename TV, bool MULTI>
+__global__ __launch_bounds__(kPanelThreads) void persistent_kernel(PersistArgs<TS, TV> a) {
+  // gemv: 2048 staged z + 256 row sums; gemvp: 2048; interleave: 32 x 65
+  __shared__ TV lds[2304];
+  if (a.frozen && a.frozen[0]) return;   // uniform over the grid
+  auto grid = cooperative_groups::this_grid();
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  if (MULTI) {
+    const uint32_t nt = (a.n_cols + 63) / 64;
+    for (uint32_t q = b; q < nt; q += G)
+      interleave_tile<TV, true>(a.X, a.ldx, a.n_cols, a.p, 8, a.xi, q, 0, *(TV(*)[32][65])lds);
+    phase_barrier(grid);
+  }
+  for (uint32_t q = b; q < a.n_a; q += G) {
+    if (MULTI)
+      gemvp_task<TS, TV, false, 8, true>(a.tasks_a[q], a.zsrc, a.arena, a.xi, a.t, a.t_len, a.Y, a.ldy, a.part,
+                                          a.part_len, a.alpha, a.p, lds);
+    else
+      gemv_task<TS, TV, false, true>(a.tasks_a[q], 0, a.zsrc, a.arena, a.X, a.ldx, a.t, a.t_len, a.Y, a.ldy,
+                                      a.part, a.part_len, a.alpha, lds, lds + kZChunk);
+  }
+  phase_barrier(grid);
+  for (uint32_t q = b; q < a.n_ra * a.p; q += G)
+    reduce_task<TV, true, true>(a.tasks_ra[q % a.n_ra], q / a.n_ra, a.part, a.part_len, a.t, a.t_len, a.alpha,
+                                MULTI ? 8 : 0);
+  phase_barrier(grid);
+  for (uint32_t q = b; q < a.n_b; q += G) {
+    if (MULTI)
+      gemvp_task<TS, TV, true, 8, true>(a.tasks_b[q], a.zsrc, a.arena, a.xi, a.t, a.t_len, a.Y, a.ldy, a.part,
+                                         a.part_len, a.alpha, a.p, lds);
+    else
+      gemv_task<TS, TV, true, true>(a.tasks_b[q], 0, a.zsrc, a.arena, a.X, a.ldx, a.t, a.t_len, a.Y, a.ldy,
+                                     a.part, a.part_len, a.alpha, lds, lds + kZChunk);
+  }
+  phase_barrier(grid);
+  for (uint32_t q = b; q < a.n_r * a.p; q += G)
+    reduce_task<TV, false, true>(a.tasks_r[q % a.n_r], q / a.n_r, a.part, a.part_len, a.Y, a.ldy, a.alpha, 0);
 }
 
 // ------------------------------------------------------------------------
@@ -1319,6 +1409,36 @@ struct Workspace {
 
 static Workspace own_workspace(ahm_hmat h) { return Workspace{h->tbuf.p, h->pbuf.p, h->xbuf.p, h->xbbuf.p}; }
 
+// one persistent cooperative launch (form 2 of ahm_hmat_set_fused); the grid
+// is the device's resident capacity for the kernel, at most the largest
+// phase's task count
+template <typename TS, typename TV>
+static int launch_persistent(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X, uint32_t ldx, TV* Y,
+                             uint32_t ldy, uint32_t p) {
+  const bool multi = p > 1;
+  const void* kern = multi ? reinterpret_cast<const void*>(&persistent_kernel<TS, TV, true>)
+                           : reinterpret_cast<const void*>(&persistent_kernel<TS, TV, false>);
+  int& cap = h->pgrid[(sizeof(TV) == 8 ? 0 : 2) + (multi ? 1 : 0)];
+  if (!cap) {
+    int nb = 0, ncu = 0;
+    AHM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kPanelThreads, 0));
+    AHM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->ctx->device));
+    if (nb <= 0 || ncu <= 0) return fail(AHM_ERR_UNSUPPORTED, "persistent product: no resident workgroups");
+    cap = nb * ncu;
+  }
+  uint64_t work = std::max<uint64_t>({h->n_a, h->n_b, (uint64_t)h->n_ra * p, (uint64_t)h->n_r * p,
+                                      multi ? (h->n_cols + 63) / 64 : 0, 1});
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)cap, work);
+  PersistArgs<TS, TV> a{(const GemvTask*)h->tasks_a.p, (const GemvTask*)h->tasks_b.p,
+                        (const ReduceTask*)h->tasks_ra.p, (const ReduceTask*)h->tasks_r.p,
+                        (const uint32_t*)h->zsrc.p, (const TS*)h->arena.p, X, Y, (TV*)ws.t, (TV*)ws.part,
+                        (TV*)ws.xi, ldx, ldy, p, h->n_cols, h->t_len, h->part_len,
+                        h->n_a, h->n_b, h->n_ra, h->n_r, alpha, ws.gate[0].frozen};
+  void* args[] = {&a};
+  AHM_HIP(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(kPanelThreads), args, 0, h->ctx->stream));
+  return AHM_OK;
+}
+
 template <typename TS, typename TV>
 static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X, uint32_t ldx, TV* Y,
                          uint32_t ldy, uint32_t p, hipEvent_t ev_mid = nullptr) {
@@ -1329,7 +1449,14 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
   // symmetric storage, one right-hand side: the transposed part rides on the
   // row-panel pass (after sym_prepare); wider products add it afterwards
   const bool sym1 = p == 1 && h->sym && h->tr.sym_built;
-  if (h->fused && p <= 8 && !ws.gate[0].frozen && !h->sym) {
+  if (h->fused == 2 && p <= 8 && !ws.gate[0].seg && !h->sym && !ev_mid) {
+    // a cooperative launch is not recorded into a graph: captured products
+    // take the separate launches
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    AHM_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs == hipStreamCaptureStatusNone) return launch_persistent<TS, TV>(h, ws, alpha, X, ldx, Y, ldy, p);
+  }
+  if (h->fused == 1 && p <= 8 && !ws.gate[0].frozen && !h->sym) {
     if (h->fz_n)
       hipLaunchKernelGGL((fused_kernel<TS, TV>), dim3(h->fz_n), dim3(kPanelThreads), 0, s,
                          (const GemvTask*)h->fz_panels.p, (const GemvTask*)h->fz_apanels.p,
@@ -1972,12 +2099,13 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
 
 int ahm_hmat_set_fused(ahm_hmat h, int on) {
   if (!h) return fail(AHM_ERR_INVALID, "ahm_hmat_set_fused: NULL handle");
+  if (on < 0 || on > 2) return fail(AHM_ERR_INVALID, "ahm_hmat_set_fused: form %d", on);
   if (on && h->sym) return fail(AHM_ERR_UNSUPPORTED, "ahm_hmat_set_fused: symmetric matrices");
-  if (on && !h->fz_built) {
+  if (on == 1 && !h->fz_built) {
     AHM_HIP(hipSetDevice(h->ctx->device));
     AHM_TRY(build_fused(h));
   }
-  h->fused = on ? 1 : 0;
+  h->fused = on;
   return AHM_OK;
 }
 
@@ -2012,11 +2140,11 @@ int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps, uint
   AHM_HIP(hipEventCreate(&e1));
   int st = AHM_OK;
   const int saved_fused = h->fused;
-  for (int fu = 0; fu <= (p <= 8 && !h->sym ? 1 : 0) && !st; ++fu) {
-    if (fu && !h->fz_built && (st = build_fused(h))) break;
+  for (int fu = 0; fu <= (p <= 8 && !h->sym ? 2 : 0) && !st; ++fu) {
+    if (fu == 1 && !h->fz_built && (st = build_fused(h))) break;
     h->fused = fu;
     for (uint64_t cv : cand) {
-      if (fu && cv != cand[0]) break;   // the single launch runs unchunked panels
+      if (fu == 1 && cv != cand[0]) break;   // the row-panel single launch runs unchunked panels
       if ((st = set_chunking(h, cv))) break;
       if ((st = ensure_tbuf(h, p, es)) || (st = ensure_pbuf(h, p, es))) break;
       auto run = [&]() {

@@ -310,16 +310,16 @@ def bench_batched(args):
     t0 = time.perf_counter()
     tile, ms_tuned, fused = H.autotune(p=p, reps=3)
     t_tune = time.perf_counter() - t0
-    log(f"autotuner ({t_tune:.1f} s): tile {tile} values, {'one launch' if fused else 'three launches'}, "
+    form_names = {0: "three_launches", 1: "row_panel_single_launch", 2: "persistent_single_launch"}
+    log(f"autotuner ({t_tune:.1f} s): tile {tile} values, {form_names[fused]}, "
         f"{ms_tuned:.3f} ms (default {ms_default:.3f} ms)")
-    # both forms at the tuned tile size, then the tuned form for the line
+    # every form at the tuned tile size, then the tuned form for the line
     forms = {}
-    for f in (False, True):
+    for f in (0, 1, 2):
         H.set_fused(f)
-        forms["single_launch_ms" if f else "three_launches_ms"] = round(
-            timed(lambda: H.matvec(X, Y), torch, args.steps, args.warmup), 4)
+        forms[form_names[f] + "_ms"] = round(timed(lambda: H.matvec(X, Y), torch, args.steps, args.warmup), 4)
     H.set_fused(fused)
-    log(f"8 RHS: three launches {forms['three_launches_ms']:.3f} ms, one launch {forms['single_launch_ms']:.3f} ms")
+    log("8 RHS: " + ", ".join(f"{k} {v:.3f}" for k, v in forms.items()))
     ms = timed(lambda: H.matvec(X, Y), torch, args.steps, args.warmup)
     B_alg = 8.0 * st["nvals"] + 8.0 * p * 3 * Ntot
     F_alg = 2.0 * p * st["nvals"]
@@ -361,7 +361,7 @@ def bench_batched(args):
                    "config": "cfg4", "N_total": Ntot, "sizes_min_max": [min(sizes), max(sizes)],
                    "nvals": st["nvals"]},
         "TF/s": round(F_alg / ms / 1e9, 3), "matvec_1rhs_ms": round(ms1, 4),
-        "autotune": {"tile_values": tile, "fused_single_launch": fused, "seconds": round(t_tune, 2),
+        "autotune": {"tile_values": tile, "launch_form": form_names[fused], "seconds": round(t_tune, 2),
                      "default_form_ms": round(ms_default, 4), **forms},
         "fused_wait_aborted": H.fused_status(),
         "assembly_s": round(t_asm, 3), "aca_retries": ast["n_retries"], "gmres_batch": res,

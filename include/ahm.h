@@ -284,20 +284,26 @@ int ahm_hmat_matvec_profile(ahm_hmat h, double alpha, const void* X, uint32_t ld
                             double* ms_phase_a, double* ms_phase_b);
 /* Autotuner: times `reps` products with p right-hand sides for each tile
  * size (column-chunk size of the row panels and V^T panels: 4K .. 192K
- * values, or none) and, for p <= 8, the single launch (ahm_hmat_set_fused),
- * and keeps the fastest.  fused (may be NULL) receives the launch form (0:
- * three kernels, 1: single launch).  Products agree to the rounding of the
- * chunk sums (the order of the summands changes). */
+ * values, or none) and, for p <= 8, the single-launch forms
+ * (ahm_hmat_set_fused), and keeps the fastest.  fused (may be NULL) receives
+ * the launch form (0: three kernels, 1: row-panel single launch, 2:
+ * persistent single launch).  Products agree to the rounding of the chunk
+ * sums (the order of the summands changes). */
 int ahm_hmat_autotune(ahm_hmat h, uint32_t p, ahm_dtype vec, uint32_t reps,
                       uint64_t* chunk_values, double* best_ms, int* fused);
-/* on != 0: products of 1 to 8 right-hand sides (ahm_hmat_matvec and the
- * calls built on it, except the solvers' gated products) in ONE launch: one
- * workgroup per row panel, which forms the V^T x values of its own low-rank
- * leaves (each leaf's t serves its own row panel only), so no workgroup
- * waits for another, and X / Y are used in place.  The first call builds the
- * plan (row panels, U-column groups).  on == 0: the three-kernel form.  No
- * counterpart in the reference (whose mltaGeHVec loops over the leaves,
- * H/mltaGeHVec.cpp:38-59). */
+/* Launch form of products of 1 to 8 right-hand sides (ahm_hmat_matvec and
+ * the calls built on it).  on == 1: ONE launch with one workgroup per row
+ * panel, which forms the V^T x values of its own low-rank leaves (each
+ * leaf's t serves its own row panel only), so no workgroup waits for
+ * another, and X / Y are used in place; the first call builds the plan (row
+ * panels, U-column groups); not for the solvers' gated products.  on == 2:
+ * ONE persistent cooperative launch of as many workgroups as stay resident,
+ * running the tasks of the three-kernel form phase after phase with grid
+ * barriers between them (same results bit for bit; the solvers' single-system
+ * products too; products recorded into a hipGraph keep the separate
+ * launches).  on == 0: the three-kernel form.  Symmetric matrices: form 0
+ * only.  No counterpart in the reference (whose mltaGeHVec loops over the
+ * leaves, H/mltaGeHVec.cpp:38-59). */
 int ahm_hmat_set_fused(ahm_hmat h, int on);
 /* aborted: always 0 (the single launch has no bounded waits to give up). */
 int ahm_hmat_fused_status(ahm_hmat h, int* aborted);
