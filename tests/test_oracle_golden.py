@@ -219,3 +219,40 @@ def test_hlu_fixture_matrix_is_the_oracles(name, oracle_api):
     for k in ("b1", "n1", "b2", "n2"):
         assert np.array_equal(lv[k], g[f"L_{k}"])
     assert int(g["gmres_rc"]) == 0 and int(g["gmres_nsteps"]) <= 5
+
+
+def test_all_core_baselines_match_serial(case):
+    """The CPU baselines bench.py times on all host cores: the oracle's
+    leaf-parallel product (or_hmat_matvec_omp, private y per thread, summed)
+    and, where the compiled reference exists, its own leaf products run the
+    same way and its GMRes on that operator, against the serial products and
+    solver (same values to rounding; GMRes steps within one, true residual
+    within twice the tolerance)."""
+    import oracle as O
+    g, T, H = case
+    o = O.Oracle()
+    x = g["x"]
+    y1 = H.matvec(x)
+    for nt in (1, 3, 8):
+        yo = o.matvec_omp(H, x, nthreads=nt)
+        assert np.max(np.abs(yo - y1)) <= 1e-13 * np.max(np.abs(y1)), nt
+    try:
+        R = O.Reference()
+    except FileNotFoundError:
+        return
+    from conftest import golden_points
+    pts = golden_points(g)
+    TR = R.tree(pts, int(g["bmin"]), float(g["eta2"]), int(g["kind"]))
+    L = H.layout()
+    HR = R.from_data(TR, L["rank"], L["offset"], L["data"])
+    yr = R.matvec_omp(HR, x, nthreads=4)
+    assert np.max(np.abs(yr - g["y"])) <= 1e-13 * np.max(np.abs(g["y"]))
+    tol = float(g["gmres_tol"])
+    xs, rc, e, n = R.gmres_omp(HR, g["b"], eps=tol, m=int(g["gmres_m"]), nsteps=int(g["gmres_nsteps_in"]),
+                               nthreads=4)
+    # the thread-summed products round differently: the iterates drift apart
+    # by about tol; both reach it in about as many steps
+    assert rc == int(g["gmres_rc"]) and abs(n - int(g["gmres_nsteps"])) <= 1
+    if rc == 0:
+        b = g["b"]
+        assert np.linalg.norm(b - HR.matvec(xs)) <= 2 * tol * np.linalg.norm(b)
