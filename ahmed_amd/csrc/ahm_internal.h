@@ -258,6 +258,10 @@ struct ahm_hmat_s {
   ahm::DeviceBuf arena;       // nvals values in panel layout
   ahm::DeviceBuf tasks_a, tasks_b, zsrc, tasks_r, tasks_ra;
   uint32_t n_a = 0, n_b = 0, n_r = 0, n_ra = 0;
+  // multi-RHS MFMA launches: their own chunking of the panels (tasks,
+  // reductions) and workgroup maps (task << 2 | row piece)
+  ahm::DeviceBuf mm_tasks_a, mm_tasks_b, mm_tasks_ra, mm_tasks_r, mm_map_a, mm_map_b;
+  uint32_t n_mm_a = 0, n_mm_b = 0, n_mm_ra = 0, n_mm_r = 0;
   uint32_t part_len = 0;      // partial-sum slots per right-hand side (shared by
                               // the phase-A and phase-B chunk sums, used in turn)
   ahm::DeviceBuf pbuf;        // partial sums: part_len * p values

@@ -495,6 +495,10 @@ __global__ __launch_bounds__(kPanelThreads) void fused_kernel(
 // reach output rows beyond m, which are never stored.
 constexpr int kMmChunk = 128;
 constexpr int kMmRhs = 32;
+// LDS output tile of a task piece (at most 8 row tiles): [kMmRhs][128 + 1]
+// over the Z chunk's kMmChunk x (kMmRhs + 1) values
+constexpr int kMmStageLd = 129;
+static_assert(kMmRhs * kMmStageLd <= kMmChunk * (kMmRhs + 1), "output tile exceeds the Z chunk's LDS");
 
 typedef double mm_f64x4 __attribute__((ext_vector_type(4)));
 typedef float mm_f32x4 __attribute__((ext_vector_type(4)));
@@ -621,9 +625,9 @@ __device__ __forceinline__ void mm_store(const GemvTask& tk, int rt, int ct, con
 template <typename T, bool PHASE_B, int NW, bool HALF, bool TWO>
 __device__ __forceinline__ void mm_task(const GemvTask& tk, const uint32_t* __restrict__ zsrc,
                                         const T* __restrict__ arena, const T* __restrict__ xg, T* __restrict__ tg,
-                                        T (&zs)[kMmChunk][kMmRhs + 1], int w, int hrt, int hct, uint32_t rhs0,
-                                        uint32_t p, T* __restrict__ y, uint32_t ldy, T* __restrict__ part,
-                                        uint32_t part_len, T alpha) {
+                                        T (&zs)[kMmChunk][kMmRhs + 1], int w, int hrt, int hct, int rt0,
+                                        int cnt, uint32_t rhs0, uint32_t p, T* __restrict__ y, uint32_t ldy,
+                                        T* __restrict__ part, uint32_t part_len, T alpha) {
   typedef typename MmAcc<T>::type acc_t;
   constexpr int NA = NW + (HALF ? 1 : 0) > 0 ? NW + (HALF ? 1 : 0) : 1;
   constexpr bool WORK = NW > 0 || HALF;
@@ -673,27 +677,76 @@ __device__ __forceinline__ void mm_task(const GemvTask& tk, const uint32_t* __re
     if (!PIPE) mm_load<T, NW, HALF, NA>(A0, P, r, m, lob, w, hrt);
     mm_mma<T, NW, HALF, TWO, true, NA>(A0, r, W, li, lk, hct, zs, acc, acch);
   }
+  if (!PHASE_B && !tk.partial) {
+    // t rows hold their 32 right-hand sides contiguously: direct stores
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      mm_store<T, PHASE_B>(tk, w + 4 * q, 0, acc[q][0], lane, rhs0, p, tg, y, ldy, part, part_len, alpha);
+      if (TWO) mm_store<T, PHASE_B>(tk, w + 4 * q, 1, acc[q][1], lane, rhs0, p, tg, y, ldy, part, part_len, alpha);
+    }
+    if (HALF) mm_store<T, PHASE_B>(tk, hrt, hct, acch, lane, rhs0, p, tg, y, ldy, part, part_len, alpha);
+    return;
+  }
+  // y and the partial sums are right-hand-side major: the piece's output
+  // goes through LDS (ys[rhs][row], over the Z chunk's space) so that each
+  // wavefront store covers 64 consecutive rows of one right-hand side
+  T (*ys)[kMmStageLd] = reinterpret_cast<T (*)[kMmStageLd]>(&zs[0][0]);
+  auto put = [&](int rt, int ct, const acc_t& c) {
+    const int jj = ct * 16 + li;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ys[jj][(rt - rt0) * 16 + MmAcc<T>::row(lane, r)] = c[r];
+  };
+  __syncthreads();   // every wavefront is done with the Z chunk
 #pragma unroll
   for (int q = 0; q < NW; ++q) {
-    mm_store<T, PHASE_B>(tk, w + 4 * q, 0, acc[q][0], lane, rhs0, p, tg, y, ldy, part, part_len, alpha);
-    if (TWO) mm_store<T, PHASE_B>(tk, w + 4 * q, 1, acc[q][1], lane, rhs0, p, tg, y, ldy, part, part_len, alpha);
+    put(w + 4 * q, 0, acc[q][0]);
+    if (TWO) put(w + 4 * q, 1, acc[q][1]);
   }
-  if (HALF) mm_store<T, PHASE_B>(tk, hrt, hct, acch, lane, rhs0, p, tg, y, ldy, part, part_len, alpha);
+  if (HALF) put(hrt, hct, acch);
+  __syncthreads();
+  const uint32_t rbase = (uint32_t)rt0 * 16u;
+  const uint32_t rows = min((uint32_t)cnt * 16u, m - rbase);
+  for (int jj = (int)(threadIdx.x >> 6); jj < kMmRhs; jj += kPanelThreads / 64) {
+    const uint32_t j = rhs0 + jj;
+    if (j >= p) break;
+    for (uint32_t i = (uint32_t)lane; i < rows; i += 64) {
+      const T v = ys[jj][i];
+      if (tk.partial)
+        part[(uint64_t)j * part_len + tk.out + rbase + i] = v;
+      else
+        y[(uint64_t)j * ldy + tk.out + rbase + i] += alpha * v;
+    }
+  }
+}
+
+// Workgroup map of the MFMA launches: entry (task << 2) | piece, piece 0 the
+// whole task, 1 / 2 the first ceil(nrt/2) / last floor(nrt/2) row tiles of a
+// task cut in two (tasks of more than kMmSplitTiles row tiles), so that a
+// wavefront holds at most two whole row tiles and its panel loads run one
+// round ahead; both halves gather the same Z chunks.
+constexpr uint32_t kMmSplitTiles = 8;
+
+__device__ __forceinline__ void mm_piece(uint32_t e, int nrt, int& rt0, int& cnt) {
+  const uint32_t piece = e & 3u;
+  rt0 = piece == 2 ? (nrt + 1) / 2 : 0;
+  cnt = piece == 0 ? nrt : piece == 1 ? (nrt + 1) / 2 : nrt / 2;
 }
 
 template <typename T, bool PHASE_B>
-__global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
-    const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ zsrc,
+__global__ __launch_bounds__(kPanelThreads) __attribute__((amdgpu_waves_per_eu(4))) void panel_mm_kernel(
+    const GemvTask* __restrict__ tasks, const uint32_t* __restrict__ map, const uint32_t* __restrict__ zsrc,
     const T* __restrict__ arena, const T* __restrict__ xi, uint32_t n_x, T* __restrict__ ti,
     uint32_t t_len, T* __restrict__ y, uint32_t ldy, T* __restrict__ part,
     uint32_t part_len, T alpha, uint32_t p) {
   __shared__ T zs[kMmChunk][kMmRhs + 1];
-  const GemvTask tk = tasks[blockIdx.x];
+  const uint32_t e = map[blockIdx.x];
+  const GemvTask tk = tasks[e >> 2];
   const uint32_t g = blockIdx.y, rhs0 = g * kMmRhs;
   const T* xg = xi + (uint64_t)g * n_x * kMmRhs;
   T* tg = ti + (uint64_t)g * t_len * kMmRhs;
+  int rt0, nrt;
+  mm_piece(e, (int)(tk.m + 15) / 16, rt0, nrt);
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int nrt = (int)(tk.m + 15) / 16;
   const bool two = p - rhs0 > 16;   // second column tile needed
   // this wavefront's units: nwhole row tiles w + 4q with both column tiles,
   // and at most one row tile hrt with column tile hct alone
@@ -715,9 +768,10 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
     nwhole = (nrt - w + 3) >> 2;   // row tiles w, w+4, ... below nrt
   }
 #define AHM_MM_TASK(NW, HALF, TWO)                                                                    \
-  mm_task<T, PHASE_B, NW, HALF, TWO>(tk, zsrc, arena, xg, tg, zs, w, hrt, hct, rhs0, p, y, ldy, part, \
-                                     part_len, alpha)
+  mm_task<T, PHASE_B, NW, HALF, TWO>(tk, zsrc, arena, xg, tg, zs, rt0 + w, hrt, hct, rt0, nrt, rhs0, p, y, \
+                                     ldy, part, part_len, alpha)
   const int shape = nwhole * 4 + (hrt >= 0 ? 2 : 0) + (two ? 1 : 0);
+  if (hrt >= 0) hrt += rt0;   // row tiles of the task, not of the piece
   switch (shape) {
     case 0: case 1: AHM_MM_TASK(0, false, false); break;
     case 2: AHM_MM_TASK(0, true, false); break;
@@ -729,13 +783,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_mm_kernel(
     case 8: AHM_MM_TASK(2, false, false); break;
     case 9: AHM_MM_TASK(2, false, true); break;
     case 10: AHM_MM_TASK(2, true, false); break;
-    case 11: AHM_MM_TASK(2, true, true); break;
-    case 12: AHM_MM_TASK(3, false, false); break;
-    case 13: AHM_MM_TASK(3, false, true); break;
-    case 14: AHM_MM_TASK(3, true, false); break;
-    case 15: AHM_MM_TASK(3, true, true); break;
-    case 16: AHM_MM_TASK(4, false, false); break;
-    default: AHM_MM_TASK(4, false, true); break;
+    default: AHM_MM_TASK(2, true, true); break;
   }
 #undef AHM_MM_TASK
 }
@@ -1346,6 +1394,62 @@ static int upload_vec(DeviceBuf& b, const std::vector<T, A>& v, hipStream_t s) {
   return AHM_OK;
 }
 
+// The MFMA launches (multi-RHS products of 9 or more right-hand sides) run
+// their own chunking of the panels: a task there carries 32 times the
+// arithmetic and output per stored value of a single-RHS task, so row panels
+// run whole (kMmChunkB is never reached) and V^T panels in chunks of about
+// kMmChunkA values, which cuts the partial sums written and reduced.
+// Each task of more than kMmSplitTiles row tiles runs as two workgroups
+// (mm_piece); the map lists the workgroups as (task << 2) | piece.
+constexpr uint64_t kMmChunkA = 128 * 1024;
+constexpr uint64_t kMmChunkB = 1ull << 40;
+
+static void mm_map(const std::vector<GemvTask>& T, std::vector<uint32_t>& map) {
+  map.clear();
+  map.reserve(T.size() * 2);
+  for (size_t i = 0; i < T.size(); ++i) {
+    const uint32_t e = (uint32_t)i << 2;
+    if ((T[i].m + 15) / 16 > kMmSplitTiles) {
+      map.push_back(e | 1u);
+      map.push_back(e | 2u);
+    } else {
+      map.push_back(e);
+    }
+  }
+}
+
+// the MFMA task lists of h's panels; widens part_len to their partial sums
+static int build_mm_tasks(ahm_hmat h, hipStream_t s) {
+  std::vector<GemvTask> a, b;
+  std::vector<ReduceTask> ra, r;
+  uint32_t part_a = 0, part_b = 0;
+  chunk_panels(h->h_panels_a, kMmChunkA, a, ra, part_a);
+  chunk_panels(h->h_panels, kMmChunkB, b, r, part_b);
+  if (a.size() >= (1u << 30) || b.size() >= (1u << 30))
+    return fail(AHM_ERR_UNSUPPORTED, "too many panel tasks for the MFMA workgroup map");
+  std::vector<uint32_t> ma, mb;
+  mm_map(a, ma);
+  mm_map(b, mb);
+  AHM_TRY(upload_vec(h->mm_tasks_a, a, s));
+  AHM_TRY(upload_vec(h->mm_tasks_b, b, s));
+  AHM_TRY(upload_vec(h->mm_tasks_ra, ra, s));
+  AHM_TRY(upload_vec(h->mm_tasks_r, r, s));
+  AHM_TRY(upload_vec(h->mm_map_a, ma, s));
+  AHM_TRY(upload_vec(h->mm_map_b, mb, s));
+  AHM_HIP(hipStreamSynchronize(s));
+  h->n_mm_a = (uint32_t)ma.size();
+  h->n_mm_b = (uint32_t)mb.size();
+  h->n_mm_ra = (uint32_t)ra.size();
+  h->n_mm_r = (uint32_t)r.size();
+  const uint32_t pl = std::max(part_a, part_b);
+  if (pl > h->part_len) {
+    h->part_len = pl;
+    h->pbuf_cap = 0;
+    h->pbuf.release();
+  }
+  return AHM_OK;
+}
+
 // multi-RHS products keep x and t interleaved in groups of 32 right-hand
 // sides (8 when p <= 8): rhs_width values per row and group
 static uint32_t rhs_width(uint32_t p) { return p <= 8 ? 8u : (uint32_t)kMmRhs; }
@@ -1449,6 +1553,9 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
   // symmetric storage, one right-hand side: the transposed part rides on the
   // row-panel pass (after sym_prepare); wider products add it afterwards
   const bool sym1 = p == 1 && h->sym && h->tr.sym_built;
+  // the phase-B reduction: the MFMA launches have their own
+  const ReduceTask* reduce_b = (const ReduceTask*)h->tasks_r.p;
+  uint32_t n_reduce_b = h->n_r;
   if (h->fused == 2 && p <= 8 && !ws.gate[0].seg && !h->sym && !ev_mid) {
     // a cooperative launch is not recorded into a graph: captured products
     // take the separate launches
@@ -1526,26 +1633,33 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
                            (const uint16_t*)ws.xb, h->n_cols, (float*)t, h->t_len, (float*)part,
                            h->part_len, p);
       } else if (h->n_a) {
-        hipLaunchKernelGGL((panel_mm_kernel<TV, false>), dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,
-                           (const GemvTask*)h->tasks_a.p, (const uint32_t*)h->zsrc.p,
+        hipLaunchKernelGGL((panel_mm_kernel<TV, false>), dim3(h->n_mm_a, ng), dim3(kPanelThreads), 0, s,
+                           (const GemvTask*)h->mm_tasks_a.p, (const uint32_t*)h->mm_map_a.p,
+                           (const uint32_t*)h->zsrc.p,
                            (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, Y, ldy, part,
                            h->part_len, alpha, p);
       }
-      if (h->n_ra)
-        hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(h->n_ra, p), dim3(kPanelThreads), 0, s,
-                           (const ReduceTask*)h->tasks_ra.p, (const TV*)part, h->part_len, t, h->t_len, alpha,
-                           pw, Gate{});
+      // the BF16 V^T X tasks keep the single-RHS chunking (their copy is
+      // laid out per task)
+      const uint32_t nra = bf16_a ? h->n_ra : h->n_mm_ra;
+      if (nra)
+        hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(nra, p), dim3(kPanelThreads), 0, s,
+                           (const ReduceTask*)(bf16_a ? h->tasks_ra.p : h->mm_tasks_ra.p), (const TV*)part,
+                           h->part_len, t, h->t_len, alpha, pw, Gate{});
       if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
       if (h->n_b)
-        hipLaunchKernelGGL((panel_mm_kernel<TV, true>), dim3(h->n_b, ng), dim3(kPanelThreads), 0, s,
-                           (const GemvTask*)h->tasks_b.p, (const uint32_t*)h->zsrc.p,
+        hipLaunchKernelGGL((panel_mm_kernel<TV, true>), dim3(h->n_mm_b, ng), dim3(kPanelThreads), 0, s,
+                           (const GemvTask*)h->mm_tasks_b.p, (const uint32_t*)h->mm_map_b.p,
+                           (const uint32_t*)h->zsrc.p,
                            (const TS*)h->arena.p, xi, h->n_cols, t, h->t_len, Y, ldy, part,
                            h->part_len, alpha, p);
+      reduce_b = (const ReduceTask*)h->mm_tasks_r.p;
+      n_reduce_b = h->n_mm_r;
     }
   }
-  if (h->n_r)
-    hipLaunchKernelGGL((chunk_reduce_kernel<TV, false>), dim3(h->n_r, p), dim3(kPanelThreads), 0, s,
-                       (const ReduceTask*)h->tasks_r.p, (const TV*)part, h->part_len, Y, ldy, alpha, 0,
+  if (n_reduce_b)
+    hipLaunchKernelGGL((chunk_reduce_kernel<TV, false>), dim3(n_reduce_b, p), dim3(kPanelThreads), 0, s,
+                       reduce_b, (const TV*)part, h->part_len, Y, ldy, alpha, 0,
                        p == 1 ? ws.gate[3] : Gate{});
   AHM_HIP(hipGetLastError());
   if (sym1) AHM_TRY(sym_rest(h, (double)alpha, Y, vec, ws.gate[3]));
@@ -1758,6 +1872,7 @@ int set_chunking(ahm_hmat h, uint64_t cv) {
   h->h_tasks_ra = std::move(ra);
   h->h_tasks_b = std::move(b);
   h->h_tasks_r = std::move(r);
+  AHM_TRY(build_mm_tasks(h, s));
   h->chunk_values = cv;
   h->tr.sym_built = false;   // the symmetric pass follows the phase-B tasks
   if (h->storage == AHM_BF16) AHM_TRY(build_bf16(h));
@@ -1843,6 +1958,7 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
     h->h_panels_a = std::move(P.a_panels);
     h->chunk_values = kChunkValues;
     h->h_zsrc = std::move(P.zsrc);
+    if ((st = build_mm_tasks(h, s))) break;
     if (keep_leaves) {
       h->leafdata.p = raw.p;
       h->leafdata.bytes = raw.bytes;
@@ -1947,6 +2063,7 @@ int ahm_hmat_convert(ahm_hmat src, ahm_dtype storage, ahm_hmat* out) {
     h->n_r = src->n_r;
     h->n_ra = src->n_ra;
     h->n_zsrc = src->n_zsrc;
+    if ((st = build_mm_tasks(h, s))) break;
     if (h->nvals)
       hipLaunchKernelGGL((cast_kernel<float, double>), dim3(4096), dim3(256), 0, s,
                          (const double*)src->arena.p, (float*)h->arena.p, h->nvals);

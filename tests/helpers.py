@@ -48,11 +48,11 @@ def relerr(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-def block_matrix(sizes, seed):
+def block_matrix(sizes, seed, rank_of=None):
     """Leaves of a block matrix on row/column segments of the given sizes:
-    dense diagonal blocks, low-rank off-diagonal blocks of ranks 1..6, with
-    the dense N x N matrix they represent (column-major leaf values as in
-    mblock: dense n1 x n2, low-rank U n1 x k then V n2 x k)."""
+    dense diagonal blocks, low-rank off-diagonal blocks of ranks 1..6 (or
+    rank_of(i, j)), with the dense N x N matrix they represent (column-major
+    leaf values as in mblock: dense n1 x n2, low-rank U n1 x k then V n2 x k)."""
     from ahmed_amd import _lib
     rng = np.random.default_rng(seed)
     b = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
@@ -66,7 +66,7 @@ def block_matrix(sizes, seed):
                 k, vals = -1, rng.standard_normal(mi * mj)
                 A[b[i]:b[i + 1], b[j]:b[j + 1]] = vals.reshape(mj, mi).T
             else:
-                k = int(1 + (i * 7 + j * 3) % 6)
+                k = int(1 + (i * 7 + j * 3) % 6) if rank_of is None else int(rank_of(i, j))
                 vals = rng.standard_normal(k * (mi + mj))
                 U = vals[:mi * k].reshape(k, mi).T
                 V = vals[mi * k:].reshape(k, mj).T

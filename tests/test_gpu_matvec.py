@@ -216,6 +216,36 @@ def test_mfma_product_every_panel_height(ctx, p):
         assert relerr(Y32.cpu().numpy().T.astype(np.float64), A @ X) <= 5e-6
 
 
+# V^T panels (the summed ranks of one column segment's low-rank leaves) of
+# every height from one past 8 row tiles to 256 rows, and one of 300 rows cut
+# into 256 + 44: the MFMA launch runs V^T panels of more than 8 row tiles as
+# two workgroups of half the row tiles each.
+VT_HEIGHTS = [129, 136, 144, 152, 160, 175, 192, 200, 224, 241, 256, 300]
+
+
+@pytest.mark.parametrize("p", [17, 32, 40])
+def test_mfma_product_tall_vt_panels(ctx, p):
+    from ahmed_amd import _lib
+    nseg = len(VT_HEIGHTS)
+
+    def rank_of(i, j):   # leaf (i, j), i != j: the column's 11 ranks sum to VT_HEIGHTS[j]
+        q, r = divmod(VT_HEIGHTS[j], nseg - 1)
+        return q + (1 if (i - (i > j)) < r else 0)
+
+    N, leaves, data, A = block_matrix([64] * nseg, 5, rank_of)
+    print(f"[gpu] MFMA tall V^T panels N={N} p={p}", flush=True)
+    D = _lib.HMatrix.upload(ctx, N, N, leaves, data)
+    rng = np.random.default_rng(p + 100)
+    X = rng.standard_normal((N, p))
+    Y = dev(np.zeros((p, N)))
+    D.matvec(dev(X.T.copy()), Y, 0.75)
+    assert relerr(Y.cpu().numpy().T, 0.75 * (A @ X)) <= 1e-12
+    D32 = D.convert()
+    Y32 = dev(np.zeros((p, N)), torch.float32)
+    D32.matvec(dev(X.T.copy(), torch.float32), Y32, 1.0)
+    assert relerr(Y32.cpu().numpy().T.astype(np.float64), A @ X) <= 5e-6
+
+
 def cache_test_matrix(N, rng, poison):
     """Leaves [0, h) x [0, N) and [h, N) x [0, h) dense, [h, N) x [h, N) of rank
     6 (so products use the V^T x workspace as well as the arena); poison:
