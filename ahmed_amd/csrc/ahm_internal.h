@@ -177,11 +177,12 @@ int matvec_gated(ahm_hmat h, double alpha, const void* x, void* y, ahm_dtype vec
 void matvec_signature(ahm_hmat h, std::vector<uint64_t>& sig);
 
 // The transposed part of symmetric products (hmat_t.hip): sym_prepare builds
-// its column index and sizes its buffers for vectors of type vec (allocates;
-// call before a graph capture); sym_transposed adds alpha S^T X for the
-// leaves off the diagonal, column by column, and allocates nothing.  Both do
-// nothing for unsymmetric matrices.
-int sym_prepare(ahm_hmat h, ahm_dtype vec);
+// its column index and sizes its buffers for vectors of type vec and p
+// right-hand sides (allocates; call before a graph capture); sym_transposed
+// adds alpha S^T X for the leaves off the diagonal (32 right-hand sides at a
+// time when p >= 2) and allocates nothing.  Both do nothing for unsymmetric
+// matrices.
+int sym_prepare(ahm_hmat h, ahm_dtype vec, uint32_t p = 1);
 int sym_transposed(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* Y, uint32_t ldy, uint32_t p,
                    ahm_dtype vec, Gate g);
 // Single right-hand side, symmetric: the phase-B launch that reads each row
@@ -313,6 +314,8 @@ struct ahm_hmat_s {
     uint32_t n_sym = 0;
     ahm::DeviceBuf contrib, tprime;   // one value per panel column; t' = U^T x
     size_t contrib_cap = 0, tprime_cap = 0;
+    ahm::DeviceBuf xi;                // several right-hand sides: X rows, 32 values each
+    size_t xi_cap = 0;
   } tr;
   ahm::GmresCache* gm = nullptr;   // GMRES buffers and captured steps (gmres.hip)
   int solver_graphs = 0;           // GMRES steps replayed as hipGraphs (ahm_hmat_set_graphs)

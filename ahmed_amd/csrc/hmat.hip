@@ -2152,7 +2152,7 @@ int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void*
   if (h->storage == AHM_F64 && vec == AHM_F64) {
     AHM_TRY(ensure_tbuf(h, p, 8));
     AHM_TRY(ensure_pbuf(h, p, 8));
-    AHM_TRY(sym_prepare(h, vec));
+    AHM_TRY(sym_prepare(h, vec, p));
     AHM_TRY((launch_matvec<double, double>(h, own_workspace(h), alpha, (const double*)X, ldx, (double*)Y,
                                            ldy, p)));
     return p > 1 ? sym_transposed(h, alpha, X, ldx, Y, ldy, p, vec, Gate{}) : AHM_OK;
@@ -2160,7 +2160,7 @@ int ahm_hmat_matvec(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void*
   if ((h->storage == AHM_F32 || h->storage == AHM_BF16) && vec == AHM_F32) {
     AHM_TRY(ensure_tbuf(h, p, 4));
     AHM_TRY(ensure_pbuf(h, p, 4));
-    AHM_TRY(sym_prepare(h, vec));
+    AHM_TRY(sym_prepare(h, vec, p));
     AHM_TRY((launch_matvec<float, float>(h, own_workspace(h), (float)alpha, (const float*)X, ldx, (float*)Y,
                                          ldy, p)));
     return p > 1 ? sym_transposed(h, alpha, X, ldx, Y, ldy, p, vec, Gate{}) : AHM_OK;

@@ -279,6 +279,150 @@ __global__ __launch_bounds__(256) void csr_sum_kernel(const uint32_t* __restrict
     out[i] += alpha * s;
 }
 
+// ---- several right-hand sides (p >= 2): 32 at a time on MFMA ----------
+//
+// The same four steps with 32-wide rows: x, t' and the slots hold the 32
+// right-hand sides of one row / column contiguously (slot q at 32 q).
+// A task's column dots are the small product P^T V (W x m times m x 32) on
+// v_mfma_{f64,f32}_16x16x4: A = 16 panel columns x 4 rows, B = the same 4
+// rows of v (staged in LDS for the task's m <= 256 rows), C = 16 columns x
+// 16 right-hand sides.  In k-step s of a 16-row block, lane group g
+// (lanes 16g..16g+15) takes row 4g + s, so a lane's four panel values of the
+// block are four consecutive rows of its column; A and B use the same row
+// order, so each dot is a sum over the task's rows (in a fixed order).
+constexpr int kTmRhs = 32;
+constexpr int kTmThreads = 512;
+
+template <typename T> struct TmAcc;
+template <> struct TmAcc<double> {
+  typedef double __attribute__((ext_vector_type(4))) type;
+  static __device__ __forceinline__ type mma(double a, double b, type c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return lane / 16 + 4 * r; }
+};
+template <> struct TmAcc<float> {
+  typedef float __attribute__((ext_vector_type(4))) type;
+  static __device__ __forceinline__ type mma(float a, float b, type c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane / 16) + r; }
+};
+
+// slots[(zoff + c) * 32 + j] = sum_r P[r][c] v[(base + r) * 32 + j] for the
+// task's columns c < W; eight wavefronts take 16-column tiles in turn.  Rows
+// past m (the next column's values or the arena's slack) are weighted by
+// zero; columns past W read the last column and are not stored.
+template <typename TS, typename TV>
+__global__ __launch_bounds__(kTmThreads) void coldot_mm_kernel(const ColTask* __restrict__ tasks,
+                                                               const TS* __restrict__ arena,
+                                                               const TV* __restrict__ v,
+                                                               TV* __restrict__ slots) {
+  typedef typename TmAcc<TV>::type acc_t;
+  __shared__ TV vs[256][kTmRhs + 1];
+  const ColTask tk = tasks[blockIdx.x];
+  const int m = (int)tk.m;
+  const int nrb = (m + 15) / 16;
+  for (int e = threadIdx.x; e < nrb * 16 * kTmRhs; e += kTmThreads) {
+    const int r = e / kTmRhs, j = e % kTmRhs;
+    vs[r][j] = r < m ? v[((uint64_t)tk.base + r) * kTmRhs + j] : (TV)0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int li = lane & 15, lg = lane >> 4;
+  const uint32_t nct = (tk.W + 15) / 16;
+  for (uint32_t ct = (uint32_t)w; ct < nct; ct += kTmThreads / 64) {
+    const uint32_t c = ct * 16 + li;
+    const TS* Pc = arena + tk.data + (uint64_t)(c < tk.W ? c : tk.W - 1) * (uint32_t)m + 4 * lg;
+    acc_t a0 = acc_t{0, 0, 0, 0}, a1 = acc_t{0, 0, 0, 0};
+    TV A[4], An[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) A[s] = (TV)Pc[s];
+    for (int rb = 0; rb < nrb; ++rb) {
+      if (rb + 1 < nrb) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) An[s] = (TV)Pc[(rb + 1) * 16 + s];
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int r = rb * 16 + 4 * lg + s;
+        const TV a = r < m ? A[s] : (TV)0;
+        a0 = TmAcc<TV>::mma(a, vs[r][li], a0);
+        a1 = TmAcc<TV>::mma(a, vs[r][16 + li], a1);
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) A[s] = An[s];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t cc = ct * 16 + (uint32_t)TmAcc<TV>::row(lane, q);
+      if (cc < tk.W) {
+        TV* o = slots + (tk.zoff + cc) * kTmRhs;
+        o[li] = a0[q];
+        o[16 + li] = a1[q];
+      }
+    }
+  }
+}
+
+// t'[i * 32 + j] = sum_{k in [off[i], off[i+1])} slots[pos[k] * 32 + j]
+template <typename TV>
+__global__ __launch_bounds__(256) void csr_mm_t_kernel(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ pos,
+                                                       const TV* __restrict__ slots, uint32_t n,
+                                                       TV* __restrict__ out) {
+  const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  const uint32_t i = (uint32_t)(g / kTmRhs), j = (uint32_t)(g % kTmRhs);
+  if (i >= n) return;
+  TV s = 0;
+  for (uint32_t k = off[i]; k < off[i + 1]; ++k) s += slots[(uint64_t)pos[k] * kTmRhs + j];
+  out[(uint64_t)i * kTmRhs + j] = s;
+}
+
+// Y[j * ldy + i] += alpha * sum_{k in [off[i], off[i+1])} slots[pos[k] * 32 + j]
+// for the pj right-hand sides of the group: 64 destinations per workgroup,
+// summed with the right-hand sides across lanes, stored through LDS with the
+// destinations across lanes
+template <typename TV>
+__global__ __launch_bounds__(256) void csr_mm_y_kernel(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ pos,
+                                                       const TV* __restrict__ slots, uint32_t n,
+                                                       TV* __restrict__ Y, uint32_t ldy, uint32_t pj, TV alpha) {
+  __shared__ TV sum[kTmRhs][64 + 1];
+  const uint32_t i0 = blockIdx.x * 64u;
+  const uint32_t j = threadIdx.x % kTmRhs;
+  for (uint32_t il = threadIdx.x / kTmRhs; il < 64; il += 256 / kTmRhs) {
+    const uint32_t i = i0 + il;
+    TV s = 0;
+    if (i < n)
+      for (uint32_t k = off[i]; k < off[i + 1]; ++k) s += slots[(uint64_t)pos[k] * kTmRhs + j];
+    sum[j][il] = s;
+  }
+  __syncthreads();
+  const uint32_t il = threadIdx.x & 63;
+  for (uint32_t jj = threadIdx.x >> 6; jj < pj; jj += 4)
+    if (i0 + il < n) Y[(uint64_t)jj * ldy + i0 + il] += alpha * sum[jj][il];
+}
+
+// rows [0, n) of the pj <= 32 columns X[j * ldx + i] -> xi[i * 32 + j] (zero
+// for j >= pj), 64 rows per workgroup through LDS
+template <typename TV>
+__global__ __launch_bounds__(256) void tm_interleave_kernel(const TV* __restrict__ X, uint32_t ldx, uint32_t n,
+                                                            uint32_t pj, TV* __restrict__ xi) {
+  __shared__ TV tile[kTmRhs][64 + 1];
+  const uint32_t i0 = blockIdx.x * 64u;
+  for (uint32_t e = threadIdx.x; e < kTmRhs * 64; e += 256) {
+    const uint32_t jj = e >> 6, il = e & 63;
+    tile[jj][il] = (jj < pj && i0 + il < n) ? X[(uint64_t)jj * ldx + i0 + il] : (TV)0;
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < kTmRhs * 64; e += 256) {
+    const uint32_t il = e / kTmRhs, jj = e % kTmRhs;
+    if (i0 + il < n) xi[((uint64_t)i0 + il) * kTmRhs + jj] = tile[jj][il];
+  }
+}
+
 // column pieces of about kChunkValues values per task
 void col_tasks(const std::vector<GemvTask>& panels, std::vector<ColTask>& out) {
   for (const GemvTask& p : panels) {
@@ -323,6 +467,10 @@ int build_transposed(ahm_hmat h) {
   std::vector<ColTask> ta, tb;
   col_tasks(h->h_panels_a, ta);
   col_tasks(h->h_panels, tb);
+  // the multi-RHS column dots stage a task's rows in LDS (at most 256)
+  for (const auto* v : {&ta, &tb})
+    for (const ColTask& t : *v)
+      if (t.m > 256) return fail(AHM_ERR_UNSUPPORTED, "transposed product: panel of %u rows", t.m);
   // symmetric storage: the rows of each diagonal leaf; a row panel lies in
   // one of them, and its dense columns inside that range are the diagonal
   // leaf's own (blocks of a block tree overlap the diagonal only there)
@@ -397,18 +545,60 @@ int launch_t(ahm_hmat h, TV alpha, const TV* x, TV* y, Gate g = Gate{}) {
   return AHM_OK;
 }
 
-// slots and t' sized for vectors of es bytes (allocates; before any capture)
-int ensure_t_buffers(ahm_hmat h, size_t es) {
+// y += alpha A^T X (or the transposed part of a symmetric product) for
+// p >= 2 right-hand sides, 32 at a time; after ensure_t_buffers(h, es, 32)
+template <typename TS, typename TV>
+int launch_t_mm(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y, uint32_t ldy, uint32_t p) {
   auto& T = h->tr;
-  if (h->n_zsrc * es > T.contrib_cap) {
-    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
-    AHM_TRY(T.contrib.alloc(std::max<size_t>(1, h->n_zsrc) * es));
-    T.contrib_cap = h->n_zsrc * es;
+  hipStream_t s = h->ctx->stream;
+  TV* slots = (TV*)T.contrib.p;
+  TV* tp = (TV*)T.tprime.p;
+  TV* xi = (TV*)T.xi.p;
+  const TS* arena = (const TS*)h->arena.p;
+  for (uint32_t g0 = 0; g0 < p; g0 += kTmRhs) {
+    const uint32_t pj = std::min<uint32_t>(kTmRhs, p - g0);
+    if (h->n_rows)
+      hipLaunchKernelGGL((tm_interleave_kernel<TV>), dim3((h->n_rows + 63) / 64), dim3(256), 0, s,
+                         X + (size_t)g0 * ldx, ldx, h->n_rows, pj, xi);
+    if (T.n_b)
+      hipLaunchKernelGGL((coldot_mm_kernel<TS, TV>), dim3(T.n_b), dim3(kTmThreads), 0, s,
+                         (const ColTask*)T.tasks_b.p, arena, (const TV*)xi, slots);
+    if (h->t_len)
+      hipLaunchKernelGGL((csr_mm_t_kernel<TV>), dim3((uint32_t)(((uint64_t)h->t_len * kTmRhs + 255) / 256)),
+                         dim3(256), 0, s, (const uint32_t*)T.off_t.p, (const uint32_t*)T.pos_t.p,
+                         (const TV*)slots, h->t_len, tp);
+    if (T.n_a)
+      hipLaunchKernelGGL((coldot_mm_kernel<TS, TV>), dim3(T.n_a), dim3(kTmThreads), 0, s,
+                         (const ColTask*)T.tasks_a.p, arena, (const TV*)tp, slots);
+    if (h->n_cols)
+      hipLaunchKernelGGL((csr_mm_y_kernel<TV>), dim3((h->n_cols + 63) / 64), dim3(256), 0, s,
+                         (const uint32_t*)T.off_y.p, (const uint32_t*)T.pos_y.p, (const TV*)slots, h->n_cols,
+                         Y + (size_t)g0 * ldy, ldy, pj, alpha);
   }
-  if ((size_t)h->t_len * es > T.tprime_cap) {
+  AHM_HIP(hipGetLastError());
+  return AHM_OK;
+}
+
+// slots and t' sized for vectors of es bytes, width values per slot / row
+// (1, or 32 for several right-hand sides, which also sizes the interleaved
+// copy of X); allocates; before any capture
+int ensure_t_buffers(ahm_hmat h, size_t es, size_t width = 1) {
+  auto& T = h->tr;
+  const size_t ew = es * width;
+  if (h->n_zsrc * ew > T.contrib_cap) {
     AHM_HIP(hipStreamSynchronize(h->ctx->stream));
-    AHM_TRY(T.tprime.alloc(std::max<size_t>(1, h->t_len) * es));
-    T.tprime_cap = (size_t)h->t_len * es;
+    AHM_TRY(T.contrib.alloc(std::max<size_t>(1, h->n_zsrc) * ew));
+    T.contrib_cap = h->n_zsrc * ew;
+  }
+  if ((size_t)h->t_len * ew > T.tprime_cap) {
+    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+    AHM_TRY(T.tprime.alloc(std::max<size_t>(1, h->t_len) * ew));
+    T.tprime_cap = (size_t)h->t_len * ew;
+  }
+  if (width > 1 && (size_t)h->n_rows * ew > T.xi_cap) {
+    AHM_HIP(hipStreamSynchronize(h->ctx->stream));
+    AHM_TRY(T.xi.alloc(std::max<size_t>(1, h->n_rows) * ew));
+    T.xi_cap = (size_t)h->n_rows * ew;
   }
   return AHM_OK;
 }
@@ -449,12 +639,12 @@ int build_sym_tasks(ahm_hmat h) {
   return AHM_OK;
 }
 
-int sym_prepare(ahm_hmat h, ahm_dtype vec) {
+int sym_prepare(ahm_hmat h, ahm_dtype vec, uint32_t p) {
   if (!h->sym) return AHM_OK;
   AHM_HIP(hipSetDevice(h->ctx->device));
   AHM_TRY(build_transposed(h));
   AHM_TRY(build_sym_tasks(h));
-  return ensure_t_buffers(h, dtype_size(vec));
+  return ensure_t_buffers(h, dtype_size(vec), p > 1 ? kTmRhs : 1);
 }
 
 int sym_phase_b(ahm_hmat h, double alpha, const void* x, const void* t, void* y, void* part, ahm_dtype vec,
@@ -516,17 +706,18 @@ int sym_transposed(ahm_hmat h, double alpha, const void* X, uint32_t ldx, void* 
                    ahm_dtype vec, Gate g) {
   if (!h->sym) return AHM_OK;
   const size_t es = dtype_size(vec);
-  if (!h->tr.built || h->n_zsrc * es > h->tr.contrib_cap || (size_t)h->t_len * es > h->tr.tprime_cap)
+  const size_t ew = es * (p > 1 ? kTmRhs : 1);
+  if (!h->tr.built || h->n_zsrc * ew > h->tr.contrib_cap || (size_t)h->t_len * ew > h->tr.tprime_cap ||
+      (p > 1 && (size_t)h->n_rows * ew > h->tr.xi_cap))
     return fail(AHM_ERR_INVALID, "symmetric product: transposed part not prepared");
-  for (uint32_t j = 0; j < p; ++j) {
+  if (p > 1) {
     if (vec == AHM_F64)
-      AHM_TRY((launch_t<double, double>(h, alpha, (const double*)X + (size_t)j * ldx,
-                                        (double*)Y + (size_t)j * ldy, g)));
-    else
-      AHM_TRY((launch_t<float, float>(h, (float)alpha, (const float*)X + (size_t)j * ldx,
-                                      (float*)Y + (size_t)j * ldy, g)));
+      return launch_t_mm<double, double>(h, alpha, (const double*)X, ldx, (double*)Y, ldy, p);
+    return launch_t_mm<float, float>(h, (float)alpha, (const float*)X, ldx, (float*)Y, ldy, p);
   }
-  return AHM_OK;
+  if (vec == AHM_F64)
+    return launch_t<double, double>(h, alpha, (const double*)X, (double*)Y, g);
+  return launch_t<float, float>(h, (float)alpha, (const float*)X, (float*)Y, g);
 }
 
 }  // namespace ahm
@@ -550,16 +741,13 @@ int ahm_hmat_matvec_t(ahm_hmat h, double alpha, const void* X, uint32_t ldx, voi
   AHM_HIP(hipSetDevice(h->ctx->device));
   AHM_TRY(build_transposed(h));
   const size_t es = f64 ? 8 : 4;
-  AHM_TRY(ensure_t_buffers(h, es));
-  for (uint32_t j = 0; j < p; ++j) {
-    if (f64)
-      AHM_TRY((launch_t<double, double>(h, alpha, (const double*)X + (size_t)j * ldx,
-                                        (double*)Y + (size_t)j * ldy)));
-    else
-      AHM_TRY((launch_t<float, float>(h, (float)alpha, (const float*)X + (size_t)j * ldx,
-                                      (float*)Y + (size_t)j * ldy)));
+  AHM_TRY(ensure_t_buffers(h, es, p > 1 ? kTmRhs : 1));
+  if (p > 1) {
+    if (f64) return launch_t_mm<double, double>(h, alpha, (const double*)X, ldx, (double*)Y, ldy, p);
+    return launch_t_mm<float, float>(h, (float)alpha, (const float*)X, ldx, (float*)Y, ldy, p);
   }
-  return AHM_OK;
+  if (f64) return launch_t<double, double>(h, alpha, (const double*)X, (double*)Y);
+  return launch_t<float, float>(h, (float)alpha, (const float*)X, (float*)Y);
 }
 
 int ahm_hmat_matvec_t_host(ahm_hmat h, double alpha, const double* X, uint32_t ldx, double* Y,

@@ -1,8 +1,10 @@
 """GPU transposed H-matvec y += d A^T x (the real case of mltaGeHhVec /
 mltaGeHhGeM, H/mltaGeHVec.cpp:61-104) against the compiled reference's outputs
 (tests/golden/golden_transposed.npz) and the oracle, on the identical
-H-matrix.  Bar: FP64 within 1e-12 relative (max-norm); FP32 storage within
-5e-6 of the FP64 product; repeated products bit-identical."""
+H-matrix.  Several right-hand sides take the MFMA column dots (32 at a time)
+on panels of every height.  Bar: FP64 within 1e-12 relative (max-norm);
+FP32 storage within 5e-6 of the FP64 product; repeated products
+bit-identical."""
 import ctypes as C
 import os
 
@@ -94,3 +96,40 @@ def test_transposed_is_the_adjoint(ctx):
     rhs = float(torch.dot(dev(x), az))
     assert abs(lhs - rhs) <= 1e-12 * abs(rhs) * 10
     D.close()
+
+
+@pytest.mark.parametrize("p", [2, 17, 32, 40])
+def test_transposed_multi_rhs_mfma_every_panel_height(ctx, p):
+    """Several right-hand sides run the column dots 32 at a time on MFMA:
+    row panels of every height class, V^T panels of more than 128 rows, a
+    second group of right-hand sides (p = 40), FP64 and FP32 storage, against
+    the dense transpose."""
+    from ahmed_amd import _lib
+    from test_gpu_matvec import MFMA_SIZES, VT_HEIGHTS
+    from helpers import block_matrix
+    nseg = len(VT_HEIGHTS)
+
+    def rank_of(i, j):
+        q, r = divmod(VT_HEIGHTS[j], nseg - 1)
+        return q + (1 if (i - (i > j)) < r else 0)
+
+    for sizes, seed, rk in ((MFMA_SIZES, 11, None), ([64] * nseg, 5, rank_of)):
+        N, leaves, data, A = block_matrix(sizes, seed, rk)
+        print(f"[gpu] transposed MFMA N={N} p={p}", flush=True)
+        D = _lib.HMatrix.upload(ctx, N, N, leaves, data)
+        rng = np.random.default_rng(p)
+        X = rng.standard_normal((p, N))
+        Y0 = rng.standard_normal((p, N))
+        Y = dev(Y0)
+        D.matvec_t(dev(X), Y, -0.75)
+        ref = Y0 - 0.75 * (A.T @ X.T).T
+        assert relerr(Y.cpu().numpy(), ref) <= 1e-12
+        # the same columns one at a time (the single-RHS kernels)
+        for j in (0, p - 1):
+            y = dev(Y0[j])
+            D.matvec_t(dev(X[j]), y, -0.75)
+            assert relerr(y.cpu().numpy(), ref[j]) <= 1e-12
+        D32 = D.convert()
+        Y32 = dev(np.zeros((p, N)), torch.float32)
+        D32.matvec_t(dev(X, torch.float32), Y32, 1.0)
+        assert relerr(Y32.cpu().numpy().astype(np.float64), (A.T @ X.T).T) <= 5e-6
