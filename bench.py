@@ -185,7 +185,7 @@ def cpu_baseline(pts, leaves, data, bmin, eta2, B_alg, y_gpu_of, budget_s=20.0, 
     return out
 
 
-def symmetric_path(ctx, torch, pts, bmin, eta, weight, eps, tol, H, x, ms_geh, steps):
+def symmetric_path(ctx, torch, pts, bmin, eta, weight, eps, tol, H, x, ms_geh, steps, X=None):
     """The symmetric storage of the same matrix (the reference's BEM example:
     subdivide_sym, apprx_sym with ACAr and packed Hermitian diagonal leaves,
     mltaHeHVec, CG on HeHMatrix): assembly time, stored values against the
@@ -210,6 +210,12 @@ def symmetric_path(ctx, torch, pts, bmin, eta, weight, eps, tol, H, x, ms_geh, s
     yg = torch.zeros(N, dtype=torch.float64, device="cuda")
     H.matvec(x, yg)
     diff = float((y - yg).abs().max() / yg.abs().max())
+    ms_p = None
+    if X is not None:   # several right-hand sides (mltaHeHGeM)
+        Yp = torch.zeros_like(X)
+        S.matvec(X, Yp)
+        ms_p = timed(lambda: S.matvec(X, Yp), torch, max(3, steps // 4), 1)
+        del Yp
     b = torch.zeros(N, dtype=torch.float64, device="cuda")
     S.matvec(torch.ones(N, dtype=torch.float64, device="cuda"), b)
     xs = torch.zeros(N, dtype=torch.float64, device="cuda")
@@ -228,6 +234,7 @@ def symmetric_path(ctx, torch, pts, bmin, eta, weight, eps, tol, H, x, ms_geh, s
            "stored_values": stored, "stored_vs_unsymmetric": round(stored / H.stats()["nvals"], 4),
            "matvec_ms": round(ms, 4), "matvec_ms_unsymmetric": round(ms_geh, 4),
            "rel_diff_vs_unsymmetric_product": diff,
+           **({f"matvec_rhs{X.shape[0]}_ms": round(ms_p, 4)} if ms_p is not None else {}),
            "cg": {"seconds": round(t_cg, 4), "converged": r["converged"], "iterations": r["nsteps"],
                   "residual": r["eps"], "max_err_vs_ones": float((xs - 1).abs().max())}}
     log(f"symmetric storage: assembly {t_asm:.2f} s, {stored:.4g} values ({out['stored_vs_unsymmetric']:.3f} of "
@@ -689,6 +696,9 @@ def main():
     F_alg = 2.0 * p * st["nvals"]
     Bp = 8.0 * st["nvals"] + 8.0 * p * (N + 2 * N)
     log(f"FP64 H-matvec {p} RHS: {msp:.3f} ms, {F_alg / (msp * 1e-3) / 1e12:.2f} TF/s")
+    H.matvec_t(X, Y)           # sizes the 32-wide column-dot slots once
+    mstp = timed(lambda: H.matvec_t(X, Y), torch, max(3, args.steps // 4), 1)
+    log(f"FP64 transposed H-matvec {p} RHS: {mstp:.3f} ms, {F_alg / (mstp * 1e-3) / 1e12:.2f} TF/s")
 
     # ---- FP32 storage ----------------------------------------------------------
     H32 = H.convert()
@@ -797,7 +807,7 @@ def main():
         gm["slower_system"] = slow
         del Hs, Hs32, bs
     gm["small_system"] = small_system_gmres(ctx, torch)
-    sym = symmetric_path(ctx, torch, pts, bmin, eta, weight, eps, tol, H, x, ms, args.steps)
+    sym = symmetric_path(ctx, torch, pts, bmin, eta, weight, eps, tol, H, x, ms, args.steps, X)
     ctx.release_cached()
     # what the products alone allow: the FP64 solve's products against the
     # refinement's FP64 residual products plus its FP32 inner products
@@ -874,7 +884,8 @@ def main():
                              "TF/s": round(F_alg / (msp * 1e-3) / 1e12, 3),
                              "mfma_frac_fp64": round(F_alg / (msp * 1e-3) / 1e12 / FP64_PEAK_TFS, 4),
                              "GB/s_alg": round(Bp / (msp * 1e-3) / 1e9, 2)},
-        "matvec_transposed": {"ms": round(mst, 4), "GB/s": round(B_alg / (mst * 1e-3) / 1e9, 2)},
+        "matvec_transposed": {"ms": round(mst, 4), "GB/s": round(B_alg / (mst * 1e-3) / 1e9, 2),
+                              f"rhs{p}": {"ms": round(mstp, 4), "TF/s": round(F_alg / (mstp * 1e-3) / 1e12, 3)}},
         "fp32_storage": {"ms": round(ms32, 4), "GB/s": round(B32 / (ms32 * 1e-3) / 1e9, 2),
                          "rel_err_vs_fp64": err32,
                          "rhs32": {"ms": round(ms32p, 4), "TF/s": round(F_alg / (ms32p * 1e-3) / 1e12, 3),
