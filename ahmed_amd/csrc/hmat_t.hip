@@ -309,6 +309,27 @@ template <> struct TmAcc<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane / 16) + r; }
 };
 
+// a lane's four consecutive panel values of a 16-row block, in two 16-byte
+// loads (8-byte aligned; the target reads unaligned vector loads)
+typedef double tm_d2 __attribute__((ext_vector_type(2), aligned(8)));
+typedef float tm_f4 __attribute__((ext_vector_type(4), aligned(4)));
+template <typename TS, typename TV>
+__device__ __forceinline__ void tm_load4(TV (&A)[4], const TS* p) {
+  if constexpr (sizeof(TS) == 8) {
+    const tm_d2 lo = *(const tm_d2*)p, hi = *(const tm_d2*)(p + 2);
+    A[0] = (TV)lo.x;
+    A[1] = (TV)lo.y;
+    A[2] = (TV)hi.x;
+    A[3] = (TV)hi.y;
+  } else {
+    const tm_f4 v = *(const tm_f4*)p;
+    A[0] = (TV)v.x;
+    A[1] = (TV)v.y;
+    A[2] = (TV)v.z;
+    A[3] = (TV)v.w;
+  }
+}
+
 // slots[(zoff + c) * 32 + j] = sum_r P[r][c] v[(base + r) * 32 + j] for the
 // task's columns c < W; eight wavefronts take 16-column tiles in turn.  Rows
 // past m (the next column's values or the arena's slack) are weighted by
@@ -337,13 +358,9 @@ __global__ __launch_bounds__(kTmThreads) void coldot_mm_kernel(const ColTask* __
     const TS* Pc = arena + tk.data + (uint64_t)(c < tk.W ? c : tk.W - 1) * (uint32_t)m + 4 * lg;
     acc_t a0 = acc_t{0, 0, 0, 0}, a1 = acc_t{0, 0, 0, 0};
     TV A[4], An[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) A[s] = (TV)Pc[s];
+    tm_load4<TS, TV>(A, Pc);
     for (int rb = 0; rb < nrb; ++rb) {
-      if (rb + 1 < nrb) {
-#pragma unroll
-        for (int s = 0; s < 4; ++s) An[s] = (TV)Pc[(rb + 1) * 16 + s];
-      }
+      if (rb + 1 < nrb) tm_load4<TS, TV>(An, Pc + (rb + 1) * 16);
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int r = rb * 16 + 4 * lg + s;
@@ -366,6 +383,27 @@ __global__ __launch_bounds__(kTmThreads) void coldot_mm_kernel(const ColTask* __
   }
 }
 
+// sum of slots[pos[k] * 32 + j] over k in [k0, k1), in k order; eight
+// positions, then their eight slot values, in flight together
+template <typename TV>
+__device__ __forceinline__ TV tm_slot_sum(uint32_t k0, uint32_t k1, const uint32_t* __restrict__ pos,
+                                          const TV* __restrict__ slots, uint32_t j) {
+  TV s = 0;
+  uint32_t k = k0;
+  for (; k + 8 <= k1; k += 8) {
+    uint32_t q[8];
+    TV v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) q[u] = pos[k + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = slots[(uint64_t)q[u] * kTmRhs + j];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; k < k1; ++k) s += slots[(uint64_t)pos[k] * kTmRhs + j];
+  return s;
+}
+
 // t'[i * 32 + j] = sum_{k in [off[i], off[i+1])} slots[pos[k] * 32 + j]
 template <typename TV>
 __global__ __launch_bounds__(256) void csr_mm_t_kernel(const uint32_t* __restrict__ off,
@@ -375,9 +413,7 @@ __global__ __launch_bounds__(256) void csr_mm_t_kernel(const uint32_t* __restric
   const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
   const uint32_t i = (uint32_t)(g / kTmRhs), j = (uint32_t)(g % kTmRhs);
   if (i >= n) return;
-  TV s = 0;
-  for (uint32_t k = off[i]; k < off[i + 1]; ++k) s += slots[(uint64_t)pos[k] * kTmRhs + j];
-  out[(uint64_t)i * kTmRhs + j] = s;
+  out[(uint64_t)i * kTmRhs + j] = tm_slot_sum(off[i], off[i + 1], pos, slots, j);
 }
 
 // Y[j * ldy + i] += alpha * sum_{k in [off[i], off[i+1])} slots[pos[k] * 32 + j]
@@ -394,10 +430,7 @@ __global__ __launch_bounds__(256) void csr_mm_y_kernel(const uint32_t* __restric
   const uint32_t j = threadIdx.x % kTmRhs;
   for (uint32_t il = threadIdx.x / kTmRhs; il < 64; il += 256 / kTmRhs) {
     const uint32_t i = i0 + il;
-    TV s = 0;
-    if (i < n)
-      for (uint32_t k = off[i]; k < off[i + 1]; ++k) s += slots[(uint64_t)pos[k] * kTmRhs + j];
-    sum[j][il] = s;
+    sum[j][il] = i < n ? tm_slot_sum(off[i], off[i + 1], pos, slots, j) : (TV)0;
   }
   __syncthreads();
   const uint32_t il = threadIdx.x & 63;
