@@ -332,8 +332,9 @@ __device__ __forceinline__ void tm_load4(TV (&A)[4], const TS* p) {
 
 // slots[(zoff + c) * 32 + j] = sum_r P[r][c] v[(base + r) * 32 + j] for the
 // task's columns c < W; eight wavefronts take 16-column tiles in turn.  Rows
-// past m (the next column's values or the arena's slack) are weighted by
-// zero; columns past W read the last column and are not stored.
+// past m (the next column's values or the arena's slack, at most 31 values
+// past the task) are weighted by zero; columns past W read the last column
+// and are not stored.
 template <typename TS, typename TV>
 __global__ __launch_bounds__(kTmThreads) void coldot_mm_kernel(const ColTask* __restrict__ tasks,
                                                                const TS* __restrict__ arena,
@@ -357,19 +358,30 @@ __global__ __launch_bounds__(kTmThreads) void coldot_mm_kernel(const ColTask* __
     const uint32_t c = ct * 16 + li;
     const TS* Pc = arena + tk.data + (uint64_t)(c < tk.W ? c : tk.W - 1) * (uint32_t)m + 4 * lg;
     acc_t a0 = acc_t{0, 0, 0, 0}, a1 = acc_t{0, 0, 0, 0};
-    TV A[4], An[4];
-    tm_load4<TS, TV>(A, Pc);
-    for (int rb = 0; rb < nrb; ++rb) {
-      if (rb + 1 < nrb) tm_load4<TS, TV>(An, Pc + (rb + 1) * 16);
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int r = rb * 16 + 4 * lg + s;
-        const TV a = r < m ? A[s] : (TV)0;
-        a0 = TmAcc<TV>::mma(a, vs[r][li], a0);
-        a1 = TmAcc<TV>::mma(a, vs[r][16 + li], a1);
+    // two 16-row blocks per step, the next two in flight
+    TV A[2][4], An[2][4];
+    tm_load4<TS, TV>(A[0], Pc);
+    tm_load4<TS, TV>(A[1], Pc + 16);
+    for (int rb = 0; rb < nrb; rb += 2) {
+      if (rb + 2 < nrb) {
+        tm_load4<TS, TV>(An[0], Pc + (rb + 2) * 16);
+        tm_load4<TS, TV>(An[1], Pc + (rb + 3) * 16);
       }
 #pragma unroll
-      for (int s = 0; s < 4; ++s) A[s] = An[s];
+      for (int h = 0; h < 2; ++h) {
+        if (rb + h >= nrb) break;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int r = (rb + h) * 16 + 4 * lg + s;
+          const TV a = r < m ? A[h][s] : (TV)0;
+          a0 = TmAcc<TV>::mma(a, vs[r][li], a0);
+          a1 = TmAcc<TV>::mma(a, vs[r][16 + li], a1);
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) A[h][s] = An[h][s];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
