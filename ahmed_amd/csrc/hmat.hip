@@ -89,16 +89,28 @@ __global__ __launch_bounds__(256) void relayout_kernel(const CopyJob* __restrict
     return;
   }
   // column by column, no per-element division: a column shorter than the
-  // workgroup shares it with the next blockDim.x / rows - 1 columns
+  // workgroup shares it with the next blockDim.x / rows - 1 columns; each
+  // thread has four columns' loads in flight before it stores
   const uint32_t step = j.rows >= blockDim.x ? blockDim.x : j.rows;
   const uint32_t per = blockDim.x / step;
   const uint32_t sub = threadIdx.x / step, r0 = threadIdx.x - sub * step;
   if (sub >= per) return;
-  for (uint32_t c = sub; c < j.cols; c += per) {
-    const TS* s = src + j.src + (uint64_t)c * j.src_ld;
-    TD* d = dst + j.dst + (uint64_t)c * j.dst_ld;
-    for (uint32_t r = r0; r < j.rows; r += step) d[r] = (TD)s[r];
-  }
+  const TS* s = src + j.src;
+  TD* d = dst + j.dst;
+  for (uint32_t r = r0; r < j.rows; r += step)
+    for (uint32_t c = sub; c < j.cols; c += 4 * per) {
+      TS v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t cc = c + u * per;
+        v[u] = cc < j.cols ? s[(uint64_t)cc * j.src_ld + r] : (TS)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t cc = c + u * per;
+        if (cc < j.cols) d[(uint64_t)cc * j.dst_ld + r] = (TD)v[u];
+      }
+    }
 }
 
 template <typename TD, typename TS>
