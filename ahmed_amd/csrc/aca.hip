@@ -1703,6 +1703,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
   }
   DeviceBuf dP;
   AHM_TRY(to_device(dP, P, s));
+  timer.mark("set-up: kernels and permuted points");
   // ACA candidates and rank limits (Include/apprx.h:143-151).  The product
   // n1*n2 is taken in 64 bits (the reference's 32-bit product wraps for
   // leaves with n1*n2 >= 2^32).
@@ -1726,6 +1727,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     const uint32_t h = kcap_hint ? std::max<uint32_t>(kcap_hint[l], 1) : kAcaCapDefault;
     cap[c] = std::min(h, lim);
   }
+  timer.mark("set-up: candidates and rank caps");
   uint64_t retries = 0, slab_total = 0, slab_peak = 0;
   std::vector<size_t> todo(cand.size());
   std::iota(todo.begin(), todo.end(), 0);
@@ -1781,8 +1783,11 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     if (all && gslab.need(std::min<uint64_t>(all * 8, budget))) return fail(AHM_ERR_HIP, "ACA scratch");
     // largest slabs first: the leaves most likely to need a rerun go early,
     // so their reruns share the following batches with many small first
-    // attempts
-    std::stable_sort(todo.begin(), todo.end(), [&](size_t a, size_t b) { return cap[a] > cap[b]; });
+    // attempts (a stable counting sort: caps are at most kAcaCapMax)
+    std::vector<size_t> first(kAcaCapMax + 2, 0);
+    for (size_t c = 0; c < cand.size(); ++c) first[kAcaCapMax - std::min(cap[c], kAcaCapMax) + 1]++;
+    for (uint32_t q = 0; q <= kAcaCapMax; ++q) first[q + 1] += first[q];
+    for (size_t c = 0; c < cand.size(); ++c) todo[first[kAcaCapMax - std::min(cap[c], kAcaCapMax)]++] = c;
   }
   timer.mark("set-up (points, candidates)");
   const auto t_aca0 = std::chrono::steady_clock::now();
