@@ -1685,7 +1685,7 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
                     const KernelSpec* Ks, uint32_t nk, const uint32_t* leaf_kernel,
                     double eps, uint32_t rankmax, const uint32_t* kcap_hint,
                     std::vector<ahm_leaf_desc>& leaves, DeviceBuf& data,
-                    ahm_assembly_stats* stats, uint32_t* aca_rank_out) {
+                    ahm_assembly_stats* stats, uint32_t* aca_rank_out, std::shared_ptr<PlanJob>* plan) {
   hipStream_t s = ctx->stream;
   PhaseTimer timer("assembly");
   if (nk == 0) return fail(AHM_ERR_INVALID, "no kernel");
@@ -2017,6 +2017,9 @@ int assemble_leaves(ahm_ctx ctx, const double* pts, const uint32_t* op_perm, uin
     }
   }
   timer.mark("leaf list");
+  // the layout plan needs the leaf list only: it is built on a host thread
+  // while the leaf buffer below is copied and the dense leaves are filled
+  if (plan) *plan = plan_start(N, N, leaves);
   int st = AHM_OK;
   if (gslab.buf.p && gslab.buf.cap >= total * 8 && total) {
     // the slab scratch (no longer needed) becomes the leaf buffer
@@ -2098,8 +2101,9 @@ extern "C" int ahm_assemble(ahm_ctx ctx, const double* pts, const uint32_t* op_p
   AHM_HIP(hipSetDevice(ctx->device));
   std::vector<ahm_leaf_desc> leaves;
   DeviceBuf data;
+  std::shared_ptr<PlanJob> plan;   // after leaves: destroyed (its thread joined) first
   const KernelSpec K = make_kernel(*kernel);
   AHM_TRY(assemble_leaves(ctx, pts, op_perm, N, n_leaves, b1, n1, b2, n2, adm, i0, &K, 1, nullptr,
-                          eps, rankmax, rank_hint, leaves, data, stats, aca_rank));
-  return hmat_from_device(ctx, N, N, std::move(leaves), data, storage, keep_leaves, out);
+                          eps, rankmax, rank_hint, leaves, data, stats, aca_rank, &plan));
+  return hmat_from_device(ctx, N, N, std::move(leaves), data, storage, keep_leaves, out, plan);
 }

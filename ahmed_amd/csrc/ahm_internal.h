@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <string>
 #include <utility>
+#include <memory>
 #include <vector>
 
 #include "ahm.h"
@@ -234,9 +235,10 @@ struct GmresArgs {
 };
 int gmres_run(ahm_hmat h, const GmresArgs& a);
 
+struct PlanJob;
 int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_leaf_desc>&& leaves,
                struct DeviceBuf& raw, ahm_dtype data_dtype, ahm_dtype storage, int keep_leaves,
-               ahm_hmat* out);
+               ahm_hmat* out, std::shared_ptr<PlanJob> pre = nullptr);
 
 }  // namespace ahm
 

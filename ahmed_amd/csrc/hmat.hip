@@ -21,7 +21,9 @@
 // source list.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <numeric>
+#include <string>
 #include <thread>
 #include <type_traits>
 
@@ -1891,12 +1893,46 @@ int set_chunking(ahm_hmat h, uint64_t cv) {
   return AHM_OK;
 }
 
+struct PlanJob {
+  Plan P;
+  int st = AHM_OK;
+  std::string err;
+  uint32_t n_rows = 0, n_cols = 0, nl = 0;
+  const ahm_leaf_desc* leaves = nullptr;
+  std::thread th;
+  ~PlanJob() {
+    if (th.joinable()) th.join();
+  }
+};
+
+std::shared_ptr<PlanJob> plan_start(uint32_t n_rows, uint32_t n_cols, const std::vector<ahm_leaf_desc>& leaves) {
+  auto job = std::make_shared<PlanJob>();
+  PlanJob* j = job.get();
+  j->n_rows = n_rows;
+  j->n_cols = n_cols;
+  j->nl = (uint32_t)leaves.size();
+  j->leaves = leaves.data();
+  j->th = std::thread([j]() {
+    j->st = build_plan(j->n_rows, j->n_cols, j->nl, j->leaves, j->P);
+    if (j->st) j->err = ahm_last_error();
+  });
+  return job;
+}
+
 int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_leaf_desc>&& leaves,
                DeviceBuf& raw, ahm_dtype data_dtype, ahm_dtype storage, int keep_leaves,
-               ahm_hmat* out) {
+               ahm_hmat* out, std::shared_ptr<PlanJob> pre) {
   PhaseTimer timer("layout");
   Plan P;
-  AHM_TRY(build_plan(n_rows, n_cols, (uint32_t)leaves.size(), leaves.data(), P));
+  if (pre) {
+    if (pre->th.joinable()) pre->th.join();
+    if (pre->n_rows != n_rows || pre->n_cols != n_cols || pre->nl != leaves.size() || pre->leaves != leaves.data())
+      return fail(AHM_ERR_INVALID, "layout plan of another leaf list");
+    if (pre->st) return fail(pre->st, "%s", pre->err.c_str());
+    P = std::move(pre->P);
+  } else {
+    AHM_TRY(build_plan(n_rows, n_cols, (uint32_t)leaves.size(), leaves.data(), P));
+  }
   timer.mark("plan (panels, chunks, copy jobs)");
   for (const ahm_leaf_desc& d : leaves) {
     const uint64_t nv = leaf_values(d);
@@ -1990,8 +2026,9 @@ int hmat_build(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols, std::vector<ahm_le
 
 int hmat_from_device(ahm_ctx ctx, uint32_t n_rows, uint32_t n_cols,
                      std::vector<ahm_leaf_desc>&& leaves, DeviceBuf& raw, ahm_dtype storage,
-                     int keep_leaves, ahm_hmat* out) {
-  return hmat_build(ctx, n_rows, n_cols, std::move(leaves), raw, AHM_F64, storage, keep_leaves, out);
+                     int keep_leaves, ahm_hmat* out, std::shared_ptr<PlanJob> plan) {
+  return hmat_build(ctx, n_rows, n_cols, std::move(leaves), raw, AHM_F64, storage, keep_leaves, out,
+                    std::move(plan));
 }
 
 }  // namespace ahm

@@ -214,10 +214,11 @@ extern "C" int ahm_assemble_batch(ahm_ctx ctx, uint32_t n_mat, const ahm_tree* t
   if (hipSetDevice(ctx->device) != hipSuccess) return ahm::fail(AHM_ERR_HIP, "hipSetDevice");
   std::vector<ahm_leaf_desc> leaves;
   ahm::DeviceBuf data;
+  std::shared_ptr<ahm::PlanJob> plan;   // after leaves: destroyed (its thread joined) first
   AHM_TRY(ahm::assemble_leaves(ctx, pts.data(), op.data(), N, (uint32_t)b1.size(), b1.data(),
                                n1.data(), b2.data(), n2.data(), adm.data(), i0.data(), K.data(),
-                               n_mat, lk.data(), eps, rankmax, nullptr, leaves, data, stats));
-  AHM_TRY(ahm::hmat_from_device(ctx, N, N, std::move(leaves), data, storage, keep_leaves, out));
+                               n_mat, lk.data(), eps, rankmax, nullptr, leaves, data, stats, nullptr, &plan));
+  AHM_TRY(ahm::hmat_from_device(ctx, N, N, std::move(leaves), data, storage, keep_leaves, out, plan));
   (*out)->seg_off = off;
   return AHM_OK;
 }
