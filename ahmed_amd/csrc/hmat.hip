@@ -656,9 +656,38 @@ __device__ __forceinline__ void mm_task(const GemvTask& tk, const uint32_t* __re
   const uint32_t lob = ((uint32_t)lk * m + (uint32_t)li) * (uint32_t)sizeof(T);
   const char* P = (const char*)(arena + tk.data);
   const uint32_t R = (W + 15) / 16;
+  // FP32: each Z chunk's products are summed in accumulators of their own
+  // and then added to the running sums, so a long row panel is not one FP32
+  // chain of thousands of terms
+  constexpr bool kSplitSum = sizeof(T) == 4;
+  acc_t tot[4][2], toth = acc_t{0, 0, 0, 0};
+  if constexpr (kSplitSum) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) tot[a][b] = acc_t{0, 0, 0, 0};
+  }
+  auto fold = [&]() {
+    if constexpr (kSplitSum) {
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        tot[q][0] += acc[q][0];
+        acc[q][0] = acc_t{0, 0, 0, 0};
+        if (TWO) {
+          tot[q][1] += acc[q][1];
+          acc[q][1] = acc_t{0, 0, 0, 0};
+        }
+      }
+      if (HALF) {
+        toth += acch;
+        acch = acc_t{0, 0, 0, 0};
+      }
+    }
+  };
   auto gather = [&](uint32_t r) {
     if (r % (kMmChunk / 16)) return;
     const uint32_t c0 = r * 16;
+    if (r) fold();
     __syncthreads();
     mm_gather<T>(zs, zsrc + tk.zoff + c0, (int)min((uint32_t)kMmChunk, W - c0), xg, tg);
     __syncthreads();
@@ -690,6 +719,15 @@ __device__ __forceinline__ void mm_task(const GemvTask& tk, const uint32_t* __re
   if (WORK) {
     if (!PIPE) mm_load<T, NW, HALF, NA>(A0, P, r, m, lob, w, hrt);
     mm_mma<T, NW, HALF, TWO, true, NA>(A0, r, W, li, lk, hct, zs, acc, acch);
+  }
+  fold();
+  if constexpr (kSplitSum) {
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      acc[q][0] = tot[q][0];
+      if (TWO) acc[q][1] = tot[q][1];
+    }
+    if (HALF) acch = toth;
   }
   if (!PHASE_B && !tk.partial) {
     // t rows hold their 32 right-hand sides contiguously: direct stores
