@@ -308,6 +308,8 @@ struct ahm_hmat_s {
     bool built = false;
     ahm::DeviceBuf tasks_a, tasks_b, off_t, pos_t, off_y, pos_y;
     uint32_t n_a = 0, n_b = 0;
+    ahm::DeviceBuf tasks_a_mm, tasks_b_mm;   // the same columns in longer pieces (several RHS)
+    uint32_t n_a_mm = 0, n_b_mm = 0;
     // symmetric products: the phase-B tasks (same chunks and partial-sum
     // slots as tasks_b of the handle) with the first row of their panel,
     // for the single pass that forms row sums and column dots together

@@ -292,6 +292,7 @@ __global__ __launch_bounds__(256) void csr_sum_kernel(const uint32_t* __restrict
 // order, so each dot is a sum over the task's rows (in a fixed order).
 constexpr int kTmRhs = 32;
 constexpr int kTmThreads = 512;
+constexpr uint64_t kTmChunkValues = 4 * kChunkValues;
 
 template <typename T> struct TmAcc;
 template <> struct TmAcc<double> {
@@ -469,9 +470,10 @@ __global__ __launch_bounds__(256) void tm_interleave_kernel(const TV* __restrict
 }
 
 // column pieces of about kChunkValues values per task
-void col_tasks(const std::vector<GemvTask>& panels, std::vector<ColTask>& out) {
+void col_tasks(const std::vector<GemvTask>& panels, std::vector<ColTask>& out,
+               uint64_t chunk_values = kChunkValues) {
   for (const GemvTask& p : panels) {
-    const uint32_t cols = (uint32_t)std::max<uint64_t>(1, kChunkValues / std::max(1u, p.m));
+    const uint32_t cols = (uint32_t)std::max<uint64_t>(1, chunk_values / std::max(1u, p.m));
     for (uint32_t c0 = 0; c0 < p.W; c0 += cols) {
       ColTask t{};
       t.data = p.data + (uint64_t)c0 * p.m;
@@ -509,11 +511,15 @@ int build_transposed(ahm_hmat h) {
   auto& T = h->tr;
   if (T.built) return AHM_OK;
   if (h->n_zsrc >= 0xFFFFFFFFull) return fail(AHM_ERR_UNSUPPORTED, "transposed product: too many panel columns");
-  std::vector<ColTask> ta, tb;
+  std::vector<ColTask> ta, tb, ta_mm, tb_mm;
   col_tasks(h->h_panels_a, ta);
   col_tasks(h->h_panels, tb);
+  // multi-RHS column dots: longer column pieces (a task stages its rows of
+  // the vector in LDS once, and does 32 times the arithmetic per value)
+  col_tasks(h->h_panels_a, ta_mm, kTmChunkValues);
+  col_tasks(h->h_panels, tb_mm, kTmChunkValues);
   // the multi-RHS column dots stage a task's rows in LDS (at most 256)
-  for (const auto* v : {&ta, &tb})
+  for (const auto* v : {&ta, &tb, &ta_mm, &tb_mm})
     for (const ColTask& t : *v)
       if (t.m > 256) return fail(AHM_ERR_UNSUPPORTED, "transposed product: panel of %u rows", t.m);
   // symmetric storage: the rows of each diagonal leaf; a row panel lies in
@@ -554,6 +560,8 @@ int build_transposed(ahm_hmat h) {
   hipStream_t s = h->ctx->stream;
   AHM_TRY(upload(T.tasks_a, ta, s));
   AHM_TRY(upload(T.tasks_b, tb, s));
+  AHM_TRY(upload(T.tasks_a_mm, ta_mm, s));
+  AHM_TRY(upload(T.tasks_b_mm, tb_mm, s));
   AHM_TRY(upload(T.off_t, off_t, s));
   AHM_TRY(upload(T.pos_t, pos_t, s));
   AHM_TRY(upload(T.off_y, off_y, s));
@@ -561,6 +569,8 @@ int build_transposed(ahm_hmat h) {
   AHM_HIP(hipStreamSynchronize(s));
   T.n_a = (uint32_t)ta.size();
   T.n_b = (uint32_t)tb.size();
+  T.n_a_mm = (uint32_t)ta_mm.size();
+  T.n_b_mm = (uint32_t)tb_mm.size();
   T.built = true;
   return AHM_OK;
 }
@@ -605,16 +615,16 @@ int launch_t_mm(ahm_hmat h, TV alpha, const TV* X, uint32_t ldx, TV* Y, uint32_t
     if (h->n_rows)
       hipLaunchKernelGGL((tm_interleave_kernel<TV>), dim3((h->n_rows + 63) / 64), dim3(256), 0, s,
                          X + (size_t)g0 * ldx, ldx, h->n_rows, pj, xi);
-    if (T.n_b)
-      hipLaunchKernelGGL((coldot_mm_kernel<TS, TV>), dim3(T.n_b), dim3(kTmThreads), 0, s,
-                         (const ColTask*)T.tasks_b.p, arena, (const TV*)xi, slots);
+    if (T.n_b_mm)
+      hipLaunchKernelGGL((coldot_mm_kernel<TS, TV>), dim3(T.n_b_mm), dim3(kTmThreads), 0, s,
+                         (const ColTask*)T.tasks_b_mm.p, arena, (const TV*)xi, slots);
     if (h->t_len)
       hipLaunchKernelGGL((csr_mm_t_kernel<TV>), dim3((uint32_t)(((uint64_t)h->t_len * kTmRhs + 255) / 256)),
                          dim3(256), 0, s, (const uint32_t*)T.off_t.p, (const uint32_t*)T.pos_t.p,
                          (const TV*)slots, h->t_len, tp);
-    if (T.n_a)
-      hipLaunchKernelGGL((coldot_mm_kernel<TS, TV>), dim3(T.n_a), dim3(kTmThreads), 0, s,
-                         (const ColTask*)T.tasks_a.p, arena, (const TV*)tp, slots);
+    if (T.n_a_mm)
+      hipLaunchKernelGGL((coldot_mm_kernel<TS, TV>), dim3(T.n_a_mm), dim3(kTmThreads), 0, s,
+                         (const ColTask*)T.tasks_a_mm.p, arena, (const TV*)tp, slots);
     if (h->n_cols)
       hipLaunchKernelGGL((csr_mm_y_kernel<TV>), dim3((h->n_cols + 63) / 64), dim3(256), 0, s,
                          (const uint32_t*)T.off_y.p, (const uint32_t*)T.pos_y.p, (const TV*)slots, h->n_cols,
