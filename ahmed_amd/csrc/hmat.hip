@@ -1453,7 +1453,7 @@ static int upload_vec(DeviceBuf& b, const std::vector<T, A>& v, hipStream_t s) {
 // kMmChunkA values, which cuts the partial sums written and reduced.
 // Each task of more than kMmSplitTiles row tiles runs as two workgroups
 // (mm_piece); the map lists the workgroups as (task << 2) | piece.
-constexpr uint64_t kMmChunkA = 128 * 1024;
+constexpr uint64_t kMmChunkA = 512 * 1024;
 constexpr uint64_t kMmChunkB = 1ull << 40;
 
 static void mm_map(const std::vector<GemvTask>& T, std::vector<uint32_t>& map) {
