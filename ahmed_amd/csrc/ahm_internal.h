@@ -264,7 +264,8 @@ struct ahm_hmat_s {
   // multi-RHS MFMA launches: their own chunking of the panels (tasks,
   // reductions) and workgroup maps (task << 2 | row piece)
   ahm::DeviceBuf mm_tasks_a, mm_tasks_b, mm_tasks_ra, mm_tasks_r, mm_map_a, mm_map_b;
-  uint32_t n_mm_a = 0, n_mm_b = 0, n_mm_ra = 0, n_mm_r = 0;
+  uint32_t n_mm_a = 0, n_mm_b = 0, n_mm_ra = 0, n_mm_r = 0, n_mm_ta = 0;
+  std::vector<ahm::GemvTask> h_mm_tasks_a;   // host copy (the BF16 V^T copy follows it)
   uint32_t part_len = 0;      // partial-sum slots per right-hand side (shared by
                               // the phase-A and phase-B chunk sums, used in turn)
   ahm::DeviceBuf pbuf;        // partial sums: part_len * p values
@@ -283,7 +284,7 @@ struct ahm_hmat_s {
   size_t xbuf_cap = 0;        // bytes
   // AHM_BF16 storage (FP32 values, V^T X of multi-RHS products on BF16
   // MFMA): BF16 copy of the phase-A tasks in k-groups of 8 columns, the
-  // offset of each task's copy (parallel to tasks_a), the BF16 interleaved X
+  // offset of each task's copy (parallel to mm_tasks_a), the BF16 interleaved X
   ahm::DeviceBuf bfarena, bfoff;
   uint64_t bf_vals = 0;
   ahm::DeviceBuf xbbuf;

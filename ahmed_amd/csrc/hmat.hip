@@ -1490,6 +1490,8 @@ static int build_mm_tasks(ahm_hmat h, hipStream_t s) {
   AHM_TRY(upload_vec(h->mm_map_b, mb, s));
   AHM_HIP(hipStreamSynchronize(s));
   h->n_mm_a = (uint32_t)ma.size();
+  h->n_mm_ta = (uint32_t)a.size();
+  h->h_mm_tasks_a = a;
   h->n_mm_b = (uint32_t)mb.size();
   h->n_mm_ra = (uint32_t)ra.size();
   h->n_mm_r = (uint32_t)r.size();
@@ -1675,12 +1677,13 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
     } else {
       bool bf16_a = false;
       if constexpr (std::is_same<TV, float>::value) bf16_a = h->storage == AHM_BF16;
-      if (h->n_a && bf16_a) {
+      if (h->n_mm_ta && bf16_a) {
         // V^T X on BF16 MFMA: X rounded to BF16, FP32 accumulation
         hipLaunchKernelGGL(bf16_copy_kernel, dim3(2048), dim3(256), 0, s, (const float*)xi, (uint16_t*)ws.xb,
                            (uint64_t)h->n_cols * kMmRhs * ng);
-        hipLaunchKernelGGL(panel_bf16_kernel, dim3(h->n_a, ng), dim3(kPanelThreads), 0, s,
-                           (const GemvTask*)h->tasks_a.p, (const uint64_t*)h->bfoff.p,
+        // one workgroup per MFMA phase-A task (the BF16 copy follows that list)
+        hipLaunchKernelGGL(panel_bf16_kernel, dim3(h->n_mm_ta, ng), dim3(kPanelThreads), 0, s,
+                           (const GemvTask*)h->mm_tasks_a.p, (const uint64_t*)h->bfoff.p,
                            (const uint32_t*)h->zsrc.p, (const uint16_t*)h->bfarena.p,
                            (const uint16_t*)ws.xb, h->n_cols, (float*)t, h->t_len, (float*)part,
                            h->part_len, p);
@@ -1693,10 +1696,10 @@ static int launch_matvec(ahm_hmat h, const Workspace& ws, TV alpha, const TV* X,
       }
       // the BF16 V^T X tasks keep the single-RHS chunking (their copy is
       // laid out per task)
-      const uint32_t nra = bf16_a ? h->n_ra : h->n_mm_ra;
+      const uint32_t nra = h->n_mm_ra;
       if (nra)
         hipLaunchKernelGGL((chunk_reduce_kernel<TV, true>), dim3(nra, p), dim3(kPanelThreads), 0, s,
-                           (const ReduceTask*)(bf16_a ? h->tasks_ra.p : h->mm_tasks_ra.p), (const TV*)part,
+                           (const ReduceTask*)h->mm_tasks_ra.p, (const TV*)part,
                            h->part_len, t, h->t_len, alpha, pw, Gate{});
       if (ev_mid) AHM_HIP(hipEventRecord(ev_mid, s));
       if (h->n_b)
@@ -1799,7 +1802,7 @@ namespace ahm {
 // columns, at an offset that is a multiple of 8 values; zeroed slack after the
 // last copy, since the last row tile of a task reads up to 15 rows past m.
 static int build_bf16(ahm_hmat h) {
-  const std::vector<GemvTask>& A = h->h_tasks_a;
+  const std::vector<GemvTask>& A = h->h_mm_tasks_a;
   std::vector<uint64_t> off(A.size());
   uint64_t tot = 0;
   for (size_t i = 0; i < A.size(); ++i) {
@@ -1820,7 +1823,7 @@ static int build_bf16(ahm_hmat h) {
   AHM_TRY(upload_vec(h->bfoff, off, s));
   if (!A.empty())
     hipLaunchKernelGGL(bf16_panels_kernel, dim3((uint32_t)A.size()), dim3(256), 0, s,
-                       (const GemvTask*)h->tasks_a.p, (const uint64_t*)h->bfoff.p, (const float*)h->arena.p,
+                       (const GemvTask*)h->mm_tasks_a.p, (const uint64_t*)h->bfoff.p, (const float*)h->arena.p,
                        (uint16_t*)h->bfarena.p);
   AHM_HIP(hipGetLastError());
   AHM_HIP(hipStreamSynchronize(s));
